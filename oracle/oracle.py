@@ -1,0 +1,249 @@
+"""ctypes binding of oracle/liboracle.so (the CPU restatement, the checker).
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+PU_W = [4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 12, 16, 4, 32, 24, 32, 8, 64, 48, 64, 16]
+PU_H = [4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 12, 16, 4, 16, 24, 32, 8, 32, 48, 64, 16, 64]
+PAD = 80
+MAX_DISTORTION = 1 << 30
+FENC_STRIDE = 64
+
+
+class MeParam(C.Structure):
+    _fields_ = [("frame_w", C.c_int), ("frame_h", C.c_int), ("search_range", C.c_int),
+                ("lambda_factor", C.c_int), ("min_mv_range", C.c_int * 2),
+                ("max_mv_range", C.c_int * 2), ("use_hadamard", C.c_int),
+                ("use_fractional_me", C.c_int), ("use_fast_sub_me", C.c_int),
+                ("enable_pmvr", C.c_int)]
+
+
+class Pu(C.Structure):
+    _fields_ = [("pix_x", C.c_int), ("pix_y", C.c_int), ("w", C.c_int), ("h", C.c_int),
+                ("mvp_x", C.c_int), ("mvp_y", C.c_int)]
+
+
+class MeResult(C.Structure):
+    _fields_ = [("cost", C.c_int32), ("bcost2", C.c_int32), ("bmv_x", C.c_int16),
+                ("bmv_y", C.c_int16), ("bmv2_x", C.c_int16), ("bmv2_y", C.c_int16),
+                ("mvcost", C.c_int32), ("status", C.c_int32)]
+
+
+ME_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("bcost2", "<i4"), ("bmv_x", "<i2"), ("bmv_y", "<i2"),
+                            ("bmv2_x", "<i2"), ("bmv2_y", "<i2"), ("mvcost", "<i4"), ("status", "<i4")])
+PU_DTYPE = np.dtype([("pix_x", "<i4"), ("pix_y", "<i4"), ("w", "<i4"), ("h", "<i4"),
+                     ("mvp_x", "<i4"), ("mvp_y", "<i4")])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        ip = C.c_ssize_t
+        L.xo_sad.restype = C.c_int32
+        L.xo_sad.argtypes = [C.c_int, C.c_int, P, ip, P, ip]
+        L.xo_satd.restype = C.c_int32
+        L.xo_satd.argtypes = [C.c_int, C.c_int, P, ip, P, ip]
+        L.xo_sad_x3.argtypes = [C.c_int, C.c_int, P, P, P, P, ip, P]
+        L.xo_sad_x4.argtypes = [C.c_int, C.c_int, P, P, P, P, P, ip, P]
+        L.xo_avg.argtypes = [C.c_int, C.c_int, P, ip, P, ip, P, ip]
+        for f in (L.xo_sad_batch, L.xo_satd_batch):
+            f.argtypes = [C.c_int, C.c_int, P, ip, P, ip, P, P, C.c_int, P]
+        L.xo_mvbits_max_mvd.argtypes = [C.c_int]
+        L.xo_mvbits.argtypes = [C.c_int, C.c_int]
+        L.xo_lambda_factor.argtypes = [C.c_double]
+        L.xo_me_search.restype = C.c_int32
+        L.xo_me_search.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, ip, P, ip, P, P]
+        L.xo_me_search_batch.argtypes = [C.POINTER(MeParam), P, C.c_int, P, ip, P, ip, P, C.c_int]
+        L.xo_expand_border.argtypes = [P, ip, C.c_int, C.c_int, C.c_int]
+        L.xo_interpolate_frame.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_mc_luma.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int, P, ip]
+        L.xo_mc_chroma.argtypes = [P, P, ip, C.c_int, C.c_int, C.c_int, C.c_int, P, P, ip]
+        for f in (L.xo_luma_block_hor, L.xo_luma_block_ver, L.xo_chroma_block_hor, L.xo_chroma_block_ver):
+            f.argtypes = [P, ip, P, ip, C.c_int, C.c_int, C.c_int]
+        for f in (L.xo_luma_block_ext, L.xo_chroma_block_ext):
+            f.argtypes = [P, ip, P, ip, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_dct.argtypes = [C.c_int, P, P, C.c_int]
+        L.xo_idct.argtypes = [C.c_int, P, P, C.c_int]
+        L.xo_quant.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_dequant.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        L.xo_quant_params.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
+        L.xo_dequant_params.argtypes = [C.c_int, C.c_int, P, P]
+        L.xo_residual_batch.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
+        L.xo_get_mc_pos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_tmat.restype = C.POINTER(C.c_int16)
+        L.xo_tmat.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# padded frames: a plane is stored as a 2-D array with PAD pixels on each side;
+# the origin (0, 0) of the picture is at [PAD, PAD].
+
+class Plane:
+    """A padded 8-bit plane with the picture origin at (pad, pad)."""
+
+    def __init__(self, w, h, pad=PAD, stride=None, fill=0):
+        self.w, self.h, self.pad = w, h, pad
+        self.stride = stride if stride is not None else ((w + 2 * pad + 63) // 64) * 64
+        self.rows = h + 2 * pad + 1
+        self.buf = np.full(self.rows * self.stride + 64, fill, dtype=np.uint8)
+
+    @property
+    def origin(self):
+        return self.pad * self.stride + self.pad
+
+    def origin_ptr(self):
+        return C.c_void_p(self.buf.ctypes.data + self.origin)
+
+    def view(self):
+        """2-D view of the full padded area (rows x stride)."""
+        return self.buf[: self.rows * self.stride].reshape(self.rows, self.stride)
+
+    def picture(self):
+        v = self.view()
+        return v[self.pad:self.pad + self.h, self.pad:self.pad + self.w]
+
+    def set_picture(self, img):
+        self.picture()[...] = img
+
+    def expand_border(self):
+        lib().xo_expand_border(self.origin_ptr(), self.stride, self.w, self.h, self.pad)
+
+
+def sad(w, h, a, b):
+    """a, b: 2-D uint8 arrays (blocks)."""
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    return lib().xo_sad(w, h, ptr(a), a.strides[0], ptr(b), b.strides[0])
+
+
+def satd(w, h, a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    return lib().xo_satd(w, h, ptr(a), a.strides[0], ptr(b), b.strides[0])
+
+
+def mvbits(search_range, v):
+    return lib().xo_mvbits(search_range, v)
+
+
+def lambda_factor(lam):
+    return lib().xo_lambda_factor(lam)
+
+
+def me_param(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use_fractional_me=2,
+             use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047)):
+    p = MeParam()
+    p.frame_w, p.frame_h, p.search_range, p.lambda_factor = frame_w, frame_h, search_range, lambda_factor
+    p.min_mv_range[0], p.max_mv_range[0] = mv_range_x
+    p.min_mv_range[1], p.max_mv_range[1] = mv_range_y
+    p.use_hadamard, p.use_fractional_me = use_hadamard, use_fractional_me
+    p.use_fast_sub_me, p.enable_pmvr = use_fast_sub_me, enable_pmvr
+    return p
+
+
+def interp_planes(ref_plane, use_fractional_me=2, lcu_level=6):
+    """Build the 15 interpolated planes of a border-expanded reference plane.
+    Returns a list of 16 Plane objects (index 0 is ref_plane itself)."""
+    planes = [ref_plane] + [Plane(ref_plane.w, ref_plane.h, ref_plane.pad, ref_plane.stride)
+                            for _ in range(15)]
+    arr = (C.c_void_p * 16)(*[p.origin_ptr().value for p in planes])
+    rc = lib().xo_interpolate_frame(arr, ref_plane.stride, ref_plane.w, ref_plane.h, lcu_level,
+                                    use_fractional_me)
+    assert rc == 0
+    return planes
+
+
+def plane_ptrs(planes):
+    return (C.c_void_p * 16)(*[(p.origin_ptr().value if p is not None else None) for p in planes])
+
+
+def me_search(prm, pu_tuple, cur_plane, planes, cost_map=False):
+    """pu_tuple = (x, y, w, h, mvp_x, mvp_y).  Returns (result dict, cost_map or None)."""
+    pu = Pu(*pu_tuple)
+    res = MeResult()
+    R = prm.search_range
+    cm = np.zeros(4 * R * R, dtype=np.int32) if cost_map else None
+    lib().xo_me_search(C.byref(prm), C.byref(pu), cur_plane.origin_ptr(), cur_plane.stride,
+                       plane_ptrs(planes), planes[0].stride, C.byref(res),
+                       ptr(cm) if cm is not None else None)
+    out = {f: getattr(res, f) for f, _ in MeResult._fields_}
+    return out, (cm.reshape(2 * R, 2 * R) if cm is not None else None)
+
+
+def me_search_batch(prm, pus, cur_plane, planes, nthreads=1):
+    """pus: structured array of PU_DTYPE.  Returns structured array of ME_RESULT_DTYPE."""
+    pus = np.ascontiguousarray(pus, dtype=PU_DTYPE)
+    res = np.zeros(len(pus), dtype=ME_RESULT_DTYPE)
+    lib().xo_me_search_batch(C.byref(prm), ptr(pus), len(pus), cur_plane.origin_ptr(),
+                             cur_plane.stride, plane_ptrs(planes), planes[0].stride, ptr(res), nthreads)
+    return res
+
+
+def tmat(n):
+    p = lib().xo_tmat(n)
+    return np.ctypeslib.as_array(p, shape=(n * n,)).reshape(n, n).copy()
+
+
+def dct(n, blk):
+    blk = np.ascontiguousarray(blk, dtype=np.int16)
+    out = np.zeros((n, n), dtype=np.int16)
+    lib().xo_dct(n, ptr(blk), ptr(out), n)
+    return out
+
+
+def idct(n, coef):
+    coef = np.ascontiguousarray(coef, dtype=np.int16)
+    out = np.zeros((n, n), dtype=np.int16)
+    lib().xo_idct(n, ptr(coef), ptr(out), n)
+    return out
+
+
+def quant_params(log2n, qp, b_intra):
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    lib().xo_quant_params(log2n, qp, b_intra, C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def dequant_params(log2n, qp):
+    a, b = C.c_int(), C.c_int()
+    lib().xo_dequant_params(log2n, qp, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def residual_batch(n, qp, b_intra, blocks):
+    """blocks: int16 array (nblk, n, n).  Returns (level, recon, nnz)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.int16)
+    nblk = blocks.shape[0]
+    level = np.zeros_like(blocks)
+    recon = np.zeros_like(blocks)
+    nnz = np.zeros(nblk, dtype=np.int32)
+    lib().xo_residual_batch(n, qp, b_intra, ptr(blocks), ptr(level), ptr(recon), ptr(nnz), nblk)
+    return level, recon, nnz
+
+
+def mc_chroma(qx, qy, w, h, pu, pv):
+    """pu, pv: Plane objects (chroma, pad 40).  Returns (u_block, v_block)."""
+    du = np.zeros((h, w), dtype=np.uint8)
+    dv = np.zeros((h, w), dtype=np.uint8)
+    lib().xo_mc_chroma(ptr(du), ptr(dv), w, qx, qy, w, h, pu.origin_ptr(), pv.origin_ptr(), pu.stride)
+    return du, dv

@@ -1,0 +1,891 @@
+/*
+ * xme_oracle.c -- CPU restatement of xavs2's motion-estimation cost path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Used by tests/, by
+ * __graft_entry__.smoke() and by the CPU-baseline leg of bench.py; the product
+ * library never links or calls it.
+ *
+ * Reference: pkuvcl/xavs2 (XAVS2_BUILD 13), 8-bit build (BIT_DEPTH 8).  The
+ * reference cannot be compiled here from its own sources without the
+ * configure-generated config.h (source/common/osdep.h:64), so this file is
+ * pinned by independent checks instead (tests/test_oracle_*.py):
+ *   - xo_sad / sad_x3 / sad_x4 / avg : brute force in numpy.
+ *   - xo_satd       : H4.D.H4^T matrix form in numpy, constant-difference closed
+ *                     form (8|d| per 4x4), extreme (+-255) patterns.
+ *   - xo_mvbits     : Exp-Golomb se(v) length, and the table-size formula.
+ *   - xo_me_search  : numpy brute force (lexicographic (cost, order) minimum),
+ *                     engineered ties (flat frame, period-2 texture), PMVR on/off.
+ *   - xo_interpolate_frame : numpy separable-FIR closed form with the border
+ *                     clamp model plane_k(x,y) = interp_k(clamp(x,-4,W+3),
+ *                     clamp(y,-4,H+3)); flat frames; full-pel copy.
+ *   - luma/chroma block filters, mc_chroma : numpy FIR closed form.
+ *   - xo_dct / xo_idct : table symmetries that make the butterflies of
+ *                     transform.c equal to these matrix products, orientation
+ *                     (row-only variation -> first coefficient row), DC of a
+ *                     flat block, round trip, int16 wrap on adversarial input.
+ *   - quant/dequant : scalar formula in numpy, table relations Q*IQ ~ 2^(15+s).
+ */
+#include "xme_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* PU table, common/pixel.h:50-65 (enum LumaPU order).                       */
+const uint8_t xo_pu_w[XO_NUM_PU] = { 4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32,
+                                     16, 12, 16, 4, 32, 24, 32, 8, 64, 48, 64, 16 };
+const uint8_t xo_pu_h[XO_NUM_PU] = { 4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64,
+                                     12, 16, 4, 16, 24, 32, 8, 32, 48, 64, 16, 64 };
+
+int xo_part_index(int w, int h)
+{
+    for (int i = 0; i < XO_NUM_PU; i++)
+        if (xo_pu_w[i] == w && xo_pu_h[i] == h)
+            return i;
+    return -1;
+}
+
+static inline int clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
+static inline int clip_pel(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+static inline int iabs(int v) { return v < 0 ? -v : v; }
+
+/* ------------------------------------------------------------------------ */
+/* SAD, common/pixel.c:93-135: sum over the block of |p1 - p2|.              */
+int32_t xo_sad(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2)
+{
+    int32_t sum = 0;
+    for (int y = 0; y < h; y++, p1 += s1, p2 += s2) {
+        int32_t row = 0;
+        for (int x = 0; x < w; x++)
+            row += iabs((int)p1[x] - (int)p2[x]);
+        sum += row;
+    }
+    return sum;
+}
+
+/* SAD_x3 / SAD_x4, pixel.c:143-243: fenc stride is fixed at FENC_STRIDE. */
+void xo_sad_x3(int w, int h, const uint8_t *fenc, const uint8_t *p0, const uint8_t *p1,
+               const uint8_t *p2, intptr_t i_ref, int32_t res[3])
+{
+    res[0] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p0, i_ref);
+    res[1] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p1, i_ref);
+    res[2] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p2, i_ref);
+}
+
+void xo_sad_x4(int w, int h, const uint8_t *fenc, const uint8_t *p0, const uint8_t *p1,
+               const uint8_t *p2, const uint8_t *p3, intptr_t i_ref, int32_t res[4])
+{
+    res[0] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p0, i_ref);
+    res[1] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p1, i_ref);
+    res[2] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p2, i_ref);
+    res[3] = xo_sad(w, h, fenc, XO_FENC_STRIDE, p3, i_ref);
+}
+
+/* SATD, pixel.c:253-407.  The reference packs two 16-bit lanes per 32-bit
+ * word; for 8-bit input every transformed value lies in [-4080, 4080] and a
+ * lane accumulates at most 16 of them (<= 65280), so the packing is lossless
+ * and the result equals: for every 4x4 tile, D = A - B, T = H4 D H4^T, and
+ * SATD = (sum over tiles of sum |T|) / 2.  Each tile sum is even (the last
+ * butterfly stage yields pairs a+b, a-b with |a+b|+|a-b| = 2 max(|a|,|b|)),
+ * so the per-4x4 (:321) and per-8x4 (:347) halvings are exact and the tiling
+ * (4x4 tiles for 4/12-wide blocks, 8x4 otherwise) does not change the value. */
+static int satd4x4_sum(const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2)
+{
+    int d[4][4], t[4][4];
+    for (int y = 0; y < 4; y++)
+        for (int x = 0; x < 4; x++)
+            d[y][x] = (int)p1[y * s1 + x] - (int)p2[y * s2 + x];
+    /* horizontal 4-point Hadamard on each row */
+    for (int y = 0; y < 4; y++) {
+        int a = d[y][0] + d[y][1], b = d[y][0] - d[y][1];
+        int c = d[y][2] + d[y][3], e = d[y][2] - d[y][3];
+        t[y][0] = a + c; t[y][1] = b + e; t[y][2] = a - c; t[y][3] = b - e;
+    }
+    int sum = 0;
+    /* vertical 4-point Hadamard on each column */
+    for (int x = 0; x < 4; x++) {
+        int a = t[0][x] + t[1][x], b = t[0][x] - t[1][x];
+        int c = t[2][x] + t[3][x], e = t[2][x] - t[3][x];
+        sum += iabs(a + c) + iabs(b + e) + iabs(a - c) + iabs(b - e);
+    }
+    return sum;
+}
+
+int32_t xo_satd(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2)
+{
+    int32_t sum = 0;
+    for (int y = 0; y < h; y += 4)
+        for (int x = 0; x < w; x += 4)
+            sum += satd4x4_sum(p1 + y * s1 + x, s1, p2 + y * s2 + x, s2);
+    return sum >> 1;
+}
+
+/* AVG, pixel.c:680-719: (a + b + 1) >> 1 (the weight argument is unused). */
+void xo_avg(int w, int h, uint8_t *dst, intptr_t ds, const uint8_t *s0, intptr_t ss0,
+            const uint8_t *s1, intptr_t ss1)
+{
+    for (int y = 0; y < h; y++, dst += ds, s0 += ss0, s1 += ss1)
+        for (int x = 0; x < w; x++)
+            dst[x] = (uint8_t)((s0[x] + s1[x] + 1) >> 1);
+}
+
+void xo_sad_batch(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2,
+                  const int64_t *off1, const int64_t *off2, int n, int32_t *out)
+{
+    for (int i = 0; i < n; i++)
+        out[i] = xo_sad(w, h, p1 + off1[i], s1, p2 + off2[i], s2);
+}
+
+void xo_satd_batch(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2,
+                   const int64_t *off1, const int64_t *off2, int n, int32_t *out)
+{
+    for (int i = 0; i < n; i++)
+        out[i] = xo_satd(w, h, p1 + off1[i], s1, p2 + off2[i], s2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MV rate, encoder/me.c:592-641.  The table size follows
+ * xavs2_me_get_buf_size / xavs2_me_init with me_range = max(256, R). */
+static int mvbits_max_bits(int search_range)
+{
+    int me_range   = search_range > 256 ? search_range : 256;
+    int subpel_num = 4 * (2 * me_range + 3);
+    return 5 + 2 * (int)ceil(log(subpel_num + 1) / log(2) + 1e-10);
+}
+
+int xo_mvbits_max_mvd(int search_range)
+{
+    return (1 << (mvbits_max_bits(search_range) >> 1)) - 1;
+}
+
+int xo_mvbits(int search_range, int v)
+{
+    /* mvbits[0] = 1; entries with imin <= |v| < imax get 'bits', for
+     * bits = 3, 5, ..., max_mv_bits and imax = 1 << (bits >> 1). */
+    int a = iabs(v);
+    if (a == 0)
+        return 1;
+    int max_bits = mvbits_max_bits(search_range);
+    for (int bits = 3; bits <= max_bits; bits += 2) {
+        int imax = 1 << (bits >> 1);
+        if (a >= (imax >> 1) && a < imax)
+            return bits;
+    }
+    return -1; /* outside the reference table */
+}
+
+int xo_lambda_factor(double lambda_mode)
+{
+    return (int)((double)(1 << 16) * sqrt(lambda_mode) + 0.5);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ME search.                                                                */
+
+/* CHECK_MV_RANGE, encoder/me.h:46-48, with mv_min / mv_max packed as in
+ * xavs2_me_search (me.c:896-897).  Restated bit for bit, since the packed
+ * test is what the reference evaluates. */
+static inline uint32_t pack_mask2(int mx, int my)
+{
+    return ((uint32_t)mx << 16) | ((uint32_t)my & 0x7FFFu);
+}
+
+static inline int check_mv_range(int mx, int my, uint32_t mv_min, uint32_t mv_max)
+{
+    uint32_t p = pack_mask2(mx, my);
+    return !(((p + mv_min) | (mv_max - p)) & 0x80004000u);
+}
+
+typedef struct {
+    int mv_min_fpel[2], mv_max_fpel[2];
+    uint32_t mv_min, mv_max;
+} mv_window_t;
+
+/* MV limits of pred_inter_search_single, encoder/md_inter.c:910-926. */
+static void pu_mv_window(const xo_me_param_t *prm, const xo_pu_t *pu, mv_window_t *win)
+{
+    const int m = 6;
+    int lo, hi;
+    lo = (-64 - pu->pix_x) * 4;
+    hi = (prm->frame_w + 64 - pu->pix_x - pu->w) * 4;
+    lo = clip3(prm->min_mv_range[0], prm->max_mv_range[0], lo);
+    hi = clip3(prm->min_mv_range[0], prm->max_mv_range[0], hi);
+    win->mv_min_fpel[0] = (lo >> 2) + m;
+    win->mv_max_fpel[0] = (hi >> 2) - m;
+    lo = (-64 - pu->pix_y) * 4;
+    hi = (prm->frame_h + 64 - pu->pix_y - pu->h) * 4;
+    lo = clip3(prm->min_mv_range[1], prm->max_mv_range[1], lo);
+    hi = clip3(prm->min_mv_range[1], prm->max_mv_range[1], hi);
+    win->mv_min_fpel[1] = (lo >> 2) + m;
+    win->mv_max_fpel[1] = (hi >> 2) - m;
+    win->mv_min = pack_mask2(-win->mv_min_fpel[0], -win->mv_min_fpel[1]);
+    win->mv_max = pack_mask2(win->mv_max_fpel[0], win->mv_max_fpel[1]) | 0x8000u;
+}
+
+/* Spiral / Spiral2 patterns, encoder/me.c:129-139. */
+static const int8_t SPIRAL[9][2] = {
+    { 0, 0 }, { 0, -1 }, { 0, 1 }, { -1, -1 }, { 1, -1 }, { -1, 0 }, { 1, 0 }, { -1, 1 }, { 1, 1 }
+};
+static const int8_t SPIRAL2[9][2] = {
+    { 0, 0 }, { 0, -1 }, { -1, -1 }, { -1, 0 }, { -1, 1 }, { 0, 1 }, { 1, 1 }, { 1, 0 }, { 1, -1 }
+};
+
+/* pmvr_adapt_mv, encoder/me.c:412-424 (TH_PMVR = 2, defines.h:227).
+ * Returns 1 when the candidate is to be skipped. */
+static int pmvr_adapt(int *mx, int *my, int ctr_x, int ctr_y, int mv_x, int mv_y, int sx, int sy)
+{
+    if (iabs(mv_x - ctr_x) > 2 || iabs(mv_y - ctr_y) > 2) {
+        *mx = mv_x + sx * 2;
+        *my = mv_y + sy * 2;
+        return iabs(*mx - ctr_x) <= 2 && iabs(*my - ctr_y) <= 2;
+    }
+    *mx = mv_x + sx;
+    *my = mv_y + sy;
+    return iabs(*mx - ctr_x) > 2 || iabs(*my - ctr_y) > 2;
+}
+
+static inline int weighted_cost(int lambda, int bits) { return (lambda * bits) >> 16; }
+
+int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
+                     const uint8_t *cur, intptr_t i_cur,
+                     const uint8_t *const planes[16], intptr_t i_ref,
+                     xo_me_result_t *res, int32_t *cost_map)
+{
+    const int R = prm->search_range;
+    const int W = pu->w, H = pu->h;
+    const int lambda = prm->lambda_factor;
+    const int pmx = pu->mvp_x, pmy = pu->mvp_y;
+    const uint8_t *org = cur + (intptr_t)pu->pix_y * i_cur + pu->pix_x;
+    const intptr_t i_offset = (intptr_t)pu->pix_y * i_ref + pu->pix_x;   /* p_me->i_bias */
+    const uint8_t *fref = planes[0] + i_offset;
+    mv_window_t win;
+    pu_mv_window(prm, pu, &win);
+
+    /* MV candidate list: add_one_mv_candidate(mvp) then (0,0),
+     * md_inter.c:840-865, :957-959.  IPEL(v) = (v + 2) >> 2, me.h:60. */
+    int mvc[2][2], n_mvc = 0;
+    const int cand_q[2][2] = { { pmx, pmy }, { 0, 0 } };
+    for (int c = 0; c < 2; c++) {
+        int x = (cand_q[c][0] + 2) >> 2, y = (cand_q[c][1] + 2) >> 2;
+        x = clip3(win.mv_min_fpel[0], win.mv_max_fpel[0], x);
+        y = clip3(win.mv_min_fpel[1], win.mv_max_fpel[1], y);
+        int dup = 0;
+        for (int k = 0; k < n_mvc; k++)
+            if (mvc[k][0] == x && mvc[k][1] == y)
+                dup = 1;
+        if (!dup) {
+            mvc[n_mvc][0] = x;
+            mvc[n_mvc][1] = y;
+            n_mvc++;
+        }
+    }
+
+#define IPEL_COST(mx, my) \
+    (xo_sad(W, H, org, i_cur, fref + (intptr_t)(my) * i_ref + (mx), i_ref) + \
+     weighted_cost(lambda, xo_mvbits(R, 4 * (mx) - pmx) + xo_mvbits(R, 4 * (my) - pmy)))
+
+    /* xavs2_me_search, me.c:879-921: mvc candidates in order, strict '<'. */
+    int bcost = XO_MAX_DISTORTION, bmx = 0, bmy = 0;
+    for (int c = 0; c < n_mvc; c++) {
+        int mx = mvc[c][0], my = mvc[c][1];
+        if (check_mv_range(mx, my, win.mv_min, win.mv_max)) {
+            int cost = IPEL_COST(mx, my);
+            if (cost < bcost) { bcost = cost; bmx = mx; bmy = my; }
+        }
+    }
+    if (bcost == XO_MAX_DISTORTION) {         /* _me_error, me.c:919-921, :1181 */
+        memset(res, 0, sizeof(*res));
+        res->cost = XO_MAX_DISTORTION;
+        res->status = 1;
+        if (cost_map)
+            for (int k = 0; k < 4 * R * R; k++)
+                cost_map[k] = -1;
+        return XO_MAX_DISTORTION;
+    }
+
+    /* Full search, me.c:1156-1164: rows j in [-R, R), columns i in [-R, R). */
+    const int omx = bmx, omy = bmy;
+    for (int j = -R; j < R; j++) {
+        for (int i = -R; i < R; i++) {
+            int mx = omx + i, my = omy + j;
+            int cost = -1;
+            if (check_mv_range(mx, my, win.mv_min, win.mv_max)) {
+                cost = IPEL_COST(mx, my);
+                if (cost < bcost) { bcost = cost; bmx = mx; bmy = my; }
+            }
+            if (cost_map)
+                cost_map[(j + R) * 2 * R + (i + R)] = cost;
+        }
+    }
+#undef IPEL_COST
+
+    /* Store the full-pel result, me.c:1169-1173. */
+    res->bmv_x  = (int16_t)(4 * bmx);
+    res->bmv_y  = (int16_t)(4 * bmy);
+    res->bmv2_x = (int16_t)bmx;
+    res->bmv2_y = (int16_t)bmy;
+    res->bcost2 = bcost;
+    res->mvcost = weighted_cost(lambda, xo_mvbits(R, 4 * bmx - pmx) + xo_mvbits(R, 4 * bmy - pmy));
+    res->status = 0;
+    res->cost   = bcost;
+
+    if (!prm->use_fractional_me)
+        return bcost;
+
+    /* me_subpel_refine, me.c:479-579. */
+#define QPEL_COST(mx, my) \
+    ((prm->use_hadamard ? xo_satd : xo_sad)(W, H, org, i_cur, \
+        planes[(((my) & 3) << 2) + ((mx) & 3)] + i_offset + (intptr_t)((my) >> 2) * i_ref + ((mx) >> 2), \
+        i_ref) + weighted_cost(lambda, xo_mvbits(R, (mx) - pmx) + xo_mvbits(R, (my) - pmy)))
+
+    const int step = prm->use_fast_sub_me ? 2 : 1;
+    const int8_t (*pat)[2] = prm->use_fast_sub_me ? SPIRAL2 : SPIRAL;
+    const int ctr_x = (pmx >> 1) << 1, ctr_y = (pmy >> 1) << 1;
+    int qbx = 4 * bmx, qby = 4 * bmy;         /* search centre in qpel */
+    int best_x = qbx, best_y = qby;
+    int cost;
+    if (prm->use_hadamard)
+        bcost = QPEL_COST(qbx, qby);
+    /* else: bcost stays the integer cost (me.c:511-516) */
+
+    for (int pos = 1; pos < 9; pos += step) {  /* half-pel around the integer best */
+        int mx = qbx + pat[pos][0] * 2, my = qby + pat[pos][1] * 2;
+        cost = QPEL_COST(mx, my);
+        if (cost < bcost) { bcost = cost; best_x = mx; best_y = my; }
+    }
+    qbx = best_x;
+    qby = best_y;
+    if (prm->use_fractional_me > 1) {          /* quarter-pel around the half-pel best */
+        for (int pos = 1; pos < 9; pos += step) {
+            int mx, my;
+            if (prm->enable_pmvr) {
+                if (pmvr_adapt(&mx, &my, ctr_x, ctr_y, qbx, qby, pat[pos][0], pat[pos][1]))
+                    continue;
+            } else {
+                mx = qbx + pat[pos][0];
+                my = qby + pat[pos][1];
+            }
+            cost = QPEL_COST(mx, my);
+            if (cost < bcost) { bcost = cost; best_x = mx; best_y = my; }
+        }
+    }
+#undef QPEL_COST
+    res->bmv_x  = (int16_t)best_x;
+    res->bmv_y  = (int16_t)best_y;
+    res->cost   = bcost;
+    res->mvcost = weighted_cost(lambda, xo_mvbits(R, best_x - pmx) + xo_mvbits(R, best_y - pmy));
+    return bcost;
+}
+
+typedef struct {
+    const xo_me_param_t *prm;
+    const xo_pu_t *pus;
+    int n;
+    const uint8_t *cur;
+    intptr_t i_cur;
+    const uint8_t *const *planes;
+    intptr_t i_ref;
+    xo_me_result_t *res;
+    int next;
+    pthread_mutex_t lock;
+} batch_ctx_t;
+
+static void *batch_worker(void *arg)
+{
+    batch_ctx_t *ctx = (batch_ctx_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&ctx->lock);
+        int i = ctx->next;
+        ctx->next += 4;
+        pthread_mutex_unlock(&ctx->lock);
+        if (i >= ctx->n)
+            break;
+        int e = i + 4 < ctx->n ? i + 4 : ctx->n;
+        for (; i < e; i++)
+            xo_me_search(ctx->prm, &ctx->pus[i], ctx->cur, ctx->i_cur, ctx->planes, ctx->i_ref,
+                         &ctx->res[i], NULL);
+    }
+    return NULL;
+}
+
+void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,
+                        const uint8_t *cur, intptr_t i_cur,
+                        const uint8_t *const planes[16], intptr_t i_ref,
+                        xo_me_result_t *res, int nthreads)
+{
+    batch_ctx_t ctx = { prm, pus, n, cur, i_cur, planes, i_ref, res, 0, PTHREAD_MUTEX_INITIALIZER };
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads == 1) {
+        batch_worker(&ctx);
+        return;
+    }
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    for (int t = 0; t < nthreads; t++)
+        pthread_create(&th[t], NULL, batch_worker, &ctx);
+    for (int t = 0; t < nthreads; t++)
+        pthread_join(th[t], NULL);
+    free(th);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Borders and interpolation.                                                */
+
+/* plane_expand_border, common/frame.c:470-507: replicate the left/right
+ * columns over pad_h pixels, then the (widened) first/last rows over pad_v. */
+static void expand_border(uint8_t *pix, intptr_t stride, int w, int h, int padh, int padv,
+                          int top, int bottom)
+{
+    for (int y = 0; y < h; y++) {
+        uint8_t *row = pix + y * stride;
+        memset(row - padh, row[0], (size_t)padh);
+        memset(row + w, row[w - 1], (size_t)padh);
+    }
+    int wide = w + 2 * padh;
+    if (top)
+        for (int y = 1; y <= padv; y++)
+            memcpy(pix - padh - y * stride, pix - padh, (size_t)wide);
+    if (bottom)
+        for (int y = 1; y <= padv; y++)
+            memcpy(pix - padh + (h - 1 + y) * stride, pix - padh + (h - 1) * stride, (size_t)wide);
+}
+
+void xo_expand_border(uint8_t *plane, intptr_t stride, int w, int h, int pad)
+{
+    expand_border(plane, stride, w, h, pad, pad, 1, 1);
+}
+
+/* Luma 8-tap and chroma 4-tap phases, common/mc.c:73-91. */
+const int8_t xo_luma_taps[4][8] = {
+    { 0, 0, 0, 64, 0, 0, 0, 0 },
+    { -1, 4, -10, 57, 19, -7, 3, -1 },
+    { -1, 4, -11, 40, 40, -11, 4, -1 },
+    { -1, 3, -7, 19, 57, -10, 4, -1 },
+};
+const int8_t xo_chroma_taps[8][4] = {
+    { 0, 64, 0, 0 }, { -4, 62, 6, 0 }, { -6, 56, 15, -1 }, { -5, 47, 25, -3 },
+    { -4, 36, 36, -4 }, { -3, 25, 47, -5 }, { -1, 15, 56, -6 }, { 0, 6, 62, -4 },
+};
+
+/* 8-tap sum at s[0] with taps on s[-3*step .. 4*step] (FLT_8TAP_HOR/VER, mc.c:128-146). */
+static inline int fir8_u8(const uint8_t *s, intptr_t step, const int8_t *c)
+{
+    int v = 0;
+    for (int k = 0; k < 8; k++)
+        v += c[k] * s[(k - 3) * step];
+    return v;
+}
+static inline int fir8_s16(const int16_t *s, intptr_t step, const int8_t *c)
+{
+    int v = 0;
+    for (int k = 0; k < 8; k++)
+        v += c[k] * s[(k - 3) * step];
+    return v;
+}
+
+/* interpolate_sample_rows, mc.c:593-744, for output rows [start_y, start_y+height).
+ * tmp[3] are int16 planes of stride i_tmp whose (x=0,y=0) sits at tmp_org. */
+static void interp_rows(uint8_t *const planes[16], intptr_t stride, int w,
+                        int16_t *const tmp_org[3], intptr_t i_tmp,
+                        int start_y, int height, int b_start, int b_end, int frac)
+{
+    const int wide = w + 8;                   /* columns [-4, w+4), PAD_OFFSET = 4 */
+    /* horizontal a,b,c with 4 extra rows above and below (mc.c:636-667) */
+    for (int y = start_y - 4; y < start_y + height + 4; y++) {
+        const uint8_t *src = planes[0] + y * stride;
+        for (int x = -4; x < -4 + wide; x++) {
+            for (int dx = 1; dx <= 3; dx++) {
+                if (frac < 2 && dx != 2)
+                    continue;
+                int v = fir8_u8(src + x, 1, xo_luma_taps[dx]);
+                tmp_org[dx - 1][y * i_tmp + x] = (int16_t)v;
+                planes[dx][y * stride + x] = (uint8_t)clip_pel((v + 32) >> 6);
+            }
+        }
+    }
+    /* vertical d,h,n (mc.c:669-682) */
+    for (int y = start_y; y < start_y + height; y++) {
+        for (int x = -4; x < -4 + wide; x++) {
+            for (int dy = 1; dy <= 3; dy++) {
+                if (frac < 2 && dy != 2)
+                    continue;
+                int v = fir8_u8(planes[0] + y * stride + x, stride, xo_luma_taps[dy]);
+                planes[dy << 2][y * stride + x] = (uint8_t)clip_pel((v + 32) >> 6);
+            }
+        }
+    }
+    /* diagonal e..r from the int16 horizontal sums, MC_SHIFT = 12 (mc.c:684-724) */
+    for (int y = start_y; y < start_y + height; y++) {
+        for (int x = -4; x < -4 + wide; x++) {
+            for (int dx = 1; dx <= 3; dx++) {
+                for (int dy = 1; dy <= 3; dy++) {
+                    if (frac < 2 && (dx != 2 || dy != 2))
+                        continue;
+                    int v = fir8_s16(tmp_org[dx - 1] + y * i_tmp + x, i_tmp, xo_luma_taps[dy]);
+                    planes[(dy << 2) + dx][y * stride + x] = (uint8_t)clip_pel((v + 2048) >> 12);
+                }
+            }
+        }
+    }
+    /* border of the 15 planes over this window (mc.c:726-742) */
+    for (int k = 1; k < 16; k++) {
+        if (!planes[k])
+            continue;
+        expand_border(planes[k] + start_y * stride - 4, stride, wide, height,
+                      XO_PAD - 4, XO_PAD - 4, b_start, b_end);
+    }
+}
+
+int xo_interpolate_frame(uint8_t *const planes_in[16], intptr_t stride, int w, int h,
+                         int lcu_level, int use_fractional_me)
+{
+    if (!use_fractional_me)
+        return 0;
+    uint8_t *planes[16];
+    for (int k = 0; k < 16; k++)
+        planes[k] = planes_in[k];
+    if (use_fractional_me == 1)               /* only b, h, j exist (frame.c:343-350) */
+        for (int k = 1; k < 16; k++)
+            if (k != 2 && k != 8 && k != 10)
+                planes[k] = NULL;
+    const intptr_t i_tmp = w + 2 * XO_PAD;
+    const size_t tmp_sz = (size_t)i_tmp * (size_t)(h + 2 * XO_PAD);
+    int16_t *buf = (int16_t *)calloc(3 * tmp_sz, sizeof(int16_t));
+    if (!buf)
+        return -1;
+    int16_t *tmp_org[3];
+    for (int k = 0; k < 3; k++)
+        tmp_org[k] = buf + k * tmp_sz + XO_PAD * i_tmp + XO_PAD;
+
+    /* interpolate_lcu_row, mc.c:748-785 (one slice): MC_OFFSET = 8. */
+    const int lcu = 1 << lcu_level;
+    const int rows = (h + lcu - 1) / lcu;
+    for (int r = 0; r < rows; r++) {
+        int b_start = r == 0, b_end = r == rows - 1;
+        int y0 = r * lcu, y1 = (r + 1) * lcu;
+        y0 -= b_start ? 4 : 8;
+        y1 = b_end ? h + 8 - 4 : y1 - 8;
+        interp_rows(planes, stride, w, tmp_org, i_tmp, y0, y1 - y0, b_start, b_end, use_fractional_me);
+    }
+    free(buf);
+    return 0;
+}
+
+/* Block filters, mc.c:275-398 (8-bit: first-pass shift 0, second >> 12). */
+void xo_luma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_u8(src + y * i_src + x, 1, xo_luma_taps[dx]) + 32) >> 6);
+}
+
+void xo_luma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_u8(src + y * i_src + x, i_src, xo_luma_taps[dy]) + 32) >> 6);
+}
+
+void xo_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy)
+{
+    int16_t tmp[(64 + 7) * 64];
+    for (int y = -3; y < h + 4; y++)
+        for (int x = 0; x < w; x++)
+            tmp[(y + 3) * 64 + x] = (int16_t)fir8_u8(src + y * i_src + x, 1, xo_luma_taps[dx]);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_s16(tmp + (y + 3) * 64 + x, 64, xo_luma_taps[dy]) + 2048) >> 12);
+}
+
+static inline int fir4_u8(const uint8_t *s, intptr_t step, const int8_t *c)
+{
+    return c[0] * s[-step] + c[1] * s[0] + c[2] * s[step] + c[3] * s[2 * step];
+}
+
+void xo_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir4_u8(src + y * i_src + x, 1, xo_chroma_taps[dx]) + 32) >> 6);
+}
+
+void xo_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir4_u8(src + y * i_src + x, i_src, xo_chroma_taps[dy]) + 32) >> 6);
+}
+
+void xo_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy)
+{
+    int32_t tmp[(32 + 3) * 32];
+    const int8_t *cv = xo_chroma_taps[dy];
+    for (int y = -1; y < h + 2; y++)
+        for (int x = 0; x < w; x++)
+            tmp[(y + 1) * 32 + x] = fir4_u8(src + y * i_src + x, 1, xo_chroma_taps[dx]);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const int32_t *t = tmp + (y + 1) * 32 + x;
+            int v = cv[0] * t[-32] + cv[1] * t[0] + cv[2] * t[32] + cv[3] * t[64];
+            dst[y * i_dst + x] = (uint8_t)clip_pel((v + 2048) >> 12);
+        }
+}
+
+/* mc_luma, mc.c:559-589: copy from the interpolated plane when it exists. */
+void xo_mc_luma(uint8_t *dst, intptr_t i_dst, int qx, int qy, int w, int h,
+                const uint8_t *const planes[16], intptr_t i_ref)
+{
+    int x = qx >> 2, y = qy >> 2, dx = qx & 3, dy = qy & 3;
+    const uint8_t *p = planes[(dy << 2) + dx];
+    if (p) {
+        p += (intptr_t)y * i_ref + x;
+        for (int r = 0; r < h; r++)
+            memcpy(dst + r * i_dst, p + r * i_ref, (size_t)w);
+        return;
+    }
+    const uint8_t *src = planes[0] + (intptr_t)y * i_ref + x;
+    if (dy == 0)
+        xo_luma_block_hor(dst, i_dst, src, i_ref, w, h, dx);
+    else if (dx == 0)
+        xo_luma_block_ver(dst, i_dst, src, i_ref, w, h, dy);
+    else
+        xo_luma_block_ext(dst, i_dst, src, i_ref, w, h, dx, dy);
+}
+
+/* cu_get_mc_pos, common/mc.h:52-66: absolute qpel position of a block,
+ * clamped for MVs that point far outside the picture (keeps mv & 7). */
+int xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv)
+{
+    int imv = mv >> 2, fmv = mv & 7;
+    if (blk_pos + imv < -blk_size - 8)
+        return (-blk_size - 8) * 4 + fmv;
+    if (blk_pos + imv > img_size + 4)
+        return (img_size + 4) * 4 + fmv;
+    return blk_pos * 4 + mv;
+}
+
+/* mc_chroma, mc.c:797-829: luma qpel MV = chroma 1/8-pel position. */
+void xo_mc_chroma(uint8_t *dst_u, uint8_t *dst_v, intptr_t i_dst, int qx, int qy, int w, int h,
+                  const uint8_t *plane_u, const uint8_t *plane_v, intptr_t i_src)
+{
+    int px = qx & 7, py = qy & 7;
+    intptr_t off = (intptr_t)(qy >> 3) * i_src + (qx >> 3);
+    const uint8_t *src[2] = { plane_u + off, plane_v + off };
+    uint8_t *dst[2] = { dst_u, dst_v };
+    for (int c = 0; c < 2; c++) {
+        if (px == 0 && py == 0) {
+            for (int r = 0; r < h; r++)
+                memcpy(dst[c] + r * i_dst, src[c] + r * i_src, (size_t)w);
+        } else if (py == 0) {
+            xo_chroma_block_hor(dst[c], i_dst, src[c], i_src, w, h, px);
+        } else if (px == 0) {
+            xo_chroma_block_ver(dst[c], i_dst, src[c], i_src, w, h, py);
+        } else {
+            xo_chroma_block_ext(dst[c], i_dst, src[c], i_src, w, h, px, py);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Transform.  The AVS2 integer DCT matrices (transform.c:59-135) are
+ * T_N[k][n] = 32 for k = 0 and round(32*sqrt(2)*cos((2n+1)k*pi/(2N))) otherwise,
+ * and T_N[k][n] = T_32[k*32/N][n].  They are generated here from that closed
+ * form; tests compare the result entry by entry with the reference tables. */
+static int16_t T4m[4][4], T8m[8][8], T16m[16][16], T32m[32][32];
+static pthread_once_t tables_once = PTHREAD_ONCE_INIT;
+
+static void build_tables(void)
+{
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < 32; k++)
+        for (int n = 0; n < 32; n++)
+            T32m[k][n] = (int16_t)(k == 0 ? 32 : lround(32.0 * sqrt(2.0) * cos((2 * n + 1) * k * pi / 64.0)));
+    for (int k = 0; k < 16; k++)
+        for (int n = 0; n < 16; n++)
+            T16m[k][n] = T32m[2 * k][n];
+    for (int k = 0; k < 8; k++)
+        for (int n = 0; n < 8; n++)
+            T8m[k][n] = T32m[4 * k][n];
+    for (int k = 0; k < 4; k++)
+        for (int n = 0; n < 4; n++)
+            T4m[k][n] = T32m[8 * k][n];
+}
+
+static const int16_t *tmat(int n)
+{
+    pthread_once(&tables_once, build_tables);
+    switch (n) {
+    case 4: return &T4m[0][0];
+    case 8: return &T8m[0][0];
+    case 16: return &T16m[0][0];
+    default: return &T32m[0][0];
+    }
+}
+
+/* One forward pass (partialButterflyN, transform.c:167-498): for every input
+ * row j, out[k][j] = int16((sum_n T[k][n] * in[j][n] + rnd) >> shift).  The
+ * butterflies compute exactly this sum because T[k][N-1-n] = (-1)^k T[k][n]
+ * (checked by the tests), and every partial sum fits in int32. */
+static void fwd_pass(const int16_t *in, int16_t *out, int n, int shift)
+{
+    const int16_t *T = tmat(n);
+    const int add = (1 << shift) >> 1;
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++) {
+            int s = 0;
+            for (int m = 0; m < n; m++)
+                s += T[k * n + m] * in[j * n + m];
+            out[k * n + j] = (int16_t)((s + add) >> shift);
+        }
+}
+
+/* One inverse pass (partialButterflyInverseN): for every column j,
+ * out[j][m] = clip((sum_k T[k][m] * in[k][j] + rnd) >> shift). */
+static void inv_pass(const int16_t *in, int16_t *out, int n, int shift, int clip_depth)
+{
+    const int16_t *T = tmat(n);
+    const int add = (1 << shift) >> 1;
+    const int hi = ((1 << clip_depth) >> 1) - 1, lo = -hi - 1;
+    for (int j = 0; j < n; j++)
+        for (int m = 0; m < n; m++) {
+            int s = 0;
+            for (int k = 0; k < n; k++)
+                s += T[k * n + m] * in[k * n + j];
+            out[j * n + m] = (int16_t)clip3(lo, hi, (s + add) >> shift);
+        }
+}
+
+const int16_t *xo_tmat(int n) { return tmat(n); }
+
+static int ilog2(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
+
+/* dct_NxN_c, transform.c:1123-1270: shift1 = log2N - 2 (+1 with the 32x32
+ * stride flag), shift2 = log2N + 5. */
+void xo_dct(int n, const int16_t *src, int16_t *dst, int i_src)
+{
+    int16_t block[32 * 32], tmp[32 * 32];
+    int l = ilog2(n);
+    int shift1 = l + 5 + 8 + 1 - 16;
+    int shift2 = l + 5;
+    if (n == 32) {
+        shift1 += i_src & 1;
+        i_src &= 0xFE;
+    }
+    for (int y = 0; y < n; y++)
+        memcpy(block + y * n, src + y * i_src, sizeof(int16_t) * (size_t)n);
+    fwd_pass(block, tmp, n, shift1);
+    fwd_pass(tmp, dst, n, shift2);
+}
+
+/* idct_NxN_c, transform.c:1145-1312: shift 5 clipped to 16 bits, then
+ * shift 12 clipped to 9 bits (the 32x32 stride flag trades one bit). */
+void xo_idct(int n, const int16_t *src, int16_t *dst, int i_dst)
+{
+    int16_t block[32 * 32], tmp[32 * 32];
+    int a_flag = 0;
+    if (n == 32) {
+        a_flag = i_dst & 1;
+        i_dst &= 0xFE;
+    }
+    inv_pass(src, tmp, n, 5, 16);
+    inv_pass(tmp, block, n, 12 - a_flag, 9 + a_flag);
+    for (int y = 0; y < n; y++)
+        memcpy(dst + y * i_dst, block + y * n, sizeof(int16_t) * (size_t)n);
+}
+
+/* Quantisation tables, common/quant.c:55-97. */
+const uint16_t xo_Q_TAB[80] = {
+    32768, 29775, 27554, 25268, 23170, 21247, 19369, 17770,
+    16302, 15024, 13777, 12634, 11626, 10624,  9742,  8958,
+     8192,  7512,  6889,  6305,  5793,  5303,  4878,  4467,
+     4091,  3756,  3444,  3161,  2894,  2654,  2435,  2235,
+     2048,  1878,  1722,  1579,  1449,  1329,  1218,  1117,
+     1024,   939,   861,   790,   724,   664,   609,   558,
+      512,   470,   430,   395,   362,   332,   304,   279,
+      256,   235,   215,   197,   181,   166,   152,   140,
+      128,   116,   108,    99,    91,    83,    76,    69,
+       64,    59,    54,    49,    45,    41,    38,    35,
+};
+const uint16_t xo_IQ_TAB[80] = {
+    32768, 36061, 38968, 42495, 46341, 50535, 55437, 60424,
+    32932, 35734, 38968, 42495, 46177, 50535, 55109, 59933,
+    65535, 35734, 38968, 42577, 46341, 50617, 55027, 60097,
+    32809, 35734, 38968, 42454, 46382, 50576, 55109, 60056,
+    65535, 35734, 38968, 42495, 46320, 50515, 55109, 60076,
+    65535, 35744, 38968, 42495, 46341, 50535, 55099, 60087,
+    65535, 35734, 38973, 42500, 46341, 50535, 55109, 60097,
+    32771, 35734, 38965, 42497, 46341, 50535, 55109, 60099,
+    32768, 36061, 38968, 42495, 46341, 50535, 55437, 60424,
+    32932, 35734, 38968, 42495, 46177, 50535, 55109, 59933,
+};
+const uint8_t xo_IQ_SHIFT[80] = {
+    15, 15, 15, 15, 15, 15, 15, 15,
+    14, 14, 14, 14, 14, 14, 14, 14,
+    14, 13, 13, 13, 13, 13, 13, 13,
+    12, 12, 12, 12, 12, 12, 12, 12,
+    12, 11, 11, 11, 11, 11, 11, 11,
+    11, 10, 10, 10, 10, 10, 10, 10,
+    10,  9,  9,  9,  9,  9,  9,  9,
+     8,  8,  8,  8,  8,  8,  8,  8,
+     7,  7,  7,  7,  7,  7,  7,  7,
+     6,  6,  6,  6,  6,  6,  6,  6,
+};
+
+/* quant_c, quant.c:109-121: sign2(c) * ((|c| * scale + add) >> shift), with
+ * sign2(0) = +1 (avs2_defs.h:327-330); returns the nonzero count. */
+int xo_quant(int16_t *coef, int n, int scale, int shift, int add)
+{
+    int nz = 0;
+    for (int i = 0; i < n; i++) {
+        int c = coef[i];
+        int s = c < 0 ? -1 : 1;
+        coef[i] = (int16_t)(s * ((iabs(c) * scale + add) >> shift));
+        nz += coef[i] != 0;
+    }
+    return nz;
+}
+
+/* dequant_c, quant.c:173-184. */
+void xo_dequant(int16_t *coef, int n, int scale, int shift)
+{
+    const int add = 1 << (shift - 1);
+    for (int i = 0; i < n; i++)
+        if (coef[i] != 0)
+            coef[i] = (int16_t)clip3(-32768, 32767, (coef[i] * scale + add) >> shift);
+}
+
+/* tu_quant_forward, rdo.c:658-669: shift = 15 + 16 - 9 - log2N and
+ * add = floor(2^(shift + b_intra) * 5 / 31) (table at rdo.c:662-666). */
+void xo_quant_params(int log2n, int qp, int b_intra, int *scale, int *shift, int *add)
+{
+    int s = 15 + 16 - (8 + 1) - log2n;
+    *scale = xo_Q_TAB[qp];
+    *shift = s;
+    *add = (int)(((int64_t)1 << (s + b_intra)) * 5 / 31);
+}
+
+/* tu_quant_inverse, rdo.c:708-711. */
+void xo_dequant_params(int log2n, int qp, int *scale, int *shift)
+{
+    *scale = xo_IQ_TAB[qp];
+    *shift = xo_IQ_SHIFT[qp] + (8 + 1) + log2n - 16;
+}
+
+void xo_residual_batch(int n, int qp, int b_intra, const int16_t *in, int16_t *level,
+                       int16_t *recon, int32_t *nnz, int nblk)
+{
+    int l = ilog2(n), qs, qsh, qa, ds, dsh;
+    xo_quant_params(l, qp, b_intra, &qs, &qsh, &qa);
+    xo_dequant_params(l, qp, &ds, &dsh);
+    int16_t c[32 * 32];
+    const int nn = n * n;
+    for (int b = 0; b < nblk; b++) {
+        xo_dct(n, in + (size_t)b * nn, c, n);
+        nnz[b] = xo_quant(c, nn, qs, qsh, qa);
+        memcpy(level + (size_t)b * nn, c, sizeof(int16_t) * (size_t)nn);
+        xo_dequant(c, nn, ds, dsh);
+        xo_idct(n, c, recon + (size_t)b * nn, n);
+    }
+}

@@ -1,0 +1,152 @@
+/*
+ * xme_oracle.h -- CPU restatement of xavs2's motion-estimation cost path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by the CPU-baseline leg of bench.py,
+ * and by nothing else.  The product (libxavs2_amd.so) never links, calls or
+ * falls back to it.
+ *
+ * Every function restates the arithmetic of the reference (pkuvcl/xavs2,
+ * 8-bit build) and cites the file:line it follows.  Pixels are uint8_t
+ * (pel_t), coefficients int16_t (coeff_t), distortions int32_t (cmp_dist_t),
+ * see source/common/basic_types.h:50-62.
+ *
+ * Parity pins (which independent check covers which function) are listed in
+ * the header comment of xme_oracle.c.
+ */
+#ifndef XME_ORACLE_H
+#define XME_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XO_FENC_STRIDE    64          /* common/common.h:1350 */
+#define XO_MAX_DISTORTION (1 << 30)   /* common/defines.h:295 */
+#define XO_PAD            80          /* common/defines.h:298 (XAVS2_PAD) */
+#define XO_NUM_PU         25          /* common/pixel.h:50-65 */
+
+/* PU sizes in LumaPU order (common/pixel.h:50-65). */
+extern const uint8_t xo_pu_w[XO_NUM_PU];
+extern const uint8_t xo_pu_h[XO_NUM_PU];
+int xo_part_index(int w, int h);      /* PART_INDEX, pixel.h:178 / pixel.c:60-78; -1 if invalid */
+
+/* ---- pixel cost primitives (common/pixel.c) ---------------------------- */
+int32_t xo_sad(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2);
+void    xo_sad_x3(int w, int h, const uint8_t *fenc, const uint8_t *p0, const uint8_t *p1,
+                  const uint8_t *p2, intptr_t i_ref, int32_t res[3]);
+void    xo_sad_x4(int w, int h, const uint8_t *fenc, const uint8_t *p0, const uint8_t *p1,
+                  const uint8_t *p2, const uint8_t *p3, intptr_t i_ref, int32_t res[4]);
+int32_t xo_satd(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2);
+void    xo_avg(int w, int h, uint8_t *dst, intptr_t ds, const uint8_t *s0, intptr_t ss0,
+               const uint8_t *s1, intptr_t ss1);
+
+/* Batched helpers for tests: call i compares p1 + off1[i] with p2 + off2[i]. */
+void xo_sad_batch(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2,
+                  const int64_t *off1, const int64_t *off2, int n, int32_t *out);
+void xo_satd_batch(int w, int h, const uint8_t *p1, intptr_t s1, const uint8_t *p2, intptr_t s2,
+                   const int64_t *off1, const int64_t *off2, int n, int32_t *out);
+
+/* ---- MV rate (encoder/me.c:614-641) ------------------------------------ */
+int  xo_mvbits_max_mvd(int search_range);
+/* Value of h->mvbits[v]; v must satisfy |v| <= xo_mvbits_max_mvd(). */
+int  xo_mvbits(int search_range, int v);
+/* LAMBDA_FACTOR(sqrt(lambda)), common/common.h:85, encoder/encoder.c:329 */
+int  xo_lambda_factor(double lambda_mode);
+
+/* ---- ME search (encoder/me.c, encoder/md_inter.c) ---------------------- */
+typedef struct {
+    int frame_w, frame_h;             /* h->i_width, h->i_height */
+    int search_range;                 /* h->param->search_range (full search: [-R, R)) */
+    int lambda_factor;                /* h->i_lambda_factor */
+    int min_mv_range[2];              /* h->min_mv_range (qpel), encoder.c:849-892 */
+    int max_mv_range[2];              /* h->max_mv_range (qpel) */
+    int use_hadamard;                 /* h->param->enable_hadamard: fpel_cmp = satd : sad */
+    int use_fractional_me;            /* 0 none, 1 half-pel, 2 half + quarter */
+    int use_fast_sub_me;              /* Spiral2 with step 2 */
+    int enable_pmvr;                  /* h->param->enable_pmvr */
+} xo_me_param_t;
+
+typedef struct {
+    int pix_x, pix_y;                 /* PU position in the frame */
+    int w, h;                         /* PU size (one of the 25 LumaPU sizes) */
+    int mvp_x, mvp_y;                 /* MV predictor, qpel */
+} xo_pu_t;
+
+typedef struct {
+    int32_t cost;                     /* return value of xavs2_me_search */
+    int32_t bcost2;                   /* integer-pel best cost (p_me->bcost2) */
+    int16_t bmv_x, bmv_y;             /* p_me->bmv, qpel */
+    int16_t bmv2_x, bmv2_y;           /* p_me->bmv2, full-pel */
+    int32_t mvcost;                   /* p_me->mvcost[PDIR_FWD] */
+    int32_t status;                   /* 0 ok; 1 no legal MV candidate (_me_error) */
+} xo_me_result_t;
+
+/* Full search + sub-pel refinement for one PU against one reference.
+ *   cur    : current luma plane origin, stride i_cur (PU at cur + pix_y*i_cur + pix_x)
+ *   planes : planes[0] = padded full-pel reference origin; planes[1..15] the
+ *            interpolated planes (may be NULL when use_fractional_me == 0).
+ *   cost_map (optional, may be NULL): (2R)*(2R) integer-pel costs of the raster
+ *            pass, row-major by (j+R, i+R); illegal candidates hold -1.
+ */
+int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
+                     const uint8_t *cur, intptr_t i_cur,
+                     const uint8_t *const planes[16], intptr_t i_ref,
+                     xo_me_result_t *res, int32_t *cost_map);
+
+/* The same over n PUs on nthreads POSIX threads (CPU baseline). */
+void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,
+                        const uint8_t *cur, intptr_t i_cur,
+                        const uint8_t *const planes[16], intptr_t i_ref,
+                        xo_me_result_t *res, int nthreads);
+
+/* ---- frame borders and interpolated planes (common/frame.c, common/mc.c) */
+/* plane_expand_border over a whole plane (frame.c:470-507, :513-535). */
+void xo_expand_border(uint8_t *plane, intptr_t stride, int w, int h, int pad);
+/* interpolate_lcu_row for every LCU row of a frame (mc.c:593-785).
+ * planes[0] must already be border-expanded by XO_PAD.  Planes 1..15 get the
+ * same stride and padding. */
+int  xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int h,
+                          int lcu_level, int use_fractional_me);
+
+/* cu_get_mc_pos (common/mc.h:52-66). */
+int  xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv);
+/* Block MC (mc.c:275-398, :559-589) and chroma MC (mc.c:797-829).
+ * pix_quad_x/y are absolute positions: 4 * luma position + MV (qpel). */
+void xo_mc_luma(uint8_t *dst, intptr_t i_dst, int pix_quad_x, int pix_quad_y, int w, int h,
+                const uint8_t *const planes[16], intptr_t i_ref);
+void xo_mc_chroma(uint8_t *dst_u, uint8_t *dst_v, intptr_t i_dst, int pix_quad_x, int pix_quad_y,
+                  int w, int h, const uint8_t *plane_u, const uint8_t *plane_v, intptr_t i_src);
+void xo_luma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx);
+void xo_luma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy);
+void xo_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy);
+void xo_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx);
+void xo_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy);
+void xo_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy);
+
+/* ---- transform and quantisation (common/transform.c, common/quant.c) --- */
+void xo_dct(int n, const int16_t *src, int16_t *dst, int i_src);   /* dct_NxN_c, dst contiguous */
+void xo_idct(int n, const int16_t *src, int16_t *dst, int i_dst);  /* idct_NxN_c, src contiguous */
+int  xo_quant(int16_t *coef, int n, int scale, int shift, int add);
+void xo_dequant(int16_t *coef, int n, int scale, int shift);
+/* Parameters of tu_quant_forward / tu_quant_inverse (encoder/rdo.c:658-718). */
+void xo_quant_params(int log2n, int qp, int b_intra, int *scale, int *shift, int *add);
+void xo_dequant_params(int log2n, int qp, int *scale, int *shift);
+/* DCT -> quant -> dequant -> IDCT on nblk contiguous n*n blocks.
+ * level = quantised coefficients, recon = reconstructed residual, nnz per block. */
+void xo_residual_batch(int n, int qp, int b_intra, const int16_t *in, int16_t *level,
+                       int16_t *recon, int32_t *nnz, int nblk);
+
+/* Row-major N x N AVS2 transform matrix, N in {4, 8, 16, 32}. */
+const int16_t *xo_tmat(int n);
+extern const uint16_t xo_Q_TAB[80], xo_IQ_TAB[80];
+extern const uint8_t xo_IQ_SHIFT[80];
+extern const int8_t xo_luma_taps[4][8], xo_chroma_taps[8][4];
+
+#ifdef __cplusplus
+}
+#endif
+#endif
