@@ -1,0 +1,245 @@
+/*
+ * xavs2_amd.h -- C ABI of libxavs2_amd.so, the MI355X (gfx950) motion-estimation
+ * cost path for the AVS2 encoder xavs2.
+ *
+ * The reference encoder reaches its ME arithmetic only through the global
+ * primitive table `intrinsic_func_t g_funcs` (source/common/primitives.h:131-223,
+ * filled by xavs2_init_all_primitives, source/common/primitives.c:53-80) and the
+ * search driver xavs2_me_search (source/encoder/me.c:879).  This header exposes
+ *
+ *   1. the per-call primitive entry points with exactly the g_funcs signatures
+ *      (host pointers, synchronous; for installing into g_funcs as a drop-in),
+ *   2. batched device-side versions of the same primitives (device pointers,
+ *      asynchronous on a caller stream) -- one call replaces n table calls,
+ *   3. the frame-level ME job that replaces the per-PU xavs2_me_search calls of
+ *      one frame (full search + sub-pel refinement over every PU and reference)
+ *      in one persistent launch, replayable from a captured HIP graph,
+ *   4. the reference-plane producer (interpolate_lcu_row over a frame), chroma
+ *      MC, and the residual path DCT -> quant -> dequant -> IDCT,
+ *   5. a continuous-batching ME server for many concurrent encodes.
+ *
+ * Every entry point returns XME_OK (0) or a negative XME_ERR_* code; the text
+ * of the last error on the calling thread is available from xme_last_error().
+ * Pointers named d_* are device pointers (from xme_malloc or any HIP
+ * allocation on the current device); `stream` is a hipStream_t or NULL.
+ * Results are bit-exact with the reference's C primitives for 8-bit video.
+ */
+#ifndef XAVS2_AMD_H
+#define XAVS2_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XME_OK           0
+#define XME_ERR_ARG     -1   /* invalid argument (size, shape, range) */
+#define XME_ERR_HIP     -2   /* HIP runtime error */
+#define XME_ERR_NOMEM   -3   /* allocation failed */
+#define XME_ERR_BUDGET  -4   /* server: job does not fit the memory budget */
+#define XME_ERR_STATE   -5   /* object used in the wrong state */
+
+#define XME_NUM_PU      25   /* enum LumaPU, source/common/pixel.h:50-65 */
+#define XME_FENC_STRIDE 64   /* FENC_STRIDE, source/common/common.h:1350 */
+#define XME_PAD         80   /* XAVS2_PAD, source/common/defines.h:298 */
+#define XME_MAX_REFS    8
+#define XME_MAX_DISTORTION (1 << 30)  /* source/common/defines.h:295 */
+
+typedef uint8_t pel_t;       /* source/common/basic_types.h:50 */
+typedef int16_t coeff_t;     /* source/common/basic_types.h:60 */
+
+const char *xme_last_error(void);
+/* Library version string. */
+const char *xme_version(void);
+
+/* ---- device, memory and streams --------------------------------------- */
+int xme_device_count(int *count);
+int xme_set_device(int device);
+int xme_malloc(void **d_ptr, size_t bytes);
+int xme_free(void *d_ptr);
+int xme_host_alloc(void **h_ptr, size_t bytes);    /* pinned host memory */
+int xme_host_free(void *h_ptr);
+int xme_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int xme_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int xme_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int xme_memset(void *d_dst, int value, size_t bytes, void *stream);
+int xme_stream_create(void **stream);
+int xme_stream_destroy(void *stream);
+int xme_stream_sync(void *stream);
+int xme_device_sync(void);
+
+/* ---- 1. per-call primitives with the g_funcs signatures ---------------- *
+ * pixel_cmp_t     (source/common/pixel.h:124): pixf.sad[part], pixf.satd[part]
+ * pixel_cmp_x3_t  (pixel.h:127):               pixf.sad_x3[part] (fenc stride FENC_STRIDE)
+ * pixel_cmp_x4_t  (pixel.h:128):               pixf.sad_x4[part]
+ * Each call stages the blocks to the GPU, computes there, and returns the
+ * reference's value.  The table getters fill arrays indexed by enum LumaPU so
+ * an encoder can install them as a new g_funcs tier after
+ * xavs2_init_all_primitives (source/common/primitives.c:53-80). */
+typedef int32_t (*xme_pixel_cmp_t)(const pel_t *pix1, intptr_t i_pix1, const pel_t *pix2, intptr_t i_pix2);
+typedef void (*xme_pixel_cmp_x3_t)(const pel_t *fenc, const pel_t *pix0, const pel_t *pix1, const pel_t *pix2,
+                                   intptr_t i_stride, int32_t scores[3]);
+typedef void (*xme_pixel_cmp_x4_t)(const pel_t *fenc, const pel_t *pix0, const pel_t *pix1, const pel_t *pix2,
+                                   const pel_t *pix3, intptr_t i_stride, int32_t scores[4]);
+int xme_get_sad_table(xme_pixel_cmp_t table[XME_NUM_PU]);      /* replaces pixf.sad    (pixel.c:1141) */
+int xme_get_satd_table(xme_pixel_cmp_t table[XME_NUM_PU]);     /* replaces pixf.satd   (pixel.c:1144) */
+int xme_get_sad_x3_table(xme_pixel_cmp_x3_t table[XME_NUM_PU]);/* replaces pixf.sad_x3 (pixel.c:1142) */
+int xme_get_sad_x4_table(xme_pixel_cmp_x4_t table[XME_NUM_PU]);/* replaces pixf.sad_x4 (pixel.c:1143) */
+
+/* ---- 2. batched primitives on device buffers --------------------------- *
+ * Call i of a batch evaluates the g_funcs member on (base + off[i]) pointers. */
+/* pixf.sad[part] / pixf.satd[part] (pixel.c:93-135, :297-407). */
+int xme_sad_batch(int part, const pel_t *d_fenc, intptr_t i_fenc, const pel_t *d_fref, intptr_t i_fref,
+                  const int64_t *d_fenc_off, const int64_t *d_fref_off, int n, int32_t *d_out, void *stream);
+int xme_satd_batch(int part, const pel_t *d_fenc, intptr_t i_fenc, const pel_t *d_fref, intptr_t i_fref,
+                   const int64_t *d_fenc_off, const int64_t *d_fref_off, int n, int32_t *d_out, void *stream);
+/* pixf.sad_x3/sad_x4[part] (pixel.c:143-243): fenc has stride XME_FENC_STRIDE;
+ * d_fref_off holds 3 (or 4) offsets per call; d_scores 3 (or 4) per call. */
+int xme_sad_x3_batch(int part, const pel_t *d_fenc, const pel_t *d_fref, intptr_t i_fref,
+                     const int64_t *d_fenc_off, const int64_t *d_fref_off, int n, int32_t *d_scores, void *stream);
+int xme_sad_x4_batch(int part, const pel_t *d_fenc, const pel_t *d_fref, intptr_t i_fref,
+                     const int64_t *d_fenc_off, const int64_t *d_fref_off, int n, int32_t *d_scores, void *stream);
+
+/* ---- 3. frame-level motion estimation ---------------------------------- */
+typedef struct {
+    int32_t frame_w, frame_h;        /* h->i_width, h->i_height */
+    int32_t search_range;            /* full search over [-R, R) (me.c:1159-1160) */
+    int32_t lambda_factor;           /* h->i_lambda_factor (encoder.c:329) */
+    int32_t min_mv_range[2];         /* h->min_mv_range, qpel (encoder.c:849-892) */
+    int32_t max_mv_range[2];         /* h->max_mv_range, qpel */
+    int32_t use_hadamard;            /* enable_hadamard: fpel_cmp = satd, else sad (encoder.c:1840-1846) */
+    int32_t use_fractional_me;       /* 0 integer only, 1 + half-pel, 2 + quarter-pel */
+    int32_t use_fast_sub_me;         /* Spiral2 with step 2 (me.c:501-502) */
+    int32_t enable_pmvr;             /* PMVR quarter-pel adaptation (me.c:549-556) */
+} xme_me_params;
+
+typedef struct {
+    int16_t x, y;                    /* PU position (luma pixels) */
+    uint8_t w, h;                    /* one of the 25 LumaPU sizes */
+    uint8_t reserved[2];
+    int16_t mvp_x, mvp_y;            /* MV predictor, qpel (p_me->mvp) */
+} xme_pu;
+
+typedef struct {
+    int32_t cost;                    /* xavs2_me_search return value */
+    int32_t bcost2;                  /* p_me->bcost2: integer-pel best cost */
+    int16_t bmv_x, bmv_y;            /* p_me->bmv (qpel) */
+    int16_t bmv2_x, bmv2_y;          /* p_me->bmv2 (full-pel) */
+    int32_t mvcost;                  /* p_me->mvcost[PDIR_FWD] */
+    int32_t status;                  /* 0 ok, 1 no legal candidate (returns XME_MAX_DISTORTION) */
+} xme_result;
+
+/* One reference picture resident in device memory: planes[0] is the full-pel
+ * luma plane (origin pointer; XME_PAD pixels of replicated border around it),
+ * planes[1..15] the interpolated planes at the same stride
+ * (xavs2_frame_t::filtered, source/common/frame.c:335-367).  Unused planes may
+ * be NULL when use_fractional_me < 2. */
+typedef struct {
+    const pel_t *planes[16];
+    intptr_t stride;
+} xme_ref_desc;
+
+#define XME_PLAN_BATCHED 0   /* one persistent launch for every PU shape and reference */
+#define XME_PLAN_PLAIN   1   /* one launch per PU shape and reference (comparison path) */
+
+typedef struct xme_plan xme_plan;
+
+/* Prepare a frame layout: n_pu PUs searched against n_ref references.
+ * The plan owns the device work lists and is reused for every frame with the
+ * same layout.  Fails with XME_ERR_ARG when a PU is not a LumaPU size, when
+ * a motion-vector difference could leave the reference's mvbits table, or
+ * when n_ref > XME_MAX_REFS. */
+int xme_plan_create(const xme_me_params *params, const xme_pu *pus, int n_pu, int n_ref, int mode,
+                    xme_plan **plan);
+int xme_plan_destroy(xme_plan *plan);
+/* Per-plan counters: number of PUs x refs, integer candidates evaluated per run. */
+int xme_plan_info(const xme_plan *plan, int64_t *n_inst, int64_t *n_candidates, int64_t *n_absdiff);
+
+/* Run the search for one frame.  d_cur: current luma plane origin (no padding
+ * needed).  refs: host array of n_ref descriptors.  d_results: n_pu * n_ref
+ * entries, entry pu * n_ref + r.  d_cost_map (optional): per entry (2R)^2
+ * integer costs of the raster pass, -1 where CHECK_MV_RANGE rejects.  */
+int xme_me_run(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
+               xme_result *d_results, int32_t *d_cost_map, void *stream);
+/* Capture the kernels of xme_me_run into a HIP graph on the first call and
+ * replay it afterwards (steady-state frames).  Pointers may change between
+ * calls: they travel through a device-side frame descriptor. */
+int xme_me_run_graph(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
+                     xme_result *d_results, void *stream);
+
+/* Autotuner: time the integer-search tile shapes for (PU shape, search range)
+ * on this device and store the winners for later plans.  Returns the number
+ * of shapes tuned.  xme_tune_get reports the chosen rows-per-lane (K) and
+ * lanes-per-row (LC) for a shape. */
+int xme_tune(const xme_me_params *params, const int *parts, int n_parts, void *stream);
+int xme_tune_get(int part, int search_range, int *k_rows, int *lanes_per_row);
+
+/* ---- 4. reference planes, MC and the residual path ---------------------- */
+/* plane_expand_border over a whole plane (source/common/frame.c:470-507). */
+int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, int pad, void *stream);
+/* interpolate_lcu_row for every LCU row (source/common/mc.c:593-785): fills
+ * d_planes[1..15] (host array of device origin pointers, same stride and
+ * XME_PAD border as d_planes[0], which must already be border-expanded). */
+int xme_interpolate_frame(pel_t *const d_planes[16], intptr_t stride, int w, int h,
+                          int use_fractional_me, void *stream);
+/* mc_chroma (source/common/mc.c:797-829) for a batch of blocks: block i has
+ * chroma size w[i] x h[i] at absolute position (qx[i], qy[i]) = 4 * luma
+ * position + MV (qpel), clamped by get_mv_for_mc (mc.h:69-75) when
+ * frame_w_luma > 0.  Outputs are written at d_dst_u/v + dst_off[i]. */
+int xme_mc_chroma_batch(const pel_t *d_u, const pel_t *d_v, intptr_t i_src,
+                        const int32_t *d_blocks /* n x {luma_x, luma_y, luma_w, luma_h, mv_x, mv_y} */,
+                        int n, int frame_w_luma, int frame_h_luma,
+                        pel_t *d_dst_u, pel_t *d_dst_v, intptr_t i_dst, const int64_t *d_dst_off, void *stream);
+/* mc_luma (source/common/mc.c:559-589) from the interpolated planes, same block format. */
+int xme_mc_luma_batch(const xme_ref_desc *ref, const int32_t *d_blocks, int n, int frame_w, int frame_h,
+                      pel_t *d_dst, intptr_t i_dst, const int64_t *d_dst_off, void *stream);
+
+/* dct_NxN / idct_NxN (source/common/transform.c:1123-1312) on nblk contiguous
+ * n x n blocks (n in 4, 8, 16, 32). */
+int xme_dct_batch(int n, const coeff_t *d_src, coeff_t *d_dst, int nblk, void *stream);
+int xme_idct_batch(int n, const coeff_t *d_src, coeff_t *d_dst, int nblk, void *stream);
+/* quant_c / dequant_c (source/common/quant.c:109-184) with the parameters of
+ * tu_quant_forward / tu_quant_inverse (source/encoder/rdo.c:658-718). */
+int xme_quant_batch(int n, coeff_t *d_coef, int nblk, int qp, int b_intra, int32_t *d_nnz, void *stream);
+int xme_dequant_batch(int n, coeff_t *d_coef, int nblk, int qp, void *stream);
+/* DCT -> quant -> dequant -> IDCT fused: level = quantised coefficients,
+ * recon = reconstructed residual, nnz per block (d_nnz may be NULL). */
+int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t *d_level, coeff_t *d_recon,
+                       int32_t *d_nnz, int nblk, void *stream);
+
+/* ---- 5. serving: continuous-batching ME for many concurrent encodes ------ */
+typedef struct xme_server xme_server;
+typedef struct {
+    int64_t memory_budget;           /* bytes of device memory the server may hold */
+    int32_t max_inflight;            /* frame jobs on the device at once */
+    int32_t search_range;
+    int32_t lambda_factor;
+    int32_t n_ref;                   /* references per job */
+} xme_server_config;
+typedef struct {
+    int64_t submitted, completed, rejected;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double throughput_fps;
+    int64_t resident_bytes;
+} xme_server_stats;
+
+int xme_server_create(const xme_server_config *cfg, xme_server **srv);
+int xme_server_destroy(xme_server *srv);
+/* Register an encode (one video stream) of the given luma size. */
+int xme_server_open_stream(xme_server *srv, int width, int height, int *stream_id);
+int xme_server_close_stream(xme_server *srv, int stream_id);
+/* Submit one frame: host luma picture (width x height, stride) becomes both the
+ * current picture of this job and, afterwards, the newest reference of its stream.
+ * Returns a job id, or XME_ERR_BUDGET when admission fails. */
+int xme_server_submit(xme_server *srv, int stream_id, const pel_t *luma, intptr_t stride, int64_t *job_id);
+/* Wait for a job; copies its results (host array sized by xme_server_job_size). */
+int xme_server_job_size(xme_server *srv, int64_t job_id, int *n_results);
+int xme_server_wait(xme_server *srv, int64_t job_id, xme_result *results, double *latency_ms);
+int xme_server_stats_get(xme_server *srv, xme_server_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

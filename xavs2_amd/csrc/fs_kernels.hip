@@ -1,0 +1,418 @@
+// Integer-pel full search (xavs2_me_search, FS branch, source/encoder/me.c:879-1173)
+// for gfx950.
+//
+// Design:
+//  * a candidate-list pass evaluates the <= 2 MV candidates of every (PU, ref)
+//    in order and fixes the search centre (me.c:913-921);
+//  * the search kernel takes tasks from a work queue; a task stages one
+//    reference window and the current pixels of a group of PUs in LDS, and its
+//    waves pull items (a band of candidate rows of one PU) from an LDS counter;
+//  * a lane owns 4 adjacent candidate columns x K candidate rows.  For every
+//    current row it loads one new reference row into a K-deep register ring
+//    and issues v_qsad_pk_u16_u8 (4 candidates x 4 pixels per instruction)
+//    against each ring row, so a reference byte fetched from LDS feeds 4K
+//    absolute differences; 16-bit partial sums are folded into 32-bit ones
+//    before they can overflow;
+//  * the (cost, order) key of the best legal candidate of a wave goes to the
+//    per-(PU, ref) slot with a 64-bit atomic min, which reproduces the
+//    first-strict-minimum rule of the reference exactly.
+#include "xme_internal.h"
+#include "tune.h"
+
+#include <mutex>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Candidate-list pass: one wave per (PU, ref).
+__global__ void __launch_bounds__(64) mvc_kernel(DevParams p, const FrameDesc *__restrict__ desc,
+                                                 const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
+                                                 uint64_t *__restrict__ best, int n_inst)
+{
+    const int id = blockIdx.x;
+    if (id >= n_inst)
+        return;
+    const int lane = threadIdx.x;
+    const int pu_i = id / p.n_ref, r = id % p.n_ref;
+    const DevPu pu = pus[pu_i];
+    const MvWindow win = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
+    // add_one_mv_candidate (md_inter.c:840-865) for the MVP and (0,0)
+    int cand[2][2], n = 0;
+    const int q[2][2] = { { pu.mvp_x, pu.mvp_y }, { 0, 0 } };
+    for (int c = 0; c < 2; c++) {
+        int x = xme_clip3(win.min_fpel[0], win.max_fpel[0], (q[c][0] + 2) >> 2);
+        int y = xme_clip3(win.min_fpel[1], win.max_fpel[1], (q[c][1] + 2) >> 2);
+        bool dup = false;
+        for (int k = 0; k < n; k++)
+            dup |= cand[k][0] == x && cand[k][1] == y;
+        if (!dup) {
+            cand[n][0] = x;
+            cand[n][1] = y;
+            n++;
+        }
+    }
+    const uint8_t *cur = desc->cur + (int64_t)pu.y * desc->i_cur + pu.x;
+    const uint8_t *ref = desc->planes[r][0] + (int64_t)pu.y * desc->stride[r] + pu.x;
+    const int64_t i_ref = desc->stride[r];
+    uint64_t key = XME_KEY_NONE;
+    int bx = 0, by = 0;
+    for (int c = 0; c < n; c++) {
+        const int mx = cand[c][0], my = cand[c][1];
+        if (!xme_mv_legal(win, mx, my))
+            continue;
+        int sad = 0;
+        const uint8_t *rp = ref + (int64_t)my * i_ref + mx;
+        for (int t = lane; t < pu.w * pu.h; t += 64) {
+            const int yy = t / pu.w, xx = t % pu.w;
+            sad += abs((int)cur[(int64_t)yy * desc->i_cur + xx] - (int)rp[(int64_t)yy * i_ref + xx]);
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            sad += __shfl_xor(sad, o);
+        const int cost = sad + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+        const uint64_t k = xme_key((uint32_t)cost, (uint32_t)c);
+        if (k < key) {
+            key = k;
+            bx = mx;
+            by = my;
+        }
+    }
+    if (lane == 0) {
+        DevInst d;
+        d.omx = (int16_t)bx;
+        d.omy = (int16_t)by;
+        for (int c = 0; c < 2; c++) {
+            d.mvc[c][0] = (int16_t)(c < n ? cand[c][0] : 0);
+            d.mvc[c][1] = (int16_t)(c < n ? cand[c][1] : 0);
+        }
+        d.n_mvc = (int8_t)n;
+        d.status = key == XME_KEY_NONE ? 1 : 0;
+        d.pad = 0;
+        inst[id] = d;
+        best[id] = key;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Search kernel.
+
+// Byte-aligned dword from two aligned dwords (little endian).
+__device__ __forceinline__ uint32_t align_dw(uint32_t hi, uint32_t lo, uint32_t shift)
+{
+    return __builtin_amdgcn_alignbyte(hi, lo, shift);
+}
+
+// qsad: 4 SADs of the 4 current bytes `c` against the reference bytes
+// r[d .. d+3], d = 0..3, added to four 16-bit accumulators.
+__device__ __forceinline__ uint64_t qsad(uint32_t lo, uint32_t hi, uint32_t c, uint64_t acc)
+{
+    return __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)hi << 32) | lo, c, acc);
+}
+
+// Accumulate one strip of SW current columns over all H current rows.
+//   win  : LDS window, `pitch` bytes per row; lane reference row r starts at
+//          byte offset base + r * pitch (base may be unaligned: `shift`)
+//   cur  : LDS current pixels of this strip, `cpitch` bytes per row (4-aligned)
+template <int K, int SW>
+__device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pitch, int base, const uint8_t *__restrict__ cur,
+                                         int cpitch, int H, uint32_t (&acc)[K][4])
+{
+    constexpr int ND = SW / 4;           // current dwords per row
+    constexpr int FLUSH = 256 / SW;      // rows before a 16-bit lane could overflow (SW*rows*255 < 65536)
+    const uint32_t shift = (uint32_t)base & 3u;
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
+    const int pd = pitch >> 2;           // pitch in dwords
+    uint32_t ring[K][ND + 1];
+    uint64_t a16[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        a16[k] = 0;
+
+    auto load_row = [&](int r, uint32_t (&dst)[ND + 1]) {
+        uint32_t raw[ND + 2];
+        const uint32_t *src = w32 + r * pd;
+#pragma unroll
+        for (int t = 0; t < ND + 2; t++)
+            raw[t] = src[t];
+#pragma unroll
+        for (int t = 0; t < ND + 1; t++)
+            dst[t] = align_dw(raw[t + 1], raw[t], shift);
+    };
+
+#pragma unroll
+    for (int k = 0; k < K - 1; k++)
+        load_row(k, ring[k]);
+
+    for (int c0 = 0; c0 < H; c0 += K) {
+#pragma unroll
+        for (int cc = 0; cc < K; cc++) {
+            const int c = c0 + cc;
+            load_row(c + K - 1, ring[(cc + K - 1) % K]);
+            uint32_t cw[ND];
+            const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
+#pragma unroll
+            for (int t = 0; t < ND; t++)
+                cw[t] = cp[t];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const uint32_t(&rr)[ND + 1] = ring[(cc + k) % K];
+#pragma unroll
+                for (int t = 0; t < ND; t++)
+                    a16[k] = qsad(rr[t], rr[t + 1], cw[t], a16[k]);
+            }
+        }
+        if (((c0 + K) % FLUSH) == 0 || c0 + K >= H) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                acc[k][0] += (uint32_t)a16[k] & 0xFFFFu;
+                acc[k][1] += ((uint32_t)a16[k]) >> 16;
+                acc[k][2] += (uint32_t)(a16[k] >> 32) & 0xFFFFu;
+                acc[k][3] += (uint32_t)(a16[k] >> 48);
+                a16[k] = 0;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t t = __shfl_xor(v, o);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+
+// Process one item with K rows per lane.
+template <int K>
+__device__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, const FsItem &it, int inst_id,
+                        const uint8_t *__restrict__ win, int pitch, int wx0, int wy0, const uint8_t *__restrict__ cur,
+                        int cpitch, int cx0, int cy0, uint64_t *__restrict__ best, int32_t *__restrict__ cost_map)
+{
+    const int lane = threadIdx.x & 63;
+    const int LC = it.LC;
+    const int lc = lane % LC, rg = lane / LC;
+    const int R = p.R;
+    // LDS position of candidate (col 0, row 0): top-left of the block at
+    // MV (omx - R, omy - R)
+    const int bx = pu.x + in.omx - R - wx0;
+    const int by = pu.y + in.omy - R - wy0;
+    const int col = it.col0 + 4 * lc;
+    const int row = it.row0 + rg * K;
+    const int lane_base = (by + row) * pitch + bx + col;
+
+    uint32_t acc[K][4];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            acc[k][d] = 0;
+
+    const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
+    int x = 0;
+    for (; x + 16 <= pu.w; x += 16)
+        fs_strip<K, 16>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+    if (x + 8 <= pu.w) {
+        fs_strip<K, 8>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+        x += 8;
+    }
+    if (x + 4 <= pu.w)
+        fs_strip<K, 4>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+
+    // MV cost, legality and the (cost, order) key (me.c:187-192, me.h:70-75)
+    const MvWindow mw = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
+    const int R2 = 2 * R;
+    const int col_end = it.col0 + it.ncols, row_end = it.row0 + it.nrows;
+    uint64_t kbest = ~0ull;
+    int bits_x[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++)
+        bits_x[d] = xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = row + k;
+        const int my = in.omy + j - R;
+        const int by_bits = xme_mvbits(4 * my - pu.mvp_y);
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int i = col + d;
+            const int mx = in.omx + i - R;
+            const bool inside = i < col_end && j < row_end;
+            const bool legal = inside && xme_mv_legal(mw, mx, my);
+            const int cost = (int)acc[k][d] + xme_weighted_cost(p.lambda, bits_x[d] + by_bits);
+            const uint64_t key = legal ? xme_key((uint32_t)cost, (uint32_t)(XME_ORDER_RASTER + j * R2 + i)) : ~0ull;
+            kbest = key < kbest ? key : kbest;
+            if (cost_map != nullptr && inside)
+                cost_map[(int64_t)inst_id * R2 * R2 + j * R2 + i] = legal ? cost : -1;
+        }
+    }
+    kbest = wave_min_u64(kbest);
+    if (lane == 0 && kbest != ~0ull)
+        atomicMin(reinterpret_cast<unsigned long long *>(best + inst_id), (unsigned long long)kbest);
+}
+
+// Stage a task's reference window and current pixels in LDS.
+__device__ void stage_task(const FsTask &t, const FrameDesc *__restrict__ desc, const DevParams &p, uint8_t *lds,
+                           int pitch, int cpitch)
+{
+    const uint8_t *ref = desc->planes[t.ref][0];
+    const int64_t stride = desc->stride[t.ref];
+    // rows/columns outside the padded picture are clamped to it; only illegal
+    // candidates (never selected) can read them
+    const int xlo = -XME_PAD, xhi = p.frame_w + XME_PAD - 4;
+    const int ylo = -XME_PAD, yhi = p.frame_h + XME_PAD - 1;
+    const int ndw = t.ww >> 2;
+    uint32_t *w32 = reinterpret_cast<uint32_t *>(lds);
+    const int total = ndw * t.wh;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int r = e / ndw, q = e - r * ndw;
+        const int y = xme_clip3(ylo, yhi, t.wy0 + r);
+        const int x = t.wx0 + 4 * q;
+        const uint8_t *row = ref + (int64_t)y * stride;
+        uint32_t v;
+        if (x >= xlo && x <= xhi) {
+            v = *reinterpret_cast<const uint32_t *>(row + x);
+        } else {
+            const uint32_t b = row[xme_clip3(xlo, xhi + 3, x)];
+            v = b * 0x01010101u;
+        }
+        w32[r * (pitch >> 2) + q] = v;
+    }
+    const uint8_t *cur = desc->cur;
+    const int64_t i_cur = desc->i_cur;
+    uint32_t *c32 = reinterpret_cast<uint32_t *>(lds + pitch * t.wh);
+    const int cdw = t.cw >> 2;
+    const int ctotal = cdw * t.ch;
+    for (int e = threadIdx.x; e < ctotal; e += blockDim.x) {
+        const int r = e / cdw, q = e - r * cdw;
+        c32[r * (cpitch >> 2) + q] =
+            *reinterpret_cast<const uint32_t *>(cur + (int64_t)(t.cy0 + r) * i_cur + t.cx0 + 4 * q);
+    }
+}
+
+__device__ __forceinline__ int window_pitch(int ww)
+{
+    // dword-multiple pitch; an odd dword count spreads rows over LDS banks
+    int dw = ww >> 2;
+    return (dw | 1) << 2;
+}
+
+template <bool PERSISTENT>
+__global__ void __launch_bounds__(256) fs_kernel(DevParams p, const FrameDesc *__restrict__ desc,
+                                                 const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
+                                                 const FsTask *__restrict__ tasks, int n_tasks,
+                                                 const FsItem *__restrict__ items, uint64_t *__restrict__ best,
+                                                 uint32_t *__restrict__ counters)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ int s_task;
+    __shared__ int s_item;
+    const int32_t *cost_map = desc->cost_map;
+    int task = PERSISTENT ? -1 : (int)blockIdx.x;
+    for (;;) {
+        if (PERSISTENT) {
+            if (threadIdx.x == 0)
+                s_task = (int)atomicAdd(&counters[0], 1u);
+            __syncthreads();
+            task = s_task;
+        }
+        if (task >= n_tasks)
+            break;
+        const FsTask t = tasks[task];
+        const int pitch = window_pitch(t.ww);
+        const int cpitch = t.cw;
+        stage_task(t, desc, p, lds, pitch, cpitch);
+        if (threadIdx.x == 0)
+            s_item = t.item_begin;
+        __syncthreads();
+        const uint8_t *cur = lds + pitch * t.wh;
+        for (;;) {
+            int it_i = 0;
+            if ((threadIdx.x & 63) == 0)
+                it_i = atomicAdd(&s_item, 1);
+            it_i = __shfl(it_i, 0);
+            if (it_i >= t.item_end)
+                break;
+            const FsItem it = items[it_i];
+            const DevInst in = inst[it.inst];
+            if (in.status)
+                continue;
+            const DevPu pu = pus[it.inst / p.n_ref];
+            if (it.K == 8)
+                fs_item<8>(p, pu, in, it, it.inst, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
+                           const_cast<int32_t *>(cost_map));
+            else
+                fs_item<4>(p, pu, in, it, it.inst, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
+                           const_cast<int32_t *>(cost_map));
+        }
+        __syncthreads();   // LDS is reused by the next task
+        if (!PERSISTENT)
+            break;
+    }
+    if (PERSISTENT && threadIdx.x == 0) {
+        // the last workgroup out resets the queue for the next launch / replay
+        __threadfence();
+        const uint32_t done = atomicAdd(&counters[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&counters[0], 0u);
+            atomicExch(&counters[1], 0u);
+        }
+    }
+}
+
+} // namespace
+
+int xme_fs_max_lds(void) { return 160 * 1024 - 64; }
+
+// Allow the search kernels the full 160 KB of LDS of a gfx950 CU (once per process).
+int xme_fs_prepare(void)
+{
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        err = hipFuncSetAttribute(reinterpret_cast<const void *>(&fs_kernel<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, xme_fs_max_lds());
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute(reinterpret_cast<const void *>(&fs_kernel<false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, xme_fs_max_lds());
+    });
+    if (err != hipSuccess)
+        return xme_hip_fail(err, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+    return XME_OK;
+}
+
+int xme_launch_mvc(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
+                   uint64_t *d_best, int n_inst, hipStream_t s)
+{
+    if (n_inst <= 0)
+        return XME_OK;
+    hipLaunchKernelGGL(mvc_kernel, dim3(n_inst), dim3(64), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
+                  const FsTask *d_tasks, int n_tasks, const FsItem *d_items, uint64_t *d_best,
+                  uint32_t *d_counters, size_t lds_bytes, int grid, bool persistent, hipStream_t s)
+{
+    if (n_tasks <= 0)
+        return XME_OK;
+    int rc = xme_fs_prepare();
+    if (rc)
+        return rc;
+    if (persistent) {
+        hipLaunchKernelGGL(fs_kernel<true>, dim3(grid), dim3(256), lds_bytes, s, p, d_desc, d_pus, d_inst, d_tasks,
+                           n_tasks, d_items, d_best, d_counters);
+    } else {
+        hipLaunchKernelGGL(fs_kernel<false>, dim3(n_tasks), dim3(256), lds_bytes, s, p, d_desc, d_pus, d_inst,
+                           d_tasks, n_tasks, d_items, d_best, d_counters);
+    }
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+const void *xme_fs_kernel_ptr(bool persistent)
+{
+    return persistent ? reinterpret_cast<const void *>(&fs_kernel<true>)
+                      : reinterpret_cast<const void *>(&fs_kernel<false>);
+}

@@ -1,0 +1,724 @@
+// Host runtime of libxavs2_amd.so: errors, device memory, and the frame-level
+// motion-estimation plan (work lists for the search kernels, HIP graph replay).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "xme_internal.h"
+#include "tune.h"
+
+static thread_local char g_err[512];
+
+void xme_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int xme_hip_fail(hipError_t e, const char *what)
+{
+    xme_set_error("%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? XME_ERR_NOMEM : XME_ERR_HIP;
+}
+
+extern "C" const char *xme_last_error(void) { return g_err; }
+extern "C" const char *xme_version(void) { return "xavs2_amd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// device, memory, streams
+
+extern "C" int xme_device_count(int *count)
+{
+    XME_HIP(hipGetDeviceCount(count));
+    return XME_OK;
+}
+
+extern "C" int xme_set_device(int device)
+{
+    XME_HIP(hipSetDevice(device));
+    return XME_OK;
+}
+
+extern "C" int xme_malloc(void **p, size_t bytes)
+{
+    if (!p)
+        return XME_ERR_ARG;
+    XME_HIP(hipMalloc(p, bytes ? bytes : 1));
+    return XME_OK;
+}
+
+extern "C" int xme_free(void *p)
+{
+    if (p)
+        XME_HIP(hipFree(p));
+    return XME_OK;
+}
+
+extern "C" int xme_host_alloc(void **p, size_t bytes)
+{
+    XME_HIP(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+    return XME_OK;
+}
+
+extern "C" int xme_host_free(void *p)
+{
+    if (p)
+        XME_HIP(hipHostFree(p));
+    return XME_OK;
+}
+
+extern "C" int xme_memcpy_h2d(void *d, const void *h, size_t n, void *s)
+{
+    XME_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s));
+    XME_HIP(hipStreamSynchronize((hipStream_t)s));
+    return XME_OK;
+}
+
+extern "C" int xme_memcpy_d2h(void *h, const void *d, size_t n, void *s)
+{
+    XME_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s));
+    XME_HIP(hipStreamSynchronize((hipStream_t)s));
+    return XME_OK;
+}
+
+extern "C" int xme_memcpy_d2d(void *d, const void *src, size_t n, void *s)
+{
+    XME_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    return XME_OK;
+}
+
+extern "C" int xme_memset(void *d, int v, size_t n, void *s)
+{
+    XME_HIP(hipMemsetAsync(d, v, n, (hipStream_t)s));
+    return XME_OK;
+}
+
+extern "C" int xme_stream_create(void **s)
+{
+    hipStream_t st;
+    XME_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = st;
+    return XME_OK;
+}
+
+extern "C" int xme_stream_destroy(void *s)
+{
+    if (s)
+        XME_HIP(hipStreamDestroy((hipStream_t)s));
+    return XME_OK;
+}
+
+extern "C" int xme_stream_sync(void *s)
+{
+    XME_HIP(hipStreamSynchronize((hipStream_t)s));
+    return XME_OK;
+}
+
+extern "C" int xme_device_sync(void)
+{
+    XME_HIP(hipDeviceSynchronize());
+    return XME_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan
+
+static const uint8_t kPuW[XME_NUM_PU] = { 4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32,
+                                          16, 12, 16, 4, 32, 24, 32, 8, 64, 48, 64, 16 };
+static const uint8_t kPuH[XME_NUM_PU] = { 4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64,
+                                          12, 16, 4, 16, 24, 32, 8, 32, 48, 64, 16, 64 };
+
+int xme_part_index(int w, int h)
+{
+    for (int i = 0; i < XME_NUM_PU; i++)
+        if (kPuW[i] == w && kPuH[i] == h)
+            return i;
+    return -1;
+}
+
+int xme_pu_w(int part) { return kPuW[part]; }
+int xme_pu_h(int part) { return kPuH[part]; }
+
+// Largest |v| of the reference's mvbits table (me.c:592-612).
+static int mvbits_max_mvd(int R)
+{
+    int me_range = R > 256 ? R : 256;
+    int subpel_num = 4 * (2 * me_range + 3);
+    int max_bits = 5 + 2 * (int)std::ceil(std::log((double)subpel_num + 1) / std::log(2.0) + 1e-10);
+    return (1 << (max_bits >> 1)) - 1;
+}
+
+struct DescSlot {
+    FrameDesc *host;   // pinned
+    hipEvent_t ev;
+    bool used;
+};
+
+struct LaunchGroup {       // plain path: one launch per PU shape
+    int task_begin, task_end;
+    size_t lds;
+};
+
+struct xme_plan {
+    int device;
+    int mode;
+    DevParams p;
+    int n_pu, n_ref, n_inst;
+    int64_t n_cand, n_absdiff;
+    DevPu *d_pus = nullptr;
+    DevInst *d_inst = nullptr;
+    uint64_t *d_best = nullptr;
+    FsTask *d_tasks = nullptr;
+    FsItem *d_items = nullptr;
+    uint32_t *d_counters = nullptr;
+    FrameDesc *d_desc = nullptr;
+    int n_tasks = 0;
+    size_t lds_bytes = 0;
+    int grid = 0;
+    std::vector<LaunchGroup> groups;
+    DescSlot slots[8];
+    int slot_next = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+struct Rect {
+    int x0, y0, x1, y1;   // half-open
+    void add(const Rect &o)
+    {
+        x0 = std::min(x0, o.x0);
+        y0 = std::min(y0, o.y0);
+        x1 = std::max(x1, o.x1);
+        y1 = std::max(y1, o.y1);
+    }
+};
+
+struct HostItem {
+    FsItem it;
+    int pu;
+};
+
+// The possible full-search centres of a PU: the clipped MVP and the clipped
+// (0,0) (md_inter.c:957-959).  Which one wins depends on the pixels; the
+// window covers both.
+static int possible_centres(const DevParams &p, const DevPu &pu, int c[2][2])
+{
+    MvWindow w = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
+    const int q[2][2] = { { pu.mvp_x, pu.mvp_y }, { 0, 0 } };
+    int n = 0;
+    for (int k = 0; k < 2; k++) {
+        int x = xme_clip3(w.min_fpel[0], w.max_fpel[0], (q[k][0] + 2) >> 2);
+        int y = xme_clip3(w.min_fpel[1], w.max_fpel[1], (q[k][1] + 2) >> 2);
+        bool dup = false;
+        for (int j = 0; j < n; j++)
+            dup |= c[j][0] == x && c[j][1] == y;
+        if (!dup) {
+            c[n][0] = x;
+            c[n][1] = y;
+            n++;
+        }
+    }
+    return n;
+}
+
+// LDS read footprint of one item (picture coordinates), see fs_strip.
+static Rect item_footprint(const DevParams &p, const DevPu &pu, const FsItem &it, int cx, int cy)
+{
+    const int R = p.R;
+    const int rg = 64 / it.LC;
+    Rect r;
+    const int bx = pu.x + cx - R, by = pu.y + cy - R;
+    r.x0 = bx + it.col0 - 3;
+    r.x1 = bx + it.col0 + 4 * it.LC + pu.w + 8;
+    r.y0 = by + it.row0;
+    r.y1 = by + it.row0 + rg * it.K + pu.h - 1;
+    return r;
+}
+
+static int window_pitch_host(int ww)
+{
+    int dw = ww >> 2;
+    return (dw | 1) << 2;
+}
+
+static void items_for(const DevParams &p, const DevPu &pu, int part, int inst, int row_lo, int row_hi,
+                      std::vector<HostItem> &out)
+{
+    TuneChoice tc = xme_tune_lookup(part, p.R);
+    const int K = (pu.h % tc.K == 0) ? tc.K : 4;
+    const int LC = tc.LC;
+    const int band_rows = (64 / LC) * K, band_cols = 4 * LC;
+    const int R2 = 2 * p.R;
+    for (int r0 = row_lo; r0 < row_hi; r0 += band_rows)
+        for (int c0 = 0; c0 < R2; c0 += band_cols) {
+            HostItem h;
+            h.it.inst = inst;
+            h.it.row0 = (int16_t)r0;
+            h.it.nrows = (int16_t)std::min(band_rows, row_hi - r0);
+            h.it.col0 = (int16_t)c0;
+            h.it.ncols = (int16_t)std::min(band_cols, R2 - c0);
+            h.it.K = (uint8_t)K;
+            h.it.LC = (uint8_t)LC;
+            h.it.cost = (uint16_t)std::min(65535, (pu.w * pu.h) >> 4);
+            h.pu = inst / p.n_ref;
+            out.push_back(h);
+        }
+}
+
+// Build a task from items of one reference.  Returns the LDS bytes it needs.
+static size_t make_task(const DevParams &p, const std::vector<DevPu> &pus, const std::vector<HostItem> &its,
+                        int ref, FsTask &t)
+{
+    Rect win = { 1 << 30, 1 << 30, -(1 << 30), -(1 << 30) };
+    Rect cur = win;
+    for (const HostItem &h : its) {
+        const DevPu &pu = pus[h.pu];
+        int c[2][2];
+        int n = possible_centres(p, pu, c);
+        for (int k = 0; k < n; k++)
+            win.add(item_footprint(p, pu, h.it, c[k][0], c[k][1]));
+        cur.add(Rect{ pu.x, pu.y, pu.x + pu.w, pu.y + pu.h });
+    }
+    t.ref = ref;
+    const int wx0 = (int)std::floor(win.x0 / 4.0) * 4;
+    t.wx0 = (int16_t)wx0;
+    t.wy0 = (int16_t)win.y0;
+    t.ww = (int16_t)(((win.x1 - wx0) + 3) & ~3);
+    t.wh = (int16_t)(win.y1 - win.y0);
+    t.cx0 = (int16_t)cur.x0;
+    t.cy0 = (int16_t)cur.y0;
+    t.cw = (int16_t)(((cur.x1 - cur.x0) + 3) & ~3);
+    t.ch = (int16_t)(cur.y1 - cur.y0);
+    return (size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch;
+}
+
+} // namespace
+
+static void plan_free(xme_plan *pl)
+{
+    hipFree(pl->d_pus);
+    hipFree(pl->d_inst);
+    hipFree(pl->d_best);
+    hipFree(pl->d_tasks);
+    hipFree(pl->d_items);
+    hipFree(pl->d_counters);
+    hipFree(pl->d_desc);
+    for (auto &s : pl->slots) {
+        if (s.host)
+            hipHostFree(s.host);
+        if (s.ev)
+            hipEventDestroy(s.ev);
+    }
+    if (pl->graph_exec)
+        hipGraphExecDestroy(pl->graph_exec);
+    if (pl->graph)
+        hipGraphDestroy(pl->graph);
+    delete pl;
+}
+
+extern "C" int xme_plan_destroy(xme_plan *pl)
+{
+    if (pl)
+        plan_free(pl);
+    return XME_OK;
+}
+
+extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, int n_pu, int n_ref, int mode,
+                               xme_plan **out)
+{
+    if (!prm || !pus_in || !out || n_pu <= 0 || n_ref <= 0 || n_ref > XME_MAX_REFS ||
+        (mode != XME_PLAN_BATCHED && mode != XME_PLAN_PLAIN)) {
+        xme_set_error("xme_plan_create: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (prm->search_range < 1 || prm->search_range > 128 || prm->frame_w < 8 || prm->frame_h < 8 ||
+        prm->frame_w > 8192 || prm->frame_h > 8192 || prm->use_fractional_me < 0 || prm->use_fractional_me > 2) {
+        xme_set_error("xme_plan_create: search range must be 1..128 and the frame 8..8192 pixels");
+        return XME_ERR_ARG;
+    }
+    DevParams p;
+    p.frame_w = prm->frame_w;
+    p.frame_h = prm->frame_h;
+    p.R = prm->search_range;
+    p.lambda = prm->lambda_factor;
+    p.min_mv[0] = prm->min_mv_range[0];
+    p.min_mv[1] = prm->min_mv_range[1];
+    p.max_mv[0] = prm->max_mv_range[0];
+    p.max_mv[1] = prm->max_mv_range[1];
+    p.hadamard = prm->use_hadamard ? 1 : 0;
+    p.frac = prm->use_fractional_me;
+    p.fast = prm->use_fast_sub_me ? 1 : 0;
+    p.pmvr = prm->enable_pmvr ? 1 : 0;
+    p.n_pu = n_pu;
+    p.n_ref = n_ref;
+
+    const int max_mvd = mvbits_max_mvd(p.R);
+    std::vector<DevPu> pus(n_pu);
+    std::vector<int> parts(n_pu);
+    for (int i = 0; i < n_pu; i++) {
+        const xme_pu &q = pus_in[i];
+        int part = xme_part_index(q.w, q.h);
+        if (part < 0 || (q.x & 3) || q.x < 0 || q.y < 0 || q.x + q.w > p.frame_w || q.y + q.h > p.frame_h) {
+            xme_set_error("xme_plan_create: PU %d (%d,%d %dx%d) is not a LumaPU inside the frame at x %% 4 == 0",
+                          i, q.x, q.y, q.w, q.h);
+            return XME_ERR_ARG;
+        }
+        DevPu &d = pus[i];
+        d.x = q.x;
+        d.y = q.y;
+        d.w = q.w;
+        d.h = q.h;
+        d.part = (uint8_t)part;
+        d.pad0 = 0;
+        d.mvp_x = q.mvp_x;
+        d.mvp_y = q.mvp_y;
+        d.pad1 = 0;
+        parts[i] = part;
+        MvWindow w = xme_mv_window(p, d.x, d.y, d.w, d.h);
+        // every MV difference the search can form must stay inside mvbits
+        for (int a = 0; a < 2; a++) {
+            const int pm = a ? d.mvp_y : d.mvp_x;
+            const int lo = std::min(w.min_fpel[a], (pm + 2) >> 2) - 1, hi = std::max(w.max_fpel[a], (pm + 2) >> 2) + 1;
+            if (std::abs(4 * lo - pm) + 3 > max_mvd || std::abs(4 * hi - pm) + 3 > max_mvd) {
+                xme_set_error("xme_plan_create: PU %d: MV differences leave the mvbits table", i);
+                return XME_ERR_ARG;
+            }
+        }
+    }
+
+    xme_plan *pl = new xme_plan();
+    hipGetDevice(&pl->device);
+    pl->mode = mode;
+    pl->p = p;
+    pl->n_pu = n_pu;
+    pl->n_ref = n_ref;
+    pl->n_inst = n_pu * n_ref;
+
+    // work accounting: legal integer candidates and absolute differences
+    pl->n_cand = 0;
+    pl->n_absdiff = 0;
+    for (int i = 0; i < n_pu; i++) {
+        MvWindow w = xme_mv_window(p, pus[i].x, pus[i].y, pus[i].w, pus[i].h);
+        int c[2][2];
+        possible_centres(p, pus[i], c);
+        // with the candidate-list winner unknown on the host, count around the
+        // clipped (0,0) centre (exact whenever the MVP is (0,0))
+        const int cx = c[0][0], cy = c[0][1];
+        int64_t nx = 0, ny = 0;
+        for (int k = -p.R; k < p.R; k++) {
+            nx += (cx + k >= w.min_fpel[0] && cx + k <= w.max_fpel[0]);
+            ny += (cy + k >= w.min_fpel[1] && cy + k <= w.max_fpel[1]);
+        }
+        pl->n_cand += nx * ny * n_ref;
+        pl->n_absdiff += nx * ny * n_ref * pus[i].w * pus[i].h;
+    }
+
+    const size_t lds_cap = (size_t)xme_fs_max_lds();
+    std::vector<FsTask> tasks;
+    std::vector<FsItem> items;
+    std::vector<int> task_part;
+    const int R2 = 2 * p.R;
+
+    auto emit_task = [&](const std::vector<HostItem> &its, int ref, int part) -> size_t {
+        FsTask t;
+        size_t lds = make_task(p, pus, its, ref, t);
+        t.item_begin = (int)items.size();
+        // larger PUs first inside a task: the last items to finish are small
+        std::vector<HostItem> sorted = its;
+        std::stable_sort(sorted.begin(), sorted.end(),
+                         [](const HostItem &a, const HostItem &b) { return a.it.cost > b.it.cost; });
+        for (const HostItem &h : sorted)
+            items.push_back(h.it);
+        t.item_end = (int)items.size();
+        tasks.push_back(t);
+        task_part.push_back(part);
+        return lds;
+    };
+
+    if (mode == XME_PLAN_BATCHED) {
+        // group PUs by 64x64 CTU; split the candidate rows until the window fits
+        std::map<std::pair<int, int>, std::vector<int>> ctus;
+        for (int i = 0; i < n_pu; i++)
+            ctus[{ pus[i].y >> 6, pus[i].x >> 6 }].push_back(i);
+        size_t lds_max = 0;
+        for (auto &kv : ctus) {
+            for (int r = 0; r < n_ref; r++) {
+                int splits = 1;
+                for (;;) {
+                    bool ok = true;
+                    std::vector<std::vector<HostItem>> groups(splits);
+                    const int rows_per = (R2 + splits - 1) / splits;
+                    for (int s = 0; s < splits; s++) {
+                        for (int i : kv.second)
+                            items_for(p, pus[i], parts[i], i * n_ref + r, s * rows_per,
+                                      std::min(R2, (s + 1) * rows_per), groups[s]);
+                        FsTask t;
+                        if (!groups[s].empty() && make_task(p, pus, groups[s], r, t) > lds_cap / 3)
+                            ok = false;
+                    }
+                    if (ok || splits >= R2 / 4) {
+                        for (auto &g : groups)
+                            if (!g.empty())
+                                lds_max = std::max(lds_max, emit_task(g, r, -1));
+                        break;
+                    }
+                    splits *= 2;
+                }
+            }
+        }
+        pl->lds_bytes = lds_max;
+        // longest tasks first (all CTU tasks have similar work; edges are smaller)
+        std::vector<int> order(tasks.size());
+        for (size_t i = 0; i < order.size(); i++)
+            order[i] = (int)i;
+        auto work = [&](const FsTask &t) {
+            int64_t s = 0;
+            for (int k = t.item_begin; k < t.item_end; k++)
+                s += items[k].cost;
+            return s;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(tasks[a]) > work(tasks[b]); });
+        std::vector<FsTask> sorted;
+        for (int i : order)
+            sorted.push_back(tasks[i]);
+        tasks.swap(sorted);
+    } else {
+        // plain: every (PU, ref) is its own task; one launch per PU shape
+        for (int part = 0; part < XME_NUM_PU; part++) {
+            LaunchGroup g;
+            g.task_begin = (int)tasks.size();
+            g.lds = 0;
+            for (int i = 0; i < n_pu; i++) {
+                if (parts[i] != part)
+                    continue;
+                for (int r = 0; r < n_ref; r++) {
+                    int splits = 1;
+                    for (;;) {
+                        const int rows_per = (R2 + splits - 1) / splits;
+                        std::vector<std::vector<HostItem>> groups(splits);
+                        bool ok = true;
+                        for (int s = 0; s < splits; s++) {
+                            items_for(p, pus[i], part, i * n_ref + r, s * rows_per, std::min(R2, (s + 1) * rows_per),
+                                      groups[s]);
+                            FsTask t;
+                            if (make_task(p, pus, groups[s], r, t) > lds_cap / 2)
+                                ok = false;
+                        }
+                        if (ok || splits >= R2 / 4) {
+                            for (auto &gg : groups)
+                                if (!gg.empty())
+                                    g.lds = std::max(g.lds, emit_task(gg, r, part));
+                            break;
+                        }
+                        splits *= 2;
+                    }
+                }
+            }
+            g.task_end = (int)tasks.size();
+            if (g.task_end > g.task_begin)
+                pl->groups.push_back(g);
+        }
+    }
+    pl->n_tasks = (int)tasks.size();
+
+    for (const FsTask &t : tasks) {
+        size_t need = (size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch;
+        if (need > lds_cap) {
+            xme_set_error("xme_plan_create: a search window needs %zu bytes of LDS", need);
+            plan_free(pl);
+            return XME_ERR_ARG;
+        }
+    }
+
+    hipError_t e = hipSuccess;
+    auto up = [&](void **d, const void *h, size_t n) {
+        if (e != hipSuccess)
+            return;
+        e = hipMalloc(d, n ? n : 16);
+        if (e == hipSuccess && n)
+            e = hipMemcpy(*d, h, n, hipMemcpyHostToDevice);
+    };
+    up((void **)&pl->d_pus, pus.data(), pus.size() * sizeof(DevPu));
+    up((void **)&pl->d_tasks, tasks.data(), tasks.size() * sizeof(FsTask));
+    up((void **)&pl->d_items, items.data(), items.size() * sizeof(FsItem));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&pl->d_inst, sizeof(DevInst) * pl->n_inst);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&pl->d_best, sizeof(uint64_t) * pl->n_inst);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&pl->d_counters, 64);
+    if (e == hipSuccess)
+        e = hipMemset(pl->d_counters, 0, 64);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&pl->d_desc, sizeof(FrameDesc));
+    for (auto &s : pl->slots) {
+        s.host = nullptr;
+        s.ev = nullptr;
+        s.used = false;
+        if (e == hipSuccess)
+            e = hipHostMalloc((void **)&s.host, sizeof(FrameDesc), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&s.ev, hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        int rc = xme_hip_fail(e, "xme_plan_create");
+        plan_free(pl);
+        return rc;
+    }
+    if (mode == XME_PLAN_BATCHED) {
+        int prc = xme_fs_prepare();
+        if (prc) {
+            plan_free(pl);
+            return prc;
+        }
+        int per_cu = 0;
+        hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, xme_fs_kernel_ptr(true), 256,
+                                                                      pl->lds_bytes);
+        int cus = 0;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
+        if (oe != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        pl->grid = std::max(1, std::min(per_cu * cus, pl->n_tasks));
+    }
+    *out = pl;
+    return XME_OK;
+}
+
+extern "C" int xme_plan_info(const xme_plan *pl, int64_t *n_inst, int64_t *n_cand, int64_t *n_absdiff)
+{
+    if (!pl)
+        return XME_ERR_ARG;
+    if (n_inst)
+        *n_inst = pl->n_inst;
+    if (n_cand)
+        *n_cand = pl->n_cand;
+    if (n_absdiff)
+        *n_absdiff = pl->n_absdiff;
+    return XME_OK;
+}
+
+int xme_plan_debug(const xme_plan *pl, int *n_tasks, int64_t *lds_bytes, int *grid)
+{
+    *n_tasks = pl->n_tasks;
+    *lds_bytes = (int64_t)pl->lds_bytes;
+    *grid = pl->grid;
+    return XME_OK;
+}
+
+static int fill_desc(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_ref_desc *refs, xme_result *res,
+                     int32_t *cost_map, hipStream_t s)
+{
+    if (!cur || !refs || !res || (reinterpret_cast<uintptr_t>(cur) & 3) || (i_cur & 3)) {
+        xme_set_error("xme_me_run: the current plane must be 4-byte aligned with a stride multiple of 4");
+        return XME_ERR_ARG;
+    }
+    for (int r = 0; r < pl->n_ref; r++) {
+        if (!refs[r].planes[0] || (reinterpret_cast<uintptr_t>(refs[r].planes[0]) & 3) || (refs[r].stride & 3) ||
+            refs[r].stride < pl->p.frame_w + 2 * XME_PAD) {
+            xme_set_error("xme_me_run: reference %d needs a 4-byte aligned padded plane (stride >= width + %d)", r,
+                          2 * XME_PAD);
+            return XME_ERR_ARG;
+        }
+        for (int k = 1; k < 16; k++) {
+            const bool need = pl->p.frac == 2 || (pl->p.frac == 1 && (k == 2 || k == 8 || k == 10));
+            if (need && (!refs[r].planes[k] || (reinterpret_cast<uintptr_t>(refs[r].planes[k]) & 3))) {
+                xme_set_error("xme_me_run: reference %d lacks interpolated plane %d", r, k);
+                return XME_ERR_ARG;
+            }
+        }
+    }
+    DescSlot &slot = pl->slots[pl->slot_next];
+    pl->slot_next = (pl->slot_next + 1) % 8;
+    if (slot.used)
+        XME_HIP(hipEventSynchronize(slot.ev));
+    FrameDesc *d = slot.host;
+    memset(d, 0, sizeof(*d));
+    d->cur = cur;
+    d->i_cur = i_cur;
+    for (int r = 0; r < pl->n_ref; r++) {
+        for (int k = 0; k < 16; k++)
+            d->planes[r][k] = refs[r].planes[k];
+        d->stride[r] = refs[r].stride;
+    }
+    d->results = res;
+    d->cost_map = cost_map;
+    XME_HIP(hipMemcpyAsync(pl->d_desc, d, sizeof(FrameDesc), hipMemcpyHostToDevice, s));
+    XME_HIP(hipEventRecord(slot.ev, s));
+    slot.used = true;
+    return XME_OK;
+}
+
+static int launch_all(xme_plan *pl, hipStream_t s)
+{
+    int rc = xme_launch_mvc(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
+    if (rc)
+        return rc;
+    if (pl->mode == XME_PLAN_BATCHED) {
+        rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, pl->n_tasks, pl->d_items,
+                           pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, s);
+    } else {
+        for (const LaunchGroup &g : pl->groups) {
+            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
+                               g.task_end - g.task_begin, pl->d_items, pl->d_best, pl->d_counters, g.lds, 0, false, s);
+            if (rc)
+                return rc;
+        }
+    }
+    if (rc)
+        return rc;
+    return xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
+}
+
+extern "C" int xme_me_run(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_ref_desc *refs,
+                          xme_result *res, int32_t *cost_map, void *stream)
+{
+    if (!pl)
+        return XME_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = fill_desc(pl, cur, i_cur, refs, res, cost_map, s);
+    if (rc)
+        return rc;
+    return launch_all(pl, s);
+}
+
+extern "C" int xme_me_run_graph(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_ref_desc *refs,
+                                xme_result *res, void *stream)
+{
+    if (!pl)
+        return XME_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (s == nullptr) {
+        xme_set_error("xme_me_run_graph: needs a created stream (the null stream cannot be captured)");
+        return XME_ERR_ARG;
+    }
+    int rc = fill_desc(pl, cur, i_cur, refs, res, nullptr, s);
+    if (rc)
+        return rc;
+    if (!pl->graph_exec) {
+        XME_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = launch_all(pl, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g)
+                hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return xme_hip_fail(e, "hipStreamEndCapture");
+        pl->graph = g;
+        XME_HIP(hipGraphInstantiate(&pl->graph_exec, g, nullptr, nullptr, 0));
+    }
+    XME_HIP(hipGraphLaunch(pl->graph_exec, s));
+    return XME_OK;
+}

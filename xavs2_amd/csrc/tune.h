@@ -1,0 +1,20 @@
+// Tile-shape autotuner of the integer search (rows per lane K, lanes per
+// candidate row LC), keyed by (PU shape, search range).
+#pragma once
+
+struct TuneChoice {
+    int K;    // candidate rows per lane (4 or 8)
+    int LC;   // lanes sharing one candidate row, 4 columns each
+};
+
+// Tuned choice when one exists, else the default for the shape.
+TuneChoice xme_tune_lookup(int part, int search_range);
+TuneChoice xme_tune_default(int part, int search_range);
+void xme_tune_store(int part, int search_range, TuneChoice c);
+// Forces a choice for every shape (used by the tuner while it measures).
+void xme_tune_force(const TuneChoice *c);
+
+int xme_part_index(int w, int h);
+int xme_pu_w(int part);
+int xme_pu_h(int part);
+const void *xme_fs_kernel_ptr(bool persistent);

@@ -1,0 +1,304 @@
+"""ctypes mirror of include/xavs2_amd.h (libxavs2_amd.so).
+
+The library is the product; this module only loads it and converts numpy
+arrays.  Loading fails loudly when the shared object has not been built: there
+is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxavs2_amd.so")
+
+XME_OK = 0
+XME_PAD = 80
+XME_MAX_REFS = 8
+MAX_DISTORTION = 1 << 30
+PLAN_BATCHED, PLAN_PLAIN = 0, 1
+
+PU_W = [4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 12, 16, 4, 32, 24, 32, 8, 64, 48, 64, 16]
+PU_H = [4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 12, 16, 4, 16, 24, 32, 8, 32, 48, 64, 16, 64]
+
+
+class XmeError(RuntimeError):
+    pass
+
+
+class MeParams(C.Structure):
+    _fields_ = [("frame_w", C.c_int32), ("frame_h", C.c_int32), ("search_range", C.c_int32),
+                ("lambda_factor", C.c_int32), ("min_mv_range", C.c_int32 * 2), ("max_mv_range", C.c_int32 * 2),
+                ("use_hadamard", C.c_int32), ("use_fractional_me", C.c_int32), ("use_fast_sub_me", C.c_int32),
+                ("enable_pmvr", C.c_int32)]
+
+
+PU_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
+                     ("mvp_x", "<i2"), ("mvp_y", "<i2")])
+RESULT_DTYPE = np.dtype([("cost", "<i4"), ("bcost2", "<i4"), ("bmv_x", "<i2"), ("bmv_y", "<i2"),
+                         ("bmv2_x", "<i2"), ("bmv2_y", "<i2"), ("mvcost", "<i4"), ("status", "<i4")])
+
+
+class RefDesc(C.Structure):
+    _fields_ = [("planes", C.c_void_p * 16), ("stride", C.c_ssize_t)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_int64), ("max_inflight", C.c_int32), ("search_range", C.c_int32),
+                ("lambda_factor", C.c_int32), ("n_ref", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("throughput_fps", C.c_double), ("resident_bytes", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise XmeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P, I, Z = C.c_void_p, C.c_int, C.c_ssize_t
+        L.xme_last_error.restype = C.c_char_p
+        L.xme_version.restype = C.c_char_p
+        L.xme_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.xme_free.argtypes = [P]
+        L.xme_memcpy_h2d.argtypes = [P, P, C.c_size_t, P]
+        L.xme_memcpy_d2h.argtypes = [P, P, C.c_size_t, P]
+        L.xme_memcpy_d2d.argtypes = [P, P, C.c_size_t, P]
+        L.xme_memset.argtypes = [P, I, C.c_size_t, P]
+        L.xme_stream_create.argtypes = [C.POINTER(C.c_void_p)]
+        L.xme_stream_destroy.argtypes = [P]
+        L.xme_stream_sync.argtypes = [P]
+        L.xme_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.xme_plan_create.argtypes = [C.POINTER(MeParams), P, I, I, I, C.POINTER(C.c_void_p)]
+        L.xme_plan_destroy.argtypes = [P]
+        L.xme_plan_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.xme_me_run.argtypes = [P, P, Z, P, P, P, P]
+        L.xme_me_run_graph.argtypes = [P, P, Z, P, P, P]
+        L.xme_tune.argtypes = [C.POINTER(MeParams), P, I, P]
+        L.xme_tune_get.argtypes = [I, I, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.xme_expand_border.argtypes = [P, Z, I, I, I, P]
+        L.xme_interpolate_frame.argtypes = [P, Z, I, I, I, P]
+        L.xme_sad_batch.argtypes = [I, P, Z, P, Z, P, P, I, P, P]
+        L.xme_satd_batch.argtypes = [I, P, Z, P, Z, P, P, I, P, P]
+        L.xme_sad_x3_batch.argtypes = [I, P, P, Z, P, P, I, P, P]
+        L.xme_sad_x4_batch.argtypes = [I, P, P, Z, P, P, I, P, P]
+        L.xme_mc_chroma_batch.argtypes = [P, P, Z, P, I, I, I, P, P, Z, P, P]
+        L.xme_mc_luma_batch.argtypes = [P, P, I, I, I, P, Z, P, P]
+        L.xme_dct_batch.argtypes = [I, P, P, I, P]
+        L.xme_idct_batch.argtypes = [I, P, P, I, P]
+        L.xme_quant_batch.argtypes = [I, P, I, I, I, P, P]
+        L.xme_dequant_batch.argtypes = [I, P, I, I, P]
+        L.xme_residual_batch.argtypes = [I, I, I, P, P, P, P, I, P]
+        L.xme_get_sad_table.argtypes = [P]
+        L.xme_get_satd_table.argtypes = [P]
+        L.xme_get_sad_x3_table.argtypes = [P]
+        L.xme_get_sad_x4_table.argtypes = [P]
+        L.xme_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+        L.xme_server_destroy.argtypes = [P]
+        L.xme_server_open_stream.argtypes = [P, I, I, C.POINTER(C.c_int)]
+        L.xme_server_close_stream.argtypes = [P, I]
+        L.xme_server_submit.argtypes = [P, I, P, Z, C.POINTER(C.c_int64)]
+        L.xme_server_job_size.argtypes = [P, C.c_int64, C.POINTER(C.c_int)]
+        L.xme_server_wait.argtypes = [P, C.c_int64, P, C.POINTER(C.c_double)]
+        L.xme_server_stats_get.argtypes = [P, C.POINTER(ServerStats)]
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != XME_OK:
+        raise XmeError(f"{what} failed ({rc}): {lib().xme_last_error().decode()}")
+    return rc
+
+
+def ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().xme_malloc(C.byref(p), max(self.nbytes, 1)), "xme_malloc")
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.xme_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def at(self, offset=0):
+        return C.c_void_p(self.ptr + int(offset))
+
+    def upload(self, arr, offset=0, stream=None):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes + offset <= self.nbytes
+        check(lib().xme_memcpy_h2d(self.at(offset), ptr(arr), arr.nbytes, stream), "h2d")
+
+    def download(self, shape, dtype, offset=0, stream=None):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes + offset <= self.nbytes
+        check(lib().xme_memcpy_d2h(ptr(out), self.at(offset), out.nbytes, stream), "d2h")
+        return out
+
+    def zero(self, stream=None):
+        check(lib().xme_memset(self.at(0), 0, self.nbytes, stream), "memset")
+
+
+def upload(arr):
+    arr = np.ascontiguousarray(arr)
+    b = DeviceBuffer(arr.nbytes)
+    b.upload(arr)
+    return b
+
+
+class DevicePlane:
+    """A padded 8-bit plane in device memory, laid out like xavs2's frame planes:
+    the picture origin sits `pad` rows and columns into the allocation."""
+
+    def __init__(self, w, h, pad=XME_PAD, stride=None):
+        self.w, self.h, self.pad = w, h, pad
+        self.stride = stride if stride is not None else ((w + 2 * pad + 255) // 256) * 256
+        self.rows = h + 2 * pad + 1
+        self.buf = DeviceBuffer(self.rows * self.stride + 256)
+
+    @property
+    def origin(self):
+        return self.buf.ptr + self.pad * self.stride + self.pad
+
+    def upload_padded(self, padded):
+        """padded: (h + 2 pad) x (w + 2 pad) uint8 array (full padded area)."""
+        full = np.zeros((self.rows, self.stride), np.uint8)
+        full[:padded.shape[0], :padded.shape[1]] = padded
+        self.buf.upload(full.ravel())
+
+    def upload_picture(self, img):
+        full = np.zeros((self.rows, self.stride), np.uint8)
+        full[self.pad:self.pad + self.h, self.pad:self.pad + self.w] = img
+        self.buf.upload(full.ravel())
+
+    def download_padded(self):
+        full = self.buf.download(self.rows * self.stride, np.uint8)[: self.rows * self.stride]
+        return full.reshape(self.rows, self.stride)[: self.h + 2 * self.pad, : self.w + 2 * self.pad]
+
+    def expand_border(self, stream=None):
+        check(lib().xme_expand_border(C.c_void_p(self.origin), self.stride, self.w, self.h, self.pad, stream),
+              "xme_expand_border")
+
+
+class DeviceRef:
+    """A reference picture: full-pel plane plus its 15 interpolated planes."""
+
+    def __init__(self, w, h, frac=2):
+        self.w, self.h, self.frac = w, h, frac
+        self.planes = [DevicePlane(w, h)]
+        stride = self.planes[0].stride
+        for k in range(1, 16):
+            need = frac == 2 or (frac == 1 and k in (2, 8, 10))
+            self.planes.append(DevicePlane(w, h, stride=stride) if need else None)
+
+    @property
+    def stride(self):
+        return self.planes[0].stride
+
+    def load(self, img, stream=None):
+        """Upload a picture, expand its border and build the interpolated planes on the GPU."""
+        self.planes[0].upload_picture(img)
+        self.planes[0].expand_border(stream)
+        if self.frac:
+            arr = (C.c_void_p * 16)(*[(p.origin if p is not None else None) for p in self.planes])
+            check(lib().xme_interpolate_frame(arr, self.stride, self.w, self.h, self.frac, stream),
+                  "xme_interpolate_frame")
+
+    def desc(self):
+        d = RefDesc()
+        for k in range(16):
+            d.planes[k] = self.planes[k].origin if self.planes[k] is not None else None
+        d.stride = self.stride
+        return d
+
+
+def me_params(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use_fractional_me=2,
+              use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047)):
+    p = MeParams()
+    p.frame_w, p.frame_h, p.search_range, p.lambda_factor = frame_w, frame_h, search_range, lambda_factor
+    p.min_mv_range[0], p.max_mv_range[0] = mv_range_x
+    p.min_mv_range[1], p.max_mv_range[1] = mv_range_y
+    p.use_hadamard, p.use_fractional_me = use_hadamard, use_fractional_me
+    p.use_fast_sub_me, p.enable_pmvr = use_fast_sub_me, enable_pmvr
+    return p
+
+
+def make_pus(tuples):
+    """tuples: iterable of (x, y, w, h, mvp_x, mvp_y)."""
+    a = np.zeros(len(tuples), dtype=PU_DTYPE)
+    for i, t in enumerate(tuples):
+        a[i] = (t[0], t[1], t[2], t[3], 0, 0, t[4], t[5])
+    return a
+
+
+class Plan:
+    def __init__(self, params, pus, n_ref, mode=PLAN_BATCHED):
+        self.params = params
+        self.pus = np.ascontiguousarray(pus, dtype=PU_DTYPE)
+        self.n_ref = n_ref
+        h = C.c_void_p()
+        check(lib().xme_plan_create(C.byref(params), ptr(self.pus), len(self.pus), n_ref, mode, C.byref(h)),
+              "xme_plan_create")
+        self.h = h
+        self.n = len(self.pus) * n_ref
+        self.results = DeviceBuffer(self.n * RESULT_DTYPE.itemsize)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.xme_plan_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().xme_plan_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "xme_plan_info")
+        return a.value, b.value, c.value
+
+    def _refs(self, refs):
+        arr = (RefDesc * len(refs))()
+        for i, r in enumerate(refs):
+            arr[i] = r.desc()
+        return arr
+
+    def run(self, cur_plane, refs, cost_map=None, stream=None):
+        self._ref_arr = self._refs(refs)
+        check(lib().xme_me_run(self.h, C.c_void_p(cur_plane.origin), cur_plane.stride, self._ref_arr,
+                               C.c_void_p(self.results.ptr), C.c_void_p(cost_map.ptr) if cost_map else None,
+                               stream), "xme_me_run")
+
+    def run_graph(self, cur_plane, refs, stream):
+        self._ref_arr = self._refs(refs)
+        check(lib().xme_me_run_graph(self.h, C.c_void_p(cur_plane.origin), cur_plane.stride, self._ref_arr,
+                                     C.c_void_p(self.results.ptr), stream), "xme_me_run_graph")
+
+    def fetch(self, stream=None):
+        return self.results.download(self.n, RESULT_DTYPE, stream=stream)
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib().xme_stream_create(C.byref(s)), "xme_stream_create")
+        self.h = s
+
+    def sync(self):
+        check(lib().xme_stream_sync(self.h), "xme_stream_sync")
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.xme_stream_destroy(self.h)
+            self.h = None
