@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X motion-estimation cost path.
+
+Workload (one step = one frame-level ME job): batched multi-reference ME at
+3840x2160 8-bit -- 4 reference pictures (seeded smooth + noise fields), a
+current picture built from per-region mixtures of the references displaced by
+MVs in [-48, 48]^2, a seeded mixed PU tiling (8x8 ... 64x64 incl. 64x16,
+16x64, 32x8, 8x32), full search over [-64, 64)^2 with cost SAD + lambda * MV
+bits (lambda factor 4 * 65536, MVP (0,0)), then half- and quarter-pel SATD
+refinement.  The whole frame is one persistent launch replayed from a HIP
+graph.
+
+Printed: one JSON line with candidate evaluations per second (batched path),
+the roofline of the integer-search kernel, the plain one-launch-per-PU-shape
+GPU path on the same job, per-frame latency p50/p99, and the CPU oracle timed
+on a bounded sample of the same job.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ME cand-evals/s + roofline frac, batched vs plain; p50/p99 frame-ME latency"
+VALU_PEAK_TABSDIFF = 256 * 4 * 16 * 2.4e9 * 4 / 1e12     # v_sad_u8 / v_qsad: 157.3 T absdiff/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def build_inputs(w, h, n_ref, seed):
+    from xavs2_amd import synth
+    refs = [synth.smooth_field(w, h, seed=seed + 101 * r) for r in range(n_ref)]
+    cur = synth.multi_ref_current(refs, 32, 48, seed=seed + 7)
+    pus = synth.mixed_pus(w, h, seed=seed + 3)
+    return refs, cur, pus
+
+
+def legal_counts(w, h, x, y, pw, ph, R, mvr_x=(-8192, 8191), mvr_y=(-2048, 2047)):
+    """Legal full-pel displacements per axis of a PU searched around (0,0)
+    (MV limits of md_inter.c:910-926, raster [-R, R))."""
+    def axis(pos, size, frame, lim):
+        lo = min(max((-64 - pos) * 4, lim[0]), lim[1])
+        hi = min(max((frame + 64 - pos - size) * 4, lim[0]), lim[1])
+        lo, hi = (lo >> 2) + 6, (hi >> 2) - 6
+        return max(0, min(R - 1, hi) - max(-R, lo) + 1)
+    return axis(x, pw, w, mvr_x), axis(y, ph, h, mvr_y)
+
+
+def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
+    """The oracle (CPU restatement) timed on a bounded sample of the same job:
+    all PUs of the first CTU rows against reference 0, grown until the sample
+    takes about budget_s seconds.  Also checks the GPU results of the sample."""
+    from oracle import oracle as O
+    w, h = refs[0].shape[1], refs[0].shape[0]
+    p = O.Plane(w, h)
+    p.set_picture(refs[0])
+    p.expand_border()
+    planes = O.interp_planes(p, 2)
+    cp = O.Plane(w, h)
+    cp.set_picture(cur)
+    prm = O.me_param(w, h, *prm_args)
+    rows = sorted({pu[1] // 64 for pu in pus})
+    sel_rows = 1
+    elapsed = 0.0
+    while True:
+        idx = [i for i, pu in enumerate(pus) if pu[1] // 64 < sel_rows]
+        arr = np.array([(pus[i][0], pus[i][1], pus[i][2], pus[i][3], 0, 0) for i in idx], dtype=O.PU_DTYPE)
+        t0 = time.perf_counter()
+        res = O.me_search_batch(prm, arr, cp, planes, nthreads=threads)
+        elapsed = time.perf_counter() - t0
+        if elapsed * 2 > budget_s or sel_rows >= len(rows):
+            break
+        sel_rows = min(len(rows), max(sel_rows + 1, int(sel_rows * budget_s / max(elapsed, 1e-3) * 0.8)))
+    # legal integer candidates of the sample (MVP (0,0): centre (0,0), window clipped by the MV range)
+    R = prm.search_range
+    n_cand = 0
+    for i in idx:
+        x, y, pw, ph = pus[i]
+        nx, ny = legal_counts(w, h, x, y, pw, ph, R)
+        n_cand += nx * ny
+    ok = True
+    for k, i in enumerate(idx):
+        g = gpu_res[i * n_ref + 0]
+        for f in ("cost", "bcost2", "bmv_x", "bmv_y", "bmv2_x", "bmv2_y", "mvcost", "status"):
+            if int(g[f]) != int(res[k][f]):
+                ok = False
+    return {
+        "value": n_cand / elapsed,
+        "unit": "cand-evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(idx)} PUs ({sel_rows} CTU rows of 3840x2160) x 1 ref, full search [-64,64)^2 + "
+                  f"half/quarter-pel SATD, {elapsed:.2f} s on {threads} threads",
+        "parity_with_gpu_on_sample": ok,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-reps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--width", type=int, default=3840)
+    ap.add_argument("--height", type=int, default=2160)
+    ap.add_argument("--refs", type=int, default=4)
+    ap.add_argument("--range", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the ME path runs on a single MI355X; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from xavs2_amd import lib as X
+
+    w, h, n_ref, R = args.width, args.height, args.refs, args.range
+    lam = 4 * 65536
+    log(f"generating {w}x{h} inputs, {n_ref} refs")
+    refs, cur, pus = build_inputs(w, h, n_ref, seed=2026)
+    stream = X.Stream()
+    d_refs = []
+    for r in refs:
+        d = X.DeviceRef(w, h, frac=2)
+        d.load(r, stream.h)
+        d_refs.append(d)
+    dcur = X.DevicePlane(w, h, pad=0, stride=((w + 255) // 256) * 256)
+    dcur.upload_picture(cur)
+    stream.sync()
+    prm = X.me_params(w, h, R, lam, use_hadamard=1, use_fractional_me=2, use_fast_sub_me=0, enable_pmvr=0)
+
+    part_of = {(X.PU_W[i], X.PU_H[i]): i for i in range(25)}
+    parts = sorted({part_of[(pw, ph)] for (_, _, pw, ph) in pus})
+    tiles = {}
+    if not args.no_tune:
+        log(f"autotuning tile shapes for {len(parts)} PU shapes at R={R}")
+        tiles = X.tune(prm, parts, stream.h)
+    pu_arr = X.make_pus([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus])
+    log(f"planning {len(pus)} PUs x {n_ref} refs")
+    plan = X.Plan(prm, pu_arr, n_ref, X.PLAN_BATCHED)
+    n_inst, n_cand, n_absdiff = plan.info()
+
+    # warmup (also captures the graph)
+    for _ in range(max(1, args.warmup)):
+        plan.run_graph(dcur, d_refs, stream.h)
+    stream.sync()
+    gpu_res = plan.fetch()
+
+    # timed region: K graph replays bracketed by synchronizations
+    log(f"timing {args.steps} steps")
+    X.lib().xme_device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run_graph(dcur, d_refs, stream.h)
+    stream.sync()
+    X.lib().xme_device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n_cand * args.steps / elapsed
+    again = plan.fetch()
+    stable = bool(np.array_equal(again, gpu_res))
+
+    # per-frame latency: each replay synchronized on its own
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        plan.run_graph(dcur, d_refs, stream.h)
+        stream.sync()
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+
+    # roofline of the integer-search kernel: HIP events on its stream
+    search_ms, job_ms, _ = plan.time(dcur, d_refs, stream.h, 5)
+    achieved = n_absdiff / (search_ms * 1e-3) / 1e12
+    log(f"search kernel {search_ms:.3f} ms, {achieved:.1f} T absdiff/s")
+
+    # plain path: one launch per PU shape, no persistent queue, no graph
+    log("timing the plain path")
+    plain = X.Plan(prm, pu_arr, n_ref, X.PLAN_PLAIN)
+    _, plain_ms, _ = plain.time(dcur, d_refs, stream.h, max(1, args.plain_reps))
+    plain_res = plain.fetch(stream.h)
+    plain_equal = bool(np.array_equal(plain_res, gpu_res))
+    plain_value = n_cand / (plain_ms * 1e-3)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"timing the CPU oracle on a sample ({threads} threads)")
+        cpu = cpu_baseline((R, lam, 1, 2, 0, 0), refs, cur, pus, gpu_res, n_ref, args.cpu_seconds, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cand-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched multi-ref ME {w}x{h} 8-bit, {n_ref} refs, {len(pus)} mixed PUs "
+                        f"(8x8..64x64 incl. 64x16/16x64/32x8/8x32), full search [-{R},{R})^2 + half/quarter-pel "
+                        "SATD refine, one persistent launch replayed from a HIP graph",
+            "pu_refs": int(n_inst),
+            "integer_candidates_per_step": int(n_cand),
+            "absdiffs_per_step": int(n_absdiff),
+            "lambda_factor": lam,
+            "tiles_K_LC": {str(k): v for k, v in tiles.items()},
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": VALU_PEAK_TABSDIFF,
+            "unit": "Tabsdiff/s",
+            "frac": achieved / VALU_PEAK_TABSDIFF,
+            "traffic": None,
+            "kernel": "fs_kernel<persistent> (integer full search)",
+            "kernel_ms": search_ms,
+        },
+        "plain": {"value": plain_value, "ms_per_step": plain_ms, "results_equal": plain_equal},
+        "speedup_vs_plain": value / plain_value,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean())},
+        "results_stable_across_replays": stable,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -169,6 +169,17 @@ int xme_me_run(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref
 int xme_me_run_graph(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
                      xme_result *d_results, void *stream);
 
+/* Measurement hook: runs the job `reps` times on `stream` with HIP events
+ * around the integer-search kernel and around the whole job; returns the mean
+ * durations in milliseconds.  Per-run job durations go to job_ms[reps] when
+ * it is not NULL. */
+int xme_me_time(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
+                xme_result *d_results, void *stream, int reps, double *search_ms, double *job_ms_mean,
+                double *job_ms);
+/* A plain streaming device copy (one read and one write per byte), used to
+ * calibrate memory-traffic counters. */
+int xme_copy_kernel(void *d_dst, const void *d_src, size_t bytes, void *stream);
+
 /* Autotuner: time the integer-search tile shapes for (PU shape, search range)
  * on this device and store the winners for later plans.  Returns the number
  * of shapes tuned.  xme_tune_get reports the chosen rows-per-lane (K) and

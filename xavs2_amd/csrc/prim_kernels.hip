@@ -4,6 +4,8 @@
 #include "xme_internal.h"
 #include "tune.h"
 
+#include <algorithm>
+
 namespace {
 
 __device__ __forceinline__ uint32_t load_dw_any(const uint8_t *p)
@@ -172,4 +174,27 @@ extern "C" int xme_sad_x4_batch(int part, const pel_t *fenc, const pel_t *fref, 
                                 const int64_t *o2, int n, int32_t *scores, void *stream)
 {
     return launch_xn(4, part, fenc, fref, i_fref, o1, o2, n, scores, stream);
+}
+
+namespace {
+__global__ void __launch_bounds__(256) copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+} // namespace
+
+extern "C" int xme_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!dst || !src || (bytes & 15) || (reinterpret_cast<uintptr_t>(dst) & 15) ||
+        (reinterpret_cast<uintptr_t>(src) & 15)) {
+        xme_set_error("xme_copy_kernel: 16-byte aligned buffers and size required");
+        return XME_ERR_ARG;
+    }
+    const size_t n = bytes / 16;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(copy_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<uint4 *>(dst), reinterpret_cast<const uint4 *>(src), n);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
 }

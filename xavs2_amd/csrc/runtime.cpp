@@ -722,3 +722,64 @@ extern "C" int xme_me_run_graph(xme_plan *pl, const pel_t *cur, intptr_t i_cur, 
     XME_HIP(hipGraphLaunch(pl->graph_exec, s));
     return XME_OK;
 }
+
+extern "C" int xme_me_time(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_ref_desc *refs,
+                           xme_result *res, void *stream, int reps, double *search_ms, double *job_ms_mean,
+                           double *job_ms)
+{
+    if (!pl || reps < 1)
+        return XME_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<hipEvent_t> ev(4 * reps, nullptr);
+    for (auto &e : ev)
+        XME_HIP(hipEventCreate(&e));
+    int rc = XME_OK;
+    for (int r = 0; r < reps && rc == XME_OK; r++) {
+        rc = fill_desc(pl, cur, i_cur, refs, res, nullptr, s);
+        if (rc)
+            break;
+        XME_HIP(hipEventRecord(ev[4 * r + 0], s));
+        rc = xme_launch_mvc(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
+        if (rc)
+            break;
+        XME_HIP(hipEventRecord(ev[4 * r + 1], s));
+        if (pl->mode == XME_PLAN_BATCHED) {
+            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, pl->n_tasks, pl->d_items,
+                               pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, s);
+        } else {
+            for (const LaunchGroup &g : pl->groups) {
+                rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
+                                   g.task_end - g.task_begin, pl->d_items, pl->d_best, pl->d_counters, g.lds, 0,
+                                   false, s);
+                if (rc)
+                    break;
+            }
+        }
+        if (rc)
+            break;
+        XME_HIP(hipEventRecord(ev[4 * r + 2], s));
+        rc = xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
+        XME_HIP(hipEventRecord(ev[4 * r + 3], s));
+    }
+    if (rc == XME_OK)
+        XME_HIP(hipStreamSynchronize(s));
+    double sum_s = 0, sum_j = 0;
+    for (int r = 0; r < reps && rc == XME_OK; r++) {
+        float a = 0, b = 0;
+        XME_HIP(hipEventElapsedTime(&a, ev[4 * r + 1], ev[4 * r + 2]));
+        XME_HIP(hipEventElapsedTime(&b, ev[4 * r + 0], ev[4 * r + 3]));
+        sum_s += a;
+        sum_j += b;
+        if (job_ms)
+            job_ms[r] = b;
+    }
+    for (auto &e : ev)
+        hipEventDestroy(e);
+    if (rc)
+        return rc;
+    if (search_ms)
+        *search_ms = sum_s / reps;
+    if (job_ms_mean)
+        *job_ms_mean = sum_j / reps;
+    return XME_OK;
+}

@@ -81,6 +81,8 @@ def lib():
         L.xme_plan_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xme_me_run.argtypes = [P, P, Z, P, P, P, P]
         L.xme_me_run_graph.argtypes = [P, P, Z, P, P, P]
+        L.xme_me_time.argtypes = [P, P, Z, P, P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), P]
+        L.xme_copy_kernel.argtypes = [P, P, C.c_size_t, P]
         L.xme_tune.argtypes = [C.POINTER(MeParams), P, I, P]
         L.xme_tune_get.argtypes = [I, I, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.xme_expand_border.argtypes = [P, Z, I, I, I, P]
@@ -287,6 +289,29 @@ class Plan:
 
     def fetch(self, stream=None):
         return self.results.download(self.n, RESULT_DTYPE, stream=stream)
+
+    def time(self, cur_plane, refs, stream, reps):
+        """Runs the job reps times; returns (mean search-kernel ms, mean job ms, per-run job ms)."""
+        self._ref_arr = self._refs(refs)
+        a, b = C.c_double(), C.c_double()
+        per = np.zeros(reps, dtype=np.float64)
+        check(lib().xme_me_time(self.h, C.c_void_p(cur_plane.origin), cur_plane.stride, self._ref_arr,
+                                C.c_void_p(self.results.ptr), stream, reps, C.byref(a), C.byref(b), ptr(per)),
+              "xme_me_time")
+        return a.value, b.value, per
+
+
+def tune(params, parts, stream=None):
+    arr = np.ascontiguousarray(parts, dtype=np.int32)
+    n = lib().xme_tune(C.byref(params), ptr(arr), len(arr), stream)
+    if n < 0:
+        check(n, "xme_tune")
+    out = {}
+    for p in parts:
+        k, lc = C.c_int(), C.c_int()
+        lib().xme_tune_get(int(p), params.search_range, C.byref(k), C.byref(lc))
+        out[int(p)] = (k.value, lc.value)
+    return out
 
 
 class Stream:

@@ -64,20 +64,22 @@ def multi_ref_current(refs, region, mv_range, seed, noise=3.0):
 # shapes of the sub-pel configuration: 8x8, 16x16, 32x32, 64x64, 64x16, 16x64,
 # 32x8, 8x32.
 def _split(x, y, s, rng, out, small_bias):
+    # split probabilities chosen so that the expected area is ~10 % 64x64,
+    # ~30 % 32x32 and asymmetric (64x16, 16x64, 32x8, 8x32), ~60 % 8x8/16x16
     r = rng.random()
     if s == 64:
         if r < 0.10:
             out.append((x, y, 64, 64)); return
-        if r < 0.15:
+        if r < 0.125:
             out += [(x, y, 64, 16), (x, y + 16, 64, 16), (x, y + 32, 64, 16), (x, y + 48, 64, 16)]; return
-        if r < 0.20:
+        if r < 0.15:
             out += [(x, y, 16, 64), (x + 16, y, 16, 64), (x + 32, y, 16, 64), (x + 48, y, 16, 64)]; return
     if s == 32:
-        if r < 0.30:
+        if r < 0.15:
             out.append((x, y, 32, 32)); return
-        if r < 0.45:
+        if r < 0.222:
             out += [(x, y + k, 32, 8) for k in range(0, 32, 8)]; return
-        if r < 0.60:
+        if r < 0.294:
             out += [(x + k, y, 8, 32) for k in range(0, 32, 8)]; return
     if s == 16:
         if r < 0.5:
