@@ -889,3 +889,11 @@ void xo_residual_batch(int n, int qp, int b_intra, const int16_t *in, int16_t *l
         xo_idct(n, c, recon + (size_t)b * nn, n);
     }
 }
+
+/* CHECK_MV_RANGE of a PU's MV box, exported for tests (me.h:46-48, me.c:896-897). */
+int xo_check_mv_range(int mx, int my, int min_x, int min_y, int max_x, int max_y)
+{
+    uint32_t mv_min = pack_mask2(-min_x, -min_y);
+    uint32_t mv_max = pack_mask2(max_x, max_y) | 0x8000u;
+    return check_mv_range(mx, my, mv_min, mv_max);
+}

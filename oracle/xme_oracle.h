@@ -56,6 +56,8 @@ int  xo_mvbits_max_mvd(int search_range);
 int  xo_mvbits(int search_range, int v);
 /* LAMBDA_FACTOR(sqrt(lambda)), common/common.h:85, encoder/encoder.c:329 */
 int  xo_lambda_factor(double lambda_mode);
+/* CHECK_MV_RANGE (me.h:46-48) for the full-pel box [min_x, max_x] x [min_y, max_y]. */
+int  xo_check_mv_range(int mx, int my, int min_x, int min_y, int max_x, int max_y);
 
 /* ---- ME search (encoder/me.c, encoder/md_inter.c) ---------------------- */
 typedef struct {

@@ -184,3 +184,23 @@ def test_batch_equals_single_calls():
     for k, p in enumerate(pus):
         single, _ = O.me_search(prm, tuple(int(v) for v in p), cur_plane(cur), planes)
         assert single["cost"] == batch[k]["cost"] and single["bmv_x"] == batch[k]["bmv_x"]
+
+
+def test_packed_mv_range_check_equals_box_test():
+    """CHECK_MV_RANGE packs (mx, my) into one word; for the MV boxes an encoder
+    forms (|MV| below 8192 full-pel) it must agree with the plain box test, which
+    the GPU search uses for bands that lie wholly inside the box."""
+    L = O.lib()
+    L.xo_check_mv_range.argtypes = [O.C.c_int] * 6
+    rng = np.random.default_rng(0)
+    boxes = [(-58, -58, 1913, 1073), (-122, -70, 58, 30), (-8192 // 4 + 6, -2048 // 4 + 6, 2041, 505),
+             (-200, -300, -100, -250), (100, 40, 300, 41)]
+    for (x0, y0, x1, y1) in boxes:
+        xs = np.unique(np.concatenate([np.arange(x0 - 3, x0 + 4), np.arange(x1 - 3, x1 + 4),
+                                       rng.integers(x0 - 500, x1 + 500, 60)]))
+        ys = np.unique(np.concatenate([np.arange(y0 - 3, y0 + 4), np.arange(y1 - 3, y1 + 4),
+                                       rng.integers(y0 - 500, y1 + 500, 60)]))
+        for mx in xs:
+            for my in ys:
+                box = x0 <= mx <= x1 and y0 <= my <= y1
+                assert bool(L.xo_check_mv_range(int(mx), int(my), x0, y0, x1, y1)) == box, (mx, my, x0, y0, x1, y1)

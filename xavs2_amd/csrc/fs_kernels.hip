@@ -152,12 +152,80 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
 #pragma unroll
             for (int t = 0; t < ND; t++)
                 cw[t] = cp[t];
+            // k innermost: consecutive qsads update different accumulators; the
+            // scheduling barriers keep the compiler from pulling a dependent
+            // qsad next to its producer
+#pragma unroll
+            for (int t = 0; t < ND; t++) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const uint32_t(&rr)[ND + 1] = ring[(cc + k) % K];
+                    a16[k] = qsad(rr[t], rr[t + 1], cw[t], a16[k]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (((c0 + K) % FLUSH) == 0 || c0 + K >= H) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const uint32_t(&rr)[ND + 1] = ring[(cc + k) % K];
+                acc[k][0] += (uint32_t)a16[k] & 0xFFFFu;
+                acc[k][1] += ((uint32_t)a16[k]) >> 16;
+                acc[k][2] += (uint32_t)(a16[k] >> 32) & 0xFFFFu;
+                acc[k][3] += (uint32_t)(a16[k] >> 48);
+                a16[k] = 0;
+            }
+        }
+    }
+}
+
+// The same for a dword-aligned lane base: the 8 reference bytes of every qsad
+// are loaded straight into a register pair (two dwords of LDS per pair, no
+// realignment and no register copies); LDS bandwidth is not the bottleneck.
+template <int K, int SW>
+__device__ __forceinline__ void fs_strip_aligned(const uint8_t *__restrict__ win, int pitch, int base,
+                                                 const uint8_t *__restrict__ cur, int cpitch, int H,
+                                                 uint32_t (&acc)[K][4])
+{
+    constexpr int ND = SW / 4;
+    constexpr int FLUSH = 256 / SW;
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + base);
+    const int pd = pitch >> 2;
+    uint64_t ring[K][ND];
+    uint64_t a16[K];
 #pragma unroll
-                for (int t = 0; t < ND; t++)
-                    a16[k] = qsad(rr[t], rr[t + 1], cw[t], a16[k]);
+    for (int k = 0; k < K; k++)
+        a16[k] = 0;
+
+    auto load_row = [&](int r, uint64_t (&dst)[ND]) {
+        const uint32_t *src = w32 + r * pd;
+#pragma unroll
+        for (int t = 0; t < ND; t++)
+            dst[t] = ((uint64_t)src[t + 1] << 32) | src[t];
+    };
+
+#pragma unroll
+    for (int k = 0; k < K - 1; k++)
+        load_row(k, ring[k]);
+
+    for (int c0 = 0; c0 < H; c0 += K) {
+#pragma unroll
+        for (int cc = 0; cc < K; cc++) {
+            const int c = c0 + cc;
+            load_row(c + K - 1, ring[(cc + K - 1) % K]);
+            uint32_t cw[ND];
+            const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
+#pragma unroll
+            for (int t = 0; t < ND; t++)
+                cw[t] = cp[t];
+            // k innermost: consecutive qsads update different accumulators; the
+            // scheduling barriers keep the compiler from pulling a dependent
+            // qsad next to its producer
+#pragma unroll
+            for (int t = 0; t < ND; t++) {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(cc + k) % K][t], cw[t], a16[k]);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (((c0 + K) % FLUSH) == 0 || c0 + K >= H) {
@@ -184,10 +252,18 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 }
 
 // Process one item with K rows per lane.
+// Per-wave cache of the MV-cost terms of the lane's candidates, valid for one
+// item (band, MVP) and one search centre.
+struct TermState {
+    bool ok;
+    int cx, cy;
+};
+
 template <int K>
 __device__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, const FsItem &it, int inst_id,
                         const uint8_t *__restrict__ win, int pitch, int wx0, int wy0, const uint8_t *__restrict__ cur,
-                        int cpitch, int cx0, int cy0, uint64_t *__restrict__ best, int32_t *__restrict__ cost_map)
+                        int cpitch, int cx0, int cy0, uint64_t *__restrict__ best, int32_t *__restrict__ cost_map,
+                        uint32_t *__restrict__ term, TermState &ts)
 {
     const int lane = threadIdx.x & 63;
     const int LC = it.LC;
@@ -210,45 +286,112 @@ __device__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, 
 
     const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
     int x = 0;
-    for (; x + 16 <= pu.w; x += 16)
-        fs_strip<K, 16>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
-    if (x + 8 <= pu.w) {
-        fs_strip<K, 8>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
-        x += 8;
+    if ((lane_base & 3) == 0) {      // uniform: col and the pitch are multiples of 4
+        for (; x + 16 <= pu.w; x += 16)
+            fs_strip_aligned<K, 16>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+        if (x + 8 <= pu.w) {
+            fs_strip_aligned<K, 8>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+            x += 8;
+        }
+        if (x + 4 <= pu.w)
+            fs_strip_aligned<K, 4>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+    } else {
+        for (; x + 16 <= pu.w; x += 16)
+            fs_strip<K, 16>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+        if (x + 8 <= pu.w) {
+            fs_strip<K, 8>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
+            x += 8;
+        }
+        if (x + 4 <= pu.w)
+            fs_strip<K, 4>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
     }
-    if (x + 4 <= pu.w)
-        fs_strip<K, 4>(win, pitch, lane_base + x, cblk + x, cpitch, pu.h, acc);
 
-    // MV cost, legality and the (cost, order) key (me.c:187-192, me.h:70-75)
+    // MV cost, legality and the first strict minimum (ME_COST_IPEL, me.c:187-192;
+    // COPY3_IF_LT, me.h:70-75).  A lane visits its candidates in evaluation
+    // order (rows j, then columns i), so a strict '<' on the cost alone keeps
+    // the earliest of equal costs; the order joins the key only at the end.
     const MvWindow mw = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
     const int R2 = 2 * R;
     const int col_end = it.col0 + it.ncols, row_end = it.row0 + it.nrows;
-    uint64_t kbest = ~0ull;
     int bits_x[4];
 #pragma unroll
     for (int d = 0; d < 4; d++)
         bits_x[d] = xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x);
+    // uniform fast path: the band is complete and lies inside the legal MV box
+    // (for |MV| < 8192 the packed CHECK_MV_RANGE equals the box test)
+    const int bmx0 = in.omx + it.col0 - R, bmy0 = in.omy + it.row0 - R;
+    const bool full = it.ncols == 4 * it.LC && it.nrows == (64 / it.LC) * K;
+    const bool fast = full && bmx0 >= mw.min_fpel[0] && bmx0 + it.ncols - 1 <= mw.max_fpel[0] &&
+                      bmy0 >= mw.min_fpel[1] && bmy0 + it.nrows - 1 <= mw.max_fpel[1];
+    const uint32_t lam = (uint32_t)p.lambda;   // < 2^24 (checked when the plan is made)
+    uint32_t bcost = 0xFFFFFFFFu;
+    int bidx = 0;
+    if (fast) {
+        // term = (MV cost << 5) | candidate index, computed once per item and
+        // centre; then key = (SAD << 5) + term and one unsigned min per
+        // candidate (SAD < 2^21 and MV cost < 2^15, so no overflow)
+        if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int j = row + k;
-        const int my = in.omy + j - R;
-        const int by_bits = xme_mvbits(4 * my - pu.mvp_y);
+            for (int k = 0; k < K; k++) {
+                const int by_bits = xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y);
 #pragma unroll
-        for (int d = 0; d < 4; d++) {
-            const int i = col + d;
-            const int mx = in.omx + i - R;
-            const bool inside = i < col_end && j < row_end;
-            const bool legal = inside && xme_mv_legal(mw, mx, my);
-            const int cost = (int)acc[k][d] + xme_weighted_cost(p.lambda, bits_x[d] + by_bits);
-            const uint64_t key = legal ? xme_key((uint32_t)cost, (uint32_t)(XME_ORDER_RASTER + j * R2 + i)) : ~0ull;
-            kbest = key < kbest ? key : kbest;
-            if (cost_map != nullptr && inside)
-                cost_map[(int64_t)inst_id * R2 * R2 + j * R2 + i] = legal ? cost : -1;
+                for (int d = 0; d < 4; d++)
+                    term[(k * 4 + d) * 64 + lane] =
+                        ((__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << 5) | (uint32_t)(k * 4 + d);
+            }
+            ts.ok = true;
+            ts.cx = in.omx;
+            ts.cy = in.omy;
+        }
+        uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+                m = min(m, (acc[k][d] << 5) + term[(k * 4 + d) * 64 + lane]);
+        bcost = m >> 5;
+        bidx = (int)(m & 31u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = row + k;
+            const int my = in.omy + j - R;
+            const int by_bits = xme_mvbits(4 * my - pu.mvp_y);
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int i = col + d;
+                const bool legal = i < col_end && j < row_end && xme_mv_legal(mw, in.omx + i - R, my);
+                const uint32_t cost = acc[k][d] + (__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16);
+                if (legal && cost < bcost) {
+                    bcost = cost;
+                    bidx = k * 4 + d;
+                }
+            }
         }
     }
+    uint64_t kbest = ~0ull;
+    if (bcost != 0xFFFFFFFFu)
+        kbest = xme_key(bcost, (uint32_t)(XME_ORDER_RASTER + (row + (bidx >> 2)) * R2 + col + (bidx & 3)));
     kbest = wave_min_u64(kbest);
     if (lane == 0 && kbest != ~0ull)
         atomicMin(reinterpret_cast<unsigned long long *>(best + inst_id), (unsigned long long)kbest);
+    if (cost_map != nullptr) {       // diagnostic output: every raster cost, -1 where illegal
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = row + k;
+            const int my = in.omy + j - R;
+            const int by_bits = xme_mvbits(4 * my - pu.mvp_y);
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int i = col + d;
+                if (i < col_end && j < row_end) {
+                    const bool legal = xme_mv_legal(mw, in.omx + i - R, my);
+                    const int cost = (int)(acc[k][d] + (__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16));
+                    cost_map[(int64_t)inst_id * R2 * R2 + j * R2 + i] = legal ? cost : -1;
+                }
+            }
+        }
+    }
 }
 
 // Stage a task's reference window and current pixels in LDS.
@@ -297,12 +440,18 @@ __device__ __forceinline__ int window_pitch(int ww)
     return (dw | 1) << 2;
 }
 
-template <bool PERSISTENT>
-__global__ void __launch_bounds__(256) fs_kernel(DevParams p, const FrameDesc *__restrict__ desc,
+// Byte offset of the per-wave MV-cost term tables (16-byte aligned, after the
+// window and the current pixels); each of the 4 waves owns 8 * 4 * 64 dwords.
+__host__ __device__ inline int fs_term_offset(int staged_bytes) { return (staged_bytes + 15) & ~15; }
+
+// KMAX bounds the rows per lane the kernel is compiled for: the K = 4 variant
+// fits 128 VGPRs and runs 4 waves per SIMD, the K = 8 variant 2 waves.
+template <bool PERSISTENT, int KMAX>
+__global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                  const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
                                                  const FsTask *__restrict__ tasks, int n_tasks,
-                                                 const FsItem *__restrict__ items, uint64_t *__restrict__ best,
-                                                 uint32_t *__restrict__ counters)
+                                                 const FsItem *__restrict__ items, const int32_t *__restrict__ item_insts,
+                                                 uint64_t *__restrict__ best, uint32_t *__restrict__ counters)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ int s_task;
@@ -326,6 +475,8 @@ __global__ void __launch_bounds__(256) fs_kernel(DevParams p, const FrameDesc *_
             s_item = t.item_begin;
         __syncthreads();
         const uint8_t *cur = lds + pitch * t.wh;
+        uint32_t *term = reinterpret_cast<uint32_t *>(lds + fs_term_offset(pitch * t.wh + t.cw * t.ch)) +
+                         (threadIdx.x >> 6) * (8 * 4 * 64);
         for (;;) {
             int it_i = 0;
             if ((threadIdx.x & 63) == 0)
@@ -334,16 +485,22 @@ __global__ void __launch_bounds__(256) fs_kernel(DevParams p, const FrameDesc *_
             if (it_i >= t.item_end)
                 break;
             const FsItem it = items[it_i];
-            const DevInst in = inst[it.inst];
-            if (in.status)
-                continue;
-            const DevPu pu = pus[it.inst / p.n_ref];
-            if (it.K == 8)
-                fs_item<8>(p, pu, in, it, it.inst, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
-                           const_cast<int32_t *>(cost_map));
-            else
-                fs_item<4>(p, pu, in, it, it.inst, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
-                           const_cast<int32_t *>(cost_map));
+            TermState ts;
+            ts.ok = false;
+            ts.cx = ts.cy = 0;
+            for (int g = 0; g < it.count; g++) {
+                const int inst_id = __builtin_amdgcn_readfirstlane(item_insts[it.list + g]);
+                const DevInst in = inst[inst_id];
+                if (in.status)
+                    continue;
+                const DevPu pu = pus[inst_id / p.n_ref];
+                if (KMAX == 8 && it.K == 8)
+                    fs_item<KMAX>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
+                                  const_cast<int32_t *>(cost_map), term, ts);
+                else
+                    fs_item<4>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
+                               const_cast<int32_t *>(cost_map), term, ts);
+            }
         }
         __syncthreads();   // LDS is reused by the next task
         if (!PERSISTENT)
@@ -370,11 +527,11 @@ int xme_fs_prepare(void)
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-        err = hipFuncSetAttribute(reinterpret_cast<const void *>(&fs_kernel<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, xme_fs_max_lds());
-        if (err == hipSuccess)
-            err = hipFuncSetAttribute(reinterpret_cast<const void *>(&fs_kernel<false>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, xme_fs_max_lds());
+        const void *ks[4] = { xme_fs_kernel_ptr(true, 4), xme_fs_kernel_ptr(true, 8), xme_fs_kernel_ptr(false, 4),
+                              xme_fs_kernel_ptr(false, 8) };
+        for (const void *k : ks)
+            if (err == hipSuccess)
+                err = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, xme_fs_max_lds());
     });
     if (err != hipSuccess)
         return xme_hip_fail(err, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
@@ -392,27 +549,27 @@ int xme_launch_mvc(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_p
 }
 
 int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
-                  const FsTask *d_tasks, int n_tasks, const FsItem *d_items, uint64_t *d_best,
-                  uint32_t *d_counters, size_t lds_bytes, int grid, bool persistent, hipStream_t s)
+                  const FsTask *d_tasks, int n_tasks, const FsItem *d_items, const int32_t *d_item_insts,
+                  uint64_t *d_best, uint32_t *d_counters, size_t lds_bytes, int grid, bool persistent, int kmax,
+                  hipStream_t s)
 {
     if (n_tasks <= 0)
         return XME_OK;
     int rc = xme_fs_prepare();
     if (rc)
         return rc;
-    if (persistent) {
-        hipLaunchKernelGGL(fs_kernel<true>, dim3(grid), dim3(256), lds_bytes, s, p, d_desc, d_pus, d_inst, d_tasks,
-                           n_tasks, d_items, d_best, d_counters);
-    } else {
-        hipLaunchKernelGGL(fs_kernel<false>, dim3(n_tasks), dim3(256), lds_bytes, s, p, d_desc, d_pus, d_inst,
-                           d_tasks, n_tasks, d_items, d_best, d_counters);
-    }
-    XME_HIP(hipGetLastError());
+    const void *k = xme_fs_kernel_ptr(persistent, kmax);
+    void *args[] = { (void *)&p, (void *)&d_desc, (void *)&d_pus, (void *)&d_inst, (void *)&d_tasks,
+                     (void *)&n_tasks, (void *)&d_items, (void *)&d_item_insts, (void *)&d_best, (void *)&d_counters };
+    XME_HIP(hipLaunchKernel(k, dim3(persistent ? grid : n_tasks), dim3(256), args, lds_bytes, s));
     return XME_OK;
 }
 
-const void *xme_fs_kernel_ptr(bool persistent)
+const void *xme_fs_kernel_ptr(bool persistent, int kmax)
 {
-    return persistent ? reinterpret_cast<const void *>(&fs_kernel<true>)
-                      : reinterpret_cast<const void *>(&fs_kernel<false>);
+    if (persistent)
+        return kmax == 8 ? reinterpret_cast<const void *>(&fs_kernel<true, 8>)
+                         : reinterpret_cast<const void *>(&fs_kernel<true, 4>);
+    return kmax == 8 ? reinterpret_cast<const void *>(&fs_kernel<false, 8>)
+                     : reinterpret_cast<const void *>(&fs_kernel<false, 4>);
 }

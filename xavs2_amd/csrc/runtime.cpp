@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -177,11 +178,13 @@ struct xme_plan {
     uint64_t *d_best = nullptr;
     FsTask *d_tasks = nullptr;
     FsItem *d_items = nullptr;
+    int32_t *d_item_insts = nullptr;
     uint32_t *d_counters = nullptr;
     FrameDesc *d_desc = nullptr;
     int n_tasks = 0;
     size_t lds_bytes = 0;
     int grid = 0;
+    int kmax = 8;              // rows-per-lane bound of the search kernel variant
     std::vector<LaunchGroup> groups;
     DescSlot slots[8];
     int slot_next = 0;
@@ -205,7 +208,7 @@ struct Rect {
 
 struct HostItem {
     FsItem it;
-    int pu;
+    std::vector<int> insts;   // instance ids pu * n_ref + ref (one shape, one MVP)
 };
 
 // The possible full-search centres of a PU: the clipped MVP and the clipped
@@ -251,28 +254,54 @@ static int window_pitch_host(int ww)
     return (dw | 1) << 2;
 }
 
-static void items_for(const DevParams &p, const DevPu &pu, int part, int inst, int row_lo, int row_hi,
-                      std::vector<HostItem> &out)
+// Items for a group of instances sharing PU shape and MVP: one item per band
+// of candidate rows/columns, covering every instance of the group.
+static void items_for(const DevParams &p, const std::vector<DevPu> &pus, const std::vector<int> &group, int part,
+                      int row_lo, int row_hi, std::vector<HostItem> &out)
 {
+    const DevPu &pu = pus[group[0] / p.n_ref];
     TuneChoice tc = xme_tune_lookup(part, p.R);
-    const int K = (pu.h % tc.K == 0) ? tc.K : 4;
+    const int Kt = std::min(tc.K, xme_fs_kmax());
+    const int K = (pu.h % Kt == 0) ? Kt : 4;
     const int LC = tc.LC;
     const int band_rows = (64 / LC) * K, band_cols = 4 * LC;
     const int R2 = 2 * p.R;
     for (int r0 = row_lo; r0 < row_hi; r0 += band_rows)
         for (int c0 = 0; c0 < R2; c0 += band_cols) {
             HostItem h;
-            h.it.inst = inst;
+            h.it.list = 0;
+            h.it.count = (int16_t)group.size();
             h.it.row0 = (int16_t)r0;
             h.it.nrows = (int16_t)std::min(band_rows, row_hi - r0);
             h.it.col0 = (int16_t)c0;
             h.it.ncols = (int16_t)std::min(band_cols, R2 - c0);
             h.it.K = (uint8_t)K;
             h.it.LC = (uint8_t)LC;
-            h.it.cost = (uint16_t)std::min(65535, (pu.w * pu.h) >> 4);
-            h.pu = inst / p.n_ref;
+            h.it.pad[0] = h.it.pad[1] = 0;
+            h.it.cost = (uint16_t)std::min<int64_t>(65535, ((int64_t)pu.w * pu.h * group.size()) >> 4);
+            h.insts = group;
             out.push_back(h);
         }
+}
+
+// Split instances of one shape and MVP into groups whose items carry similar
+// work (about 1024 pixels of PU area per item).
+static std::vector<std::vector<int>> make_groups(const std::vector<DevPu> &pus, const std::vector<int> &insts,
+                                                 int n_ref, int max_group)
+{
+    std::map<std::tuple<int, int, int>, std::vector<int>> by_key;
+    for (int id : insts) {
+        const DevPu &pu = pus[id / n_ref];
+        by_key[{ pu.part, pu.mvp_x, pu.mvp_y }].push_back(id);
+    }
+    std::vector<std::vector<int>> out;
+    for (auto &kv : by_key) {
+        const DevPu &pu = pus[kv.second[0] / n_ref];
+        const int g = std::max(1, std::min(max_group, 1024 / (pu.w * pu.h)));
+        for (size_t i = 0; i < kv.second.size(); i += g)
+            out.emplace_back(kv.second.begin() + i, kv.second.begin() + std::min(kv.second.size(), i + g));
+    }
+    return out;
 }
 
 // Build a task from items of one reference.  Returns the LDS bytes it needs.
@@ -282,12 +311,14 @@ static size_t make_task(const DevParams &p, const std::vector<DevPu> &pus, const
     Rect win = { 1 << 30, 1 << 30, -(1 << 30), -(1 << 30) };
     Rect cur = win;
     for (const HostItem &h : its) {
-        const DevPu &pu = pus[h.pu];
-        int c[2][2];
-        int n = possible_centres(p, pu, c);
-        for (int k = 0; k < n; k++)
-            win.add(item_footprint(p, pu, h.it, c[k][0], c[k][1]));
-        cur.add(Rect{ pu.x, pu.y, pu.x + pu.w, pu.y + pu.h });
+        for (int id : h.insts) {
+            const DevPu &pu = pus[id / p.n_ref];
+            int c[2][2];
+            int n = possible_centres(p, pu, c);
+            for (int k = 0; k < n; k++)
+                win.add(item_footprint(p, pu, h.it, c[k][0], c[k][1]));
+            cur.add(Rect{ pu.x, pu.y, pu.x + pu.w, pu.y + pu.h });
+        }
     }
     t.ref = ref;
     const int wx0 = (int)std::floor(win.x0 / 4.0) * 4;
@@ -299,7 +330,8 @@ static size_t make_task(const DevParams &p, const std::vector<DevPu> &pus, const
     t.cy0 = (int16_t)cur.y0;
     t.cw = (int16_t)(((cur.x1 - cur.x0) + 3) & ~3);
     t.ch = (int16_t)(cur.y1 - cur.y0);
-    return (size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch;
+    const size_t staged = (size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch;
+    return ((staged + 15) & ~(size_t)15) + XME_FS_TERM_BYTES;
 }
 
 } // namespace
@@ -311,6 +343,7 @@ static void plan_free(xme_plan *pl)
     hipFree(pl->d_best);
     hipFree(pl->d_tasks);
     hipFree(pl->d_items);
+    hipFree(pl->d_item_insts);
     hipFree(pl->d_counters);
     hipFree(pl->d_desc);
     for (auto &s : pl->slots) {
@@ -339,6 +372,10 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     if (!prm || !pus_in || !out || n_pu <= 0 || n_ref <= 0 || n_ref > XME_MAX_REFS ||
         (mode != XME_PLAN_BATCHED && mode != XME_PLAN_PLAIN)) {
         xme_set_error("xme_plan_create: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (prm->lambda_factor < 0 || prm->lambda_factor >= (1 << 24)) {
+        xme_set_error("xme_plan_create: lambda_factor must lie in [0, 2^24) (sqrt(lambda) < 256)");
         return XME_ERR_ARG;
     }
     if (prm->search_range < 1 || prm->search_range > 128 || prm->frame_w < 8 || prm->frame_h < 8 ||
@@ -399,6 +436,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     xme_plan *pl = new xme_plan();
     hipGetDevice(&pl->device);
     pl->mode = mode;
+    pl->kmax = xme_fs_kmax();
     pl->p = p;
     pl->n_pu = n_pu;
     pl->n_ref = n_ref;
@@ -426,6 +464,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     const size_t lds_cap = (size_t)xme_fs_max_lds();
     std::vector<FsTask> tasks;
     std::vector<FsItem> items;
+    std::vector<int32_t> item_insts;
     std::vector<int> task_part;
     const int R2 = 2 * p.R;
 
@@ -437,8 +476,14 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         std::vector<HostItem> sorted = its;
         std::stable_sort(sorted.begin(), sorted.end(),
                          [](const HostItem &a, const HostItem &b) { return a.it.cost > b.it.cost; });
-        for (const HostItem &h : sorted)
-            items.push_back(h.it);
+        for (const HostItem &h : sorted) {
+            FsItem it = h.it;
+            it.list = (int32_t)item_insts.size();
+            it.count = (int16_t)h.insts.size();
+            for (int id : h.insts)
+                item_insts.push_back(id);
+            items.push_back(it);
+        }
         t.item_end = (int)items.size();
         tasks.push_back(t);
         task_part.push_back(part);
@@ -458,12 +503,17 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
                     bool ok = true;
                     std::vector<std::vector<HostItem>> groups(splits);
                     const int rows_per = (R2 + splits - 1) / splits;
+                    std::vector<int> insts;
+                    for (int i : kv.second)
+                        insts.push_back(i * n_ref + r);
+                    const auto inst_groups = make_groups(pus, insts, n_ref, XME_MAX_GROUP);
                     for (int s = 0; s < splits; s++) {
-                        for (int i : kv.second)
-                            items_for(p, pus[i], parts[i], i * n_ref + r, s * rows_per,
+                        for (const auto &ig : inst_groups)
+                            items_for(p, pus, ig, pus[ig[0] / n_ref].part, s * rows_per,
                                       std::min(R2, (s + 1) * rows_per), groups[s]);
                         FsTask t;
-                        if (!groups[s].empty() && make_task(p, pus, groups[s], r, t) > lds_cap / 3)
+                        // two workgroups of the search kernel share a CU
+                        if (!groups[s].empty() && make_task(p, pus, groups[s], r, t) > lds_cap / 2)
                             ok = false;
                     }
                     if (ok || splits >= R2 / 4) {
@@ -508,8 +558,8 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
                         std::vector<std::vector<HostItem>> groups(splits);
                         bool ok = true;
                         for (int s = 0; s < splits; s++) {
-                            items_for(p, pus[i], part, i * n_ref + r, s * rows_per, std::min(R2, (s + 1) * rows_per),
-                                      groups[s]);
+                            items_for(p, pus, std::vector<int>{ i * n_ref + r }, part, s * rows_per,
+                                      std::min(R2, (s + 1) * rows_per), groups[s]);
                             FsTask t;
                             if (make_task(p, pus, groups[s], r, t) > lds_cap / 2)
                                 ok = false;
@@ -532,7 +582,8 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     pl->n_tasks = (int)tasks.size();
 
     for (const FsTask &t : tasks) {
-        size_t need = (size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch;
+        size_t need = (((size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch + 15) & ~(size_t)15) +
+                      XME_FS_TERM_BYTES;
         if (need > lds_cap) {
             xme_set_error("xme_plan_create: a search window needs %zu bytes of LDS", need);
             plan_free(pl);
@@ -551,6 +602,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     up((void **)&pl->d_pus, pus.data(), pus.size() * sizeof(DevPu));
     up((void **)&pl->d_tasks, tasks.data(), tasks.size() * sizeof(FsTask));
     up((void **)&pl->d_items, items.data(), items.size() * sizeof(FsItem));
+    up((void **)&pl->d_item_insts, item_insts.data(), item_insts.size() * sizeof(int32_t));
     if (e == hipSuccess)
         e = hipMalloc((void **)&pl->d_inst, sizeof(DevInst) * pl->n_inst);
     if (e == hipSuccess)
@@ -582,7 +634,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
             return prc;
         }
         int per_cu = 0;
-        hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, xme_fs_kernel_ptr(true), 256,
+        hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, xme_fs_kernel_ptr(true, pl->kmax), 256,
                                                                       pl->lds_bytes);
         int cus = 0;
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
@@ -665,11 +717,12 @@ static int launch_all(xme_plan *pl, hipStream_t s)
         return rc;
     if (pl->mode == XME_PLAN_BATCHED) {
         rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, pl->n_tasks, pl->d_items,
-                           pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, s);
+                           pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, pl->kmax, s);
     } else {
         for (const LaunchGroup &g : pl->groups) {
             rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
-                               g.task_end - g.task_begin, pl->d_items, pl->d_best, pl->d_counters, g.lds, 0, false, s);
+                               g.task_end - g.task_begin, pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters, g.lds, 0,
+                               false, pl->kmax, s);
             if (rc)
                 return rc;
         }
@@ -745,12 +798,12 @@ extern "C" int xme_me_time(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const
         XME_HIP(hipEventRecord(ev[4 * r + 1], s));
         if (pl->mode == XME_PLAN_BATCHED) {
             rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, pl->n_tasks, pl->d_items,
-                               pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, s);
+                               pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, pl->kmax, s);
         } else {
             for (const LaunchGroup &g : pl->groups) {
                 rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
-                                   g.task_end - g.task_begin, pl->d_items, pl->d_best, pl->d_counters, g.lds, 0,
-                                   false, s);
+                                   g.task_end - g.task_begin, pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters,
+                                   g.lds, 0, false, pl->kmax, s);
                 if (rc)
                     break;
             }

@@ -2,6 +2,7 @@
 // LC lanes per candidate row) is timed on the device for a PU shape and
 // search range; winners are kept for the process and used by later plans.
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -24,6 +25,28 @@ int pow2_at_least(int v)
     return p;
 }
 } // namespace
+
+namespace {
+int g_kmax = 0;
+}
+
+int xme_fs_kmax(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_kmax == 0) {
+        // K = 8 (2 waves per SIMD) measured faster than K = 4 (4 waves per
+        // SIMD) on the 2160p mixed-PU job: per-qsad overhead dominates
+        const char *e = getenv("XME_FS_K");
+        g_kmax = (e && atoi(e) == 4) ? 4 : 8;
+    }
+    return g_kmax;
+}
+
+void xme_fs_set_kmax(int kmax)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_kmax = kmax == 8 ? 8 : 4;
+}
 
 TuneChoice xme_tune_default(int part, int R)
 {

@@ -17,4 +17,7 @@ void xme_tune_force(const TuneChoice *c);
 int xme_part_index(int w, int h);
 int xme_pu_w(int part);
 int xme_pu_h(int part);
-const void *xme_fs_kernel_ptr(bool persistent);
+const void *xme_fs_kernel_ptr(bool persistent, int kmax);
+// Rows-per-lane bound of the search kernel used by new plans (4 or 8).
+int xme_fs_kmax(void);
+void xme_fs_set_kmax(int kmax);

@@ -58,14 +58,19 @@ struct FsTask {
     int16_t cw, ch;            // (cw % 4 == 0)
 };
 
-// A wave work item: one band of candidate rows/columns of one (PU, reference).
+// A wave work item: one band of candidate rows/columns for a group of
+// (PU, reference) instances of one PU shape and one MVP, listed at
+// item_insts[list .. list + count).  Instance ids are pu * n_ref + ref.
 struct FsItem {
-    int32_t inst;              // pu * n_ref + ref
+    int32_t list;
+    int16_t count;
+    uint16_t cost;             // relative work (host-side ordering only)
     int16_t row0, nrows;       // candidate rows [row0, row0 + nrows) of [0, 2R)
     int16_t col0, ncols;       // candidate columns [col0, col0 + ncols) of [0, 2R)
     uint8_t K, LC;             // rows per lane, lanes per candidate row
-    uint16_t cost;             // relative work (host-side ordering only)
+    uint8_t pad[2];
 };
+#define XME_MAX_GROUP 16
 
 // Order of a candidate in xavs2_me_search: the MV candidate list comes first
 // (me.c:913-917), then the raster rows j, columns i (me.c:1159-1163).
@@ -151,8 +156,12 @@ int xme_hip_fail(hipError_t e, const char *what);
 int xme_launch_mvc(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
                    uint64_t *d_best, int n_inst, hipStream_t s);
 int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
-                  const FsTask *d_tasks, int n_tasks, const FsItem *d_items, uint64_t *d_best,
-                  uint32_t *d_counters, size_t lds_bytes, int grid, bool persistent, hipStream_t s);
+                  const FsTask *d_tasks, int n_tasks, const FsItem *d_items, const int32_t *d_item_insts,
+                  uint64_t *d_best, uint32_t *d_counters, size_t lds_bytes, int grid, bool persistent, int kmax,
+                  hipStream_t s);
+// LDS bytes of a search task beyond its staged window and current pixels
+// (per-wave MV-cost term tables of 4 waves x 8 x 4 x 64 dwords).
+#define XME_FS_TERM_BYTES (4 * 8 * 4 * 64 * 4)
 int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
                       const uint64_t *d_best, int n_inst, hipStream_t s);
 int xme_fs_max_lds(void);
