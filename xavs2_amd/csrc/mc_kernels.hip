@@ -4,8 +4,10 @@
 // 4-tap intpl_chroma_block_* filters (mc.c:275-337).  Block positions follow
 // get_mv_for_mc / cu_get_mc_pos (source/common/mc.h:52-75).
 //
-// One workgroup per block; a thread computes one output pixel.  Every sum is a
-// plain 32-bit integer expression of the reference's taps and roundings.
+// One workgroup per block; a thread computes one output pixel.  Pixel FIRs are
+// v_dot4_i32_i8 products of the packed taps with (pixel - 128) bytes plus
+// 128 * 64 (every AVS2 filter's taps sum to 64), so each sum is an exact 32-bit
+// integer; the compiler then has no 16-bit form to narrow the sums into.
 #include "xme_internal.h"
 
 namespace {
@@ -18,6 +20,32 @@ __constant__ int8_t c_chroma[8][4] = { { 0, 64, 0, 0 }, { -4, 62, 6, 0 }, { -6, 
                                        { -4, 36, 36, -4 }, { -3, 25, 47, -5 }, { -1, 15, 56, -6 }, { 0, 6, 62, -4 } };
 
 __device__ __forceinline__ int clip_pel(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+
+__device__ __forceinline__ int pack_taps(const int8_t *t)
+{
+    return (int)((uint32_t)(uint8_t)t[0] | ((uint32_t)(uint8_t)t[1] << 8) | ((uint32_t)(uint8_t)t[2] << 16) |
+                 ((uint32_t)(uint8_t)t[3] << 24));
+}
+
+// four pixels s[0], s[step], s[2 step], s[3 step] as (pixel - 128) signed bytes
+__device__ __forceinline__ int pack_px(const uint8_t *s, int64_t step)
+{
+    return (int)(((uint32_t)s[0] | ((uint32_t)s[step] << 8) | ((uint32_t)s[2 * step] << 16) |
+                  ((uint32_t)s[3 * step] << 24)) ^ 0x80808080u);
+}
+
+// sum_k t_k * s[(k - 3) step], k = 0..7 (FLT_8TAP_HOR / FLT_8TAP_VER, mc.c:128-146)
+__device__ __forceinline__ int fir8(const uint8_t *s, int64_t step, int tlo, int thi)
+{
+    return __builtin_amdgcn_sdot4(thi, pack_px(s + step, step),
+                                  __builtin_amdgcn_sdot4(tlo, pack_px(s - 3 * step, step), 8192, false), false);
+}
+
+// sum_k t_k * s[(k - 1) step], k = 0..3 (FLT_4TAP_HOR / FLT_4TAP_VER)
+__device__ __forceinline__ int fir4(const uint8_t *s, int64_t step, int t)
+{
+    return __builtin_amdgcn_sdot4(t, pack_px(s - step, step), 8192, false);
+}
 
 // cu_get_mc_pos (mc.h:52-66); img_size <= 0 disables the clamp.
 __device__ __forceinline__ int mc_pos(int img_size, int blk_size, int blk_pos, int mv)
@@ -50,6 +78,7 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
     const int qx = mc_pos(fw, lw, lx, bl[4]), qy = mc_pos(fh, lh, ly, bl[5]);
     const int w = lw >> 1, h = lh >> 1;
     const int px = qx & 7, py = qy & 7;
+    const int th = pack_taps(c_chroma[px]);
     const int64_t off = (int64_t)(qy >> 3) * i_src + (qx >> 3);
     for (int t = threadIdx.x; t < 2 * w * h; t += blockDim.x) {
         const int c = t / (w * h), e = t % (w * h), y = e / w, x = e % w;
@@ -58,20 +87,15 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
         if (px == 0 && py == 0) {
             v = s[0];
         } else if (py == 0) {
-            v = clip_pel((c_chroma[px][0] * s[-1] + c_chroma[px][1] * s[0] + c_chroma[px][2] * s[1] +
-                          c_chroma[px][3] * s[2] + 32) >> 6);
+            v = clip_pel((fir4(s, 1, th) + 32) >> 6);
         } else if (px == 0) {
-            v = clip_pel((c_chroma[py][0] * s[-i_src] + c_chroma[py][1] * s[0] + c_chroma[py][2] * s[i_src] +
-                          c_chroma[py][3] * s[2 * i_src] + 32) >> 6);
+            v = clip_pel((fir4(s, i_src, pack_taps(c_chroma[py])) + 32) >> 6);
         } else {
+            // intpl_chroma_block_ext_c (mc.c:313-337): 8-bit first pass unshifted
             int acc = 0;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint8_t *r = s + (int64_t)(k - 1) * i_src;
-                const int hs = c_chroma[px][0] * r[-1] + c_chroma[px][1] * r[0] + c_chroma[px][2] * r[1] +
-                               c_chroma[px][3] * r[2];
-                acc += c_chroma[py][k] * hs;
-            }
+            for (int k = 0; k < 4; k++)
+                acc += (int)c_chroma[py][k] * fir4(s + (int64_t)(k - 1) * i_src, 1, th);
             v = clip_pel((acc + 2048) >> 12);
         }
         (c ? dv : du)[dst_off[b] + (int64_t)y * i_dst + x] = (uint8_t)v;
@@ -90,6 +114,8 @@ __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_re
     const int qx = mc_pos(fw, w, lx, bl[4]), qy = mc_pos(fh, h, ly, bl[5]);
     const int x0 = qx >> 2, y0 = qy >> 2, dx = qx & 3, dy = qy & 3;
     const uint8_t *plane = pl.p[(dy << 2) + dx];
+    const int hlo = pack_taps(c_luma[dx]), hhi = pack_taps(c_luma[dx] + 4);
+    const int vlo = pack_taps(c_luma[dy]), vhi = pack_taps(c_luma[dy] + 4);
     for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
         const int y = t / w, x = t % w;
         int v;
@@ -98,28 +124,17 @@ __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_re
         } else {
             const uint8_t *s = pl.p[0] + (int64_t)(y0 + y) * i_ref + x0 + x;
             if (dy == 0) {
-                int a = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    a += c_luma[dx][k] * s[k - 3];
-                v = clip_pel((a + 32) >> 6);
+                v = clip_pel((fir8(s, 1, hlo, hhi) + 32) >> 6);
             } else if (dx == 0) {
+                v = clip_pel((fir8(s, i_ref, vlo, vhi) + 32) >> 6);
+            } else {
+                // intpl_luma_block_ext_c (mc.c:362-398): the mct_t (int16)
+                // temporaries hold the horizontal sums unchanged, which lie in
+                // [-19 * 255, 88 * 255] for 8-bit pixels
                 int a = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++)
-                    a += c_luma[dy][k] * s[(int64_t)(k - 3) * i_ref];
-                v = clip_pel((a + 32) >> 6);
-            } else {
-                int a = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const uint8_t *r = s + (int64_t)(k - 3) * i_ref;
-                    int hsum = 0;
-#pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        hsum += c_luma[dx][j] * r[j - 3];
-                    a += c_luma[dy][k] * (int)(int16_t)hsum;     // mct_t temporaries (mc.c:373-378)
-                }
+                    a += (int)c_luma[dy][k] * fir8(s + (int64_t)(k - 3) * i_ref, 1, hlo, hhi);
                 v = clip_pel((a + 2048) >> 12);
             }
         }
