@@ -29,6 +29,14 @@ sys.path.insert(0, ROOT)
 METRIC = "ME cand-evals/s + roofline frac, batched vs plain; p50/p99 frame-ME latency"
 VALU_PEAK_TABSDIFF = 256 * 4 * 16 * 2.4e9 * 4 / 1e12     # v_sad_u8 / v_qsad: 157.3 T absdiff/s
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of the persistent search kernel on the default
+# workload (3840x2160, 4 refs, R=64, shipped tile table), measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
+# reading for a 256 MiB copy kernel (FETCH x2.000, WRITE x1.000):
+# 149,441.3 KiB fetched -> 306.0 MB, 54,008.1 KiB written -> 55.3 MB.
+# The counters cannot be read from inside this process, so the figure is quoted
+# for the workload it was measured on and null for any other.
+MEASURED_TRAFFIC = {(3840, 2160, 4, 64): 149441.3 * 1024 * 2.0 + 54008.1 * 1024 * 1.0}
 
 
 def log(msg):
@@ -104,6 +112,18 @@ def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
     }
 
 
+def copy_probe(X, mib):
+    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its
+    dispatches against mib MiB give the scale of the HBM traffic counters."""
+    n = mib << 20
+    src, dst = X.DeviceBuffer(n), X.DeviceBuffer(n)
+    src.zero()
+    for _ in range(20):
+        X.check(X.lib().xme_copy_kernel(dst.at(), src.at(), n, None), "xme_copy_kernel")
+    X.lib().xme_device_sync()
+    print(json.dumps({"copy_probe_bytes": n, "dispatches": 20}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +132,10 @@ def main():
     ap.add_argument("--plain-reps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default tile shapes, no table, no tuning")
+    ap.add_argument("--retune", action="store_true", help="ignore the shipped tile table and tune every shape")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--refs", type=int, default=4)
@@ -124,6 +147,10 @@ def main():
         sys.exit(2)
 
     from xavs2_amd import lib as X
+
+    if args.copy_probe:
+        copy_probe(X, args.copy_probe)
+        return
 
     w, h, n_ref, R = args.width, args.height, args.refs, args.range
     lam = 4 * 65536
@@ -142,10 +169,12 @@ def main():
 
     part_of = {(X.PU_W[i], X.PU_H[i]): i for i in range(25)}
     parts = sorted({part_of[(pw, ph)] for (_, _, pw, ph) in pus})
-    tiles = {}
+    tiles, tuned_now = {}, []
     if not args.no_tune:
-        log(f"autotuning tile shapes for {len(parts)} PU shapes at R={R}")
-        tiles = X.tune(prm, parts, stream.h)
+        from xavs2_amd import tiles as T
+        tiles, tuned_now = T.install(prm, parts, stream.h, retune=args.retune)
+        log(f"tile shapes for {len(parts)} PU shapes at R={R}: {len(parts) - len(tuned_now)} from the shipped "
+            f"table, {len(tuned_now)} tuned now")
     pu_arr = X.make_pus([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus])
     log(f"planning {len(pus)} PUs x {n_ref} refs")
     plan = X.Plan(prm, pu_arr, n_ref, X.PLAN_BATCHED)
@@ -221,6 +250,7 @@ def main():
             "absdiffs_per_step": int(n_absdiff),
             "lambda_factor": lam,
             "tiles_K_LC": {str(k): v for k, v in tiles.items()},
+            "tiles_tuned_in_run": tuned_now,
         },
         "roofline": {
             "bound": "valu",
@@ -228,7 +258,8 @@ def main():
             "peak": VALU_PEAK_TABSDIFF,
             "unit": "Tabsdiff/s",
             "frac": achieved / VALU_PEAK_TABSDIFF,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((w, h, n_ref, R)) if not (args.no_tune or args.retune) else None,
+            "traffic_unit": "HBM bytes per search-kernel dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "kernel": "fs_kernel<persistent> (integer full search)",
             "kernel_ms": search_ms,
         },

@@ -190,3 +190,14 @@ extern "C" int xme_tune(const xme_me_params *prm, const int *parts, int n_parts,
     hipFree(d_res);
     return tuned;
 }
+
+extern "C" int xme_tune_set(int part, int R, int k_rows, int lanes_per_row)
+{
+    const bool lc_ok = lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == 64;
+    if (part < 0 || part >= XME_NUM_PU || R < 1 || R > 128 || (k_rows != 4 && k_rows != 8) || !lc_ok) {
+        xme_set_error("xme_tune_set: part 0..24, R 1..128, K 4 or 8, LC 8/16/32/64");
+        return XME_ERR_ARG;
+    }
+    xme_tune_store(part, R, TuneChoice{ k_rows, lanes_per_row });
+    return XME_OK;
+}

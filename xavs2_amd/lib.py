@@ -85,6 +85,7 @@ def lib():
         L.xme_copy_kernel.argtypes = [P, P, C.c_size_t, P]
         L.xme_tune.argtypes = [C.POINTER(MeParams), P, I, P]
         L.xme_tune_get.argtypes = [I, I, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.xme_tune_set.argtypes = [I, I, I, I]
         L.xme_expand_border.argtypes = [P, Z, I, I, I, P]
         L.xme_interpolate_frame.argtypes = [P, Z, I, I, I, P]
         L.xme_sad_batch.argtypes = [I, P, Z, P, Z, P, P, I, P, P]
@@ -312,6 +313,16 @@ def tune(params, parts, stream=None):
         lib().xme_tune_get(int(p), params.search_range, C.byref(k), C.byref(lc))
         out[int(p)] = (k.value, lc.value)
     return out
+
+
+def tune_set(part, search_range, k_rows, lanes_per_row):
+    check(lib().xme_tune_set(int(part), int(search_range), int(k_rows), int(lanes_per_row)), "xme_tune_set")
+
+
+def tune_get(part, search_range):
+    k, lc = C.c_int(), C.c_int()
+    check(lib().xme_tune_get(int(part), int(search_range), C.byref(k), C.byref(lc)), "xme_tune_get")
+    return k.value, lc.value
 
 
 class Stream:
