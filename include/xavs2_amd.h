@@ -224,35 +224,60 @@ int xme_dequant_batch(int n, coeff_t *d_coef, int nblk, int qp, void *stream);
 int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t *d_level, coeff_t *d_recon,
                        int32_t *d_nnz, int nblk, void *stream);
 
-/* ---- 5. serving: continuous-batching ME for many concurrent encodes ------ */
+/* ---- 5. serving: continuous-batching ME for many concurrent encodes ------ *
+ * The frame-level ME of many encodes (streams) share one GPU.  Each stream
+ * keeps its last n_ref reconstructed references, with their 15 interpolated
+ * planes, resident in device memory.  A submitted frame is queued; the
+ * scheduler admits queued frames in arrival order whenever one of the
+ * max_inflight device slots is free and the frame's working memory fits the
+ * memory budget beside everything resident, so frames that arrive while
+ * others run start as soon as a slot frees and their kernels overlap on the
+ * device.  A frame is searched against the min(frame index, n_ref) previous
+ * frames of its stream and then becomes the stream's newest reference. */
 typedef struct xme_server xme_server;
 typedef struct {
     int64_t memory_budget;           /* bytes of device memory the server may hold */
-    int32_t max_inflight;            /* frame jobs on the device at once */
-    int32_t search_range;
-    int32_t lambda_factor;
-    int32_t n_ref;                   /* references per job */
+    int32_t max_inflight;            /* frame jobs on the device at once (1..16) */
+    int32_t search_range;            /* full search over [-R, R) */
+    int32_t lambda_factor;           /* h->i_lambda_factor */
+    int32_t n_ref;                   /* references per job (1..XME_MAX_REFS) */
+    int32_t use_hadamard;            /* as in xme_me_params */
+    int32_t use_fractional_me;
+    int32_t use_fast_sub_me;
+    int32_t enable_pmvr;
 } xme_server_config;
 typedef struct {
     int64_t submitted, completed, rejected;
-    double p50_ms, p99_ms, mean_ms, max_ms;
-    double throughput_fps;
-    int64_t resident_bytes;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> results on the host */
+    double throughput_fps;                    /* completed / (last completion - first submit) */
+    int64_t resident_bytes;                   /* references, planes and plans of open streams */
+    int64_t inflight_bytes;                   /* working memory of admitted frames */
+    int64_t peak_bytes;                       /* largest resident + inflight seen */
 } xme_server_stats;
 
 int xme_server_create(const xme_server_config *cfg, xme_server **srv);
+/* Waits for every admitted frame, then releases all device memory. */
 int xme_server_destroy(xme_server *srv);
-/* Register an encode (one video stream) of the given luma size. */
-int xme_server_open_stream(xme_server *srv, int width, int height, int *stream_id);
+/* Register an encode of the given luma size whose frames search the n_pu PUs
+ * (positions, sizes, MV predictors) of `pus`.  Fails with XME_ERR_BUDGET when
+ * its resident references and plans do not fit the memory budget. */
+int xme_server_open_stream(xme_server *srv, int width, int height, const xme_pu *pus, int n_pu, int *stream_id);
+/* Waits for the stream's frames, then frees its references and plans.  Jobs
+ * of the stream that were not waited for stay readable until waited. */
 int xme_server_close_stream(xme_server *srv, int stream_id);
-/* Submit one frame: host luma picture (width x height, stride) becomes both the
- * current picture of this job and, afterwards, the newest reference of its stream.
- * Returns a job id, or XME_ERR_BUDGET when admission fails. */
+/* Submit one frame: the host luma picture (width x height at `stride`) is
+ * copied, so the caller may reuse it on return.  Returns a job id, or
+ * XME_ERR_BUDGET (counted as rejected) when the frame could not fit the
+ * budget even with nothing else in flight. */
 int xme_server_submit(xme_server *srv, int stream_id, const pel_t *luma, intptr_t stride, int64_t *job_id);
-/* Wait for a job; copies its results (host array sized by xme_server_job_size). */
+/* Result count of a job: n_pu * refs searched (0 for a stream's first frame). */
 int xme_server_job_size(xme_server *srv, int64_t job_id, int *n_results);
+/* Waits for a job, copies its results (entry pu * refs + r, reference r = 0
+ * the previous frame) and forgets the job.  latency_ms: submit -> done. */
 int xme_server_wait(xme_server *srv, int64_t job_id, xme_result *results, double *latency_ms);
 int xme_server_stats_get(xme_server *srv, xme_server_stats *stats);
+/* Clears the latency record and counters (e.g. after a warm-up). */
+int xme_server_stats_reset(xme_server *srv);
 
 #ifdef __cplusplus
 }

@@ -191,6 +191,7 @@ struct xme_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t graph_stream = nullptr;
+    size_t dev_bytes = 0;
 };
 
 namespace {
@@ -596,6 +597,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         if (e != hipSuccess)
             return;
         e = hipMalloc(d, n ? n : 16);
+        pl->dev_bytes += n ? n : 16;
         if (e == hipSuccess && n)
             e = hipMemcpy(*d, h, n, hipMemcpyHostToDevice);
     };
@@ -613,6 +615,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         e = hipMemset(pl->d_counters, 0, 64);
     if (e == hipSuccess)
         e = hipMalloc((void **)&pl->d_desc, sizeof(FrameDesc));
+    pl->dev_bytes += (sizeof(DevInst) + sizeof(uint64_t)) * pl->n_inst + 64 + sizeof(FrameDesc);
     for (auto &s : pl->slots) {
         s.host = nullptr;
         s.ev = nullptr;
@@ -658,6 +661,8 @@ extern "C" int xme_plan_info(const xme_plan *pl, int64_t *n_inst, int64_t *n_can
         *n_absdiff = pl->n_absdiff;
     return XME_OK;
 }
+
+size_t xme_plan_device_bytes(const xme_plan *pl) { return pl ? pl->dev_bytes : 0; }
 
 int xme_plan_debug(const xme_plan *pl, int *n_tasks, int64_t *lds_bytes, int *grid)
 {

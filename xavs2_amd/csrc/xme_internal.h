@@ -166,3 +166,5 @@ int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *
                       const uint64_t *d_best, int n_inst, hipStream_t s);
 int xme_fs_max_lds(void);
 int xme_fs_prepare(void);
+// Device bytes a plan holds (work lists, per-instance state, descriptor).
+size_t xme_plan_device_bytes(const xme_plan *pl);

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxavs2_amd.so")
 
 XME_OK = 0
+XME_ERR_ARG, XME_ERR_HIP, XME_ERR_NOMEM, XME_ERR_BUDGET, XME_ERR_STATE = -1, -2, -3, -4, -5
 XME_PAD = 80
 XME_MAX_REFS = 8
 MAX_DISTORTION = 1 << 30
@@ -45,13 +46,15 @@ class RefDesc(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_int64), ("max_inflight", C.c_int32), ("search_range", C.c_int32),
-                ("lambda_factor", C.c_int32), ("n_ref", C.c_int32)]
+                ("lambda_factor", C.c_int32), ("n_ref", C.c_int32), ("use_hadamard", C.c_int32),
+                ("use_fractional_me", C.c_int32), ("use_fast_sub_me", C.c_int32), ("enable_pmvr", C.c_int32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("throughput_fps", C.c_double), ("resident_bytes", C.c_int64)]
+                ("throughput_fps", C.c_double), ("resident_bytes", C.c_int64), ("inflight_bytes", C.c_int64),
+                ("peak_bytes", C.c_int64)]
 
 
 _lib = None
@@ -105,7 +108,8 @@ def lib():
         L.xme_get_sad_x4_table.argtypes = [P]
         L.xme_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
         L.xme_server_destroy.argtypes = [P]
-        L.xme_server_open_stream.argtypes = [P, I, I, C.POINTER(C.c_int)]
+        L.xme_server_open_stream.argtypes = [P, I, I, P, I, C.POINTER(C.c_int)]
+        L.xme_server_stats_reset.argtypes = [P]
         L.xme_server_close_stream.argtypes = [P, I]
         L.xme_server_submit.argtypes = [P, I, P, Z, C.POINTER(C.c_int64)]
         L.xme_server_job_size.argtypes = [P, C.c_int64, C.POINTER(C.c_int)]
@@ -338,3 +342,65 @@ class Stream:
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.xme_stream_destroy(self.h)
             self.h = None
+
+
+class Server:
+    """The continuous-batching frame-ME server (xme_server_*)."""
+
+    def __init__(self, memory_budget, max_inflight=4, search_range=64, lambda_factor=4 * 65536, n_ref=2,
+                 use_hadamard=1, use_fractional_me=2, use_fast_sub_me=0, enable_pmvr=0):
+        cfg = ServerConfig()
+        cfg.memory_budget, cfg.max_inflight, cfg.search_range = int(memory_budget), max_inflight, search_range
+        cfg.lambda_factor, cfg.n_ref = lambda_factor, n_ref
+        cfg.use_hadamard, cfg.use_fractional_me = use_hadamard, use_fractional_me
+        cfg.use_fast_sub_me, cfg.enable_pmvr = use_fast_sub_me, enable_pmvr
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().xme_server_create(C.byref(cfg), C.byref(h)), "xme_server_create")
+        self.h = h
+
+    def open_stream(self, w, h, pus):
+        """pus: structured array of PU_DTYPE (see make_pus).  Raises XmeError
+        (code XME_ERR_BUDGET = -4) when the stream does not fit the budget."""
+        pus = np.ascontiguousarray(pus, dtype=PU_DTYPE)
+        sid = C.c_int()
+        check(lib().xme_server_open_stream(self.h, w, h, ptr(pus), len(pus), C.byref(sid)), "xme_server_open_stream")
+        return sid.value
+
+    def close_stream(self, sid):
+        check(lib().xme_server_close_stream(self.h, sid), "xme_server_close_stream")
+
+    def submit(self, sid, luma):
+        """Returns the job id; returns None when the frame is rejected for the budget."""
+        luma = np.ascontiguousarray(luma, dtype=np.uint8)
+        jid = C.c_int64()
+        rc = lib().xme_server_submit(self.h, sid, ptr(luma), luma.strides[0], C.byref(jid))
+        if rc == XME_ERR_BUDGET:
+            return None
+        check(rc, "xme_server_submit")
+        return jid.value
+
+    def wait(self, jid):
+        """Returns (results structured array, latency in ms)."""
+        n = C.c_int()
+        check(lib().xme_server_job_size(self.h, jid, C.byref(n)), "xme_server_job_size")
+        res = np.zeros(n.value, dtype=RESULT_DTYPE)
+        lat = C.c_double()
+        check(lib().xme_server_wait(self.h, jid, ptr(res) if n.value else None, C.byref(lat)), "xme_server_wait")
+        return res, lat.value
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().xme_server_stats_get(self.h, C.byref(s)), "xme_server_stats_get")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        check(lib().xme_server_stats_reset(self.h), "xme_server_stats_reset")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.xme_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -115,3 +115,26 @@ def mixed_pus(w, h, seed):
 
 def shape_tiling(w, h, pw, ph):
     return [(x, y, pw, ph) for y in range(0, h - ph + 1, ph) for x in range(0, w - pw + 1, pw)]
+
+
+def moving_sequence(w, h, n_frames, seed, pan=2, local=32, block=32, noise=3.0):
+    """A moving smooth+noise sequence: frame t is a crop of one larger smooth
+    field panned by a seeded global motion of at most `pan` px per frame and
+    axis, with every block x block tile displaced by a seeded local MV in
+    [-local, local]^2 (drawn afresh per frame) and fresh noise."""
+    rng = np.random.default_rng(seed)
+    gx, gy = (int(v) for v in rng.integers(-pan, pan + 1, size=2))
+    margin = local + pan * n_frames + 1
+    big = smooth_field(w + 2 * margin, h + 2 * margin, seed=seed + 1)
+    nby, nbx = (h + block - 1) // block, (w + block - 1) // block
+    yy = np.arange(h)[:, None]
+    xx = np.arange(w)[None, :]
+    frames = []
+    for t in range(n_frames):
+        mv = rng.integers(-local, local + 1, size=(nby, nbx, 2))
+        dx = np.repeat(np.repeat(mv[:, :, 0], block, 0), block, 1)[:h, :w]
+        dy = np.repeat(np.repeat(mv[:, :, 1], block, 0), block, 1)[:h, :w]
+        f = big[yy + margin + gy * t + dy, xx + margin + gx * t + dx].astype(np.float64)
+        f += rng.normal(0.0, noise, f.shape)
+        frames.append(np.clip(np.rint(f), 0, 255).astype(np.uint8))
+    return frames
