@@ -112,6 +112,179 @@ def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
     }
 
 
+def transform_leg(X, blocks_total, reps, seed=91):
+    """DCT -> flat quant -> dequant -> IDCT over blocks_total int16 blocks, a
+    quarter each of 4x4, 8x8, 16x16 and 32x32 (Laplacian b=6 residuals clipped
+    to +-255, 30 % all-zero blocks), at QP 22, 27, 32, 37, 42.  Algorithmic HBM
+    bytes: 6 per coefficient (int16 residual in, level and reconstruction out)."""
+    from oracle import oracle as O
+    qps = (22, 27, 32, 37, 42)
+    rng = np.random.default_rng(seed)
+    per = blocks_total // 4
+    bufs = []
+    n_elem = 0
+    sample = {}
+    for n in (4, 8, 16, 32):
+        host = np.empty((per, n, n), np.int16)
+        step = max(1, (1 << 24) // (n * n))
+        for b0 in range(0, per, step):
+            b1 = min(per, b0 + step)
+            v = np.rint(rng.laplace(0.0, 6.0, (b1 - b0, n, n))).clip(-255, 255)
+            v[rng.random(b1 - b0) < 0.3] = 0
+            host[b0:b1] = v
+        sample[n] = host[:256].copy()
+        src = X.upload(host)
+        bufs.append((n, src, X.DeviceBuffer(host.nbytes), X.DeviceBuffer(host.nbytes), X.DeviceBuffer(4 * per)))
+        n_elem += host.size
+        del host
+    stream = X.Stream()
+
+    def one_pass(qp):
+        for n, src, lv, rc, nz in bufs:
+            X.check(X.lib().xme_residual_batch(n, qp, 0, src.at(), lv.at(), rc.at(), nz.at(), per, stream.h),
+                    "xme_residual_batch")
+
+    one_pass(qps[0])
+    stream.sync()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for qp in qps:
+            one_pass(qp)
+        stream.sync()
+        times.append((time.perf_counter() - t0) / len(qps))
+    ms = min(times) * 1e3
+    gbs = 6.0 * n_elem / (ms * 1e-3) / 1e9
+    # the last pass (QP 42) against the oracle on the first 256 blocks of each size
+    ok = True
+    for n, src, lv, rc, nz in bufs:
+        el, er, en = O.residual_batch(n, qps[-1], 0, sample[n])
+        ok = ok and np.array_equal(lv.download(el.shape, np.int16), el) and \
+            np.array_equal(rc.download(er.shape, np.int16), er) and np.array_equal(nz.download(256, np.int32), en)
+    return {
+        "workload": f"{blocks_total} int16 blocks (1/4 each 4x4, 8x8, 16x16, 32x32; Laplacian b=6, +-255, 30 % zero), "
+                    "DCT -> flat quant -> dequant -> IDCT, inter rounding, QP 22/27/32/37/42",
+        "coefficients": int(n_elem),
+        "ms_per_qp_pass": ms,
+        "coef_per_s": n_elem / (ms * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "bytes_per_coefficient": 6},
+        "parity_with_oracle_on_sample": bool(ok),
+    }
+
+
+SERVE_STREAMS = [(1280, 720)] * 6 + [(1920, 1080)] * 6 + [(3840, 2160)] * 4
+
+
+def serving_leg(X, seconds, loads, slots, n_cycle, share=0, seed=77):
+    """16 concurrent encodes (6 x 720p, 6 x 1080p, 4 x 2160p), each a moving
+    smooth+noise sequence (pan <= 2 px/frame, local block MVs in [-32, 32]^2),
+    frames arriving as Poisson processes at 30-60 fps per stream (scaled by the
+    load factor); every frame is a 2-reference ME job over a mixed PU tiling,
+    R = 64, through the server under a 64 GB budget."""
+    import queue
+    import threading
+
+    from xavs2_amd import synth
+    rng = np.random.default_rng(seed)
+    fps = rng.uniform(30.0, 60.0, len(SERVE_STREAMS))
+    log(f"serving: generating {len(SERVE_STREAMS)} sequences of {n_cycle} frames")
+    seqs = [synth.moving_sequence(w, h, n_cycle, seed=seed + 10 * i) for i, (w, h) in enumerate(SERVE_STREAMS)]
+    srv = X.Server(64 << 30, max_inflight=slots, search_range=64, n_ref=2, workgroup_share_pct=share)
+    sids = []
+    for i, (w, h) in enumerate(SERVE_STREAMS):
+        pus = [(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in synth.mixed_pus(w, h, seed + 10 * i + 1)]
+        sids.append(srv.open_stream(w, h, X.make_pus(pus)))
+    frame_no = [0] * len(SERVE_STREAMS)
+
+    def submit(i):
+        j = srv.submit(sids[i], seqs[i][frame_no[i] % n_cycle])
+        frame_no[i] += 1
+        return j
+
+    # warm-up: every stream reaches two references (and captures its plans' graphs)
+    warm = [submit(i) for _ in range(3) for i in range(len(SERVE_STREAMS))]
+    for j in warm:
+        srv.wait(j)
+    out = []
+    for load in loads:
+        sched = []
+        for i in range(len(SERVE_STREAMS)):
+            t = float(rng.exponential(1.0 / (fps[i] * load)))
+            while t < seconds:
+                sched.append((t, i))
+                t += float(rng.exponential(1.0 / (fps[i] * load)))
+        sched.sort()
+        srv.reset_stats()
+        q = queue.Queue()
+        lat = [[] for _ in SERVE_STREAMS]
+
+        def waiter():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                i, j = item
+                lat[i].append(srv.wait(j)[1])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        late = 0.0
+        for t, i in sched:
+            dt = t0 + t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            else:
+                late = max(late, -dt)
+            q.put((i, submit(i)))
+        q.put(None)
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        per_class = {}
+        for (w, h) in sorted(set(SERVE_STREAMS)):
+            v = np.concatenate([np.array(lat[i]) for i in range(len(SERVE_STREAMS)) if SERVE_STREAMS[i] == (w, h)])
+            per_class[f"{w}x{h}"] = {"frames": int(len(v)), "p50_ms": float(np.percentile(v, 50)),
+                                     "p99_ms": float(np.percentile(v, 99)), "max_ms": float(v.max())}
+        offered = len(sched) / seconds
+        rec = {
+            "load": load,
+            "offered_fps": offered,
+            "achieved_fps": st["throughput_fps"],
+            "frames": int(st["completed"]),
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "max_ms": st["max_ms"],
+            "p99_over_p50": st["p99_ms"] / st["p50_ms"] if st["p50_ms"] else None,
+            "per_resolution": per_class,
+            "peak_device_bytes": int(st["peak_bytes"]),
+            "submit_late_max_ms": late * 1e3,
+            "wall_s": wall,
+        }
+        log(f"serving load {load}: offered {offered:.0f} fps, achieved {rec['achieved_fps']:.0f} fps, "
+            f"p50 {rec['p50_ms']:.2f} ms p99 {rec['p99_ms']:.2f} ms")
+        out.append(rec)
+    resident = srv.stats()["resident_bytes"]
+    srv.close()
+    rated = None
+    for rec in out:
+        keeps_up = rec["achieved_fps"] >= 0.97 * rec["offered_fps"]
+        tails = all(c["p99_ms"] <= 2.0 * c["p50_ms"] for c in rec["per_resolution"].values())
+        if keeps_up and tails:
+            rated = rec["load"]
+    return {
+        "workload": "16 concurrent encodes (6x1280x720, 6x1920x1080, 4x3840x2160), Poisson arrivals at 30-60 fps "
+                    "per stream x load, 2-ref ME over mixed PUs, R=64, 64 GB budget, "
+                    f"{slots} device slots, search launches capped at {share or 100} % of the device, "
+                    f"{seconds:.0f} s per load",
+        "stream_fps": [round(float(f), 2) for f in fps],
+        "resident_bytes": int(resident),
+        "loads": out,
+        "rated_load": rated,
+        "rated_load_rule": "largest load with achieved >= 97 % of offered fps and p99 <= 2 x p50 within every "
+                           "resolution class",
+    }
+
+
 def copy_probe(X, mib):
     """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its
     dispatches against mib MiB give the scale of the HBM traffic counters."""
@@ -134,6 +307,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes, no table, no tuning")
     ap.add_argument("--retune", action="store_true", help="ignore the shipped tile table and tune every shape")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-loads", default="0.5,0.75,1.0")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-frames", type=int, default=4, help="distinct frames per stream, cycled")
+    ap.add_argument("--serve-share", type=int, default=0,
+                    help="%% of a full-device search launch per frame in the serving leg (0 = 100)")
+    ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
+    ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
     ap.add_argument("--width", type=int, default=3840)
@@ -222,6 +404,16 @@ def main():
     plain_equal = bool(np.array_equal(plain_res, gpu_res))
     plain_value = n_cand / (plain_ms * 1e-3)
 
+    del plain
+    transform = None
+    if not args.no_transform:
+        log(f"residual transform leg: {args.transform_blocks} blocks")
+        transform = transform_leg(X, args.transform_blocks, reps=5)
+    serving = None
+    if not args.no_serve:
+        loads = [float(v) for v in args.serve_loads.split(",") if v]
+        serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -269,6 +461,8 @@ def main():
                        "mean": float(lat.mean())},
         "results_stable_across_replays": stable,
         "cpu_baseline": cpu,
+        "transform": transform,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -156,6 +156,11 @@ int xme_plan_create(const xme_me_params *params, const xme_pu *pus, int n_pu, in
 int xme_plan_destroy(xme_plan *plan);
 /* Per-plan counters: number of PUs x refs, integer candidates evaluated per run. */
 int xme_plan_info(const xme_plan *plan, int64_t *n_inst, int64_t *n_candidates, int64_t *n_absdiff);
+/* Caps the workgroups of the batched plan's persistent search launch (0 =
+ * as many as the device holds at once), so that searches of concurrent
+ * frames share the compute units instead of queueing behind each other. */
+int xme_plan_set_max_workgroups(xme_plan *plan, int max_workgroups);
+int xme_plan_workgroups(const xme_plan *plan, int *workgroups, int *device_fill);
 
 /* Run the search for one frame.  d_cur: current luma plane origin (no padding
  * needed).  refs: host array of n_ref descriptors.  d_results: n_pu * n_ref
@@ -245,6 +250,7 @@ typedef struct {
     int32_t use_fractional_me;
     int32_t use_fast_sub_me;
     int32_t enable_pmvr;
+    int32_t workgroup_share_pct;     /* % of a full-device search launch per frame (1..100; 0 = 100) */
 } xme_server_config;
 typedef struct {
     int64_t submitted, completed, rejected;

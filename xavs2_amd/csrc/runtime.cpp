@@ -184,6 +184,7 @@ struct xme_plan {
     int n_tasks = 0;
     size_t lds_bytes = 0;
     int grid = 0;
+    int grid_full = 0;         // occupancy-sized grid, before any cap
     int kmax = 8;              // rows-per-lane bound of the search kernel variant
     std::vector<LaunchGroup> groups;
     DescSlot slots[8];
@@ -643,7 +644,8 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
         if (oe != hipSuccess || per_cu < 1)
             per_cu = 1;
-        pl->grid = std::max(1, std::min(per_cu * cus, pl->n_tasks));
+        pl->grid_full = std::max(1, std::min(per_cu * cus, pl->n_tasks));
+        pl->grid = pl->grid_full;
     }
     *out = pl;
     return XME_OK;
@@ -659,6 +661,37 @@ extern "C" int xme_plan_info(const xme_plan *pl, int64_t *n_inst, int64_t *n_can
         *n_cand = pl->n_cand;
     if (n_absdiff)
         *n_absdiff = pl->n_absdiff;
+    return XME_OK;
+}
+
+extern "C" int xme_plan_set_max_workgroups(xme_plan *pl, int max_wg)
+{
+    if (!pl || max_wg < 0) {
+        xme_set_error("xme_plan_set_max_workgroups: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (pl->mode != XME_PLAN_BATCHED)
+        return XME_OK;
+    const int g = max_wg == 0 ? pl->grid_full : std::max(1, std::min(max_wg, pl->grid_full));
+    if (g != pl->grid && pl->graph_exec) {
+        // the captured launch holds the old grid
+        hipGraphExecDestroy(pl->graph_exec);
+        hipGraphDestroy(pl->graph);
+        pl->graph_exec = nullptr;
+        pl->graph = nullptr;
+    }
+    pl->grid = g;
+    return XME_OK;
+}
+
+extern "C" int xme_plan_workgroups(const xme_plan *pl, int *grid, int *grid_full)
+{
+    if (!pl)
+        return XME_ERR_ARG;
+    if (grid)
+        *grid = pl->grid;
+    if (grid_full)
+        *grid_full = pl->grid_full;
     return XME_OK;
 }
 

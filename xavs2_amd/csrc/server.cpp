@@ -349,7 +349,8 @@ extern "C" int xme_server_create(const xme_server_config *cfg, xme_server **out)
 {
     if (!cfg || !out || cfg->memory_budget <= 0 || cfg->max_inflight < 1 || cfg->max_inflight > kMaxSlots ||
         cfg->search_range < 1 || cfg->search_range > 128 || cfg->n_ref < 1 || cfg->n_ref > XME_MAX_REFS ||
-        cfg->lambda_factor < 0 || cfg->use_fractional_me < 0 || cfg->use_fractional_me > 2) {
+        cfg->lambda_factor < 0 || cfg->use_fractional_me < 0 || cfg->use_fractional_me > 2 ||
+        cfg->workgroup_share_pct < 0 || cfg->workgroup_share_pct > 100) {
         xme_set_error("xme_server_create: bad configuration");
         return XME_ERR_ARG;
     }
@@ -444,6 +445,12 @@ extern "C" int xme_server_open_stream(xme_server *S, int w, int h, const xme_pu 
         }
         st->plans.push_back(p);
         st->resident += (int64_t)xme_plan_device_bytes(p);
+        const int share = S->cfg.workgroup_share_pct;
+        if (share > 0 && share < 100) {
+            int g = 0, full = 0;
+            xme_plan_workgroups(p, &g, &full);
+            xme_plan_set_max_workgroups(p, std::max(1, (full * share + 99) / 100));
+        }
     }
     st->resident += (int64_t)(ring_bytes * n_slots);
     std::vector<void *> drop;
