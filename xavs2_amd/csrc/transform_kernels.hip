@@ -3,30 +3,39 @@
 // N = 4, 8, 16, 32 (source/common/transform.c:167-1312, source/common/quant.c:109-184,
 // parameters of tu_quant_forward / tu_quant_inverse, source/encoder/rdo.c:658-718).
 //
-// A workgroup stages 4096 coefficients (4096 / N^2 blocks) in LDS.  Every 1-D
-// pass of the reference's partial butterflies is an exact integer product of a
-// matrix row with a block row, so each pass here reads a contiguous row of
-// int16 pairs and issues v_dot2_i32_i16 against a matrix row that is uniform
-// across the wave (scalar loads); it writes its outputs transposed, which is
-// the order the next pass reads (as in the reference, where pass outputs are
-// stored at dst[k * line + j]).  The 32-bit sums are then shifted, wrapped to
-// int16 (forward) or clipped (inverse) exactly like the reference.
+// Every 1-D pass of the reference's partial butterflies is an exact integer
+// matrix product, so each pass runs on the matrix cores:
+//
+//  * A wave owns a 32x32 tile holding (32/N)^2 blocks.  A block-diagonal tile
+//    matrix M (T_N repeated on the diagonal; |T| <= 90 fits int8) transforms
+//    every block of the tile at once.
+//  * A data operand (int16) is split exactly into int8 halves:
+//    v = 256 * hi + lo, hi = v >> 8 (signed), lo = v & 255.  With
+//    lo_s = lo - 128 (signed) each pass is
+//    D = M.lo_s + 128 * (sums of M) + 256 * M.hi, i.e. two
+//    v_mfma_i32_32x32x32_i8 and a shift-add.  The 128 * sums term and the
+//    rounding offset start the accumulator.
+//  * The MFMA result layout (lane = column, registers = rows
+//    (i % 4) + 8 (i / 4) + 4 (lane / 32)) is used directly as the next pass's
+//    data operand: the summation index K is permuted the same way in the
+//    constant operand, so no pass needs a transpose.
+//
+// Forward:  D1 = X . M^T           (tmp^T; shift log2N - 2, int16 wrap)
+//           D2 = M . D1            (coefficients; shift log2N + 5, int16 wrap)
+//           quantise, dequantise in registers
+// Inverse:  D3 = C^T . M           (first pass; shift 5, clip to int16)
+//           D4 = D3 . M            (residual; shift 12, clip to [-256, 255])
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 
 #include "xme_internal.h"
 
 namespace {
 
-typedef short short2_t __attribute__((ext_vector_type(2)));
-
-// Forward matrices T_N (row k, pairs of columns) and inverse matrices T_N^T
-// (row m holds T[k][m], pairs of k), N = 4, 8, 16, 32 at dword offsets
-// 0, 8, 40, 168 (N * N / 2 dwords of int16 pairs per matrix).
-__constant__ uint32_t c_fwd[8 + 32 + 128 + 512];
-__constant__ uint32_t c_inv[8 + 32 + 128 + 512];
-
-__host__ __device__ constexpr int mat_off(int n) { return n == 4 ? 0 : n == 8 ? 8 : n == 16 ? 40 : 168; }
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 // quant_c / dequant_c parameters
 struct QParams {
@@ -36,130 +45,214 @@ struct QParams {
 
 enum { DO_FWD = 1, DO_QUANT = 2, DO_DEQUANT = 4, DO_INV = 8 };
 
-template <int N>
-struct Geo {
-    static constexpr int NB = 4096 / (N * N);       // blocks per workgroup
-    static constexpr int ROWS = NB * N;             // block rows per workgroup
-    static constexpr int NP = N / 2;                // int16 pairs per row
-    // each thread: RT rows x OT outputs per pass; a wave shares its output range
-    static constexpr int OT = N == 32 ? 16 : N == 16 ? 16 : N;
-    static constexpr int RT = N == 32 ? 1 : N == 16 ? 1 : N == 8 ? 2 : 4;
-    static constexpr int OSPLIT = N / OT;           // thread groups splitting the outputs
+// Constant MFMA operands and accumulator starts of one block size, per lane.
+struct TileConsts {
+    uint32_t f1b[64][4];    // B of D1 = X . M^T: lane col k, K bytes n = 16 (lane / 32) + t: M[k][n]
+    uint32_t f2a[64][4];    // A of D2 = M . D1: lane row k2, K bytes j = rho(lane, t): M[k2][j]
+    uint32_t ib[64][4];     // B of D3, D4: lane col n, K bytes m = rho(lane, t): M[m][n]
+    int32_t rowsum[64];     // 128 * sum_n M[lane % 32][n]
+    int32_t colsum[64];     // 128 * sum_m M[m][lane % 32]
+    int32_t rowsum_rho[64][16];  // 128 * sum_j M[rho(lane, i)][j]
 };
+__device__ TileConsts g_tc[4];   // N = 4, 8, 16, 32
 
-__device__ __forceinline__ int dot_row(const uint32_t *__restrict__ row, const uint32_t *__restrict__ coef, int np,
-                                       int acc)
+__host__ __device__ constexpr int log2n_of(int n) { return n == 4 ? 2 : n == 8 ? 3 : n == 16 ? 4 : 5; }
+// tile row held by result register i of a lane in the upper (1) or lower (0) half-wave
+__host__ __device__ constexpr int rho(int half, int i) { return (i & 3) + 8 * (i >> 2) + 4 * half; }
+
+__device__ __forceinline__ v16i splat16(int v)
 {
-    for (int t = 0; t < np; t++)
-        acc = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, row[t]), __builtin_bit_cast(short2_t, coef[t]),
-                                     acc, false);
-    return acc;
+    v16i r;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        r[i] = v;
+    return r;
 }
 
-// One 1-D pass over every row of every block: in[b][r][0..N) -> out[b][m][r]
-// for all m (outputs written transposed).  mode 0: forward (wrap to int16),
-// 1: inverse (clip to [lo, hi]).
-template <int N, bool INV>
-__device__ __forceinline__ void pass(const int16_t *__restrict__ in, int16_t *__restrict__ out, int shift, int lo,
-                                     int hi)
+// D = data . B (data as the A operand) with data = 256 hi + lo_s + 128
+__device__ __forceinline__ v16i mm_data_a(v4i hi, v4i lo, v4i b, v16i acc)
 {
-    using G = Geo<N>;
-    const int tid = threadIdx.x;
-    const int og = tid / (256 / G::OSPLIT);          // output group: uniform across a wave
-    const int lt = tid % (256 / G::OSPLIT);
-    const uint32_t *mat = (INV ? c_inv : c_fwd) + mat_off(N);
-    const int add = (1 << shift) >> 1;
+    v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(lo, b, acc, 0, 0, 0);
+    v16i h = __builtin_amdgcn_mfma_i32_32x32x32_i8(hi, b, splat16(0), 0, 0, 0);
 #pragma unroll
-    for (int rr = 0; rr < G::RT; rr++) {
-        const int grow = lt * G::RT + rr;            // row index within the workgroup
-        if (grow >= G::ROWS)
-            break;
-        const int b = grow / N, r = grow % N;
-        uint32_t row[G::NP];
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(in + (b * N + r) * N);
+    for (int i = 0; i < 16; i++)
+        d[i] += h[i] << 8;
+    return d;
+}
+
+// D = A . data (data as the B operand)
+__device__ __forceinline__ v16i mm_data_b(v4i a, v4i hi, v4i lo, v16i acc)
+{
+    v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, lo, acc, 0, 0, 0);
+    v16i h = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, hi, splat16(0), 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < G::NP; t++)
-            row[t] = src[t];
+    for (int i = 0; i < 16; i++)
+        d[i] += h[i] << 8;
+    return d;
+}
+
+// Splits the low 16 bits of 16 values into the hi / lo_s byte operands
+// (K byte t = value t).  Taking bytes 0 and 1 is the int16 wrap.
+__device__ __forceinline__ void split16(const int (&v)[16], v4i &hi, v4i &lo)
+{
 #pragma unroll
-        for (int o = 0; o < G::OT; o++) {
-            const int m = og * G::OT + o;
-            const int s = (dot_row(row, mat + m * G::NP, G::NP, 0) + add) >> shift;
-            out[(b * N + m) * N + r] = INV ? (int16_t)xme_clip3(lo, hi, s) : (int16_t)s;
-        }
+    for (int w = 0; w < 4; w++) {
+        const uint32_t t01 = __builtin_amdgcn_perm((uint32_t)v[4 * w + 1], (uint32_t)v[4 * w], 0x04000501u);
+        const uint32_t t23 = __builtin_amdgcn_perm((uint32_t)v[4 * w + 3], (uint32_t)v[4 * w + 2], 0x04000501u);
+        hi[w] = (int)__builtin_amdgcn_perm(t23, t01, 0x05040100u);
+        lo[w] = (int)(__builtin_amdgcn_perm(t23, t01, 0x07060302u) ^ 0x80808080u);
     }
+}
+
+// Splits 16 int16 values held as 8 dwords (value 2d in the low half of d).
+__device__ __forceinline__ void split_pairs(const uint32_t (&d)[8], v4i &hi, v4i &lo)
+{
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        hi[w] = (int)__builtin_amdgcn_perm(d[2 * w + 1], d[2 * w], 0x07050301u);
+        lo[w] = (int)(__builtin_amdgcn_perm(d[2 * w + 1], d[2 * w], 0x06040200u) ^ 0x80808080u);
+    }
+}
+
+__device__ __forceinline__ v4i load4(const uint32_t (&p)[4])
+{
+    v4i r;
+    r[0] = (int)p[0];
+    r[1] = (int)p[1];
+    r[2] = (int)p[2];
+    r[3] = (int)p[3];
+    return r;
+}
+
+// Offset (in coefficients) of tile element (y, x) of tile `tile`.
+template <int N>
+__device__ __forceinline__ int64_t tile_off(int64_t tile, int y, int x)
+{
+    constexpr int NB = 32 / N;
+    return (tile * NB * NB + (y / N) * NB + x / N) * (N * N) + (y % N) * N + (x % N);
 }
 
 template <int N>
 __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict__ src, int16_t *__restrict__ level,
                                                        int16_t *__restrict__ recon, int32_t *__restrict__ nnz,
-                                                       int nblk, QParams q, int mode, int fshift1)
+                                                       int nblk, QParams q, int mode)
 {
-    using G = Geo<N>;
-    __shared__ __align__(16) int16_t A[4096];
-    __shared__ __align__(16) int16_t B[4096];
-    __shared__ __align__(16) int16_t L[4096];
-    __shared__ int s_nnz[G::NB];
-    const int tid = threadIdx.x;
-    const int blk0 = blockIdx.x * G::NB;
-    const int nb = min(G::NB, nblk - blk0);
-    const int nel = nb * N * N;
-    const int64_t base = (int64_t)blk0 * N * N;
+    constexpr int NB = 32 / N, BPT = NB * NB, L2 = log2n_of(N);
+    const TileConsts &tc = g_tc[L2 - 2];
+    const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
+    const int ntiles = (nblk + BPT - 1) / BPT;
+    const v4i f1b = load4(tc.f1b[lane]), f2a = load4(tc.f2a[lane]), ib = load4(tc.ib[lane]);
+    const int s1 = L2 - 2, s2 = L2 + 5;
+    const int acc_f1 = tc.rowsum[lane] + ((1 << s1) >> 1);
+    const int acc_i1 = tc.colsum[lane] + 16, acc_i2 = tc.colsum[lane] + 2048;
+    v16i acc_f2;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        acc_f2[i] = tc.rowsum_rho[lane][i] + (1 << (s2 - 1));
 
-    // load (16-byte chunks; 4096 elements = 512 chunks)
-    if (mode & DO_FWD) {
-        for (int e = tid; e < nel / 8; e += 256)
-            reinterpret_cast<uint4 *>(A)[e] = reinterpret_cast<const uint4 *>(src + base)[e];
-    } else {
-        // coefficients arrive row-major [k2][k]; the passes read them as rows of the transpose
-        for (int e = tid; e < nel; e += 256) {
-            const int b = e / (N * N), rem = e % (N * N), k2 = rem / N, k = rem % N;
-            A[(b * N + k) * N + k2] = src[base + e];
-        }
-    }
-    if (tid < G::NB)
-        s_nnz[tid] = 0;
-    __syncthreads();
-    int16_t *coefT = A;           // coefficients as rows of the transpose: coefT[b][col][row]
-    if (mode & DO_FWD) {
-        pass<N, false>(A, B, fshift1, 0, 0);                       // rows -> B[b][k][j]
-        __syncthreads();
-        pass<N, false>(B, A, (N == 4 ? 2 : N == 8 ? 3 : N == 16 ? 4 : 5) + 5, 0, 0);  // -> A[b][m][r]: coef(m, r) at A[b][m][r]
-        __syncthreads();
-        // A now holds coef[b][m][r] with m the vertical frequency (row) and r
-        // the horizontal frequency: the transpose layout the inverse reads is
-        // A[b][r][m]; quantise in place while transposing into B
-        for (int e = tid; e < nel; e += 256) {
-            const int b = e / (N * N), rem = e % (N * N), m = rem / N, r = rem % N;
-            int c = A[e];
-            if (mode & DO_QUANT) {
-                const int a = c < 0 ? -c : c;
-                c = (int16_t)((c < 0 ? -1 : 1) * ((a * q.qscale + q.qadd) >> q.qshift));
-                if (c != 0)
-                    atomicAdd(&s_nnz[b], 1);
+    for (int64_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
+        v4i hi, lo;
+        if (mode & DO_FWD) {
+            // F1 operand: tile row `col`, columns 16 half .. 16 half + 15
+            uint32_t d[8];
+            const int y = col;
+#pragma unroll
+            for (int piece = 0; piece < 16 / (N < 16 ? N : 16); piece++) {
+                constexpr int W = N < 16 ? N : 16;          // contiguous coefficients per piece
+                const int x = 16 * half + piece * W;
+                const int64_t blk = t * BPT + (y / N) * NB + x / N;
+                const int16_t *p = src + tile_off<N>(t, y, x);
+#pragma unroll
+                for (int k = 0; k < W / 2; k++)
+                    d[piece * (W / 2) + k] = blk < nblk ? reinterpret_cast<const uint32_t *>(p)[k] : 0u;
             }
-            L[e] = (int16_t)c;                                        // row-major [m][r]
-            int d = c;
-            if ((mode & DO_DEQUANT) && c != 0)
-                d = xme_clip3(-32768, 32767, (c * q.dscale + (1 << (q.dshift - 1))) >> q.dshift);
-            B[(b * N + r) * N + m] = (int16_t)d;
+            split_pairs(d, hi, lo);
+            v16i D = mm_data_a(hi, lo, f1b, splat16(acc_f1));
+            int v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                v[i] = D[i] >> s1;
+            split16(v, hi, lo);
+            D = mm_data_b(f2a, hi, lo, acc_f2);
+            // coefficients: row rho(half, i), column col
+            int c[16];
+            uint64_t nzmask[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                c[i] = (int)(int16_t)(D[i] >> s2);
+                if (mode & DO_QUANT) {
+                    const int a = c[i] < 0 ? -c[i] : c[i];
+                    const int m = (int)(((uint32_t)a * (uint32_t)q.qscale + (uint32_t)q.qadd) >> q.qshift);
+                    c[i] = (int)(int16_t)(c[i] < 0 ? -m : m);
+                    nzmask[i] = __builtin_amdgcn_ballot_w64(c[i] != 0);
+                }
+            }
+            // levels (or plain coefficients) out
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int y2 = rho(half, i);
+                const int64_t blk = t * BPT + (y2 / N) * NB + col / N;
+                if (blk < nblk)
+                    level[tile_off<N>(t, y2, col)] = (int16_t)c[i];
+            }
+            if ((mode & DO_QUANT) && nnz) {
+                // per-block nonzero counts from the ballots: bits 0-31 hold row
+                // rho(0, i), bits 32-63 row rho(1, i), bit = column
+                int mine = 0;
+#pragma unroll
+                for (int b = 0; b < BPT; b++) {
+                    const int rb = b / NB, cb = b % NB;
+                    const uint32_t cm = (N == 32 ? 0xffffffffu : ((1u << N) - 1u) << (cb * N));
+                    int cnt = 0;
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        if (rho(0, i) / N == rb)
+                            cnt += __builtin_popcount((uint32_t)nzmask[i] & cm);
+                        if (rho(1, i) / N == rb)
+                            cnt += __builtin_popcount((uint32_t)(nzmask[i] >> 32) & cm);
+                    }
+                    if (lane == b)
+                        mine = cnt;
+                }
+                if (lane < BPT && t * BPT + lane < nblk)
+                    nnz[t * BPT + lane] = mine;
+            }
+            if (mode & DO_INV) {
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    int d = c[i];
+                    if ((mode & DO_DEQUANT) && d != 0)
+                        d = xme_clip3(-32768, 32767, (d * q.dscale + (1 << (q.dshift - 1))) >> q.dshift);
+                    c[i] = d;
+                }
+                split16(c, hi, lo);
+            }
+        } else {
+            // inverse only: I1 operand lane row j = col, K bytes m = rho(half, t)
+            int v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int y = rho(half, i);
+                const int64_t blk = t * BPT + (y / N) * NB + col / N;
+                v[i] = blk < nblk ? src[tile_off<N>(t, y, col)] : 0;
+            }
+            split16(v, hi, lo);
         }
-        coefT = B;
-        __syncthreads();
-        for (int e = tid; e < nel / 8; e += 256)
-            reinterpret_cast<uint4 *>(level + base)[e] = reinterpret_cast<const uint4 *>(L)[e];
-        if (nnz && tid < nb)
-            nnz[blk0 + tid] = s_nnz[tid];
-    }
-    if (mode & DO_INV) {
-        int16_t *other = coefT == A ? B : A;
-        pass<N, true>(coefT, other, 5, -32768, 32767);               // -> other[b][m'][j] = tmp[j][m']
-        __syncthreads();
-        pass<N, true>(other, L, 12, -256, 255);                      // -> L[b][m2][x]: recon at row x, col m2
-        __syncthreads();
-        // L holds recon transposed: recon[b][x][m2] = L[b][m2][x]
-        for (int e = tid; e < nel; e += 256) {
-            const int b = e / (N * N), rem = e % (N * N), x = rem / N, m2 = rem % N;
-            recon[base + e] = L[(b * N + m2) * N + x];
+        if (mode & DO_INV) {
+            v16i D = mm_data_a(hi, lo, ib, splat16(acc_i1));
+            int v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                v[i] = xme_clip3(-32768, 32767, D[i] >> 5);
+            split16(v, hi, lo);
+            D = mm_data_a(hi, lo, ib, splat16(acc_i2));
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int y = rho(half, i);
+                const int64_t blk = t * BPT + (y / N) * NB + col / N;
+                if (blk < nblk)
+                    recon[tile_off<N>(t, y, col)] = (int16_t)xme_clip3(-256, 255, D[i] >> 12);
+            }
         }
     }
 }
@@ -171,26 +264,44 @@ hipError_t g_tab_err = hipSuccess;
 // the values of g_T4..g_T32 (transform.c:59-135); T_N[k][n] = T_32[k*32/N][n].
 void upload_tables()
 {
-    int16_t T32[32][32];
+    int T32[32][32];
     const double pi = 3.14159265358979323846;
     for (int k = 0; k < 32; k++)
         for (int n = 0; n < 32; n++)
-            T32[k][n] = (int16_t)(k == 0 ? 32 : std::lround(32.0 * std::sqrt(2.0) * std::cos((2 * n + 1) * k * pi / 64.0)));
-    uint32_t fwd[8 + 32 + 128 + 512], inv[8 + 32 + 128 + 512];
-    for (int n : { 4, 8, 16, 32 }) {
-        const int s = 32 / n, off = mat_off(n);
-        for (int k = 0; k < n; k++)
-            for (int j = 0; j < n; j += 2) {
-                const uint16_t a = (uint16_t)T32[k * s][j], b = (uint16_t)T32[k * s][j + 1];
-                fwd[off + k * n / 2 + j / 2] = (uint32_t)a | ((uint32_t)b << 16);
-                // inverse row m = k, pair (T[j][m], T[j+1][m])
-                const uint16_t c = (uint16_t)T32[j * s][k], d = (uint16_t)T32[(j + 1) * s][k];
-                inv[off + k * n / 2 + j / 2] = (uint32_t)c | ((uint32_t)d << 16);
+            T32[k][n] = k == 0 ? 32 : (int)std::lround(32.0 * std::sqrt(2.0) * std::cos((2 * n + 1) * k * pi / 64.0));
+    static TileConsts tc[4];
+    for (int l2 = 2; l2 <= 5; l2++) {
+        const int n = 1 << l2, s = 32 / n;
+        TileConsts &c = tc[l2 - 2];
+        // block-diagonal tile matrix
+        auto M = [&](int r, int q) { return r / n == q / n ? T32[(r % n) * s][q % n] : 0; };
+        for (int lane = 0; lane < 64; lane++) {
+            const int col = lane & 31, half = lane >> 5;
+            uint8_t f1b[16], f2a[16], ib[16];
+            for (int t = 0; t < 16; t++) {
+                f1b[t] = (uint8_t)(int8_t)M(col, 16 * half + t);
+                f2a[t] = (uint8_t)(int8_t)M(col, rho(half, t));
+                ib[t] = (uint8_t)(int8_t)M(rho(half, t), col);
             }
+            memcpy(c.f1b[lane], f1b, 16);
+            memcpy(c.f2a[lane], f2a, 16);
+            memcpy(c.ib[lane], ib, 16);
+            int rs = 0, cs = 0;
+            for (int k = 0; k < 32; k++) {
+                rs += M(col, k);
+                cs += M(k, col);
+            }
+            c.rowsum[lane] = 128 * rs;
+            c.colsum[lane] = 128 * cs;
+            for (int i = 0; i < 16; i++) {
+                int r = 0;
+                for (int k = 0; k < 32; k++)
+                    r += M(rho(half, i), k);
+                c.rowsum_rho[lane][i] = 128 * r;
+            }
+        }
     }
-    g_tab_err = hipMemcpyToSymbol(HIP_SYMBOL(c_fwd), fwd, sizeof(fwd));
-    if (g_tab_err == hipSuccess)
-        g_tab_err = hipMemcpyToSymbol(HIP_SYMBOL(c_inv), inv, sizeof(inv));
+    g_tab_err = hipMemcpyToSymbol(HIP_SYMBOL(g_tc), tc, sizeof(tc));
 }
 
 // Quantisation tables, source/common/quant.c:55-97.
@@ -232,23 +343,22 @@ int launch(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t *n
     std::call_once(g_tab_once, upload_tables);
     if (g_tab_err != hipSuccess)
         return xme_hip_fail(g_tab_err, "upload transform tables");
-    const int fshift1 = log2n(n) - 2;
+    // one wave per 32x32 tile of (32/n)^2 blocks; waves loop over tiles
+    const int bpt = (32 / n) * (32 / n);
+    const int64_t tiles = ((int64_t)nblk + bpt - 1) / bpt;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 256 * 16));
     switch (n) {
     case 4:
-        hipLaunchKernelGGL(residual_kernel<4>, dim3((nblk + Geo<4>::NB - 1) / Geo<4>::NB), dim3(256), 0, s, src,
-                           level, recon, nnz, nblk, q, mode, fshift1);
+        hipLaunchKernelGGL(residual_kernel<4>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
         break;
     case 8:
-        hipLaunchKernelGGL(residual_kernel<8>, dim3((nblk + Geo<8>::NB - 1) / Geo<8>::NB), dim3(256), 0, s, src,
-                           level, recon, nnz, nblk, q, mode, fshift1);
+        hipLaunchKernelGGL(residual_kernel<8>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
         break;
     case 16:
-        hipLaunchKernelGGL(residual_kernel<16>, dim3((nblk + Geo<16>::NB - 1) / Geo<16>::NB), dim3(256), 0, s, src,
-                           level, recon, nnz, nblk, q, mode, fshift1);
+        hipLaunchKernelGGL(residual_kernel<16>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
         break;
     default:
-        hipLaunchKernelGGL(residual_kernel<32>, dim3((nblk + Geo<32>::NB - 1) / Geo<32>::NB), dim3(256), 0, s, src,
-                           level, recon, nnz, nblk, q, mode, fshift1);
+        hipLaunchKernelGGL(residual_kernel<32>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
         break;
     }
     XME_HIP(hipGetLastError());
