@@ -1,13 +1,13 @@
 // Sub-pel refinement (me_subpel_refine, source/encoder/me.c:479-579) and the
 // store of the full-pel result (me.c:1169-1173) for gfx950.
 //
-// One wave per (PU, ref).  The candidates of a refinement stage (8 half-pel
-// points around the integer best, then 8 quarter-pel points around the
-// half-pel best) are independent, so a stage evaluates all of them at once:
-// lanes take (candidate, 4x4 tile) pairs, compute the tile's Hadamard sum
-// (or SAD) from the resident interpolated planes and add it to the
-// candidate's LDS counter; the stage then applies the reference's sequential
-// strict-'<' update in Spiral order.
+// One wave per (PU, ref).  The candidates of a refinement stage (the start
+// point and 8 half-pel points around the integer best, then 8 quarter-pel
+// points around the half-pel best) are independent, so a stage evaluates all
+// of them at once: lanes take (candidate, 4x4 tile) pairs, compute the tile's
+// Hadamard sum (or SAD) from the resident interpolated planes and add it to
+// the candidate's LDS counter; the stage then applies the reference's
+// sequential strict-'<' update in Spiral order.
 #include "xme_internal.h"
 
 namespace {
@@ -17,39 +17,39 @@ __constant__ int8_t c_spiral[9][2] = { { 0, 0 }, { 0, -1 }, { 0, 1 }, { -1, -1 }
 __constant__ int8_t c_spiral2[9][2] = { { 0, 0 }, { 0, -1 }, { -1, -1 }, { -1, 0 }, { -1, 1 },
                                         { 0, 1 }, { 1, 1 }, { 1, 0 }, { 1, -1 } };
 
+// Four bytes at any address.  The dword after the aligned one is read even
+// when the address is aligned; reference rows have XME_PAD bytes of padding
+// on the right, so it stays inside the plane.
 __device__ __forceinline__ uint32_t load_unaligned_dw(const uint8_t *p)
 {
     const uintptr_t a = reinterpret_cast<uintptr_t>(p);
     const uint32_t *q = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)(a & 3);
-    // the second dword holds needed bytes only when the address is unaligned
-    return sh ? __builtin_amdgcn_alignbyte(q[1], q[0], sh) : q[0];
+    return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
 __device__ __forceinline__ int bytei(uint32_t v, int i) { return (int)((v >> (8 * i)) & 0xFFu); }
 
-// Sum of |H4 D H4^T| over one 4x4 tile (before the final halving), or the SAD.
-__device__ __forceinline__ int tile_cost(const uint8_t *cur, int64_t i_cur, const uint8_t *ref, int64_t i_ref,
-                                         bool hadamard)
+// Sum of |H4 D H4^T| over one 4x4 tile (before the final halving), or the SAD,
+// of current rows c[0..3] against the reference tile at `ref`.
+__device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], const uint8_t *ref, int64_t i_ref, bool hadamard)
 {
-    int d[4][4];
+    uint32_t rr[4];
 #pragma unroll
-    for (int y = 0; y < 4; y++) {
-        const uint32_t c = *reinterpret_cast<const uint32_t *>(cur + y * i_cur);
-        const uint32_t r = load_unaligned_dw(ref + y * i_ref);
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-            d[y][x] = bytei(c, x) - bytei(r, x);
-    }
-    int s = 0;
+    for (int y = 0; y < 4; y++)
+        rr[y] = load_unaligned_dw(ref + y * i_ref);
     if (!hadamard) {
+        int s = 0;
 #pragma unroll
         for (int y = 0; y < 4; y++)
-#pragma unroll
-            for (int x = 0; x < 4; x++)
-                s += abs(d[y][x]);
+            s = (int)__builtin_amdgcn_sad_u8(cr[y], rr[y], (uint32_t)s);
         return s;
     }
+    int d[4][4];
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+            d[y][x] = bytei(cr[y], x) - bytei(rr[y], x);
 #pragma unroll
     for (int y = 0; y < 4; y++) {
         const int a = d[y][0] + d[y][1], b = d[y][0] - d[y][1];
@@ -59,6 +59,7 @@ __device__ __forceinline__ int tile_cost(const uint8_t *cur, int64_t i_cur, cons
         d[y][2] = a - c;
         d[y][3] = b - e;
     }
+    int s = 0;
 #pragma unroll
     for (int x = 0; x < 4; x++) {
         const int a = d[0][x] + d[1][x], b = d[0][x] - d[1][x];
@@ -79,44 +80,196 @@ struct RefineCtx {
     int lambda, pmx, pmy;
 };
 
-// Costs of n <= 8 quarter-pel candidates (ME_COST_QPEL, me.c:344-349).
-__device__ void eval_costs(const RefineCtx &c, const int (*mv)[2], int n, int *s_sum, int *out)
+// Makes this wave's LDS writes visible to its own later LDS reads.
+__device__ __forceinline__ void wave_sync()
 {
-    const int lane = threadIdx.x;
-    if (lane < 8)
-        s_sum[lane] = 0;
-    __syncthreads();
-    const int tw = c.w >> 2, ntiles = tw * (c.h >> 2);
-    for (int t = lane; t < n * ntiles; t += 64) {
-        const int k = t / ntiles, tile = t - k * ntiles;
-        const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
-        const int mx = mv[k][0], my = mv[k][1];
-        const uint8_t *plane = c.planes[((my & 3) << 2) + (mx & 3)];
-        const uint8_t *ref = plane + c.off + (int64_t)((my >> 2) + ty) * c.i_ref + (mx >> 2) + tx;
-        const int v = tile_cost(c.cur + ty * c.i_cur + tx, c.i_cur, ref, c.i_ref, c.hadamard);
-        atomicAdd(&s_sum[k], v);
-    }
-    __syncthreads();
-    for (int k = 0; k < n; k++) {
-        const int dist = c.hadamard ? (s_sum[k] >> 1) : s_sum[k];
-        out[k] = dist + xme_weighted_cost(c.lambda, xme_mvbits(mv[k][0] - c.pmx) + xme_mvbits(mv[k][1] - c.pmy));
-    }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ const uint8_t *cand_ref(const RefineCtx &c, int mx, int my, int ty, int tx)
+{
+    const uint8_t *plane = c.planes[((my & 3) << 2) + (mx & 3)];
+    return plane + c.off + (int64_t)((my >> 2) + ty) * c.i_ref + (mx >> 2) + tx;
 }
 
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 
-__global__ void __launch_bounds__(64) subpel_kernel(DevParams p, const FrameDesc *__restrict__ desc,
-                                                    const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
-                                                    const uint64_t *__restrict__ best, int n_inst)
+// Candidate k of a refinement stage around (bx, by) (quarter-pel units):
+// stage 0 is the start point (k < c0) and the half-pel Spiral points
+// (me.c:511-538), stage 1 the quarter-pel points with the PMVR adaptation
+// (me.c:546-573, pmvr_adapt_mv me.c:412-424).  *skip marks a point the
+// reference does not evaluate.
+struct Stage {
+    int bx, by, c0, step, stage;
+    bool pmvr;
+    int ctr_x, ctr_y;
+    const int8_t (*pat)[2];
+};
+
+__device__ __forceinline__ void cand_mv(const Stage &st, int k, int &mx, int &my, bool &skip)
 {
-    __shared__ int s_sum[8];
-    const int id = blockIdx.x;
-    if (id >= n_inst)
+    skip = false;
+    if (st.stage == 0) {
+        if (k < st.c0) {
+            mx = st.bx;
+            my = st.by;
+            return;
+        }
+        const int pos = 1 + (k - st.c0) * st.step;
+        mx = st.bx + 2 * st.pat[pos][0];
+        my = st.by + 2 * st.pat[pos][1];
         return;
-    const int lane = threadIdx.x;
-    const DevPu pu = pus[id / p.n_ref];
-    const int r = id % p.n_ref;
+    }
+    const int pos = 1 + k * st.step;
+    if (st.pmvr) {
+        if (iabs(st.bx - st.ctr_x) > 2 || iabs(st.by - st.ctr_y) > 2) {
+            mx = st.bx + st.pat[pos][0] * 2;
+            my = st.by + st.pat[pos][1] * 2;
+            skip = iabs(mx - st.ctr_x) <= 2 && iabs(my - st.ctr_y) <= 2;
+        } else {
+            mx = st.bx + st.pat[pos][0];
+            my = st.by + st.pat[pos][1];
+            skip = iabs(mx - st.ctr_x) > 2 || iabs(my - st.ctr_y) > 2;
+        }
+        return;
+    }
+    mx = st.bx + st.pat[pos][0];
+    my = st.by + st.pat[pos][1];
+}
+
+// Distortion sums (SATD before halving, or SAD) of the stage's n <= 9
+// candidates into s_sum[k].  With P = the PU's 4x4 tile count rounded up to a
+// power of two, a PU of at most 64 tiles gives each group of P lanes one
+// candidate at a time (a lane = a tile) and sums the group with lane
+// shuffles; a larger PU gives every lane P / 64 tiles of every candidate and
+// sums the whole wave.  The current rows of a lane's tile are loaded once.
+__device__ void stage_sums(const RefineCtx &c, const Stage &st, int n, int *s_sum)
+{
+    const int lane = threadIdx.x & 63;
+    const int tw = c.w >> 2, ntiles = tw * (c.h >> 2);
+    int P = 1;
+    while (P < ntiles)
+        P <<= 1;
+    if (P <= 64) {
+        const int groups = 64 / P;
+        const int g = lane / P, tile = lane % P;
+        const bool has_tile = tile < ntiles;
+        const int ty = has_tile ? (tile / tw) * 4 : 0, tx = has_tile ? (tile % tw) * 4 : 0;
+        uint32_t cr[4];
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+            cr[y] = has_tile ? *reinterpret_cast<const uint32_t *>(c.cur + (ty + y) * c.i_cur + tx) : 0u;
+        for (int k0 = 0; k0 < n; k0 += groups) {
+            const int k = k0 + g;
+            int v = 0;
+            if (has_tile && k < n) {
+                int mx, my;
+                bool skip;
+                cand_mv(st, k, mx, my, skip);
+                if (!skip)
+                    v = tile_cost(cr, cand_ref(c, mx, my, ty, tx), c.i_ref, c.hadamard);
+            }
+            for (int o = 1; o < P; o <<= 1)
+                v += __shfl_xor(v, o);
+            if (tile == 0 && k < n)
+                s_sum[k] = v;
+        }
+    } else {
+        int acc[9];
+        int cmx[9], cmy[9];
+        bool cskip[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            acc[k] = 0;
+            cand_mv(st, k < n ? k : 0, cmx[k], cmy[k], cskip[k]);
+        }
+        for (int tile = lane; tile < ntiles; tile += 64) {
+            const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
+            uint32_t cr[4];
+#pragma unroll
+            for (int y = 0; y < 4; y++)
+                cr[y] = *reinterpret_cast<const uint32_t *>(c.cur + (ty + y) * c.i_cur + tx);
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (k < n && !cskip[k])
+                    acc[k] += tile_cost(cr, cand_ref(c, cmx[k], cmy[k], ty, tx), c.i_ref, c.hadamard);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (k < n) {
+                int v = acc[k];
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1)
+                    v += __shfl_xor(v, o);
+                if (lane == 0)
+                    s_sum[k] = v;
+            }
+        }
+    }
+    wave_sync();
+}
+
+// Lane k < n: the full cost of candidate k (ME_COST_QPEL, me.c:344-349),
+// 0x7fffffff for other lanes and skipped points.
+__device__ __forceinline__ int stage_cost(const RefineCtx &c, const Stage &st, int n, const int *s_sum)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane >= n)
+        return 0x7fffffff;
+    int mx, my;
+    bool skip;
+    cand_mv(st, lane, mx, my, skip);
+    if (skip)
+        return 0x7fffffff;
+    const int dist = c.hadamard ? (s_sum[lane] >> 1) : s_sum[lane];
+    return dist + xme_weighted_cost(c.lambda, xme_mvbits(mx - c.pmx) + xme_mvbits(my - c.pmy));
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+__device__ __forceinline__ int pack_cost_idx(int cost, int k)
+{
+    // costs stay below 2^27 (a 64x64 SATD is below 2^22), 4 bits of order
+    return cost == 0x7fffffff ? 0x7fffffff : (cost << 4) | k;
+}
+
+constexpr int SP_WAVES = 4;
+
+// One wave per (PU, reference).  Waves walk the instances reference-major
+// (neighbouring PUs of one reference together) and each XCD takes a
+// contiguous range of workgroups, so the L2 of an XCD sees the plane lines of
+// neighbouring PUs back to back.
+//
+// The reference updates the best point sequentially with a strict '<' over
+// the points of a stage in Spiral order; that is the first minimum among the
+// points below the stage's starting cost, i.e. the wave minimum of
+// (cost << 4 | k) compared against the starting cost.
+__global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDesc *__restrict__ desc,
+                                                     const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
+                                                     const uint64_t *__restrict__ best, int n_inst)
+{
+    __shared__ int s_sum[SP_WAVES][16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // gridDim.x is a multiple of 8: workgroup b runs on XCD b % 8
+    const int per_xcd = gridDim.x / 8;
+    const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int logical = __builtin_amdgcn_readfirstlane(lb * SP_WAVES + wave);
+    if (logical >= n_inst)
+        return;
+    const int n_pu = p.n_pu;
+    const int r = logical / n_pu;
+    const int pu_i = logical - r * n_pu;
+    const int id = pu_i * p.n_ref + r;
+    int *sum = s_sum[wave];
+    const DevPu pu = pus[pu_i];
     const DevInst in = inst[id];
     const uint64_t key = best[id];
     xme_result res;
@@ -161,69 +314,51 @@ __global__ void __launch_bounds__(64) subpel_kernel(DevParams p, const FrameDesc
         c.lambda = p.lambda;
         c.pmx = pu.mvp_x;
         c.pmy = pu.mvp_y;
-        const int step = p.fast ? 2 : 1;
-        const int8_t(*pat)[2] = p.fast ? c_spiral2 : c_spiral;
-        int mv[8][2], cost[8];
-        if (c.hadamard) {                        // me.c:511-516
-            mv[0][0] = bx;
-            mv[0][1] = by;
-            eval_costs(c, mv, 1, s_sum, cost);
-            bcost = cost[0];
+        Stage st;
+        st.step = p.fast ? 2 : 1;
+        st.pat = p.fast ? c_spiral2 : c_spiral;
+        st.pmvr = p.pmvr != 0;
+        st.ctr_x = (pu.mvp_x >> 1) << 1;
+        st.ctr_y = (pu.mvp_y >> 1) << 1;
+        const int npts = p.fast ? 4 : 8;
+        // stage 0: the SATD start cost (me.c:511-516) and the half-pel points
+        // around the integer best (me.c:522-538)
+        st.stage = 0;
+        st.bx = bx;
+        st.by = by;
+        st.c0 = c.hadamard ? 1 : 0;
+        int n = st.c0 + npts;
+        stage_sums(c, st, n, sum);
+        int cost = stage_cost(c, st, n, sum);
+        if (c.hadamard)
+            bcost = __shfl(cost, 0);
+        uint32_t m = wave_min_u32(lane >= st.c0 ? (uint32_t)pack_cost_idx(cost, lane) : 0x7fffffffu);
+        if ((int)(m >> 4) < bcost && m != 0x7fffffffu) {
+            int mx, my;
+            bool skip;
+            cand_mv(st, (int)(m & 15), mx, my, skip);
+            bcost = (int)(m >> 4);
+            bx = mx;
+            by = my;
         }
-        // half-pel around the integer best (me.c:522-538)
-        int n = 0;
-        for (int pos = 1; pos < 9; pos += step, n++) {
-            mv[n][0] = bx + 2 * pat[pos][0];
-            mv[n][1] = by + 2 * pat[pos][1];
-        }
-        eval_costs(c, mv, n, s_sum, cost);
-        int nbx = bx, nby = by;
-        for (int k = 0; k < n; k++)
-            if (cost[k] < bcost) {
-                bcost = cost[k];
-                nbx = mv[k][0];
-                nby = mv[k][1];
-            }
-        bx = nbx;
-        by = nby;
         if (p.frac > 1) {                        // quarter-pel (me.c:546-573)
-            const int ctr_x = (pu.mvp_x >> 1) << 1, ctr_y = (pu.mvp_y >> 1) << 1;
-            n = 0;
-            for (int pos = 1; pos < 9; pos += step) {
+            st.stage = 1;
+            st.bx = bx;
+            st.by = by;
+            st.c0 = 0;
+            n = npts;
+            wave_sync();
+            stage_sums(c, st, n, sum);
+            cost = stage_cost(c, st, n, sum);
+            m = wave_min_u32((uint32_t)pack_cost_idx(cost, lane < 16 ? lane : 0));
+            if (m != 0x7fffffffu && (int)(m >> 4) < bcost) {
                 int mx, my;
-                if (p.pmvr) {                    // pmvr_adapt_mv (me.c:412-424)
-                    bool skip;
-                    if (iabs(bx - ctr_x) > 2 || iabs(by - ctr_y) > 2) {
-                        mx = bx + pat[pos][0] * 2;
-                        my = by + pat[pos][1] * 2;
-                        skip = iabs(mx - ctr_x) <= 2 && iabs(my - ctr_y) <= 2;
-                    } else {
-                        mx = bx + pat[pos][0];
-                        my = by + pat[pos][1];
-                        skip = iabs(mx - ctr_x) > 2 || iabs(my - ctr_y) > 2;
-                    }
-                    if (skip)
-                        continue;
-                } else {
-                    mx = bx + pat[pos][0];
-                    my = by + pat[pos][1];
-                }
-                mv[n][0] = mx;
-                mv[n][1] = my;
-                n++;
+                bool skip;
+                cand_mv(st, (int)(m & 15), mx, my, skip);
+                bcost = (int)(m >> 4);
+                bx = mx;
+                by = my;
             }
-            if (n > 0)
-                eval_costs(c, mv, n, s_sum, cost);
-            nbx = bx;
-            nby = by;
-            for (int k = 0; k < n; k++)
-                if (cost[k] < bcost) {
-                    bcost = cost[k];
-                    nbx = mv[k][0];
-                    nby = mv[k][1];
-                }
-            bx = nbx;
-            by = nby;
         }
     }
     if (lane == 0) {
@@ -242,7 +377,10 @@ int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *
 {
     if (n_inst <= 0)
         return XME_OK;
-    hipLaunchKernelGGL(subpel_kernel, dim3(n_inst), dim3(64), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
+    // a multiple of 8 workgroups (one range per XCD), SP_WAVES instances each
+    const int wgs = (n_inst + SP_WAVES - 1) / SP_WAVES;
+    const int grid = (wgs + 7) / 8 * 8;
+    hipLaunchKernelGGL(subpel_kernel, dim3(grid), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }
