@@ -241,14 +241,19 @@ __device__ __forceinline__ void fs_strip_aligned(const uint8_t *__restrict__ win
     }
 }
 
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+// Minimum over the 64 lanes, returned to every lane: DPP inside each row of
+// 16 lanes (quad swaps, then row rotations by 4 and 8), then the four rows.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint64_t t = __shfl_xor(v, o);
-        v = t < v ? t : v;
-    }
-    return v;
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false));  // row_ror:4
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return min(min(a, b), min(c, d));
 }
 
 // Process one item with K rows per lane.
@@ -260,7 +265,7 @@ struct TermState {
 };
 
 template <int K>
-__device__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, const FsItem &it, int inst_id,
+__device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, const FsItem &it, int inst_id,
                         const uint8_t *__restrict__ win, int pitch, int wx0, int wy0, const uint8_t *__restrict__ cur,
                         int cpitch, int cx0, int cy0, uint64_t *__restrict__ best, int32_t *__restrict__ cost_map,
                         uint32_t *__restrict__ term, TermState &ts)
@@ -369,12 +374,18 @@ __device__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, 
             }
         }
     }
-    uint64_t kbest = ~0ull;
-    if (bcost != 0xFFFFFFFFu)
-        kbest = xme_key(bcost, (uint32_t)(XME_ORDER_RASTER + (row + (bidx >> 2)) * R2 + col + (bidx & 3)));
-    kbest = wave_min_u64(kbest);
-    if (lane == 0 && kbest != ~0ull)
+    // wave minimum of (cost << 11 | order inside the band): the band's
+    // candidates in raster order are (local row) * 4 LC + local column < 256 K
+    // <= 2^11, and a cost is below 2^21 (SAD <= 64 * 64 * 255, MV cost < 2^14)
+    const int lrow = rg * K + (bidx >> 2), lcol = 4 * lc + (bidx & 3);
+    uint32_t lkey = bcost == 0xFFFFFFFFu ? 0xFFFFFFFFu : (bcost << 11) | (uint32_t)(lrow * 4 * LC + lcol);
+    lkey = wave_min_u32(lkey);
+    if (lkey != 0xFFFFFFFFu && lane == 0) {
+        const int lo = (int)(lkey & 2047u);
+        const int j = it.row0 + lo / (4 * LC), i = it.col0 + lo % (4 * LC);
+        const uint64_t kbest = xme_key(lkey >> 11, (uint32_t)(XME_ORDER_RASTER + j * R2 + i));
         atomicMin(reinterpret_cast<unsigned long long *>(best + inst_id), (unsigned long long)kbest);
+    }
     if (cost_map != nullptr) {       // diagnostic output: every raster cost, -1 where illegal
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -388,6 +399,174 @@ __device__ void fs_item(const DevParams &p, const DevPu &pu, const DevInst &in, 
                     const bool legal = xme_mv_legal(mw, in.omx + i - R, my);
                     const int cost = (int)(acc[k][d] + (__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16));
                     cost_map[(int64_t)inst_id * R2 * R2 + j * R2 + i] = legal ? cost : -1;
+                }
+            }
+        }
+    }
+}
+
+// Narrow PUs (8 pixels wide, at most 16 rows) carry little work per
+// candidate, so per-call costs dominate the generic path.  Here a lane takes
+// K = 16 candidate rows; the SAD and the MV cost of each candidate are summed
+// in the qsad's u16 lanes (8 * 16 * 255 + MV cost < 2^16), so the strip needs
+// no flush and the epilogue reads the u16 sums directly.  The packed MV costs
+// depend only on the lane, the item and the search centre; they are kept in
+// the wave's LDS term area (16 x 64 u64 = 8 KB, the size of the generic path's
+// term table) across the item's instances.
+
+template <int H, bool ALIGNED>
+__device__ __forceinline__ void narrow_strip(const uint8_t *__restrict__ win, int pitch, int base,
+                                             const uint8_t *__restrict__ cur, int cpitch,
+                                             uint64_t (&a16)[XME_NARROW_K])
+{
+    constexpr int K = XME_NARROW_K;
+    const uint32_t shift = (uint32_t)base & 3u;
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
+    const int pd = pitch >> 2;
+    uint64_t ring[K][2];
+    auto load_row = [&](int r, uint64_t (&dst)[2]) {
+        const uint32_t *src = w32 + r * pd;
+        uint32_t d0, d1, d2;
+        if (ALIGNED) {
+            d0 = src[0];
+            d1 = src[1];
+            d2 = src[2];
+        } else {
+            const uint32_t r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+            d0 = __builtin_amdgcn_alignbyte(r1, r0, shift);
+            d1 = __builtin_amdgcn_alignbyte(r2, r1, shift);
+            d2 = __builtin_amdgcn_alignbyte(r3, r2, shift);
+        }
+        dst[0] = ((uint64_t)d1 << 32) | d0;
+        dst[1] = ((uint64_t)d2 << 32) | d1;
+    };
+#pragma unroll
+    for (int k = 0; k < K - 1; k++)
+        load_row(k, ring[k]);
+#pragma unroll
+    for (int c = 0; c < H; c++) {
+        load_row(c + K - 1, ring[(c + K - 1) % K]);
+        const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
+        const uint32_t cw0 = cp[0], cw1 = cp[1];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(c + k) % K][0], cw0, a16[k]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(c + k) % K][1], cw1, a16[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int H>
+__device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &pu, const DevInst &in, const FsItem &it, int inst_id,
+                               const uint8_t *__restrict__ win, int pitch, int wx0, int wy0,
+                               const uint8_t *__restrict__ cur, int cpitch, int cx0, int cy0,
+                               uint64_t *__restrict__ best, int32_t *__restrict__ cost_map,
+                               uint32_t *__restrict__ term, TermState &ts)
+{
+    constexpr int K = XME_NARROW_K;
+    const int lane = threadIdx.x & 63;
+    const int LC = it.LC;
+    const int lc = lane % LC, rg = lane / LC;
+    const int R = p.R, R2 = 2 * R;
+    const int bx = pu.x + in.omx - R - wx0;
+    const int by = pu.y + in.omy - R - wy0;
+    const int col = it.col0 + 4 * lc;
+    const int row = it.row0 + rg * K;
+    const int lane_base = (by + row) * pitch + bx + col;
+    uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]
+    if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
+        // ME_COST_IPEL (me.c:187-192): (lambda * (bits(4 dx) + bits(4 dy))) >> 16
+        const uint32_t lam = (uint32_t)p.lambda;
+        int bits_x[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            bits_x[d] = xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int by_bits = xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y);
+            uint64_t v = 0;
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+                v |= (uint64_t)(__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << (16 * d);
+            init[k * 64 + lane] = v;
+        }
+        ts.ok = true;
+        ts.cx = in.omx;
+        ts.cy = in.omy;
+    }
+    uint64_t a16[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        a16[k] = init[k * 64 + lane];
+    const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
+    if ((lane_base & 3) == 0)      // uniform: col and the pitch are multiples of 4
+        narrow_strip<H, true>(win, pitch, lane_base, cblk, cpitch, a16);
+    else
+        narrow_strip<H, false>(win, pitch, lane_base, cblk, cpitch, a16);
+
+    // first strict minimum in raster order (COPY3_IF_LT, me.h:70-75): the lane's
+    // best as (cost << 6 | k * 4 + d), then the wave's as (cost << 12 | order in
+    // the band); band order < 4 LC * (64 / LC) * 16 = 4096
+    const MvWindow mw = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
+    const int col_end = it.col0 + it.ncols, row_end = it.row0 + it.nrows;
+    const int bmx0 = in.omx + it.col0 - R, bmy0 = in.omy + it.row0 - R;
+    const bool full = it.ncols == 4 * LC && it.nrows == (64 / LC) * K;
+    const bool fast = full && bmx0 >= mw.min_fpel[0] && bmx0 + it.ncols - 1 <= mw.max_fpel[0] &&
+                      bmy0 >= mw.min_fpel[1] && bmy0 + it.nrows - 1 <= mw.max_fpel[1];
+    uint32_t m = 0xFFFFFFFFu;
+    if (fast) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t lo = (uint32_t)a16[k], hi = (uint32_t)(a16[k] >> 32);
+            const uint32_t k0 = ((lo & 0xFFFFu) << 6) | (uint32_t)(k * 4 + 0);
+            const uint32_t k1 = ((lo >> 16) << 6) | (uint32_t)(k * 4 + 1);
+            const uint32_t k2 = ((hi & 0xFFFFu) << 6) | (uint32_t)(k * 4 + 2);
+            const uint32_t k3 = ((hi >> 16) << 6) | (uint32_t)(k * 4 + 3);
+            m = min(m, min(min(k0, k1), min(k2, k3)));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = row + k;
+            const int my = in.omy + j - R;
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int i = col + d;
+                const bool legal = i < col_end && j < row_end && xme_mv_legal(mw, in.omx + i - R, my);
+                const uint32_t v = (uint32_t)(a16[k] >> (16 * d)) & 0xFFFFu;
+                if (legal)
+                    m = min(m, (v << 6) | (uint32_t)(k * 4 + d));
+            }
+        }
+    }
+    uint32_t lkey = 0xFFFFFFFFu;
+    if (m != 0xFFFFFFFFu) {
+        const int idx = (int)(m & 63u);
+        const int lrow = rg * K + (idx >> 2), lcol = 4 * lc + (idx & 3);
+        lkey = ((m >> 6) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
+    }
+    lkey = wave_min_u32(lkey);
+    if (lkey != 0xFFFFFFFFu && lane == 0) {
+        const int lo = (int)(lkey & 4095u);
+        const int j = it.row0 + lo / (4 * LC), i = it.col0 + lo % (4 * LC);
+        const uint64_t kbest = xme_key(lkey >> 12, (uint32_t)(XME_ORDER_RASTER + j * R2 + i));
+        atomicMin(reinterpret_cast<unsigned long long *>(best + inst_id), (unsigned long long)kbest);
+    }
+    if (cost_map != nullptr) {       // diagnostic output: every raster cost, -1 where illegal
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = row + k;
+            const int my = in.omy + j - R;
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int i = col + d;
+                if (i < col_end && j < row_end) {
+                    const bool legal = xme_mv_legal(mw, in.omx + i - R, my);
+                    const int v = (int)((a16[k] >> (16 * d)) & 0xFFFFu);
+                    cost_map[(int64_t)inst_id * R2 * R2 + j * R2 + i] = legal ? v : -1;
                 }
             }
         }
@@ -444,8 +623,11 @@ __device__ __forceinline__ int window_pitch(int ww)
 // window and the current pixels); each of the 4 waves owns 8 * 4 * 64 dwords.
 __host__ __device__ inline int fs_term_offset(int staged_bytes) { return (staged_bytes + 15) & ~15; }
 
-// KMAX bounds the rows per lane the kernel is compiled for: the K = 4 variant
-// fits 128 VGPRs and runs 4 waves per SIMD, the K = 8 variant 2 waves.
+// KMAX selects the item paths the kernel is compiled with: 4 (K = 4 items;
+// fits 128 VGPRs, 4 waves per SIMD), 8 (K = 8 and K = 4 items, 2 waves per
+// SIMD) or XME_NARROW_K (the narrow-PU items only).  The narrow path lives in
+// its own variant because the register allocation of one kernel holding both
+// it and the K = 8 path spills.
 template <bool PERSISTENT, int KMAX>
 __global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                  const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
@@ -488,18 +670,43 @@ __global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p,
             TermState ts;
             ts.ok = false;
             ts.cx = ts.cy = 0;
+            // the item's instances (<= XME_MAX_GROUP) are fetched at once, one
+            // per lane, and handed to the wave one after the other
+            const int lane = threadIdx.x & 63;
+            int my_id = 0;
+            uint4 my_in = make_uint4(0, 0, 0, 0), my_pu = make_uint4(0, 0, 0, 0);
+            if (lane < it.count) {
+                my_id = item_insts[it.list + lane];
+                my_in = reinterpret_cast<const uint4 *>(inst)[my_id];
+                my_pu = reinterpret_cast<const uint4 *>(pus)[my_id / p.n_ref];
+            }
             for (int g = 0; g < it.count; g++) {
-                const int inst_id = __builtin_amdgcn_readfirstlane(item_insts[it.list + g]);
-                const DevInst in = inst[inst_id];
+                const int inst_id = __builtin_amdgcn_readlane(my_id, g);
+                const uint4 vin = make_uint4(__builtin_amdgcn_readlane(my_in.x, g), __builtin_amdgcn_readlane(my_in.y, g),
+                                             __builtin_amdgcn_readlane(my_in.z, g), __builtin_amdgcn_readlane(my_in.w, g));
+                const DevInst in = __builtin_bit_cast(DevInst, vin);
                 if (in.status)
                     continue;
-                const DevPu pu = pus[inst_id / p.n_ref];
-                if (KMAX == 8 && it.K == 8)
-                    fs_item<KMAX>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
-                                  const_cast<int32_t *>(cost_map), term, ts);
-                else
+                const uint4 vpu = make_uint4(__builtin_amdgcn_readlane(my_pu.x, g), __builtin_amdgcn_readlane(my_pu.y, g),
+                                             __builtin_amdgcn_readlane(my_pu.z, g), __builtin_amdgcn_readlane(my_pu.w, g));
+                const DevPu pu = __builtin_bit_cast(DevPu, vpu);
+                if constexpr (KMAX == XME_NARROW_K) {
+                    if (pu.h == 16)
+                        fs_item_narrow<16>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
+                                           best, const_cast<int32_t *>(cost_map), term, ts);
+                    else if (pu.h == 8)
+                        fs_item_narrow<8>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
+                                          best, const_cast<int32_t *>(cost_map), term, ts);
+                    else
+                        fs_item_narrow<4>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
+                                          best, const_cast<int32_t *>(cost_map), term, ts);
+                } else if (KMAX == 8 && it.K == 8) {
+                    fs_item<8>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
+                               const_cast<int32_t *>(cost_map), term, ts);
+                } else {
                     fs_item<4>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
                                const_cast<int32_t *>(cost_map), term, ts);
+                }
             }
         }
         __syncthreads();   // LDS is reused by the next task
@@ -527,8 +734,9 @@ int xme_fs_prepare(void)
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-        const void *ks[4] = { xme_fs_kernel_ptr(true, 4), xme_fs_kernel_ptr(true, 8), xme_fs_kernel_ptr(false, 4),
-                              xme_fs_kernel_ptr(false, 8) };
+        const void *ks[6] = { xme_fs_kernel_ptr(true, 4), xme_fs_kernel_ptr(true, 8), xme_fs_kernel_ptr(false, 4),
+                              xme_fs_kernel_ptr(false, 8), xme_fs_kernel_ptr(true, XME_NARROW_K),
+                              xme_fs_kernel_ptr(false, XME_NARROW_K) };
         for (const void *k : ks)
             if (err == hipSuccess)
                 err = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, xme_fs_max_lds());
@@ -567,6 +775,9 @@ int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pu
 
 const void *xme_fs_kernel_ptr(bool persistent, int kmax)
 {
+    if (kmax == XME_NARROW_K)
+        return persistent ? reinterpret_cast<const void *>(&fs_kernel<true, XME_NARROW_K>)
+                          : reinterpret_cast<const void *>(&fs_kernel<false, XME_NARROW_K>);
     if (persistent)
         return kmax == 8 ? reinterpret_cast<const void *>(&fs_kernel<true, 8>)
                          : reinterpret_cast<const void *>(&fs_kernel<true, 4>);

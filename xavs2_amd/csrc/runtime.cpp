@@ -145,6 +145,7 @@ int xme_part_index(int w, int h)
 }
 
 int xme_pu_w(int part) { return kPuW[part]; }
+bool xme_narrow_shape(int part) { return kPuW[part] == 8 && kPuH[part] <= 16; }
 int xme_pu_h(int part) { return kPuH[part]; }
 
 // Largest |v| of the reference's mvbits table (me.c:592-612).
@@ -165,6 +166,7 @@ struct DescSlot {
 struct LaunchGroup {       // plain path: one launch per PU shape
     int task_begin, task_end;
     size_t lds;
+    int variant;           // kernel variant: the plan's K bound, or XME_NARROW_K
 };
 
 struct xme_plan {
@@ -185,6 +187,11 @@ struct xme_plan {
     size_t lds_bytes = 0;
     int grid = 0;
     int grid_full = 0;         // occupancy-sized grid, before any cap
+    // batched plans: tasks [n_tasks - n_tasks_narrow, n_tasks) hold the
+    // narrow-PU items and run in a second persistent launch
+    int n_tasks_narrow = 0;
+    size_t lds_narrow = 0;
+    int grid_narrow = 0, grid_narrow_full = 0;
     int kmax = 8;              // rows-per-lane bound of the search kernel variant
     std::vector<LaunchGroup> groups;
     DescSlot slots[8];
@@ -258,13 +265,24 @@ static int window_pitch_host(int ww)
 
 // Items for a group of instances sharing PU shape and MVP: one item per band
 // of candidate rows/columns, covering every instance of the group.
+// Whether a PU shape's items go to the narrow-PU kernel variant.
+static bool uses_narrow(const DevParams &p, int part)
+{
+    return xme_narrow_shape(part) && xme_fs_kmax() == 8 && xme_tune_lookup(part, p.R).K == XME_NARROW_K;
+}
+
 static void items_for(const DevParams &p, const std::vector<DevPu> &pus, const std::vector<int> &group, int part,
                       int row_lo, int row_hi, std::vector<HostItem> &out)
 {
     const DevPu &pu = pus[group[0] / p.n_ref];
     TuneChoice tc = xme_tune_lookup(part, p.R);
-    const int Kt = std::min(tc.K, xme_fs_kmax());
-    const int K = (pu.h % Kt == 0) ? Kt : 4;
+    int K;
+    if (uses_narrow(p, part)) {
+        K = XME_NARROW_K;                         // the narrow-PU kernel variant
+    } else {
+        const int Kt = std::min(std::min(tc.K, 8), xme_fs_kmax());
+        K = (pu.h % Kt == 0) ? Kt : 4;
+    }
     const int LC = tc.LC;
     const int band_rows = (64 / LC) * K, band_cols = 4 * LC;
     const int R2 = 2 * p.R;
@@ -464,6 +482,8 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     }
 
     const size_t lds_cap = (size_t)xme_fs_max_lds();
+    // search windows are split until this many workgroups share a CU's LDS
+    const size_t lds_share = lds_cap / (size_t)xme_fs_wg_per_cu();
     std::vector<FsTask> tasks;
     std::vector<FsItem> items;
     std::vector<int32_t> item_insts;
@@ -493,63 +513,70 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     };
 
     if (mode == XME_PLAN_BATCHED) {
-        // group PUs by 64x64 CTU; split the candidate rows until the window fits
+        // group PUs by 64x64 CTU; split the candidate rows until the window fits.
+        // Pass 0 builds the generic tasks, pass 1 the narrow-PU tasks.
         std::map<std::pair<int, int>, std::vector<int>> ctus;
         for (int i = 0; i < n_pu; i++)
             ctus[{ pus[i].y >> 6, pus[i].x >> 6 }].push_back(i);
-        size_t lds_max = 0;
-        for (auto &kv : ctus) {
-            for (int r = 0; r < n_ref; r++) {
-                int splits = 1;
-                for (;;) {
-                    bool ok = true;
-                    std::vector<std::vector<HostItem>> groups(splits);
-                    const int rows_per = (R2 + splits - 1) / splits;
-                    std::vector<int> insts;
-                    for (int i : kv.second)
-                        insts.push_back(i * n_ref + r);
-                    const auto inst_groups = make_groups(pus, insts, n_ref, XME_MAX_GROUP);
-                    for (int s = 0; s < splits; s++) {
-                        for (const auto &ig : inst_groups)
-                            items_for(p, pus, ig, pus[ig[0] / n_ref].part, s * rows_per,
-                                      std::min(R2, (s + 1) * rows_per), groups[s]);
-                        FsTask t;
-                        // two workgroups of the search kernel share a CU
-                        if (!groups[s].empty() && make_task(p, pus, groups[s], r, t) > lds_cap / 2)
-                            ok = false;
+        size_t lds_pass[2] = { 0, 0 };
+        int begin_pass[2] = { 0, 0 };
+        for (int pass = 0; pass < 2; pass++) {
+            begin_pass[pass] = (int)tasks.size();
+            for (auto &kv : ctus) {
+                std::vector<int> members;
+                for (int i : kv.second)
+                    if (uses_narrow(p, pus[i].part) == (pass == 1))
+                        members.push_back(i);
+                if (members.empty())
+                    continue;
+                for (int r = 0; r < n_ref; r++) {
+                    int splits = 1;
+                    for (;;) {
+                        bool ok = true;
+                        std::vector<std::vector<HostItem>> groups(splits);
+                        const int rows_per = (R2 + splits - 1) / splits;
+                        std::vector<int> insts;
+                        for (int i : members)
+                            insts.push_back(i * n_ref + r);
+                        const auto inst_groups = make_groups(pus, insts, n_ref, XME_MAX_GROUP);
+                        for (int s = 0; s < splits; s++) {
+                            for (const auto &ig : inst_groups)
+                                items_for(p, pus, ig, pus[ig[0] / n_ref].part, s * rows_per,
+                                          std::min(R2, (s + 1) * rows_per), groups[s]);
+                            FsTask t;
+                            if (!groups[s].empty() && make_task(p, pus, groups[s], r, t) > lds_share)
+                                ok = false;
+                        }
+                        if (ok || splits >= R2 / 4) {
+                            for (auto &g : groups)
+                                if (!g.empty())
+                                    lds_pass[pass] = std::max(lds_pass[pass], emit_task(g, r, -1));
+                            break;
+                        }
+                        splits *= 2;
                     }
-                    if (ok || splits >= R2 / 4) {
-                        for (auto &g : groups)
-                            if (!g.empty())
-                                lds_max = std::max(lds_max, emit_task(g, r, -1));
-                        break;
-                    }
-                    splits *= 2;
                 }
             }
+            // longest tasks first (all CTU tasks have similar work; edges are smaller)
+            auto work = [&](const FsTask &t) {
+                int64_t w = 0;
+                for (int k = t.item_begin; k < t.item_end; k++)
+                    w += items[k].cost;
+                return w;
+            };
+            std::stable_sort(tasks.begin() + begin_pass[pass], tasks.end(),
+                             [&](const FsTask &a, const FsTask &b) { return work(a) > work(b); });
         }
-        pl->lds_bytes = lds_max;
-        // longest tasks first (all CTU tasks have similar work; edges are smaller)
-        std::vector<int> order(tasks.size());
-        for (size_t i = 0; i < order.size(); i++)
-            order[i] = (int)i;
-        auto work = [&](const FsTask &t) {
-            int64_t s = 0;
-            for (int k = t.item_begin; k < t.item_end; k++)
-                s += items[k].cost;
-            return s;
-        };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(tasks[a]) > work(tasks[b]); });
-        std::vector<FsTask> sorted;
-        for (int i : order)
-            sorted.push_back(tasks[i]);
-        tasks.swap(sorted);
+        pl->lds_bytes = lds_pass[0];
+        pl->lds_narrow = lds_pass[1];
+        pl->n_tasks_narrow = (int)tasks.size() - begin_pass[1];
     } else {
         // plain: every (PU, ref) is its own task; one launch per PU shape
         for (int part = 0; part < XME_NUM_PU; part++) {
             LaunchGroup g;
             g.task_begin = (int)tasks.size();
             g.lds = 0;
+            g.variant = uses_narrow(p, part) ? XME_NARROW_K : xme_fs_kmax();
             for (int i = 0; i < n_pu; i++) {
                 if (parts[i] != part)
                     continue;
@@ -563,7 +590,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
                             items_for(p, pus, std::vector<int>{ i * n_ref + r }, part, s * rows_per,
                                       std::min(R2, (s + 1) * rows_per), groups[s]);
                             FsTask t;
-                            if (make_task(p, pus, groups[s], r, t) > lds_cap / 2)
+                            if (make_task(p, pus, groups[s], r, t) > lds_share)
                                 ok = false;
                         }
                         if (ok || splits >= R2 / 4) {
@@ -637,15 +664,20 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
             plan_free(pl);
             return prc;
         }
-        int per_cu = 0;
-        hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, xme_fs_kernel_ptr(true, pl->kmax), 256,
-                                                                      pl->lds_bytes);
         int cus = 0;
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
-        if (oe != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        pl->grid_full = std::max(1, std::min(per_cu * cus, pl->n_tasks));
+        auto fill = [&](int variant, size_t lds, int n) {
+            int per_cu = 0;
+            hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, xme_fs_kernel_ptr(true, variant),
+                                                                          256, lds);
+            if (oe != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            return std::max(1, std::min(per_cu * cus, n));
+        };
+        pl->grid_full = fill(pl->kmax, pl->lds_bytes, pl->n_tasks - pl->n_tasks_narrow);
         pl->grid = pl->grid_full;
+        pl->grid_narrow_full = fill(XME_NARROW_K, pl->lds_narrow, pl->n_tasks_narrow);
+        pl->grid_narrow = pl->grid_narrow_full;
     }
     *out = pl;
     return XME_OK;
@@ -673,7 +705,8 @@ extern "C" int xme_plan_set_max_workgroups(xme_plan *pl, int max_wg)
     if (pl->mode != XME_PLAN_BATCHED)
         return XME_OK;
     const int g = max_wg == 0 ? pl->grid_full : std::max(1, std::min(max_wg, pl->grid_full));
-    if (g != pl->grid && pl->graph_exec) {
+    const int gn = max_wg == 0 ? pl->grid_narrow_full : std::max(1, std::min(max_wg, pl->grid_narrow_full));
+    if ((g != pl->grid || gn != pl->grid_narrow) && pl->graph_exec) {
         // the captured launch holds the old grid
         hipGraphExecDestroy(pl->graph_exec);
         hipGraphDestroy(pl->graph);
@@ -681,6 +714,7 @@ extern "C" int xme_plan_set_max_workgroups(xme_plan *pl, int max_wg)
         pl->graph = nullptr;
     }
     pl->grid = g;
+    pl->grid_narrow = gn;
     return XME_OK;
 }
 
@@ -748,23 +782,38 @@ static int fill_desc(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_r
     return XME_OK;
 }
 
+// The integer search: batched plans run the generic tasks and then the
+// narrow-PU tasks as two persistent launches (each with its own queue
+// counters); plain plans launch once per PU shape.
+static int launch_search(xme_plan *pl, hipStream_t s)
+{
+    int rc = XME_OK;
+    if (pl->mode == XME_PLAN_BATCHED) {
+        const int n_gen = pl->n_tasks - pl->n_tasks_narrow;
+        rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, n_gen, pl->d_items,
+                           pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, pl->kmax, s);
+        if (rc == XME_OK)
+            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + n_gen, pl->n_tasks_narrow,
+                               pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters + 4, pl->lds_narrow,
+                               pl->grid_narrow, true, XME_NARROW_K, s);
+        return rc;
+    }
+    for (const LaunchGroup &g : pl->groups) {
+        rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
+                           g.task_end - g.task_begin, pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters, g.lds, 0,
+                           false, g.variant, s);
+        if (rc)
+            return rc;
+    }
+    return XME_OK;
+}
+
 static int launch_all(xme_plan *pl, hipStream_t s)
 {
     int rc = xme_launch_mvc(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
     if (rc)
         return rc;
-    if (pl->mode == XME_PLAN_BATCHED) {
-        rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, pl->n_tasks, pl->d_items,
-                           pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, pl->kmax, s);
-    } else {
-        for (const LaunchGroup &g : pl->groups) {
-            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
-                               g.task_end - g.task_begin, pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters, g.lds, 0,
-                               false, pl->kmax, s);
-            if (rc)
-                return rc;
-        }
-    }
+    rc = launch_search(pl, s);
     if (rc)
         return rc;
     return xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
@@ -834,18 +883,7 @@ extern "C" int xme_me_time(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const
         if (rc)
             break;
         XME_HIP(hipEventRecord(ev[4 * r + 1], s));
-        if (pl->mode == XME_PLAN_BATCHED) {
-            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, pl->n_tasks, pl->d_items,
-                               pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, pl->kmax, s);
-        } else {
-            for (const LaunchGroup &g : pl->groups) {
-                rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,
-                                   g.task_end - g.task_begin, pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters,
-                                   g.lds, 0, false, pl->kmax, s);
-                if (rc)
-                    break;
-            }
-        }
+        rc = launch_search(pl, s);
         if (rc)
             break;
         XME_HIP(hipEventRecord(ev[4 * r + 2], s));

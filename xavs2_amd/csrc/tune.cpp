@@ -42,6 +42,16 @@ int xme_fs_kmax(void)
     return g_kmax;
 }
 
+int xme_fs_wg_per_cu(void)
+{
+    static const int v = [] {
+        const char *e = getenv("XME_FS_WG_PER_CU");
+        const int n = e ? atoi(e) : 2;
+        return n >= 1 && n <= 4 ? n : 2;
+    }();
+    return v;
+}
+
 void xme_fs_set_kmax(int kmax)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -53,7 +63,7 @@ TuneChoice xme_tune_default(int part, int R)
     TuneChoice c;
     // enough 4-column lanes to span the 2R candidate columns (at most a wave)
     c.LC = std::min(64, std::max(8, pow2_at_least((2 * R + 3) / 4)));
-    c.K = (xme_pu_h(part) % 8 == 0) ? 8 : 4;
+    c.K = xme_narrow_shape(part) ? 16 : (xme_pu_h(part) % 8 == 0) ? 8 : 4;
     return c;
 }
 
@@ -148,8 +158,10 @@ extern "C" int xme_tune(const xme_me_params *prm, const int *parts, int n_parts,
         if ((int)pus.size() > 8192)
             pus.resize(8192);
         std::vector<TuneChoice> cands;
-        for (int K : { 4, 8 }) {
-            if (ph % K)
+        for (int K : { 4, 8, 16 }) {
+            if (K == 16 && !xme_narrow_shape(part))
+                continue;
+            if (K != 16 && ph % K)
                 continue;
             for (int LC : { 8, 16, 32, 64 }) {
                 if (4 * LC > 2 * 2 * p.search_range + 3 && LC > 8)
@@ -194,8 +206,11 @@ extern "C" int xme_tune(const xme_me_params *prm, const int *parts, int n_parts,
 extern "C" int xme_tune_set(int part, int R, int k_rows, int lanes_per_row)
 {
     const bool lc_ok = lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == 64;
-    if (part < 0 || part >= XME_NUM_PU || R < 1 || R > 128 || (k_rows != 4 && k_rows != 8) || !lc_ok) {
-        xme_set_error("xme_tune_set: part 0..24, R 1..128, K 4 or 8, LC 8/16/32/64");
+    const bool k_ok = k_rows == 4 || k_rows == 8 || (k_rows == 16 && part >= 0 && part < XME_NUM_PU &&
+                                                        xme_narrow_shape(part));
+    if (part < 0 || part >= XME_NUM_PU || R < 1 || R > 128 || !k_ok || !lc_ok) {
+        xme_set_error("xme_tune_set: part 0..24, R 1..128, K 4 or 8 (16 for 8-wide shapes of <= 16 rows), "
+                      "LC 8/16/32/64");
         return XME_ERR_ARG;
     }
     xme_tune_store(part, R, TuneChoice{ k_rows, lanes_per_row });

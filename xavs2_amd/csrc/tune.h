@@ -3,7 +3,7 @@
 #pragma once
 
 struct TuneChoice {
-    int K;    // candidate rows per lane (4 or 8)
+    int K;    // candidate rows per lane (4 or 8; 16 for narrow shapes)
     int LC;   // lanes sharing one candidate row, 4 columns each
 };
 
@@ -15,9 +15,14 @@ void xme_tune_store(int part, int search_range, TuneChoice c);
 void xme_tune_force(const TuneChoice *c);
 
 int xme_part_index(int w, int h);
+// 8 pixels wide and at most 16 rows: eligible for 16 candidate rows per lane.
+bool xme_narrow_shape(int part);
 int xme_pu_w(int part);
 int xme_pu_h(int part);
 const void *xme_fs_kernel_ptr(bool persistent, int kmax);
 // Rows-per-lane bound of the search kernel used by new plans (4 or 8).
 int xme_fs_kmax(void);
 void xme_fs_set_kmax(int kmax);
+// Search workgroups that share a CU's LDS (windows are split to fit), 1..4;
+// XME_FS_WG_PER_CU overrides the default of 2.
+int xme_fs_wg_per_cu(void);

@@ -47,6 +47,8 @@ struct DevInst {
     int16_t pad;
 };
 
+static_assert(sizeof(DevPu) == 16 && sizeof(DevInst) == 16, "the search kernel moves these as uint4");
+
 // A workgroup task of the integer search: a set of items that share one
 // reference window staged in LDS.
 struct FsTask {
@@ -159,6 +161,9 @@ int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pu
                   const FsTask *d_tasks, int n_tasks, const FsItem *d_items, const int32_t *d_item_insts,
                   uint64_t *d_best, uint32_t *d_counters, size_t lds_bytes, int grid, bool persistent, int kmax,
                   hipStream_t s);
+// Rows per lane of the narrow-PU search path (8-pixel-wide PUs of <= 16 rows),
+// which has a kernel variant of its own.
+#define XME_NARROW_K 16
 // LDS bytes of a search task beyond its staged window and current pixels
 // (per-wave MV-cost term tables of 4 waves x 8 x 4 x 64 dwords).
 #define XME_FS_TERM_BYTES (4 * 8 * 4 * 64 * 4)
