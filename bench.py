@@ -29,14 +29,14 @@ sys.path.insert(0, ROOT)
 METRIC = "ME cand-evals/s + roofline frac, batched vs plain; p50/p99 frame-ME latency"
 VALU_PEAK_TABSDIFF = 256 * 4 * 16 * 2.4e9 * 4 / 1e12     # v_sad_u8 / v_qsad: 157.3 T absdiff/s
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch of the persistent search kernel on the default
-# workload (3840x2160, 4 refs, R=64, shipped tile table), measured with
+# HBM bytes per search step (the two persistent search dispatches) on the
+# default workload (3840x2160, 4 refs, R=64, shipped tile table), measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
 # reading for a 256 MiB copy kernel (FETCH x2.000, WRITE x1.000):
-# 149,441.3 KiB fetched -> 306.0 MB, 54,008.1 KiB written -> 55.3 MB.
-# The counters cannot be read from inside this process, so the figure is quoted
-# for the workload it was measured on and null for any other.
-MEASURED_TRAFFIC = {(3840, 2160, 4, 64): 149441.3 * 1024 * 2.0 + 54008.1 * 1024 * 1.0}
+# (245,592.1 + 187,595.6) KiB fetched -> 887.2 MB, (15,948.1 + 19,270.3) KiB
+# written -> 36.1 MB.  The counters cannot be read from inside this process, so
+# the figure is quoted for the workload it was measured on and null for any other.
+MEASURED_TRAFFIC = {(3840, 2160, 4, 64): (245592.1 + 187595.6) * 1024 * 2.0 + (15948.1 + 19270.3) * 1024 * 1.0}
 
 
 def log(msg):
@@ -451,8 +451,8 @@ def main():
             "unit": "Tabsdiff/s",
             "frac": achieved / VALU_PEAK_TABSDIFF,
             "traffic": MEASURED_TRAFFIC.get((w, h, n_ref, R)) if not (args.no_tune or args.retune) else None,
-            "traffic_unit": "HBM bytes per search-kernel dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "kernel": "fs_kernel<persistent> (integer full search)",
+            "traffic_unit": "HBM bytes per search step (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "kernel": "integer full search: fs_kernel<true,8> + fs_kernel<true,16> (narrow PUs), both persistent",
             "kernel_ms": search_ms,
         },
         "plain": {"value": plain_value, "ms_per_step": plain_ms, "results_equal": plain_equal},
