@@ -121,21 +121,28 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
     const uint32_t shift = (uint32_t)base & 3u;
     const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
     const int pd = pitch >> 2;           // pitch in dwords
-    uint32_t ring[K][ND + 1];
+    // rows are kept as the qsad's 8-byte operands (pairs of realigned dwords):
+    // a ring of 32-bit words read in overlapping pairs is merged by the
+    // compiler into misaligned 64-bit accesses and ends up in scratch
+    uint64_t ring[K][ND];
     uint64_t a16[K];
 #pragma unroll
     for (int k = 0; k < K; k++)
         a16[k] = 0;
 
-    auto load_row = [&](int r, uint32_t (&dst)[ND + 1]) {
+    auto load_row = [&](int r, uint64_t (&dst)[ND]) {
         uint32_t raw[ND + 2];
         const uint32_t *src = w32 + r * pd;
 #pragma unroll
         for (int t = 0; t < ND + 2; t++)
             raw[t] = src[t];
+        uint32_t al[ND + 1];
 #pragma unroll
         for (int t = 0; t < ND + 1; t++)
-            dst[t] = align_dw(raw[t + 1], raw[t], shift);
+            al[t] = align_dw(raw[t + 1], raw[t], shift);
+#pragma unroll
+        for (int t = 0; t < ND; t++)
+            dst[t] = ((uint64_t)al[t + 1] << 32) | al[t];
     };
 
 #pragma unroll
@@ -158,10 +165,8 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
 #pragma unroll
             for (int t = 0; t < ND; t++) {
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const uint32_t(&rr)[ND + 1] = ring[(cc + k) % K];
-                    a16[k] = qsad(rr[t], rr[t + 1], cw[t], a16[k]);
-                }
+                for (int k = 0; k < K; k++)
+                    a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(cc + k) % K][t], cw[t], a16[k]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
