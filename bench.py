@@ -176,7 +176,7 @@ def transform_leg(X, blocks_total, reps, seed=91):
 SERVE_STREAMS = [(1280, 720)] * 6 + [(1920, 1080)] * 6 + [(3840, 2160)] * 4
 
 
-def serving_leg(X, seconds, loads, slots, n_cycle, share=0, seed=77):
+def serving_leg(X, seconds, loads, slots, n_cycle, share=0, split=0, seed=77):
     """16 concurrent encodes (6 x 720p, 6 x 1080p, 4 x 2160p), each a moving
     smooth+noise sequence (pan <= 2 px/frame, local block MVs in [-32, 32]^2),
     frames arriving as Poisson processes at 30-60 fps per stream (scaled by the
@@ -190,7 +190,8 @@ def serving_leg(X, seconds, loads, slots, n_cycle, share=0, seed=77):
     fps = rng.uniform(30.0, 60.0, len(SERVE_STREAMS))
     log(f"serving: generating {len(SERVE_STREAMS)} sequences of {n_cycle} frames")
     seqs = [synth.moving_sequence(w, h, n_cycle, seed=seed + 10 * i) for i, (w, h) in enumerate(SERVE_STREAMS)]
-    srv = X.Server(64 << 30, max_inflight=slots, search_range=64, n_ref=2, workgroup_share_pct=share)
+    srv = X.Server(64 << 30, max_inflight=slots, search_range=64, n_ref=2, workgroup_share_pct=share,
+                   launch_split_absdiff=split)
     sids = []
     for i, (w, h) in enumerate(SERVE_STREAMS):
         pus = [(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in synth.mixed_pus(w, h, seed + 10 * i + 1)]
@@ -274,7 +275,8 @@ def serving_leg(X, seconds, loads, slots, n_cycle, share=0, seed=77):
     return {
         "workload": "16 concurrent encodes (6x1280x720, 6x1920x1080, 4x3840x2160), Poisson arrivals at 30-60 fps "
                     "per stream x load, 2-ref ME over mixed PUs, R=64, 64 GB budget, "
-                    f"{slots} device slots, search launches capped at {share or 100} % of the device, "
+                    f"{slots} device slots, search launches capped at {share or 100} % of the device and "
+                    f"split at {'8e9' if split == 0 else ('none' if split < 0 else f'{split:.0e}')} absdiffs, "
                     f"{seconds:.0f} s per load",
         "stream_fps": [round(float(f), 2) for f in fps],
         "resident_bytes": int(resident),
@@ -308,10 +310,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes, no table, no tuning")
     ap.add_argument("--retune", action="store_true", help="ignore the shipped tile table and tune every shape")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
-    ap.add_argument("--serve-loads", default="0.5,0.75,1.0")
+    ap.add_argument("--serve-seconds", type=float, default=2.5)
+    ap.add_argument("--serve-loads", default="0.1,0.2,0.4,0.7,1.0")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-frames", type=int, default=4, help="distinct frames per stream, cycled")
+    ap.add_argument("--serve-split", type=float, default=0,
+                    help="search work per launch in the serving leg, absdiffs (0: server default, <0: no split)")
     ap.add_argument("--serve-share", type=int, default=0,
                     help="%% of a full-device search launch per frame in the serving leg (0 = 100)")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
@@ -412,7 +416,8 @@ def main():
     serving = None
     if not args.no_serve:
         loads = [float(v) for v in args.serve_loads.split(",") if v]
-        serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share)
+        serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share,
+                              int(args.serve_split))
 
     cpu = None
     if not args.no_cpu:

@@ -161,6 +161,10 @@ int xme_plan_info(const xme_plan *plan, int64_t *n_inst, int64_t *n_candidates, 
  * frames share the compute units instead of queueing behind each other. */
 int xme_plan_set_max_workgroups(xme_plan *plan, int max_workgroups);
 int xme_plan_workgroups(const xme_plan *plan, int *workgroups, int *device_fill);
+/* Splits the batched plan's search into launches of at most this many
+ * byte absdiffs each (0 = one launch per kernel variant), so that the kernels
+ * of other frames can start between them. */
+int xme_plan_set_launch_split(xme_plan *plan, int64_t max_absdiff_per_launch);
 
 /* Run the search for one frame.  d_cur: current luma plane origin (no padding
  * needed).  refs: host array of n_ref descriptors.  d_results: n_pu * n_ref
@@ -251,6 +255,7 @@ typedef struct {
     int32_t use_fast_sub_me;
     int32_t enable_pmvr;
     int32_t workgroup_share_pct;     /* % of a full-device search launch per frame (1..100; 0 = 100) */
+    int64_t launch_split_absdiff;    /* search work per launch (0: 8e9; < 0: no split) */
 } xme_server_config;
 typedef struct {
     int64_t submitted, completed, rejected;

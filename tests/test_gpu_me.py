@@ -171,6 +171,24 @@ def test_plain_equals_batched_and_graph_replays():
         assert np.array_equal(plain.fetch(), exp), f"eager run after replay {rep}"
 
 
+def test_split_launches_and_workgroup_cap_keep_results():
+    """The search split into many small launches, and with a capped grid, gives
+    the same results, also over graph replays (the captured graph is rebuilt)."""
+    w, h = 256, 192
+    pus = [(x, y, pw, ph, int(v) % 9 - 4, int(v) % 7 - 3)
+           for v, (x, y, pw, ph) in enumerate(synth.mixed_pus(w, h, 5))]
+    plan, dcur, drefs, exp = run_case(w, h, pus, 16, n_ref=2, seed=23)
+    s = X.Stream()
+    for split, cap in ((2_000_000, 0), (50_000_000, 7), (0, 0)):
+        X.check(X.lib().xme_plan_set_launch_split(plan.h, split), "split")
+        X.check(X.lib().xme_plan_set_max_workgroups(plan.h, cap), "cap")
+        for rep in range(3):
+            plan.results.zero(s.h)
+            plan.run_graph(dcur, drefs, s.h)
+            s.sync()
+            assert np.array_equal(plan.fetch(), exp), (split, cap, rep)
+
+
 def test_no_legal_candidate_status():
     w, h = 64, 64
     pus = [(16, 16, 16, 16, 0, 0)]

@@ -192,6 +192,10 @@ struct xme_plan {
     int n_tasks_narrow = 0;
     size_t lds_narrow = 0;
     int grid_narrow = 0, grid_narrow_full = 0;
+    // absdiffs per task (upper bound: every candidate of every item) and the
+    // launches the batched search is split into: {first task, task count}
+    std::vector<int64_t> task_work;
+    std::vector<std::pair<int, int>> chunks_gen, chunks_nar;
     int kmax = 8;              // rows-per-lane bound of the search kernel variant
     std::vector<LaunchGroup> groups;
     DescSlot slots[8];
@@ -265,6 +269,30 @@ static int window_pitch_host(int ww)
 
 // Items for a group of instances sharing PU shape and MVP: one item per band
 // of candidate rows/columns, covering every instance of the group.
+// Splits the batched search into launches of at most max_work absdiffs each
+// (0: one launch per kernel variant).  Tasks keep their order.
+static void split_launches(xme_plan *pl, int64_t max_work)
+{
+    const int n_gen = pl->n_tasks - pl->n_tasks_narrow;
+    auto split = [&](int b, int e, std::vector<std::pair<int, int>> &out) {
+        out.clear();
+        int first = b;
+        int64_t sum = 0;
+        for (int t = b; t < e; t++) {
+            sum += pl->task_work[t];
+            if (max_work > 0 && sum >= max_work && t + 1 < e) {
+                out.push_back({ first, t + 1 - first });
+                first = t + 1;
+                sum = 0;
+            }
+        }
+        if (e > first)
+            out.push_back({ first, e - first });
+    };
+    split(0, n_gen, pl->chunks_gen);
+    split(n_gen, pl->n_tasks, pl->chunks_nar);
+}
+
 // Whether a PU shape's items go to the narrow-PU kernel variant.
 static bool uses_narrow(const DevParams &p, int part)
 {
@@ -609,6 +637,17 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         }
     }
     pl->n_tasks = (int)tasks.size();
+    pl->task_work.resize(tasks.size());
+    for (size_t ti = 0; ti < tasks.size(); ti++) {
+        int64_t w = 0;
+        for (int k = tasks[ti].item_begin; k < tasks[ti].item_end; k++) {
+            const FsItem &it = items[k];
+            const DevPu &pu = pus[item_insts[it.list] / n_ref];
+            w += (int64_t)it.count * pu.w * pu.h * it.nrows * it.ncols;
+        }
+        pl->task_work[ti] = w;
+    }
+    split_launches(pl, 0);
 
     for (const FsTask &t : tasks) {
         size_t need = (((size_t)window_pitch_host(t.ww) * t.wh + (size_t)t.cw * t.ch + 15) & ~(size_t)15) +
@@ -718,6 +757,25 @@ extern "C" int xme_plan_set_max_workgroups(xme_plan *pl, int max_wg)
     return XME_OK;
 }
 
+extern "C" int xme_plan_set_launch_split(xme_plan *pl, int64_t max_absdiff_per_launch)
+{
+    if (!pl || max_absdiff_per_launch < 0) {
+        xme_set_error("xme_plan_set_launch_split: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (pl->mode != XME_PLAN_BATCHED)
+        return XME_OK;
+    split_launches(pl, max_absdiff_per_launch);
+    if (pl->graph_exec) {
+        // the captured graph holds the old launches
+        hipGraphExecDestroy(pl->graph_exec);
+        hipGraphDestroy(pl->graph);
+        pl->graph_exec = nullptr;
+        pl->graph = nullptr;
+    }
+    return XME_OK;
+}
+
 extern "C" int xme_plan_workgroups(const xme_plan *pl, int *grid, int *grid_full)
 {
     if (!pl)
@@ -789,14 +847,23 @@ static int launch_search(xme_plan *pl, hipStream_t s)
 {
     int rc = XME_OK;
     if (pl->mode == XME_PLAN_BATCHED) {
-        const int n_gen = pl->n_tasks - pl->n_tasks_narrow;
-        rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks, n_gen, pl->d_items,
-                           pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes, pl->grid, true, pl->kmax, s);
-        if (rc == XME_OK)
-            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + n_gen, pl->n_tasks_narrow,
-                               pl->d_items, pl->d_item_insts, pl->d_best, pl->d_counters + 4, pl->lds_narrow,
-                               pl->grid_narrow, true, XME_NARROW_K, s);
-        return rc;
+        // launches of one variant run in stream order and share its queue
+        // counters (the last workgroup of a launch resets them)
+        for (const auto &c : pl->chunks_gen) {
+            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + c.first, c.second, pl->d_items,
+                               pl->d_item_insts, pl->d_best, pl->d_counters, pl->lds_bytes,
+                               std::min(pl->grid, c.second), true, pl->kmax, s);
+            if (rc)
+                return rc;
+        }
+        for (const auto &c : pl->chunks_nar) {
+            rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + c.first, c.second, pl->d_items,
+                               pl->d_item_insts, pl->d_best, pl->d_counters + 4, pl->lds_narrow,
+                               std::min(pl->grid_narrow, c.second), true, XME_NARROW_K, s);
+            if (rc)
+                return rc;
+        }
+        return XME_OK;
     }
     for (const LaunchGroup &g : pl->groups) {
         rc = xme_launch_fs(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_tasks + g.task_begin,

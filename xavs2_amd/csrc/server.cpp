@@ -445,6 +445,8 @@ extern "C" int xme_server_open_stream(xme_server *S, int w, int h, const xme_pu 
         }
         st->plans.push_back(p);
         st->resident += (int64_t)xme_plan_device_bytes(p);
+        const int64_t split = S->cfg.launch_split_absdiff;
+        xme_plan_set_launch_split(p, split == 0 ? (int64_t)8000000000LL : split < 0 ? 0 : split);
         const int share = S->cfg.workgroup_share_pct;
         if (share > 0 && share < 100) {
             int g = 0, full = 0;
