@@ -338,6 +338,17 @@ def main():
         copy_probe(X, args.copy_probe)
         return
 
+    # the serving leg runs first: its device slots are the process's first HIP
+    # streams, so each gets a hardware queue of its own (GPU_MAX_HW_QUEUES = 4)
+    serving = None
+    if not args.no_serve:
+        if not args.no_tune:
+            from xavs2_amd import tiles as T
+            T.install(X.me_params(1920, 1080, 64, 4 * 65536), list(range(25)), retune=args.retune)
+        loads = [float(v) for v in args.serve_loads.split(",") if v]
+        serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share,
+                              int(args.serve_split))
+
     w, h, n_ref, R = args.width, args.height, args.refs, args.range
     lam = 4 * 65536
     log(f"generating {w}x{h} inputs, {n_ref} refs")
@@ -413,12 +424,6 @@ def main():
     if not args.no_transform:
         log(f"residual transform leg: {args.transform_blocks} blocks")
         transform = transform_leg(X, args.transform_blocks, reps=5)
-    serving = None
-    if not args.no_serve:
-        loads = [float(v) for v in args.serve_loads.split(",") if v]
-        serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share,
-                              int(args.serve_split))
-
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
