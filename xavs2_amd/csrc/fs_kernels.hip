@@ -25,15 +25,21 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // Candidate-list pass: one wave per (PU, ref).
-__global__ void __launch_bounds__(64) mvc_kernel(DevParams p, const FrameDesc *__restrict__ desc,
-                                                 const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
-                                                 uint64_t *__restrict__ best, int n_inst)
+// One wave per (PU, reference), four per workgroup; workgroup b runs on XCD
+// b % 8, and the instances are walked reference-major in contiguous ranges per
+// XCD so that neighbouring PUs of one reference share an L2.
+__global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *__restrict__ desc,
+                                                  const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
+                                                  uint64_t *__restrict__ best, int n_inst)
 {
-    const int id = blockIdx.x;
-    if (id >= n_inst)
+    const int per_xcd = gridDim.x / 8;
+    const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int logical = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
+    if (logical >= n_inst)
         return;
-    const int lane = threadIdx.x;
-    const int pu_i = id / p.n_ref, r = id % p.n_ref;
+    const int lane = threadIdx.x & 63;
+    const int r = logical / p.n_pu, pu_i = logical - r * p.n_pu;
+    const int id = pu_i * p.n_ref + r;
     const DevPu pu = pus[pu_i];
     const MvWindow win = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
     // add_one_mv_candidate (md_inter.c:840-865) for the MVP and (0,0)
@@ -60,15 +66,22 @@ __global__ void __launch_bounds__(64) mvc_kernel(DevParams p, const FrameDesc *_
         const int mx = cand[c][0], my = cand[c][1];
         if (!xme_mv_legal(win, mx, my))
             continue;
-        int sad = 0;
+        // 4 pixels per lane step: the current rows are dword aligned (PU x and
+        // the stride are multiples of 4); reference dwords are realigned, the
+        // extra dword read stays inside the plane's padding
+        uint32_t sad = 0;
         const uint8_t *rp = ref + (int64_t)my * i_ref + mx;
-        for (int t = lane; t < pu.w * pu.h; t += 64) {
-            const int yy = t / pu.w, xx = t % pu.w;
-            sad += abs((int)cur[(int64_t)yy * desc->i_cur + xx] - (int)rp[(int64_t)yy * i_ref + xx]);
+        const int gw = pu.w >> 2;
+        for (int t = lane; t < gw * pu.h; t += 64) {
+            const int yy = t / gw, xx = (t - yy * gw) * 4;
+            const uint32_t c4 = *reinterpret_cast<const uint32_t *>(cur + (int64_t)yy * desc->i_cur + xx);
+            const uintptr_t a = reinterpret_cast<uintptr_t>(rp + (int64_t)yy * i_ref + xx);
+            const uint32_t *q4 = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+            sad = __builtin_amdgcn_sad_u8(c4, __builtin_amdgcn_alignbyte(q4[1], q4[0], (uint32_t)(a & 3)), sad);
         }
         for (int o = 32; o > 0; o >>= 1)
-            sad += __shfl_xor(sad, o);
-        const int cost = sad + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+            sad += (uint32_t)__shfl_xor((int)sad, o);
+        const int cost = (int)sad + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
         const uint64_t k = xme_key((uint32_t)cost, (uint32_t)c);
         if (k < key) {
             key = k;
@@ -756,7 +769,9 @@ int xme_launch_mvc(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_p
 {
     if (n_inst <= 0)
         return XME_OK;
-    hipLaunchKernelGGL(mvc_kernel, dim3(n_inst), dim3(64), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
+    // a multiple of 8 workgroups (one range per XCD), 4 instances each
+    const int grid = ((n_inst + 3) / 4 + 7) / 8 * 8;
+    hipLaunchKernelGGL(mvc_kernel, dim3(grid), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }
