@@ -196,8 +196,9 @@ int xme_copy_kernel(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int xme_tune(const xme_me_params *params, const int *parts, int n_parts, void *stream);
 int xme_tune_get(int part, int search_range, int *k_rows, int *lanes_per_row);
 /* Installs a choice measured earlier (a shipped tuning table); K rows per lane
- * is 4 or 8 (8 falls back to 4 for PU heights it does not divide), LC is
- * 8, 16, 32 or 64 lanes per candidate row. */
+ * is 4 or 8 (8 falls back to 4 for PU heights it does not divide), or 16 for
+ * PUs 8 or 16 pixels wide and at most 16 rows high (the short-PU kernel);
+ * LC is 8, 16, 32 or 64 lanes per candidate row. */
 int xme_tune_set(int part, int search_range, int k_rows, int lanes_per_row);
 
 /* ---- 4. reference planes, MC and the residual path ---------------------- */

@@ -20,6 +20,7 @@
 #include "tune.h"
 
 #include <mutex>
+#include <utility>
 
 namespace {
 
@@ -125,6 +126,14 @@ __device__ __forceinline__ uint64_t qsad(uint32_t lo, uint32_t hi, uint32_t c, u
 //   win  : LDS window, `pitch` bytes per row; lane reference row r starts at
 //          byte offset base + r * pitch (base may be unaligned: `shift`)
 //   cur  : LDS current pixels of this strip, `cpitch` bytes per row (4-aligned)
+// Window rows live in a ring of K rows kept as the qsad's 8-byte operands
+// (pairs of realigned dwords: a ring of 32-bit words read in overlapping pairs
+// is merged by the compiler into misaligned 64-bit accesses and ends up in
+// scratch).  At current row c, window row c + K - 1 enters the slot freed by
+// row c - 1; it is first read by the last qsad of the first column group.
+// Current rows are double-buffered: row c + 1 is loaded while row c is
+// summed (H is a multiple of K, which is even, so c and cc share parity; the
+// load past the last row rereads row H - 1).
 template <int K, int SW>
 __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pitch, int base, const uint8_t *__restrict__ cur,
                                          int cpitch, int H, uint32_t (&acc)[K][4])
@@ -133,16 +142,12 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
     constexpr int FLUSH = 256 / SW;      // rows before a 16-bit lane could overflow (SW*rows*255 < 65536)
     const uint32_t shift = (uint32_t)base & 3u;
     const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
-    const int pd = pitch >> 2;           // pitch in dwords
-    // rows are kept as the qsad's 8-byte operands (pairs of realigned dwords):
-    // a ring of 32-bit words read in overlapping pairs is merged by the
-    // compiler into misaligned 64-bit accesses and ends up in scratch
+    const int pd = pitch >> 2;
     uint64_t ring[K][ND];
     uint64_t a16[K];
 #pragma unroll
     for (int k = 0; k < K; k++)
         a16[k] = 0;
-
     auto load_row = [&](int r, uint64_t (&dst)[ND]) {
         uint32_t raw[ND + 2];
         const uint32_t *src = w32 + r * pd;
@@ -157,21 +162,25 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
         for (int t = 0; t < ND; t++)
             dst[t] = ((uint64_t)al[t + 1] << 32) | al[t];
     };
-
 #pragma unroll
     for (int k = 0; k < K - 1; k++)
         load_row(k, ring[k]);
-
+    uint32_t cwb[2][ND];
+#pragma unroll
+    for (int t = 0; t < ND; t++)
+        cwb[0][t] = reinterpret_cast<const uint32_t *>(cur)[t];
     for (int c0 = 0; c0 < H; c0 += K) {
+        const bool last = c0 + K >= H;
 #pragma unroll
         for (int cc = 0; cc < K; cc++) {
             const int c = c0 + cc;
             load_row(c + K - 1, ring[(cc + K - 1) % K]);
-            uint32_t cw[ND];
-            const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
+            {
+                const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + min(c + 1, H - 1) * cpitch);
 #pragma unroll
-            for (int t = 0; t < ND; t++)
-                cw[t] = cp[t];
+                for (int t = 0; t < ND; t++)
+                    cwb[(cc + 1) & 1][t] = cp[t];
+            }
             // k innermost: consecutive qsads update different accumulators; the
             // scheduling barriers keep the compiler from pulling a dependent
             // qsad next to its producer
@@ -179,11 +188,11 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
             for (int t = 0; t < ND; t++) {
 #pragma unroll
                 for (int k = 0; k < K; k++)
-                    a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(cc + k) % K][t], cw[t], a16[k]);
+                    a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(cc + k) % K][t], cwb[cc & 1][t], a16[k]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (((c0 + K) % FLUSH) == 0 || c0 + K >= H) {
+        if (((c0 + K) % FLUSH) == 0 || last) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 acc[k][0] += (uint32_t)a16[k] & 0xFFFFu;
@@ -197,8 +206,7 @@ __device__ __forceinline__ void fs_strip(const uint8_t *__restrict__ win, int pi
 }
 
 // The same for a dword-aligned lane base: the 8 reference bytes of every qsad
-// are loaded straight into a register pair (two dwords of LDS per pair, no
-// realignment and no register copies); LDS bandwidth is not the bottleneck.
+// are loaded straight into a register pair (no realignment).
 template <int K, int SW>
 __device__ __forceinline__ void fs_strip_aligned(const uint8_t *__restrict__ win, int pitch, int base,
                                                  const uint8_t *__restrict__ cur, int cpitch, int H,
@@ -213,40 +221,40 @@ __device__ __forceinline__ void fs_strip_aligned(const uint8_t *__restrict__ win
 #pragma unroll
     for (int k = 0; k < K; k++)
         a16[k] = 0;
-
     auto load_row = [&](int r, uint64_t (&dst)[ND]) {
         const uint32_t *src = w32 + r * pd;
 #pragma unroll
         for (int t = 0; t < ND; t++)
             dst[t] = ((uint64_t)src[t + 1] << 32) | src[t];
     };
-
 #pragma unroll
     for (int k = 0; k < K - 1; k++)
         load_row(k, ring[k]);
-
+    uint32_t cwb[2][ND];
+#pragma unroll
+    for (int t = 0; t < ND; t++)
+        cwb[0][t] = reinterpret_cast<const uint32_t *>(cur)[t];
     for (int c0 = 0; c0 < H; c0 += K) {
+        const bool last = c0 + K >= H;
 #pragma unroll
         for (int cc = 0; cc < K; cc++) {
             const int c = c0 + cc;
             load_row(c + K - 1, ring[(cc + K - 1) % K]);
-            uint32_t cw[ND];
-            const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
+            {
+                const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + min(c + 1, H - 1) * cpitch);
 #pragma unroll
-            for (int t = 0; t < ND; t++)
-                cw[t] = cp[t];
-            // k innermost: consecutive qsads update different accumulators; the
-            // scheduling barriers keep the compiler from pulling a dependent
-            // qsad next to its producer
+                for (int t = 0; t < ND; t++)
+                    cwb[(cc + 1) & 1][t] = cp[t];
+            }
 #pragma unroll
             for (int t = 0; t < ND; t++) {
 #pragma unroll
                 for (int k = 0; k < K; k++)
-                    a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(cc + k) % K][t], cw[t], a16[k]);
+                    a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(cc + k) % K][t], cwb[cc & 1][t], a16[k]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (((c0 + K) % FLUSH) == 0 || c0 + K >= H) {
+        if (((c0 + K) % FLUSH) == 0 || last) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 acc[k][0] += (uint32_t)a16[k] & 0xFFFFu;
@@ -423,6 +431,73 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
     }
 }
 
+// Short PUs (at most 16 rows, 8 or 16 pixels wide) keep their H x 4NQ
+// current pixels in registers, NQ = width / 4.  Each of the H + K - 1 window
+// rows of the lane is loaded once, realigned to NQ + 1 dwords, and meets every
+// current row c = r - k with k in the lane's K candidate rows (a ring of K
+// window rows would need K x 2NQ registers).  Row r + 1 is loaded while row r
+// is summed.
+template <int H, int NQ, bool ALIGNED, int r>
+__device__ __forceinline__ void short_row(const uint32_t *__restrict__ w32, int pd, uint32_t shift,
+                                          const uint32_t (&cw)[H][NQ], uint32_t (&raw)[ALIGNED ? NQ + 1 : NQ + 2],
+                                          uint64_t (&a16)[XME_NARROW_K])
+{
+    constexpr int K = XME_NARROW_K;
+    constexpr int NW = ALIGNED ? NQ + 1 : NQ + 2;
+    constexpr int k_lo = r - (H - 1) > 0 ? r - (H - 1) : 0;
+    constexpr int k_hi = r < K - 1 ? r : K - 1;
+    uint32_t d[NQ + 1];
+#pragma unroll
+    for (int q = 0; q <= NQ; q++)
+        d[q] = ALIGNED ? raw[q] : __builtin_amdgcn_alignbyte(raw[q + 1], raw[q], shift);
+    if constexpr (r + 1 < H + K - 1) {
+        const uint32_t *src = w32 + (r + 1) * pd;
+#pragma unroll
+        for (int q = 0; q < NW; q++)
+            raw[q] = src[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const uint64_t w = ((uint64_t)d[q + 1] << 32) | d[q];
+#pragma unroll
+        for (int k = k_lo; k <= k_hi; k++)
+            a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(w, cw[r - k][q], a16[k]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int H, int NQ, bool ALIGNED, int... Rs>
+__device__ __forceinline__ void short_rows(std::integer_sequence<int, Rs...>, const uint32_t *__restrict__ w32, int pd,
+                                           uint32_t shift, const uint32_t (&cw)[H][NQ],
+                                           uint32_t (&raw)[ALIGNED ? NQ + 1 : NQ + 2], uint64_t (&a16)[XME_NARROW_K])
+{
+    (short_row<H, NQ, ALIGNED, Rs>(w32, pd, shift, cw, raw, a16), ...);
+}
+
+template <int H, int NQ, bool ALIGNED>
+__device__ __forceinline__ void short_strip(const uint8_t *__restrict__ win, int pitch, int base,
+                                            const uint8_t *__restrict__ cur, int cpitch,
+                                            uint64_t (&a16)[XME_NARROW_K])
+{
+    constexpr int NW = ALIGNED ? NQ + 1 : NQ + 2;
+    const uint32_t shift = (uint32_t)base & 3u;
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
+    const int pd = pitch >> 2;
+    uint32_t cw[H][NQ];
+#pragma unroll
+    for (int c = 0; c < H; c++) {
+        const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            cw[c][q] = cp[q];
+    }
+    uint32_t raw[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++)
+        raw[q] = w32[q];
+    short_rows<H, NQ, ALIGNED>(std::make_integer_sequence<int, H + XME_NARROW_K - 1>{}, w32, pd, shift, cw, raw, a16);
+}
+
 // Narrow PUs (8 pixels wide, at most 16 rows) carry little work per
 // candidate, so per-call costs dominate the generic path.  Here a lane takes
 // K = 16 candidate rows; the SAD and the MV cost of each candidate are summed
@@ -431,51 +506,6 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
 // depend only on the lane, the item and the search centre; they are kept in
 // the wave's LDS term area (16 x 64 u64 = 8 KB, the size of the generic path's
 // term table) across the item's instances.
-
-template <int H, bool ALIGNED>
-__device__ __forceinline__ void narrow_strip(const uint8_t *__restrict__ win, int pitch, int base,
-                                             const uint8_t *__restrict__ cur, int cpitch,
-                                             uint64_t (&a16)[XME_NARROW_K])
-{
-    constexpr int K = XME_NARROW_K;
-    const uint32_t shift = (uint32_t)base & 3u;
-    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
-    const int pd = pitch >> 2;
-    uint64_t ring[K][2];
-    auto load_row = [&](int r, uint64_t (&dst)[2]) {
-        const uint32_t *src = w32 + r * pd;
-        uint32_t d0, d1, d2;
-        if (ALIGNED) {
-            d0 = src[0];
-            d1 = src[1];
-            d2 = src[2];
-        } else {
-            const uint32_t r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
-            d0 = __builtin_amdgcn_alignbyte(r1, r0, shift);
-            d1 = __builtin_amdgcn_alignbyte(r2, r1, shift);
-            d2 = __builtin_amdgcn_alignbyte(r3, r2, shift);
-        }
-        dst[0] = ((uint64_t)d1 << 32) | d0;
-        dst[1] = ((uint64_t)d2 << 32) | d1;
-    };
-#pragma unroll
-    for (int k = 0; k < K - 1; k++)
-        load_row(k, ring[k]);
-#pragma unroll
-    for (int c = 0; c < H; c++) {
-        load_row(c + K - 1, ring[(c + K - 1) % K]);
-        const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
-        const uint32_t cw0 = cp[0], cw1 = cp[1];
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(c + k) % K][0], cw0, a16[k]);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            a16[k] = __builtin_amdgcn_qsad_pk_u16_u8(ring[(c + k) % K][1], cw1, a16[k]);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
 
 template <int H>
 __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &pu, const DevInst &in, const FsItem &it, int inst_id,
@@ -521,9 +551,9 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
         a16[k] = init[k * 64 + lane];
     const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
     if ((lane_base & 3) == 0)      // uniform: col and the pitch are multiples of 4
-        narrow_strip<H, true>(win, pitch, lane_base, cblk, cpitch, a16);
+        short_strip<H, 2, true>(win, pitch, lane_base, cblk, cpitch, a16);
     else
-        narrow_strip<H, false>(win, pitch, lane_base, cblk, cpitch, a16);
+        short_strip<H, 2, false>(win, pitch, lane_base, cblk, cpitch, a16);
 
     // first strict minimum in raster order (COPY3_IF_LT, me.h:70-75): the lane's
     // best as (cost << 6 | k * 4 + d), then the wave's as (cost << 12 | order in
@@ -591,8 +621,126 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     }
 }
 
+// 16-pixel-wide PUs of at most 16 rows use the same 16 candidate rows per
+// lane.  Their SAD alone fills the u16 lanes (16 * 16 * 255 < 2^16), so the MV
+// costs (packed u16, cached in the wave's LDS term area like the narrow path's)
+// are added in 32 bits in the epilogue.
+template <int H>
+__device__ __forceinline__ void fs_item_wide16(const DevParams &p, const DevPu &pu, const DevInst &in,
+                                               const FsItem &it, int inst_id, const uint8_t *__restrict__ win,
+                                               int pitch, int wx0, int wy0, const uint8_t *__restrict__ cur,
+                                               int cpitch, int cx0, int cy0, uint64_t *__restrict__ best,
+                                               int32_t *__restrict__ cost_map, uint32_t *__restrict__ term,
+                                               TermState &ts)
+{
+    constexpr int K = XME_NARROW_K;
+    const int lane = threadIdx.x & 63;
+    const int LC = it.LC;
+    const int lc = lane % LC, rg = lane / LC;
+    const int R = p.R, R2 = 2 * R;
+    const int bx = pu.x + in.omx - R - wx0;
+    const int by = pu.y + in.omy - R - wy0;
+    const int col = it.col0 + 4 * lc;
+    const int row = it.row0 + rg * K;
+    const int lane_base = (by + row) * pitch + bx + col;
+    uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]: packed u16 MV costs
+    if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
+        const uint32_t lam = (uint32_t)p.lambda;
+        int bits_x[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            bits_x[d] = xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int by_bits = xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y);
+            uint64_t v = 0;
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+                v |= (uint64_t)(__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << (16 * d);
+            init[k * 64 + lane] = v;
+        }
+        ts.ok = true;
+        ts.cx = in.omx;
+        ts.cy = in.omy;
+    }
+    uint64_t a16[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        a16[k] = 0;
+    const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
+    if ((lane_base & 3) == 0)      // uniform: col and the pitch are multiples of 4
+        short_strip<H, 4, true>(win, pitch, lane_base, cblk, cpitch, a16);
+    else
+        short_strip<H, 4, false>(win, pitch, lane_base, cblk, cpitch, a16);
+
+    const MvWindow mw = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
+    const int col_end = it.col0 + it.ncols, row_end = it.row0 + it.nrows;
+    const int bmx0 = in.omx + it.col0 - R, bmy0 = in.omy + it.row0 - R;
+    const bool full = it.ncols == 4 * LC && it.nrows == (64 / LC) * K;
+    const bool fast = full && bmx0 >= mw.min_fpel[0] && bmx0 + it.ncols - 1 <= mw.max_fpel[0] &&
+                      bmy0 >= mw.min_fpel[1] && bmy0 + it.nrows - 1 <= mw.max_fpel[1];
+    // cost = SAD + MV cost < 2^17: the lane key (cost << 6 | k * 4 + d) fits
+    uint32_t m = 0xFFFFFFFFu;
+    if (fast) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint64_t a = a16[k], mv = init[k * 64 + lane];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const uint32_t v = ((uint32_t)(a >> (16 * d)) & 0xFFFFu) + ((uint32_t)(mv >> (16 * d)) & 0xFFFFu);
+                m = min(m, (v << 6) | (uint32_t)(k * 4 + d));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint64_t a = a16[k], mv = init[k * 64 + lane];
+            const int j = row + k;
+            const int my = in.omy + j - R;
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int i = col + d;
+                const bool legal = i < col_end && j < row_end && xme_mv_legal(mw, in.omx + i - R, my);
+                const uint32_t v = ((uint32_t)(a >> (16 * d)) & 0xFFFFu) + ((uint32_t)(mv >> (16 * d)) & 0xFFFFu);
+                if (legal)
+                    m = min(m, (v << 6) | (uint32_t)(k * 4 + d));
+            }
+        }
+    }
+    uint32_t lkey = 0xFFFFFFFFu;
+    if (m != 0xFFFFFFFFu) {
+        const int idx = (int)(m & 63u);
+        const int lrow = rg * K + (idx >> 2), lcol = 4 * lc + (idx & 3);
+        lkey = ((m >> 6) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
+    }
+    lkey = wave_min_u32(lkey);
+    if (lkey != 0xFFFFFFFFu && lane == 0) {
+        const int lo = (int)(lkey & 4095u);
+        const int j = it.row0 + lo / (4 * LC), i = it.col0 + lo % (4 * LC);
+        const uint64_t kbest = xme_key(lkey >> 12, (uint32_t)(XME_ORDER_RASTER + j * R2 + i));
+        atomicMin(reinterpret_cast<unsigned long long *>(best + inst_id), (unsigned long long)kbest);
+    }
+    if (cost_map != nullptr) {       // diagnostic output: every raster cost, -1 where illegal
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint64_t a = a16[k], mv = init[k * 64 + lane];
+            const int j = row + k;
+            const int my = in.omy + j - R;
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int i = col + d;
+                if (i < col_end && j < row_end) {
+                    const bool legal = xme_mv_legal(mw, in.omx + i - R, my);
+                    const int v = (int)(((a >> (16 * d)) & 0xFFFFu) + ((mv >> (16 * d)) & 0xFFFFu));
+                    cost_map[(int64_t)inst_id * R2 * R2 + j * R2 + i] = legal ? v : -1;
+                }
+            }
+        }
+    }
+}
+
 // Stage a task's reference window and current pixels in LDS.
-__device__ void stage_task(const FsTask &t, const FrameDesc *__restrict__ desc, const DevParams &p, uint8_t *lds,
+__device__ __forceinline__ void stage_task(const FsTask &t, const FrameDesc *__restrict__ desc, const DevParams &p, uint8_t *lds,
                            int pitch, int cpitch)
 {
     const uint8_t *ref = desc->planes[t.ref][0];
@@ -709,7 +857,20 @@ __global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p,
                                              __builtin_amdgcn_readlane(my_pu.z, g), __builtin_amdgcn_readlane(my_pu.w, g));
                 const DevPu pu = __builtin_bit_cast(DevPu, vpu);
                 if constexpr (KMAX == XME_NARROW_K) {
-                    if (pu.h == 16)
+                    if (pu.w == 16) {
+                        if (pu.h == 16)
+                            fs_item_wide16<16>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
+                                               t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
+                        else if (pu.h == 12)
+                            fs_item_wide16<12>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
+                                               t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
+                        else if (pu.h == 8)
+                            fs_item_wide16<8>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
+                                              t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
+                        else
+                            fs_item_wide16<4>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
+                                              t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
+                    } else if (pu.h == 16)
                         fs_item_narrow<16>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
                                            best, const_cast<int32_t *>(cost_map), term, ts);
                     else if (pu.h == 8)

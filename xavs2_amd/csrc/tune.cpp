@@ -209,7 +209,7 @@ extern "C" int xme_tune_set(int part, int R, int k_rows, int lanes_per_row)
     const bool k_ok = k_rows == 4 || k_rows == 8 || (k_rows == 16 && part >= 0 && part < XME_NUM_PU &&
                                                         xme_narrow_shape(part));
     if (part < 0 || part >= XME_NUM_PU || R < 1 || R > 128 || !k_ok || !lc_ok) {
-        xme_set_error("xme_tune_set: part 0..24, R 1..128, K 4 or 8 (16 for 8-wide shapes of <= 16 rows), "
+        xme_set_error("xme_tune_set: part 0..24, R 1..128, K 4 or 8 (16 for 8- or 16-wide shapes of <= 16 rows), "
                       "LC 8/16/32/64");
         return XME_ERR_ARG;
     }
