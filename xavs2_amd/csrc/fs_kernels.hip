@@ -435,26 +435,35 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
 // current pixels in registers, NQ = width / 4.  Each of the H + K - 1 window
 // rows of the lane is loaded once, realigned to NQ + 1 dwords, and meets every
 // current row c = r - k with k in the lane's K candidate rows (a ring of K
-// window rows would need K x 2NQ registers).  Row r + 1 is loaded while row r
-// is summed.
+// window rows would need K x 2NQ registers).  Window rows are loaded PF rows
+// ahead: the first and last rows feed only a few candidate rows, too few
+// qsads to cover the LDS latency of the next row.
+template <int NQ, bool ALIGNED>
+struct ShortRows {
+    static constexpr int NW = ALIGNED ? NQ + 1 : NQ + 2;   // dwords loaded per window row
+    static constexpr int PF = NQ == 2 ? 4 : 2;              // rows in flight
+};
+
 template <int H, int NQ, bool ALIGNED, int r>
 __device__ __forceinline__ void short_row(const uint32_t *__restrict__ w32, int pd, uint32_t shift,
-                                          const uint32_t (&cw)[H][NQ], uint32_t (&raw)[ALIGNED ? NQ + 1 : NQ + 2],
+                                          const uint32_t (&cw)[H][NQ],
+                                          uint32_t (&raw)[ShortRows<NQ, ALIGNED>::PF][ShortRows<NQ, ALIGNED>::NW],
                                           uint64_t (&a16)[XME_NARROW_K])
 {
     constexpr int K = XME_NARROW_K;
-    constexpr int NW = ALIGNED ? NQ + 1 : NQ + 2;
+    constexpr int NW = ShortRows<NQ, ALIGNED>::NW;
+    constexpr int PF = ShortRows<NQ, ALIGNED>::PF;
     constexpr int k_lo = r - (H - 1) > 0 ? r - (H - 1) : 0;
     constexpr int k_hi = r < K - 1 ? r : K - 1;
     uint32_t d[NQ + 1];
 #pragma unroll
     for (int q = 0; q <= NQ; q++)
-        d[q] = ALIGNED ? raw[q] : __builtin_amdgcn_alignbyte(raw[q + 1], raw[q], shift);
-    if constexpr (r + 1 < H + K - 1) {
-        const uint32_t *src = w32 + (r + 1) * pd;
+        d[q] = ALIGNED ? raw[r % PF][q] : __builtin_amdgcn_alignbyte(raw[r % PF][q + 1], raw[r % PF][q], shift);
+    if constexpr (r + PF < H + K - 1) {
+        const uint32_t *src = w32 + (r + PF) * pd;
 #pragma unroll
         for (int q = 0; q < NW; q++)
-            raw[q] = src[q];
+            raw[r % PF][q] = src[q];
     }
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
@@ -469,7 +478,8 @@ __device__ __forceinline__ void short_row(const uint32_t *__restrict__ w32, int 
 template <int H, int NQ, bool ALIGNED, int... Rs>
 __device__ __forceinline__ void short_rows(std::integer_sequence<int, Rs...>, const uint32_t *__restrict__ w32, int pd,
                                            uint32_t shift, const uint32_t (&cw)[H][NQ],
-                                           uint32_t (&raw)[ALIGNED ? NQ + 1 : NQ + 2], uint64_t (&a16)[XME_NARROW_K])
+                                           uint32_t (&raw)[ShortRows<NQ, ALIGNED>::PF][ShortRows<NQ, ALIGNED>::NW],
+                                           uint64_t (&a16)[XME_NARROW_K])
 {
     (short_row<H, NQ, ALIGNED, Rs>(w32, pd, shift, cw, raw, a16), ...);
 }
@@ -479,22 +489,30 @@ __device__ __forceinline__ void short_strip(const uint8_t *__restrict__ win, int
                                             const uint8_t *__restrict__ cur, int cpitch,
                                             uint64_t (&a16)[XME_NARROW_K])
 {
-    constexpr int NW = ALIGNED ? NQ + 1 : NQ + 2;
+    constexpr int NW = ShortRows<NQ, ALIGNED>::NW;
+    constexpr int PF = ShortRows<NQ, ALIGNED>::PF;
+    static_assert(PF <= H + XME_NARROW_K - 1, "prefetch depth");
     const uint32_t shift = (uint32_t)base & 3u;
     const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win + (base & ~3));
     const int pd = pitch >> 2;
     uint32_t cw[H][NQ];
+    uint32_t raw[PF][NW];
+    // issue order: what the first rows need comes first
 #pragma unroll
-    for (int c = 0; c < H; c++) {
+    for (int q = 0; q < NQ; q++)
+        cw[0][q] = reinterpret_cast<const uint32_t *>(cur)[q];
+#pragma unroll
+    for (int i = 0; i < PF; i++)
+#pragma unroll
+        for (int q = 0; q < NW; q++)
+            raw[i][q] = w32[i * pd + q];
+#pragma unroll
+    for (int c = 1; c < H; c++) {
         const uint32_t *cp = reinterpret_cast<const uint32_t *>(cur + c * cpitch);
 #pragma unroll
         for (int q = 0; q < NQ; q++)
             cw[c][q] = cp[q];
     }
-    uint32_t raw[NW];
-#pragma unroll
-    for (int q = 0; q < NW; q++)
-        raw[q] = w32[q];
     short_rows<H, NQ, ALIGNED>(std::make_integer_sequence<int, H + XME_NARROW_K - 1>{}, w32, pd, shift, cw, raw, a16);
 }
 
@@ -564,16 +582,19 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     const bool full = it.ncols == 4 * LC && it.nrows == (64 / LC) * K;
     const bool fast = full && bmx0 >= mw.min_fpel[0] && bmx0 + it.ncols - 1 <= mw.max_fpel[0] &&
                       bmy0 >= mw.min_fpel[1] && bmy0 + it.nrows - 1 <= mw.max_fpel[1];
+    // lane key (cost << 16 | k * 4 + d): the u16 halves become keys with one
+    // shift-or or and-or each, and min3 takes two keys at a time
     uint32_t m = 0xFFFFFFFFu;
     if (fast) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint32_t lo = (uint32_t)a16[k], hi = (uint32_t)(a16[k] >> 32);
-            const uint32_t k0 = ((lo & 0xFFFFu) << 6) | (uint32_t)(k * 4 + 0);
-            const uint32_t k1 = ((lo >> 16) << 6) | (uint32_t)(k * 4 + 1);
-            const uint32_t k2 = ((hi & 0xFFFFu) << 6) | (uint32_t)(k * 4 + 2);
-            const uint32_t k3 = ((hi >> 16) << 6) | (uint32_t)(k * 4 + 3);
-            m = min(m, min(min(k0, k1), min(k2, k3)));
+            const uint32_t k0 = (lo << 16) | (uint32_t)(k * 4 + 0);
+            const uint32_t k1 = (lo & 0xFFFF0000u) | (uint32_t)(k * 4 + 1);
+            const uint32_t k2 = (hi << 16) | (uint32_t)(k * 4 + 2);
+            const uint32_t k3 = (hi & 0xFFFF0000u) | (uint32_t)(k * 4 + 3);
+            m = min(m, min(k0, k1));
+            m = min(m, min(k2, k3));
         }
     } else {
 #pragma unroll
@@ -586,7 +607,7 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
                 const bool legal = i < col_end && j < row_end && xme_mv_legal(mw, in.omx + i - R, my);
                 const uint32_t v = (uint32_t)(a16[k] >> (16 * d)) & 0xFFFFu;
                 if (legal)
-                    m = min(m, (v << 6) | (uint32_t)(k * 4 + d));
+                    m = min(m, (v << 16) | (uint32_t)(k * 4 + d));
             }
         }
     }
@@ -594,7 +615,7 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     if (m != 0xFFFFFFFFu) {
         const int idx = (int)(m & 63u);
         const int lrow = rg * K + (idx >> 2), lcol = 4 * lc + (idx & 3);
-        lkey = ((m >> 6) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
+        lkey = ((m >> 16) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
     }
     lkey = wave_min_u32(lkey);
     if (lkey != 0xFFFFFFFFu && lane == 0) {
@@ -663,10 +684,12 @@ __device__ __forceinline__ void fs_item_wide16(const DevParams &p, const DevPu &
         ts.cx = in.omx;
         ts.cy = in.omy;
     }
-    uint64_t a16[K];
+    uint64_t a16[K], mvc[K];
 #pragma unroll
-    for (int k = 0; k < K; k++)
+    for (int k = 0; k < K; k++) {
         a16[k] = 0;
+        mvc[k] = init[k * 64 + lane];   // read now: the loads land during the strip
+    }
     const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
     if ((lane_base & 3) == 0)      // uniform: col and the pitch are multiples of 4
         short_strip<H, 4, true>(win, pitch, lane_base, cblk, cpitch, a16);
@@ -684,17 +707,20 @@ __device__ __forceinline__ void fs_item_wide16(const DevParams &p, const DevPu &
     if (fast) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const uint64_t a = a16[k], mv = init[k * 64 + lane];
+            const uint64_t a = a16[k], mv = mvc[k];
+            uint32_t key[4];
 #pragma unroll
             for (int d = 0; d < 4; d++) {
                 const uint32_t v = ((uint32_t)(a >> (16 * d)) & 0xFFFFu) + ((uint32_t)(mv >> (16 * d)) & 0xFFFFu);
-                m = min(m, (v << 6) | (uint32_t)(k * 4 + d));
+                key[d] = (v << 6) | (uint32_t)(k * 4 + d);
             }
+            m = min(m, min(key[0], key[1]));
+            m = min(m, min(key[2], key[3]));
         }
     } else {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const uint64_t a = a16[k], mv = init[k * 64 + lane];
+            const uint64_t a = a16[k], mv = mvc[k];
             const int j = row + k;
             const int my = in.omy + j - R;
 #pragma unroll
@@ -723,7 +749,7 @@ __device__ __forceinline__ void fs_item_wide16(const DevParams &p, const DevPu &
     if (cost_map != nullptr) {       // diagnostic output: every raster cost, -1 where illegal
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const uint64_t a = a16[k], mv = init[k * 64 + lane];
+            const uint64_t a = a16[k], mv = mvc[k];
             const int j = row + k;
             const int my = in.omy + j - R;
 #pragma unroll
