@@ -17,13 +17,19 @@ __constant__ int8_t c_spiral[9][2] = { { 0, 0 }, { 0, -1 }, { 0, 1 }, { -1, -1 }
 __constant__ int8_t c_spiral2[9][2] = { { 0, 0 }, { 0, -1 }, { -1, -1 }, { -1, 0 }, { -1, 1 },
                                         { 0, 1 }, { 1, 1 }, { 1, 0 }, { 1, -1 } };
 
+// Picture and plane bytes in device memory.  The pointers come from the
+// frame descriptor as generic pointers; typing them global gives global (not
+// flat) loads.
+typedef const __attribute__((address_space(1))) uint8_t *gbytes;
+typedef const __attribute__((address_space(1))) uint32_t *gdwords;
+
 // Four bytes at any address.  The dword after the aligned one is read even
 // when the address is aligned; reference rows have XME_PAD bytes of padding
 // on the right, so it stays inside the plane.
-__device__ __forceinline__ uint32_t load_unaligned_dw(const uint8_t *p)
+__device__ __forceinline__ uint32_t load_unaligned_dw(gbytes p)
 {
-    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    const uint32_t *q = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+    const uintptr_t a = (uintptr_t)p;
+    gdwords q = (gdwords)(a & ~(uintptr_t)3);
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
@@ -31,7 +37,7 @@ __device__ __forceinline__ int bytei(uint32_t v, int i) { return (int)((v >> (8 
 
 // Sum of |H4 D H4^T| over one 4x4 tile (before the final halving), or the SAD,
 // of current rows c[0..3] against the reference tile at `ref`.
-__device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], const uint8_t *ref, int64_t i_ref, bool hadamard)
+__device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], gbytes ref, int64_t i_ref, bool hadamard)
 {
     uint32_t rr[4];
 #pragma unroll
@@ -70,7 +76,7 @@ __device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], const uint8_t 
 }
 
 struct RefineCtx {
-    const uint8_t *cur;        // PU origin in the current picture
+    gbytes cur;                // PU origin in the current picture
     int64_t i_cur;
     const uint8_t *const *planes;
     int64_t i_ref;
@@ -80,17 +86,9 @@ struct RefineCtx {
     int lambda, pmx, pmy;
 };
 
-// Makes this wave's LDS writes visible to its own later LDS reads.
-__device__ __forceinline__ void wave_sync()
+__device__ __forceinline__ gbytes cand_ref(const RefineCtx &c, int mx, int my, int ty, int tx)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ const uint8_t *cand_ref(const RefineCtx &c, int mx, int my, int ty, int tx)
-{
-    const uint8_t *plane = c.planes[((my & 3) << 2) + (mx & 3)];
+    const gbytes plane = (gbytes)c.planes[((my & 3) << 2) + (mx & 3)];
     return plane + c.off + (int64_t)((my >> 2) + ty) * c.i_ref + (mx >> 2) + tx;
 }
 
@@ -139,19 +137,83 @@ __device__ __forceinline__ void cand_mv(const Stage &st, int k, int &mx, int &my
     my = st.by + st.pat[pos][1];
 }
 
-// Distortion sums (SATD before halving, or SAD) of the stage's n <= 9
-// candidates into s_sum[k].  With P = the PU's 4x4 tile count rounded up to a
-// power of two, a PU of at most 64 tiles gives each group of P lanes one
-// candidate at a time (a lane = a tile) and sums the group with lane
-// shuffles; a larger PU gives every lane P / 64 tiles of every candidate and
-// sums the whole wave.  The current rows of a lane's tile are loaded once.
-__device__ void stage_sums(const RefineCtx &c, const Stage &st, int n, int *s_sum)
+// Cross-lane reductions with DPP (no LDS traffic).  They run with all 64
+// lanes active.  row_shr and the row broadcasts move data to higher lanes, so
+// a sum or minimum collects in the last lane of its group.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_add(int v)
+{
+    return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, true);
+}
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_min(int v)
+{
+    // lanes without a source keep the identity
+    return min(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over each aligned group of P lanes (P a power of two <= 64, uniform);
+// the sum is in the group's last lane.
+__device__ __forceinline__ int group_sum(int v, int P)
+{
+    if (P >= 2)
+        v = dpp_add<0xB1>(v);          // quad_perm [1,0,3,2]
+    if (P >= 4)
+        v = dpp_add<0x4E>(v);          // quad_perm [2,3,0,1]
+    if (P >= 8)
+        v = dpp_add<0x114>(v);         // row_shr:4
+    if (P >= 16)
+        v = dpp_add<0x118>(v);         // row_shr:8
+    if (P >= 32)
+        v = dpp_add<0x142, 0xA>(v);    // row_bcast:15 into rows 1 and 3
+    if (P >= 64)
+        v = dpp_add<0x143, 0xC>(v);    // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// Minimum of non-negative keys over the wave, returned to every lane.
+__device__ __forceinline__ int wave_min_key(int v)
+{
+    v = dpp_min<0xB1>(v);
+    v = dpp_min<0x4E>(v);
+    v = dpp_min<0x114>(v);
+    v = dpp_min<0x118>(v);
+    v = dpp_min<0x142, 0xA>(v);
+    v = dpp_min<0x143, 0xC>(v);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ int pack_cost_idx(int cost, int k)
+{
+    // costs stay below 2^27 (a 64x64 SATD is below 2^22), 4 bits of order
+    return cost == 0x7fffffff ? 0x7fffffff : (cost << 4) | k;
+}
+
+// ME_COST_QPEL (me.c:344-349) of candidate (mx, my) with distortion sum s
+// (SATD before halving, or SAD).
+__device__ __forceinline__ int cand_cost(const RefineCtx &c, int s, int mx, int my)
+{
+    const int dist = c.hadamard ? (s >> 1) : s;
+    return dist + xme_weighted_cost(c.lambda, xme_mvbits(mx - c.pmx) + xme_mvbits(my - c.pmy));
+}
+
+// Evaluates the stage's n <= 9 candidates and returns the wave minimum of
+// (cost << 4 | k) over the candidates k >= st.c0 that the reference
+// evaluates (0x7fffffff when none); *start gets the cost of candidate 0.
+// With P = the PU's 4x4 tile count rounded up to a power of two, a PU of at
+// most 64 tiles gives each group of P lanes one candidate at a time (a lane =
+// a tile); the group's last lane completes the sum and the cost.  A larger PU
+// gives every lane P / 64 tiles of every candidate and sums the whole wave.
+// The current rows of a lane's tile are loaded once.
+__device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start)
 {
     const int lane = threadIdx.x & 63;
     const int tw = c.w >> 2, ntiles = tw * (c.h >> 2);
     int P = 1;
     while (P < ntiles)
         P <<= 1;
+    int m = 0x7fffffff;
     if (P <= 64) {
         const int groups = 64 / P;
         const int g = lane / P, tile = lane % P;
@@ -160,85 +222,61 @@ __device__ void stage_sums(const RefineCtx &c, const Stage &st, int n, int *s_su
         uint32_t cr[4];
 #pragma unroll
         for (int y = 0; y < 4; y++)
-            cr[y] = has_tile ? *reinterpret_cast<const uint32_t *>(c.cur + (ty + y) * c.i_cur + tx) : 0u;
+            cr[y] = has_tile ? *(gdwords)(c.cur + (ty + y) * c.i_cur + tx) : 0u;
+        int start_v = 0x7fffffff;
         for (int k0 = 0; k0 < n; k0 += groups) {
             const int k = k0 + g;
-            int v = 0;
-            if (has_tile && k < n) {
-                int mx, my;
-                bool skip;
+            int mx = 0, my = 0;
+            bool skip = true;
+            if (k < n)
                 cand_mv(st, k, mx, my, skip);
-                if (!skip)
-                    v = tile_cost(cr, cand_ref(c, mx, my, ty, tx), c.i_ref, c.hadamard);
+            int v = 0;
+            if (has_tile && !skip)
+                v = tile_cost(cr, cand_ref(c, mx, my, ty, tx), c.i_ref, c.hadamard);
+            v = group_sum(v, P);
+            if (tile == P - 1 && !skip) {
+                const int cost = cand_cost(c, v, mx, my);
+                if (k == 0)
+                    start_v = cost;
+                if (k >= st.c0)
+                    m = min(m, pack_cost_idx(cost, k));
             }
-            for (int o = 1; o < P; o <<= 1)
-                v += __shfl_xor(v, o);
-            if (tile == 0 && k < n)
-                s_sum[k] = v;
         }
-    } else {
-        int acc[9];
-        int cmx[9], cmy[9];
-        bool cskip[9];
+        *start = __builtin_amdgcn_readlane(start_v, P - 1);   // k = 0: group 0 of the first pass
+        return wave_min_key(m);
+    }
+    int acc[9];
+    int cmx[9], cmy[9];
+    bool cskip[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            acc[k] = 0;
-            cand_mv(st, k < n ? k : 0, cmx[k], cmy[k], cskip[k]);
-        }
-        for (int tile = lane; tile < ntiles; tile += 64) {
-            const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
-            uint32_t cr[4];
+    for (int k = 0; k < 9; k++) {
+        acc[k] = 0;
+        cand_mv(st, k < n ? k : 0, cmx[k], cmy[k], cskip[k]);
+    }
+    for (int tile = lane; tile < ntiles; tile += 64) {
+        const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
+        uint32_t cr[4];
 #pragma unroll
-            for (int y = 0; y < 4; y++)
-                cr[y] = *reinterpret_cast<const uint32_t *>(c.cur + (ty + y) * c.i_cur + tx);
+        for (int y = 0; y < 4; y++)
+            cr[y] = *(gdwords)(c.cur + (ty + y) * c.i_cur + tx);
 #pragma unroll
-            for (int k = 0; k < 9; k++)
-                if (k < n && !cskip[k])
-                    acc[k] += tile_cost(cr, cand_ref(c, cmx[k], cmy[k], ty, tx), c.i_ref, c.hadamard);
-        }
+        for (int k = 0; k < 9; k++)
+            if (k < n && !cskip[k])
+                acc[k] += tile_cost(cr, cand_ref(c, cmx[k], cmy[k], ty, tx), c.i_ref, c.hadamard);
+    }
+    *start = 0x7fffffff;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            if (k < n) {
-                int v = acc[k];
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1)
-                    v += __shfl_xor(v, o);
-                if (lane == 0)
-                    s_sum[k] = v;
-            }
+    for (int k = 0; k < 9; k++) {
+        if (k < n && !cskip[k]) {          // uniform: the candidates do not depend on the lane
+            const int s = __builtin_amdgcn_readlane(group_sum(acc[k], 64), 63);
+            const int cost = cand_cost(c, s, cmx[k], cmy[k]);
+            if (k == 0)
+                *start = cost;
+            if (k >= st.c0)
+                m = min(m, pack_cost_idx(cost, k));
         }
     }
-    wave_sync();
-}
-
-// Lane k < n: the full cost of candidate k (ME_COST_QPEL, me.c:344-349),
-// 0x7fffffff for other lanes and skipped points.
-__device__ __forceinline__ int stage_cost(const RefineCtx &c, const Stage &st, int n, const int *s_sum)
-{
-    const int lane = threadIdx.x & 63;
-    if (lane >= n)
-        return 0x7fffffff;
-    int mx, my;
-    bool skip;
-    cand_mv(st, lane, mx, my, skip);
-    if (skip)
-        return 0x7fffffff;
-    const int dist = c.hadamard ? (s_sum[lane] >> 1) : s_sum[lane];
-    return dist + xme_weighted_cost(c.lambda, xme_mvbits(mx - c.pmx) + xme_mvbits(my - c.pmy));
-}
-
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
-{
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1)
-        v = min(v, (uint32_t)__shfl_xor((int)v, o));
-    return v;
-}
-
-__device__ __forceinline__ int pack_cost_idx(int cost, int k)
-{
-    // costs stay below 2^27 (a 64x64 SATD is below 2^22), 4 bits of order
-    return cost == 0x7fffffff ? 0x7fffffff : (cost << 4) | k;
+    return m;
 }
 
 constexpr int SP_WAVES = 4;
@@ -256,7 +294,6 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
                                                      const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
                                                      const uint64_t *__restrict__ best, int n_inst)
 {
-    __shared__ int s_sum[SP_WAVES][16];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // gridDim.x is a multiple of 8: workgroup b runs on XCD b % 8
     const int per_xcd = gridDim.x / 8;
@@ -268,7 +305,6 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
     const int r = logical / n_pu;
     const int pu_i = logical - r * n_pu;
     const int id = pu_i * p.n_ref + r;
-    int *sum = s_sum[wave];
     const DevPu pu = pus[pu_i];
     const DevInst in = inst[id];
     const uint64_t key = best[id];
@@ -289,8 +325,10 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
     const int bcost_int = (int)(key >> 32);
     int bmx, bmy;
     if (order < XME_ORDER_RASTER) {
-        bmx = in.mvc[order][0];
-        bmy = in.mvc[order][1];
+        // a select, not an index: a dynamically indexed copy of `in` would be
+        // placed in LDS
+        bmx = order == 0 ? in.mvc[0][0] : in.mvc[1][0];
+        bmy = order == 0 ? in.mvc[0][1] : in.mvc[1][1];
     } else {
         const int o = (int)order - XME_ORDER_RASTER, R2 = 2 * p.R;
         bmx = in.omx + (o % R2) - p.R;
@@ -303,7 +341,7 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
     int bx = 4 * bmx, by = 4 * bmy, bcost = bcost_int;
     if (p.frac) {
         RefineCtx c;
-        c.cur = desc->cur + (int64_t)pu.y * desc->i_cur + pu.x;
+        c.cur = (gbytes)desc->cur + (int64_t)pu.y * desc->i_cur + pu.x;
         c.i_cur = desc->i_cur;
         c.planes = desc->planes[r];
         c.i_ref = desc->stride[r];
@@ -328,11 +366,10 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
         st.by = by;
         st.c0 = c.hadamard ? 1 : 0;
         int n = st.c0 + npts;
-        stage_sums(c, st, n, sum);
-        int cost = stage_cost(c, st, n, sum);
+        int start = 0;
+        uint32_t m = (uint32_t)stage_min(c, st, n, &start);
         if (c.hadamard)
-            bcost = __shfl(cost, 0);
-        uint32_t m = wave_min_u32(lane >= st.c0 ? (uint32_t)pack_cost_idx(cost, lane) : 0x7fffffffu);
+            bcost = start;
         if ((int)(m >> 4) < bcost && m != 0x7fffffffu) {
             int mx, my;
             bool skip;
@@ -347,10 +384,7 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
             st.by = by;
             st.c0 = 0;
             n = npts;
-            wave_sync();
-            stage_sums(c, st, n, sum);
-            cost = stage_cost(c, st, n, sum);
-            m = wave_min_u32((uint32_t)pack_cost_idx(cost, lane < 16 ? lane : 0));
+            m = (uint32_t)stage_min(c, st, n, &start);
             if (m != 0x7fffffffu && (int)(m >> 4) < bcost) {
                 int mx, my;
                 bool skip;
