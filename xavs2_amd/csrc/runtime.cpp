@@ -181,6 +181,8 @@ struct xme_plan {
     FsTask *d_tasks = nullptr;
     FsItem *d_items = nullptr;
     int32_t *d_item_insts = nullptr;
+    int32_t *d_sp_list = nullptr;      // sub-pel PU order: small PUs, then large ones
+    int n_sp_small = 0, n_sp_large = 0;
     uint32_t *d_counters = nullptr;
     FrameDesc *d_desc = nullptr;
     int n_tasks = 0;
@@ -392,6 +394,7 @@ static void plan_free(xme_plan *pl)
     hipFree(pl->d_tasks);
     hipFree(pl->d_items);
     hipFree(pl->d_item_insts);
+    hipFree(pl->d_sp_list);
     hipFree(pl->d_counters);
     hipFree(pl->d_desc);
     for (auto &s : pl->slots) {
@@ -672,6 +675,19 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     up((void **)&pl->d_tasks, tasks.data(), tasks.size() * sizeof(FsTask));
     up((void **)&pl->d_items, items.data(), items.size() * sizeof(FsItem));
     up((void **)&pl->d_item_insts, item_insts.data(), item_insts.size() * sizeof(int32_t));
+    {
+        std::vector<int32_t> sp_list;
+        sp_list.reserve(pus.size());
+        for (int large = 0; large < 2; large++)
+            for (size_t i = 0; i < pus.size(); i++)
+                if (xme_subpel_large(pus[i].w, pus[i].h) == (large == 1))
+                    sp_list.push_back((int32_t)i);
+        pl->n_sp_large = 0;
+        for (const DevPu &q : pus)
+            pl->n_sp_large += xme_subpel_large(q.w, q.h) ? 1 : 0;
+        pl->n_sp_small = (int)pus.size() - pl->n_sp_large;
+        up((void **)&pl->d_sp_list, sp_list.data(), sp_list.size() * sizeof(int32_t));
+    }
     if (e == hipSuccess)
         e = hipMalloc((void **)&pl->d_inst, sizeof(DevInst) * pl->n_inst);
     if (e == hipSuccess)
@@ -883,7 +899,8 @@ static int launch_all(xme_plan *pl, hipStream_t s)
     rc = launch_search(pl, s);
     if (rc)
         return rc;
-    return xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
+    return xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->d_sp_list, pl->n_sp_small,
+                             pl->n_sp_large, s);
 }
 
 extern "C" int xme_me_run(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_ref_desc *refs,
@@ -954,7 +971,8 @@ extern "C" int xme_me_time(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const
         if (rc)
             break;
         XME_HIP(hipEventRecord(ev[4 * r + 2], s));
-        rc = xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->n_inst, s);
+        rc = xme_launch_subpel(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->d_sp_list, pl->n_sp_small,
+                               pl->n_sp_large, s);
         XME_HIP(hipEventRecord(ev[4 * r + 3], s));
     }
     if (rc == XME_OK)

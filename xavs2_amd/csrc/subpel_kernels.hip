@@ -33,10 +33,23 @@ __device__ __forceinline__ uint32_t load_unaligned_dw(gbytes p)
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
-__device__ __forceinline__ int bytei(uint32_t v, int i) { return (int)((v >> (8 * i)) & 0xFFu); }
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+
+// Byte x of two rows as an i16 pair (lo: row `lo_row`, hi: row `hi_row`).
+__device__ __forceinline__ s16x2 byte_pair(uint32_t hi_row, uint32_t lo_row, int x)
+{
+    // v_perm_b32 selectors: 0-3 bytes of lo_row, 4-7 bytes of hi_row, 0x0C zero
+    const uint32_t sel = (uint32_t)x | (0x0Cu << 8) | ((uint32_t)(4 + x) << 16) | (0x0Cu << 24);
+    return as_s16x2(__builtin_amdgcn_perm(hi_row, lo_row, sel));
+}
 
 // Sum of |H4 D H4^T| over one 4x4 tile (before the final halving), or the SAD,
-// of current rows c[0..3] against the reference tile at `ref`.
+// of current rows c[0..3] against the reference tile at `ref`.  The Hadamard
+// runs on i16 pairs, two rows per register (|values| <= 2040 after both
+// passes).  Its last butterfly level is folded: |a + c| + |a - c| =
+// 2 max(|a|, |c|), so the sum is even and twice the sum of the pair maxima.
 __device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], gbytes ref, int64_t i_ref, bool hadamard)
 {
     uint32_t rr[4];
@@ -50,29 +63,34 @@ __device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], gbytes ref, in
             s = (int)__builtin_amdgcn_sad_u8(cr[y], rr[y], (uint32_t)s);
         return s;
     }
-    int d[4][4];
-#pragma unroll
-    for (int y = 0; y < 4; y++)
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-            d[y][x] = bytei(cr[y], x) - bytei(rr[y], x);
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-        const int a = d[y][0] + d[y][1], b = d[y][0] - d[y][1];
-        const int c = d[y][2] + d[y][3], e = d[y][2] - d[y][3];
-        d[y][0] = a + c;
-        d[y][1] = b + e;
-        d[y][2] = a - c;
-        d[y][3] = b - e;
-    }
-    int s = 0;
+    s16x2 d[8];
 #pragma unroll
     for (int x = 0; x < 4; x++) {
-        const int a = d[0][x] + d[1][x], b = d[0][x] - d[1][x];
-        const int c = d[2][x] + d[3][x], e = d[2][x] - d[3][x];
-        s += abs(a + c) + abs(b + e) + abs(a - c) + abs(b - e);
+        d[x] = byte_pair(cr[1], cr[0], x) - byte_pair(rr[1], rr[0], x);
+        d[4 + x] = byte_pair(cr[3], cr[2], x) - byte_pair(rr[3], rr[2], x);
     }
-    return s;
+    // horizontal pass, two rows at a time
+    s16x2 t[8];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const s16x2 *v = d + 4 * h;
+        const s16x2 a = v[0] + v[1], b = v[0] - v[1], c = v[2] + v[3], e = v[2] - v[3];
+        t[4 * h + 0] = a + c;
+        t[4 * h + 1] = b + e;
+        t[4 * h + 2] = a - c;
+        t[4 * h + 3] = b - e;
+    }
+    // vertical pass per column x: rows (0, 1) in t[x], rows (2, 3) in t[4 + x]
+    s16x2 acc = { 0, 0 };
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        const s16x2 s = t[x] + t[4 + x];        // (r0 + r2, r1 + r3)
+        const s16x2 df = t[x] - t[4 + x];       // (r0 - r2, r1 - r3)
+        const s16x2 lo = { s.x, df.x }, hi = { s.y, df.y };
+        const s16x2 mx = __builtin_elementwise_max(lo, hi), mn = __builtin_elementwise_min(lo, hi);
+        acc += __builtin_elementwise_max(mx, -mn);   // (max(|s.x|, |s.y|), max(|df.x|, |df.y|))
+    }
+    return 2 * ((int)acc.x + (int)acc.y);
 }
 
 struct RefineCtx {
@@ -206,6 +224,7 @@ __device__ __forceinline__ int cand_cost(const RefineCtx &c, int s, int mx, int 
 // a tile); the group's last lane completes the sum and the cost.  A larger PU
 // gives every lane P / 64 tiles of every candidate and sums the whole wave.
 // The current rows of a lane's tile are loaded once.
+template <bool LARGE>
 __device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start)
 {
     const int lane = threadIdx.x & 63;
@@ -214,7 +233,7 @@ __device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start)
     while (P < ntiles)
         P <<= 1;
     int m = 0x7fffffff;
-    if (P <= 64) {
+    if constexpr (!LARGE) {        // P <= 64 (the PU lists are split on this)
         const int groups = 64 / P;
         const int g = lane / P, tile = lane % P;
         const bool has_tile = tile < ntiles;
@@ -244,39 +263,40 @@ __device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start)
         }
         *start = __builtin_amdgcn_readlane(start_v, P - 1);   // k = 0: group 0 of the first pass
         return wave_min_key(m);
-    }
-    int acc[9];
-    int cmx[9], cmy[9];
-    bool cskip[9];
+    } else {
+        int acc[9];
+        int cmx[9], cmy[9];
+        bool cskip[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        acc[k] = 0;
-        cand_mv(st, k < n ? k : 0, cmx[k], cmy[k], cskip[k]);
-    }
-    for (int tile = lane; tile < ntiles; tile += 64) {
-        const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
-        uint32_t cr[4];
-#pragma unroll
-        for (int y = 0; y < 4; y++)
-            cr[y] = *(gdwords)(c.cur + (ty + y) * c.i_cur + tx);
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-            if (k < n && !cskip[k])
-                acc[k] += tile_cost(cr, cand_ref(c, cmx[k], cmy[k], ty, tx), c.i_ref, c.hadamard);
-    }
-    *start = 0x7fffffff;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        if (k < n && !cskip[k]) {          // uniform: the candidates do not depend on the lane
-            const int s = __builtin_amdgcn_readlane(group_sum(acc[k], 64), 63);
-            const int cost = cand_cost(c, s, cmx[k], cmy[k]);
-            if (k == 0)
-                *start = cost;
-            if (k >= st.c0)
-                m = min(m, pack_cost_idx(cost, k));
+        for (int k = 0; k < 9; k++) {
+            acc[k] = 0;
+            cand_mv(st, k < n ? k : 0, cmx[k], cmy[k], cskip[k]);
         }
+        for (int tile = lane; tile < ntiles; tile += 64) {
+            const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
+            uint32_t cr[4];
+#pragma unroll
+            for (int y = 0; y < 4; y++)
+                cr[y] = *(gdwords)(c.cur + (ty + y) * c.i_cur + tx);
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (k < n && !cskip[k])
+                    acc[k] += tile_cost(cr, cand_ref(c, cmx[k], cmy[k], ty, tx), c.i_ref, c.hadamard);
+        }
+        *start = 0x7fffffff;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (k < n && !cskip[k]) {          // uniform: the candidates do not depend on the lane
+                const int s = __builtin_amdgcn_readlane(group_sum(acc[k], 64), 63);
+                const int cost = cand_cost(c, s, cmx[k], cmy[k]);
+                if (k == 0)
+                    *start = cost;
+                if (k >= st.c0)
+                    m = min(m, pack_cost_idx(cost, k));
+            }
+        }
+        return m;
     }
-    return m;
 }
 
 constexpr int SP_WAVES = 4;
@@ -290,20 +310,24 @@ constexpr int SP_WAVES = 4;
 // the points of a stage in Spiral order; that is the first minimum among the
 // points below the stage's starting cost, i.e. the wave minimum of
 // (cost << 4 | k) compared against the starting cost.
+// LARGE: PUs of more than 64 4x4 tiles, a variant of its own so that the
+// small-PU variant (nearly every instance) needs few registers and keeps
+// more waves in flight to cover the latency of its plane reads.
+template <bool LARGE>
 __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                      const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
-                                                     const uint64_t *__restrict__ best, int n_inst)
+                                                     const uint64_t *__restrict__ best,
+                                                     const int32_t *__restrict__ list, int n_list)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // gridDim.x is a multiple of 8: workgroup b runs on XCD b % 8
     const int per_xcd = gridDim.x / 8;
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int logical = __builtin_amdgcn_readfirstlane(lb * SP_WAVES + wave);
-    if (logical >= n_inst)
+    if (logical >= n_list * p.n_ref)
         return;
-    const int n_pu = p.n_pu;
-    const int r = logical / n_pu;
-    const int pu_i = logical - r * n_pu;
+    const int r = logical / n_list;
+    const int pu_i = list[logical - r * n_list];
     const int id = pu_i * p.n_ref + r;
     const DevPu pu = pus[pu_i];
     const DevInst in = inst[id];
@@ -367,7 +391,7 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
         st.c0 = c.hadamard ? 1 : 0;
         int n = st.c0 + npts;
         int start = 0;
-        uint32_t m = (uint32_t)stage_min(c, st, n, &start);
+        uint32_t m = (uint32_t)stage_min<LARGE>(c, st, n, &start);
         if (c.hadamard)
             bcost = start;
         if ((int)(m >> 4) < bcost && m != 0x7fffffffu) {
@@ -384,7 +408,7 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
             st.by = by;
             st.c0 = 0;
             n = npts;
-            m = (uint32_t)stage_min(c, st, n, &start);
+            m = (uint32_t)stage_min<LARGE>(c, st, n, &start);
             if (m != 0x7fffffffu && (int)(m >> 4) < bcost) {
                 int mx, my;
                 bool skip;
@@ -406,15 +430,25 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
 
 } // namespace
 
+bool xme_subpel_large(int w, int h) { return (w >> 2) * (h >> 2) > 64; }
+
 int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
-                      const uint64_t *d_best, int n_inst, hipStream_t s)
+                      const uint64_t *d_best, const int32_t *d_list, int n_small, int n_large, hipStream_t s)
 {
-    if (n_inst <= 0)
-        return XME_OK;
     // a multiple of 8 workgroups (one range per XCD), SP_WAVES instances each
-    const int wgs = (n_inst + SP_WAVES - 1) / SP_WAVES;
-    const int grid = (wgs + 7) / 8 * 8;
-    hipLaunchKernelGGL(subpel_kernel, dim3(grid), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
-    XME_HIP(hipGetLastError());
+    auto grid_for = [&](int n_list) {
+        const int wgs = (n_list * p.n_ref + SP_WAVES - 1) / SP_WAVES;
+        return (wgs + 7) / 8 * 8;
+    };
+    if (n_small > 0) {
+        hipLaunchKernelGGL(subpel_kernel<false>, dim3(grid_for(n_small)), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+                           d_best, d_list, n_small);
+        XME_HIP(hipGetLastError());
+    }
+    if (n_large > 0) {
+        hipLaunchKernelGGL(subpel_kernel<true>, dim3(grid_for(n_large)), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+                           d_best, d_list + n_small, n_large);
+        XME_HIP(hipGetLastError());
+    }
     return XME_OK;
 }

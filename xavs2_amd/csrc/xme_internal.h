@@ -167,8 +167,12 @@ int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pu
 // LDS bytes of a search task beyond its staged window and current pixels
 // (per-wave MV-cost term tables of 4 waves x 8 x 4 x 64 dwords).
 #define XME_FS_TERM_BYTES (4 * 8 * 4 * 64 * 4)
+// Sub-pel refinement of every instance.  `d_list` holds PU indices: the
+// first n_small PUs have at most 64 4x4 tiles (xme_subpel_large false), the
+// next n_large more; each part gets a kernel variant of its own.
+bool xme_subpel_large(int w, int h);
 int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
-                      const uint64_t *d_best, int n_inst, hipStream_t s);
+                      const uint64_t *d_best, const int32_t *d_list, int n_small, int n_large, hipStream_t s);
 int xme_fs_max_lds(void);
 int xme_fs_prepare(void);
 // Device bytes a plan holds (work lists, per-instance state, descriptor).
