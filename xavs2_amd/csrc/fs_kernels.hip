@@ -431,8 +431,8 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
     }
 }
 
-// Short PUs (at most 16 rows, 8 or 16 pixels wide) keep their H x 4NQ
-// current pixels in registers, NQ = width / 4.  Each of the H + K - 1 window
+// Short PUs (at most 256 pixels) keep their H x 4NQ current pixels in
+// registers, NQ = width / 4.  Each of the H + K - 1 window
 // rows of the lane is loaded once, realigned to NQ + 1 dwords, and meets every
 // current row c = r - k with k in the lane's K candidate rows (a ring of K
 // window rows would need K x 2NQ registers).  Window rows are loaded PF rows
@@ -441,7 +441,7 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
 template <int NQ, bool ALIGNED>
 struct ShortRows {
     static constexpr int NW = ALIGNED ? NQ + 1 : NQ + 2;   // dwords loaded per window row
-    static constexpr int PF = NQ == 2 ? 4 : 2;              // rows in flight
+    static constexpr int PF = NQ <= 2 ? 4 : NQ <= 4 ? 2 : 1;   // rows in flight
 };
 
 template <int H, int NQ, bool ALIGNED, int r>
@@ -642,12 +642,12 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     }
 }
 
-// 16-pixel-wide PUs of at most 16 rows use the same 16 candidate rows per
-// lane.  Their SAD alone fills the u16 lanes (16 * 16 * 255 < 2^16), so the MV
-// costs (packed u16, cached in the wave's LDS term area like the narrow path's)
-// are added in 32 bits in the epilogue.
-template <int H>
-__device__ __forceinline__ void fs_item_wide16(const DevParams &p, const DevPu &pu, const DevInst &in,
+// The other PUs of at most 256 pixels (4, 12, 16 or 32 wide; 8 x 32) use the
+// same 16 candidate rows per lane.  Their SAD alone may fill the u16 lanes
+// (256 * 255 < 2^16), so the MV costs (packed u16, cached in the wave's LDS
+// term area like the narrow path's) are added in 32 bits in the epilogue.
+template <int H, int NQ>
+__device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu, const DevInst &in,
                                                const FsItem &it, int inst_id, const uint8_t *__restrict__ win,
                                                int pitch, int wx0, int wy0, const uint8_t *__restrict__ cur,
                                                int cpitch, int cx0, int cy0, uint64_t *__restrict__ best,
@@ -692,9 +692,9 @@ __device__ __forceinline__ void fs_item_wide16(const DevParams &p, const DevPu &
     }
     const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
     if ((lane_base & 3) == 0)      // uniform: col and the pitch are multiples of 4
-        short_strip<H, 4, true>(win, pitch, lane_base, cblk, cpitch, a16);
+        short_strip<H, NQ, true>(win, pitch, lane_base, cblk, cpitch, a16);
     else
-        short_strip<H, 4, false>(win, pitch, lane_base, cblk, cpitch, a16);
+        short_strip<H, NQ, false>(win, pitch, lane_base, cblk, cpitch, a16);
 
     const MvWindow mw = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
     const int col_end = it.col0 + it.ncols, row_end = it.row0 + it.nrows;
@@ -883,28 +883,30 @@ __global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p,
                                              __builtin_amdgcn_readlane(my_pu.z, g), __builtin_amdgcn_readlane(my_pu.w, g));
                 const DevPu pu = __builtin_bit_cast(DevPu, vpu);
                 if constexpr (KMAX == XME_NARROW_K) {
-                    if (pu.w == 16) {
-                        if (pu.h == 16)
-                            fs_item_wide16<16>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
-                                               t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
-                        else if (pu.h == 12)
-                            fs_item_wide16<12>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
-                                               t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
-                        else if (pu.h == 8)
-                            fs_item_wide16<8>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
-                                              t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
-                        else
-                            fs_item_wide16<4>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0,
-                                              t.cy0, best, const_cast<int32_t *>(cost_map), term, ts);
-                    } else if (pu.h == 16)
-                        fs_item_narrow<16>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
-                                           best, const_cast<int32_t *>(cost_map), term, ts);
-                    else if (pu.h == 8)
-                        fs_item_narrow<8>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
-                                          best, const_cast<int32_t *>(cost_map), term, ts);
-                    else
-                        fs_item_narrow<4>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0,
-                                          best, const_cast<int32_t *>(cost_map), term, ts);
+#define XME_WIDE(HH, NQQ)                                                                                     \
+    fs_item_wide<HH, NQQ>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,   \
+                          const_cast<int32_t *>(cost_map), term, ts)
+#define XME_NARROW(HH)                                                                                        \
+    fs_item_narrow<HH>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,      \
+                       const_cast<int32_t *>(cost_map), term, ts)
+                    // the shapes of xme_narrow_shape (host side)
+                    switch (pu.w * 64 + pu.h) {
+                    case 8 * 64 + 16: XME_NARROW(16); break;
+                    case 8 * 64 + 8: XME_NARROW(8); break;
+                    case 8 * 64 + 4: XME_NARROW(4); break;
+                    case 16 * 64 + 16: XME_WIDE(16, 4); break;
+                    case 16 * 64 + 12: XME_WIDE(12, 4); break;
+                    case 16 * 64 + 8: XME_WIDE(8, 4); break;
+                    case 16 * 64 + 4: XME_WIDE(4, 4); break;
+                    case 32 * 64 + 8: XME_WIDE(8, 8); break;
+                    case 8 * 64 + 32: XME_WIDE(32, 2); break;
+                    case 12 * 64 + 16: XME_WIDE(16, 3); break;
+                    case 4 * 64 + 16: XME_WIDE(16, 1); break;
+                    case 4 * 64 + 8: XME_WIDE(8, 1); break;
+                    default: XME_WIDE(4, 1); break;   // 4 x 4
+                    }
+#undef XME_WIDE
+#undef XME_NARROW
                 } else if (KMAX == 8 && it.K == 8) {
                     fs_item<8>(p, pu, in, it, inst_id, lds, pitch, t.wx0, t.wy0, cur, cpitch, t.cx0, t.cy0, best,
                                const_cast<int32_t *>(cost_map), term, ts);

@@ -145,7 +145,7 @@ int xme_part_index(int w, int h)
 }
 
 int xme_pu_w(int part) { return kPuW[part]; }
-bool xme_narrow_shape(int part) { return (kPuW[part] == 8 || kPuW[part] == 16) && kPuH[part] <= 16; }
+bool xme_narrow_shape(int part) { return kPuW[part] * kPuH[part] <= 256; }
 int xme_pu_h(int part) { return kPuH[part]; }
 
 // Largest |v| of the reference's mvbits table (me.c:592-612).

@@ -15,8 +15,8 @@ void xme_tune_store(int part, int search_range, TuneChoice c);
 void xme_tune_force(const TuneChoice *c);
 
 int xme_part_index(int w, int h);
-// 8 or 16 pixels wide and at most 16 rows: eligible for 16 candidate rows per
-// lane (the short-PU kernel variant).
+// At most 256 pixels (every shape up to 16 x 16, 8 x 32 and 32 x 8):
+// eligible for 16 candidate rows per lane (the short-PU kernel variant).
 bool xme_narrow_shape(int part);
 int xme_pu_w(int part);
 int xme_pu_h(int part);
