@@ -335,7 +335,8 @@ static void items_for(const DevParams &p, const std::vector<DevPu> &pus, const s
 }
 
 // Split instances of one shape and MVP into groups whose items carry similar
-// work (about 1024 pixels of PU area per item).
+// work (about 4096 pixels of PU area per item: enough calls per item to hide
+// the latency of fetching the next item's instances).
 static std::vector<std::vector<int>> make_groups(const std::vector<DevPu> &pus, const std::vector<int> &insts,
                                                  int n_ref, int max_group)
 {
@@ -347,7 +348,7 @@ static std::vector<std::vector<int>> make_groups(const std::vector<DevPu> &pus, 
     std::vector<std::vector<int>> out;
     for (auto &kv : by_key) {
         const DevPu &pu = pus[kv.second[0] / n_ref];
-        const int g = std::max(1, std::min(max_group, 1024 / (pu.w * pu.h)));
+        const int g = std::max(1, std::min(max_group, 4096 / (pu.w * pu.h)));
         for (size_t i = 0; i < kv.second.size(); i += g)
             out.emplace_back(kv.second.begin() + i, kv.second.begin() + std::min(kv.second.size(), i + g));
     }

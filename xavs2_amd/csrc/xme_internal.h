@@ -72,7 +72,9 @@ struct FsItem {
     uint8_t K, LC;             // rows per lane, lanes per candidate row
     uint8_t pad[2];
 };
-#define XME_MAX_GROUP 16
+// Instances of one item: one per lane when the search kernel fetches them.
+#define XME_MAX_GROUP 64
+static_assert(XME_MAX_GROUP <= 64, "an item's instances are fetched one per lane");
 
 // Order of a candidate in xavs2_me_search: the MV candidate list comes first
 // (me.c:913-917), then the raster rows j, columns i (me.c:1159-1163).
