@@ -16,6 +16,7 @@
 //  * the (cost, order) key of the best legal candidate of a wave goes to the
 //    per-(PU, ref) slot with a 64-bit atomic min, which reproduces the
 //    first-strict-minimum rule of the reference exactly.
+#include "xme_dpp.h"
 #include "xme_internal.h"
 #include "tune.h"
 
@@ -58,31 +59,46 @@ __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *
             n++;
         }
     }
-    const uint8_t *cur = desc->cur + (int64_t)pu.y * desc->i_cur + pu.x;
-    const uint8_t *ref = desc->planes[r][0] + (int64_t)pu.y * desc->stride[r] + pu.x;
+    // 4 pixels per lane step: the current rows are dword aligned (PU x and
+    // the stride are multiples of 4); reference dwords are realigned, the
+    // extra dword read stays inside the plane's padding.  Both candidates are
+    // summed in one pass (their loads in flight together), then the costs are
+    // compared in list order.
+    typedef const __attribute__((address_space(1))) uint8_t *gbytes;
+    typedef const __attribute__((address_space(1))) uint32_t *gdwords;
+    const gbytes cur = (gbytes)desc->cur + (int64_t)pu.y * desc->i_cur + pu.x;
+    const gbytes ref = (gbytes)desc->planes[r][0] + (int64_t)pu.y * desc->stride[r] + pu.x;
     const int64_t i_ref = desc->stride[r];
+    bool ok[2];
+    gbytes rp[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        ok[c] = c < n && xme_mv_legal(win, cand[c][0], cand[c][1]);
+        rp[c] = ref + (int64_t)(ok[c] ? cand[c][1] : 0) * i_ref + (ok[c] ? cand[c][0] : 0);
+    }
+    uint32_t sad[2] = { 0, 0 };
+    const int gw = pu.w >> 2;
+    for (int t = lane; t < gw * pu.h; t += 64) {
+        const int yy = t / gw, xx = (t - yy * gw) * 4;
+        const uint32_t c4 = *(gdwords)(cur + (int64_t)yy * desc->i_cur + xx);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            if (ok[c]) {     // uniform
+                const uintptr_t a = (uintptr_t)(rp[c] + (int64_t)yy * i_ref + xx);
+                const gdwords q4 = (gdwords)(a & ~(uintptr_t)3);
+                sad[c] = __builtin_amdgcn_sad_u8(c4, __builtin_amdgcn_alignbyte(q4[1], q4[0], (uint32_t)(a & 3)), sad[c]);
+            }
+        }
+    }
     uint64_t key = XME_KEY_NONE;
     int bx = 0, by = 0;
-    for (int c = 0; c < n; c++) {
-        const int mx = cand[c][0], my = cand[c][1];
-        if (!xme_mv_legal(win, mx, my))
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        if (!ok[c])
             continue;
-        // 4 pixels per lane step: the current rows are dword aligned (PU x and
-        // the stride are multiples of 4); reference dwords are realigned, the
-        // extra dword read stays inside the plane's padding
-        uint32_t sad = 0;
-        const uint8_t *rp = ref + (int64_t)my * i_ref + mx;
-        const int gw = pu.w >> 2;
-        for (int t = lane; t < gw * pu.h; t += 64) {
-            const int yy = t / gw, xx = (t - yy * gw) * 4;
-            const uint32_t c4 = *reinterpret_cast<const uint32_t *>(cur + (int64_t)yy * desc->i_cur + xx);
-            const uintptr_t a = reinterpret_cast<uintptr_t>(rp + (int64_t)yy * i_ref + xx);
-            const uint32_t *q4 = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
-            sad = __builtin_amdgcn_sad_u8(c4, __builtin_amdgcn_alignbyte(q4[1], q4[0], (uint32_t)(a & 3)), sad);
-        }
-        for (int o = 32; o > 0; o >>= 1)
-            sad += (uint32_t)__shfl_xor((int)sad, o);
-        const int cost = (int)sad + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+        const int mx = cand[c][0], my = cand[c][1];
+        const int cost = wave_sum((int)sad[c]) +
+                         xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
         const uint64_t k = xme_key((uint32_t)cost, (uint32_t)c);
         if (k < key) {
             key = k;
@@ -113,13 +129,6 @@ __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *
 __device__ __forceinline__ uint32_t align_dw(uint32_t hi, uint32_t lo, uint32_t shift)
 {
     return __builtin_amdgcn_alignbyte(hi, lo, shift);
-}
-
-// qsad: 4 SADs of the 4 current bytes `c` against the reference bytes
-// r[d .. d+3], d = 0..3, added to four 16-bit accumulators.
-__device__ __forceinline__ uint64_t qsad(uint32_t lo, uint32_t hi, uint32_t c, uint64_t acc)
-{
-    return __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)hi << 32) | lo, c, acc);
 }
 
 // Accumulate one strip of SW current columns over all H current rows.
@@ -855,7 +864,7 @@ __global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p,
             int it_i = 0;
             if ((threadIdx.x & 63) == 0)
                 it_i = atomicAdd(&s_item, 1);
-            it_i = __shfl(it_i, 0);
+            it_i = __builtin_amdgcn_readlane(it_i, 0);
             if (it_i >= t.item_end)
                 break;
             const FsItem it = items[it_i];
