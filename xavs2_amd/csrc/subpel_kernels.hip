@@ -170,16 +170,18 @@ __device__ __forceinline__ int cand_cost(const RefineCtx &c, int s, int mx, int 
     return dist + xme_weighted_cost(c.lambda, xme_mvbits(mx - c.pmx) + xme_mvbits(my - c.pmy));
 }
 
-// Evaluates the stage's n <= 9 candidates and returns the wave minimum of
+// Evaluates the stage's n <= 9 candidates and returns the minimum of
 // (cost << 4 | k) over the candidates k >= st.c0 that the reference
 // evaluates (0x7fffffff when none); *start gets the cost of candidate 0.
 // With P = the PU's 4x4 tile count rounded up to a power of two, a PU of at
-// most 64 tiles gives each group of P lanes one candidate at a time (a lane =
-// a tile); the group's last lane completes the sum and the cost.  A larger PU
-// gives every lane P / 64 tiles of every candidate and sums the whole wave.
-// The current rows of a lane's tile are loaded once.
+// most 64 tiles is one wave's: each group of P lanes takes one candidate at a
+// time (a lane = a tile) and the group's last lane completes the sum and the
+// cost.  A larger PU (LARGE) is the whole workgroup's: a lane = a tile of
+// 4 waves x 64, each wave sums its part of every candidate and the parts meet
+// in LDS (`part`, 4 x 16 ints).  The current rows of a lane's tile are loaded
+// once.
 template <bool LARGE>
-__device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start)
+__device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start, int *part)
 {
     const int lane = threadIdx.x & 63;
     const int tw = c.w >> 2, ntiles = tw * (c.h >> 2);
@@ -218,77 +220,72 @@ __device__ int stage_min(const RefineCtx &c, const Stage &st, int n, int *start)
         *start = __builtin_amdgcn_readlane(start_v, P - 1);   // k = 0: group 0 of the first pass
         return wave_min_key(m);
     } else {
-        int acc[9];
-        int cmx[9], cmy[9];
-        bool cskip[9];
+        const int wave = threadIdx.x >> 6;
+        const int tile = wave * 64 + lane;
+        const bool has_tile = tile < ntiles;
+        const int ty = has_tile ? (tile / tw) * 4 : 0, tx = has_tile ? (tile % tw) * 4 : 0;
+        uint32_t cr[4];
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            acc[k] = 0;
-            cand_mv(st, k < n ? k : 0, cmx[k], cmy[k], cskip[k]);
+        for (int y = 0; y < 4; y++)
+            cr[y] = has_tile ? *(gdwords)(c.cur + (ty + y) * c.i_cur + tx) : 0u;
+        for (int k = 0; k < n; k++) {
+            int mx, my;
+            bool skip;
+            cand_mv(st, k, mx, my, skip);
+            if (skip)                  // uniform: the candidates do not depend on the lane
+                continue;
+            int v = 0;
+            if (has_tile)
+                v = tile_cost(cr, cand_ref(c, mx, my, ty, tx), c.i_ref, c.hadamard);
+            v = wave_sum(v);
+            if (lane == 0)
+                part[wave * 16 + k] = v;
         }
-        for (int tile = lane; tile < ntiles; tile += 64) {
-            const int ty = (tile / tw) * 4, tx = (tile % tw) * 4;
-            uint32_t cr[4];
-#pragma unroll
-            for (int y = 0; y < 4; y++)
-                cr[y] = *(gdwords)(c.cur + (ty + y) * c.i_cur + tx);
-#pragma unroll
-            for (int k = 0; k < 9; k++)
-                if (k < n && !cskip[k])
-                    acc[k] += tile_cost(cr, cand_ref(c, cmx[k], cmy[k], ty, tx), c.i_ref, c.hadamard);
-        }
+        __syncthreads();
         *start = 0x7fffffff;
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            if (k < n && !cskip[k]) {          // uniform: the candidates do not depend on the lane
-                const int s = __builtin_amdgcn_readlane(group_sum(acc[k], 64), 63);
-                const int cost = cand_cost(c, s, cmx[k], cmy[k]);
-                if (k == 0)
-                    *start = cost;
-                if (k >= st.c0)
-                    m = min(m, pack_cost_idx(cost, k));
-            }
+        for (int k = 0; k < n; k++) {
+            int mx, my;
+            bool skip;
+            cand_mv(st, k, mx, my, skip);
+            if (skip)
+                continue;
+            const int cost = cand_cost(c, part[k] + part[16 + k] + part[32 + k] + part[48 + k], mx, my);
+            if (k == 0)
+                *start = cost;
+            if (k >= st.c0)
+                m = min(m, pack_cost_idx(cost, k));
         }
+        __syncthreads();              // `part` is written again by the next stage
         return m;
     }
 }
 
 constexpr int SP_WAVES = 4;
 
-// One wave per (PU, reference).  Waves walk the instances reference-major
-// (neighbouring PUs of one reference together) and each XCD takes a
-// contiguous range of workgroups, so the L2 of an XCD sees the plane lines of
-// neighbouring PUs back to back.
+// One workgroup of 4 waves takes either one instance of a PU of more than 64
+// tiles (the first n_large * n_ref workgroups, so the long instances start
+// first) or 4 instances of smaller PUs, one per wave.  Instances are walked
+// reference-major (neighbouring PUs of one reference together) and each XCD
+// takes a contiguous range of workgroups, so the L2 of an XCD sees the plane
+// lines of neighbouring PUs back to back.
 //
 // The reference updates the best point sequentially with a strict '<' over
 // the points of a stage in Spiral order; that is the first minimum among the
-// points below the stage's starting cost, i.e. the wave minimum of
+// points below the stage's starting cost, i.e. the minimum of
 // (cost << 4 | k) compared against the starting cost.
-// LARGE: PUs of more than 64 4x4 tiles, a variant of its own so that the
-// small-PU variant (nearly every instance) needs few registers and keeps
-// more waves in flight to cover the latency of its plane reads.
 template <bool LARGE>
-__global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDesc *__restrict__ desc,
-                                                     const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
-                                                     const uint64_t *__restrict__ best,
-                                                     const int32_t *__restrict__ list, int n_list)
+__device__ void refine(const DevParams &p, const FrameDesc *__restrict__ desc, const DevPu *__restrict__ pus,
+                       const DevInst *__restrict__ inst, const uint64_t *__restrict__ best, int pu_i, int r, int *part)
 {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // gridDim.x is a multiple of 8: workgroup b runs on XCD b % 8
-    const int per_xcd = gridDim.x / 8;
-    const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const int logical = __builtin_amdgcn_readfirstlane(lb * SP_WAVES + wave);
-    if (logical >= n_list * p.n_ref)
-        return;
-    const int r = logical / n_list;
-    const int pu_i = list[logical - r * n_list];
+    const int lane = threadIdx.x & 63;
+    const bool leader = LARGE ? threadIdx.x == 0 : lane == 0;
     const int id = pu_i * p.n_ref + r;
     const DevPu pu = pus[pu_i];
     const DevInst in = inst[id];
     const uint64_t key = best[id];
     xme_result res;
-    if (in.status || key >= XME_KEY_NONE) {
-        if (lane == 0) {
+    if (in.status || key >= XME_KEY_NONE) {       // uniform over the instance's waves
+        if (leader) {
             // _me_error (me.c:919-921, :1181): only the return value is defined
             res.cost = XME_MAX_DISTORTION;
             res.bcost2 = 0;
@@ -345,7 +342,7 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
         st.c0 = c.hadamard ? 1 : 0;
         int n = st.c0 + npts;
         int start = 0;
-        uint32_t m = (uint32_t)stage_min<LARGE>(c, st, n, &start);
+        uint32_t m = (uint32_t)stage_min<LARGE>(c, st, n, &start, part);
         if (c.hadamard)
             bcost = start;
         if ((int)(m >> 4) < bcost && m != 0x7fffffffu) {
@@ -362,7 +359,7 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
             st.by = by;
             st.c0 = 0;
             n = npts;
-            m = (uint32_t)stage_min<LARGE>(c, st, n, &start);
+            m = (uint32_t)stage_min<LARGE>(c, st, n, &start, part);
             if (m != 0x7fffffffu && (int)(m >> 4) < bcost) {
                 int mx, my;
                 bool skip;
@@ -373,13 +370,42 @@ __global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDes
             }
         }
     }
-    if (lane == 0) {
+    if (leader) {
         res.cost = bcost;
         res.bmv_x = (int16_t)bx;
         res.bmv_y = (int16_t)by;
         res.mvcost = xme_weighted_cost(p.lambda, xme_mvbits(bx - pu.mvp_x) + xme_mvbits(by - pu.mvp_y));
         desc->results[id] = res;
     }
+}
+
+__global__ void __launch_bounds__(256) subpel_kernel(DevParams p, const FrameDesc *__restrict__ desc,
+                                                     const DevPu *__restrict__ pus, const DevInst *__restrict__ inst,
+                                                     const uint64_t *__restrict__ best,
+                                                     const int32_t *__restrict__ list, int n_small, int n_large)
+{
+    __shared__ int s_part[4 * 16];
+    // Workgroup b runs on XCD b % 8.  The large instances take the first
+    // blocks in dispatch order (spread over the XCDs), padded to a multiple of
+    // 8; the small ones then map to contiguous ranges per XCD.
+    const int n_large_wg = n_large * p.n_ref;
+    const int large_blocks = (n_large_wg + 7) / 8 * 8;
+    const int b = (int)blockIdx.x;
+    if (b < large_blocks) {
+        if (b < n_large_wg) {
+            const int r = b / n_large;
+            refine<true>(p, desc, pus, inst, best, list[n_small + b - r * n_large], r, s_part);
+        }
+        return;
+    }
+    const int bs = b - large_blocks;                     // bs % 8 == b % 8
+    const int per_xcd = ((int)gridDim.x - large_blocks) / 8;
+    const int lb = (bs % 8) * per_xcd + bs / 8;
+    const int logical = __builtin_amdgcn_readfirstlane(lb * SP_WAVES + (int)(threadIdx.x >> 6));
+    if (logical >= n_small * p.n_ref)
+        return;
+    const int r = logical / n_small;
+    refine<false>(p, desc, pus, inst, best, list[logical - r * n_small], r, nullptr);
 }
 
 } // namespace
@@ -389,20 +415,16 @@ bool xme_subpel_large(int w, int h) { return (w >> 2) * (h >> 2) > 64; }
 int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
                       const uint64_t *d_best, const int32_t *d_list, int n_small, int n_large, hipStream_t s)
 {
-    // a multiple of 8 workgroups (one range per XCD), SP_WAVES instances each
-    auto grid_for = [&](int n_list) {
-        const int wgs = (n_list * p.n_ref + SP_WAVES - 1) / SP_WAVES;
-        return (wgs + 7) / 8 * 8;
-    };
-    if (n_small > 0) {
-        hipLaunchKernelGGL(subpel_kernel<false>, dim3(grid_for(n_small)), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
-                           d_best, d_list, n_small);
-        XME_HIP(hipGetLastError());
-    }
-    if (n_large > 0) {
-        hipLaunchKernelGGL(subpel_kernel<true>, dim3(grid_for(n_large)), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
-                           d_best, d_list + n_small, n_large);
-        XME_HIP(hipGetLastError());
-    }
+    // n_large * n_ref workgroups for the large PUs (rounded up to a multiple
+    // of 8), then SP_WAVES small instances per workgroup (a multiple of 8
+    // workgroups: one range per XCD)
+    const int large_blocks = (n_large * p.n_ref + 7) / 8 * 8;
+    const int small_blocks = ((n_small * p.n_ref + SP_WAVES - 1) / SP_WAVES + 7) / 8 * 8;
+    const int grid = large_blocks + small_blocks;
+    if (grid <= 0)
+        return XME_OK;
+    hipLaunchKernelGGL(subpel_kernel, dim3(grid), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best, d_list, n_small,
+                       n_large);
+    XME_HIP(hipGetLastError());
     return XME_OK;
 }
