@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # default workload (3840x2160, 4 refs, R=64, shipped tile table), measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
 # reading for a 256 MiB copy kernel (FETCH x2.000, WRITE x1.000):
-# (245,592.1 + 187,595.6) KiB fetched -> 887.2 MB, (15,948.1 + 19,270.3) KiB
-# written -> 36.1 MB.  The counters cannot be read from inside this process, so
+# (104,490.1 + 182,787.8) KiB fetched -> 588.3 MB, (1,803.9 + 26,294.3) KiB
+# written -> 28.8 MB.  The counters cannot be read from inside this process, so
 # the figure is quoted for the workload it was measured on and null for any other.
-MEASURED_TRAFFIC = {(3840, 2160, 4, 64): (245592.1 + 187595.6) * 1024 * 2.0 + (15948.1 + 19270.3) * 1024 * 1.0}
+MEASURED_TRAFFIC = {(3840, 2160, 4, 64): (104490.1 + 182787.8) * 1024 * 2.0 + (1803.9 + 26294.3) * 1024 * 1.0}
 
 
 def log(msg):
@@ -462,7 +462,7 @@ def main():
             "frac": achieved / VALU_PEAK_TABSDIFF,
             "traffic": MEASURED_TRAFFIC.get((w, h, n_ref, R)) if not (args.no_tune or args.retune) else None,
             "traffic_unit": "HBM bytes per search step (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "kernel": "integer full search: fs_kernel<true,8> + fs_kernel<true,16> (narrow PUs), both persistent",
+            "kernel": "integer full search: fs_kernel<true,8> (generic PUs) + fs_kernel<true,16> (PUs of <= 256 px), both persistent",
             "kernel_ms": search_ms,
         },
         "plain": {"value": plain_value, "ms_per_step": plain_ms, "results_equal": plain_equal},

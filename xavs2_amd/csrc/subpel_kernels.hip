@@ -34,23 +34,11 @@ __device__ __forceinline__ uint32_t load_unaligned_dw(gbytes p)
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
-
-// Byte x of two rows as an i16 pair (lo: row `lo_row`, hi: row `hi_row`).
-__device__ __forceinline__ s16x2 byte_pair(uint32_t hi_row, uint32_t lo_row, int x)
-{
-    // v_perm_b32 selectors: 0-3 bytes of lo_row, 4-7 bytes of hi_row, 0x0C zero
-    const uint32_t sel = (uint32_t)x | (0x0Cu << 8) | ((uint32_t)(4 + x) << 16) | (0x0Cu << 24);
-    return as_s16x2(__builtin_amdgcn_perm(hi_row, lo_row, sel));
-}
+__device__ __forceinline__ int bytei(uint32_t v, int i) { return (int)((v >> (8 * i)) & 0xFFu); }
 
 // Sum of |H4 D H4^T| over one 4x4 tile (before the final halving), or the SAD,
-// of current rows c[0..3] against the reference tile at `ref`.  The Hadamard
-// runs on i16 pairs, two rows per register (|values| <= 2040 after both
-// passes).  Its last butterfly level is folded: |a + c| + |a - c| =
-// 2 max(|a|, |c|), so the sum is even and twice the sum of the pair maxima.
+// of current rows c[0..3] against the reference tile at `ref`.  32-bit
+// arithmetic throughout: packed 16-bit integer forms are avoided on this target.
 __device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], gbytes ref, int64_t i_ref, bool hadamard)
 {
     uint32_t rr[4];
@@ -64,34 +52,29 @@ __device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], gbytes ref, in
             s = (int)__builtin_amdgcn_sad_u8(cr[y], rr[y], (uint32_t)s);
         return s;
     }
-    s16x2 d[8];
+    int d[4][4];
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+            d[y][x] = bytei(cr[y], x) - bytei(rr[y], x);
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const int a = d[y][0] + d[y][1], b = d[y][0] - d[y][1];
+        const int c = d[y][2] + d[y][3], e = d[y][2] - d[y][3];
+        d[y][0] = a + c;
+        d[y][1] = b + e;
+        d[y][2] = a - c;
+        d[y][3] = b - e;
+    }
+    int s = 0;
 #pragma unroll
     for (int x = 0; x < 4; x++) {
-        d[x] = byte_pair(cr[1], cr[0], x) - byte_pair(rr[1], rr[0], x);
-        d[4 + x] = byte_pair(cr[3], cr[2], x) - byte_pair(rr[3], rr[2], x);
+        const int a = d[0][x] + d[1][x], b = d[0][x] - d[1][x];
+        const int c = d[2][x] + d[3][x], e = d[2][x] - d[3][x];
+        s += abs(a + c) + abs(b + e) + abs(a - c) + abs(b - e);
     }
-    // horizontal pass, two rows at a time
-    s16x2 t[8];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const s16x2 *v = d + 4 * h;
-        const s16x2 a = v[0] + v[1], b = v[0] - v[1], c = v[2] + v[3], e = v[2] - v[3];
-        t[4 * h + 0] = a + c;
-        t[4 * h + 1] = b + e;
-        t[4 * h + 2] = a - c;
-        t[4 * h + 3] = b - e;
-    }
-    // vertical pass per column x: rows (0, 1) in t[x], rows (2, 3) in t[4 + x]
-    s16x2 acc = { 0, 0 };
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-        const s16x2 s = t[x] + t[4 + x];        // (r0 + r2, r1 + r3)
-        const s16x2 df = t[x] - t[4 + x];       // (r0 - r2, r1 - r3)
-        const s16x2 lo = { s.x, df.x }, hi = { s.y, df.y };
-        const s16x2 mx = __builtin_elementwise_max(lo, hi), mn = __builtin_elementwise_min(lo, hi);
-        acc += __builtin_elementwise_max(mx, -mn);   // (max(|s.x|, |s.y|), max(|df.x|, |df.y|))
-    }
-    return 2 * ((int)acc.x + (int)acc.y);
+    return s;
 }
 
 struct RefineCtx {
