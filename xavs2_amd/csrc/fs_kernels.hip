@@ -276,6 +276,21 @@ __device__ __forceinline__ void fs_strip_aligned(const uint8_t *__restrict__ win
     }
 }
 
+// A lane's first candidate column and row in an item: 4 columns x K rows at
+// its place in the band, pulled back inside the item when the band is cut
+// short (the last band of 2R, or a window split into row ranges).  Such a lane
+// then repeats candidates of another lane, which gives the same keys, and its
+// reads stay inside the item's window footprint (item_footprint, host side).
+__device__ __forceinline__ int lane_col(const FsItem &it, int lc)
+{
+    return min(it.col0 + 4 * lc, it.col0 + max((int)it.ncols, 4) - 4);
+}
+
+__device__ __forceinline__ int lane_row(const FsItem &it, int rg, int K)
+{
+    return min(it.row0 + rg * K, it.row0 + max((int)it.nrows, K) - K);
+}
+
 // Minimum over the 64 lanes, returned to every lane: DPP inside each row of
 // 16 lanes (quad swaps, then row rotations by 4 and 8), then the four rows.
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
@@ -313,8 +328,8 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
     // MV (omx - R, omy - R)
     const int bx = pu.x + in.omx - R - wx0;
     const int by = pu.y + in.omy - R - wy0;
-    const int col = it.col0 + 4 * lc;
-    const int row = it.row0 + rg * K;
+    const int col = lane_col(it, lc);
+    const int row = lane_row(it, rg, K);
     const int lane_base = (by + row) * pitch + bx + col;
 
     uint32_t acc[K][4];
@@ -412,7 +427,7 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
     // wave minimum of (cost << 11 | order inside the band): the band's
     // candidates in raster order are (local row) * 4 LC + local column < 256 K
     // <= 2^11, and a cost is below 2^21 (SAD <= 64 * 64 * 255, MV cost < 2^14)
-    const int lrow = rg * K + (bidx >> 2), lcol = 4 * lc + (bidx & 3);
+    const int lrow = row - it.row0 + (bidx >> 2), lcol = col - it.col0 + (bidx & 3);
     uint32_t lkey = bcost == 0xFFFFFFFFu ? 0xFFFFFFFFu : (bcost << 11) | (uint32_t)(lrow * 4 * LC + lcol);
     lkey = wave_min_u32(lkey);
     if (lkey != 0xFFFFFFFFu && lane == 0) {
@@ -548,8 +563,8 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     const int R = p.R, R2 = 2 * R;
     const int bx = pu.x + in.omx - R - wx0;
     const int by = pu.y + in.omy - R - wy0;
-    const int col = it.col0 + 4 * lc;
-    const int row = it.row0 + rg * K;
+    const int col = lane_col(it, lc);
+    const int row = lane_row(it, rg, K);
     const int lane_base = (by + row) * pitch + bx + col;
     uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]
     if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
@@ -623,7 +638,7 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     uint32_t lkey = 0xFFFFFFFFu;
     if (m != 0xFFFFFFFFu) {
         const int idx = (int)(m & 63u);
-        const int lrow = rg * K + (idx >> 2), lcol = 4 * lc + (idx & 3);
+        const int lrow = row - it.row0 + (idx >> 2), lcol = col - it.col0 + (idx & 3);
         lkey = ((m >> 16) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
     }
     lkey = wave_min_u32(lkey);
@@ -670,8 +685,8 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     const int R = p.R, R2 = 2 * R;
     const int bx = pu.x + in.omx - R - wx0;
     const int by = pu.y + in.omy - R - wy0;
-    const int col = it.col0 + 4 * lc;
-    const int row = it.row0 + rg * K;
+    const int col = lane_col(it, lc);
+    const int row = lane_row(it, rg, K);
     const int lane_base = (by + row) * pitch + bx + col;
     uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]: packed u16 MV costs
     if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
@@ -745,7 +760,7 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     uint32_t lkey = 0xFFFFFFFFu;
     if (m != 0xFFFFFFFFu) {
         const int idx = (int)(m & 63u);
-        const int lrow = rg * K + (idx >> 2), lcol = 4 * lc + (idx & 3);
+        const int lrow = row - it.row0 + (idx >> 2), lcol = col - it.col0 + (idx & 3);
         lkey = ((m >> 6) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
     }
     lkey = wave_min_u32(lkey);

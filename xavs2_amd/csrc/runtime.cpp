@@ -250,16 +250,18 @@ static int possible_centres(const DevParams &p, const DevPu &pu, int c[2][2])
 }
 
 // LDS read footprint of one item (picture coordinates), see fs_strip.
+// Window bytes an item's lanes read at search centre (cx, cy): the item's
+// candidate columns (at least 4) and rows (at least K; lanes beyond them are
+// pulled back inside, see lane_col/lane_row in fs_kernels.hip), plus the PU.
 static Rect item_footprint(const DevParams &p, const DevPu &pu, const FsItem &it, int cx, int cy)
 {
     const int R = p.R;
-    const int rg = 64 / it.LC;
     Rect r;
     const int bx = pu.x + cx - R, by = pu.y + cy - R;
     r.x0 = bx + it.col0 - 3;
-    r.x1 = bx + it.col0 + 4 * it.LC + pu.w + 8;
+    r.x1 = bx + it.col0 + std::max((int)it.ncols, 4) + pu.w + 8;
     r.y0 = by + it.row0;
-    r.y1 = by + it.row0 + rg * it.K + pu.h - 1;
+    r.y1 = by + it.row0 + std::max((int)it.nrows, (int)it.K) + pu.h - 1;
     return r;
 }
 
@@ -313,7 +315,11 @@ static void items_for(const DevParams &p, const std::vector<DevPu> &pus, const s
         const int Kt = std::min(std::min(tc.K, 8), xme_fs_kmax());
         K = (pu.h % Kt == 0) ? Kt : 4;
     }
-    const int LC = tc.LC;
+    // a band no taller than the rows asked for (a window split into row
+    // ranges): more lanes per row, fewer row groups
+    int LC = tc.LC;
+    while (LC < 64 && (64 / LC) * K > row_hi - row_lo)
+        LC *= 2;
     const int band_rows = (64 / LC) * K, band_cols = 4 * LC;
     const int R2 = 2 * p.R;
     for (int r0 = row_lo; r0 < row_hi; r0 += band_rows)
