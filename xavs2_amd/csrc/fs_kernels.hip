@@ -308,10 +308,23 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 
 // Process one item with K rows per lane.
 // Per-wave cache of the MV-cost terms of the lane's candidates, valid for one
-// item (band, MVP) and one search centre.
+// item (band), one search centre and one MVP.
 struct TermState {
     bool ok;
-    int cx, cy;
+    int cx, cy;     // search centre the cached MV costs were made for
+    int px, py;     // and the MVP
+    __device__ bool fresh(const DevInst &in, const DevPu &pu) const
+    {
+        return ok && cx == in.omx && cy == in.omy && px == pu.mvp_x && py == pu.mvp_y;
+    }
+    __device__ void set(const DevInst &in, const DevPu &pu)
+    {
+        ok = true;
+        cx = in.omx;
+        cy = in.omy;
+        px = pu.mvp_x;
+        py = pu.mvp_y;
+    }
 };
 
 template <int K>
@@ -385,7 +398,7 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
         // term = (MV cost << 5) | candidate index, computed once per item and
         // centre; then key = (SAD << 5) + term and one unsigned min per
         // candidate (SAD < 2^21 and MV cost < 2^15, so no overflow)
-        if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
+        if (!ts.fresh(in, pu)) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int by_bits = xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y);
@@ -394,9 +407,7 @@ __device__ __forceinline__ void fs_item(const DevParams &p, const DevPu &pu, con
                     term[(k * 4 + d) * 64 + lane] =
                         ((__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << 5) | (uint32_t)(k * 4 + d);
             }
-            ts.ok = true;
-            ts.cx = in.omx;
-            ts.cy = in.omy;
+            ts.set(in, pu);
         }
         uint32_t m = 0xFFFFFFFFu;
 #pragma unroll
@@ -567,7 +578,7 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     const int row = lane_row(it, rg, K);
     const int lane_base = (by + row) * pitch + bx + col;
     uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]
-    if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
+    if (!ts.fresh(in, pu)) {
         // ME_COST_IPEL (me.c:187-192): (lambda * (bits(4 dx) + bits(4 dy))) >> 16
         const uint32_t lam = (uint32_t)p.lambda;
         int bits_x[4];
@@ -583,9 +594,7 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
                 v |= (uint64_t)(__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << (16 * d);
             init[k * 64 + lane] = v;
         }
-        ts.ok = true;
-        ts.cx = in.omx;
-        ts.cy = in.omy;
+        ts.set(in, pu);
     }
     uint64_t a16[K];
 #pragma unroll
@@ -689,7 +698,7 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     const int row = lane_row(it, rg, K);
     const int lane_base = (by + row) * pitch + bx + col;
     uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]: packed u16 MV costs
-    if (!ts.ok || ts.cx != in.omx || ts.cy != in.omy) {
+    if (!ts.fresh(in, pu)) {
         const uint32_t lam = (uint32_t)p.lambda;
         int bits_x[4];
 #pragma unroll
@@ -704,9 +713,7 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
                 v |= (uint64_t)(__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << (16 * d);
             init[k * 64 + lane] = v;
         }
-        ts.ok = true;
-        ts.cx = in.omx;
-        ts.cy = in.omy;
+        ts.set(in, pu);
     }
     uint64_t a16[K], mvc[K];
 #pragma unroll
@@ -885,7 +892,7 @@ __global__ void __launch_bounds__(256, KMAX == 4 ? 4 : 2) fs_kernel(DevParams p,
             const FsItem it = items[it_i];
             TermState ts;
             ts.ok = false;
-            ts.cx = ts.cy = 0;
+            ts.cx = ts.cy = ts.px = ts.py = 0;
             // the item's instances (<= XME_MAX_GROUP) are fetched at once, one
             // per lane, and handed to the wave one after the other
             const int lane = threadIdx.x & 63;

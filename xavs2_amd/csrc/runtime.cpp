@@ -303,23 +303,32 @@ static bool uses_narrow(const DevParams &p, int part)
     return xme_narrow_shape(part) && xme_fs_kmax() == 8 && xme_tune_lookup(part, p.R).K == XME_NARROW_K;
 }
 
+// Tile shape of the items of a PU shape over candidate rows [row_lo, row_hi):
+// K rows per lane and LC lanes per candidate row (the tuned choice, with a
+// band no taller than the rows asked for when a window is split into row
+// ranges: more lanes per row, fewer row groups).  Returns the item count.
+static int band_shape(const DevParams &p, int part, int row_lo, int row_hi, int &K, int &LC)
+{
+    const TuneChoice tc = xme_tune_lookup(part, p.R);
+    if (uses_narrow(p, part)) {
+        K = XME_NARROW_K;                         // the short-PU kernel variant
+    } else {
+        const int Kt = std::min(std::min(tc.K, 8), xme_fs_kmax());
+        K = (xme_pu_h(part) % Kt == 0) ? Kt : 4;
+    }
+    LC = tc.LC;
+    while (LC < 64 && (64 / LC) * K > row_hi - row_lo)
+        LC *= 2;
+    const int band_rows = (64 / LC) * K, band_cols = 4 * LC, R2 = 2 * p.R;
+    return ((row_hi - row_lo + band_rows - 1) / band_rows) * ((R2 + band_cols - 1) / band_cols);
+}
+
 static void items_for(const DevParams &p, const std::vector<DevPu> &pus, const std::vector<int> &group, int part,
                       int row_lo, int row_hi, std::vector<HostItem> &out)
 {
     const DevPu &pu = pus[group[0] / p.n_ref];
-    TuneChoice tc = xme_tune_lookup(part, p.R);
-    int K;
-    if (uses_narrow(p, part)) {
-        K = XME_NARROW_K;                         // the narrow-PU kernel variant
-    } else {
-        const int Kt = std::min(std::min(tc.K, 8), xme_fs_kmax());
-        K = (pu.h % Kt == 0) ? Kt : 4;
-    }
-    // a band no taller than the rows asked for (a window split into row
-    // ranges): more lanes per row, fewer row groups
-    int LC = tc.LC;
-    while (LC < 64 && (64 / LC) * K > row_hi - row_lo)
-        LC *= 2;
+    int K, LC;
+    band_shape(p, part, row_lo, row_hi, K, LC);
     const int band_rows = (64 / LC) * K, band_cols = 4 * LC;
     const int R2 = 2 * p.R;
     for (int r0 = row_lo; r0 < row_hi; r0 += band_rows)
@@ -340,21 +349,32 @@ static void items_for(const DevParams &p, const std::vector<DevPu> &pus, const s
         }
 }
 
-// Split instances of one shape and MVP into groups whose items carry similar
-// work (about 4096 pixels of PU area per item: enough calls per item to hide
-// the latency of fetching the next item's instances).
+// Split instances of one shape into groups whose items carry similar work
+// (about 4096 pixels of PU area per item: enough calls per item to hide the
+// latency of fetching the next item's instances), with at least 4 items per
+// shape so that each of the 4 waves of a task's workgroup gets work
+// (items_per_group(part): the bands one group yields).  MVPs may differ inside
+// a group: a wave keeps its cached MV costs while consecutive instances share
+// search centre and MVP, so instances are kept in PU order, MVPs together.
+template <typename ItemsPerGroup>
 static std::vector<std::vector<int>> make_groups(const std::vector<DevPu> &pus, const std::vector<int> &insts,
-                                                 int n_ref, int max_group)
+                                                 int n_ref, int max_group, ItemsPerGroup items_per_group)
 {
-    std::map<std::tuple<int, int, int>, std::vector<int>> by_key;
-    for (int id : insts) {
-        const DevPu &pu = pus[id / n_ref];
-        by_key[{ pu.part, pu.mvp_x, pu.mvp_y }].push_back(id);
-    }
+    std::map<int, std::vector<int>> by_key;
+    std::vector<int> sorted(insts);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
+        const DevPu &pa = pus[a / n_ref], &pb = pus[b / n_ref];
+        return std::make_pair(pa.mvp_x, pa.mvp_y) < std::make_pair(pb.mvp_x, pb.mvp_y);
+    });
+    for (int id : sorted)
+        by_key[pus[id / n_ref].part].push_back(id);
     std::vector<std::vector<int>> out;
     for (auto &kv : by_key) {
         const DevPu &pu = pus[kv.second[0] / n_ref];
-        const int g = std::max(1, std::min(max_group, 4096 / (pu.w * pu.h)));
+        const int n = (int)kv.second.size();
+        const int min_groups = (4 + items_per_group(kv.first) - 1) / items_per_group(kv.first);
+        const int g = std::max(1, std::min(std::min(max_group, 4096 / (pu.w * pu.h)),
+                                           (n + min_groups - 1) / min_groups));
         for (size_t i = 0; i < kv.second.size(); i += g)
             out.emplace_back(kv.second.begin() + i, kv.second.begin() + std::min(kv.second.size(), i + g));
     }
@@ -576,7 +596,10 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
                         std::vector<int> insts;
                         for (int i : members)
                             insts.push_back(i * n_ref + r);
-                        const auto inst_groups = make_groups(pus, insts, n_ref, XME_MAX_GROUP);
+                        const auto inst_groups = make_groups(pus, insts, n_ref, XME_MAX_GROUP, [&](int part) {
+                            int K, LC;
+                            return band_shape(p, part, 0, rows_per, K, LC);
+                        });
                         for (int s = 0; s < splits; s++) {
                             for (const auto &ig : inst_groups)
                                 items_for(p, pus, ig, pus[ig[0] / n_ref].part, s * rows_per,
