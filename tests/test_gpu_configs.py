@@ -7,7 +7,9 @@
   true MVs applied through the AVS2 8-tap planes, PU shapes 8x8, 16x16, 32x32,
   64x64, 64x16, 16x64, 32x8, 8x32, full search [-32, 32)^2 then half- and
   quarter-pel Spiral points with SATD, and chroma MC of the 960x540 planes at
-  the chosen MVs.
+  the chosen MVs;
+* full search [-64, 64)^2 + sub-pel over mixed PU shapes, 1920x1024, with a
+  random MVP per PU (split search windows).
 """
 import ctypes as C
 import os
@@ -63,6 +65,23 @@ def test_integer_full_search_1080p_16x16():
     dref.load(ref)
     got = gpu_run(X.me_params(W, H, 32, LAM, use_fractional_me=0), pus, cur, dref)
     exp = oracle_run(O.me_param(W, H, 32, LAM, use_fractional_me=0), pus, cur, oracle_planes(ref))
+    assert_same(got, exp, pus)
+
+
+def test_random_mvps_mixed_shapes_1080p_range64():
+    """Every PU its own MVP (+-16 pixels): search centres spread over a CTU, so
+    CTU windows exceed their LDS share and are split into candidate-row ranges,
+    and the short bands of those ranges exercise the lanes pulled back inside
+    an item.  Mixed shapes 8x8..64x64, R = 64, sub-pel refinement on."""
+    ref = synth.smooth_field(W, H, seed=400)
+    cur, _ = synth.displaced(ref, 16, 24, seed=401)
+    rng = np.random.default_rng(402)
+    pus = [(x, y, pw, ph, int(rng.integers(-64, 65)), int(rng.integers(-64, 65)))
+           for (x, y, pw, ph) in synth.mixed_pus(W, 1024, 402)]
+    dref = X.DeviceRef(W, H, frac=2)
+    dref.load(ref)
+    got = gpu_run(X.me_params(W, H, 64, LAM), pus, cur, dref)
+    exp = oracle_run(O.me_param(W, H, 64, LAM), pus, cur, oracle_planes(ref))
     assert_same(got, exp, pus)
 
 
