@@ -551,6 +551,30 @@ __device__ __forceinline__ void short_strip(const uint8_t *__restrict__ win, int
     short_rows<H, NQ, ALIGNED>(std::make_integer_sequence<int, H + XME_NARROW_K - 1>{}, w32, pd, shift, cw, raw, a16);
 }
 
+// The packed MV costs of a short-PU lane's 16 x 4 candidates into the wave's
+// LDS term area: init[k * 64 + lane] holds columns d = 0..3 of candidate row
+// k as u16.  ME_COST_IPEL (me.c:187-192) is (lambda * (bits(4 dx) +
+// bits(4 dy))) >> 16; lambda < 2^24 and the bit counts are small, so
+// lambda * bx + lambda * by is the same sum (below 2^31) and the products
+// come out of the 64-candidate loop.
+__device__ __forceinline__ void short_mv_costs(const DevParams &p, const DevPu &pu, const DevInst &in, int col, int row,
+                                               uint64_t *__restrict__ init, int lane)
+{
+    const uint32_t lam = (uint32_t)p.lambda;
+    const int R = p.R;
+    uint32_t lx[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++)
+        lx[d] = __umul24(lam, (uint32_t)xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x));
+#pragma unroll
+    for (int k = 0; k < XME_NARROW_K; k++) {
+        const uint32_t ly = __umul24(lam, (uint32_t)xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y));
+        const uint32_t lo = ((lx[1] + ly) & 0xFFFF0000u) | ((lx[0] + ly) >> 16);
+        const uint32_t hi = ((lx[3] + ly) & 0xFFFF0000u) | ((lx[2] + ly) >> 16);
+        init[k * 64 + lane] = ((uint64_t)hi << 32) | lo;
+    }
+}
+
 // Narrow PUs (8 pixels wide, at most 16 rows) carry little work per
 // candidate, so per-call costs dominate the generic path.  Here a lane takes
 // K = 16 candidate rows; the SAD and the MV cost of each candidate are summed
@@ -579,21 +603,7 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
     const int lane_base = (by + row) * pitch + bx + col;
     uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]
     if (!ts.fresh(in, pu)) {
-        // ME_COST_IPEL (me.c:187-192): (lambda * (bits(4 dx) + bits(4 dy))) >> 16
-        const uint32_t lam = (uint32_t)p.lambda;
-        int bits_x[4];
-#pragma unroll
-        for (int d = 0; d < 4; d++)
-            bits_x[d] = xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x);
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int by_bits = xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y);
-            uint64_t v = 0;
-#pragma unroll
-            for (int d = 0; d < 4; d++)
-                v |= (uint64_t)(__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << (16 * d);
-            init[k * 64 + lane] = v;
-        }
+        short_mv_costs(p, pu, in, col, row, init, lane);
         ts.set(in, pu);
     }
     uint64_t a16[K];
@@ -699,20 +709,7 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     const int lane_base = (by + row) * pitch + bx + col;
     uint64_t *init = reinterpret_cast<uint64_t *>(term);    // [k][lane]: packed u16 MV costs
     if (!ts.fresh(in, pu)) {
-        const uint32_t lam = (uint32_t)p.lambda;
-        int bits_x[4];
-#pragma unroll
-        for (int d = 0; d < 4; d++)
-            bits_x[d] = xme_mvbits(4 * (in.omx + col + d - R) - pu.mvp_x);
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int by_bits = xme_mvbits(4 * (in.omy + row + k - R) - pu.mvp_y);
-            uint64_t v = 0;
-#pragma unroll
-            for (int d = 0; d < 4; d++)
-                v |= (uint64_t)(__umul24(lam, (uint32_t)(bits_x[d] + by_bits)) >> 16) << (16 * d);
-            init[k * 64 + lane] = v;
-        }
+        short_mv_costs(p, pu, in, col, row, init, lane);
         ts.set(in, pu);
     }
     uint64_t a16[K], mvc[K];
