@@ -316,8 +316,9 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=4, help="distinct frames per stream, cycled")
     ap.add_argument("--serve-split", type=float, default=0,
                     help="search work per launch in the serving leg, absdiffs (0: server default, <0: no split)")
-    ap.add_argument("--serve-share", type=int, default=0,
-                    help="%% of a full-device search launch per frame in the serving leg (0 = 100)")
+    ap.add_argument("--serve-share", type=int, default=75,
+                    help="%% of a full-device search launch per frame in the serving leg (0 = 100); a cap "
+                         "leaves CUs for frames that arrive while a large one searches")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
