@@ -174,9 +174,15 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                 v[i] = D[i] >> s1;
             split16(v, hi, lo);
             D = mm_data_b(f2a, hi, lo, acc_f2);
-            // coefficients: row rho(half, i), column col
+            // coefficients: row rho(half, i), column col.  Nonzero counts per
+            // block: lane b < BPT adds the nonzeros of its block (row block b / NB,
+            // column block b % NB) from each row's ballot as it is made, so no
+            // ballot stays live (bits 0-31 hold row rho(0, i), bits 32-63 row
+            // rho(1, i), bit = column)
             int c[16];
-            uint64_t nzmask[16];
+            const int my_rb = lane / NB, my_cb = lane % NB;
+            const uint32_t my_cm = N == 32 ? 0xffffffffu : ((1u << N) - 1u) << (my_cb * N);
+            int mine = 0;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 c[i] = (int)(int16_t)(D[i] >> s2);
@@ -184,7 +190,11 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                     const int a = c[i] < 0 ? -c[i] : c[i];
                     const int m = (int)(((uint32_t)a * (uint32_t)q.qscale + (uint32_t)q.qadd) >> q.qshift);
                     c[i] = (int)(int16_t)(c[i] < 0 ? -m : m);
-                    nzmask[i] = __builtin_amdgcn_ballot_w64(c[i] != 0);
+                    const uint64_t nz = __builtin_amdgcn_ballot_w64(c[i] != 0);
+                    if (rho(0, i) / N == my_rb)
+                        mine += __builtin_popcount((uint32_t)nz & my_cm);
+                    if (rho(1, i) / N == my_rb)
+                        mine += __builtin_popcount((uint32_t)(nz >> 32) & my_cm);
                 }
             }
             // levels (or plain coefficients) out
@@ -195,28 +205,8 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                 if (blk < nblk)
                     level[tile_off<N>(t, y2, col)] = (int16_t)c[i];
             }
-            if ((mode & DO_QUANT) && nnz) {
-                // per-block nonzero counts from the ballots: bits 0-31 hold row
-                // rho(0, i), bits 32-63 row rho(1, i), bit = column
-                int mine = 0;
-#pragma unroll
-                for (int b = 0; b < BPT; b++) {
-                    const int rb = b / NB, cb = b % NB;
-                    const uint32_t cm = (N == 32 ? 0xffffffffu : ((1u << N) - 1u) << (cb * N));
-                    int cnt = 0;
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        if (rho(0, i) / N == rb)
-                            cnt += __builtin_popcount((uint32_t)nzmask[i] & cm);
-                        if (rho(1, i) / N == rb)
-                            cnt += __builtin_popcount((uint32_t)(nzmask[i] >> 32) & cm);
-                    }
-                    if (lane == b)
-                        mine = cnt;
-                }
-                if (lane < BPT && t * BPT + lane < nblk)
-                    nnz[t * BPT + lane] = mine;
-            }
+            if ((mode & DO_QUANT) && nnz && lane < BPT && t * BPT + lane < nblk)
+                nnz[t * BPT + lane] = mine;
             if (mode & DO_INV) {
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
