@@ -23,7 +23,11 @@ class MeParam(C.Structure):
                 ("lambda_factor", C.c_int), ("min_mv_range", C.c_int * 2),
                 ("max_mv_range", C.c_int * 2), ("use_hadamard", C.c_int),
                 ("use_fractional_me", C.c_int), ("use_fast_sub_me", C.c_int),
-                ("enable_pmvr", C.c_int)]
+                ("enable_pmvr", C.c_int), ("me_method", C.c_int)]
+
+
+# h->param->me_method (common/common.h:145-151)
+ME_FS, ME_DIA, ME_HEX = 0, 1, 2
 
 
 class Pu(C.Structure):
@@ -68,6 +72,8 @@ def lib():
         L.xo_me_search.restype = C.c_int32
         L.xo_me_search.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, ip, P, ip, P, P]
         L.xo_me_search_batch.argtypes = [C.POINTER(MeParam), P, C.c_int, P, ip, P, ip, P, C.c_int]
+        L.xo_me_pattern_path.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, ip, P, ip, C.c_int, C.c_int,
+                                         C.c_int, P, C.c_int]
         L.xo_expand_border.argtypes = [P, ip, C.c_int, C.c_int, C.c_int]
         L.xo_interpolate_frame.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_mc_luma.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int, P, ip]
@@ -151,8 +157,10 @@ def lambda_factor(lam):
 
 
 def me_param(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use_fractional_me=2,
-             use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047)):
+             use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047),
+             me_method=ME_FS):
     p = MeParam()
+    p.me_method = me_method
     p.frame_w, p.frame_h, p.search_range, p.lambda_factor = frame_w, frame_h, search_range, lambda_factor
     p.min_mv_range[0], p.max_mv_range[0] = mv_range_x
     p.min_mv_range[1], p.max_mv_range[1] = mv_range_y
@@ -188,6 +196,18 @@ def me_search(prm, pu_tuple, cur_plane, planes, cost_map=False):
                        ptr(cm) if cm is not None else None)
     out = {f: getattr(res, f) for f, _ in MeResult._fields_}
     return out, (cm.reshape(2 * R, 2 * R) if cm is not None else None)
+
+
+def me_pattern_path(prm, pu_tuple, cur_plane, ref_plane, start, start_cost, max_pts=4096):
+    """Accepted points of the DIA / HEX integer stage from `start` (full-pel)
+    with cost `start_cost`: a list of (x, y), the start first, the integer
+    best last."""
+    path = np.zeros(2 * max_pts, dtype=np.int32)
+    n = lib().xo_me_pattern_path(C.byref(prm), C.byref(Pu(*pu_tuple)), cur_plane.origin_ptr(), cur_plane.stride,
+                                 ref_plane.origin_ptr(), ref_plane.stride, start[0], start[1], start_cost,
+                                 ptr(path), max_pts)
+    assert n <= max_pts
+    return [(int(path[2 * k]), int(path[2 * k + 1])) for k in range(n)]
 
 
 def me_search_batch(prm, pus, cur_plane, planes, nthreads=1):

@@ -248,6 +248,149 @@ static int pmvr_adapt(int *mx, int *my, int ctr_x, int ctr_y, int mv_x, int mv_y
 
 static inline int weighted_cost(int lambda, int bits) { return (lambda * bits) >> 16; }
 
+/* Radius-2 hexagon and radius-1 diamond with their wrap-around entries, and
+ * the (x - 1) mod 6 / mod 4 index tables, encoder/me.c:83-112. */
+static const int8_t HEX2[8][2] = {
+    { -1, -2 }, { -2, 0 }, { -1, 2 }, { 1, 2 }, { 2, 0 }, { 1, -2 }, { -1, -2 }, { -2, 0 }
+};
+static const int8_t M1MOD6[8] = { 5, 0, 1, 2, 3, 4, 5, 0 };
+static const int8_t DIA1[6][2] = {
+    { 0, -1 }, { -1, 0 }, { 0, 1 }, { 1, 0 }, { 0, -1 }, { -1, 0 }
+};
+static const int8_t M1MOD4[6] = { 3, 0, 1, 2, 3, 0 };
+
+typedef struct {
+    int W, H, R, lambda, pmx, pmy;
+    const uint8_t *org;
+    intptr_t i_cur;
+    const uint8_t *fref;
+    intptr_t i_ref;
+    uint32_t mv_min, mv_max;
+    int bcost, bmx, bmy, dir;
+    int *path, max_pts, n_pts;
+} pat_ctx_t;
+
+/* One point of ME_COST_IPEL_X3_DIR / ME_COST_IPEL_X4_DIR (me.c:223-287): the
+ * SAD plus MV_COST_IPEL, evaluated without a range check, taken on a strict
+ * '<' with its direction code. */
+static void pat_point(pat_ctx_t *c, int mx, int my, int d)
+{
+    int cost = xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
+               weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+    if (cost < c->bcost) {
+        c->bcost = cost;
+        c->bmx = mx;
+        c->bmy = my;
+        c->dir = d;
+    }
+}
+
+static void pat_accept(pat_ctx_t *c)
+{
+    if (c->path && c->n_pts < c->max_pts) {
+        c->path[2 * c->n_pts] = c->bmx;
+        c->path[2 * c->n_pts + 1] = c->bmy;
+    }
+    c->n_pts++;
+}
+
+/* The integer stage of XAVS2_ME_HEX (me.c:1107-1131, falling through to the
+ * diamond) and XAVS2_ME_DIA (me.c:1133-1154), from the best MV candidate. */
+static void pattern_search(pat_ctx_t *c, int method)
+{
+    int omx, omy;
+    pat_accept(c);
+    if (method == XO_ME_HEX) {
+        /* the full hexagon around the start, then half hexagons that skip
+         * the points the previous step already evaluated */
+        c->dir = 0;
+        omx = c->bmx;
+        omy = c->bmy;
+        pat_point(c, omx - 1, omy - 2, 6);
+        pat_point(c, omx + 1, omy - 2, 5);
+        pat_point(c, omx - 2, omy, 1);
+        pat_point(c, omx + 2, omy, 4);
+        pat_point(c, omx - 1, omy + 2, 2);
+        pat_point(c, omx + 1, omy + 2, 3);
+        if (c->dir) {
+            pat_accept(c);
+            int idx = c->dir - 1;
+            for (int i = 0; i < c->R && check_mv_range(c->bmx, c->bmy, c->mv_min, c->mv_max); i++) {
+                c->dir = 0;
+                omx = c->bmx;
+                omy = c->bmy;
+                for (int k = 0; k < 3; k++)
+                    pat_point(c, omx + HEX2[idx + k][0], omy + HEX2[idx + k][1], k + 1);
+                if (!c->dir)
+                    break;
+                pat_accept(c);
+                idx = M1MOD6[c->dir + idx - 1];
+            }
+        }
+    }
+    /* small diamond, then half diamonds */
+    c->dir = 0;
+    if (check_mv_range(c->bmx, c->bmy, c->mv_min, c->mv_max)) {
+        omx = c->bmx;
+        omy = c->bmy;
+        pat_point(c, omx, omy - 1, 4);
+        pat_point(c, omx - 1, omy, 1);
+        pat_point(c, omx + 1, omy, 3);
+        pat_point(c, omx, omy + 1, 2);
+    }
+    if (c->dir) {
+        pat_accept(c);
+        int idx = c->dir - 1;
+        for (int i = 0; i < c->R && check_mv_range(c->bmx, c->bmy, c->mv_min, c->mv_max); i++) {
+            c->dir = 0;
+            omx = c->bmx;
+            omy = c->bmy;
+            for (int k = 0; k < 3; k++)
+                pat_point(c, omx + DIA1[idx + k][0], omy + DIA1[idx + k][1], k + 1);
+            if (!c->dir)
+                break;
+            pat_accept(c);
+            idx = M1MOD4[c->dir + idx - 1];
+        }
+    }
+}
+
+static void pat_init(pat_ctx_t *c, const xo_me_param_t *prm, const xo_pu_t *pu, const uint8_t *cur,
+                     intptr_t i_cur, const uint8_t *plane0, intptr_t i_ref, const mv_window_t *win)
+{
+    memset(c, 0, sizeof(*c));
+    c->W = pu->w;
+    c->H = pu->h;
+    c->R = prm->search_range;
+    c->lambda = prm->lambda_factor;
+    c->pmx = pu->mvp_x;
+    c->pmy = pu->mvp_y;
+    c->org = cur + (intptr_t)pu->pix_y * i_cur + pu->pix_x;
+    c->i_cur = i_cur;
+    c->fref = plane0 + (intptr_t)pu->pix_y * i_ref + pu->pix_x;
+    c->i_ref = i_ref;
+    c->mv_min = win->mv_min;
+    c->mv_max = win->mv_max;
+}
+
+int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu,
+                       const uint8_t *cur, intptr_t i_cur,
+                       const uint8_t *plane0, intptr_t i_ref,
+                       int sx, int sy, int scost, int *path, int max_pts)
+{
+    mv_window_t win;
+    pat_ctx_t c;
+    pu_mv_window(prm, pu, &win);
+    pat_init(&c, prm, pu, cur, i_cur, plane0, i_ref, &win);
+    c.bcost = scost;
+    c.bmx = sx;
+    c.bmy = sy;
+    c.path = path;
+    c.max_pts = max_pts;
+    pattern_search(&c, prm->me_method);
+    return c.n_pts;
+}
+
 int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
                      const uint8_t *cur, intptr_t i_cur,
                      const uint8_t *const planes[16], intptr_t i_ref,
@@ -305,6 +448,19 @@ int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
         return XO_MAX_DISTORTION;
     }
 
+    if (prm->me_method == XO_ME_DIA || prm->me_method == XO_ME_HEX) {
+        pat_ctx_t c;
+        pat_init(&c, prm, pu, cur, i_cur, planes[0], i_ref, &win);
+        c.bcost = bcost;
+        c.bmx = bmx;
+        c.bmy = bmy;
+        pattern_search(&c, prm->me_method);
+        bcost = c.bcost;
+        bmx = c.bmx;
+        bmy = c.bmy;
+        goto store;
+    }
+
     /* Full search, me.c:1156-1164: rows j in [-R, R), columns i in [-R, R). */
     const int omx = bmx, omy = bmy;
     for (int j = -R; j < R; j++) {
@@ -321,6 +477,7 @@ int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
     }
 #undef IPEL_COST
 
+store:
     /* Store the full-pel result, me.c:1169-1173. */
     res->bmv_x  = (int16_t)(4 * bmx);
     res->bmv_y  = (int16_t)(4 * bmy);

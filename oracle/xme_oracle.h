@@ -70,7 +70,13 @@ typedef struct {
     int use_fractional_me;            /* 0 none, 1 half-pel, 2 half + quarter */
     int use_fast_sub_me;              /* Spiral2 with step 2 */
     int enable_pmvr;                  /* h->param->enable_pmvr */
+    int me_method;                    /* h->param->me_method: XO_ME_FS, XO_ME_DIA or XO_ME_HEX */
 } xo_me_param_t;
+
+/* enum me_mothod_e, common/common.h:145-151 (UMH and TZ are not restated). */
+#define XO_ME_FS  0
+#define XO_ME_DIA 1
+#define XO_ME_HEX 2
 
 typedef struct {
     int pix_x, pix_y;                 /* PU position in the frame */
@@ -87,17 +93,29 @@ typedef struct {
     int32_t status;                   /* 0 ok; 1 no legal MV candidate (_me_error) */
 } xo_me_result_t;
 
-/* Full search + sub-pel refinement for one PU against one reference.
+/* Integer search (full search, or the DIA / HEX pattern search by
+ * prm->me_method) + sub-pel refinement for one PU against one reference.
  *   cur    : current luma plane origin, stride i_cur (PU at cur + pix_y*i_cur + pix_x)
  *   planes : planes[0] = padded full-pel reference origin; planes[1..15] the
  *            interpolated planes (may be NULL when use_fractional_me == 0).
  *   cost_map (optional, may be NULL): (2R)*(2R) integer-pel costs of the raster
- *            pass, row-major by (j+R, i+R); illegal candidates hold -1.
+ *            pass, row-major by (j+R, i+R); illegal candidates hold -1.  Full
+ *            search only: the pattern searches leave it untouched.
  */
 int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
                      const uint8_t *cur, intptr_t i_cur,
                      const uint8_t *const planes[16], intptr_t i_ref,
                      xo_me_result_t *res, int32_t *cost_map);
+
+/* The integer stage of a pattern search (prm->me_method XO_ME_DIA or
+ * XO_ME_HEX) from the full-pel start point (sx, sy) with cost scost, which
+ * xo_me_search takes from the MV candidate list.  Writes the accepted points
+ * in order to path[0..2*n) as (x, y) pairs (at most max_pts, the start point
+ * first) and returns their count n; the last one is the integer best. */
+int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu,
+                       const uint8_t *cur, intptr_t i_cur,
+                       const uint8_t *plane0, intptr_t i_ref,
+                       int sx, int sy, int scost, int *path, int max_pts);
 
 /* The same over n PUs on nthreads POSIX threads (CPU baseline). */
 void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,

@@ -1,0 +1,228 @@
+"""Pins the oracle's DIA and HEX integer searches (encoder/me.c:1101-1154).
+
+* Hand-worked paths on a flat frame, where every SAD is 0 and a point's cost
+  is its MV bits alone, so each step can be worked out on paper from the
+  hexagon / diamond tables and the strict '<' (equal-cost points are not
+  taken).
+* On textured frames, checked against an independent numpy cost:
+  - every accepted move is a hexagon move (HEX stage) or a diamond move;
+  - the cost falls strictly along the path;
+  - each stage that ends on its own ends in a local minimum: no point of the
+    full hexagon (diamond) around its last point is strictly cheaper.  A
+    wrong table entry or a wrong index update of the half patterns leaves a
+    cheaper neighbour unvisited and fails this.
+* The integer result of the whole search is the last point of the path.
+* Where the current picture is the reference moved by a small whole-pixel
+  shift, both searches find the shift, as the full search does.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import npref
+from tests.test_oracle_interp import smooth_frame
+
+HEX = [(-1, -2), (1, -2), (-2, 0), (2, 0), (-1, 2), (1, 2)]
+DIA = [(0, -1), (-1, 0), (1, 0), (0, 1)]
+
+
+def plane_of(img):
+    h, w = img.shape
+    p = O.Plane(w, h)
+    p.set_picture(img)
+    p.expand_border()
+    return p
+
+
+def cur_plane(img):
+    h, w = img.shape
+    c = O.Plane(w, h)
+    c.set_picture(img)
+    return c
+
+
+def cost_fn(cur, ref, pu, mvp, lam):
+    """SAD against the border-clamped reference + lambda * MV bits (numpy)."""
+    x, y, w, h = pu
+    ref_pad = npref.clamp_picture(ref, O.PAD)
+    blk = cur[y:y + h, x:x + w].astype(np.int64)
+
+    def bits(v):
+        return 1 if v == 0 else npref.se_len(v)
+
+    def cost(mx, my):
+        r = ref_pad[O.PAD + y + my:O.PAD + y + my + h, O.PAD + x + mx:O.PAD + x + mx + w].astype(np.int64)
+        return int(np.abs(blk - r).sum()) + ((lam * (bits(4 * mx - mvp[0]) + bits(4 * my - mvp[1]))) >> 16)
+    return cost
+
+
+def run_path(method, cur, ref, pu, mvp, lam, start, R=16):
+    w, h = cur.shape[1], cur.shape[0]
+    prm = O.me_param(w, h, R, lam, use_fractional_me=0, me_method=method)
+    cost = cost_fn(cur, ref, pu, mvp, lam)
+    return O.me_pattern_path(prm, pu + mvp, cur_plane(cur), plane_of(ref), start, cost(*start)), cost
+
+
+@pytest.mark.parametrize("method,start,expected", [
+    # HEX from (3, 6): hexagon -> (1, 6) [bits 7 + 11], half hexagon -> (0, 4) [1 + 11],
+    # the next half hexagon finds nothing below 12; diamond -> (0, 3) [1 + 9], and
+    # the half diamond's (0, 2) costs 10 as well, which a strict '<' does not take
+    (O.ME_HEX, (3, 6), [(3, 6), (1, 6), (0, 4), (0, 3)]),
+    # DIA from (2, 5): diamond -> (1, 5) [7 + 11]; half diamond -> (0, 5) [1 + 11];
+    # then (0, 4) and (0, 6) cost 12 too: stop
+    (O.ME_DIA, (2, 5), [(2, 5), (1, 5), (0, 5)]),
+    # DIA from (3, 6): every diamond point costs >= 20, no move
+    (O.ME_DIA, (3, 6), [(3, 6)]),
+    # DIA from (1, 1) [14]: (1, 0) and (0, 1) both cost 8; the diamond lists
+    # (0, -1) before (-1, 0), so (1, 0) is taken; the half diamond then reaches
+    # (0, 0) [2], around which every point costs 8
+    (O.ME_DIA, (1, 1), [(1, 1), (1, 0), (0, 0)]),
+])
+def test_hand_worked_paths_on_a_flat_frame(method, start, expected):
+    flat = np.full((64, 64), 90, np.uint8)
+    path, _ = run_path(method, flat, flat, (16, 16, 16, 16), (0, 0), 65536, start)
+    assert path == expected
+
+
+def check_path(method, path, cost, legal, R):
+    # a stage only continues from a point inside the MV window
+    assert all(legal(*p) for p in path[:-1]), path
+    costs = [cost(*p) for p in path]
+    assert all(b < a for a, b in zip(costs, costs[1:])), (path, costs)
+    moves = [(b[0] - a[0], b[1] - a[1]) for a, b in zip(path, path[1:])]
+    n_hex = 0
+    if method == O.ME_HEX:
+        while n_hex < len(moves) and moves[n_hex] in HEX:
+            n_hex += 1
+    assert all(m in DIA for m in moves[n_hex:]), moves
+    if method == O.ME_HEX:
+        # the hexagon stage ended on its own unless its point left the MV
+        # window or it ran its R + 1 steps
+        c = path[n_hex]
+        if legal(*c) and n_hex < R + 1:
+            assert all(cost(c[0] + dx, c[1] + dy) >= costs[n_hex] for dx, dy in HEX), (path, c)
+    c = path[-1]
+    n_dia = len(moves) - n_hex
+    if legal(*c) and n_dia < R + 1:
+        assert all(cost(c[0] + dx, c[1] + dy) >= costs[-1] for dx, dy in DIA), (path, c)
+
+
+@pytest.mark.parametrize("method", [O.ME_DIA, O.ME_HEX])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_paths_fall_and_end_in_local_minima(method, seed):
+    w, h = 160, 112
+    rng = np.random.default_rng(seed)
+    ref = smooth_frame(w, h, seed)
+    cur = np.roll(ref, (int(rng.integers(-6, 7)), int(rng.integers(-6, 7))), axis=(0, 1))
+    cur = np.clip(cur.astype(int) + rng.integers(-4, 5, cur.shape), 0, 255).astype(np.uint8)
+    lam = 4 * 65536
+    R = 8
+    for _ in range(12):
+        pw, ph = [(8, 8), (16, 16), (16, 8), (8, 32), (32, 16)][int(rng.integers(0, 5))]
+        pu = (int(rng.integers(0, (w - pw) // 4 + 1)) * 4, int(rng.integers(0, h - ph + 1)), pw, ph)
+        mvp = (int(rng.integers(-20, 21)), int(rng.integers(-20, 21)))
+        (xmin, xmax), (ymin, ymax) = npref.mv_window(w, h, *pu)
+        start = (int(np.clip((mvp[0] + 2) >> 2, xmin, xmax)), int(np.clip((mvp[1] + 2) >> 2, ymin, ymax)))
+        path, cost = run_path(method, cur, ref, pu, mvp, lam, start, R)
+        check_path(method, path, cost, lambda mx, my: xmin <= mx <= xmax and ymin <= my <= ymax, R)
+
+
+@pytest.mark.parametrize("method", [O.ME_DIA, O.ME_HEX])
+def test_search_result_is_the_path_end_and_edges_stop_it(method):
+    """PUs at the picture corners, where the hexagon reaches past the MV
+    window: the search's integer result is the path's last point."""
+    w, h = 96, 64
+    ref = smooth_frame(w, h, 9)
+    cur = np.roll(ref, (5, -7), axis=(0, 1))
+    lam = 4 * 65536
+    planes = [plane_of(ref)] + [None] * 15
+    for pu, mvp in [((0, 0, 16, 16), (-300, -300)), ((80, 48, 16, 16), (300, 300)), ((40, 24, 8, 8), (0, 0)),
+                    ((0, 48, 8, 16), (-270, 60))]:
+        prm = O.me_param(w, h, 16, lam, use_fractional_me=0, me_method=method)
+        res, _ = O.me_search(prm, pu + mvp, cur_plane(cur), planes)
+        exp_cost, exp_mv = None, None
+        (xmin, xmax), (ymin, ymax) = npref.mv_window(w, h, *pu)
+        cost = cost_fn(cur, ref, pu, mvp, lam)
+        best = None
+        for q in (mvp, (0, 0)):
+            c = (min(max((q[0] + 2) >> 2, xmin), xmax), min(max((q[1] + 2) >> 2, ymin), ymax))
+            if best is None or cost(*c) < best[0]:
+                best = (cost(*c), c)
+        path, _ = run_path(method, cur, ref, pu, mvp, lam, best[1])
+        exp_cost, exp_mv = cost(*path[-1]), path[-1]
+        assert (res["bcost2"], res["bmv2_x"], res["bmv2_y"]) == (exp_cost, exp_mv[0], exp_mv[1])
+        assert (res["bmv_x"], res["bmv_y"]) == (4 * exp_mv[0], 4 * exp_mv[1])
+
+
+@pytest.mark.parametrize("method", [O.ME_DIA, O.ME_HEX])
+@pytest.mark.parametrize("shift", [(1, 0), (-2, 1), (2, 2)])
+def test_small_shift_found_like_full_search(method, shift):
+    w, h = 128, 96
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    ref = np.clip(np.rint(128 + 50 * np.sin(xx / 11.0 + 0.3) * np.cos(yy / 9.0 + 1.1)), 0, 255).astype(np.uint8)
+    # pred = ref[p + mv]: the current picture is the reference moved by -shift
+    cur = np.roll(ref, (-shift[1], -shift[0]), axis=(0, 1))
+    planes = [plane_of(ref)] + [None] * 15
+    lam = 4 * 65536
+    for pu in [(32, 32, 16, 16), (64, 40, 16, 16), (48, 24, 32, 32)]:
+        prm = O.me_param(w, h, 8, lam, use_fractional_me=0, me_method=method)
+        res, _ = O.me_search(prm, pu + (0, 0), cur_plane(cur), planes)
+        fs_cost, fs_mv = npref.fs_brute(cur, npref.clamp_picture(ref, O.PAD), O.PAD, pu, 8, lam)
+        assert fs_mv == shift
+        assert (res["bmv2_x"], res["bmv2_y"], res["bcost2"]) == (shift[0], shift[1], fs_cost)
+
+
+def low_freq(w, h, seed):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    f = 128 + 60 * np.sin(xx / 23.0 + rng.uniform(0, 6)) * np.cos(yy / 19.0 + rng.uniform(0, 6))
+    return np.clip(np.rint(f + rng.normal(0, 1, (h, w))), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("method", [O.ME_DIA, O.ME_HEX])
+def test_long_walks_turn_every_way(method):
+    """Shifts of up to 14 pixels on a low-frequency field: walks of many steps
+    that turn in every direction, so that every entry of the index tables of
+    the half patterns is used."""
+    w, h = 192, 160
+    lam = 4 * 65536
+    R = 24
+    turns = set()
+    for seed in range(6):
+        rng = np.random.default_rng(100 + seed)
+        ref = low_freq(w, h, seed)
+        shift = (int(rng.integers(-14, 15)), int(rng.integers(-14, 15)))
+        cur = np.roll(ref, (-shift[1], -shift[0]), axis=(0, 1))
+        for _ in range(6):
+            pu = (int(rng.integers(6, 30)) * 4, int(rng.integers(32, h - 48)), 16, 16)
+            (xmin, xmax), (ymin, ymax) = npref.mv_window(w, h, *pu)
+            path, cost = run_path(method, cur, ref, pu, (0, 0), lam, (0, 0), R)
+            check_path(method, path, cost, lambda mx, my: xmin <= mx <= xmax and ymin <= my <= ymax, R)
+            moves = [(b[0] - a[0], b[1] - a[1]) for a, b in zip(path, path[1:])]
+            turns.update(zip(moves, moves[1:]))
+    pat = HEX if method == O.ME_HEX else DIA
+    # consecutive moves of the same pattern: every move is followed by some
+    # other move at least once
+    firsts = {a for a, b in turns if a in pat and b in pat}
+    assert firsts == set(pat), sorted(turns)
+
+
+@pytest.mark.parametrize("method", [O.ME_DIA, O.ME_HEX])
+def test_walks_stop_where_they_leave_the_mv_window(method):
+    """An MV range of +-10 qpel leaves the window [-4, 4] (full-pel, after the
+    6-pixel margin) while the picture moved by 9 pixels: the walks reach the
+    window edge; the point past it ends the walk."""
+    w, h = 160, 128
+    ref = low_freq(w, h, 7)
+    lam = 4 * 65536
+    left = 0
+    for shift in [(9, 0), (-9, 3), (2, 9), (-7, -8)]:
+        cur = np.roll(ref, (-shift[1], -shift[0]), axis=(0, 1))
+        for pu in [(48, 40, 16, 16), (80, 64, 8, 8), (64, 48, 32, 16)]:
+            prm = O.me_param(w, h, 16, lam, use_fractional_me=0, me_method=method,
+                             mv_range_x=(-40, 40), mv_range_y=(-40, 40))
+            cost = cost_fn(cur, ref, pu, (0, 0), lam)
+            path = O.me_pattern_path(prm, pu + (0, 0), cur_plane(cur), plane_of(ref), (0, 0), cost(0, 0))
+            check_path(method, path, cost, lambda mx, my: -4 <= mx <= 4 and -4 <= my <= 4, 16)
+            left += not (-4 <= path[-1][0] <= 4 and -4 <= path[-1][1] <= 4)
+    assert left > 0
