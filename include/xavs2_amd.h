@@ -113,7 +113,17 @@ typedef struct {
     int32_t use_fractional_me;       /* 0 integer only, 1 + half-pel, 2 + quarter-pel */
     int32_t use_fast_sub_me;         /* Spiral2 with step 2 (me.c:501-502) */
     int32_t enable_pmvr;             /* PMVR quarter-pel adaptation (me.c:549-556) */
+    int32_t me_method;               /* h->param->me_method: XME_ME_FS, XME_ME_DIA or XME_ME_HEX */
 } xme_me_params;
+
+/* Integer search methods, enum me_mothod_e (source/common/common.h:145-151).
+ * XME_ME_FS is the full search over [-R, R)^2 (me.c:1156-1164); XME_ME_DIA
+ * the small-diamond descent (me.c:1133-1154); XME_ME_HEX the hexagon descent
+ * followed by the diamond (me.c:1107-1131).  UMH (3) and TZ (4) are not
+ * provided. */
+#define XME_ME_FS  0
+#define XME_ME_DIA 1
+#define XME_ME_HEX 2
 
 typedef struct {
     int16_t x, y;                    /* PU position (luma pixels) */
@@ -154,8 +164,14 @@ typedef struct xme_plan xme_plan;
 int xme_plan_create(const xme_me_params *params, const xme_pu *pus, int n_pu, int n_ref, int mode,
                     xme_plan **plan);
 int xme_plan_destroy(xme_plan *plan);
-/* Per-plan counters: number of PUs x refs, integer candidates evaluated per run. */
+/* Per-plan counters: number of PUs x refs, integer candidates evaluated per
+ * run and their byte absdiffs (full search; 0 for the pattern searches, whose
+ * work depends on the pictures: see xme_plan_search_count). */
 int xme_plan_info(const xme_plan *plan, int64_t *n_inst, int64_t *n_candidates, int64_t *n_absdiff);
+/* Pattern-search plans: the integer points evaluated, and their byte
+ * absdiffs, summed over every run of the plan so far (a blocking read after
+ * the plan's work on the null stream; 0 for full-search plans). */
+int xme_plan_search_count(const xme_plan *plan, int64_t *n_points, int64_t *n_absdiff);
 /* Caps the workgroups of the batched plan's persistent search launch (0 =
  * as many as the device holds at once), so that searches of concurrent
  * frames share the compute units instead of queueing behind each other. */
@@ -168,8 +184,9 @@ int xme_plan_set_launch_split(xme_plan *plan, int64_t max_absdiff_per_launch);
 
 /* Run the search for one frame.  d_cur: current luma plane origin (no padding
  * needed).  refs: host array of n_ref descriptors.  d_results: n_pu * n_ref
- * entries, entry pu * n_ref + r.  d_cost_map (optional): per entry (2R)^2
- * integer costs of the raster pass, -1 where CHECK_MV_RANGE rejects.  */
+ * entries, entry pu * n_ref + r.  d_cost_map (optional, full search only):
+ * per entry (2R)^2 integer costs of the raster pass, -1 where CHECK_MV_RANGE
+ * rejects.  */
 int xme_me_run(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
                xme_result *d_results, int32_t *d_cost_map, void *stream);
 /* Capture the kernels of xme_me_run into a HIP graph on the first call and

@@ -206,6 +206,10 @@ struct xme_plan {
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t graph_stream = nullptr;
     size_t dev_bytes = 0;
+    // pattern-search plans: instance ids per wave and class bounds, counters
+    int32_t *d_pat_list = nullptr;
+    int pat_wave_end[4] = { 0, 0, 0, 0 };
+    unsigned long long *d_pat_counts = nullptr;
 };
 
 namespace {
@@ -422,6 +426,8 @@ static void plan_free(xme_plan *pl)
     hipFree(pl->d_items);
     hipFree(pl->d_item_insts);
     hipFree(pl->d_sp_list);
+    hipFree(pl->d_pat_list);
+    hipFree(pl->d_pat_counts);
     hipFree(pl->d_counters);
     hipFree(pl->d_desc);
     for (auto &s : pl->slots) {
@@ -461,7 +467,13 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         xme_set_error("xme_plan_create: search range must be 1..128 and the frame 8..8192 pixels");
         return XME_ERR_ARG;
     }
+    if (prm->me_method != XME_ME_FS && prm->me_method != XME_ME_DIA && prm->me_method != XME_ME_HEX) {
+        xme_set_error("xme_plan_create: me_method %d: only full search (0), DIA (1) and HEX (2) are provided",
+                      prm->me_method);
+        return XME_ERR_ARG;
+    }
     DevParams p;
+    p.method = prm->me_method;
     p.frame_w = prm->frame_w;
     p.frame_h = prm->frame_h;
     p.R = prm->search_range;
@@ -501,9 +513,13 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         parts[i] = part;
         MvWindow w = xme_mv_window(p, d.x, d.y, d.w, d.h);
         // every MV difference the search can form must stay inside mvbits
+        // (the pattern searches evaluate points up to 2 pixels outside the
+        // window)
+        const int margin = p.method == XME_ME_FS ? 1 : 2;
         for (int a = 0; a < 2; a++) {
             const int pm = a ? d.mvp_y : d.mvp_x;
-            const int lo = std::min(w.min_fpel[a], (pm + 2) >> 2) - 1, hi = std::max(w.max_fpel[a], (pm + 2) >> 2) + 1;
+            const int lo = std::min(w.min_fpel[a], (pm + 2) >> 2) - margin;
+            const int hi = std::max(w.max_fpel[a], (pm + 2) >> 2) + margin;
             if (std::abs(4 * lo - pm) + 3 > max_mvd || std::abs(4 * hi - pm) + 3 > max_mvd) {
                 xme_set_error("xme_plan_create: PU %d: MV differences leave the mvbits table", i);
                 return XME_ERR_ARG;
@@ -523,7 +539,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     // work accounting: legal integer candidates and absolute differences
     pl->n_cand = 0;
     pl->n_absdiff = 0;
-    for (int i = 0; i < n_pu; i++) {
+    for (int i = 0; i < n_pu && p.method == XME_ME_FS; i++) {
         MvWindow w = xme_mv_window(p, pus[i].x, pus[i].y, pus[i].w, pus[i].h);
         int c[2][2];
         possible_centres(p, pus[i], c);
@@ -570,7 +586,25 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         return lds;
     };
 
-    if (mode == XME_PLAN_BATCHED) {
+    std::vector<int32_t> pat_list;
+    if (p.method != XME_ME_FS) {
+        // pattern searches: one launch in either mode, the instances of each
+        // PU-size class packed 64 / S per wave, larger classes first
+        static const int per_wave[4] = { 4, 2, 1, 1 };
+        int waves = 0;
+        std::vector<int32_t> cls[4];
+        for (int i = 0; i < n_pu; i++)
+            for (int r = 0; r < n_ref; r++)
+                cls[xme_pattern_class(pus[i].w, pus[i].h)].push_back(i * n_ref + r);
+        for (int c = 0; c < 4; c++) {
+            for (int32_t id : cls[c])
+                pat_list.push_back(id);
+            while (pat_list.size() % per_wave[c])
+                pat_list.push_back(-1);
+            waves += (int)cls[c].size() / per_wave[c] + ((int)cls[c].size() % per_wave[c] ? 1 : 0);
+            pl->pat_wave_end[c] = waves;
+        }
+    } else if (mode == XME_PLAN_BATCHED) {
         // group PUs by 64x64 CTU; split the candidate rows until the window fits.
         // Pass 0 builds the generic tasks, pass 1 the narrow-PU tasks.
         std::map<std::pair<int, int>, std::vector<int>> ctus;
@@ -632,7 +666,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         pl->lds_narrow = lds_pass[1];
         pl->n_tasks_narrow = (int)tasks.size() - begin_pass[1];
     } else {
-        // plain: every (PU, ref) is its own task; one launch per PU shape
+        // plain full search: every (PU, ref) is its own task; one launch per PU shape
         for (int part = 0; part < XME_NUM_PU; part++) {
             LaunchGroup g;
             g.task_begin = (int)tasks.size();
@@ -705,6 +739,13 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     up((void **)&pl->d_tasks, tasks.data(), tasks.size() * sizeof(FsTask));
     up((void **)&pl->d_items, items.data(), items.size() * sizeof(FsItem));
     up((void **)&pl->d_item_insts, item_insts.data(), item_insts.size() * sizeof(int32_t));
+    up((void **)&pl->d_pat_list, pat_list.data(), pat_list.size() * sizeof(int32_t));
+    if (e == hipSuccess && p.method != XME_ME_FS) {
+        e = hipMalloc((void **)&pl->d_pat_counts, 2 * sizeof(unsigned long long));
+        pl->dev_bytes += 2 * sizeof(unsigned long long);
+        if (e == hipSuccess)
+            e = hipMemset(pl->d_pat_counts, 0, 2 * sizeof(unsigned long long));
+    }
     {
         std::vector<int32_t> sp_list;
         sp_list.reserve(pus.size());
@@ -743,7 +784,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         plan_free(pl);
         return rc;
     }
-    if (mode == XME_PLAN_BATCHED) {
+    if (mode == XME_PLAN_BATCHED && p.method == XME_ME_FS) {
         int prc = xme_fs_prepare();
         if (prc) {
             plan_free(pl);
@@ -778,6 +819,20 @@ extern "C" int xme_plan_info(const xme_plan *pl, int64_t *n_inst, int64_t *n_can
         *n_cand = pl->n_cand;
     if (n_absdiff)
         *n_absdiff = pl->n_absdiff;
+    return XME_OK;
+}
+
+extern "C" int xme_plan_search_count(const xme_plan *pl, int64_t *n_points, int64_t *n_absdiff)
+{
+    if (!pl)
+        return XME_ERR_ARG;
+    unsigned long long c[2] = { 0, 0 };
+    if (pl->d_pat_counts)
+        XME_HIP(hipMemcpy(c, pl->d_pat_counts, sizeof(c), hipMemcpyDeviceToHost));
+    if (n_points)
+        *n_points = (int64_t)c[0];
+    if (n_absdiff)
+        *n_absdiff = (int64_t)c[1];
     return XME_OK;
 }
 
@@ -892,6 +947,9 @@ static int fill_desc(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_r
 static int launch_search(xme_plan *pl, hipStream_t s)
 {
     int rc = XME_OK;
+    if (pl->p.method != XME_ME_FS)
+        return xme_launch_pattern(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->d_pat_list,
+                                  pl->pat_wave_end, pl->d_pat_counts, s);
     if (pl->mode == XME_PLAN_BATCHED) {
         // launches of one variant run in stream order and share its queue
         // counters (the last workgroup of a launch resets them)
@@ -938,6 +996,10 @@ extern "C" int xme_me_run(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const 
 {
     if (!pl)
         return XME_ERR_ARG;
+    if (cost_map && pl->p.method != XME_ME_FS) {
+        xme_set_error("xme_me_run: cost maps are defined for the full search only");
+        return XME_ERR_ARG;
+    }
     hipStream_t s = (hipStream_t)stream;
     int rc = fill_desc(pl, cur, i_cur, refs, res, cost_map, s);
     if (rc)

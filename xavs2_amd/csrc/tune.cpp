@@ -134,6 +134,7 @@ extern "C" int xme_tune(const xme_me_params *prm, const int *parts, int n_parts,
     p.frame_w = W;
     p.frame_h = H;
     p.use_fractional_me = 0;
+    p.me_method = XME_ME_FS;       // the tiles are those of the full search
     xme_ref_desc ref;
     memset(&ref, 0, sizeof(ref));
     ref.planes[0] = d_ref + (size_t)XME_PAD * stride + XME_PAD;

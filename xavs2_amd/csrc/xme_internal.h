@@ -26,6 +26,7 @@ struct DevParams {
     int32_t min_mv[2], max_mv[2];
     int32_t hadamard, frac, fast, pmvr;
     int32_t n_pu, n_ref;
+    int32_t method;            // XME_ME_FS, XME_ME_DIA or XME_ME_HEX
 };
 
 // Per-frame pointers, read by the kernels from device memory so that a captured
@@ -175,6 +176,15 @@ int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pu
 bool xme_subpel_large(int w, int h);
 int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
                       const uint64_t *d_best, const int32_t *d_list, int n_small, int n_large, hipStream_t s);
+// Pattern searches (XME_ME_DIA, XME_ME_HEX) of every instance: d_list holds
+// the instance ids of the four PU-size classes (xme_pattern_class), padded
+// per wave (4, 2, 1 and 1 ids per wave, -1 for none); wave_end[c] is the
+// first wave after class c.  d_counts[0..1] accumulate the points evaluated
+// and their byte absdiffs.
+int xme_pattern_class(int w, int h);
+int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
+                       uint64_t *d_best, const int32_t *d_list, const int wave_end[4], unsigned long long *d_counts,
+                       hipStream_t s);
 int xme_fs_max_lds(void);
 int xme_fs_prepare(void);
 // Device bytes a plan holds (work lists, per-instance state, descriptor).

@@ -31,7 +31,11 @@ class MeParams(C.Structure):
     _fields_ = [("frame_w", C.c_int32), ("frame_h", C.c_int32), ("search_range", C.c_int32),
                 ("lambda_factor", C.c_int32), ("min_mv_range", C.c_int32 * 2), ("max_mv_range", C.c_int32 * 2),
                 ("use_hadamard", C.c_int32), ("use_fractional_me", C.c_int32), ("use_fast_sub_me", C.c_int32),
-                ("enable_pmvr", C.c_int32)]
+                ("enable_pmvr", C.c_int32), ("me_method", C.c_int32)]
+
+
+# integer search methods (XME_ME_*, h->param->me_method)
+ME_FS, ME_DIA, ME_HEX = 0, 1, 2
 
 
 PU_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
@@ -86,6 +90,7 @@ def lib():
         L.xme_plan_workgroups.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.xme_plan_set_launch_split.argtypes = [P, C.c_int64]
         L.xme_plan_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.xme_plan_search_count.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xme_me_run.argtypes = [P, P, Z, P, P, P, P]
         L.xme_me_run_graph.argtypes = [P, P, Z, P, P, P]
         L.xme_me_time.argtypes = [P, P, Z, P, P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), P]
@@ -239,8 +244,10 @@ class DeviceRef:
 
 
 def me_params(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use_fractional_me=2,
-              use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047)):
+              use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047),
+              me_method=ME_FS):
     p = MeParams()
+    p.me_method = me_method
     p.frame_w, p.frame_h, p.search_range, p.lambda_factor = frame_w, frame_h, search_range, lambda_factor
     p.min_mv_range[0], p.max_mv_range[0] = mv_range_x
     p.min_mv_range[1], p.max_mv_range[1] = mv_range_y
@@ -278,6 +285,12 @@ class Plan:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         check(lib().xme_plan_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "xme_plan_info")
         return a.value, b.value, c.value
+
+    def search_count(self):
+        """(points, byte absdiffs) evaluated by a pattern-search plan over all its runs."""
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().xme_plan_search_count(self.h, C.byref(a), C.byref(b)), "xme_plan_search_count")
+        return a.value, b.value
 
     def _refs(self, refs):
         arr = (RefDesc * len(refs))()
