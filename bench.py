@@ -13,7 +13,8 @@ graph.
 Printed: one JSON line with candidate evaluations per second (batched path),
 the roofline of the integer-search kernel, the plain one-launch-per-PU-shape
 GPU path on the same job, per-frame latency p50/p99, and the CPU oracle timed
-on a bounded sample of the same job.
+on a bounded sample of the same job.  Side legs: the DIA and HEX pattern
+searches on the same frame, the residual transform, and the serving loop.
 """
 import argparse
 import json
@@ -110,6 +111,67 @@ def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
                   f"half/quarter-pel SATD, {elapsed:.2f} s on {threads} threads",
         "parity_with_gpu_on_sample": ok,
     }
+
+
+def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, steps, cpu):
+    """The DIA and HEX integer searches (+ the same sub-pel refinement) on the
+    same frame and PUs: job time per frame (graph replays), search-kernel time
+    (HIP events), integer points evaluated, and parity with the oracle for
+    every PU against reference 0, which is also the CPU time reported."""
+    from oracle import oracle as O
+    pu_arr = X.make_pus([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus])
+    planes0, cp = None, None
+    if cpu:
+        p = O.Plane(w, h)
+        p.set_picture(refs[0])
+        p.expand_border()
+        planes0 = O.interp_planes(p, 2)
+        cp = O.Plane(w, h)
+        cp.set_picture(cur)
+    threads = min(16, os.cpu_count() or 1)
+    out = {}
+    for name, method in (("dia", X.ME_DIA), ("hex", X.ME_HEX)):
+        plan = X.Plan(X.me_params(w, h, R, lam, me_method=method), pu_arr, n_ref, X.PLAN_BATCHED)
+        for _ in range(2):
+            plan.run_graph(dcur, d_refs, stream.h)
+        stream.sync()
+        res = plan.fetch()
+        X.lib().xme_device_sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.run_graph(dcur, d_refs, stream.h)
+        stream.sync()
+        X.lib().xme_device_sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        points, absdiffs = plan.search_count()
+        search_ms, job_ms, _ = plan.time(dcur, d_refs, stream.h, 5)
+        stable = bool(np.array_equal(plan.fetch(), res))
+        n_inst = len(pus) * n_ref
+        leg = {
+            "ms_per_frame": ms,
+            "pu_refs_per_s": n_inst / (ms * 1e-3),
+            "search_kernel_ms": search_ms,
+            "job_ms_events": job_ms,
+            "points_per_frame": points,
+            "points_per_pu_ref": points / n_inst,
+            "absdiffs_per_frame": absdiffs,
+            "search_tabsdiff_per_s": absdiffs / (search_ms * 1e-3) / 1e12,
+            "results_stable_across_replays": stable,
+        }
+        if cpu:
+            arr = np.array([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus], dtype=O.PU_DTYPE)
+            prm_o = O.me_param(w, h, R, lam, me_method=method)
+            t0 = time.perf_counter()
+            exp = O.me_search_batch(prm_o, arr, cp, planes0, nthreads=threads)
+            el = time.perf_counter() - t0
+            got = res[0::n_ref]
+            leg["parity_with_oracle_ref0"] = bool(all(np.array_equal(got[f], exp[f]) for f in exp.dtype.names))
+            leg["cpu_oracle"] = {"pu_refs_per_s": len(pus) / el, "threads": threads, "seconds": el,
+                                 "sample": f"all {len(pus)} PUs x reference 0"}
+        log(f"{name}: {ms:.3f} ms/frame, search {search_ms:.3f} ms, {points / n_inst:.1f} points per PU-ref")
+        out[name] = leg
+        del plan
+    return out
 
 
 def transform_leg(X, blocks_total, reps, seed=91):
@@ -320,6 +382,7 @@ def main():
                     help="%% of a full-device search launch per frame in the serving leg (0 = 100); a cap "
                          "leaves CUs for frames that arrive while a large one searches")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
+    ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX pattern-search leg")
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
@@ -421,6 +484,11 @@ def main():
     plain_value = n_cand / (plain_ms * 1e-3)
 
     del plain
+    pattern = None
+    if not args.no_pattern:
+        log("pattern-search leg (DIA, HEX)")
+        pattern = pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, args.steps,
+                              not args.no_cpu)
     transform = None
     if not args.no_transform:
         log(f"residual transform leg: {args.transform_blocks} blocks")
@@ -472,6 +540,7 @@ def main():
                        "mean": float(lat.mean())},
         "results_stable_across_replays": stable,
         "cpu_baseline": cpu,
+        "pattern_search": pattern,
         "transform": transform,
         "serving": serving,
     }

@@ -168,9 +168,9 @@ int xme_plan_destroy(xme_plan *plan);
  * run and their byte absdiffs (full search; 0 for the pattern searches, whose
  * work depends on the pictures: see xme_plan_search_count). */
 int xme_plan_info(const xme_plan *plan, int64_t *n_inst, int64_t *n_candidates, int64_t *n_absdiff);
-/* Pattern-search plans: the integer points evaluated, and their byte
- * absdiffs, summed over every run of the plan so far (a blocking read after
- * the plan's work on the null stream; 0 for full-search plans). */
+/* Pattern-search plans: the integer points the last run evaluated, and their
+ * byte absdiffs (a blocking read after the plan's work on the null stream; 0
+ * for full-search plans). */
 int xme_plan_search_count(const xme_plan *plan, int64_t *n_points, int64_t *n_absdiff);
 /* Caps the workgroups of the batched plan's persistent search launch (0 =
  * as many as the device holds at once), so that searches of concurrent

@@ -3,13 +3,15 @@
 //
 // The search of one instance is a chain of dependent steps (encoder/me.c:
 // 1101-1154): each step evaluates 3, 4 or 6 points around the current best
-// and moves to the first strict minimum.  A group of S lanes (16, 32 or 64 by
-// PU area) walks one instance; a wave holds 64 / S instances of one size
-// class and steps until all of them have stopped.  Every lane of a group
-// holds the instance's state, sums its share of the PU's pixel quads for all
-// points of the step, and the group's DPP sums come back to every lane with
-// one bpermute per point.  Reference pixels are read from global memory (a
-// step reads a few PU-sized areas next to the previous step's, from L2).
+// and moves to the first strict minimum.  A group of S lanes (4 to 64, so that
+// a lane holds at most 4 of the PU's pixel quads, 16 for the largest PUs)
+// walks one instance; a wave holds 64 / S instances of one size class and
+// steps until all of them have stopped.  Every lane of a group
+// holds the instance's state.  A step stages the box its points read (the PU
+// grown by 2 pixels a side) from the reference plane into the wave's LDS with
+// row-contiguous dword loads, then sums each lane's share of the PU's pixel
+// quads for all points of the step from LDS; the group's DPP sums come back
+// to every lane with one bpermute per point.
 #include "xme_internal.h"
 #include "xme_dpp.h"
 
@@ -48,11 +50,52 @@ __device__ __forceinline__ int tab(uint32_t t, int i, int bias) { return (int)((
 // Stages: the first hexagon, half hexagons, the first diamond, half diamonds.
 enum { kHexFirst = 0, kHexHalf = 1, kDiaFirst = 2, kDiaHalf = 3, kDone = 4 };
 
-// One wave: 64 / S instances of PUs with at most S * QMAX pixel quads.
-template <int S, int QMAX>
+// LDS dwords per wave: the step box of a 64x64 PU, 68 rows x 18 dwords.
+#define XME_PAT_BOX_DW 1224
+
+// One wave: 64 / S instances of PUs with at most S * QMAX pixel quads, whose
+// step box (PU + 2 pixels on every side, dword aligned) takes at most S * NB
+// dwords.
+// Offsets and direction code of point k of a step (k < the step's count).
+__device__ __forceinline__ void point_of(int phase, int idx, int k, int &dx, int &dy, int &dc)
+{
+    if (phase == kHexFirst) {
+        dx = tab(kHex6X, k, 2);
+        dy = tab(kHex6Y, k, 2);
+        dc = tab(kHex6D, k, 0);
+    } else if (phase == kHexHalf) {
+        dx = tab(kHex2X, min(idx + k, 7), 2);
+        dy = tab(kHex2Y, min(idx + k, 7), 2);
+        dc = k + 1;
+    } else if (phase == kDiaFirst) {
+        dx = tab(kDia4X, k, 2);
+        dy = tab(kDia4Y, k, 2);
+        dc = tab(kDia4D, k, 0);
+    } else {
+        dx = tab(kDia1X, min(idx + k, 5), 2);
+        dy = tab(kDia1Y, min(idx + k, 5), 2);
+        dc = k + 1;
+    }
+}
+
+// COPY4_IF_LT (me.h:76-82) of one point: SAD + MV_COST_IPEL against the
+// running best with a strict '<'.
+__device__ __forceinline__ void take_point(const DevParams &p, const DevPu &pu, int mx, int my, int dc, int sad,
+                                           int &bcost, int &nbx, int &nby, int &dir)
+{
+    const int cost = sad + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+    if (cost < bcost) {
+        bcost = cost;
+        nbx = mx;
+        nby = my;
+        dir = dc;
+    }
+}
+
+template <int S, int QMAX, int NB, bool SEQ>
 __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ desc, const DevPu *__restrict__ pus,
                              DevInst *__restrict__ inst, uint64_t *__restrict__ best, int id,
-                             unsigned long long *__restrict__ counts)
+                             uint32_t *__restrict__ points, uint32_t *__restrict__ box)
 {
     const int lane = threadIdx.x & 63;
     const int l = lane & (S - 1);
@@ -76,12 +119,22 @@ __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ d
     const int r = id >= 0 ? id - (id / p.n_ref) * p.n_ref : 0;
     const MvWindow win = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
     const int64_t i_ref = desc->stride[r];
-    const gbytes ref = (gbytes)desc->planes[r][0] + (int64_t)pu.y * i_ref + pu.x;
+    const gbytes plane = (gbytes)desc->planes[r][0];
+    // the group's box in LDS: pu.h + 4 rows of DW dwords, in the group's
+    // fixed share of the wave's region (the groups of a wave may hold PUs of
+    // different shapes); this lane stages slots l, l + S, ... of it, starting
+    // at row rr0, dword cc0 and moving on by SR rows and SC dwords per slot
+    const int DW = (pu.w + 10) >> 2;
+    uint32_t *gbox = box + (lane / S) * (XME_PAT_BOX_DW * S / 64);
+    const int nbox = (pu.h + 4) * DW;
+    const int rr0 = l / DW, cc0 = l - rr0 * DW, SR = S / DW, SC = S - SR * DW;
+    const int32_t i_ref32 = (int32_t)i_ref;
 
-    // this lane's pixel quads of the PU: current dwords and offsets
+    // this lane's pixel quads of the PU: current dwords and byte offsets of
+    // the quads in the box
     const int gw = pu.w >> 2, quads = gw * pu.h;
     uint32_t c4[QMAX];
-    int32_t off[QMAX];
+    int32_t qoff[QMAX];
     int nq = 0;
     {
         const gbytes cur = (gbytes)desc->cur + (int64_t)pu.y * desc->i_cur + pu.x;
@@ -89,11 +142,11 @@ __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ d
         for (int q = 0; q < QMAX; q++) {
             const int u = l + q * S;
             c4[q] = 0;
-            off[q] = 0;
+            qoff[q] = 0;
             if (phase != kDone && u < quads) {
                 const int yy = u / gw, xx = (u - yy * gw) * 4;
                 c4[q] = *(gdwords)(cur + (int64_t)yy * desc->i_cur + xx);
-                off[q] = yy * (int32_t)i_ref + xx;
+                qoff[q] = yy * DW * 4 + xx;
                 nq = q + 1;
             }
         }
@@ -112,66 +165,105 @@ __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ d
             phase = kDone;
         if (__builtin_amdgcn_ballot_w64(phase != kDone) == 0)
             break;
-        const int n = phase == kHexFirst ? 6 : phase == kDiaFirst ? 4 : phase == kDone ? 0 : 3;
-        int dx[6], dy[6], dc[6];
+        // every point of a step lies within 2 pixels of the current best,
+        // which is inside the MV window: the box (PU + 2 pixels a side, from
+        // the dword below) stays inside the plane's padding
+        const int x0 = pu.x + bmx - 2, ax0 = x0 & ~3, sh = x0 - ax0;
+        {
+            // in batches of at most 8 loads in flight per lane (registers)
+            const gbytes rows = plane + (int64_t)(pu.y + bmy - 2) * i_ref + ax0;
+            constexpr int NBB = NB < 8 ? NB : 8;
+            // the slot offsets are the same every step: opaque copies of the
+            // start keep the compiler from holding all NB of them in
+            // registers across the loop
+            int lo = l, rr = rr0, cc = cc0;
+            asm volatile("" : "+v"(lo), "+v"(rr), "+v"(cc));
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            if (phase == kHexFirst) {
-                dx[k] = tab(kHex6X, k, 2);
-                dy[k] = tab(kHex6Y, k, 2);
-                dc[k] = tab(kHex6D, k, 0);
-            } else if (phase == kHexHalf) {
-                dx[k] = tab(kHex2X, min(idx + k, 7), 2);
-                dy[k] = tab(kHex2Y, min(idx + k, 7), 2);
-                dc[k] = k + 1;
-            } else if (phase == kDiaFirst) {
-                dx[k] = tab(kDia4X, k, 2);
-                dy[k] = tab(kDia4Y, k, 2);
-                dc[k] = tab(kDia4D, k, 0);
-            } else {
-                dx[k] = tab(kDia1X, min(idx + k, 5), 2);
-                dy[k] = tab(kDia1Y, min(idx + k, 5), 2);
-                dc[k] = k + 1;
+            for (int b0 = 0; b0 < NB; b0 += NBB) {
+                uint32_t v[NBB];
+#pragma unroll
+                for (int b = 0; b < NBB; b++) {
+                    const int t = lo + (b0 + b) * S;
+                    v[b] = 0;
+                    if (phase != kDone && b0 + b < NB && t < nbox)
+                        v[b] = *(gdwords)(rows + (rr * i_ref32 + 4 * cc));
+                    cc += SC;
+                    rr += SR;
+                    if (cc >= DW) {
+                        cc -= DW;
+                        rr++;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < NBB; b++) {
+                    const int t = lo + (b0 + b) * S;
+                    if (phase != kDone && b0 + b < NB && t < nbox)
+                        gbox[t] = v[b];
+                }
+                // keeps the next batch's loads after these stores
+                __builtin_amdgcn_wave_barrier();
             }
         }
-        // SADs of the step's points (ME_COST_IPEL_X3_DIR / _X4_DIR evaluate
-        // them without a range check; they lie at most 2 pixels outside the
-        // MV window, inside the plane's padding)
-        uint32_t sad[6];
+        __builtin_amdgcn_wave_barrier();
+        const int n = phase == kHexFirst ? 6 : phase == kDiaFirst ? 4 : phase == kDone ? 0 : 3;
+        int dir = 0, nbx = bmx, nby = bmy;
+        if constexpr (SEQ) {
+            // one point at a time: few registers for PUs of many quads
+#pragma unroll 1
+            for (int k = 0; k < 6; k++) {
+                int dx, dy, dc;
+                point_of(phase, idx, k, dx, dy, dc);
+                uint32_t sad = 0;
+                if (k < n) {
+                    const int pb = (dy + 2) * DW * 4 + sh + dx + 2;
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            sad[k] = 0;
-            if (k < n) {
-                const gbytes base = ref + (int64_t)(bmy + dy[k]) * i_ref + (bmx + dx[k]);
+                    for (int q = 0; q < QMAX; q++) {
+                        if (q < nq) {
+                            const int a = qoff[q] + pb;
+                            const uint32_t d0 = gbox[a >> 2], d1 = gbox[(a >> 2) + 1];
+                            sad = __builtin_amdgcn_sad_u8(c4[q], __builtin_amdgcn_alignbyte(d1, d0, (uint32_t)(a & 3)), sad);
+                        }
+                    }
+                }
+                int sum = group_sum((int)sad, S);
+                sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
+                if (k < n)
+                    take_point(p, pu, bmx + dx, bmy + dy, dc, sum, bcost, nbx, nby, dir);
+            }
+        } else {
+            // SADs of all the step's points, then their sums in order
+            uint32_t sad[6];
 #pragma unroll
-                for (int q = 0; q < QMAX; q++) {
-                    if (q < nq) {
-                        const uintptr_t a = (uintptr_t)(base + off[q]);
-                        const gdwords d = (gdwords)(a & ~(uintptr_t)3);
-                        sad[k] = __builtin_amdgcn_sad_u8(c4[q], __builtin_amdgcn_alignbyte(d[1], d[0], (uint32_t)(a & 3)),
-                                                         sad[k]);
+            for (int k = 0; k < 6; k++) {
+                int dx, dy, dc;
+                point_of(phase, idx, k, dx, dy, dc);
+                sad[k] = 0;
+                if (k < n) {
+                    const int pb = (dy + 2) * DW * 4 + sh + dx + 2;
+#pragma unroll
+                    for (int q = 0; q < QMAX; q++) {
+                        if (q < nq) {
+                            const int a = qoff[q] + pb;
+                            const uint32_t d0 = gbox[a >> 2], d1 = gbox[(a >> 2) + 1];
+                            sad[k] = __builtin_amdgcn_sad_u8(c4[q], __builtin_amdgcn_alignbyte(d1, d0, (uint32_t)(a & 3)),
+                                                             sad[k]);
+                        }
                     }
                 }
             }
-        }
-        // group sums back to every lane of the group, then the points in
-        // order with a strict '<' (COPY4_IF_LT, me.h:76-82)
-        int dir = 0, nbx = bmx, nby = bmy;
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            int s = group_sum((int)sad[k], S);
-            s = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, s);
-            if (k < n) {
-                const int mx = bmx + dx[k], my = bmy + dy[k];
-                const int cost = s + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
-                if (cost < bcost) {
-                    bcost = cost;
-                    nbx = mx;
-                    nby = my;
-                    dir = dc[k];
+            for (int k = 0; k < 6; k++) {
+                int sum = group_sum((int)sad[k], S);
+                sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
+                if (k < n) {
+                    int dx, dy, dc;
+                    point_of(phase, idx, k, dx, dy, dc);
+                    take_point(p, pu, bmx + dx, bmy + dy, dc, sum, bcost, nbx, nby, dir);
                 }
             }
         }
+        // the next step's box overwrites this one's only after every lane's reads
+        __builtin_amdgcn_wave_barrier();
         evals += (uint32_t)n;
         bmx = nbx;
         bmy = nby;
@@ -195,38 +287,59 @@ __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ d
                 phase = kDone;
         }
     }
-    if (id >= 0 && l == S - 1 && evals) {
-        // the sub-pel pass reads the integer best as the centre's raster
-        // candidate (0, 0): order XME_ORDER_RASTER + R * 2R + R
-        inst[id].omx = (int16_t)bmx;
-        inst[id].omy = (int16_t)bmy;
-        best[id] = xme_key((uint32_t)bcost, (uint32_t)(XME_ORDER_RASTER + p.R * 2 * p.R + p.R));
-        atomicAdd(&counts[0], (unsigned long long)evals);
-        atomicAdd(&counts[1], (unsigned long long)evals * (unsigned long long)(pu.w * pu.h));
+    if (id >= 0 && l == S - 1) {
+        if (evals) {
+            // the sub-pel pass reads the integer best as the centre's raster
+            // candidate (0, 0): order XME_ORDER_RASTER + R * 2R + R
+            inst[id].omx = (int16_t)bmx;
+            inst[id].omy = (int16_t)bmy;
+            best[id] = xme_key((uint32_t)bcost, (uint32_t)(XME_ORDER_RASTER + p.R * 2 * p.R + p.R));
+        }
+        points[id] = evals;
     }
 }
 
-// Wave w of the launch: classes of 16-, 32-, 64- and 64-lane groups
-// (PUs of <= 64 px, < 256 px, <= 1024 px, larger) at wave offsets
-// wave_end[0..3]; list holds 64 / S instance ids per wave (-1: none).
+// Wave ranges of one launch, largest PUs first: range k holds PUs of
+// xme_pattern_class 5 - k, (64 / S) instance ids per wave from list[first[k]].
+struct PatLaunch {
+    int end[6];      // first wave after range k
+    int first[6];    // list index of range k's first id
+};
+
+// Launch ranges by class (quads = w * h / 4 of the PU), with their largest
+// box of (rows + 4) x ((w + 10) / 4) dwords per group, which fits the group's
+// share XME_PAT_BOX_DW * S / 64 (76, 153, 306, 612, 1224 dwords for S = 4,
+// 8, 16, 32, 64), and the staging slots per lane:
+//  5: > 256 quads, S 64, <= 16 quads and <= 20 slots per lane (64x64: 1224);
+//  4: <= 256, S 64, 7 slots (16x64: 408);  3: <= 128, S 32, 7 (16x32: 216);
+//  2: <= 64, S 16, 9 (8x32: 144);  1: <= 32, S 8, 10 (8x16: 80);
+//  0: <= 16, S 4, 15 (4x16: 60).  Every lane holds at most 4 quads of the
+// PU below class 5.
 __global__ void __launch_bounds__(256) pattern_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                       const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
                                                       uint64_t *__restrict__ best, const int32_t *__restrict__ list,
-                                                      int4 wave_end, unsigned long long *__restrict__ counts)
+                                                      PatLaunch L, uint32_t *__restrict__ points)
 {
+    __shared__ uint32_t boxes[4][XME_PAT_BOX_DW];
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     const int lane = threadIdx.x & 63;
-    if (w < wave_end.x) {
-        pattern_wave<16, 1>(p, desc, pus, inst, best, list[w * 64 / 16 + lane / 16], counts);
-    } else if (w < wave_end.y) {
-        const int e0 = wave_end.x * 4;
-        pattern_wave<32, 2>(p, desc, pus, inst, best, list[e0 + (w - wave_end.x) * 2 + lane / 32], counts);
-    } else if (w < wave_end.z) {
-        const int e0 = wave_end.x * 4 + (wave_end.y - wave_end.x) * 2;
-        pattern_wave<64, 4>(p, desc, pus, inst, best, list[e0 + (w - wave_end.y)], counts);
-    } else if (w < wave_end.w) {
-        const int e0 = wave_end.x * 4 + (wave_end.y - wave_end.x) * 2 + (wave_end.z - wave_end.y);
-        pattern_wave<64, 16>(p, desc, pus, inst, best, list[e0 + (w - wave_end.z)], counts);
+    uint32_t *box = boxes[threadIdx.x >> 6];
+    if (w < L.end[0]) {
+        pattern_wave<64, 16, 20, true>(p, desc, pus, inst, best, list[L.first[0] + w], points, box);
+    } else if (w < L.end[1]) {
+        pattern_wave<64, 4, 7, false>(p, desc, pus, inst, best, list[L.first[1] + (w - L.end[0])], points, box);
+    } else if (w < L.end[2]) {
+        pattern_wave<32, 4, 7, false>(p, desc, pus, inst, best, list[L.first[2] + (w - L.end[1]) * 2 + lane / 32],
+                                      points, box);
+    } else if (w < L.end[3]) {
+        pattern_wave<16, 4, 9, false>(p, desc, pus, inst, best, list[L.first[3] + (w - L.end[2]) * 4 + lane / 16],
+                                      points, box);
+    } else if (w < L.end[4]) {
+        pattern_wave<8, 4, 10, false>(p, desc, pus, inst, best, list[L.first[4] + (w - L.end[3]) * 8 + lane / 8],
+                                      points, box);
+    } else if (w < L.end[5]) {
+        pattern_wave<4, 4, 15, false>(p, desc, pus, inst, best, list[L.first[5] + (w - L.end[4]) * 16 + lane / 4],
+                                      points, box);
     }
 }
 
@@ -234,20 +347,29 @@ __global__ void __launch_bounds__(256) pattern_kernel(DevParams p, const FrameDe
 
 int xme_pattern_class(int w, int h)
 {
-    const int px = w * h;
-    return px <= 64 ? 0 : px < 256 ? 1 : px <= 1024 ? 2 : 3;
+    const int q = w * h / 4;
+    return q <= 16 ? 0 : q <= 32 ? 1 : q <= 64 ? 2 : q <= 128 ? 3 : q <= 256 ? 4 : 5;
 }
 
 int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
-                       uint64_t *d_best, const int32_t *d_list, const int wave_end[4], unsigned long long *d_counts,
+                       uint64_t *d_best, const int32_t *d_list, const int wave_end[6], uint32_t *d_points,
                        hipStream_t s)
 {
-    const int waves = wave_end[3];
+    const int waves = wave_end[5];
     if (waves <= 0)
         return XME_OK;
-    const int4 we = make_int4(wave_end[0], wave_end[1], wave_end[2], wave_end[3]);
+    // the list holds the ranges back to back, 64 / S ids per wave
+    static const int per_wave[6] = { 1, 1, 2, 4, 8, 16 };
+    PatLaunch L;
+    int first = 0, prev = 0;
+    for (int k = 0; k < 6; k++) {
+        L.end[k] = wave_end[k];
+        L.first[k] = first;
+        first += (wave_end[k] - prev) * per_wave[k];
+        prev = wave_end[k];
+    }
     hipLaunchKernelGGL(pattern_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best,
-                       d_list, we, d_counts);
+                       d_list, L, d_points);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }

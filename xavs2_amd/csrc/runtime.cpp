@@ -208,8 +208,9 @@ struct xme_plan {
     size_t dev_bytes = 0;
     // pattern-search plans: instance ids per wave and class bounds, counters
     int32_t *d_pat_list = nullptr;
-    int pat_wave_end[4] = { 0, 0, 0, 0 };
-    unsigned long long *d_pat_counts = nullptr;
+    int pat_wave_end[6] = { 0, 0, 0, 0, 0, 0 };
+    uint32_t *d_pat_points = nullptr;     // points evaluated per instance, last run
+    std::vector<int32_t> inst_area;        // PU area per instance
 };
 
 namespace {
@@ -427,7 +428,7 @@ static void plan_free(xme_plan *pl)
     hipFree(pl->d_item_insts);
     hipFree(pl->d_sp_list);
     hipFree(pl->d_pat_list);
-    hipFree(pl->d_pat_counts);
+    hipFree(pl->d_pat_points);
     hipFree(pl->d_counters);
     hipFree(pl->d_desc);
     for (auto &s : pl->slots) {
@@ -589,20 +590,20 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     std::vector<int32_t> pat_list;
     if (p.method != XME_ME_FS) {
         // pattern searches: one launch in either mode, the instances of each
-        // PU-size class packed 64 / S per wave, larger classes first
-        static const int per_wave[4] = { 4, 2, 1, 1 };
+        // PU-size class packed 64 / S per wave, the largest class first
+        static const int per_wave[6] = { 1, 1, 2, 4, 8, 16 };
         int waves = 0;
-        std::vector<int32_t> cls[4];
+        std::vector<int32_t> cls[6];
         for (int i = 0; i < n_pu; i++)
             for (int r = 0; r < n_ref; r++)
-                cls[xme_pattern_class(pus[i].w, pus[i].h)].push_back(i * n_ref + r);
-        for (int c = 0; c < 4; c++) {
-            for (int32_t id : cls[c])
+                cls[5 - xme_pattern_class(pus[i].w, pus[i].h)].push_back(i * n_ref + r);
+        for (int k = 0; k < 6; k++) {
+            for (int32_t id : cls[k])
                 pat_list.push_back(id);
-            while (pat_list.size() % per_wave[c])
-                pat_list.push_back(-1);
-            waves += (int)cls[c].size() / per_wave[c] + ((int)cls[c].size() % per_wave[c] ? 1 : 0);
-            pl->pat_wave_end[c] = waves;
+            const int nw = ((int)cls[k].size() + per_wave[k] - 1) / per_wave[k];
+            pat_list.resize(pat_list.size() + (size_t)(nw * per_wave[k] - (int)cls[k].size()), -1);
+            waves += nw;
+            pl->pat_wave_end[k] = waves;
         }
     } else if (mode == XME_PLAN_BATCHED) {
         // group PUs by 64x64 CTU; split the candidate rows until the window fits.
@@ -741,10 +742,13 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     up((void **)&pl->d_item_insts, item_insts.data(), item_insts.size() * sizeof(int32_t));
     up((void **)&pl->d_pat_list, pat_list.data(), pat_list.size() * sizeof(int32_t));
     if (e == hipSuccess && p.method != XME_ME_FS) {
-        e = hipMalloc((void **)&pl->d_pat_counts, 2 * sizeof(unsigned long long));
-        pl->dev_bytes += 2 * sizeof(unsigned long long);
+        e = hipMalloc((void **)&pl->d_pat_points, sizeof(uint32_t) * pl->n_inst);
+        pl->dev_bytes += sizeof(uint32_t) * pl->n_inst;
         if (e == hipSuccess)
-            e = hipMemset(pl->d_pat_counts, 0, 2 * sizeof(unsigned long long));
+            e = hipMemset(pl->d_pat_points, 0, sizeof(uint32_t) * pl->n_inst);
+        pl->inst_area.resize(pl->n_inst);
+        for (int i = 0; i < pl->n_inst; i++)
+            pl->inst_area[i] = pus[i / n_ref].w * pus[i / n_ref].h;
     }
     {
         std::vector<int32_t> sp_list;
@@ -826,13 +830,19 @@ extern "C" int xme_plan_search_count(const xme_plan *pl, int64_t *n_points, int6
 {
     if (!pl)
         return XME_ERR_ARG;
-    unsigned long long c[2] = { 0, 0 };
-    if (pl->d_pat_counts)
-        XME_HIP(hipMemcpy(c, pl->d_pat_counts, sizeof(c), hipMemcpyDeviceToHost));
+    int64_t pts = 0, ad = 0;
+    if (pl->d_pat_points) {
+        std::vector<uint32_t> h(pl->n_inst);
+        XME_HIP(hipMemcpy(h.data(), pl->d_pat_points, sizeof(uint32_t) * pl->n_inst, hipMemcpyDeviceToHost));
+        for (int i = 0; i < pl->n_inst; i++) {
+            pts += h[i];
+            ad += (int64_t)h[i] * pl->inst_area[i];
+        }
+    }
     if (n_points)
-        *n_points = (int64_t)c[0];
+        *n_points = pts;
     if (n_absdiff)
-        *n_absdiff = (int64_t)c[1];
+        *n_absdiff = ad;
     return XME_OK;
 }
 
@@ -949,7 +959,7 @@ static int launch_search(xme_plan *pl, hipStream_t s)
     int rc = XME_OK;
     if (pl->p.method != XME_ME_FS)
         return xme_launch_pattern(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->d_pat_list,
-                                  pl->pat_wave_end, pl->d_pat_counts, s);
+                                  pl->pat_wave_end, pl->d_pat_points, s);
     if (pl->mode == XME_PLAN_BATCHED) {
         // launches of one variant run in stream order and share its queue
         // counters (the last workgroup of a launch resets them)

@@ -176,14 +176,15 @@ int xme_launch_fs(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pu
 bool xme_subpel_large(int w, int h);
 int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, const DevInst *d_inst,
                       const uint64_t *d_best, const int32_t *d_list, int n_small, int n_large, hipStream_t s);
-// Pattern searches (XME_ME_DIA, XME_ME_HEX) of every instance: d_list holds
-// the instance ids of the four PU-size classes (xme_pattern_class), padded
-// per wave (4, 2, 1 and 1 ids per wave, -1 for none); wave_end[c] is the
-// first wave after class c.  d_counts[0..1] accumulate the points evaluated
-// and their byte absdiffs.
+// Pattern searches (XME_ME_DIA, XME_ME_HEX) of every instance.  The six PU
+// size classes of xme_pattern_class (0: <= 16 pixel quads ... 5: > 256) run
+// as wave ranges, largest class first: range k holds class 5 - k with
+// 1, 1, 2, 4, 8, 16 instance ids per wave (d_list back to back, padded per
+// wave with -1); wave_end[k] is the first wave after range k.  d_points[id]
+// receives the points instance id evaluated.
 int xme_pattern_class(int w, int h);
 int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
-                       uint64_t *d_best, const int32_t *d_list, const int wave_end[4], unsigned long long *d_counts,
+                       uint64_t *d_best, const int32_t *d_list, const int wave_end[6], uint32_t *d_points,
                        hipStream_t s);
 int xme_fs_max_lds(void);
 int xme_fs_prepare(void);

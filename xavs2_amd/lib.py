@@ -287,7 +287,7 @@ class Plan:
         return a.value, b.value, c.value
 
     def search_count(self):
-        """(points, byte absdiffs) evaluated by a pattern-search plan over all its runs."""
+        """(points, byte absdiffs) evaluated by the last run of a pattern-search plan."""
         a, b = C.c_int64(), C.c_int64()
         check(lib().xme_plan_search_count(self.h, C.byref(a), C.byref(b)), "xme_plan_search_count")
         return a.value, b.value
