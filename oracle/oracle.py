@@ -23,11 +23,16 @@ class MeParam(C.Structure):
                 ("lambda_factor", C.c_int), ("min_mv_range", C.c_int * 2),
                 ("max_mv_range", C.c_int * 2), ("use_hadamard", C.c_int),
                 ("use_fractional_me", C.c_int), ("use_fast_sub_me", C.c_int),
-                ("enable_pmvr", C.c_int), ("me_method", C.c_int)]
+                ("enable_pmvr", C.c_int), ("me_method", C.c_int), ("umh_big_hex_level", C.c_int)]
 
 
 # h->param->me_method (common/common.h:145-151)
-ME_FS, ME_DIA, ME_HEX = 0, 1, 2
+ME_FS, ME_DIA, ME_HEX, ME_UMH = 0, 1, 2, 3
+
+# xo_umh_ctx_t: per-(PU, reference) state read by the UMH search
+UMH_CTX_DTYPE = np.dtype([("mvp1_x", "<i4"), ("mvp1_y", "<i4"), ("mvp2_x", "<i4"), ("mvp2_y", "<i4"),
+                          ("mvp3_x", "<i4"), ("mvp3_y", "<i4"), ("ref_idx", "<i4"), ("pred_sad", "<i4"),
+                          ("pred_sad_uplayer", "<i4"), ("pad", "<i4"), ("beta2", "<f8"), ("beta3", "<f8")])
 
 
 class Pu(C.Structure):
@@ -72,8 +77,11 @@ def lib():
         L.xo_me_search.restype = C.c_int32
         L.xo_me_search.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, ip, P, ip, P, P]
         L.xo_me_search_batch.argtypes = [C.POINTER(MeParam), P, C.c_int, P, ip, P, ip, P, C.c_int]
-        L.xo_me_pattern_path.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, ip, P, ip, C.c_int, C.c_int,
-                                         C.c_int, P, C.c_int]
+        L.xo_me_pattern_path.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, P, C.c_int, P]
+        L.xo_me_search_ctx.restype = C.c_int32
+        L.xo_me_search_ctx.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, P]
+        L.xo_me_search_batch_ctx.argtypes = [C.POINTER(MeParam), P, P, C.c_int, P, ip, P, ip, P, C.c_int]
         L.xo_expand_border.argtypes = [P, ip, C.c_int, C.c_int, C.c_int]
         L.xo_interpolate_frame.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_mc_luma.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int, P, ip]
@@ -158,9 +166,10 @@ def lambda_factor(lam):
 
 def me_param(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use_fractional_me=2,
              use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047),
-             me_method=ME_FS):
+             me_method=ME_FS, umh_big_hex_level=0):
     p = MeParam()
     p.me_method = me_method
+    p.umh_big_hex_level = umh_big_hex_level
     p.frame_w, p.frame_h, p.search_range, p.lambda_factor = frame_w, frame_h, search_range, lambda_factor
     p.min_mv_range[0], p.max_mv_range[0] = mv_range_x
     p.min_mv_range[1], p.max_mv_range[1] = mv_range_y
@@ -198,24 +207,35 @@ def me_search(prm, pu_tuple, cur_plane, planes, cost_map=False):
     return out, (cm.reshape(2 * R, 2 * R) if cm is not None else None)
 
 
-def me_pattern_path(prm, pu_tuple, cur_plane, ref_plane, start, start_cost, max_pts=4096):
-    """Accepted points of the DIA / HEX integer stage from `start` (full-pel)
-    with cost `start_cost`: a list of (x, y), the start first, the integer
-    best last."""
+def me_pattern_path(prm, pu_tuple, cur_plane, ref_plane, start, start_cost, max_pts=4096, ctx=None, p0=None,
+                    with_route=False):
+    """Points of the DIA / HEX / UMH integer stage from `start` (full-pel)
+    with cost `start_cost`, the best after each step that moved it: a list of
+    (x, y), the start first, the integer best last.  ctx: a UMH_CTX_DTYPE
+    record; p0: the first MV candidate (default: start).  with_route also
+    returns the UMH route code (see xo_me_pattern_path)."""
     path = np.zeros(2 * max_pts, dtype=np.int32)
-    n = lib().xo_me_pattern_path(C.byref(prm), C.byref(Pu(*pu_tuple)), cur_plane.origin_ptr(), cur_plane.stride,
-                                 ref_plane.origin_ptr(), ref_plane.stride, start[0], start[1], start_cost,
-                                 ptr(path), max_pts)
+    route = C.c_int(0)
+    c = np.ascontiguousarray(ctx, dtype=UMH_CTX_DTYPE) if ctx is not None else None
+    p0 = start if p0 is None else p0
+    n = lib().xo_me_pattern_path(C.byref(prm), C.byref(Pu(*pu_tuple)), ptr(c) if c is not None else None,
+                                 cur_plane.origin_ptr(), cur_plane.stride, ref_plane.origin_ptr(), ref_plane.stride,
+                                 start[0], start[1], start_cost, p0[0], p0[1], ptr(path), max_pts, C.byref(route))
     assert n <= max_pts
-    return [(int(path[2 * k]), int(path[2 * k + 1])) for k in range(n)]
+    pts = [(int(path[2 * k]), int(path[2 * k + 1])) for k in range(n)]
+    return (pts, route.value) if with_route else pts
 
 
-def me_search_batch(prm, pus, cur_plane, planes, nthreads=1):
-    """pus: structured array of PU_DTYPE.  Returns structured array of ME_RESULT_DTYPE."""
+def me_search_batch(prm, pus, cur_plane, planes, nthreads=1, ctxs=None):
+    """pus: structured array of PU_DTYPE; ctxs: UMH_CTX_DTYPE array, one per
+    PU, or None.  Returns structured array of ME_RESULT_DTYPE."""
     pus = np.ascontiguousarray(pus, dtype=PU_DTYPE)
     res = np.zeros(len(pus), dtype=ME_RESULT_DTYPE)
-    lib().xo_me_search_batch(C.byref(prm), ptr(pus), len(pus), cur_plane.origin_ptr(),
-                             cur_plane.stride, plane_ptrs(planes), planes[0].stride, ptr(res), nthreads)
+    c = np.ascontiguousarray(ctxs, dtype=UMH_CTX_DTYPE) if ctxs is not None else None
+    assert c is None or len(c) == len(pus)
+    lib().xo_me_search_batch_ctx(C.byref(prm), ptr(pus), ptr(c) if c is not None else None, len(pus),
+                                 cur_plane.origin_ptr(), cur_plane.stride, plane_ptrs(planes), planes[0].stride,
+                                 ptr(res), nthreads)
     return res
 
 

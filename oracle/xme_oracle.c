@@ -259,6 +259,22 @@ static const int8_t DIA1[6][2] = {
 };
 static const int8_t M1MOD4[6] = { 3, 0, 1, 2, 3, 0 };
 
+/* UMH: the big hexagons and the 5x5 grid, encoder/me.c:69-79, :116-126.  The
+ * grid repeats (0, 2) where (0, -2) would complete it: (0, -2) is never
+ * evaluated, (0, 2) twice. */
+static const int8_t HEX4[16][2] = {
+    { 0, -4 }, { 0, 4 }, { -2, -3 }, { 2, -3 }, { -4, -2 }, { 4, -2 }, { -4, -1 }, { 4, -1 },
+    { -4, 0 }, { 4, 0 }, { -4, 1 }, { 4, 1 }, { -4, 2 }, { 4, 2 }, { -2, 3 }, { 2, 3 }
+};
+static const int8_t FAST_HEX4[8][2] = {
+    { 0, -4 }, { 0, 4 }, { -2, -3 }, { 2, -3 }, { -4, 0 }, { 4, 0 }, { -2, 3 }, { 2, 3 }
+};
+static const int8_t GRID[24][2] = {
+    { -1, -1 }, { 0, -1 }, { 1, -1 }, { -1, 0 }, { 1, 0 }, { -1, 1 }, { 0, 1 }, { 1, 1 },
+    { -2, -2 }, { -1, -2 }, { 0, 2 }, { 1, -2 }, { 2, -2 }, { -2, -1 }, { 2, -1 }, { -2, 0 },
+    { 2, 0 }, { -2, 1 }, { 2, 1 }, { -2, 2 }, { -1, 2 }, { 0, 2 }, { 1, 2 }, { 2, 2 }
+};
+
 typedef struct {
     int W, H, R, lambda, pmx, pmy;
     const uint8_t *org;
@@ -294,13 +310,146 @@ static void pat_accept(pat_ctx_t *c)
     c->n_pts++;
 }
 
-/* The integer stage of XAVS2_ME_HEX (me.c:1107-1131, falling through to the
- * diamond) and XAVS2_ME_DIA (me.c:1133-1154), from the best MV candidate. */
-static void pattern_search(pat_ctx_t *c, int method)
+/* ME_COST_IPEL (me.c:187-193): a range-checked point, strict '<', no
+ * direction code. */
+static void ipel_point(pat_ctx_t *c, int mx, int my)
 {
-    int omx, omy;
+    if (!check_mv_range(mx, my, c->mv_min, c->mv_max))
+        return;
+    int cost = xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
+               weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+    if (cost < c->bcost) {
+        c->bcost = cost;
+        c->bmx = mx;
+        c->bmy = my;
+    }
+}
+
+/* DIA_ITER (me.c:328-334) through ME_COST_IPEL_X4 (me.c:241-267): with all
+ * four points in the window they are taken in order without checks, else
+ * each one is checked; either way it is the four range-checked points in
+ * order. */
+static void dia_iter(pat_ctx_t *c, int mx, int my)
+{
+    ipel_point(c, mx, my - 1);
+    ipel_point(c, mx - 1, my);
+    ipel_point(c, mx + 1, my);
+    ipel_point(c, mx, my + 1);
+}
+
+static inline int ipel(int v) { return (v + 2) >> 2; }       /* IPEL, me.h:60 */
+
+/* EARLY_TERMINATION (me.c:165-170) with the search's beta + 1 (me.c:883-884):
+ * 3 jumps to the diamond (umh_step_3), 2 to the hexagon (umh_step_2). */
+static int early_termination(const pat_ctx_t *c, int pred_sad, double beta2, double beta3)
+{
+    if (c->bcost < pred_sad * beta3)
+        return 3;
+    if (c->bcost < pred_sad * beta2)
+        return 2;
+    return 0;
+}
+
+static void accept_if_moved(pat_ctx_t *c, int *lx, int *ly)
+{
+    if (c->bmx != *lx || c->bmy != *ly) {
+        pat_accept(c);
+        *lx = c->bmx;
+        *ly = c->bmy;
+    }
+}
+
+/* The UMH steps before the hexagon (me.c:1030-1099), from the best MV
+ * candidate; (p0x, p0y) is the first candidate (mvc[0], the MVP).  Returns
+ * the stage to go on with, 2 (hexagon) or 3 (diamond), and in *where the
+ * point it left: 0 after the big hexagons, 1..5 the early termination that
+ * jumped (after the predictor points, the second diamond, the cross, the
+ * grid, a big-hexagon level). */
+static int umh_prefix(pat_ctx_t *c, const xo_umh_ctx_t *u, int p0x, int p0y, int big_hex_level, int *where)
+{
+    const double beta2 = u->beta2 + 1, beta3 = u->beta3 + 1;
+    int lx = c->bmx, ly = c->bmy, et;
+    int pmx = p0x, pmy = p0y;                     /* pmv = MAKEDWORD(mvc[0]) */
+    dia_iter(c, p0x, p0y);
+    if ((pmx || pmy) && (c->bmx != p0x || c->bmy != p0y)) {
+        dia_iter(c, c->bmx, c->bmy);
+        pmx = c->bmx;
+        pmy = c->bmy;
+    }
+    if (u->mvp1_x || u->mvp1_y)
+        ipel_point(c, ipel(u->mvp1_x), ipel(u->mvp1_y));
+    accept_if_moved(c, &lx, &ly);
+    *where = 1;
+    if ((et = early_termination(c, u->pred_sad_uplayer, beta2, beta3)))
+        return et;
+    if (u->ref_idx > 0)
+        ipel_point(c, ipel(u->mvp2_x), ipel(u->mvp2_y));
+    if (u->mvp3_x || u->mvp3_y)
+        ipel_point(c, ipel(u->mvp3_x), ipel(u->mvp3_y));
+    if (pmx != c->bmx || pmy != c->bmy)
+        dia_iter(c, c->bmx, c->bmy);
+    accept_if_moved(c, &lx, &ly);
+    *where = 2;
+    if ((et = early_termination(c, u->pred_sad, beta2, beta3)))
+        return et;
+    /* unsymmetrical cross: odd steps to +-R horizontally, +-R/2 vertically */
+    int omx = c->bmx, omy = c->bmy;
+    for (int i = 1; i <= c->R; i += 2) {
+        ipel_point(c, omx + i, omy);
+        ipel_point(c, omx - i, omy);
+    }
+    for (int j = 1; j <= c->R / 2; j += 2) {
+        ipel_point(c, omx, omy + j);
+        ipel_point(c, omx, omy - j);
+    }
+    accept_if_moved(c, &lx, &ly);
+    *where = 3;
+    if ((et = early_termination(c, u->pred_sad, beta2, beta3)))
+        return et;
+    omx = c->bmx;
+    omy = c->bmy;
+    for (int i = 0; i < 24; i++)
+        ipel_point(c, omx + GRID[i][0], omy + GRID[i][1]);
+    accept_if_moved(c, &lx, &ly);
+    *where = 4;
+    if ((et = early_termination(c, u->pred_sad, beta2, beta3)))
+        return et;
+    if (big_hex_level) {
+        const int n = big_hex_level == 2 ? 16 : 8;
+        const int8_t (*pat)[2] = big_hex_level == 2 ? HEX4 : FAST_HEX4;
+        for (int j = 1; j <= c->R / 4; j++) {
+            omx = c->bmx;
+            omy = c->bmy;
+            for (int i = 0; i < n; i++)
+                ipel_point(c, omx + pat[i][0] * j, omy + pat[i][1] * j);
+            accept_if_moved(c, &lx, &ly);
+            if (c->bmx != omx || c->bmy != omy) {
+                *where = 5;
+                if ((et = early_termination(c, u->pred_sad, beta2, beta3)))
+                    return et;
+            }
+        }
+    }
+    *where = 0;
+    return 2;
+}
+
+/* The integer stage of XAVS2_ME_HEX (me.c:1107-1131, falling through to the
+ * diamond), XAVS2_ME_DIA (me.c:1133-1154) and XAVS2_ME_UMH (me.c:1030-1099,
+ * then the hexagon or the diamond), from the best MV candidate. */
+static int pattern_search(pat_ctx_t *c, int method, const xo_umh_ctx_t *u, int p0x, int p0y, int big_hex_level)
+{
+    int omx, omy, route = 0;
     pat_accept(c);
-    if (method == XO_ME_HEX) {
+    int hexagon = method == XO_ME_HEX;
+    if (method == XO_ME_UMH) {
+        static const xo_umh_ctx_t zero;
+        int where = 0;
+        const int to = umh_prefix(c, u ? u : &zero, p0x, p0y, big_hex_level, &where);
+        hexagon = to == 2;
+        route = where | (to << 4);
+    }
+    if (hexagon) {
         /* the full hexagon around the start, then half hexagons that skip
          * the points the previous step already evaluated */
         c->dir = 0;
@@ -353,6 +502,7 @@ static void pattern_search(pat_ctx_t *c, int method)
             idx = M1MOD4[c->dir + idx - 1];
         }
     }
+    return route;
 }
 
 static void pat_init(pat_ctx_t *c, const xo_me_param_t *prm, const xo_pu_t *pu, const uint8_t *cur,
@@ -373,10 +523,10 @@ static void pat_init(pat_ctx_t *c, const xo_me_param_t *prm, const xo_pu_t *pu, 
     c->mv_max = win->mv_max;
 }
 
-int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu,
+int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
                        const uint8_t *cur, intptr_t i_cur,
                        const uint8_t *plane0, intptr_t i_ref,
-                       int sx, int sy, int scost, int *path, int max_pts)
+                       int sx, int sy, int scost, int p0x, int p0y, int *path, int max_pts, int *route)
 {
     mv_window_t win;
     pat_ctx_t c;
@@ -387,14 +537,16 @@ int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu,
     c.bmy = sy;
     c.path = path;
     c.max_pts = max_pts;
-    pattern_search(&c, prm->me_method);
+    const int r = pattern_search(&c, prm->me_method, ctx, p0x, p0y, prm->umh_big_hex_level);
+    if (route)
+        *route = r;
     return c.n_pts;
 }
 
-int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
-                     const uint8_t *cur, intptr_t i_cur,
-                     const uint8_t *const planes[16], intptr_t i_ref,
-                     xo_me_result_t *res, int32_t *cost_map)
+static int32_t me_search_impl(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                              const uint8_t *cur, intptr_t i_cur,
+                              const uint8_t *const planes[16], intptr_t i_ref,
+                              xo_me_result_t *res, int32_t *cost_map)
 {
     const int R = prm->search_range;
     const int W = pu->w, H = pu->h;
@@ -448,13 +600,13 @@ int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
         return XO_MAX_DISTORTION;
     }
 
-    if (prm->me_method == XO_ME_DIA || prm->me_method == XO_ME_HEX) {
+    if (prm->me_method == XO_ME_DIA || prm->me_method == XO_ME_HEX || prm->me_method == XO_ME_UMH) {
         pat_ctx_t c;
         pat_init(&c, prm, pu, cur, i_cur, planes[0], i_ref, &win);
         c.bcost = bcost;
         c.bmx = bmx;
         c.bmy = bmy;
-        pattern_search(&c, prm->me_method);
+        pattern_search(&c, prm->me_method, ctx, mvc[0][0], mvc[0][1], prm->umh_big_hex_level);
         bcost = c.bcost;
         bmx = c.bmx;
         bmy = c.bmy;
@@ -536,9 +688,26 @@ store:
     return bcost;
 }
 
+int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
+                     const uint8_t *cur, intptr_t i_cur,
+                     const uint8_t *const planes[16], intptr_t i_ref,
+                     xo_me_result_t *res, int32_t *cost_map)
+{
+    return me_search_impl(prm, pu, NULL, cur, i_cur, planes, i_ref, res, cost_map);
+}
+
+int32_t xo_me_search_ctx(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                         const uint8_t *cur, intptr_t i_cur,
+                         const uint8_t *const planes[16], intptr_t i_ref,
+                         xo_me_result_t *res)
+{
+    return me_search_impl(prm, pu, ctx, cur, i_cur, planes, i_ref, res, NULL);
+}
+
 typedef struct {
     const xo_me_param_t *prm;
     const xo_pu_t *pus;
+    const xo_umh_ctx_t *ctxs;
     int n;
     const uint8_t *cur;
     intptr_t i_cur;
@@ -561,8 +730,8 @@ static void *batch_worker(void *arg)
             break;
         int e = i + 4 < ctx->n ? i + 4 : ctx->n;
         for (; i < e; i++)
-            xo_me_search(ctx->prm, &ctx->pus[i], ctx->cur, ctx->i_cur, ctx->planes, ctx->i_ref,
-                         &ctx->res[i], NULL);
+            me_search_impl(ctx->prm, &ctx->pus[i], ctx->ctxs ? &ctx->ctxs[i] : NULL, ctx->cur, ctx->i_cur,
+                           ctx->planes, ctx->i_ref, &ctx->res[i], NULL);
     }
     return NULL;
 }
@@ -572,7 +741,15 @@ void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,
                         const uint8_t *const planes[16], intptr_t i_ref,
                         xo_me_result_t *res, int nthreads)
 {
-    batch_ctx_t ctx = { prm, pus, n, cur, i_cur, planes, i_ref, res, 0, PTHREAD_MUTEX_INITIALIZER };
+    xo_me_search_batch_ctx(prm, pus, NULL, n, cur, i_cur, planes, i_ref, res, nthreads);
+}
+
+void xo_me_search_batch_ctx(const xo_me_param_t *prm, const xo_pu_t *pus, const xo_umh_ctx_t *ctxs, int n,
+                            const uint8_t *cur, intptr_t i_cur,
+                            const uint8_t *const planes[16], intptr_t i_ref,
+                            xo_me_result_t *res, int nthreads)
+{
+    batch_ctx_t ctx = { prm, pus, ctxs, n, cur, i_cur, planes, i_ref, res, 0, PTHREAD_MUTEX_INITIALIZER };
     if (nthreads < 1)
         nthreads = 1;
     if (nthreads == 1) {

@@ -70,13 +70,30 @@ typedef struct {
     int use_fractional_me;            /* 0 none, 1 half-pel, 2 half + quarter */
     int use_fast_sub_me;              /* Spiral2 with step 2 */
     int enable_pmvr;                  /* h->param->enable_pmvr */
-    int me_method;                    /* h->param->me_method: XO_ME_FS, XO_ME_DIA or XO_ME_HEX */
+    int me_method;                    /* h->param->me_method: XO_ME_FS, _DIA, _HEX or _UMH */
+    int umh_big_hex_level;            /* h->UMH_big_hex_level (presets.c:341): 0, 1 or 2 */
 } xo_me_param_t;
 
-/* enum me_mothod_e, common/common.h:145-151 (UMH and TZ are not restated). */
+/* enum me_mothod_e, common/common.h:145-151 (TZ is not restated). */
 #define XO_ME_FS  0
 #define XO_ME_DIA 1
 #define XO_ME_HEX 2
+#define XO_ME_UMH 3
+
+/* The per-(PU, reference) encoder state the UMH search reads from xavs2_me_t
+ * (set by md_inter.c:640-718): predictor MVs in qpel, the reference index,
+ * the predicted SADs and the beta thresholds (the search compares against
+ * beta + 1, me.c:883-884). */
+typedef struct {
+    int mvp1_x, mvp1_y;               /* p_me->mvp1: upper-layer MV ((0,0): none) */
+    int mvp2_x, mvp2_y;               /* p_me->mvp2: scaled MV of the previous reference */
+    int mvp3_x, mvp3_y;               /* p_me->mvp3: scaled co-located MV ((0,0): none) */
+    int ref_idx;                      /* p_me->i_ref_idx */
+    int pred_sad;                     /* p_me->pred_sad */
+    int pred_sad_uplayer;             /* p_me->pred_sad_uplayer */
+    int pad;
+    double beta2, beta3;              /* p_me->beta2, p_me->beta3 */
+} xo_umh_ctx_t;
 
 typedef struct {
     int pix_x, pix_y;                 /* PU position in the frame */
@@ -107,21 +124,39 @@ int32_t xo_me_search(const xo_me_param_t *prm, const xo_pu_t *pu,
                      const uint8_t *const planes[16], intptr_t i_ref,
                      xo_me_result_t *res, int32_t *cost_map);
 
-/* The integer stage of a pattern search (prm->me_method XO_ME_DIA or
- * XO_ME_HEX) from the full-pel start point (sx, sy) with cost scost, which
- * xo_me_search takes from the MV candidate list.  Writes the accepted points
- * in order to path[0..2*n) as (x, y) pairs (at most max_pts, the start point
- * first) and returns their count n; the last one is the integer best. */
-int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu,
+/* The same with the UMH context of the PU (ctx may be NULL for the other
+ * methods; NULL under XO_ME_UMH reads as an all-zero context). */
+int32_t xo_me_search_ctx(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                         const uint8_t *cur, intptr_t i_cur,
+                         const uint8_t *const planes[16], intptr_t i_ref,
+                         xo_me_result_t *res);
+
+/* The integer stage of a pattern search (prm->me_method XO_ME_DIA, XO_ME_HEX
+ * or XO_ME_UMH) from the full-pel start point (sx, sy) with cost scost, which
+ * xo_me_search takes from the MV candidate list; (p0x, p0y) is the first
+ * candidate (the clipped MVP), where UMH starts its first diamond.  Writes
+ * the best point after each step in order to path[0..2*n) as (x, y) pairs
+ * when it moved (at most max_pts, the start point first) and returns their
+ * count n; the last one is the integer best.  *route receives, for UMH, the
+ * stage the prefix ended in: 0 the big hexagon ran out, 1..4 an early
+ * termination after the predictor points, the first cross centre, the cross,
+ * the grid, or 5 one after a big-hexagon level; and 2 or 3 in bits 4..7 for a
+ * jump to the hexagon or the diamond. */
+int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
                        const uint8_t *cur, intptr_t i_cur,
                        const uint8_t *plane0, intptr_t i_ref,
-                       int sx, int sy, int scost, int *path, int max_pts);
+                       int sx, int sy, int scost, int p0x, int p0y, int *path, int max_pts, int *route);
 
-/* The same over n PUs on nthreads POSIX threads (CPU baseline). */
+/* The same over n PUs on nthreads POSIX threads (CPU baseline); ctxs holds
+ * one UMH context per PU or is NULL. */
 void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,
                         const uint8_t *cur, intptr_t i_cur,
                         const uint8_t *const planes[16], intptr_t i_ref,
                         xo_me_result_t *res, int nthreads);
+void xo_me_search_batch_ctx(const xo_me_param_t *prm, const xo_pu_t *pus, const xo_umh_ctx_t *ctxs, int n,
+                            const uint8_t *cur, intptr_t i_cur,
+                            const uint8_t *const planes[16], intptr_t i_ref,
+                            xo_me_result_t *res, int nthreads);
 
 /* ---- frame borders and interpolated planes (common/frame.c, common/mc.c) */
 /* plane_expand_border over a whole plane (frame.c:470-507, :513-535). */

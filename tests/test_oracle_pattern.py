@@ -226,3 +226,221 @@ def test_walks_stop_where_they_leave_the_mv_window(method):
             check_path(method, path, cost, lambda mx, my: -4 <= mx <= 4 and -4 <= my <= 4, 16)
             left += not (-4 <= path[-1][0] <= 4 and -4 <= path[-1][1] <= 4)
     assert left > 0
+
+
+# ---------------------------------------------------------------------------
+# UMH (me.c:1030-1099, then the hexagon or the diamond)
+
+def umh_ctx(**kw):
+    c = np.zeros(1, dtype=O.UMH_CTX_DTYPE)
+    c["beta2"] = -1.0          # beta + 1 = 0: no early termination
+    c["beta3"] = -1.0
+    for k, v in kw.items():
+        c[k] = v
+    return c
+
+
+def umh_search(cur, ref, pu, mvp, lam, R, ctx, level=0):
+    w, h = cur.shape[1], cur.shape[0]
+    prm = O.me_param(w, h, R, lam, use_fractional_me=0, me_method=O.ME_UMH, umh_big_hex_level=level)
+    res = O.me_search_batch(prm, np.array([pu + mvp], dtype=O.PU_DTYPE), cur_plane(cur),
+                            [plane_of(ref)] + [None] * 15, ctxs=ctx)
+    return (int(res["bmv2_x"][0]), int(res["bmv2_y"][0])), int(res["bcost2"][0])
+
+
+def umh_route(cur, ref, pu, mvp, lam, R, ctx, level=0):
+    w, h = cur.shape[1], cur.shape[0]
+    prm = O.me_param(w, h, R, lam, use_fractional_me=0, me_method=O.ME_UMH, umh_big_hex_level=level)
+    cost = cost_fn(cur, ref, pu, mvp, lam)
+    (xmin, xmax), (ymin, ymax) = npref.mv_window(w, h, *pu)
+    # the MV candidates (clipped MVP, clipped (0, 0)); the walk starts at the
+    # cheaper (the first on a tie), UMH's first diamond goes around the MVP
+    p0 = (min(max((mvp[0] + 2) >> 2, xmin), xmax), min(max((mvp[1] + 2) >> 2, ymin), ymax))
+    z = (min(max(0, xmin), xmax), min(max(0, ymin), ymax))
+    start = z if cost(*z) < cost(*p0) else p0
+    path, route = O.me_pattern_path(prm, pu + mvp, cur_plane(cur), plane_of(ref), start, cost(*start), ctx=ctx,
+                                    p0=p0, with_route=True)
+    return path, route, cost
+
+
+def test_umh_grid_repeats_0_2_and_never_tries_0_minus_2():
+    """Rows repeat with period 4 and the current picture is the reference
+    moved by 2 rows, so (0, 2) and (0, -2) match exactly (SAD 0) and cost the
+    same MV bits.  No earlier point matches: the diamonds and the vertical
+    cross step by odd rows.  The grid lists (0, 2) at entry 10, where (0, -2)
+    would complete it, and (0, 2) again at entry 21, so the search ends at
+    (0, 2) [bits 1 + 9]; a grid with (0, -2) at entry 10 would take (0, -2)
+    first and keep it."""
+    w, h = 96, 96
+    rows = np.array([20, 200, 90, 150], np.uint8)
+    ref = np.repeat(rows[np.arange(h) % 4][:, None], w, axis=1)
+    cur = np.roll(ref, -2, axis=0)
+    mv, cost = umh_search(cur, ref, (32, 32, 16, 16), (0, 0), 65536, 8, umh_ctx())
+    assert (mv, cost) == ((0, 2), 10)
+
+
+# lambda 250 on 8x8 PUs of a noise texture: a point one step off (0, 0) costs
+# 6 more bits (1500), more than the SAD (about 5400) varies between
+# neighbouring points, so the diamonds before the cross keep (0, 0) as the
+# centre of the cross and the big hexagons (each test asserts it), while an
+# exact match (SAD 0, at most 14 bits) still beats every other point
+LAM_HEAVY = 250 * 65536
+
+
+def centre_stays(cur, ref, pu):
+    cost = cost_fn(cur, ref, pu, (0, 0), LAM_HEAVY)
+    return all(cost(dx, dy) >= cost(0, 0) for dx, dy in DIA)
+
+
+@pytest.mark.parametrize("shift", [(7, 0), (-5, 0), (0, 3), (0, -3), (-7, 0)])
+def test_umh_cross_finds_odd_horizontal_and_short_vertical_shifts(shift):
+    """A noise texture moved by a whole-pixel shift that lies on the
+    unsymmetrical cross around (0, 0) (odd steps up to R across, up to R/2
+    down and up): only that point matches, and the search ends there."""
+    w, h = 128, 96
+    rng = np.random.default_rng(5)
+    ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    cur = np.roll(ref, (-shift[1], -shift[0]), axis=(0, 1))
+    assert centre_stays(cur, ref, (48, 40, 8, 8))
+    mv, _ = umh_search(cur, ref, (48, 40, 8, 8), (0, 0), LAM_HEAVY, 8, umh_ctx())
+    assert mv == shift
+
+
+def test_umh_vertical_cross_stops_at_half_the_range():
+    """(0, 5) lies past R/2 = 4 on the vertical arm at R = 8 (and off the grid
+    and the small patterns): nothing leads there.  At R = 12 the arm reaches
+    it."""
+    w, h = 128, 96
+    rng = np.random.default_rng(6)
+    ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    cur = np.roll(ref, -5, axis=0)
+    assert centre_stays(cur, ref, (48, 40, 8, 8))
+    mv, _ = umh_search(cur, ref, (48, 40, 8, 8), (0, 0), LAM_HEAVY, 8, umh_ctx())
+    assert mv != (0, 5)
+    mv, _ = umh_search(cur, ref, (48, 40, 8, 8), (0, 0), LAM_HEAVY, 12, umh_ctx())
+    assert mv == (0, 5)
+
+
+@pytest.mark.parametrize("level,shift,found", [
+    (2, (8, 4), True),      # HEX4 entry (4, 2) at level 2
+    (1, (8, 4), False),     # FAST_HEX4 has no (4, 2)
+    (1, (-4, 6), True),     # FAST_HEX4 entry (-2, 3) at level 2
+    (0, (-4, 6), False),    # no big hexagons
+    (2, (0, -12), True),    # HEX4 entry (0, -4) at level 3 (R = 16: levels 1..4)
+])
+def test_umh_big_hexagons(level, shift, found):
+    """Matches off the cross and the grid, at a big-hexagon point around
+    (0, 0) (the heavy lambda keeps every earlier centre at (0, 0))."""
+    w, h = 160, 128
+    rng = np.random.default_rng(7)
+    ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    cur = np.roll(ref, (-shift[1], -shift[0]), axis=(0, 1))
+    assert centre_stays(cur, ref, (64, 56, 8, 8))
+    mv, _ = umh_search(cur, ref, (64, 56, 8, 8), (0, 0), LAM_HEAVY, 16, umh_ctx(), level)
+    assert (mv == shift) == found
+
+
+@pytest.mark.parametrize("kw,route", [
+    (dict(), 0 | 2 << 4),
+    (dict(pred_sad_uplayer=10 ** 6, beta3=1.0), 1 | 3 << 4),
+    (dict(pred_sad_uplayer=10 ** 6, beta2=1.0), 1 | 2 << 4),
+    (dict(pred_sad=10 ** 6, beta3=1.0), 2 | 3 << 4),
+    (dict(pred_sad=10 ** 6, beta2=1.0), 2 | 2 << 4),
+])
+def test_umh_early_termination_routes(kw, route):
+    """beta + 1 scales pred_sad_uplayer at the first check and pred_sad at the
+    later ones; beta3 jumps to the diamond, beta2 to the hexagon."""
+    w, h = 128, 96
+    ref = smooth_frame(w, h, 3)
+    cur = np.roll(ref, (2, -3), axis=(0, 1))
+    _, r, _ = umh_route(cur, ref, (48, 40, 16, 16), (0, 0), 4 * 65536, 8, umh_ctx(**kw))
+    assert r == route
+
+
+def test_umh_early_termination_is_a_strict_double_compare():
+    """On a flat frame with MVP (0, 0) the best after the predictor points is
+    (0, 0) at 2 bits (lambda factor 65536).  pred_sad_uplayer 2 with beta3 0
+    gives the bound 2 * (0 + 1) = 2: 2 < 2 fails; beta3 1e-9 gives
+    2.000000002, which it passes."""
+    flat = np.full((64, 64), 90, np.uint8)
+    _, r, _ = umh_route(flat, flat, (16, 16, 16, 16), (0, 0), 65536, 8, umh_ctx(pred_sad_uplayer=2, beta3=0.0))
+    assert r & 15 != 1
+    _, r, _ = umh_route(flat, flat, (16, 16, 16, 16), (0, 0), 65536, 8, umh_ctx(pred_sad_uplayer=2, beta3=1e-9))
+    assert r == 1 | 3 << 4
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_umh_paths_fall_and_end_in_diamond_minima(level, seed):
+    w, h = 160, 112
+    rng = np.random.default_rng(seed)
+    ref = low_freq(w, h, seed)
+    cur = np.roll(ref, (int(rng.integers(-9, 10)), int(rng.integers(-9, 10))), axis=(0, 1))
+    cur = np.clip(cur.astype(int) + rng.integers(-3, 4, cur.shape), 0, 255).astype(np.uint8)
+    lam = 4 * 65536
+    for _ in range(8):
+        pw, ph = [(8, 8), (16, 16), (32, 16), (16, 32)][int(rng.integers(0, 4))]
+        pu = (int(rng.integers(8, (w - pw) // 4 - 8)) * 4, int(rng.integers(24, h - ph - 24)), pw, ph)
+        mvp = (int(rng.integers(-20, 21)), int(rng.integers(-20, 21)))
+        ctx = umh_ctx(mvp1_x=int(rng.integers(-30, 31)), mvp1_y=int(rng.integers(-30, 31)),
+                      mvp3_x=int(rng.integers(-30, 31)), ref_idx=1, mvp2_x=8, mvp2_y=-12)
+        path, _, cost = umh_route(cur, ref, pu, mvp, lam, 16, ctx, level)
+        (xmin, xmax), (ymin, ymax) = npref.mv_window(w, h, *pu)
+        legal = lambda mx, my: xmin <= mx <= xmax and ymin <= my <= ymax
+        assert all(legal(*p) for p in path[:-1])
+        costs = [cost(*p) for p in path]
+        assert all(b < a for a, b in zip(costs, costs[1:])), (path, costs)
+        c = path[-1]
+        if legal(*c):
+            assert all(cost(c[0] + dx, c[1] + dy) >= costs[-1] for dx, dy in DIA), (path, c)
+        res = umh_search(cur, ref, pu, mvp, lam, 16, ctx, level)
+        assert res == (c, costs[-1])
+
+
+@pytest.mark.parametrize("shift", [(1, 0), (-2, 1), (2, 2), (-3, -1)])
+def test_umh_small_shift_found_like_full_search(shift):
+    w, h = 128, 96
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    ref = np.clip(np.rint(128 + 50 * np.sin(xx / 11.0 + 0.3) * np.cos(yy / 9.0 + 1.1)), 0, 255).astype(np.uint8)
+    cur = np.roll(ref, (-shift[1], -shift[0]), axis=(0, 1))
+    lam = 4 * 65536
+    for pu in [(32, 32, 16, 16), (64, 40, 16, 16)]:
+        fs_cost, fs_mv = npref.fs_brute(cur, npref.clamp_picture(ref, O.PAD), O.PAD, pu, 8, lam)
+        assert fs_mv == shift
+        for level in (0, 2):
+            assert umh_search(cur, ref, pu, (0, 0), lam, 8, umh_ctx(), level) == (shift, fs_cost)
+
+
+@pytest.mark.parametrize("ref_idx,found", [(0, False), (1, True)])
+def test_umh_tries_mvp2_only_for_later_references(ref_idx, found):
+    """A noise texture matching at (5, 7) only, off every pattern around
+    (0, 0); mvp2 = (20, 28) qpel rounds to it and is tried when ref_idx > 0
+    (me.c:1048-1050)."""
+    w, h = 128, 96
+    rng = np.random.default_rng(8)
+    ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    cur = np.roll(ref, (-7, -5), axis=(0, 1))
+    assert centre_stays(cur, ref, (48, 40, 8, 8))
+    mv, _ = umh_search(cur, ref, (48, 40, 8, 8), (0, 0), LAM_HEAVY, 8,
+                       umh_ctx(ref_idx=ref_idx, mvp2_x=20, mvp2_y=28))
+    assert (mv == (5, 7)) == found
+
+
+@pytest.mark.parametrize("mvp,first,second_diamond", [((0, 0), (1, 0), False), ((0, 4), (1, 1), True)])
+def test_umh_second_diamond_needs_a_nonzero_mvp(mvp, first, second_diamond):
+    """A horizontal ramp moved by 3 pixels.  The first diamond goes around
+    the MVP candidate (IPEL of the MVP: (0, 0), or (0, 1) for MVP (0, 4) qpel,
+    which is cheaper than (0, 0) there) and moves the best one step right.
+    Only with an MVP candidate other than (0, 0) does a second diamond run
+    before the first early termination (me.c:1033-1036), one more step right.
+    The bound pred_sad_uplayer = cost(first), beta3 = 0, tells the two apart:
+    only a cost below cost(first) jumps."""
+    w, h = 128, 96
+    ref = np.repeat((2 * np.arange(w)).astype(np.uint8)[None, :], h, axis=0)
+    cur = np.roll(ref, -3, axis=1)
+    pu, lam = (48, 40, 16, 16), 4 * 65536
+    cost = cost_fn(cur, ref, pu, mvp, lam)
+    nxt = (first[0] + 1, first[1])
+    assert cost(*nxt) < cost(*first) < min(cost(0, 0), cost((mvp[0] + 2) >> 2, (mvp[1] + 2) >> 2))
+    _, route, _ = umh_route(cur, ref, pu, mvp, lam, 8, umh_ctx(pred_sad_uplayer=cost(*first), beta3=0.0))
+    assert (route == 1 | 3 << 4) == second_diamond
