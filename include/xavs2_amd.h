@@ -113,17 +113,35 @@ typedef struct {
     int32_t use_fractional_me;       /* 0 integer only, 1 + half-pel, 2 + quarter-pel */
     int32_t use_fast_sub_me;         /* Spiral2 with step 2 (me.c:501-502) */
     int32_t enable_pmvr;             /* PMVR quarter-pel adaptation (me.c:549-556) */
-    int32_t me_method;               /* h->param->me_method: XME_ME_FS, XME_ME_DIA or XME_ME_HEX */
+    int32_t me_method;               /* h->param->me_method: XME_ME_FS, _DIA, _HEX or _UMH */
+    int32_t umh_big_hex_level;       /* h->UMH_big_hex_level (presets.c:341): 0, 1 or 2 */
 } xme_me_params;
 
 /* Integer search methods, enum me_mothod_e (source/common/common.h:145-151).
  * XME_ME_FS is the full search over [-R, R)^2 (me.c:1156-1164); XME_ME_DIA
  * the small-diamond descent (me.c:1133-1154); XME_ME_HEX the hexagon descent
- * followed by the diamond (me.c:1107-1131).  UMH (3) and TZ (4) are not
+ * followed by the diamond (me.c:1107-1131); XME_ME_UMH the UMH steps
+ * (me.c:1030-1099) followed by the hexagon or the diamond.  TZ (4) is not
  * provided. */
 #define XME_ME_FS  0
 #define XME_ME_DIA 1
 #define XME_ME_HEX 2
+#define XME_ME_UMH 3
+
+/* Per (PU, reference) encoder state that the UMH search reads from
+ * xavs2_me_t (set by pred_inter_search_single's helpers,
+ * source/encoder/md_inter.c:640-718).  MVs in qpel; the search compares the
+ * best cost against pred_sad * (beta + 1) (me.c:165-170, :883-884). */
+typedef struct {
+    int32_t mvp1_x, mvp1_y;          /* p_me->mvp1 ((0,0): none) */
+    int32_t mvp2_x, mvp2_y;          /* p_me->mvp2 (tried when ref_idx > 0) */
+    int32_t mvp3_x, mvp3_y;          /* p_me->mvp3 ((0,0): none) */
+    int32_t ref_idx;                 /* p_me->i_ref_idx */
+    int32_t pred_sad;                /* p_me->pred_sad */
+    int32_t pred_sad_uplayer;        /* p_me->pred_sad_uplayer */
+    int32_t reserved;
+    double beta2, beta3;             /* p_me->beta2, p_me->beta3 */
+} xme_umh_ctx;
 
 typedef struct {
     int16_t x, y;                    /* PU position (luma pixels) */
@@ -172,6 +190,13 @@ int xme_plan_info(const xme_plan *plan, int64_t *n_inst, int64_t *n_candidates, 
  * byte absdiffs (a blocking read after the plan's work on the null stream; 0
  * for full-search plans). */
 int xme_plan_search_count(const xme_plan *plan, int64_t *n_points, int64_t *n_absdiff);
+/* UMH plans: the encoder state of every (PU, reference), ctx[pu * n_ref + r],
+ * for the following runs (copied to the device on `stream`; the caller's
+ * array may be reused once the stream reaches that point).  A UMH plan fails
+ * to run with XME_ERR_STATE until it has been set.  No reference
+ * counterpart: the reference reads these fields from xavs2_me_t per call
+ * (me.c:1030-1099). */
+int xme_plan_set_umh_context(xme_plan *plan, const xme_umh_ctx *ctx, void *stream);
 /* Caps the workgroups of the batched plan's persistent search launch (0 =
  * as many as the device holds at once), so that searches of concurrent
  * frames share the compute units instead of queueing behind each other. */

@@ -1,4 +1,4 @@
-"""GPU parity of the DIA and HEX integer searches (followed by the sub-pel
+"""GPU parity of the DIA, HEX and UMH integer searches (followed by the sub-pel
 refinement) against the oracle: every PU shape, MV predictors, several
 references, MV windows narrowed so that walks reach their edges, graph
 replays, and a 1080p frame of mixed shapes with a random MVP per PU."""
@@ -80,7 +80,7 @@ def test_cost_map_and_unknown_method_are_rejected():
     with pytest.raises(RuntimeError):
         plan.run(cur, [ref], cost_map=cm)
     with pytest.raises(RuntimeError):
-        X.Plan(X.me_params(64, 64, 8, LAM, me_method=3), X.make_pus([(16, 16, 16, 16, 0, 0)]), 1)
+        X.Plan(X.me_params(64, 64, 8, LAM, me_method=4), X.make_pus([(16, 16, 16, 16, 0, 0)]), 1)
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -102,3 +102,153 @@ def test_random_mvps_mixed_shapes_1080p_range64(method):
     p.expand_border()
     exp = oracle_results(O.me_param(W, H, 64, LAM, me_method=method), pus, cur, [O.interp_planes(p, 2)], 1)
     compare(got, exp, pus, 1)
+
+
+# ---------------------------------------------------------------------------
+# UMH: the same kernels with the per-(PU, reference) encoder context
+
+def random_umh_ctx(n_pu, n_ref, seed, zero_frac=0.3):
+    """Seeded contexts: predictor MVs (some (0,0)), ref_idx = the reference
+    index, predicted SADs and betas spread so that every early-termination
+    route is taken somewhere."""
+    rng = np.random.default_rng(seed)
+    n = n_pu * n_ref
+    c = np.zeros(n, dtype=X.UMH_CTX_DTYPE)
+    for f in ("mvp1", "mvp2", "mvp3"):
+        for a in ("_x", "_y"):
+            c[f + a] = rng.integers(-96, 97, n)
+    for f in ("mvp1", "mvp3"):
+        z = rng.random(n) < zero_frac
+        c[f + "_x"][z] = 0
+        c[f + "_y"][z] = 0
+    c["ref_idx"] = np.arange(n) % n_ref
+    c["pred_sad"] = rng.integers(0, 6000, n)
+    c["pred_sad_uplayer"] = rng.integers(0, 6000, n)
+    c["beta2"] = rng.uniform(-1.0, 0.6, n)
+    c["beta3"] = rng.uniform(-1.0, 0.2, n)
+    return c
+
+
+def run_umh_case(w, h, pus, R, n_ref, seed, level, prm_kw=None, ctx=None):
+    prm_kw = dict(prm_kw or {})
+    refs = [synth.smooth_field(w, h, seed=seed * 10 + r) for r in range(n_ref)]
+    cur = synth.multi_ref_current(refs, 16, min(R, 24), seed=seed + 99)
+    d_refs = []
+    for r in refs:
+        d = X.DeviceRef(w, h, frac=2)
+        d.load(r)
+        d_refs.append(d)
+    dcur = device_cur(cur)
+    ctx = random_umh_ctx(len(pus), n_ref, seed + 5) if ctx is None else ctx
+    plan = X.Plan(X.me_params(w, h, R, LAM, me_method=X.ME_UMH, umh_big_hex_level=level, **prm_kw),
+                  X.make_pus(pus), n_ref)
+    plan.set_umh_context(ctx)
+    plan.run(dcur, d_refs)
+    got = plan.fetch()
+    prm_o = O.me_param(w, h, R, LAM, me_method=O.ME_UMH, umh_big_hex_level=level, **prm_kw)
+    arr = np.array([tuple(p) for p in pus], dtype=O.PU_DTYPE)
+    cp = O.Plane(w, h)
+    cp.set_picture(cur)
+    exp = np.zeros(len(pus) * n_ref, dtype=X.RESULT_DTYPE)
+    for r in range(n_ref):
+        p = O.Plane(w, h)
+        p.set_picture(refs[r])
+        p.expand_border()
+        res = O.me_search_batch(prm_o, arr, cp, O.interp_planes(p, 2), nthreads=8,
+                                ctxs=np.ascontiguousarray(ctx[r::n_ref]).view(O.UMH_CTX_DTYPE))
+        for f in X.RESULT_DTYPE.names:
+            exp[f][r::n_ref] = res[f]
+    compare(got, exp, pus, n_ref)
+    return plan, dcur, d_refs, exp
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_umh_all_25_shapes_two_refs(level):
+    w, h = 256, 160
+    rng = np.random.default_rng(61 + level)
+    pus = []
+    for part in range(25):
+        pw, ph = X.PU_W[part], X.PU_H[part]
+        for _ in range(4):
+            x = int(rng.integers(0, (w - pw) // 4 + 1)) * 4
+            y = int(rng.integers(0, h - ph + 1))
+            pus.append((x, y, pw, ph, int(rng.integers(-60, 61)), int(rng.integers(-60, 61))))
+    run_umh_case(w, h, pus, 16, 2, 67 + level, level)
+
+
+def test_umh_no_early_termination_and_switches():
+    """betas of -1 (bound 0): every instance runs the cross, the grid, the
+    big hexagons, the hexagon and the diamond; sub-pel switches on top."""
+    w, h = 192, 128
+    pus = [(x, y, pw, ph, 6, -3) for (x, y, pw, ph) in synth.mixed_pus(w, h, 11)]
+    ctx = random_umh_ctx(len(pus), 1, 71)
+    ctx["beta2"] = -1.0
+    ctx["beta3"] = -1.0
+    for kw in (dict(), dict(use_fractional_me=0), dict(use_hadamard=0), dict(enable_pmvr=1)):
+        run_umh_case(w, h, pus, 32, 1, 73, 2, kw, ctx)
+
+
+def test_umh_narrow_mv_window_and_edges():
+    w, h = 192, 128
+    pus = [(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in synth.mixed_pus(w, h, 13)]
+    pus += [(0, 0, 16, 16, -40, -40), (176, 112, 16, 16, 40, 40), (0, 64, 64, 64, 0, 0)]
+    run_umh_case(w, h, pus, 16, 2, 79, 2, dict(mv_range_x=(-40, 40), mv_range_y=(-40, 40)))
+
+
+def test_umh_graph_replays_and_context_updates():
+    w, h = 256, 192
+    pus = [(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in synth.mixed_pus(w, h, 17)]
+    plan, dcur, drefs, exp = run_umh_case(w, h, pus, 32, 2, 83, 1)
+    s = X.Stream()
+    for rep in range(3):
+        plan.results.zero(s.h)
+        plan.run_graph(dcur, drefs, s.h)
+        s.sync()
+        assert np.array_equal(plan.fetch(), exp), f"replay {rep}"
+    # a new context changes the results of the replayed graph like a plain run
+    ctx2 = random_umh_ctx(len(pus), 2, 89)
+    plan.set_umh_context(ctx2, s.h)
+    plan.run_graph(dcur, drefs, s.h)
+    s.sync()
+    got_graph = plan.fetch()
+    plan.run(dcur, drefs)
+    assert np.array_equal(plan.fetch(), got_graph)
+
+
+def test_umh_needs_its_context():
+    prm = X.me_params(64, 64, 8, LAM, me_method=X.ME_UMH)
+    plan = X.Plan(prm, X.make_pus([(16, 16, 16, 16, 0, 0)]), 1)
+    ref = X.DeviceRef(64, 64, frac=2)
+    ref.load(synth.smooth_field(64, 64, seed=1))
+    cur = device_cur(synth.smooth_field(64, 64, seed=2))
+    with pytest.raises(RuntimeError):
+        plan.run(cur, [ref])
+    with pytest.raises(RuntimeError):
+        X.Plan(X.me_params(64, 64, 8, LAM, me_method=X.ME_UMH, umh_big_hex_level=3),
+               X.make_pus([(16, 16, 16, 16, 0, 0)]), 1)
+
+
+def test_umh_random_mvps_mixed_shapes_1080p_range64():
+    W, H = 1920, 1080
+    ref = synth.smooth_field(W, H, seed=600)
+    cur, _ = synth.displaced(ref, 16, 24, seed=601)
+    rng = np.random.default_rng(602)
+    pus = [(x, y, pw, ph, int(rng.integers(-64, 65)), int(rng.integers(-64, 65)))
+           for (x, y, pw, ph) in synth.mixed_pus(W, 1024, 602)]
+    ctx = random_umh_ctx(len(pus), 1, 603)
+    dref = X.DeviceRef(W, H, frac=2)
+    dref.load(ref)
+    dcur = device_cur(cur)
+    plan = X.Plan(X.me_params(W, H, 64, LAM, me_method=X.ME_UMH, umh_big_hex_level=2), X.make_pus(pus), 1)
+    plan.set_umh_context(ctx)
+    plan.run(dcur, [dref])
+    got = plan.fetch()
+    p = O.Plane(W, H)
+    p.set_picture(ref)
+    p.expand_border()
+    cp = O.Plane(W, H)
+    cp.set_picture(cur)
+    exp = O.me_search_batch(O.me_param(W, H, 64, LAM, me_method=O.ME_UMH, umh_big_hex_level=2),
+                            np.array([tuple(q) for q in pus], dtype=O.PU_DTYPE), cp, O.interp_planes(p, 2),
+                            nthreads=16, ctxs=ctx.view(O.UMH_CTX_DTYPE))
+    compare(got, exp.astype(X.RESULT_DTYPE), pus, 1)

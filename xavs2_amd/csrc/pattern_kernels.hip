@@ -47,6 +47,35 @@ constexpr uint32_t kDia4D = pack8(4, 1, 3, 2, 0, 0, 0, 0, 0);
 
 __device__ __forceinline__ int tab(uint32_t t, int i, int bias) { return (int)((t >> (3 * i)) & 7u) - bias; }
 
+// UMH tables (me.c:69-79, :116-126), 4 bits per entry (entry + 4): the big
+// hexagons HEX4 (16 points) and FAST_HEX4 (8), and the 5x5 grid in two
+// halves of 12 entries.  The grid lists (0, 2) at entries 10 and 21 and never
+// (0, -2), as the reference does.
+constexpr uint64_t pack16(const int (&v)[16])
+{
+    uint64_t r = 0;
+    for (int i = 0; i < 16; i++)
+        r |= (uint64_t)(v[i] + 4) << (4 * i);
+    return r;
+}
+constexpr int kHex4XV[16] = { 0, 0, -2, 2, -4, 4, -4, 4, -4, 4, -4, 4, -4, 4, -2, 2 };
+constexpr int kHex4YV[16] = { -4, 4, -3, -3, -2, -2, -1, -1, 0, 0, 1, 1, 2, 2, 3, 3 };
+constexpr int kFHex4XV[16] = { 0, 0, -2, 2, -4, 4, -2, 2, 0, 0, 0, 0, 0, 0, 0, 0 };
+constexpr int kFHex4YV[16] = { -4, 4, -3, -3, 0, 0, 3, 3, 0, 0, 0, 0, 0, 0, 0, 0 };
+constexpr int kGridXV[24] = { -1, 0, 1, -1, 1, -1, 0, 1, -2, -1, 0, 1, 2, -2, 2, -2, 2, -2, 2, -2, -1, 0, 1, 2 };
+constexpr int kGridYV[24] = { -1, -1, -1, 0, 0, 1, 1, 1, -2, -2, 2, -2, -2, -1, -1, 0, 0, 1, 1, 2, 2, 2, 2, 2 };
+constexpr int kGridXA[16] = { -1, 0, 1, -1, 1, -1, 0, 1, -2, -1, 0, 1, 0, 0, 0, 0 };
+constexpr int kGridXB[16] = { 2, -2, 2, -2, 2, -2, 2, -2, -1, 0, 1, 2, 0, 0, 0, 0 };
+constexpr int kGridYA[16] = { -1, -1, -1, 0, 0, 1, 1, 1, -2, -2, 2, -2, 0, 0, 0, 0 };
+constexpr int kGridYB[16] = { -2, -1, -1, 0, 0, 1, 1, 2, 2, 2, 2, 2, 0, 0, 0, 0 };
+constexpr uint64_t kHex4X = pack16(kHex4XV), kHex4Y = pack16(kHex4YV);
+constexpr uint64_t kFHex4X = pack16(kFHex4XV), kFHex4Y = pack16(kFHex4YV);
+constexpr uint64_t kGridXa = pack16(kGridXA), kGridXb = pack16(kGridXB);
+constexpr uint64_t kGridYa = pack16(kGridYA), kGridYb = pack16(kGridYB);
+static_assert(kGridXV[10] == 0 && kGridYV[10] == 2 && kGridXV[21] == 0 && kGridYV[21] == 2, "grid entries 10, 21");
+
+__device__ __forceinline__ int tab4(uint64_t t, int i) { return (int)((t >> (4 * i)) & 15u) - 4; }
+
 // Stages: the first hexagon, half hexagons, the first diamond, half diamonds.
 enum { kHexFirst = 0, kHexHalf = 1, kDiaFirst = 2, kDiaHalf = 3, kDone = 4 };
 
@@ -92,10 +121,327 @@ __device__ __forceinline__ void take_point(const DevParams &p, const DevPu &pu, 
     }
 }
 
-template <int S, int QMAX, int NB, bool SEQ>
-__device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ desc, const DevPu *__restrict__ pus,
+// UMH stages before the hexagon (me.c:1030-1099).  Evaluating stages hold
+// their points; the others are decisions.
+enum {
+    kUDia0 = 0,      // DIA_ITER around the MVP candidate mvc[0]
+    kUDia1,          // DIA_ITER around the best, if the MVP is not (0,0) and the best moved
+    kUMvp1,          // IPEL(mvp1), if mvp1 is not (0,0)
+    kUMvp23,         // IPEL(mvp2) if ref_idx > 0, IPEL(mvp3) if mvp3 is not (0,0)
+    kUDia2,          // DIA_ITER around the best, if it is not pmv
+    kUCross,         // the unsymmetrical cross
+    kUGrid,          // the 5x5 grid
+    kUBig,           // one big hexagon level
+    kUOff            // left for the hexagon or the diamond
+};
+
+struct UmhState {
+    int st, bi;                 // stage, batch of 6 points within it
+    int cx, cy;                 // stage centre
+    int pmx, pmy;               // pmv (me.c:910)
+    int level;                  // big hexagon level j
+    int p0x, p0y;               // mvc[0]
+};
+
+// Point t of the current UMH stage; false for a point the stage skips (a
+// missing predictor).  The range check is the caller's.
+__device__ __forceinline__ bool umh_point(const UmhState &u, const xme_umh_ctx &c, int R, int big, int t, int &mx,
+                                          int &my)
+{
+    const int ch = (R + 1) >> 1;           // odd i <= R
+    switch (u.st) {
+    case kUDia0:
+    case kUDia1:
+    case kUDia2:
+        mx = u.cx + tab(kDia4X, t, 2);
+        my = u.cy + tab(kDia4Y, t, 2);
+        return t < 4;
+    case kUMvp1:
+        mx = (c.mvp1_x + 2) >> 2;
+        my = (c.mvp1_y + 2) >> 2;
+        return t == 0;
+    case kUMvp23:
+        mx = t == 0 ? (c.mvp2_x + 2) >> 2 : (c.mvp3_x + 2) >> 2;
+        my = t == 0 ? (c.mvp2_y + 2) >> 2 : (c.mvp3_y + 2) >> 2;
+        return t == 0 ? c.ref_idx > 0 : (t == 1 && (c.mvp3_x | c.mvp3_y) != 0);
+    case kUCross:
+        if (t < 2 * ch) {
+            const int i = 2 * (t >> 1) + 1;
+            mx = u.cx + ((t & 1) ? -i : i);
+            my = u.cy;
+        } else {
+            const int j = 2 * ((t - 2 * ch) >> 1) + 1;
+            mx = u.cx;
+            my = u.cy + (((t - 2 * ch) & 1) ? -j : j);
+        }
+        return t < 2 * ch + 2 * (((R >> 1) + 1) >> 1);
+    case kUGrid:
+        mx = u.cx + (t < 12 ? tab4(kGridXa, t) : tab4(kGridXb, t - 12));
+        my = u.cy + (t < 12 ? tab4(kGridYa, t) : tab4(kGridYb, t - 12));
+        return t < 24;
+    default:   // kUBig
+        mx = u.cx + u.level * (big == 2 ? tab4(kHex4X, t & 15) : tab4(kFHex4X, t & 15));
+        my = u.cy + u.level * (big == 2 ? tab4(kHex4Y, t & 15) : tab4(kFHex4Y, t & 15));
+        return t < (big == 2 ? 16 : 8);
+    }
+}
+
+__device__ __forceinline__ int umh_stage_points(int st, int R, int big)
+{
+    return st == kUMvp1 ? 1 : st == kUMvp23 ? 2 : st == kUCross ? 2 * ((R + 1) >> 1) + 2 * (((R >> 1) + 1) >> 1)
+         : st == kUGrid ? 24 : st == kUBig ? (big == 2 ? 16 : 8) : 4;
+}
+
+// EARLY_TERMINATION (me.c:165-170): the search's phase to go on with, or -1.
+__device__ __forceinline__ int umh_early(int bcost, int pred_sad, double beta2, double beta3)
+{
+    if ((double)bcost < (double)pred_sad * beta3)
+        return kDiaFirst;
+    if ((double)bcost < (double)pred_sad * beta2)
+        return kHexFirst;
+    return -1;
+}
+
+// Moves a group to its next evaluating UMH stage through the decisions
+// between stages, or out of the prefix (st = kUOff, phase set).
+__device__ __forceinline__ void umh_next(UmhState &u, const xme_umh_ctx &c, int R, int big, int bcost, int bmx,
+                                         int bmy, int &phase)
+{
+    const double beta2 = c.beta2 + 1, beta3 = c.beta3 + 1;
+    for (;;) {
+        int et = -1;
+        switch (u.st) {
+        case kUDia1:       // entered after kUDia0
+            if ((u.pmx | u.pmy) != 0 && (bmx != u.p0x || bmy != u.p0y)) {
+                u.cx = bmx;
+                u.cy = bmy;
+                return;
+            }
+            u.st = kUMvp1;
+            continue;
+        case kUMvp1:
+            if ((c.mvp1_x | c.mvp1_y) != 0)
+                return;
+            u.st = kUMvp23 + 100;      // the first early termination comes first
+            continue;
+        case kUMvp23 + 100:
+            et = umh_early(bcost, c.pred_sad_uplayer, beta2, beta3);
+            if (et >= 0)
+                break;
+            u.st = kUMvp23;
+            if (c.ref_idx > 0 || (c.mvp3_x | c.mvp3_y) != 0)
+                return;
+            u.st = kUDia2;
+            continue;
+        case kUDia2:
+            if (u.pmx != bmx || u.pmy != bmy) {
+                u.cx = bmx;
+                u.cy = bmy;
+                return;
+            }
+            u.st = kUCross + 100;
+            continue;
+        case kUCross + 100:
+            et = umh_early(bcost, c.pred_sad, beta2, beta3);
+            if (et >= 0)
+                break;
+            u.st = kUCross;
+            u.cx = bmx;
+            u.cy = bmy;
+            return;
+        case kUGrid + 100:
+            et = umh_early(bcost, c.pred_sad, beta2, beta3);
+            if (et >= 0)
+                break;
+            u.st = kUGrid;
+            u.cx = bmx;
+            u.cy = bmy;
+            return;
+        case kUBig + 100:
+            et = umh_early(bcost, c.pred_sad, beta2, beta3);
+            if (et >= 0)
+                break;
+            if (big == 0 || R / 4 < 1) {
+                et = kHexFirst;
+                break;
+            }
+            u.st = kUBig;
+            u.level = 1;
+            u.cx = bmx;
+            u.cy = bmy;
+            return;
+        case kUBig + 200:  // after a level: early termination if it moved
+            if (bmx != u.cx || bmy != u.cy) {
+                et = umh_early(bcost, c.pred_sad, beta2, beta3);
+                if (et >= 0)
+                    break;
+            }
+            if (++u.level > R / 4) {
+                et = kHexFirst;
+                break;
+            }
+            u.st = kUBig;
+            u.cx = bmx;
+            u.cy = bmy;
+            return;
+        default:
+            return;
+        }
+        // out of the prefix
+        u.st = kUOff;
+        phase = et;
+        return;
+    }
+}
+
+// After the last batch of an evaluating stage: the stage that follows.
+__device__ __forceinline__ void umh_done_stage(UmhState &u, int bmx, int bmy)
+{
+    switch (u.st) {
+    case kUDia0:
+        u.st = kUDia1;
+        break;
+    case kUDia1:
+        u.pmx = bmx;
+        u.pmy = bmy;
+        u.st = kUMvp1;
+        break;
+    case kUMvp1:
+        u.st = kUMvp23 + 100;
+        break;
+    case kUMvp23:
+        u.st = kUDia2;
+        break;
+    case kUDia2:
+        u.st = kUCross + 100;
+        break;
+    case kUCross:
+        u.st = kUGrid + 100;
+        break;
+    case kUGrid:
+        u.st = kUBig + 100;
+        break;
+    default:   // kUBig
+        u.st = kUBig + 200;
+        break;
+    }
+    u.bi = 0;
+}
+
+// The UMH steps of every live group of the wave, up to the phase each one
+// goes on with (kHexFirst or kDiaFirst).  Points are read straight from the
+// reference plane (they lie anywhere in the window); every point is range
+// checked first (ME_COST_IPEL, me.c:187-193), so reads stay inside the
+// plane's padding.
+template <int S, int QMAX, bool SEQ>
+__device__ __forceinline__ void umh_prefix(const DevParams &p, const DevPu &pu, const xme_umh_ctx &c, const MvWindow &win,
+                           gbytes ref, int64_t i_ref, const uint32_t (&c4)[QMAX], int nq, int p0x, int p0y,
+                           int &phase, int &bcost, int &bmx, int &bmy, uint32_t &evals)
+{
+    const int lane = threadIdx.x & 63;
+    const int l = lane & (S - 1);
+    // plane offsets of this lane's quads
+    int32_t qg[QMAX];
+    {
+        const int gw = max(pu.w >> 2, 1);
+#pragma unroll
+        for (int q = 0; q < QMAX; q++) {
+            const int u = l + q * S;
+            const int yy = u / gw, xx = (u - yy * gw) * 4;
+            qg[q] = yy * (int32_t)i_ref + xx;
+        }
+    }
+    UmhState u;
+    u.st = phase == kDone ? kUOff : kUDia0;
+    u.bi = 0;
+    u.cx = p0x;
+    u.cy = p0y;
+    u.pmx = p0x;
+    u.pmy = p0y;
+    u.level = 0;
+    u.p0x = p0x;
+    u.p0y = p0y;
+    if (u.st != kUOff)
+        phase = kDone;   // set again when the prefix ends
+    for (;;) {
+        if (__builtin_amdgcn_ballot_w64(u.st != kUOff) == 0)
+            break;
+        const bool live = u.st != kUOff;
+        const int n_st = live ? umh_stage_points(u.st, p.R, p.big_hex) : 0;
+        int nbx = bmx, nby = bmy;
+        auto eval = [&](int k, int &mx, int &my, bool &ok) -> uint32_t {
+            const int t = u.bi * 6 + k;
+            ok = live && t < n_st && umh_point(u, c, p.R, p.big_hex, t, mx, my) && xme_mv_legal(win, mx, my);
+            uint32_t sad = 0;
+            if (ok) {
+                const gbytes base = ref + (int64_t)my * i_ref + mx;
+#pragma unroll
+                for (int q = 0; q < QMAX; q++) {
+                    if (q < nq) {
+                        const uintptr_t a = (uintptr_t)(base + qg[q]);
+                        const gdwords d = (gdwords)(a & ~(uintptr_t)3);
+                        sad = __builtin_amdgcn_sad_u8(c4[q], __builtin_amdgcn_alignbyte(d[1], d[0], (uint32_t)(a & 3)),
+                                                      sad);
+                    }
+                }
+            }
+            return sad;
+        };
+        auto take = [&](int sum, int mx, int my) {
+            const int cost = sum + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+            if (cost < bcost) {
+                bcost = cost;
+                nbx = mx;
+                nby = my;
+            }
+        };
+        if constexpr (SEQ) {
+#pragma unroll 1
+            for (int k = 0; k < 6; k++) {
+                int mx = 0, my = 0;
+                bool ok;
+                const uint32_t sad = eval(k, mx, my, ok);
+                int sum = group_sum((int)sad, S);
+                sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
+                if (ok) {
+                    take(sum, mx, my);
+                    evals++;
+                }
+            }
+        } else {
+            uint32_t sad[6];
+            bool ok[6];
+            int mxs[6], mys[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                sad[k] = eval(k, mxs[k], mys[k], ok[k]);
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                int sum = group_sum((int)sad[k], S);
+                sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
+                if (ok[k]) {
+                    take(sum, mxs[k], mys[k]);
+                    evals++;
+                }
+            }
+        }
+        bmx = nbx;
+        bmy = nby;
+        if (live) {
+            u.bi++;
+            if (u.bi * 6 >= n_st) {
+                umh_done_stage(u, bmx, bmy);
+                umh_next(u, c, p.R, p.big_hex, bcost, bmx, bmy, phase);
+            }
+        }
+    }
+}
+
+template <int S, int QMAX, int NB, bool SEQ, bool UMH>
+__device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ desc, const DevPu *__restrict__ pus,
                              DevInst *__restrict__ inst, uint64_t *__restrict__ best, int id,
-                             uint32_t *__restrict__ points, uint32_t *__restrict__ box)
+                             uint32_t *__restrict__ points, uint32_t *__restrict__ box,
+                             const xme_umh_ctx *__restrict__ umh)
 {
     const int lane = threadIdx.x & 63;
     const int l = lane & (S - 1);
@@ -113,7 +459,7 @@ __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ d
             // the candidate-list pass leaves the best candidate as the centre
             bmx = order == 0 ? in.mvc[0][0] : in.mvc[1][0];
             bmy = order == 0 ? in.mvc[0][1] : in.mvc[1][1];
-            phase = p.method == XME_ME_HEX ? kHexFirst : kDiaFirst;
+            phase = p.method == XME_ME_DIA ? kDiaFirst : kHexFirst;
         }
     }
     const int r = id >= 0 ? id - (id / p.n_ref) * p.n_ref : 0;
@@ -154,6 +500,14 @@ __device__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ d
 
     int idx = 0, hex_i = 0, dia_i = 0;
     uint32_t evals = 0;
+    if constexpr (UMH) {
+        // one point at a time: the prefix's many far points, few registers
+        xme_umh_ctx c = {};
+        if (phase != kDone)
+            c = umh[id];
+        umh_prefix<S, QMAX, true>(p, pu, c, win, plane + (int64_t)pu.y * i_ref + pu.x, i_ref, c4, nq, in.mvc[0][0],
+                                 in.mvc[0][1], phase, bcost, bmx, bmy, evals);
+    }
     for (;;) {
         // stage ends that need no evaluation (loop heads of me.c:1120, :1136, :1145)
         const bool legal = xme_mv_legal(win, bmx, bmy);
@@ -315,31 +669,33 @@ struct PatLaunch {
 //  2: <= 64, S 16, 9 (8x32: 144);  1: <= 32, S 8, 10 (8x16: 80);
 //  0: <= 16, S 4, 15 (4x16: 60).  Every lane holds at most 4 quads of the
 // PU below class 5.
+template <bool UMH>
 __global__ void __launch_bounds__(256) pattern_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                       const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
                                                       uint64_t *__restrict__ best, const int32_t *__restrict__ list,
-                                                      PatLaunch L, uint32_t *__restrict__ points)
+                                                      PatLaunch L, uint32_t *__restrict__ points,
+                                                      const xme_umh_ctx *__restrict__ umh)
 {
     __shared__ uint32_t boxes[4][XME_PAT_BOX_DW];
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     const int lane = threadIdx.x & 63;
     uint32_t *box = boxes[threadIdx.x >> 6];
     if (w < L.end[0]) {
-        pattern_wave<64, 16, 20, true>(p, desc, pus, inst, best, list[L.first[0] + w], points, box);
+        pattern_wave<64, 16, 20, true, UMH>(p, desc, pus, inst, best, list[L.first[0] + w], points, box, umh);
     } else if (w < L.end[1]) {
-        pattern_wave<64, 4, 7, false>(p, desc, pus, inst, best, list[L.first[1] + (w - L.end[0])], points, box);
+        pattern_wave<64, 4, 7, false, UMH>(p, desc, pus, inst, best, list[L.first[1] + (w - L.end[0])], points, box, umh);
     } else if (w < L.end[2]) {
-        pattern_wave<32, 4, 7, false>(p, desc, pus, inst, best, list[L.first[2] + (w - L.end[1]) * 2 + lane / 32],
-                                      points, box);
+        pattern_wave<32, 4, 7, false, UMH>(p, desc, pus, inst, best, list[L.first[2] + (w - L.end[1]) * 2 + lane / 32],
+                                      points, box, umh);
     } else if (w < L.end[3]) {
-        pattern_wave<16, 4, 9, false>(p, desc, pus, inst, best, list[L.first[3] + (w - L.end[2]) * 4 + lane / 16],
-                                      points, box);
+        pattern_wave<16, 4, 9, false, UMH>(p, desc, pus, inst, best, list[L.first[3] + (w - L.end[2]) * 4 + lane / 16],
+                                      points, box, umh);
     } else if (w < L.end[4]) {
-        pattern_wave<8, 4, 10, false>(p, desc, pus, inst, best, list[L.first[4] + (w - L.end[3]) * 8 + lane / 8],
-                                      points, box);
+        pattern_wave<8, 4, 10, false, UMH>(p, desc, pus, inst, best, list[L.first[4] + (w - L.end[3]) * 8 + lane / 8],
+                                      points, box, umh);
     } else if (w < L.end[5]) {
-        pattern_wave<4, 4, 15, false>(p, desc, pus, inst, best, list[L.first[5] + (w - L.end[4]) * 16 + lane / 4],
-                                      points, box);
+        pattern_wave<4, 4, 15, false, UMH>(p, desc, pus, inst, best, list[L.first[5] + (w - L.end[4]) * 16 + lane / 4],
+                                      points, box, umh);
     }
 }
 
@@ -353,7 +709,7 @@ int xme_pattern_class(int w, int h)
 
 int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
                        uint64_t *d_best, const int32_t *d_list, const int wave_end[6], uint32_t *d_points,
-                       hipStream_t s)
+                       const xme_umh_ctx *d_umh, hipStream_t s)
 {
     const int waves = wave_end[5];
     if (waves <= 0)
@@ -368,8 +724,12 @@ int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu 
         first += (wave_end[k] - prev) * per_wave[k];
         prev = wave_end[k];
     }
-    hipLaunchKernelGGL(pattern_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best,
-                       d_list, L, d_points);
+    if (p.method == XME_ME_UMH)
+        hipLaunchKernelGGL(pattern_kernel<true>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+                           d_best, d_list, L, d_points, d_umh);
+    else
+        hipLaunchKernelGGL(pattern_kernel<false>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+                           d_best, d_list, L, d_points, d_umh);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }

@@ -210,6 +210,8 @@ struct xme_plan {
     int32_t *d_pat_list = nullptr;
     int pat_wave_end[6] = { 0, 0, 0, 0, 0, 0 };
     uint32_t *d_pat_points = nullptr;     // points evaluated per instance, last run
+    xme_umh_ctx *d_umh = nullptr;         // UMH plans: per-instance encoder state
+    bool umh_set = false;
     std::vector<int32_t> inst_area;        // PU area per instance
 };
 
@@ -429,6 +431,7 @@ static void plan_free(xme_plan *pl)
     hipFree(pl->d_sp_list);
     hipFree(pl->d_pat_list);
     hipFree(pl->d_pat_points);
+    hipFree(pl->d_umh);
     hipFree(pl->d_counters);
     hipFree(pl->d_desc);
     for (auto &s : pl->slots) {
@@ -468,13 +471,15 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         xme_set_error("xme_plan_create: search range must be 1..128 and the frame 8..8192 pixels");
         return XME_ERR_ARG;
     }
-    if (prm->me_method != XME_ME_FS && prm->me_method != XME_ME_DIA && prm->me_method != XME_ME_HEX) {
-        xme_set_error("xme_plan_create: me_method %d: only full search (0), DIA (1) and HEX (2) are provided",
-                      prm->me_method);
+    if (prm->me_method < XME_ME_FS || prm->me_method > XME_ME_UMH || prm->umh_big_hex_level < 0 ||
+        prm->umh_big_hex_level > 2) {
+        xme_set_error("xme_plan_create: me_method %d / umh_big_hex_level %d: full search (0), DIA (1), HEX (2) "
+                      "and UMH (3, big-hexagon level 0..2) are provided", prm->me_method, prm->umh_big_hex_level);
         return XME_ERR_ARG;
     }
     DevParams p;
     p.method = prm->me_method;
+    p.big_hex = prm->umh_big_hex_level;
     p.frame_w = prm->frame_w;
     p.frame_h = prm->frame_h;
     p.R = prm->search_range;
@@ -750,6 +755,10 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         for (int i = 0; i < pl->n_inst; i++)
             pl->inst_area[i] = pus[i / n_ref].w * pus[i / n_ref].h;
     }
+    if (e == hipSuccess && p.method == XME_ME_UMH) {
+        e = hipMalloc((void **)&pl->d_umh, sizeof(xme_umh_ctx) * pl->n_inst);
+        pl->dev_bytes += sizeof(xme_umh_ctx) * pl->n_inst;
+    }
     {
         std::vector<int32_t> sp_list;
         sp_list.reserve(pus.size());
@@ -823,6 +832,18 @@ extern "C" int xme_plan_info(const xme_plan *pl, int64_t *n_inst, int64_t *n_can
         *n_cand = pl->n_cand;
     if (n_absdiff)
         *n_absdiff = pl->n_absdiff;
+    return XME_OK;
+}
+
+extern "C" int xme_plan_set_umh_context(xme_plan *pl, const xme_umh_ctx *ctx, void *stream)
+{
+    if (!pl || !ctx || pl->p.method != XME_ME_UMH) {
+        xme_set_error("xme_plan_set_umh_context: needs a UMH plan and a context array");
+        return XME_ERR_ARG;
+    }
+    XME_HIP(hipMemcpyAsync(pl->d_umh, ctx, sizeof(xme_umh_ctx) * pl->n_inst, hipMemcpyHostToDevice,
+                           (hipStream_t)stream));
+    pl->umh_set = true;
     return XME_OK;
 }
 
@@ -957,9 +978,14 @@ static int fill_desc(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_r
 static int launch_search(xme_plan *pl, hipStream_t s)
 {
     int rc = XME_OK;
-    if (pl->p.method != XME_ME_FS)
+    if (pl->p.method != XME_ME_FS) {
+        if (pl->p.method == XME_ME_UMH && !pl->umh_set) {
+            xme_set_error("xme_me_run: a UMH plan needs xme_plan_set_umh_context before it runs");
+            return XME_ERR_STATE;
+        }
         return xme_launch_pattern(pl->p, pl->d_desc, pl->d_pus, pl->d_inst, pl->d_best, pl->d_pat_list,
-                                  pl->pat_wave_end, pl->d_pat_points, s);
+                                  pl->pat_wave_end, pl->d_pat_points, pl->d_umh, s);
+    }
     if (pl->mode == XME_PLAN_BATCHED) {
         // launches of one variant run in stream order and share its queue
         // counters (the last workgroup of a launch resets them)

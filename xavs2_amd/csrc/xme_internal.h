@@ -26,7 +26,8 @@ struct DevParams {
     int32_t min_mv[2], max_mv[2];
     int32_t hadamard, frac, fast, pmvr;
     int32_t n_pu, n_ref;
-    int32_t method;            // XME_ME_FS, XME_ME_DIA or XME_ME_HEX
+    int32_t method;            // XME_ME_FS, _DIA, _HEX or _UMH
+    int32_t big_hex;           // UMH big-hexagon level (0, 1, 2)
 };
 
 // Per-frame pointers, read by the kernels from device memory so that a captured
@@ -185,7 +186,7 @@ int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *
 int xme_pattern_class(int w, int h);
 int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
                        uint64_t *d_best, const int32_t *d_list, const int wave_end[6], uint32_t *d_points,
-                       hipStream_t s);
+                       const xme_umh_ctx *d_umh, hipStream_t s);
 int xme_fs_max_lds(void);
 int xme_fs_prepare(void);
 // Device bytes a plan holds (work lists, per-instance state, descriptor).

@@ -31,11 +31,16 @@ class MeParams(C.Structure):
     _fields_ = [("frame_w", C.c_int32), ("frame_h", C.c_int32), ("search_range", C.c_int32),
                 ("lambda_factor", C.c_int32), ("min_mv_range", C.c_int32 * 2), ("max_mv_range", C.c_int32 * 2),
                 ("use_hadamard", C.c_int32), ("use_fractional_me", C.c_int32), ("use_fast_sub_me", C.c_int32),
-                ("enable_pmvr", C.c_int32), ("me_method", C.c_int32)]
+                ("enable_pmvr", C.c_int32), ("me_method", C.c_int32), ("umh_big_hex_level", C.c_int32)]
 
 
 # integer search methods (XME_ME_*, h->param->me_method)
-ME_FS, ME_DIA, ME_HEX = 0, 1, 2
+ME_FS, ME_DIA, ME_HEX, ME_UMH = 0, 1, 2, 3
+
+# xme_umh_ctx: per-(PU, reference) encoder state of the UMH search
+UMH_CTX_DTYPE = np.dtype([("mvp1_x", "<i4"), ("mvp1_y", "<i4"), ("mvp2_x", "<i4"), ("mvp2_y", "<i4"),
+                          ("mvp3_x", "<i4"), ("mvp3_y", "<i4"), ("ref_idx", "<i4"), ("pred_sad", "<i4"),
+                          ("pred_sad_uplayer", "<i4"), ("reserved", "<i4"), ("beta2", "<f8"), ("beta3", "<f8")])
 
 
 PU_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
@@ -91,6 +96,7 @@ def lib():
         L.xme_plan_set_launch_split.argtypes = [P, C.c_int64]
         L.xme_plan_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xme_plan_search_count.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.xme_plan_set_umh_context.argtypes = [P, P, P]
         L.xme_me_run.argtypes = [P, P, Z, P, P, P, P]
         L.xme_me_run_graph.argtypes = [P, P, Z, P, P, P]
         L.xme_me_time.argtypes = [P, P, Z, P, P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), P]
@@ -245,9 +251,10 @@ class DeviceRef:
 
 def me_params(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use_fractional_me=2,
               use_fast_sub_me=0, enable_pmvr=0, mv_range_x=(-8192, 8191), mv_range_y=(-2048, 2047),
-              me_method=ME_FS):
+              me_method=ME_FS, umh_big_hex_level=0):
     p = MeParams()
     p.me_method = me_method
+    p.umh_big_hex_level = umh_big_hex_level
     p.frame_w, p.frame_h, p.search_range, p.lambda_factor = frame_w, frame_h, search_range, lambda_factor
     p.min_mv_range[0], p.max_mv_range[0] = mv_range_x
     p.min_mv_range[1], p.max_mv_range[1] = mv_range_y
@@ -291,6 +298,13 @@ class Plan:
         a, b = C.c_int64(), C.c_int64()
         check(lib().xme_plan_search_count(self.h, C.byref(a), C.byref(b)), "xme_plan_search_count")
         return a.value, b.value
+
+    def set_umh_context(self, ctx, stream=None):
+        """ctx: UMH_CTX_DTYPE array of n_pu * n_ref entries (pu * n_ref + r)."""
+        self._umh = np.ascontiguousarray(ctx, dtype=UMH_CTX_DTYPE)
+        assert len(self._umh) == self.n
+        check(lib().xme_plan_set_umh_context(self.h, ptr(self._umh), stream), "xme_plan_set_umh_context")
+        check(lib().xme_stream_sync(stream), "sync")
 
     def _refs(self, refs):
         arr = (RefDesc * len(refs))()
