@@ -113,11 +113,37 @@ def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
     }
 
 
+def umh_contexts(X, n_pu, n_ref, seed):
+    """Seeded synthetic encoder state for the UMH search, one record per (PU,
+    reference): predictor MVs within +-24 pixels (a third of mvp1 and mvp3
+    absent), ref_idx = the reference index, predicted SADs up to 6000 and
+    betas in [-1, 0.6] / [-1, 0.2], so that a share of the searches stops
+    early at each termination point."""
+    rng = np.random.default_rng(seed)
+    n = n_pu * n_ref
+    c = np.zeros(n, dtype=X.UMH_CTX_DTYPE)
+    for f in ("mvp1", "mvp2", "mvp3"):
+        for a in ("_x", "_y"):
+            c[f + a] = rng.integers(-96, 97, n)
+    for f in ("mvp1", "mvp3"):
+        z = rng.random(n) < 0.3
+        c[f + "_x"][z] = 0
+        c[f + "_y"][z] = 0
+    c["ref_idx"] = np.arange(n) % n_ref
+    c["pred_sad"] = rng.integers(0, 6000, n)
+    c["pred_sad_uplayer"] = rng.integers(0, 6000, n)
+    c["beta2"] = rng.uniform(-1.0, 0.6, n)
+    c["beta3"] = rng.uniform(-1.0, 0.2, n)
+    return c
+
+
 def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, steps, cpu):
-    """The DIA and HEX integer searches (+ the same sub-pel refinement) on the
-    same frame and PUs: job time per frame (graph replays), search-kernel time
-    (HIP events), integer points evaluated, and parity with the oracle for
-    every PU against reference 0, which is also the CPU time reported."""
+    """The DIA, HEX and UMH integer searches (+ the same sub-pel refinement)
+    on the same frame and PUs: job time per frame (graph replays),
+    search-kernel time (HIP events), integer points evaluated, and parity with
+    the oracle for every PU against reference 0, which is also the CPU time
+    reported.  UMH runs at big-hexagon level 2 on seeded synthetic encoder
+    state (umh_contexts)."""
     from oracle import oracle as O
     pu_arr = X.make_pus([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus])
     planes0, cp = None, None
@@ -130,8 +156,13 @@ def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, st
         cp.set_picture(cur)
     threads = min(16, os.cpu_count() or 1)
     out = {}
-    for name, method in (("dia", X.ME_DIA), ("hex", X.ME_HEX)):
-        plan = X.Plan(X.me_params(w, h, R, lam, me_method=method), pu_arr, n_ref, X.PLAN_BATCHED)
+    ctx = umh_contexts(X, len(pus), n_ref, seed=2027)
+    for name, method in (("dia", X.ME_DIA), ("hex", X.ME_HEX), ("umh", X.ME_UMH)):
+        level = 2 if method == X.ME_UMH else 0
+        plan = X.Plan(X.me_params(w, h, R, lam, me_method=method, umh_big_hex_level=level), pu_arr, n_ref,
+                      X.PLAN_BATCHED)
+        if method == X.ME_UMH:
+            plan.set_umh_context(ctx)
         for _ in range(2):
             plan.run_graph(dcur, d_refs, stream.h)
         stream.sync()
@@ -160,9 +191,10 @@ def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, st
         }
         if cpu:
             arr = np.array([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus], dtype=O.PU_DTYPE)
-            prm_o = O.me_param(w, h, R, lam, me_method=method)
+            prm_o = O.me_param(w, h, R, lam, me_method=method, umh_big_hex_level=level)
+            c0 = np.ascontiguousarray(ctx[0::n_ref]).view(O.UMH_CTX_DTYPE) if method == X.ME_UMH else None
             t0 = time.perf_counter()
-            exp = O.me_search_batch(prm_o, arr, cp, planes0, nthreads=threads)
+            exp = O.me_search_batch(prm_o, arr, cp, planes0, nthreads=threads, ctxs=c0)
             el = time.perf_counter() - t0
             got = res[0::n_ref]
             leg["parity_with_oracle_ref0"] = bool(all(np.array_equal(got[f], exp[f]) for f in exp.dtype.names))
@@ -382,7 +414,7 @@ def main():
                     help="%% of a full-device search launch per frame in the serving leg (0 = 100); a cap "
                          "leaves CUs for frames that arrive while a large one searches")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
-    ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX pattern-search leg")
+    ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX / UMH pattern-search leg")
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
@@ -486,7 +518,7 @@ def main():
     del plain
     pattern = None
     if not args.no_pattern:
-        log("pattern-search leg (DIA, HEX)")
+        log("pattern-search leg (DIA, HEX, UMH)")
         pattern = pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, args.steps,
                               not args.no_cpu)
     transform = None
