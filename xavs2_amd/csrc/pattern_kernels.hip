@@ -329,28 +329,28 @@ __device__ __forceinline__ void umh_done_stage(UmhState &u, int bmx, int bmy)
 }
 
 // The UMH steps of every live group of the wave, up to the phase each one
-// goes on with (kHexFirst or kDiaFirst).  Points are read straight from the
-// reference plane (they lie anywhere in the window); every point is range
-// checked first (ME_COST_IPEL, me.c:187-193), so reads stay inside the
-// plane's padding.
-template <int S, int QMAX, bool SEQ>
-__device__ __forceinline__ void umh_prefix(const DevParams &p, const DevPu &pu, const xme_umh_ctx &c, const MvWindow &win,
-                           gbytes ref, int64_t i_ref, const uint32_t (&c4)[QMAX], int nq, int p0x, int p0y,
-                           int &phase, int &bcost, int &bmx, int &bmy, uint32_t &evals)
+// goes on with (kHexFirst or kDiaFirst).  Each lane of a group takes one point
+// of the stage per round and sums the whole PU there, with the current PU
+// staged in the group's LDS share (`cur`, gw dwords a row, read by every lane
+// of the group at once), so a round evaluates S points.  A round's points are
+// taken as the reference takes them one by one: the first strict minimum,
+// i.e. the smallest (cost, point) key, if its cost is below the running best
+// (COPY3_IF_LT in order, ME_COST_IPEL, me.c:187-193).  Every point is range
+// checked first, so the reads (straight from the plane: the points lie
+// anywhere in the window) stay inside the plane's padding.
+//
+// LP > 1 (64-lane groups): LP lanes share a point, lane j of them summing
+// quads j, j + LP, ... of every row, so that a row of a point is one
+// coalesced read; a round then evaluates S / LP points.
+template <int S, int GWMAX, int LP>
+__device__ __forceinline__ void umh_prefix_lanes(const DevParams &p, const DevPu &pu, const xme_umh_ctx &c,
+                                                 const MvWindow &win, gbytes ref, int64_t i_ref,
+                                                 const uint32_t *cur, int p0x, int p0y, int &phase, int &bcost,
+                                                 int &bmx, int &bmy, uint32_t &evals)
 {
     const int lane = threadIdx.x & 63;
     const int l = lane & (S - 1);
-    // plane offsets of this lane's quads
-    int32_t qg[QMAX];
-    {
-        const int gw = max(pu.w >> 2, 1);
-#pragma unroll
-        for (int q = 0; q < QMAX; q++) {
-            const int u = l + q * S;
-            const int yy = u / gw, xx = (u - yy * gw) * 4;
-            qg[q] = yy * (int32_t)i_ref + xx;
-        }
-    }
+    const int gw = pu.w >> 2;
     UmhState u;
     u.st = phase == kDone ? kUOff : kUDia0;
     u.bi = 0;
@@ -368,68 +368,72 @@ __device__ __forceinline__ void umh_prefix(const DevParams &p, const DevPu &pu, 
             break;
         const bool live = u.st != kUOff;
         const int n_st = live ? umh_stage_points(u.st, p.R, p.big_hex) : 0;
-        int nbx = bmx, nby = bmy;
-        auto eval = [&](int k, int &mx, int &my, bool &ok) -> uint32_t {
-            const int t = u.bi * 6 + k;
-            ok = live && t < n_st && umh_point(u, c, p.R, p.big_hex, t, mx, my) && xme_mv_legal(win, mx, my);
+        constexpr int PR = S / LP;             // points per round
+        const int pi = l / LP, j = l % LP;     // point of the round, lane within it
+        const int t = u.bi * PR + pi;
+        int mx = 0, my = 0;
+        const bool ok = live && t < n_st && umh_point(u, c, p.R, p.big_hex, t, mx, my) && xme_mv_legal(win, mx, my);
+        int key = 0x7fffffff;
+        if constexpr (LP > 1) {
             uint32_t sad = 0;
-            if (ok) {
-                const gbytes base = ref + (int64_t)my * i_ref + mx;
+            if (ok && j < gw) {
+                const uintptr_t a = (uintptr_t)(ref + (int64_t)my * i_ref + mx + 4 * j);
+                const gdwords q0 = (gdwords)(a & ~(uintptr_t)3);
+                const uint32_t sh = (uint32_t)(a & 3);
+                const int64_t rstep = i_ref >> 2;
+                constexpr int KK = GWMAX / LP > 1 ? GWMAX / LP : 1;   // quads of a row per lane
+#pragma unroll 4
+                for (int r = 0; r < pu.h; r++) {
+                    const gdwords rp = q0 + r * rstep;
 #pragma unroll
-                for (int q = 0; q < QMAX; q++) {
-                    if (q < nq) {
-                        const uintptr_t a = (uintptr_t)(base + qg[q]);
-                        const gdwords d = (gdwords)(a & ~(uintptr_t)3);
-                        sad = __builtin_amdgcn_sad_u8(c4[q], __builtin_amdgcn_alignbyte(d[1], d[0], (uint32_t)(a & 3)),
-                                                      sad);
-                    }
+                    for (int kk = 0; kk < KK; kk++)
+                        if (j + kk * LP < gw)
+                            sad = __builtin_amdgcn_sad_u8(cur[r * gw + j + kk * LP],
+                                                          __builtin_amdgcn_alignbyte(rp[kk * LP + 1], rp[kk * LP], sh), sad);
                 }
             }
-            return sad;
-        };
-        auto take = [&](int sum, int mx, int my) {
-            const int cost = sum + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
-            if (cost < bcost) {
-                bcost = cost;
-                nbx = mx;
-                nby = my;
+            const int sum = group_sum((int)sad, LP);    // in the point's last lane
+            if (ok && j == LP - 1) {
+                const int cost = sum + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+                key = (cost << 6) | pi;
             }
-        };
-        if constexpr (SEQ) {
-#pragma unroll 1
-            for (int k = 0; k < 6; k++) {
-                int mx = 0, my = 0;
-                bool ok;
-                const uint32_t sad = eval(k, mx, my, ok);
-                int sum = group_sum((int)sad, S);
-                sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
-                if (ok) {
-                    take(sum, mx, my);
-                    evals++;
-                }
-            }
-        } else {
-            uint32_t sad[6];
-            bool ok[6];
-            int mxs[6], mys[6];
+        } else if (ok) {
+            const uintptr_t a = (uintptr_t)(ref + (int64_t)my * i_ref + mx);
+            const gdwords row0 = (gdwords)(a & ~(uintptr_t)3);
+            const uint32_t sh = (uint32_t)(a & 3);
+            const int64_t rstep = i_ref >> 2;
+            uint32_t sad = 0;
+            for (int r = 0; r < pu.h; r++) {
+                const gdwords rp = row0 + r * rstep;
+                uint32_t d[GWMAX + 1];
 #pragma unroll
-            for (int k = 0; k < 6; k++)
-                sad[k] = eval(k, mxs[k], mys[k], ok[k]);
+                for (int k = 0; k <= GWMAX; k++)
+                    d[k] = k <= gw ? rp[k] : 0u;
 #pragma unroll
-            for (int k = 0; k < 6; k++) {
-                int sum = group_sum((int)sad[k], S);
-                sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
-                if (ok[k]) {
-                    take(sum, mxs[k], mys[k]);
-                    evals++;
-                }
+                for (int k = 0; k < GWMAX; k++)
+                    if (k < gw)
+                        sad = __builtin_amdgcn_sad_u8(cur[r * gw + k], __builtin_amdgcn_alignbyte(d[k + 1], d[k], sh), sad);
             }
+            const int cost = (int)sad + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+            // costs stay below 2^21 (a 64x64 PU: 1,044,480 + lambda bits),
+            // the point of the round takes the low 6 bits
+            key = (cost << 6) | pi;
         }
-        bmx = nbx;
-        bmy = nby;
+        int kmin = group_min(key, S);
+        kmin = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, kmin);
+        // the group's points of the round, the same count in every lane
+        int nok = group_sum(ok && j == 0 ? 1 : 0, S);
+        evals += (uint32_t)__builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, nok);
+        if (live && kmin != 0x7fffffff && (kmin >> 6) < bcost) {
+            bcost = kmin >> 6;
+            int wx = 0, wy = 0;
+            umh_point(u, c, p.R, p.big_hex, u.bi * PR + (kmin & 63), wx, wy);
+            bmx = wx;
+            bmy = wy;
+        }
         if (live) {
             u.bi++;
-            if (u.bi * 6 >= n_st) {
+            if (u.bi * PR >= n_st) {
                 umh_done_stage(u, bmx, bmy);
                 umh_next(u, c, p.R, p.big_hex, bcost, bmx, bmy, phase);
             }
@@ -501,12 +505,23 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
     int idx = 0, hex_i = 0, dia_i = 0;
     uint32_t evals = 0;
     if constexpr (UMH) {
-        // one point at a time: the prefix's many far points, few registers
         xme_umh_ctx c = {};
         if (phase != kDone)
             c = umh[id];
-        umh_prefix<S, QMAX, true>(p, pu, c, win, plane + (int64_t)pu.y * i_ref + pu.x, i_ref, c4, nq, in.mvc[0][0],
-                                 in.mvc[0][1], phase, bcost, bmx, bmy, evals);
+        // the group's current quads, row-major, in its LDS share (free until
+        // the first step box; at most 1024 dwords for 64x64 in 1224)
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+            if (q < nq)
+                gbox[l + q * S] = c4[q];
+        __builtin_amdgcn_wave_barrier();
+        // 64-lane groups (PUs of 192 quads and more): 16 lanes a point.  For
+        // the smaller PUs a lane a point measured faster: sharing a point
+        // cuts the points of a round, and rounds carry the stage logic
+        umh_prefix_lanes<S, (S <= 8 ? 4 : S <= 32 ? 8 : 16), (S == 64 ? 16 : 1)>(
+            p, pu, c, win, plane + (int64_t)pu.y * i_ref + pu.x, i_ref, gbox, in.mvc[0][0], in.mvc[0][1], phase,
+            bcost, bmx, bmy, evals);
+        __builtin_amdgcn_wave_barrier();
     }
     for (;;) {
         // stage ends that need no evaluation (loop heads of me.c:1120, :1136, :1145)

@@ -37,6 +37,25 @@ __device__ __forceinline__ int group_sum(int v, int P)
     return v;
 }
 
+// Minimum over each aligned group of P lanes (P a power of two <= 64,
+// uniform); the minimum is in the group's last lane.
+__device__ __forceinline__ int group_min(int v, int P)
+{
+    if (P >= 2)
+        v = dpp_min<0xB1>(v);
+    if (P >= 4)
+        v = dpp_min<0x4E>(v);
+    if (P >= 8)
+        v = dpp_min<0x114>(v);
+    if (P >= 16)
+        v = dpp_min<0x118>(v);
+    if (P >= 32)
+        v = dpp_min<0x142, 0xA>(v);
+    if (P >= 64)
+        v = dpp_min<0x143, 0xC>(v);
+    return v;
+}
+
 // Sum over the wave, returned to every lane.
 __device__ __forceinline__ int wave_sum(int v) { return __builtin_amdgcn_readlane(group_sum(v, 64), 63); }
 
