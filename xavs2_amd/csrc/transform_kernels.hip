@@ -124,12 +124,21 @@ __device__ __forceinline__ v4i load4(const uint32_t (&p)[4])
     return r;
 }
 
-// Offset (in coefficients) of tile element (y, x) of tile `tile`.
+// Offset (in coefficients) of element (y, x) of tile t, split into its row
+// and column parts: t * 1024 + rel_y(y) + rel_x(x) (a tile is NB x NB blocks
+// of N x N, each block contiguous, row-major within).  A lane's rows rho(half, i) give
+// rel_y(rho(half, i)) = rel_y(rho(0, i)) + half * rel_y(4) for every N (the
+// 4-row step never leaves an N-row block for N >= 8, and moves one block row
+// for N = 4), so a lane's 16 elements are one base plus compile-time offsets.
 template <int N>
-__device__ __forceinline__ int64_t tile_off(int64_t tile, int y, int x)
+__host__ __device__ constexpr int rel_y(int y)
 {
-    constexpr int NB = 32 / N;
-    return (tile * NB * NB + (y / N) * NB + x / N) * (N * N) + (y % N) * N + (x % N);
+    return (y / N) * (32 / N) * (N * N) + (y % N) * N;
+}
+template <int N>
+__host__ __device__ constexpr int rel_x(int x)
+{
+    return (x / N) * (N * N) + x % N;
 }
 
 template <int N>
@@ -152,19 +161,24 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
 
     for (int64_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
         v4i hi, lo;
+        // this lane's elements (rho(half, i), col): e_lane + rel_y(rho(0, i)),
+        // in block b_lane + (rho(0, i) / N) * NB
+        const int64_t e_lane = t * 1024 + rel_x<N>(col) + half * rel_y<N>(4);
+        const int64_t b_lane = t * BPT + col / N + (N == 4 ? half * NB : 0);
         if (mode & DO_FWD) {
             // F1 operand: tile row `col`, columns 16 half .. 16 half + 15
             uint32_t d[8];
-            const int y = col;
+            // pieces of W contiguous coefficients: row `col`, columns
+            // 16 half + piece W, i.e. blocks (col / N, 16 half / N + piece)
+            const int16_t *p0 = src + t * 1024 + rel_y<N>(col) + rel_x<N>(16 * half);
+            const int64_t blk0 = t * BPT + (col / N) * NB + (16 * half) / N;
 #pragma unroll
             for (int piece = 0; piece < 16 / (N < 16 ? N : 16); piece++) {
-                constexpr int W = N < 16 ? N : 16;          // contiguous coefficients per piece
-                const int x = 16 * half + piece * W;
-                const int64_t blk = t * BPT + (y / N) * NB + x / N;
-                const int16_t *p = src + tile_off<N>(t, y, x);
+                constexpr int W = N < 16 ? N : 16;
+                const uint32_t *p = reinterpret_cast<const uint32_t *>(p0 + piece * (N * N));
 #pragma unroll
                 for (int k = 0; k < W / 2; k++)
-                    d[piece * (W / 2) + k] = blk < nblk ? reinterpret_cast<const uint32_t *>(p)[k] : 0u;
+                    d[piece * (W / 2) + k] = blk0 + piece < nblk ? p[k] : 0u;
             }
             split_pairs(d, hi, lo);
             v16i D = mm_data_a(hi, lo, f1b, splat16(acc_f1));
@@ -199,12 +213,9 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             }
             // levels (or plain coefficients) out
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int y2 = rho(half, i);
-                const int64_t blk = t * BPT + (y2 / N) * NB + col / N;
-                if (blk < nblk)
-                    level[tile_off<N>(t, y2, col)] = (int16_t)c[i];
-            }
+            for (int i = 0; i < 16; i++)
+                if (b_lane + (rho(0, i) / N) * NB < nblk)
+                    level[e_lane + rel_y<N>(rho(0, i))] = (int16_t)c[i];
             if ((mode & DO_QUANT) && nnz && lane < BPT && t * BPT + lane < nblk)
                 nnz[t * BPT + lane] = mine;
             if (mode & DO_INV) {
@@ -221,11 +232,8 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             // inverse only: I1 operand lane row j = col, K bytes m = rho(half, t)
             int v[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int y = rho(half, i);
-                const int64_t blk = t * BPT + (y / N) * NB + col / N;
-                v[i] = blk < nblk ? src[tile_off<N>(t, y, col)] : 0;
-            }
+            for (int i = 0; i < 16; i++)
+                v[i] = b_lane + (rho(0, i) / N) * NB < nblk ? src[e_lane + rel_y<N>(rho(0, i))] : 0;
             split16(v, hi, lo);
         }
         if (mode & DO_INV) {
@@ -237,12 +245,9 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             split16(v, hi, lo);
             D = mm_data_a(hi, lo, ib, splat16(acc_i2));
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int y = rho(half, i);
-                const int64_t blk = t * BPT + (y / N) * NB + col / N;
-                if (blk < nblk)
-                    recon[tile_off<N>(t, y, col)] = (int16_t)xme_clip3(-256, 255, D[i] >> 12);
-            }
+            for (int i = 0; i < 16; i++)
+                if (b_lane + (rho(0, i) / N) * NB < nblk)
+                    recon[e_lane + rel_y<N>(rho(0, i))] = (int16_t)xme_clip3(-256, 255, D[i] >> 12);
         }
     }
 }
