@@ -13,8 +13,9 @@ graph.
 Printed: one JSON line with candidate evaluations per second (batched path),
 the roofline of the integer-search kernel, the plain one-launch-per-PU-shape
 GPU path on the same job, per-frame latency p50/p99, and the CPU oracle timed
-on a bounded sample of the same job.  Side legs: the DIA and HEX pattern
-searches on the same frame, the residual transform, and the serving loop.
+on a bounded sample of the same job.  Side legs: the DIA, HEX and UMH
+pattern searches on the same frame, the residual transform, and the serving
+loop.
 """
 import argparse
 import json

@@ -110,7 +110,7 @@ typedef struct {
     int32_t status;                   /* 0 ok; 1 no legal MV candidate (_me_error) */
 } xo_me_result_t;
 
-/* Integer search (full search, or the DIA / HEX pattern search by
+/* Integer search (full search, or the DIA / HEX / UMH pattern search by
  * prm->me_method) + sub-pel refinement for one PU against one reference.
  *   cur    : current luma plane origin, stride i_cur (PU at cur + pix_y*i_cur + pix_x)
  *   planes : planes[0] = padded full-pel reference origin; planes[1..15] the

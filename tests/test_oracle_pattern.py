@@ -1,4 +1,5 @@
-"""Pins the oracle's DIA and HEX integer searches (encoder/me.c:1101-1154).
+"""Pins the oracle's DIA, HEX (encoder/me.c:1101-1154) and UMH
+(me.c:1030-1099) integer searches.  For DIA and HEX:
 
 * Hand-worked paths on a flat frame, where every SAD is 0 and a point's cost
   is its MV bits alone, so each step can be worked out on paper from the
@@ -14,6 +15,12 @@
 * The integer result of the whole search is the last point of the path.
 * Where the current picture is the reference moved by a small whole-pixel
   shift, both searches find the shift, as the full search does.
+
+For UMH, pictures built so that one stage alone can find the match: the
+grid's repeated (0, 2), points on the cross and on the big hexagons, the
+predictor points; early-termination routes forced through the context
+(including the strict double bound); walks that fall and end in diamond
+minima.
 """
 import numpy as np
 import pytest
