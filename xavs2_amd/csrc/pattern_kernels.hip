@@ -692,7 +692,21 @@ __global__ void __launch_bounds__(256) pattern_kernel(DevParams p, const FrameDe
                                                       const xme_umh_ctx *__restrict__ umh)
 {
     __shared__ uint32_t boxes[4][XME_PAT_BOX_DW];
-    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    // Ranges are whole workgroups.  Workgroup b runs on XCD b % 8; within
+    // its range, the workgroups of one XCD take a contiguous run of the
+    // range's list (a bijection on the range), so that one L2 sees
+    // neighbouring PUs of one reference.
+    int w;
+    {
+        const int b = blockIdx.x;
+        int k = 0;
+        while (k < 5 && b * 4 >= L.end[k])
+            k++;
+        const int gs = (k == 0 ? 0 : L.end[k - 1]) >> 2, G = (L.end[k] >> 2) - gs;
+        const int g = b - gs, x = g & 7, y = g >> 3, per = G >> 3, rem = G & 7;
+        const int gp = x * per + min(x, rem) + y;
+        w = __builtin_amdgcn_readfirstlane((gs + gp) * 4 + (int)(threadIdx.x >> 6));
+    }
     const int lane = threadIdx.x & 63;
     uint32_t *box = boxes[threadIdx.x >> 6];
     if (w < L.end[0]) {

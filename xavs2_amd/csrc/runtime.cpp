@@ -595,17 +595,21 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     std::vector<int32_t> pat_list;
     if (p.method != XME_ME_FS) {
         // pattern searches: one launch in either mode, the instances of each
-        // PU-size class packed 64 / S per wave, the largest class first
+        // PU-size class packed 64 / S per wave, the largest class first.
+        // A class lists its instances reference-major in PU order and is
+        // padded to whole workgroups of 4 waves: the kernel hands each XCD a
+        // contiguous run of a class's workgroups, so one L2 sees neighbouring
+        // PUs of one reference plane.
         static const int per_wave[6] = { 1, 1, 2, 4, 8, 16 };
         int waves = 0;
         std::vector<int32_t> cls[6];
-        for (int i = 0; i < n_pu; i++)
-            for (int r = 0; r < n_ref; r++)
+        for (int r = 0; r < n_ref; r++)
+            for (int i = 0; i < n_pu; i++)
                 cls[5 - xme_pattern_class(pus[i].w, pus[i].h)].push_back(i * n_ref + r);
         for (int k = 0; k < 6; k++) {
             for (int32_t id : cls[k])
                 pat_list.push_back(id);
-            const int nw = ((int)cls[k].size() + per_wave[k] - 1) / per_wave[k];
+            const int nw = ((((int)cls[k].size() + per_wave[k] - 1) / per_wave[k]) + 3) & ~3;
             pat_list.resize(pat_list.size() + (size_t)(nw * per_wave[k] - (int)cls[k].size()), -1);
             waves += nw;
             pl->pat_wave_end[k] = waves;

@@ -180,8 +180,9 @@ int xme_launch_subpel(const DevParams &p, const FrameDesc *d_desc, const DevPu *
 // Pattern searches (XME_ME_DIA, XME_ME_HEX) of every instance.  The six PU
 // size classes of xme_pattern_class (0: <= 16 pixel quads ... 5: > 256) run
 // as wave ranges, largest class first: range k holds class 5 - k with
-// 1, 1, 2, 4, 8, 16 instance ids per wave (d_list back to back, padded per
-// wave with -1); wave_end[k] is the first wave after range k.  d_points[id]
+// 1, 1, 2, 4, 8, 16 instance ids per wave (d_list back to back, padded with
+// -1 to whole workgroups of 4 waves); wave_end[k] is the first wave after
+// range k, a multiple of 4.  d_points[id]
 // receives the points instance id evaluated.
 int xme_pattern_class(int w, int h);
 int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_pus, DevInst *d_inst,
