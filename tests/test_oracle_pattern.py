@@ -451,3 +451,34 @@ def test_umh_second_diamond_needs_a_nonzero_mvp(mvp, first, second_diamond):
     assert cost(*nxt) < cost(*first) < min(cost(0, 0), cost((mvp[0] + 2) >> 2, (mvp[1] + 2) >> 2))
     _, route, _ = umh_route(cur, ref, pu, mvp, lam, 8, umh_ctx(pred_sad_uplayer=cost(*first), beta3=0.0))
     assert (route == 1 | 3 << 4) == second_diamond
+
+
+def test_umh_gpu_test_contexts_take_every_route():
+    """The seeded contexts of the UMH GPU parity test (all 25 shapes, two
+    references, big-hexagon level 2) send searches through the whole prefix
+    and out at each of the four early-termination points, to the hexagon and
+    to the diamond: the GPU test compares every route with the oracle."""
+    from tests.test_gpu_pattern import random_umh_ctx
+    from xavs2_amd import synth
+    w, h, R, level, seed = 256, 160, 16, 2, 69
+    rng = np.random.default_rng(61 + level)
+    pus = []
+    for part in range(25):
+        pw, ph = O.PU_W[part], O.PU_H[part]
+        for _ in range(4):
+            x = int(rng.integers(0, (w - pw) // 4 + 1)) * 4
+            y = int(rng.integers(0, h - ph + 1))
+            pus.append((x, y, pw, ph, int(rng.integers(-60, 61)), int(rng.integers(-60, 61))))
+    refs = [synth.smooth_field(w, h, seed=seed * 10 + r) for r in range(2)]
+    cur = synth.multi_ref_current(refs, 16, min(R, 24), seed=seed + 99)
+    ctx = random_umh_ctx(len(pus), 2, seed + 5)
+    routes = set()
+    for r in range(2):
+        for i, pu in enumerate(pus):
+            c = np.ascontiguousarray(ctx[i * 2 + r:i * 2 + r + 1]).view(O.UMH_CTX_DTYPE)
+            _, route, _ = umh_route(cur, refs[r], pu[:4], pu[4:], 4 * 65536, R, c, level)
+            routes.add((route & 15, route >> 4))
+    assert (0, 2) in routes
+    for where in (1, 2, 3, 4):
+        assert any(k == where for k, _ in routes), (where, sorted(routes))
+    assert {t for _, t in routes} == {2, 3}
