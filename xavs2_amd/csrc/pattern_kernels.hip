@@ -602,10 +602,11 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
         } else {
             // SADs of all the step's points, then their sums in order
             uint32_t sad[6];
+            int dxs[6], dys[6], dcs[6];
 #pragma unroll
             for (int k = 0; k < 6; k++) {
-                int dx, dy, dc;
-                point_of(phase, idx, k, dx, dy, dc);
+                point_of(phase, idx, k, dxs[k], dys[k], dcs[k]);
+                const int dx = dxs[k], dy = dys[k];
                 sad[k] = 0;
                 if (k < n) {
                     const int pb = (dy + 2) * DW * 4 + sh + dx + 2;
@@ -624,11 +625,8 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
             for (int k = 0; k < 6; k++) {
                 int sum = group_sum((int)sad[k], S);
                 sum = __builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, sum);
-                if (k < n) {
-                    int dx, dy, dc;
-                    point_of(phase, idx, k, dx, dy, dc);
-                    take_point(p, pu, bmx + dx, bmy + dy, dc, sum, bcost, nbx, nby, dir);
-                }
+                if (k < n)
+                    take_point(p, pu, bmx + dxs[k], bmy + dys[k], dcs[k], sum, bcost, nbx, nby, dir);
             }
         }
         // the next step's box overwrites this one's only after every lane's reads
