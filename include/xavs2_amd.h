@@ -299,6 +299,8 @@ typedef struct {
     int32_t enable_pmvr;
     int32_t workgroup_share_pct;     /* % of a full-device search launch per frame (1..100; 0 = 100) */
     int64_t launch_split_absdiff;    /* search work per launch (0: 8e9; < 0: no split) */
+    int32_t me_method;               /* XME_ME_FS, XME_ME_DIA or XME_ME_HEX (UMH needs per-job state) */
+    int32_t reserved;
 } xme_server_config;
 typedef struct {
     int64_t submitted, completed, rejected;

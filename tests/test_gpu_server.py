@@ -16,10 +16,10 @@ LAM = 4 * 65536
 GIB = 1 << 30
 
 
-def oracle_frame_results(frames, f, pus, n_ref, R):
+def oracle_frame_results(frames, f, pus, n_ref, R, me_method=O.ME_FS):
     """What the server must return for frame f: PU-major, reference r = frame f-1-r."""
     h, w = frames[0].shape
-    prm = O.me_param(w, h, R, LAM)
+    prm = O.me_param(w, h, R, LAM, me_method=me_method)
     arr = np.array([(p[0], p[1], p[2], p[3], p[4], p[5]) for p in pus], dtype=O.PU_DTYPE)
     cur = O.Plane(w, h)
     cur.set_picture(frames[f])
@@ -61,6 +61,27 @@ def test_stream_results_match_oracle():
     srv.close_stream(sid)
     assert srv.stats()["resident_bytes"] == 0
     srv.close()
+
+
+@pytest.mark.parametrize("method", [X.ME_DIA, X.ME_HEX])
+def test_pattern_search_server_matches_oracle(method):
+    """A server configured for the diamond or hexagon search: every frame's
+    results equal the oracle's pattern search."""
+    w, h, R, n_ref = 192, 128, 32, 2
+    frames = synth.moving_sequence(w, h, 4, seed=13, local=8)
+    pus = pu_tuples(w, h, 6)
+    srv = X.Server(4 * GIB, max_inflight=2, search_range=R, n_ref=n_ref, me_method=method)
+    sid = srv.open_stream(w, h, X.make_pus(pus))
+    jobs = [srv.submit(sid, fr) for fr in frames]
+    for f, j in enumerate(jobs):
+        got, _ = srv.wait(j)
+        exp = oracle_frame_results(frames, f, pus, n_ref, R, method)
+        bad = np.nonzero(got != exp)[0]
+        assert len(bad) == 0, f"frame {f}: {len(bad)} mismatches, first {bad[0]}: {got[bad[0]]} vs {exp[bad[0]]}"
+    srv.close_stream(sid)
+    srv.close()
+    with pytest.raises(RuntimeError):
+        X.Server(GIB, max_inflight=2, search_range=16, n_ref=2, me_method=X.ME_UMH)
 
 
 def run_streams(specs, n_frames, interleave, max_inflight, budget=8 * GIB, R=32):

@@ -350,7 +350,8 @@ extern "C" int xme_server_create(const xme_server_config *cfg, xme_server **out)
     if (!cfg || !out || cfg->memory_budget <= 0 || cfg->max_inflight < 1 || cfg->max_inflight > kMaxSlots ||
         cfg->search_range < 1 || cfg->search_range > 128 || cfg->n_ref < 1 || cfg->n_ref > XME_MAX_REFS ||
         cfg->lambda_factor < 0 || cfg->use_fractional_me < 0 || cfg->use_fractional_me > 2 ||
-        cfg->workgroup_share_pct < 0 || cfg->workgroup_share_pct > 100) {
+        cfg->workgroup_share_pct < 0 || cfg->workgroup_share_pct > 100 ||
+        (cfg->me_method != XME_ME_FS && cfg->me_method != XME_ME_DIA && cfg->me_method != XME_ME_HEX)) {
         xme_set_error("xme_server_create: bad configuration");
         return XME_ERR_ARG;
     }
@@ -434,6 +435,7 @@ extern "C" int xme_server_open_stream(xme_server *S, int w, int h, const xme_pu 
     prm.use_fractional_me = S->cfg.use_fractional_me;
     prm.use_fast_sub_me = S->cfg.use_fast_sub_me;
     prm.enable_pmvr = S->cfg.enable_pmvr;
+    prm.me_method = S->cfg.me_method;
     // the plans first (they validate the PUs); their memory counts as resident
     for (int r = 1; r <= S->cfg.n_ref; r++) {
         xme_plan *p = nullptr;
