@@ -26,21 +26,26 @@
 namespace {
 
 // ---------------------------------------------------------------------------
-// Candidate-list pass: one wave per (PU, ref).
-// One wave per (PU, reference), four per workgroup; workgroup b runs on XCD
-// b % 8, and the instances are walked reference-major in contiguous ranges per
-// XCD so that neighbouring PUs of one reference share an L2.
+// Candidate-list pass: a group of 16 lanes per (PU, reference), four groups a
+// wave, 16 a workgroup; workgroup b runs on XCD b % 8, and the instances are
+// walked reference-major in contiguous ranges per XCD so that neighbouring
+// PUs of one reference share an L2.  A PU of more than 16 pixel quads loops
+// over them (the groups of a wave run their loops independently; the sums
+// follow once the wave has reconverged).
 __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                   const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
                                                   uint64_t *__restrict__ best, int n_inst)
 {
+    constexpr int S = 16;
     const int per_xcd = gridDim.x / 8;
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const int logical = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
-    if (logical >= n_inst)
-        return;
-    const int lane = threadIdx.x & 63;
-    const int r = logical / p.n_pu, pu_i = logical - r * p.n_pu;
+    const int lane = threadIdx.x & 63, l = lane & (S - 1);
+    const int logical = lb * 16 + (int)(threadIdx.x >> 4);
+    if (__builtin_amdgcn_readfirstlane(lb * 16 + (int)((threadIdx.x >> 6) << 2)) >= n_inst)
+        return;   // the whole wave is past the end
+    const bool live = logical < n_inst;
+    const int li = live ? logical : 0;
+    const int r = li / p.n_pu, pu_i = li - r * p.n_pu;
     const int id = pu_i * p.n_ref + r;
     const DevPu pu = pus[pu_i];
     const MvWindow win = xme_mv_window(p, pu.x, pu.y, pu.w, pu.h);
@@ -73,23 +78,30 @@ __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *
     gbytes rp[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        ok[c] = c < n && xme_mv_legal(win, cand[c][0], cand[c][1]);
+        ok[c] = live && c < n && xme_mv_legal(win, cand[c][0], cand[c][1]);
         rp[c] = ref + (int64_t)(ok[c] ? cand[c][1] : 0) * i_ref + (ok[c] ? cand[c][0] : 0);
     }
     uint32_t sad[2] = { 0, 0 };
     const int gw = pu.w >> 2;
-    for (int t = lane; t < gw * pu.h; t += 64) {
+    for (int t = l; live && t < gw * pu.h; t += S) {
         const int yy = t / gw, xx = (t - yy * gw) * 4;
         const uint32_t c4 = *(gdwords)(cur + (int64_t)yy * desc->i_cur + xx);
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            if (ok[c]) {     // uniform
+            if (ok[c]) {
                 const uintptr_t a = (uintptr_t)(rp[c] + (int64_t)yy * i_ref + xx);
                 const gdwords q4 = (gdwords)(a & ~(uintptr_t)3);
                 sad[c] = __builtin_amdgcn_sad_u8(c4, __builtin_amdgcn_alignbyte(q4[1], q4[0], (uint32_t)(a & 3)), sad[c]);
             }
         }
     }
+    // group sums in the group's last lane
+    int tot[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+        tot[c] = group_sum((int)sad[c], S);
+    if (!live || l != S - 1)
+        return;
     uint64_t key = XME_KEY_NONE;
     int bx = 0, by = 0;
 #pragma unroll
@@ -97,8 +109,7 @@ __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *
         if (!ok[c])
             continue;
         const int mx = cand[c][0], my = cand[c][1];
-        const int cost = wave_sum((int)sad[c]) +
-                         xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
+        const int cost = tot[c] + xme_weighted_cost(p.lambda, xme_mvbits(4 * mx - pu.mvp_x) + xme_mvbits(4 * my - pu.mvp_y));
         const uint64_t k = xme_key((uint32_t)cost, (uint32_t)c);
         if (k < key) {
             key = k;
@@ -106,20 +117,18 @@ __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *
             by = my;
         }
     }
-    if (lane == 0) {
-        DevInst d;
-        d.omx = (int16_t)bx;
-        d.omy = (int16_t)by;
-        for (int c = 0; c < 2; c++) {
-            d.mvc[c][0] = (int16_t)(c < n ? cand[c][0] : 0);
-            d.mvc[c][1] = (int16_t)(c < n ? cand[c][1] : 0);
-        }
-        d.n_mvc = (int8_t)n;
-        d.status = key == XME_KEY_NONE ? 1 : 0;
-        d.pad = 0;
-        inst[id] = d;
-        best[id] = key;
+    DevInst d;
+    d.omx = (int16_t)bx;
+    d.omy = (int16_t)by;
+    for (int c = 0; c < 2; c++) {
+        d.mvc[c][0] = (int16_t)(c < n ? cand[c][0] : 0);
+        d.mvc[c][1] = (int16_t)(c < n ? cand[c][1] : 0);
     }
+    d.n_mvc = (int8_t)n;
+    d.status = key == XME_KEY_NONE ? 1 : 0;
+    d.pad = 0;
+    inst[id] = d;
+    best[id] = key;
 }
 
 // ---------------------------------------------------------------------------
@@ -986,8 +995,8 @@ int xme_launch_mvc(const DevParams &p, const FrameDesc *d_desc, const DevPu *d_p
 {
     if (n_inst <= 0)
         return XME_OK;
-    // a multiple of 8 workgroups (one range per XCD), 4 instances each
-    const int grid = ((n_inst + 3) / 4 + 7) / 8 * 8;
+    // a multiple of 8 workgroups (one range per XCD), 16 instances each
+    const int grid = ((n_inst + 15) / 16 + 7) / 8 * 8;
     hipLaunchKernelGGL(mvc_kernel, dim3(grid), dim3(256), 0, s, p, d_desc, d_pus, d_inst, d_best, n_inst);
     XME_HIP(hipGetLastError());
     return XME_OK;
