@@ -252,3 +252,17 @@ def test_umh_random_mvps_mixed_shapes_1080p_range64():
                             np.array([tuple(q) for q in pus], dtype=O.PU_DTYPE), cp, O.interp_planes(p, 2),
                             nthreads=16, ctxs=ctx.view(O.UMH_CTX_DTYPE))
     compare(got, exp.astype(X.RESULT_DTYPE), pus, 1)
+
+
+@pytest.mark.parametrize("method", [X.ME_DIA, X.ME_HEX, X.ME_UMH])
+def test_no_legal_candidate_status(method):
+    """An MV range that leaves no legal candidate (me.c:919-921): status 1 and
+    XME_MAX_DISTORTION, as the oracle returns, for every pattern search."""
+    w, h = 64, 64
+    pus = [(16, 16, 16, 16, 0, 0), (32, 8, 8, 8, 0, 0)]
+    kw = dict(mv_range_y=(40, 41))
+    if method == X.ME_UMH:
+        ctx = random_umh_ctx(len(pus), 1, 97)
+        run_umh_case(w, h, pus, 4, 1, 2, 2, kw, ctx)
+    else:
+        run_case(w, h, pus, 4, seed=2, prm_kw=dict(me_method=method, **kw))
