@@ -35,6 +35,15 @@ UMH_CTX_DTYPE = np.dtype([("mvp1_x", "<i4"), ("mvp1_y", "<i4"), ("mvp2_x", "<i4"
                           ("pred_sad_uplayer", "<i4"), ("pad", "<i4"), ("beta2", "<f8"), ("beta3", "<f8")])
 
 
+class SymJob(C.Structure):
+    _fields_ = [(f, C.c_int) for f in ("pix_x", "pix_y", "w", "h", "mv_x", "mv_y", "mvp_x", "mvp_y",
+                                       "mvp1_x", "mvp1_y", "dist_fwd", "dist_bwd", "b_slice")]
+
+
+class SymResult(C.Structure):
+    _fields_ = [("cost", C.c_int32), ("mv_x", C.c_int32), ("mv_y", C.c_int32), ("mvcost", C.c_int32)]
+
+
 class Pu(C.Structure):
     _fields_ = [("pix_x", C.c_int), ("pix_y", C.c_int), ("w", C.c_int), ("h", C.c_int),
                 ("mvp_x", C.c_int), ("mvp_y", C.c_int)]
@@ -79,6 +88,9 @@ def lib():
         L.xo_me_search_batch.argtypes = [C.POINTER(MeParam), P, C.c_int, P, ip, P, ip, P, C.c_int]
         L.xo_me_pattern_path.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, C.c_int, C.c_int,
                                          C.c_int, C.c_int, C.c_int, P, C.c_int, P]
+        L.xo_scale_mv_skip.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.xo_me_search_sym.restype = C.c_int32
+        L.xo_me_search_sym.argtypes = [C.POINTER(MeParam), C.POINTER(SymJob), P, ip, P, P, ip, C.POINTER(SymResult)]
         L.xo_me_search_ctx.restype = C.c_int32
         L.xo_me_search_ctx.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, P]
         L.xo_me_search_batch_ctx.argtypes = [C.POINTER(MeParam), P, P, C.c_int, P, ip, P, ip, P, C.c_int]
@@ -224,6 +236,20 @@ def me_pattern_path(prm, pu_tuple, cur_plane, ref_plane, start, start_cost, max_
     assert n <= max_pts
     pts = [(int(path[2 * k]), int(path[2 * k + 1])) for k in range(n)]
     return (pts, route.value) if with_route else pts
+
+
+def me_search_sym(prm, job, cur_plane, planes1, planes2):
+    """xavs2_me_search_sym for one job: a dict with the SymJob fields.
+    Returns (cost, mv_x, mv_y, mvcost)."""
+    j = SymJob(**job)
+    r = SymResult()
+    lib().xo_me_search_sym(C.byref(prm), C.byref(j), cur_plane.origin_ptr(), cur_plane.stride,
+                           plane_ptrs(planes1), plane_ptrs(planes2), planes1[0].stride, C.byref(r))
+    return r.cost, r.mv_x, r.mv_y, r.mvcost
+
+
+def scale_mv_skip(mv, dist_dst, dist_src):
+    return lib().xo_scale_mv_skip(mv, dist_dst, dist_src)
 
 
 def me_search_batch(prm, pus, cur_plane, planes, nthreads=1, ctxs=None):

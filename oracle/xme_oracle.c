@@ -704,6 +704,101 @@ int32_t xo_me_search_ctx(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_u
     return me_search_impl(prm, pu, ctx, cur, i_cur, planes, i_ref, res, NULL);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Symmetric bi-prediction refinement.                                       */
+
+int xo_scale_mv_skip(int mv, int dist_dst, int dist_src)
+{
+    /* (int16_t)((mv * dist_dst * (MULTI / dist_src) + HALF_MULTI) >> OFFSET),
+     * MULTI 16384, OFFSET 14 (defines.h:215-218) */
+    return (int16_t)((mv * dist_dst * (16384 / dist_src) + 8192) >> 14);
+}
+
+/* ME_COST_QPEL_SYM (me.c:353-382): both MVs range-checked against the qpel
+ * window, both planes present, fpel_cmp of the current PU against the
+ * rounded average + MV_COST_FPEL of the first MV; else XO_MAX_DISTORTION. */
+static int sym_cost(const xo_me_param_t *prm, const xo_sym_job_t *j, const mv_window_t *qw,
+                    const uint8_t *org, intptr_t i_cur, const uint8_t *const p1[16], const uint8_t *const p2[16],
+                    intptr_t i_ref, int mx, int my)
+{
+    int sx = xo_scale_mv_skip(mx, j->dist_bwd, j->dist_fwd);
+    int sy = xo_scale_mv_skip(my, j->dist_bwd, j->dist_fwd);
+    if (j->b_slice) {
+        sx = -sx;
+        sy = -sy;
+    }
+    if (!check_mv_range(mx, my, qw->mv_min, qw->mv_max) || !check_mv_range(sx, sy, qw->mv_min, qw->mv_max))
+        return XO_MAX_DISTORTION;
+    const uint8_t *a = p1[((my & 3) << 2) + (mx & 3)];
+    const uint8_t *b = p2[((sy & 3) << 2) + (sx & 3)];
+    if (!a || !b)
+        return XO_MAX_DISTORTION;
+    const intptr_t off = (intptr_t)j->pix_y * i_ref + j->pix_x;
+    a += off + (intptr_t)(my >> 2) * i_ref + (mx >> 2);
+    b += off + (intptr_t)(sy >> 2) * i_ref + (sx >> 2);
+    uint8_t pred[64 * 64];
+    xo_avg(j->w, j->h, pred, 64, a, i_ref, b, i_ref);
+    const int d = (prm->use_hadamard ? xo_satd : xo_sad)(j->w, j->h, org, i_cur, pred, 64);
+    return d + weighted_cost(prm->lambda_factor, xo_mvbits(prm->search_range, mx - j->mvp_x) +
+                                                  xo_mvbits(prm->search_range, my - j->mvp_y));
+}
+
+int32_t xo_me_search_sym(const xo_me_param_t *prm, const xo_sym_job_t *job,
+                         const uint8_t *cur, intptr_t i_cur,
+                         const uint8_t *const planes1[16], const uint8_t *const planes2[16], intptr_t i_ref,
+                         xo_sym_result_t *res)
+{
+    /* p_me->mv_min / mv_max: the qpel limits of md_inter.c:913-924 */
+    mv_window_t qw;
+    {
+        int lo = clip3(prm->min_mv_range[0], prm->max_mv_range[0], (-64 - job->pix_x) * 4);
+        int hi = clip3(prm->min_mv_range[0], prm->max_mv_range[0], (prm->frame_w + 64 - job->pix_x - job->w) * 4);
+        int lo_y = clip3(prm->min_mv_range[1], prm->max_mv_range[1], (-64 - job->pix_y) * 4);
+        int hi_y = clip3(prm->min_mv_range[1], prm->max_mv_range[1], (prm->frame_h + 64 - job->pix_y - job->h) * 4);
+        qw.mv_min = pack_mask2(-lo, -lo_y);
+        qw.mv_max = pack_mask2(hi, hi_y) | 0x8000u;
+    }
+    const uint8_t *org = cur + (intptr_t)job->pix_y * i_cur + job->pix_x;
+    const int ctr_x = (job->mvp1_x >> 1) << 1, ctr_y = (job->mvp1_y >> 1) << 1;
+    int bx = job->mv_x, by = job->mv_y, bcost = XO_MAX_DISTORTION;
+    res->mvcost = -1;
+    if (!prm->use_fractional_me) {
+        bcost = sym_cost(prm, job, &qw, org, i_cur, planes1, planes2, i_ref, bx, by);
+        res->cost = bcost;
+        res->mv_x = bx;
+        res->mv_y = by;
+        return bcost;
+    }
+    /* half-pel: Spiral[0..4] when hadamard, else Spiral[1..4] (me.c:1210-1211) */
+    const int cx = job->mv_x, cy = job->mv_y;
+    for (int pos = prm->use_hadamard ? 0 : 1; pos < 5; pos++) {
+        const int mx = cx + SPIRAL[pos][0] * 2, my = cy + SPIRAL[pos][1] * 2;
+        const int c = sym_cost(prm, job, &qw, org, i_cur, planes1, planes2, i_ref, mx, my);
+        if (c < bcost) { bcost = c; bx = mx; by = my; }
+    }
+    if (prm->use_fractional_me >= 2) {
+        const int qx = bx, qy = by;           /* mv->v = bmv.v */
+        for (int pos = 1; pos < 5; pos++) {
+            int mx, my;
+            if (prm->enable_pmvr) {
+                if (pmvr_adapt(&mx, &my, ctr_x, ctr_y, qx, qy, SPIRAL[pos][0], SPIRAL[pos][1]))
+                    continue;
+            } else {
+                mx = qx + SPIRAL[pos][0];
+                my = qy + SPIRAL[pos][1];
+            }
+            const int c = sym_cost(prm, job, &qw, org, i_cur, planes1, planes2, i_ref, mx, my);
+            if (c < bcost) { bcost = c; bx = mx; by = my; }
+        }
+    }
+    res->cost = bcost;
+    res->mv_x = bx;
+    res->mv_y = by;
+    res->mvcost = weighted_cost(prm->lambda_factor, xo_mvbits(prm->search_range, bx - job->mvp_x) +
+                                                    xo_mvbits(prm->search_range, by - job->mvp_y));
+    return bcost;
+}
+
 typedef struct {
     const xo_me_param_t *prm;
     const xo_pu_t *pus;

@@ -147,6 +147,32 @@ int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh
                        const uint8_t *plane0, intptr_t i_ref,
                        int sx, int sy, int scost, int p0x, int p0y, int *path, int max_pts, int *route);
 
+/* Symmetric bi-prediction refinement, xavs2_me_search_sym (me.c:1190-1262),
+ * for frame (not field) sequences.  The second MV mirrors the first,
+ * scaled by the distances (scale_mv_skip, predict.h:148-151); the prediction
+ * is the rounded average of the two references' planes. */
+typedef struct {
+    int pix_x, pix_y, w, h;           /* the PU */
+    int mv_x, mv_y;                   /* *mv on entry (qpel) */
+    int mvp_x, mvp_y;                 /* p_me->mvp (MV cost) */
+    int mvp1_x, mvp1_y;               /* p_me->mvp1 (PMVR centre) */
+    int dist_fwd, dist_bwd;           /* p_me->i_distance_1st / _2nd (> 0) */
+    int b_slice;                      /* h->i_type == SLICE_TYPE_B: the second MV is negated */
+} xo_sym_job_t;
+
+typedef struct {
+    int32_t cost;                     /* return value (XO_MAX_DISTORTION if no point was legal) */
+    int32_t mv_x, mv_y;               /* *mv on return (qpel) */
+    int32_t mvcost;                   /* p_me->mvcost[PDIR_SYM]; -1 where the search does not set it */
+} xo_sym_result_t;
+
+/* scale_mv_skip (predict.h:148-151). */
+int xo_scale_mv_skip(int mv, int dist_dst, int dist_src);
+int32_t xo_me_search_sym(const xo_me_param_t *prm, const xo_sym_job_t *job,
+                         const uint8_t *cur, intptr_t i_cur,
+                         const uint8_t *const planes1[16], const uint8_t *const planes2[16], intptr_t i_ref,
+                         xo_sym_result_t *res);
+
 /* The same over n PUs on nthreads POSIX threads (CPU baseline); ctxs holds
  * one UMH context per PU or is NULL. */
 void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,
