@@ -243,6 +243,46 @@ int xme_tune_get(int part, int search_range, int *k_rows, int *lanes_per_row);
  * LC is 8, 16, 32 or 64 lanes per candidate row. */
 int xme_tune_set(int part, int search_range, int k_rows, int lanes_per_row);
 
+/* Symmetric bi-prediction refinement, xavs2_me_search_sym
+ * (source/encoder/me.c:1190-1262), for a batch of PUs of one frame against one
+ * pair of references (ref1 = p_fref_1st, ref2 = p_fref_2nd; the reference
+ * addresses both with ref1's stride, so the strides must match).  Each job
+ * starts from mv (qpel) and evaluates ME_COST_QPEL_SYM (me.c:353-382): the
+ * rounded average of ref1 at mv and ref2 at the mirrored MV
+ * scale_mv_skip(mv, dist_bwd, dist_fwd) (source/common/predict.h:148-151,
+ * negated when b_slice), both range-checked against the PU's qpel window
+ * (md_inter.c:913-924), costed with SATD (use_hadamard) or SAD plus the MV
+ * rate against mvp.  Half-pel Spiral[0..4] (Spiral[1..4] with SAD), then
+ * quarter-pel Spiral[1..4] with PMVR centred on mvp1 when enable_pmvr.
+ * Frame sequences (b_field_sequence == 0, scale_mv_skip_y = scale_mv_skip).
+ * Uses frame_w/h, search_range (mvbits table), lambda_factor, min/max_mv_range,
+ * use_hadamard, use_fractional_me and enable_pmvr of prm.  d_cur must be
+ * 4-byte aligned with i_cur % 4 == 0. */
+typedef struct {
+    int16_t x, y;                    /* PU position (luma pixels, x % 4 == 0) */
+    uint8_t w, h;                    /* PU size, multiples of 4 up to 64 */
+    uint8_t reserved[2];
+    int16_t mv_x, mv_y;              /* start MV, qpel (*mv) */
+    int16_t mvp_x, mvp_y;            /* p_me->mvp */
+    int16_t mvp1_x, mvp1_y;          /* p_me->mvp1 (PMVR centre) */
+    int16_t dist_fwd, dist_bwd;      /* p_me->i_distance_1st, i_distance_2nd */
+    int32_t b_slice;                 /* h->i_type == SLICE_TYPE_B: mirror the scaled MV */
+} xme_sym_job;
+
+typedef struct {
+    int32_t cost;                    /* return value (XME_MAX_DISTORTION when no point had a cost) */
+    int16_t mv_x, mv_y;              /* *mv on return (qpel) */
+    int32_t mvcost;                  /* p_me->mvcost[PDIR_SYM]; -1 when use_fractional_me == 0 (not set) */
+    int32_t status;                  /* 0 ok; 2 job not evaluated: outside the frame, a distance of 0 or
+                                        beyond +-255, an MV beyond +-8191 qpel (also once scaled) or an
+                                        MV difference outside the mvbits table (the caller's
+                                        check_mv_range / check_mvd gate, md_inter.c:1177-1184) */
+} xme_sym_result;
+
+int xme_me_search_sym_batch(const xme_me_params *prm, const xme_sym_job *d_jobs, int n, const pel_t *d_cur,
+                            intptr_t i_cur, const xme_ref_desc *ref1, const xme_ref_desc *ref2,
+                            xme_sym_result *d_results, void *stream);
+
 /* ---- 4. reference planes, MC and the residual path ---------------------- */
 /* plane_expand_border over a whole plane (source/common/frame.c:470-507). */
 int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, int pad, void *stream);

@@ -36,7 +36,7 @@ def test_header_declares_the_boundary():
                  "xme_me_run", "xme_me_run_graph", "xme_interpolate_frame", "xme_expand_border",
                  "xme_mc_luma_batch", "xme_mc_chroma_batch", "xme_dct_batch", "xme_idct_batch", "xme_quant_batch",
                  "xme_dequant_batch", "xme_residual_batch", "xme_server_create", "xme_server_submit",
-                 "xme_server_wait"):
+                 "xme_server_wait", "xme_me_search_sym_batch"):
         assert must in names
 
 
@@ -64,3 +64,45 @@ def test_no_cpu_fallback_in_the_product():
     assert "xo_" not in out
     deps = subprocess.run(["ldd", LIB], capture_output=True, text=True).stdout
     assert "liboracle" not in deps
+
+
+LAYOUTS = {
+    "xme_pu": "PU_DTYPE", "xme_result": "RESULT_DTYPE", "xme_umh_ctx": "UMH_CTX_DTYPE",
+    "xme_sym_job": "SYM_JOB_DTYPE", "xme_sym_result": "SYM_RESULT_DTYPE",
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof / offsetof of the header's records, compiled with gcc, against the
+    numpy dtypes and ctypes structures that mirror them."""
+    from xavs2_amd import lib as X
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "xavs2_amd.h"', 'int main(void) {']
+    for cname, dname in LAYOUTS.items():
+        dt = getattr(X, dname)
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        # numpy field names mirror the header's, except the padding bytes
+        for f in dt.names:
+            if f in ("r0", "r1"):
+                continue
+            src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    for cname, sname in (("xme_me_params", "MeParams"), ("xme_server_config", "ServerConfig"),
+                         ("xme_ref_desc", "RefDesc")):
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(c), "-o", str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                               check=True).stdout.splitlines())
+    for cname, dname in LAYOUTS.items():
+        dt = getattr(X, dname)
+        assert int(got[cname]) == dt.itemsize, cname
+        for f in dt.names:
+            if f not in ("r0", "r1"):
+                assert int(got[f"{cname}.{f}"]) == dt.fields[f][1], (cname, f)
+    import ctypes
+    for cname, sname in (("xme_me_params", "MeParams"), ("xme_server_config", "ServerConfig"),
+                         ("xme_ref_desc", "RefDesc")):
+        assert int(got[cname]) == ctypes.sizeof(getattr(X, sname)), cname

@@ -1,0 +1,370 @@
+// Symmetric bi-prediction refinement (xavs2_me_search_sym,
+// source/encoder/me.c:1190-1262) for batches of PUs, gfx950.
+//
+// One wave per job.  A stage's candidates (the half-pel Spiral points around
+// the start MV, then the quarter-pel points around the half-pel best) are
+// independent: groups of P lanes take one candidate each, a lane a 4x4 tile
+// of the PU (more than one when the PU has more than 64 tiles).  A lane
+// forms the prediction of its tile as the rounded average of the two
+// references' planes at the candidate MV and its mirrored, distance-scaled
+// MV, then takes the tile's Hadamard sum (or SAD) against the current PU.
+// The group's last lane holds the candidate's cost; the wave then applies the
+// reference's sequential strict-'<' update in Spiral order from uniform
+// registers.
+#include "xme_dpp.h"
+#include "xme_satd.h"
+#include "xme_internal.h"
+
+namespace {
+
+__constant__ int8_t c_sym_spiral[5][2] = { { 0, 0 }, { 0, -1 }, { 0, 1 }, { -1, -1 }, { 1, -1 } };
+
+typedef const __attribute__((address_space(1))) uint8_t *gbytes;
+typedef const __attribute__((address_space(1))) uint32_t *gdwords;
+
+struct SymPlanes {
+    const uint8_t *p1[16];
+    const uint8_t *p2[16];
+};
+
+struct SymParams {
+    int frame_w, frame_h;
+    int min_mv[2], max_mv[2];
+    int lambda, hadamard, frac, pmvr, max_mvd;
+};
+
+constexpr int SYM_WAVES = 4;
+constexpr int SYM_MAX_MV = 8191;     // |MV| bound of a valid job (qpel), see sym_job_ok
+
+// Four bytes at any address; the dword after the aligned one is read, which
+// the XME_PAD columns of right padding keep inside the plane.
+__device__ __forceinline__ uint32_t sym_load_dw(gbytes p)
+{
+    const uintptr_t a = (uintptr_t)p;
+    gdwords q = (gdwords)(a & ~(uintptr_t)3);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
+}
+
+// (a + b + 1) >> 1 on each of four bytes (g_funcs.pixf.avg, pixel.c:1007-1019).
+__device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b)
+{
+    return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu);
+}
+
+// Plane k of 16 without dynamic indexing of the kernel-argument array (a
+// select chain over uniform registers).
+__device__ __forceinline__ const uint8_t *pick_plane(const uint8_t *const (&pl)[16], int k)
+{
+    const uint8_t *r = pl[0];
+#pragma unroll
+    for (int q = 1; q < 16; q++)
+        r = k == q ? pl[q] : r;
+    return r;
+}
+
+// scale_mv_skip (source/common/predict.h:148-151).
+__device__ __forceinline__ int scale_mv(int mv, int dist_dst, int dist_src)
+{
+    return (int16_t)((mv * dist_dst * (16384 / dist_src) + 8192) >> 14);
+}
+
+__device__ __forceinline__ int64_t scale_mv_wide(int mv, int dist_dst, int dist_src)
+{
+    return ((int64_t)mv * dist_dst * (16384 / dist_src) + 8192) >> 14;
+}
+
+// The qpel MV window of the job (md_inter.c:913-924) packed as in
+// xavs2_me_search_sym (me.c:1207-1208), with the plain limits beside it.
+struct SymWindow {
+    int lo[2], hi[2];
+    uint32_t mv_min, mv_max;
+};
+
+__device__ __forceinline__ bool sym_legal(const SymWindow &w, int mx, int my)
+{
+    // CHECK_MV_RANGE (me.h:46-48) as the reference evaluates it; the plain
+    // compare keeps every read inside the padded planes for any input
+    const uint32_t v = xme_pack2(mx, my);
+    const bool packed = !(((v + w.mv_min) | (w.mv_max - v)) & 0x80004000u);
+    return packed && mx >= w.lo[0] && mx <= w.hi[0] && my >= w.lo[1] && my <= w.hi[1];
+}
+
+// A job the kernel evaluates: a PU of 4x4 tiles inside the frame at a column
+// that is a multiple of 4, distances that keep scale_mv_skip inside int, and
+// MVs (both the candidates within 4 qpel of the start and their mirrors)
+// within +-SYM_MAX_MV, whose differences to the MVP stay inside the mvbits
+// table.  The reference's caller gates the search the same way
+// (check_mv_range / check_mvd, md_inter.c:1177-1184).
+__device__ __forceinline__ bool sym_job_ok(const SymParams &p, const xme_sym_job &j)
+{
+    if ((j.w & 3) || (j.h & 3) || j.w < 4 || j.h < 4 || j.w > 64 || j.h > 64)
+        return false;
+    if ((j.x & 3) || j.x < 0 || j.y < 0 || j.x + j.w > p.frame_w || j.y + j.h > p.frame_h)
+        return false;
+    if (j.dist_fwd == 0 || j.dist_fwd > 255 || j.dist_fwd < -255 || j.dist_bwd > 255 || j.dist_bwd < -255)
+        return false;
+    const int mv[2] = { j.mv_x, j.mv_y }, mvp[2] = { j.mvp_x, j.mvp_y };
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+#pragma unroll
+        for (int e = -4; e <= 4; e += 8) {
+            const int v = mv[a] + e;
+            if (v > SYM_MAX_MV || v < -SYM_MAX_MV)
+                return false;
+            if ((int64_t)(v < 0 ? -v : v) * (j.dist_bwd < 0 ? -j.dist_bwd : j.dist_bwd) *
+                    (16384 / (j.dist_fwd < 0 ? -j.dist_fwd : j.dist_fwd)) + 8192 > 0x7fffffffLL)
+                return false;
+            const int64_t s = scale_mv_wide(v, j.dist_bwd, j.dist_fwd);
+            if (s > SYM_MAX_MV || s < -SYM_MAX_MV)
+                return false;
+            const int d = v - mvp[a];
+            if (d > p.max_mvd || d < -p.max_mvd)
+                return false;
+        }
+    }
+    return true;
+}
+
+struct SymJob {
+    gbytes cur;                // PU origin in the current picture
+    int64_t i_cur, i_ref, off; // off: PU offset in the reference planes
+    int w, h, tw, ntiles;
+    int mvp_x, mvp_y, dd, ds;
+    bool neg, hadamard;
+    int lambda;
+    SymWindow win;
+};
+
+// ME_COST_QPEL_SYM (me.c:353-382) of candidate (mx, my), summed over the
+// lane's tiles t0, t0 + P, ...; returns -1 when the candidate has no cost
+// (an MV outside the window or a plane that is not built).
+__device__ __forceinline__ int sym_tiles(const SymJob &c, const SymPlanes &pl, int mx, int my, int t0, int P)
+{
+    int sx = scale_mv(mx, c.dd, c.ds), sy = scale_mv(my, c.dd, c.ds);
+    if (c.neg) {
+        sx = -sx;
+        sy = -sy;
+    }
+    if (!sym_legal(c.win, mx, my) || !sym_legal(c.win, sx, sy))
+        return -1;
+    const uint8_t *a = pick_plane(pl.p1, ((my & 3) << 2) + (mx & 3));
+    const uint8_t *b = pick_plane(pl.p2, ((sy & 3) << 2) + (sx & 3));
+    if (a == nullptr || b == nullptr)
+        return -1;
+    const gbytes ga = (gbytes)a + c.off + (int64_t)(my >> 2) * c.i_ref + (mx >> 2);
+    const gbytes gb = (gbytes)b + c.off + (int64_t)(sy >> 2) * c.i_ref + (sx >> 2);
+    int s = 0;
+    for (int t = t0; t < c.ntiles; t += P) {
+        const int ty = (t / c.tw) * 4, tx = (t % c.tw) * 4;
+        uint32_t cr[4], rr[4];
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            cr[y] = *(gdwords)(c.cur + (ty + y) * c.i_cur + tx);
+            const int64_t o = (int64_t)(ty + y) * c.i_ref + tx;
+            rr[y] = avg4(sym_load_dw(ga + o), sym_load_dw(gb + o));
+        }
+        s += xme_tile_cost_rows(cr, rr, c.hadamard);
+    }
+    return s;
+}
+
+// Candidate k of a stage (quarter-pel units).  Stage 0: the start point
+// alone (frac 0), else Spiral[pos0 + k] * 2 around the start (me.c:1227-1238).
+// Stage 1: Spiral[1 + k] around the half-pel best, with the PMVR adaptation
+// (pmvr_adapt_mv, me.c:412-424) when enabled; *skip marks a point the
+// reference does not evaluate.
+__device__ __forceinline__ void sym_cand(int stage, int pos0, int cx, int cy, bool pmvr, int ctr_x, int ctr_y, int k,
+                                         int &mx, int &my, bool &skip)
+{
+    skip = false;
+    if (stage == 0) {
+        mx = cx + 2 * c_sym_spiral[pos0 + k][0];
+        my = cy + 2 * c_sym_spiral[pos0 + k][1];
+        return;
+    }
+    const int dx = c_sym_spiral[1 + k][0], dy = c_sym_spiral[1 + k][1];
+    if (pmvr) {
+        const int ax = cx - ctr_x, ay = cy - ctr_y;
+        if (ax > 2 || ax < -2 || ay > 2 || ay < -2) {
+            mx = cx + 2 * dx;
+            my = cy + 2 * dy;
+            const int bx = mx - ctr_x, by = my - ctr_y;
+            skip = bx <= 2 && bx >= -2 && by <= 2 && by >= -2;
+        } else {
+            mx = cx + dx;
+            my = cy + dy;
+            const int bx = mx - ctr_x, by = my - ctr_y;
+            skip = bx > 2 || bx < -2 || by > 2 || by < -2;
+        }
+        return;
+    }
+    mx = cx + dx;
+    my = cy + dy;
+}
+
+// Evaluates the stage's n <= 5 candidates and applies the strict-'<' update
+// to (bcost, bx, by) in candidate order.
+__device__ __forceinline__ void sym_stage(const SymJob &c, const SymPlanes &pl, int stage, int pos0, int n, int cx,
+                                          int cy, bool pmvr, int ctr_x, int ctr_y, int P, int &bcost, int &bx, int &by)
+{
+    const int lane = threadIdx.x & 63;
+    const int groups = 64 / P, g = lane / P, t0 = lane % P;
+    for (int k0 = 0; k0 < n; k0 += groups) {
+        const int k = k0 + g;
+        int mx = cx, my = cy;
+        bool skip = true;
+        if (k < n)
+            sym_cand(stage, pos0, cx, cy, pmvr, ctr_x, ctr_y, k, mx, my, skip);
+        int s = skip ? -1 : sym_tiles(c, pl, mx, my, t0, P);
+        const bool none = s < 0;         // the same in every lane of the group
+        s = group_sum(none ? 0 : s, P);
+        int cost = XME_MAX_DISTORTION;
+        if (!none)
+            cost = (c.hadamard ? (s >> 1) : s) +
+                   xme_weighted_cost(c.lambda, xme_mvbits(mx - c.mvp_x) + xme_mvbits(my - c.mvp_y));
+        const int ng = min(groups, n - k0);
+        for (int gi = 0; gi < ng; gi++) {        // uniform: the reference's candidate order
+            const int src = gi * P + P - 1;
+            const int cc = __builtin_amdgcn_readlane(cost, src);
+            if (cc < bcost) {
+                bcost = cc;
+                bx = __builtin_amdgcn_readlane(mx, src);
+                by = __builtin_amdgcn_readlane(my, src);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int64_t i_ref, const uint8_t *cur,
+                                                  int64_t i_cur, const xme_sym_job *__restrict__ jobs, int n,
+                                                  xme_sym_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const int lane = threadIdx.x & 63;
+    const xme_sym_job j = jobs[ji];
+    xme_sym_result res;
+    if (!sym_job_ok(p, j)) {
+        if (lane == 0) {
+            res.cost = XME_MAX_DISTORTION;
+            res.mv_x = j.mv_x;
+            res.mv_y = j.mv_y;
+            res.mvcost = -1;
+            res.status = 2;
+            out[ji] = res;
+        }
+        return;
+    }
+    SymJob c;
+    c.cur = (gbytes)cur + (int64_t)j.y * i_cur + j.x;
+    c.i_cur = i_cur;
+    c.i_ref = i_ref;
+    c.off = (int64_t)j.y * i_ref + j.x;
+    c.w = j.w;
+    c.h = j.h;
+    c.tw = j.w >> 2;
+    c.ntiles = c.tw * (j.h >> 2);
+    c.mvp_x = j.mvp_x;
+    c.mvp_y = j.mvp_y;
+    c.dd = j.dist_bwd;
+    c.ds = j.dist_fwd;
+    c.neg = j.b_slice != 0;
+    c.hadamard = p.hadamard != 0;
+    c.lambda = p.lambda;
+    {
+        // p_me->mv_min / mv_max (md_inter.c:913-924)
+        const int lo_x = xme_clip3(p.min_mv[0], p.max_mv[0], (-64 - j.x) * 4);
+        const int hi_x = xme_clip3(p.min_mv[0], p.max_mv[0], (p.frame_w + 64 - j.x - j.w) * 4);
+        const int lo_y = xme_clip3(p.min_mv[1], p.max_mv[1], (-64 - j.y) * 4);
+        const int hi_y = xme_clip3(p.min_mv[1], p.max_mv[1], (p.frame_h + 64 - j.y - j.h) * 4);
+        // the plain limits also stay within 64 pixels of the picture, where
+        // the XME_PAD border holds every read, whatever min/max_mv_range say
+        c.win.lo[0] = max(lo_x, (-64 - j.x) * 4);
+        c.win.hi[0] = min(hi_x, (p.frame_w + 64 - j.x - j.w) * 4);
+        c.win.lo[1] = max(lo_y, (-64 - j.y) * 4);
+        c.win.hi[1] = min(hi_y, (p.frame_h + 64 - j.y - j.h) * 4);
+        c.win.mv_min = xme_pack2(-lo_x, -lo_y);
+        c.win.mv_max = xme_pack2(hi_x, hi_y) | 0x8000u;
+    }
+    int P = 1;
+    while (P < c.ntiles && P < 64)
+        P <<= 1;
+    int bcost = XME_MAX_DISTORTION, bx = j.mv_x, by = j.mv_y;
+    if (p.frac == 0) {
+        // the start point alone; its cost is returned even when there is none
+        // (me.c:1219-1226)
+        sym_stage(c, pl, 0, 0, 1, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
+        if (lane == 0) {
+            res.cost = bcost;
+            res.mv_x = j.mv_x;
+            res.mv_y = j.mv_y;
+            res.mvcost = -1;
+            res.status = 0;
+            out[ji] = res;
+        }
+        return;
+    }
+    // half-pel: Spiral[0..4] with SATD, Spiral[1..4] with SAD (me.c:1210-1211)
+    const int pos0 = c.hadamard ? 0 : 1;
+    sym_stage(c, pl, 0, pos0, 5 - pos0, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
+    if (p.frac >= 2) {
+        const int ctr_x = (j.mvp1_x >> 1) << 1, ctr_y = (j.mvp1_y >> 1) << 1;
+        sym_stage(c, pl, 1, 1, 4, bx, by, p.pmvr != 0, ctr_x, ctr_y, P, bcost, bx, by);
+    }
+    if (lane == 0) {
+        res.cost = bcost;
+        res.mv_x = (int16_t)bx;
+        res.mv_y = (int16_t)by;
+        res.mvcost = xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp_x) + xme_mvbits(by - j.mvp_y));
+        res.status = 0;
+        out[ji] = res;
+    }
+}
+
+} // namespace
+
+extern "C" int xme_me_search_sym_batch(const xme_me_params *prm, const xme_sym_job *d_jobs, int n,
+                                       const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref1,
+                                       const xme_ref_desc *ref2, xme_sym_result *d_results, void *stream)
+{
+    if (!prm || n < 0 || !ref1 || !ref2 || prm->use_fractional_me < 0 || prm->use_fractional_me > 2 ||
+        prm->frame_w <= 0 || prm->frame_h <= 0 || prm->search_range <= 0) {
+        xme_set_error("xme_me_search_sym_batch: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (n == 0)
+        return XME_OK;
+    if (!d_jobs || !d_cur || !d_results || !ref1->planes[0] || !ref2->planes[0]) {
+        xme_set_error("xme_me_search_sym_batch: NULL buffer");
+        return XME_ERR_ARG;
+    }
+    // the reference addresses both references with the first one's stride
+    // (me.c:1218); the current rows are read as aligned dwords
+    if (ref1->stride != ref2->stride || ((uintptr_t)d_cur & 3) || (i_cur & 3)) {
+        xme_set_error("xme_me_search_sym_batch: reference strides differ or the current plane is not 4-byte aligned");
+        return XME_ERR_ARG;
+    }
+    SymParams p;
+    p.frame_w = prm->frame_w;
+    p.frame_h = prm->frame_h;
+    p.min_mv[0] = prm->min_mv_range[0];
+    p.min_mv[1] = prm->min_mv_range[1];
+    p.max_mv[0] = prm->max_mv_range[0];
+    p.max_mv[1] = prm->max_mv_range[1];
+    p.lambda = prm->lambda_factor;
+    p.hadamard = prm->use_hadamard ? 1 : 0;
+    p.frac = prm->use_fractional_me;
+    p.pmvr = prm->enable_pmvr ? 1 : 0;
+    p.max_mvd = xme_mvbits_max_mvd(prm->search_range);
+    SymPlanes pl;
+    for (int k = 0; k < 16; k++) {
+        pl.p1[k] = ref1->planes[k];
+        pl.p2[k] = ref2->planes[k];
+    }
+    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    hipLaunchKernelGGL(sym_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, pl,
+                       (int64_t)ref1->stride, d_cur, (int64_t)i_cur, d_jobs, n, d_results);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}

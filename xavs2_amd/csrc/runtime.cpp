@@ -149,7 +149,7 @@ bool xme_narrow_shape(int part) { return kPuW[part] * kPuH[part] <= 256; }
 int xme_pu_h(int part) { return kPuH[part]; }
 
 // Largest |v| of the reference's mvbits table (me.c:592-612).
-static int mvbits_max_mvd(int R)
+int xme_mvbits_max_mvd(int R)
 {
     int me_range = R > 256 ? R : 256;
     int subpel_num = 4 * (2 * me_range + 3);
@@ -495,7 +495,7 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     p.n_pu = n_pu;
     p.n_ref = n_ref;
 
-    const int max_mvd = mvbits_max_mvd(p.R);
+    const int max_mvd = xme_mvbits_max_mvd(p.R);
     std::vector<DevPu> pus(n_pu);
     std::vector<int> parts(n_pu);
     for (int i = 0; i < n_pu; i++) {

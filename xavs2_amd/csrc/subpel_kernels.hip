@@ -9,6 +9,7 @@
 // the candidate's LDS counter; the stage then applies the reference's
 // sequential strict-'<' update in Spiral order.
 #include "xme_dpp.h"
+#include "xme_satd.h"
 #include "xme_internal.h"
 
 namespace {
@@ -34,47 +35,15 @@ __device__ __forceinline__ uint32_t load_unaligned_dw(gbytes p)
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
-__device__ __forceinline__ int bytei(uint32_t v, int i) { return (int)((v >> (8 * i)) & 0xFFu); }
-
 // Sum of |H4 D H4^T| over one 4x4 tile (before the final halving), or the SAD,
-// of current rows c[0..3] against the reference tile at `ref`.  32-bit
-// arithmetic throughout: packed 16-bit integer forms are avoided on this target.
+// of current rows c[0..3] against the reference tile at `ref`.
 __device__ __forceinline__ int tile_cost(const uint32_t (&cr)[4], gbytes ref, int64_t i_ref, bool hadamard)
 {
     uint32_t rr[4];
 #pragma unroll
     for (int y = 0; y < 4; y++)
         rr[y] = load_unaligned_dw(ref + y * i_ref);
-    if (!hadamard) {
-        int s = 0;
-#pragma unroll
-        for (int y = 0; y < 4; y++)
-            s = (int)__builtin_amdgcn_sad_u8(cr[y], rr[y], (uint32_t)s);
-        return s;
-    }
-    int d[4][4];
-#pragma unroll
-    for (int y = 0; y < 4; y++)
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-            d[y][x] = bytei(cr[y], x) - bytei(rr[y], x);
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-        const int a = d[y][0] + d[y][1], b = d[y][0] - d[y][1];
-        const int c = d[y][2] + d[y][3], e = d[y][2] - d[y][3];
-        d[y][0] = a + c;
-        d[y][1] = b + e;
-        d[y][2] = a - c;
-        d[y][3] = b - e;
-    }
-    int s = 0;
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-        const int a = d[0][x] + d[1][x], b = d[0][x] - d[1][x];
-        const int c = d[2][x] + d[3][x], e = d[2][x] - d[3][x];
-        s += abs(a + c) + abs(b + e) + abs(a - c) + abs(b - e);
-    }
-    return s;
+    return xme_tile_cost_rows(cr, rr, hadamard);
 }
 
 struct RefineCtx {

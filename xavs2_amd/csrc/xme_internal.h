@@ -103,6 +103,10 @@ __device__ inline int xme_mvbits(int v)
     return a == 0 ? 1 : 2 * (31 - __clz(a)) + 3;
 }
 
+// Largest |v| of the reference's mvbits table for search range R
+// (me.c:592-612).
+int xme_mvbits_max_mvd(int R);
+
 // WEIGHTED_COST (common/common.h:87) with the reference's int arithmetic.
 __host__ __device__ inline int xme_weighted_cost(int lambda, int bits)
 {

@@ -49,6 +49,15 @@ RESULT_DTYPE = np.dtype([("cost", "<i4"), ("bcost2", "<i4"), ("bmv_x", "<i2"), (
                          ("bmv2_x", "<i2"), ("bmv2_y", "<i2"), ("mvcost", "<i4"), ("status", "<i4")])
 
 
+# xme_sym_job / xme_sym_result: the symmetric bi-prediction batch
+SYM_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
+                          ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvp_x", "<i2"), ("mvp_y", "<i2"),
+                          ("mvp1_x", "<i2"), ("mvp1_y", "<i2"), ("dist_fwd", "<i2"), ("dist_bwd", "<i2"),
+                          ("b_slice", "<i4")])
+SYM_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvcost", "<i4"),
+                             ("status", "<i4")])
+
+
 class RefDesc(C.Structure):
     _fields_ = [("planes", C.c_void_p * 16), ("stride", C.c_ssize_t)]
 
@@ -113,6 +122,8 @@ def lib():
         L.xme_sad_x4_batch.argtypes = [I, P, P, Z, P, P, I, P, P]
         L.xme_mc_chroma_batch.argtypes = [P, P, Z, P, I, I, I, P, P, Z, P, P]
         L.xme_mc_luma_batch.argtypes = [P, P, I, I, I, P, Z, P, P]
+        L.xme_me_search_sym_batch.argtypes = [C.POINTER(MeParams), P, I, P, Z, C.POINTER(RefDesc),
+                                              C.POINTER(RefDesc), P, P]
         L.xme_dct_batch.argtypes = [I, P, P, I, P]
         L.xme_idct_batch.argtypes = [I, P, P, I, P]
         L.xme_quant_batch.argtypes = [I, P, I, I, I, P, P]
@@ -262,6 +273,22 @@ def me_params(frame_w, frame_h, search_range, lambda_factor, use_hadamard=1, use
     p.use_hadamard, p.use_fractional_me = use_hadamard, use_fractional_me
     p.use_fast_sub_me, p.enable_pmvr = use_fast_sub_me, enable_pmvr
     return p
+
+
+def me_search_sym_batch(params, jobs, cur, ref1, ref2, stream=None):
+    """xme_me_search_sym_batch on host job records (SYM_JOB_DTYPE): `cur` is a
+    DevicePlane holding the current picture, ref1 / ref2 DeviceRefs of the same
+    stride.  Returns the SYM_RESULT_DTYPE records (blocking)."""
+    jobs = np.ascontiguousarray(jobs, dtype=SYM_JOB_DTYPE)
+    n = len(jobs)
+    d_jobs = upload(jobs) if n else DeviceBuffer(1)
+    d_res = DeviceBuffer(max(n, 1) * SYM_RESULT_DTYPE.itemsize)
+    r1, r2 = ref1.desc(), ref2.desc()
+    check(lib().xme_me_search_sym_batch(C.byref(params), C.c_void_p(d_jobs.ptr), n, C.c_void_p(cur.origin),
+                                        cur.stride, C.byref(r1), C.byref(r2), C.c_void_p(d_res.ptr), stream),
+          "xme_me_search_sym_batch")
+    check(lib().xme_stream_sync(stream), "xme_stream_sync")
+    return d_res.download(n, SYM_RESULT_DTYPE)
 
 
 def make_pus(tuples):
