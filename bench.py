@@ -207,6 +207,76 @@ def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, st
     return out
 
 
+def sym_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu):
+    """Symmetric bi-prediction refinement (xme_me_search_sym_batch) of every PU
+    between references 0 and 1, as the encoder's dual-hypothesis search runs it
+    (md_inter.c:1165-1186): start MV = 4 x the PU's full-pel best against
+    reference 0 (all_best_imv), F-frame distances 2 and 4 (the second MV is
+    twice the first), MVP (0,0), SATD, half- then quarter-pel (9 points of
+    w x h pixels per PU).  Kernel time from back-to-back launches on one
+    stream; parity and the CPU time on every 4th PU."""
+    import ctypes as C
+    from oracle import oracle as O
+    r0 = gpu_res[0::n_ref]
+    arr = np.array(pus, dtype=np.int64)
+    jobs = np.zeros(len(pus), dtype=X.SYM_JOB_DTYPE)
+    jobs["x"], jobs["y"], jobs["w"], jobs["h"] = arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3]
+    jobs["mv_x"], jobs["mv_y"] = 4 * r0["bmv2_x"], 4 * r0["bmv2_y"]
+    jobs["dist_fwd"], jobs["dist_bwd"], jobs["b_slice"] = 2, 4, 0
+    prm = X.me_params(w, h, R, lam)
+    d_jobs = X.upload(jobs)
+    d_res = X.DeviceBuffer(len(jobs) * X.SYM_RESULT_DTYPE.itemsize)
+    r1, r2 = d_refs[0].desc(), d_refs[1].desc()
+    stream = X.Stream()
+
+    def launch():
+        X.check(X.lib().xme_me_search_sym_batch(C.byref(prm), C.c_void_p(d_jobs.ptr), len(jobs),
+                                                C.c_void_p(dcur.origin), dcur.stride, C.byref(r1), C.byref(r2),
+                                                C.c_void_p(d_res.ptr), stream.h), "xme_me_search_sym_batch")
+
+    for _ in range(3):
+        launch()
+    stream.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        launch()
+    stream.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    res = d_res.download(len(jobs), X.SYM_RESULT_DTYPE)
+    pix = int((arr[:, 2] * arr[:, 3]).sum()) * 9
+    leg = {
+        "workload": f"{len(jobs)} PUs of the frame, references 0 and 1, start = 4 x full-pel best of reference 0, "
+                    "distances 2 / 4 (F frame), SATD, quarter-pel",
+        "ms_per_batch": ms,
+        "pus_per_s": len(jobs) / (ms * 1e-3),
+        "candidate_pixels_per_s": pix / (ms * 1e-3),
+        "status_ok": bool(np.all(res["status"] == 0)),
+    }
+    if cpu:
+        planes = []
+        for r in refs[:2]:
+            p = O.Plane(w, h)
+            p.set_picture(r)
+            p.expand_border()
+            planes.append(O.interp_planes(p, 2))
+        cp = O.Plane(w, h)
+        cp.set_picture(cur)
+        prm_o = O.me_param(w, h, R, lam)
+        idx = range(0, len(jobs), 4)
+        t0 = time.perf_counter()
+        exp = [O.me_search_sym(prm_o, dict(pix_x=int(jobs["x"][i]), pix_y=int(jobs["y"][i]), w=int(jobs["w"][i]),
+                                           h=int(jobs["h"][i]), mv_x=int(jobs["mv_x"][i]), mv_y=int(jobs["mv_y"][i]),
+                                           mvp_x=0, mvp_y=0, mvp1_x=0, mvp1_y=0, dist_fwd=2, dist_bwd=4, b_slice=0),
+                               cp, planes[0], planes[1]) for i in idx]
+        el = time.perf_counter() - t0
+        got = [(int(res["cost"][i]), int(res["mv_x"][i]), int(res["mv_y"][i]), int(res["mvcost"][i])) for i in idx]
+        leg["parity_with_oracle_every_4th"] = got == [tuple(int(v) for v in e) for e in exp]
+        leg["cpu_oracle"] = {"pus_per_s": len(exp) / el, "threads": 1, "seconds": el,
+                             "sample": f"every 4th PU ({len(exp)}), one thread, Python loop over the C oracle"}
+    log(f"sym: {ms:.3f} ms per batch of {len(jobs)} PUs")
+    return leg
+
+
 def transform_leg(X, blocks_total, reps, seed=91):
     """DCT -> flat quant -> dequant -> IDCT over blocks_total int16 blocks, a
     quarter each of 4x4, 8x8, 16x16 and 32x32 (Laplacian b=6 residuals clipped
@@ -416,6 +486,7 @@ def main():
                          "leaves CUs for frames that arrive while a large one searches")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
     ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX / UMH pattern-search leg")
+    ap.add_argument("--no-sym", action="store_true", help="skip the symmetric bi-prediction leg")
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
@@ -522,6 +593,11 @@ def main():
         log("pattern-search leg (DIA, HEX, UMH)")
         pattern = pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, args.steps,
                               not args.no_cpu)
+    sym = None
+    if not args.no_sym and n_ref >= 2:
+        log("symmetric bi-prediction leg")
+        sym = sym_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, max(20, args.steps),
+                      not args.no_cpu)
     transform = None
     if not args.no_transform:
         log(f"residual transform leg: {args.transform_blocks} blocks")
@@ -574,6 +650,7 @@ def main():
         "results_stable_across_replays": stable,
         "cpu_baseline": cpu,
         "pattern_search": pattern,
+        "sym_bipred": sym,
         "transform": transform,
         "serving": serving,
     }
