@@ -44,6 +44,11 @@ class SymResult(C.Structure):
     _fields_ = [("cost", C.c_int32), ("mv_x", C.c_int32), ("mv_y", C.c_int32), ("mvcost", C.c_int32)]
 
 
+class BidJob(C.Structure):
+    _fields_ = [(f, C.c_int) for f in ("pix_x", "pix_y", "w", "h", "fwd_x", "fwd_y", "bwd_x", "bwd_y",
+                                       "mvp1_x", "mvp1_y", "mvp2_x", "mvp2_y")]
+
+
 class Pu(C.Structure):
     _fields_ = [("pix_x", C.c_int), ("pix_y", C.c_int), ("w", C.c_int), ("h", C.c_int),
                 ("mvp_x", C.c_int), ("mvp_y", C.c_int)]
@@ -91,6 +96,8 @@ def lib():
         L.xo_scale_mv_skip.argtypes = [C.c_int, C.c_int, C.c_int]
         L.xo_me_search_sym.restype = C.c_int32
         L.xo_me_search_sym.argtypes = [C.POINTER(MeParam), C.POINTER(SymJob), P, ip, P, P, ip, C.POINTER(SymResult)]
+        L.xo_me_search_bid.restype = C.c_int32
+        L.xo_me_search_bid.argtypes = [C.POINTER(MeParam), C.POINTER(BidJob), P, ip, P, P, ip, C.POINTER(SymResult)]
         L.xo_me_search_ctx.restype = C.c_int32
         L.xo_me_search_ctx.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, P]
         L.xo_me_search_batch_ctx.argtypes = [C.POINTER(MeParam), P, P, C.c_int, P, ip, P, ip, P, C.c_int]
@@ -244,6 +251,16 @@ def me_search_sym(prm, job, cur_plane, planes1, planes2):
     j = SymJob(**job)
     r = SymResult()
     lib().xo_me_search_sym(C.byref(prm), C.byref(j), cur_plane.origin_ptr(), cur_plane.stride,
+                           plane_ptrs(planes1), plane_ptrs(planes2), planes1[0].stride, C.byref(r))
+    return r.cost, r.mv_x, r.mv_y, r.mvcost
+
+
+def me_search_bid(prm, job, cur_plane, planes1, planes2):
+    """xavs2_me_search_bid for one job: a dict with the BidJob fields.
+    Returns (cost, fwd_x, fwd_y, mvcost)."""
+    j = BidJob(**job)
+    r = SymResult()
+    lib().xo_me_search_bid(C.byref(prm), C.byref(j), cur_plane.origin_ptr(), cur_plane.stride,
                            plane_ptrs(planes1), plane_ptrs(planes2), planes1[0].stride, C.byref(r))
     return r.cost, r.mv_x, r.mv_y, r.mvcost
 

@@ -799,6 +799,107 @@ int32_t xo_me_search_sym(const xo_me_param_t *prm, const xo_sym_job_t *job,
     return bcost;
 }
 
+/* The qpel window of md_inter.c:913-924 (p_me->mv_min / mv_max), packed as
+ * the refinement functions pack it (me.c:1207-1208, :1300-1301). */
+static mv_window_t qpel_window(const xo_me_param_t *prm, int pix_x, int pix_y, int w, int h)
+{
+    mv_window_t qw;
+    int lo = clip3(prm->min_mv_range[0], prm->max_mv_range[0], (-64 - pix_x) * 4);
+    int hi = clip3(prm->min_mv_range[0], prm->max_mv_range[0], (prm->frame_w + 64 - pix_x - w) * 4);
+    int lo_y = clip3(prm->min_mv_range[1], prm->max_mv_range[1], (-64 - pix_y) * 4);
+    int hi_y = clip3(prm->min_mv_range[1], prm->max_mv_range[1], (prm->frame_h + 64 - pix_y - h) * 4);
+    qw.mv_min = pack_mask2(-lo, -lo_y);
+    qw.mv_max = pack_mask2(hi, hi_y) | 0x8000u;
+    return qw;
+}
+
+/* ME_COST_QPEL_BID (me.c:388-398) of forward candidate (mx, my). */
+static int bid_cost(const xo_me_param_t *prm, const xo_bid_job_t *j, const mv_window_t *qw,
+                    const uint8_t *target, const uint8_t *const p1[16], intptr_t i_ref, int mx, int my, int bid_bits)
+{
+    if (!check_mv_range(mx, my, qw->mv_min, qw->mv_max) || !check_mv_range(j->bwd_x, j->bwd_y, qw->mv_min, qw->mv_max))
+        return XO_MAX_DISTORTION;
+    const uint8_t *a = p1[((my & 3) << 2) + (mx & 3)];
+    if (!a)
+        return XO_MAX_DISTORTION;
+    a += (intptr_t)j->pix_y * i_ref + j->pix_x + (intptr_t)(my >> 2) * i_ref + (mx >> 2);
+    const int d = (prm->use_hadamard ? xo_satd : xo_sad)(j->w, j->h, target, 64, a, i_ref) >> 1;
+    return d + weighted_cost(prm->lambda_factor, xo_mvbits(prm->search_range, mx - j->mvp1_x) +
+                                                  xo_mvbits(prm->search_range, my - j->mvp1_y)) + bid_bits;
+}
+
+int32_t xo_me_search_bid(const xo_me_param_t *prm, const xo_bid_job_t *job,
+                         const uint8_t *cur, intptr_t i_cur,
+                         const uint8_t *const planes1[16], const uint8_t *const planes2[16], intptr_t i_ref,
+                         xo_sym_result_t *res)
+{
+    const mv_window_t qw = qpel_window(prm, job->pix_x, job->pix_y, job->w, job->h);
+    const uint8_t *org = cur + (intptr_t)job->pix_y * i_cur + job->pix_x;
+    const int bid_bits = weighted_cost(prm->lambda_factor, xo_mvbits(prm->search_range, job->bwd_x - job->mvp2_x) +
+                                                           xo_mvbits(prm->search_range, job->bwd_y - job->mvp2_y));
+    /* the target 2 * cur - pred_bwd, clipped (me.c:1322-1337); the backward
+     * prediction comes from its plane, or from mc_luma at the clamped MV
+     * (get_mv_for_mc, mc.h:70-75) when the plane is not built */
+    uint8_t target[64 * 64];
+    memset(target, 0, sizeof(target));
+    if (check_mv_range(job->bwd_x, job->bwd_y, qw.mv_min, qw.mv_max)) {
+        uint8_t pred[64 * 64];
+        const uint8_t *p2 = planes2[((job->bwd_y & 3) << 2) + (job->bwd_x & 3)];
+        if (p2) {
+            p2 += (intptr_t)job->pix_y * i_ref + job->pix_x + (intptr_t)(job->bwd_y >> 2) * i_ref + (job->bwd_x >> 2);
+            for (int y = 0; y < job->h; y++)
+                memcpy(pred + y * 64, p2 + y * i_ref, (size_t)job->w);
+        } else {
+            const int qx = (int16_t)xo_get_mc_pos(prm->frame_w, job->w, job->pix_x, job->bwd_x);
+            const int qy = (int16_t)xo_get_mc_pos(prm->frame_h, job->h, job->pix_y, job->bwd_y);
+            xo_mc_luma(pred, 64, qx, qy, job->w, job->h, planes2, i_ref);
+        }
+        for (int y = 0; y < job->h; y++)
+            for (int x = 0; x < job->w; x++) {
+                const int16_t d = (int16_t)(org[y * i_cur + x] - pred[y * 64 + x]);
+                target[y * 64 + x] = (uint8_t)clip_pel(org[y * i_cur + x] + d);
+            }
+    }
+    const int ctr_x = (job->mvp1_x >> 1) << 1, ctr_y = (job->mvp1_y >> 1) << 1;
+    int bx = job->fwd_x, by = job->fwd_y, bcost = XO_MAX_DISTORTION;
+    res->mvcost = -1;
+    if (!prm->use_fractional_me) {
+        bcost = bid_cost(prm, job, &qw, target, planes1, i_ref, bx, by, bid_bits);
+        res->cost = bcost;
+        res->mv_x = bx;
+        res->mv_y = by;
+        return bcost;
+    }
+    /* half-pel: Spiral[0..8] when hadamard, else Spiral[1..8] (me.c:1340-1349) */
+    const int cx = job->fwd_x, cy = job->fwd_y;
+    for (int pos = prm->use_hadamard ? 0 : 1; pos < 9; pos++) {
+        const int mx = cx + SPIRAL[pos][0] * 2, my = cy + SPIRAL[pos][1] * 2;
+        const int c = bid_cost(prm, job, &qw, target, planes1, i_ref, mx, my, bid_bits);
+        if (c < bcost) { bcost = c; bx = mx; by = my; }
+    }
+    if (prm->use_fractional_me >= 2) {
+        const int qx = bx, qy = by;           /* fwd_mv->v = bmv.v */
+        for (int pos = 1; pos < 9; pos++) {
+            int mx, my;
+            if (prm->enable_pmvr) {
+                if (pmvr_adapt(&mx, &my, ctr_x, ctr_y, qx, qy, SPIRAL[pos][0], SPIRAL[pos][1]))
+                    continue;
+            } else {
+                mx = qx + SPIRAL[pos][0];
+                my = qy + SPIRAL[pos][1];
+            }
+            const int c = bid_cost(prm, job, &qw, target, planes1, i_ref, mx, my, bid_bits);
+            if (c < bcost) { bcost = c; bx = mx; by = my; }
+        }
+    }
+    res->cost = bcost;
+    res->mv_x = bx;
+    res->mv_y = by;
+    res->mvcost = weighted_cost(prm->lambda_factor, xo_mvbits(prm->search_range, bx - job->mvp1_x) +
+                                                    xo_mvbits(prm->search_range, by - job->mvp1_y)) + bid_bits;
+    return bcost;
+}
+
 typedef struct {
     const xo_me_param_t *prm;
     const xo_pu_t *pus;

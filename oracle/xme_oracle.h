@@ -173,6 +173,27 @@ int32_t xo_me_search_sym(const xo_me_param_t *prm, const xo_sym_job_t *job,
                          const uint8_t *const planes1[16], const uint8_t *const planes2[16], intptr_t i_ref,
                          xo_sym_result_t *res);
 
+/* Bi-directional refinement, xavs2_me_search_bid (me.c:1278-1392).  The
+ * backward MV stays fixed; the search refines the forward MV against the
+ * target clip(2 * cur - pred_bwd) (sub_ps / add_ps, pixel.c:780-806), whose
+ * fpel_cmp is halved once more (ME_COST_QPEL_BID, me.c:388-398).  MV costs
+ * against mvp1 (forward) and mvp2 (backward). */
+typedef struct {
+    int pix_x, pix_y, w, h;           /* the PU */
+    int fwd_x, fwd_y;                 /* *fwd_mv on entry (qpel) */
+    int bwd_x, bwd_y;                 /* *bwd_mv (qpel, not searched) */
+    int mvp1_x, mvp1_y;               /* p_me->mvp1: forward MV cost and PMVR centre */
+    int mvp2_x, mvp2_y;               /* p_me->mvp2: backward MV cost */
+} xo_bid_job_t;
+
+/* res->mv = *fwd_mv on return; res->mvcost = p_me->mvcost[PDIR_BID] (-1 where
+ * the search does not set it).  A forward candidate on a plane that is not
+ * built has no cost here (the reference reads through a NULL plane). */
+int32_t xo_me_search_bid(const xo_me_param_t *prm, const xo_bid_job_t *job,
+                         const uint8_t *cur, intptr_t i_cur,
+                         const uint8_t *const planes1[16], const uint8_t *const planes2[16], intptr_t i_ref,
+                         xo_sym_result_t *res);
+
 /* The same over n PUs on nthreads POSIX threads (CPU baseline); ctxs holds
  * one UMH context per PU or is NULL. */
 void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,
