@@ -283,6 +283,34 @@ int xme_me_search_sym_batch(const xme_me_params *prm, const xme_sym_job *d_jobs,
                             intptr_t i_cur, const xme_ref_desc *ref1, const xme_ref_desc *ref2,
                             xme_sym_result *d_results, void *stream);
 
+/* Bi-directional refinement, xavs2_me_search_bid (source/encoder/me.c:1278-1392),
+ * for a batch of PUs against one pair of references (ref1 = p_fref_1st,
+ * forward; ref2 = p_fref_2nd, backward; equal strides).  The backward MV stays
+ * fixed: the target is clip(2 * cur - pred2 at bwd) (sub_ps / add_ps,
+ * pixel.c:780-806), and the forward MV is refined around fwd by
+ * ME_COST_QPEL_BID (me.c:388-398): fpel_cmp(target, ref1 at the candidate)
+ * >> 1 plus the MV rates of the candidate against mvp1 and of bwd against
+ * mvp2.  Half-pel Spiral[0..8] (Spiral[1..8] with SAD), then quarter-pel
+ * Spiral[1..8] with PMVR centred on mvp1.  Results in xme_sym_result: mv =
+ * *fwd_mv on return, mvcost = p_me->mvcost[PDIR_BID].  Status 2 also marks a
+ * backward MV inside the window whose plane is not built (the reference
+ * predicts it with mc_luma, me.c:1329-1335; not provided).  A forward
+ * candidate on a plane that is not built has no cost (the reference reads
+ * through a NULL plane there). */
+typedef struct {
+    int16_t x, y;                    /* PU position (luma pixels, x % 4 == 0) */
+    uint8_t w, h;                    /* PU size, multiples of 4 up to 64 */
+    uint8_t reserved[2];
+    int16_t fwd_x, fwd_y;            /* *fwd_mv on entry, qpel */
+    int16_t bwd_x, bwd_y;            /* *bwd_mv, qpel (not searched) */
+    int16_t mvp1_x, mvp1_y;          /* p_me->mvp1: forward rate and PMVR centre */
+    int16_t mvp2_x, mvp2_y;          /* p_me->mvp2: backward rate */
+} xme_bid_job;
+
+int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_job *d_jobs, int n, const pel_t *d_cur,
+                            intptr_t i_cur, const xme_ref_desc *ref1, const xme_ref_desc *ref2,
+                            xme_sym_result *d_results, void *stream);
+
 /* ---- 4. reference planes, MC and the residual path ---------------------- */
 /* plane_expand_border over a whole plane (source/common/frame.c:470-507). */
 int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, int pad, void *stream);

@@ -1,5 +1,6 @@
-// Symmetric bi-prediction refinement (xavs2_me_search_sym,
-// source/encoder/me.c:1190-1262) for batches of PUs, gfx950.
+// Bi-prediction refinements for batches of PUs, gfx950: the symmetric one
+// (xavs2_me_search_sym, source/encoder/me.c:1190-1262) and the
+// bi-directional one (xavs2_me_search_bid, me.c:1278-1392).
 //
 // One wave per job.  A stage's candidates (the half-pel Spiral points around
 // the start MV, then the quarter-pel points around the half-pel best) are
@@ -10,14 +11,17 @@
 // MV, then takes the tile's Hadamard sum (or SAD) against the current PU.
 // The group's last lane holds the candidate's cost; the wave then applies the
 // reference's sequential strict-'<' update in Spiral order from uniform
-// registers.
+// registers.  The bi-directional search keeps the backward MV fixed and
+// compares reference 1 against the target clip(2 * cur - pred_bwd), which
+// each lane builds once per job for its tiles, in registers.
 #include "xme_dpp.h"
 #include "xme_satd.h"
 #include "xme_internal.h"
 
 namespace {
 
-__constant__ int8_t c_sym_spiral[5][2] = { { 0, 0 }, { 0, -1 }, { 0, 1 }, { -1, -1 }, { 1, -1 } };
+__constant__ int8_t c_bi_spiral[9][2] = { { 0, 0 }, { 0, -1 }, { 0, 1 }, { -1, -1 }, { 1, -1 },
+                                         { -1, 0 }, { 1, 0 }, { -1, 1 }, { 1, 1 } };
 
 typedef const __attribute__((address_space(1))) uint8_t *gbytes;
 typedef const __attribute__((address_space(1))) uint32_t *gdwords;
@@ -178,11 +182,11 @@ __device__ __forceinline__ void sym_cand(int stage, int pos0, int cx, int cy, bo
 {
     skip = false;
     if (stage == 0) {
-        mx = cx + 2 * c_sym_spiral[pos0 + k][0];
-        my = cy + 2 * c_sym_spiral[pos0 + k][1];
+        mx = cx + 2 * c_bi_spiral[pos0 + k][0];
+        my = cy + 2 * c_bi_spiral[pos0 + k][1];
         return;
     }
-    const int dx = c_sym_spiral[1 + k][0], dy = c_sym_spiral[1 + k][1];
+    const int dx = c_bi_spiral[1 + k][0], dy = c_bi_spiral[1 + k][1];
     if (pmvr) {
         const int ax = cx - ctr_x, ay = cy - ctr_y;
         if (ax > 2 || ax < -2 || ay > 2 || ay < -2) {
@@ -202,10 +206,24 @@ __device__ __forceinline__ void sym_cand(int stage, int pos0, int cx, int cy, bo
     my = cy + dy;
 }
 
-// Evaluates the stage's n <= 5 candidates and applies the strict-'<' update
+// ME_COST_QPEL_SYM over the lane's tiles, and the candidate's cost from the
+// group's distortion sum.
+struct SymEval {
+    const SymJob &c;
+    const SymPlanes &pl;
+    __device__ __forceinline__ int tiles(int mx, int my, int t0, int P) const { return sym_tiles(c, pl, mx, my, t0, P); }
+    __device__ __forceinline__ int cost(int s, int mx, int my) const
+    {
+        return (c.hadamard ? (s >> 1) : s) +
+               xme_weighted_cost(c.lambda, xme_mvbits(mx - c.mvp_x) + xme_mvbits(my - c.mvp_y));
+    }
+};
+
+// Evaluates the stage's n <= 9 candidates and applies the strict-'<' update
 // to (bcost, bx, by) in candidate order.
-__device__ __forceinline__ void sym_stage(const SymJob &c, const SymPlanes &pl, int stage, int pos0, int n, int cx,
-                                          int cy, bool pmvr, int ctr_x, int ctr_y, int P, int &bcost, int &bx, int &by)
+template <class Eval>
+__device__ __forceinline__ void bi_stage(const Eval &ev, int stage, int pos0, int n, int cx, int cy, bool pmvr,
+                                         int ctr_x, int ctr_y, int P, int &bcost, int &bx, int &by)
 {
     const int lane = threadIdx.x & 63;
     const int groups = 64 / P, g = lane / P, t0 = lane % P;
@@ -215,13 +233,10 @@ __device__ __forceinline__ void sym_stage(const SymJob &c, const SymPlanes &pl, 
         bool skip = true;
         if (k < n)
             sym_cand(stage, pos0, cx, cy, pmvr, ctr_x, ctr_y, k, mx, my, skip);
-        int s = skip ? -1 : sym_tiles(c, pl, mx, my, t0, P);
+        int s = skip ? -1 : ev.tiles(mx, my, t0, P);
         const bool none = s < 0;         // the same in every lane of the group
         s = group_sum(none ? 0 : s, P);
-        int cost = XME_MAX_DISTORTION;
-        if (!none)
-            cost = (c.hadamard ? (s >> 1) : s) +
-                   xme_weighted_cost(c.lambda, xme_mvbits(mx - c.mvp_x) + xme_mvbits(my - c.mvp_y));
+        const int cost = none ? XME_MAX_DISTORTION : ev.cost(s, mx, my);
         const int ng = min(groups, n - k0);
         for (int gi = 0; gi < ng; gi++) {        // uniform: the reference's candidate order
             const int src = gi * P + P - 1;
@@ -294,7 +309,7 @@ __global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int
     if (p.frac == 0) {
         // the start point alone; its cost is returned even when there is none
         // (me.c:1219-1226)
-        sym_stage(c, pl, 0, 0, 1, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
+        bi_stage(SymEval{ c, pl }, 0, 0, 1, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
         if (lane == 0) {
             res.cost = bcost;
             res.mv_x = j.mv_x;
@@ -307,16 +322,217 @@ __global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int
     }
     // half-pel: Spiral[0..4] with SATD, Spiral[1..4] with SAD (me.c:1210-1211)
     const int pos0 = c.hadamard ? 0 : 1;
-    sym_stage(c, pl, 0, pos0, 5 - pos0, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
+    const SymEval ev{ c, pl };
+    bi_stage(ev, 0, pos0, 5 - pos0, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
     if (p.frac >= 2) {
         const int ctr_x = (j.mvp1_x >> 1) << 1, ctr_y = (j.mvp1_y >> 1) << 1;
-        sym_stage(c, pl, 1, 1, 4, bx, by, p.pmvr != 0, ctr_x, ctr_y, P, bcost, bx, by);
+        bi_stage(ev, 1, 1, 4, bx, by, p.pmvr != 0, ctr_x, ctr_y, P, bcost, bx, by);
     }
     if (lane == 0) {
         res.cost = bcost;
         res.mv_x = (int16_t)bx;
         res.mv_y = (int16_t)by;
         res.mvcost = xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp_x) + xme_mvbits(by - j.mvp_y));
+        res.status = 0;
+        out[ji] = res;
+    }
+}
+
+// A bi-directional job the kernel evaluates: a PU of 4x4 tiles inside the
+// frame at a column that is a multiple of 4, a forward MV (and the
+// candidates within 4 qpel of it) and a backward MV within +-SYM_MAX_MV whose
+// differences to mvp1 / mvp2 stay inside the mvbits table (the caller's
+// check_mv_range / check_mvd gate, md_inter.c:1089-1092).
+__device__ __forceinline__ bool bid_job_ok(const SymParams &p, const xme_bid_job &j)
+{
+    if ((j.w & 3) || (j.h & 3) || j.w < 4 || j.h < 4 || j.w > 64 || j.h > 64)
+        return false;
+    if ((j.x & 3) || j.x < 0 || j.y < 0 || j.x + j.w > p.frame_w || j.y + j.h > p.frame_h)
+        return false;
+    const int fwd[2] = { j.fwd_x, j.fwd_y }, bwd[2] = { j.bwd_x, j.bwd_y };
+    const int mvp1[2] = { j.mvp1_x, j.mvp1_y }, mvp2[2] = { j.mvp2_x, j.mvp2_y };
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        if (bwd[a] > SYM_MAX_MV || bwd[a] < -SYM_MAX_MV || bwd[a] - mvp2[a] > p.max_mvd ||
+            bwd[a] - mvp2[a] < -p.max_mvd)
+            return false;
+#pragma unroll
+        for (int e = -4; e <= 4; e += 8) {
+            const int v = fwd[a] + e;
+            if (v > SYM_MAX_MV || v < -SYM_MAX_MV || v - mvp1[a] > p.max_mvd || v - mvp1[a] < -p.max_mvd)
+                return false;
+        }
+    }
+    return true;
+}
+
+// clip(2 * o - p) on each of four bytes: sub_ps then add_ps
+// (pixel.c:780-806), whose int16 difference never wraps for 8-bit pixels.
+__device__ __forceinline__ uint32_t target4(uint32_t o, uint32_t p)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int v = 2 * xme_bytei(o, i) - xme_bytei(p, i);
+        r |= (uint32_t)min(max(v, 0), 255) << (8 * i);
+    }
+    return r;
+}
+
+// ME_COST_QPEL_BID (me.c:388-398) over the lane's tiles (at most 4: a PU
+// has at most 256 tiles and P = 64 beyond 64), against the target rows
+// `tgt` built once per job.
+struct BidEval {
+    const SymJob &c;             // cur, strides, offset, tiles, window, lambda, switches
+    const uint8_t *const (&p1)[16];
+    const uint32_t (&tgt)[4][4];
+    bool bwd_legal;
+    int mvp1_x, mvp1_y, bid_bits;
+    __device__ __forceinline__ int tiles(int mx, int my, int t0, int P) const
+    {
+        if (!bwd_legal || !sym_legal(c.win, mx, my))
+            return -1;
+        const uint8_t *a = pick_plane(p1, ((my & 3) << 2) + (mx & 3));
+        if (a == nullptr)           // the reference reads through a NULL plane here
+            return -1;
+        const gbytes ga = (gbytes)a + c.off + (int64_t)(my >> 2) * c.i_ref + (mx >> 2);
+        int s = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int t = t0 + i * P;
+            if (t < c.ntiles) {
+                const int ty = (t / c.tw) * 4, tx = (t % c.tw) * 4;
+                uint32_t rr[4];
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    rr[y] = sym_load_dw(ga + (int64_t)(ty + y) * c.i_ref + tx);
+                s += xme_tile_cost_rows(tgt[i], rr, c.hadamard);
+            }
+        }
+        return s;
+    }
+    __device__ __forceinline__ int cost(int s, int mx, int my) const
+    {
+        // fpel_cmp (SATD = half the Hadamard sum, or SAD) halved once more
+        const int d = c.hadamard ? ((s >> 1) >> 1) : (s >> 1);
+        return d + xme_weighted_cost(c.lambda, xme_mvbits(mx - mvp1_x) + xme_mvbits(my - mvp1_y)) + bid_bits;
+    }
+};
+
+__global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int64_t i_ref, const uint8_t *cur,
+                                                  int64_t i_cur, const xme_bid_job *__restrict__ jobs, int n,
+                                                  xme_sym_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const int lane = threadIdx.x & 63;
+    const xme_bid_job j = jobs[ji];
+    xme_sym_result res;
+    if (!bid_job_ok(p, j)) {
+        if (lane == 0) {
+            res.cost = XME_MAX_DISTORTION;
+            res.mv_x = j.fwd_x;
+            res.mv_y = j.fwd_y;
+            res.mvcost = -1;
+            res.status = 2;
+            out[ji] = res;
+        }
+        return;
+    }
+    SymJob c;
+    c.cur = (gbytes)cur + (int64_t)j.y * i_cur + j.x;
+    c.i_cur = i_cur;
+    c.i_ref = i_ref;
+    c.off = (int64_t)j.y * i_ref + j.x;
+    c.w = j.w;
+    c.h = j.h;
+    c.tw = j.w >> 2;
+    c.ntiles = c.tw * (j.h >> 2);
+    c.mvp_x = j.mvp1_x;
+    c.mvp_y = j.mvp1_y;
+    c.dd = 1;
+    c.ds = 1;
+    c.neg = false;
+    c.hadamard = p.hadamard != 0;
+    c.lambda = p.lambda;
+    {
+        // p_me->mv_min / mv_max (md_inter.c:913-924), packed as in
+        // xavs2_me_search_bid (me.c:1300-1301)
+        const int lo_x = xme_clip3(p.min_mv[0], p.max_mv[0], (-64 - j.x) * 4);
+        const int hi_x = xme_clip3(p.min_mv[0], p.max_mv[0], (p.frame_w + 64 - j.x - j.w) * 4);
+        const int lo_y = xme_clip3(p.min_mv[1], p.max_mv[1], (-64 - j.y) * 4);
+        const int hi_y = xme_clip3(p.min_mv[1], p.max_mv[1], (p.frame_h + 64 - j.y - j.h) * 4);
+        c.win.lo[0] = max(lo_x, (-64 - j.x) * 4);
+        c.win.hi[0] = min(hi_x, (p.frame_w + 64 - j.x - j.w) * 4);
+        c.win.lo[1] = max(lo_y, (-64 - j.y) * 4);
+        c.win.hi[1] = min(hi_y, (p.frame_h + 64 - j.y - j.h) * 4);
+        c.win.mv_min = xme_pack2(-lo_x, -lo_y);
+        c.win.mv_max = xme_pack2(hi_x, hi_y) | 0x8000u;
+    }
+    int P = 1;
+    while (P < c.ntiles && P < 64)
+        P <<= 1;
+    const int t0 = lane % P;
+    const bool bwd_legal = sym_legal(c.win, j.bwd_x, j.bwd_y);
+    const uint8_t *p2 = pick_plane(pl.p2, ((j.bwd_y & 3) << 2) + (j.bwd_x & 3));
+    if (bwd_legal && p2 == nullptr) {
+        // the reference predicts from mc_luma here (me.c:1329-1335): not provided
+        if (lane == 0) {
+            res.cost = XME_MAX_DISTORTION;
+            res.mv_x = j.fwd_x;
+            res.mv_y = j.fwd_y;
+            res.mvcost = -1;
+            res.status = 2;
+            out[ji] = res;
+        }
+        return;
+    }
+    // the target rows of the lane's tiles: clip(2 * cur - pred_bwd)
+    // (me.c:1322-1337)
+    uint32_t tgt[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int t = t0 + i * P;
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+            tgt[i][y] = 0;
+        if (bwd_legal && t < c.ntiles) {
+            const int ty = (t / c.tw) * 4, tx = (t % c.tw) * 4;
+            const gbytes gb = (gbytes)p2 + c.off + (int64_t)(j.bwd_y >> 2) * i_ref + (j.bwd_x >> 2);
+#pragma unroll
+            for (int y = 0; y < 4; y++)
+                tgt[i][y] = target4(*(gdwords)(c.cur + (ty + y) * i_cur + tx),
+                                    sym_load_dw(gb + (int64_t)(ty + y) * i_ref + tx));
+        }
+    }
+    const int bid_bits = xme_weighted_cost(p.lambda, xme_mvbits(j.bwd_x - j.mvp2_x) + xme_mvbits(j.bwd_y - j.mvp2_y));
+    const BidEval ev{ c, pl.p1, tgt, bwd_legal, j.mvp1_x, j.mvp1_y, bid_bits };
+    int bcost = XME_MAX_DISTORTION, bx = j.fwd_x, by = j.fwd_y;
+    if (p.frac == 0) {
+        bi_stage(ev, 0, 0, 1, j.fwd_x, j.fwd_y, false, 0, 0, P, bcost, bx, by);
+        if (lane == 0) {
+            res.cost = bcost;
+            res.mv_x = j.fwd_x;
+            res.mv_y = j.fwd_y;
+            res.mvcost = -1;
+            res.status = 0;
+            out[ji] = res;
+        }
+        return;
+    }
+    // half-pel: Spiral[0..8] with SATD, Spiral[1..8] with SAD (me.c:1340-1349)
+    const int pos0 = c.hadamard ? 0 : 1;
+    bi_stage(ev, 0, pos0, 9 - pos0, j.fwd_x, j.fwd_y, false, 0, 0, P, bcost, bx, by);
+    if (p.frac >= 2) {
+        const int ctr_x = (j.mvp1_x >> 1) << 1, ctr_y = (j.mvp1_y >> 1) << 1;
+        bi_stage(ev, 1, 1, 8, bx, by, p.pmvr != 0, ctr_x, ctr_y, P, bcost, bx, by);
+    }
+    if (lane == 0) {
+        res.cost = bcost;
+        res.mv_x = (int16_t)bx;
+        res.mv_y = (int16_t)by;
+        res.mvcost = xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp1_x) + xme_mvbits(by - j.mvp1_y)) + bid_bits;
         res.status = 0;
         out[ji] = res;
     }
@@ -364,6 +580,50 @@ extern "C" int xme_me_search_sym_batch(const xme_me_params *prm, const xme_sym_j
     }
     const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
     hipLaunchKernelGGL(sym_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, pl,
+                       (int64_t)ref1->stride, d_cur, (int64_t)i_cur, d_jobs, n, d_results);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+extern "C" int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_job *d_jobs, int n,
+                                       const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref1,
+                                       const xme_ref_desc *ref2, xme_sym_result *d_results, void *stream)
+{
+    if (!prm || n < 0 || !ref1 || !ref2 || prm->use_fractional_me < 0 || prm->use_fractional_me > 2 ||
+        prm->frame_w <= 0 || prm->frame_h <= 0 || prm->search_range <= 0) {
+        xme_set_error("xme_me_search_bid_batch: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (n == 0)
+        return XME_OK;
+    if (!d_jobs || !d_cur || !d_results || !ref1->planes[0] || !ref2->planes[0]) {
+        xme_set_error("xme_me_search_bid_batch: NULL buffer");
+        return XME_ERR_ARG;
+    }
+    // both references are addressed with the first one's stride (me.c:1315)
+    if (ref1->stride != ref2->stride || ((uintptr_t)d_cur & 3) || (i_cur & 3)) {
+        xme_set_error("xme_me_search_bid_batch: reference strides differ or the current plane is not 4-byte aligned");
+        return XME_ERR_ARG;
+    }
+    SymParams p;
+    p.frame_w = prm->frame_w;
+    p.frame_h = prm->frame_h;
+    p.min_mv[0] = prm->min_mv_range[0];
+    p.min_mv[1] = prm->min_mv_range[1];
+    p.max_mv[0] = prm->max_mv_range[0];
+    p.max_mv[1] = prm->max_mv_range[1];
+    p.lambda = prm->lambda_factor;
+    p.hadamard = prm->use_hadamard ? 1 : 0;
+    p.frac = prm->use_fractional_me;
+    p.pmvr = prm->enable_pmvr ? 1 : 0;
+    p.max_mvd = xme_mvbits_max_mvd(prm->search_range);
+    SymPlanes pl;
+    for (int k = 0; k < 16; k++) {
+        pl.p1[k] = ref1->planes[k];
+        pl.p2[k] = ref2->planes[k];
+    }
+    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    hipLaunchKernelGGL(bid_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, pl,
                        (int64_t)ref1->stride, d_cur, (int64_t)i_cur, d_jobs, n, d_results);
     XME_HIP(hipGetLastError());
     return XME_OK;

@@ -54,6 +54,9 @@ SYM_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), 
                           ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvp_x", "<i2"), ("mvp_y", "<i2"),
                           ("mvp1_x", "<i2"), ("mvp1_y", "<i2"), ("dist_fwd", "<i2"), ("dist_bwd", "<i2"),
                           ("b_slice", "<i4")])
+BID_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
+                          ("fwd_x", "<i2"), ("fwd_y", "<i2"), ("bwd_x", "<i2"), ("bwd_y", "<i2"),
+                          ("mvp1_x", "<i2"), ("mvp1_y", "<i2"), ("mvp2_x", "<i2"), ("mvp2_y", "<i2")])
 SYM_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvcost", "<i4"),
                              ("status", "<i4")])
 
@@ -123,6 +126,8 @@ def lib():
         L.xme_mc_chroma_batch.argtypes = [P, P, Z, P, I, I, I, P, P, Z, P, P]
         L.xme_mc_luma_batch.argtypes = [P, P, I, I, I, P, Z, P, P]
         L.xme_me_search_sym_batch.argtypes = [C.POINTER(MeParams), P, I, P, Z, C.POINTER(RefDesc),
+                                              C.POINTER(RefDesc), P, P]
+        L.xme_me_search_bid_batch.argtypes = [C.POINTER(MeParams), P, I, P, Z, C.POINTER(RefDesc),
                                               C.POINTER(RefDesc), P, P]
         L.xme_dct_batch.argtypes = [I, P, P, I, P]
         L.xme_idct_batch.argtypes = [I, P, P, I, P]
@@ -279,14 +284,23 @@ def me_search_sym_batch(params, jobs, cur, ref1, ref2, stream=None):
     """xme_me_search_sym_batch on host job records (SYM_JOB_DTYPE): `cur` is a
     DevicePlane holding the current picture, ref1 / ref2 DeviceRefs of the same
     stride.  Returns the SYM_RESULT_DTYPE records (blocking)."""
-    jobs = np.ascontiguousarray(jobs, dtype=SYM_JOB_DTYPE)
+    return _bipred_batch("xme_me_search_sym_batch", SYM_JOB_DTYPE, params, jobs, cur, ref1, ref2, stream)
+
+
+def me_search_bid_batch(params, jobs, cur, ref1, ref2, stream=None):
+    """xme_me_search_bid_batch on host job records (BID_JOB_DTYPE), as
+    me_search_sym_batch."""
+    return _bipred_batch("xme_me_search_bid_batch", BID_JOB_DTYPE, params, jobs, cur, ref1, ref2, stream)
+
+
+def _bipred_batch(name, dtype, params, jobs, cur, ref1, ref2, stream):
+    jobs = np.ascontiguousarray(jobs, dtype=dtype)
     n = len(jobs)
     d_jobs = upload(jobs) if n else DeviceBuffer(1)
     d_res = DeviceBuffer(max(n, 1) * SYM_RESULT_DTYPE.itemsize)
     r1, r2 = ref1.desc(), ref2.desc()
-    check(lib().xme_me_search_sym_batch(C.byref(params), C.c_void_p(d_jobs.ptr), n, C.c_void_p(cur.origin),
-                                        cur.stride, C.byref(r1), C.byref(r2), C.c_void_p(d_res.ptr), stream),
-          "xme_me_search_sym_batch")
+    check(getattr(lib(), name)(C.byref(params), C.c_void_p(d_jobs.ptr), n, C.c_void_p(cur.origin), cur.stride,
+                               C.byref(r1), C.byref(r2), C.c_void_p(d_res.ptr), stream), name)
     check(lib().xme_stream_sync(stream), "xme_stream_sync")
     return d_res.download(n, SYM_RESULT_DTYPE)
 
