@@ -207,51 +207,35 @@ def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, st
     return out
 
 
-def sym_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu):
-    """Symmetric bi-prediction refinement (xme_me_search_sym_batch) of every PU
-    between references 0 and 1, as the encoder's dual-hypothesis search runs it
-    (md_inter.c:1165-1186): start MV = 4 x the PU's full-pel best against
-    reference 0 (all_best_imv), F-frame distances 2 and 4 (the second MV is
-    twice the first), MVP (0,0), SATD, half- then quarter-pel (9 points of
-    w x h pixels per PU).  Kernel time from back-to-back launches on one
-    stream; parity and the CPU time on every 4th PU."""
+def bipred_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu):
+    """The bi-prediction refinements of every PU between references 0 and 1,
+    as the encoder runs them: symmetric (xme_me_search_sym_batch; the
+    dual-hypothesis search, md_inter.c:1165-1186, start = 4 x the PU's
+    full-pel best against reference 0, F-frame distances 2 and 4, so the second
+    MV is twice the first; 5 half- and 4 quarter-pel points) and
+    bi-directional (xme_me_search_bid_batch; md_inter.c:1089-1094, forward
+    start and fixed backward MV = 4 x the full-pel bests against references 0
+    and 1; 9 half- and 8 quarter-pel points).  MVPs (0,0), SATD.  Kernel time
+    from back-to-back launches on one stream; parity and the CPU time on
+    every 4th PU."""
     import ctypes as C
     from oracle import oracle as O
-    r0 = gpu_res[0::n_ref]
+    r0, r1 = gpu_res[0::n_ref], gpu_res[1::n_ref]
     arr = np.array(pus, dtype=np.int64)
-    jobs = np.zeros(len(pus), dtype=X.SYM_JOB_DTYPE)
-    jobs["x"], jobs["y"], jobs["w"], jobs["h"] = arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3]
-    jobs["mv_x"], jobs["mv_y"] = 4 * r0["bmv2_x"], 4 * r0["bmv2_y"]
-    jobs["dist_fwd"], jobs["dist_bwd"], jobs["b_slice"] = 2, 4, 0
+    n = len(pus)
+    sym = np.zeros(n, dtype=X.SYM_JOB_DTYPE)
+    bid = np.zeros(n, dtype=X.BID_JOB_DTYPE)
+    for jobs in (sym, bid):
+        jobs["x"], jobs["y"], jobs["w"], jobs["h"] = arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3]
+    sym["mv_x"], sym["mv_y"] = 4 * r0["bmv2_x"], 4 * r0["bmv2_y"]
+    sym["dist_fwd"], sym["dist_bwd"], sym["b_slice"] = 2, 4, 0
+    bid["fwd_x"], bid["fwd_y"] = 4 * r0["bmv2_x"], 4 * r0["bmv2_y"]
+    bid["bwd_x"], bid["bwd_y"] = 4 * r1["bmv2_x"], 4 * r1["bmv2_y"]
     prm = X.me_params(w, h, R, lam)
-    d_jobs = X.upload(jobs)
-    d_res = X.DeviceBuffer(len(jobs) * X.SYM_RESULT_DTYPE.itemsize)
-    r1, r2 = d_refs[0].desc(), d_refs[1].desc()
+    d1, d2 = d_refs[0].desc(), d_refs[1].desc()
     stream = X.Stream()
-
-    def launch():
-        X.check(X.lib().xme_me_search_sym_batch(C.byref(prm), C.c_void_p(d_jobs.ptr), len(jobs),
-                                                C.c_void_p(dcur.origin), dcur.stride, C.byref(r1), C.byref(r2),
-                                                C.c_void_p(d_res.ptr), stream.h), "xme_me_search_sym_batch")
-
-    for _ in range(3):
-        launch()
-    stream.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        launch()
-    stream.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / reps
-    res = d_res.download(len(jobs), X.SYM_RESULT_DTYPE)
-    pix = int((arr[:, 2] * arr[:, 3]).sum()) * 9
-    leg = {
-        "workload": f"{len(jobs)} PUs of the frame, references 0 and 1, start = 4 x full-pel best of reference 0, "
-                    "distances 2 / 4 (F frame), SATD, quarter-pel",
-        "ms_per_batch": ms,
-        "pus_per_s": len(jobs) / (ms * 1e-3),
-        "candidate_pixels_per_s": pix / (ms * 1e-3),
-        "status_ok": bool(np.all(res["status"] == 0)),
-    }
+    area = int((arr[:, 2] * arr[:, 3]).sum())
+    planes, cp = None, None
     if cpu:
         planes = []
         for r in refs[:2]:
@@ -261,20 +245,62 @@ def sym_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps,
             planes.append(O.interp_planes(p, 2))
         cp = O.Plane(w, h)
         cp.set_picture(cur)
-        prm_o = O.me_param(w, h, R, lam)
-        idx = range(0, len(jobs), 4)
+    prm_o = O.me_param(w, h, R, lam)
+    out = {}
+    for name, jobs, entry, points in (("sym", sym, "xme_me_search_sym_batch", 9),
+                                      ("bid", bid, "xme_me_search_bid_batch", 17)):
+        d_jobs = X.upload(jobs)
+        d_res = X.DeviceBuffer(n * X.SYM_RESULT_DTYPE.itemsize)
+
+        def launch():
+            X.check(getattr(X.lib(), entry)(C.byref(prm), C.c_void_p(d_jobs.ptr), n, C.c_void_p(dcur.origin),
+                                            dcur.stride, C.byref(d1), C.byref(d2), C.c_void_p(d_res.ptr),
+                                            stream.h), entry)
+
+        for _ in range(3):
+            launch()
+        stream.sync()
         t0 = time.perf_counter()
-        exp = [O.me_search_sym(prm_o, dict(pix_x=int(jobs["x"][i]), pix_y=int(jobs["y"][i]), w=int(jobs["w"][i]),
-                                           h=int(jobs["h"][i]), mv_x=int(jobs["mv_x"][i]), mv_y=int(jobs["mv_y"][i]),
-                                           mvp_x=0, mvp_y=0, mvp1_x=0, mvp1_y=0, dist_fwd=2, dist_bwd=4, b_slice=0),
-                               cp, planes[0], planes[1]) for i in idx]
-        el = time.perf_counter() - t0
-        got = [(int(res["cost"][i]), int(res["mv_x"][i]), int(res["mv_y"][i]), int(res["mvcost"][i])) for i in idx]
-        leg["parity_with_oracle_every_4th"] = got == [tuple(int(v) for v in e) for e in exp]
-        leg["cpu_oracle"] = {"pus_per_s": len(exp) / el, "threads": 1, "seconds": el,
-                             "sample": f"every 4th PU ({len(exp)}), one thread, Python loop over the C oracle"}
-    log(f"sym: {ms:.3f} ms per batch of {len(jobs)} PUs")
-    return leg
+        for _ in range(reps):
+            launch()
+        stream.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        res = d_res.download(n, X.SYM_RESULT_DTYPE)
+        leg = {
+            "ms_per_batch": ms,
+            "pus_per_s": n / (ms * 1e-3),
+            "candidate_pixels_per_s": area * points / (ms * 1e-3),
+            "points_per_pu": points,
+            "status_ok": bool(np.all(res["status"] == 0)),
+        }
+        if cpu:
+            idx = range(0, n, 4)
+            t0 = time.perf_counter()
+            if name == "sym":
+                exp = [O.me_search_sym(prm_o, dict(pix_x=int(jobs["x"][i]), pix_y=int(jobs["y"][i]),
+                                                   w=int(jobs["w"][i]), h=int(jobs["h"][i]),
+                                                   mv_x=int(jobs["mv_x"][i]), mv_y=int(jobs["mv_y"][i]), mvp_x=0,
+                                                   mvp_y=0, mvp1_x=0, mvp1_y=0, dist_fwd=2, dist_bwd=4, b_slice=0),
+                                       cp, planes[0], planes[1]) for i in idx]
+            else:
+                exp = [O.me_search_bid(prm_o, dict(pix_x=int(jobs["x"][i]), pix_y=int(jobs["y"][i]),
+                                                   w=int(jobs["w"][i]), h=int(jobs["h"][i]),
+                                                   fwd_x=int(jobs["fwd_x"][i]), fwd_y=int(jobs["fwd_y"][i]),
+                                                   bwd_x=int(jobs["bwd_x"][i]), bwd_y=int(jobs["bwd_y"][i]),
+                                                   mvp1_x=0, mvp1_y=0, mvp2_x=0, mvp2_y=0),
+                                       cp, planes[0], planes[1]) for i in idx]
+            el = time.perf_counter() - t0
+            got = [(int(res["cost"][i]), int(res["mv_x"][i]), int(res["mv_y"][i]), int(res["mvcost"][i]))
+                   for i in idx]
+            leg["parity_with_oracle_every_4th"] = got == [tuple(int(v) for v in e) for e in exp]
+            leg["cpu_oracle"] = {"pus_per_s": len(exp) / el, "threads": 1, "seconds": el,
+                                 "sample": f"every 4th PU ({len(exp)}), one thread, Python loop over the C oracle"}
+        log(f"{name}: {ms:.3f} ms per batch of {n} PUs")
+        out[name] = leg
+    out["workload"] = (f"{n} PUs of the frame, references 0 and 1; sym: start = 4 x full-pel best of reference 0, "
+                       "distances 2 / 4 (F frame); bid: forward / backward = 4 x full-pel bests of references "
+                       "0 / 1; MVP (0,0), SATD, quarter-pel")
+    return out
 
 
 def transform_leg(X, blocks_total, reps, seed=91):
@@ -486,7 +512,7 @@ def main():
                          "leaves CUs for frames that arrive while a large one searches")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
     ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX / UMH pattern-search leg")
-    ap.add_argument("--no-sym", action="store_true", help="skip the symmetric bi-prediction leg")
+    ap.add_argument("--no-bipred", action="store_true", help="skip the bi-prediction (sym / bid) leg")
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
@@ -593,11 +619,11 @@ def main():
         log("pattern-search leg (DIA, HEX, UMH)")
         pattern = pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, args.steps,
                               not args.no_cpu)
-    sym = None
-    if not args.no_sym and n_ref >= 2:
-        log("symmetric bi-prediction leg")
-        sym = sym_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, max(20, args.steps),
-                      not args.no_cpu)
+    bipred = None
+    if not args.no_bipred and n_ref >= 2:
+        log("bi-prediction leg (sym, bid)")
+        bipred = bipred_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, max(20, args.steps),
+                            not args.no_cpu)
     transform = None
     if not args.no_transform:
         log(f"residual transform leg: {args.transform_blocks} blocks")
@@ -650,7 +676,7 @@ def main():
         "results_stable_across_replays": stable,
         "cpu_baseline": cpu,
         "pattern_search": pattern,
-        "sym_bipred": sym,
+        "bipred": bipred,
         "transform": transform,
         "serving": serving,
     }
