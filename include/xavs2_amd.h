@@ -343,6 +343,21 @@ int xme_dequant_batch(int n, coeff_t *d_coef, int nblk, int qp, void *stream);
  * recon = reconstructed residual, nnz per block (d_nnz may be NULL). */
 int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t *d_level, coeff_t *d_recon,
                        int32_t *d_nnz, int nblk, void *stream);
+/* Luma inter reconstruction of ntu N x N transform blocks (n in 4, 8, 16, 32)
+ * at one QP: the TU's prediction by mc_luma from ref at its PU's clamped
+ * position (rdo_get_pred_inter, source/encoder/rdo.c:1616-1633), the
+ * prediction error (sub_ps), DCT -> quant -> dequant -> IDCT
+ * (xme_residual_batch, inter), the reconstruction clip(pred + residual)
+ * (add_ps) written into d_rec and its SSD against d_cur, as the luma TU path
+ * of rdo.c:895-945 without RDOQ or the secondary transform.
+ * d_tus: ntu x {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y} (MV in qpel,
+ * the TU inside the PU).  d_resid (prediction error) and d_rres
+ * (reconstructed residual) are ntu * n * n coefficients each; d_level holds
+ * the quantised levels, d_nnz / d_ssd one value per TU. */
+int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
+                          int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
+                          coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,
+                          int32_t *d_ssd, void *stream);
 
 /* ---- 5. serving: continuous-batching ME for many concurrent encodes ------ *
  * The frame-level ME of many encodes (streams) share one GPU.  Each stream

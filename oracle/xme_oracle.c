@@ -1196,6 +1196,36 @@ int xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv)
     return blk_pos * 4 + mv;
 }
 
+void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
+                          intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
+                          uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd)
+{
+    for (int i = 0; i < ntu; i++) {
+        const int32_t *r = tus + 8 * i;
+        /* the PU's prediction (get_mv_for_mc with the PU size), then the TU's part of it */
+        uint8_t pred[64 * 64];
+        const int qx = (int16_t)xo_get_mc_pos(frame_w, r[4], r[2], r[6]);
+        const int qy = (int16_t)xo_get_mc_pos(frame_h, r[5], r[3], r[7]);
+        xo_mc_luma(pred, 64, qx, qy, r[4], r[5], planes, i_ref);
+        const uint8_t *p = pred + (r[1] - r[3]) * 64 + (r[0] - r[2]);
+        const uint8_t *c = cur + (intptr_t)r[1] * i_cur + r[0];
+        int16_t res[32 * 32], rr[32 * 32];
+        for (int y = 0; y < n; y++)                      /* sub_ps */
+            for (int x = 0; x < n; x++)
+                res[y * n + x] = (int16_t)(c[y * i_cur + x] - p[y * 64 + x]);
+        xo_residual_batch(n, qp, 0, res, level + (size_t)i * n * n, rr, nnz + i, 1);
+        uint8_t *d = rec + (intptr_t)r[1] * i_rec + r[0];
+        int e = 0;
+        for (int y = 0; y < n; y++)                      /* add_ps, then pixf.ssd */
+            for (int x = 0; x < n; x++) {
+                const int v = clip_pel(p[y * 64 + x] + rr[y * n + x]);
+                d[y * i_rec + x] = (uint8_t)v;
+                e += (c[y * i_cur + x] - v) * (c[y * i_cur + x] - v);
+            }
+        ssd[i] = e;
+    }
+}
+
 /* mc_chroma, mc.c:797-829: luma qpel MV = chroma 1/8-pel position. */
 void xo_mc_chroma(uint8_t *dst_u, uint8_t *dst_v, intptr_t i_dst, int qx, int qy, int w, int h,
                   const uint8_t *plane_u, const uint8_t *plane_v, intptr_t i_src)

@@ -214,6 +214,16 @@ void xo_expand_border(uint8_t *plane, intptr_t stride, int w, int h, int pad);
 int  xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int h,
                           int lcu_level, int use_fractional_me);
 
+/* Luma inter reconstruction of N x N TUs (rdo.c:895-945 without RDOQ or the
+ * secondary transform; prediction as rdo_get_pred_inter, rdo.c:1616-1633):
+ * tus[8 * i] = {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}; the
+ * prediction is mc_luma at the PU's clamped position, offset to the TU.
+ * Writes level (n*n per TU), the reconstruction into rec (stride i_rec, at
+ * the TU's position), nnz and the SSD against cur per TU. */
+void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
+                          intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
+                          uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd);
+
 /* cu_get_mc_pos (common/mc.h:52-66). */
 int  xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv);
 /* Block MC (mc.c:275-398, :559-589) and chroma MC (mc.c:797-829).

@@ -1,0 +1,76 @@
+"""GPU parity of the luma inter reconstruction (xme_inter_recon_batch: MC at
+the PU's clamped position -> sub_ps -> DCT / quant / dequant / IDCT ->
+add_ps -> SSD, encoder/rdo.c:895-945 and :1616-1633) against the oracle, for
+every TU size at several QPs, with MVs that get_mv_for_mc clamps, from the
+interpolated planes and from the full-pel plane alone (block filters), and a
+1080p frame of 8x8 TUs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.test_gpu_me import device_cur, oracle_cur, oracle_ref
+from tests.test_oracle_recon import tiles
+from xavs2_amd import lib as X
+from xavs2_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_recon(n, qp, cur, ref_img, tus, frac):
+    h, w = cur.shape
+    dref = X.DeviceRef(w, h, frac=frac)
+    dref.load(ref_img)
+    dcur = device_cur(cur)
+    ntu = len(tus)
+    d_tus = X.upload(np.ascontiguousarray(tus, np.int32))
+    resid, level, rres = (X.DeviceBuffer(2 * ntu * n * n) for _ in range(3))
+    rec = X.DevicePlane(w, h, pad=0, stride=dcur.stride)
+    rec.buf.zero()
+    nnz, ssd = X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)
+    rd = dref.desc()
+    X.check(X.lib().xme_inter_recon_batch(n, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(rd), w, h,
+                                          d_tus.at(), ntu, resid.at(), level.at(), rres.at(), C.c_void_p(rec.origin),
+                                          rec.stride, nnz.at(), ssd.at(), None), "xme_inter_recon_batch")
+    X.check(X.lib().xme_stream_sync(None), "sync")
+    full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
+    return (level.download(ntu * n * n, np.int16).reshape(ntu, n, n), full[:h, :w],
+            nnz.download(ntu, np.int32), ssd.download(ntu, np.int32))
+
+
+@pytest.mark.parametrize("frac", [2, 0])
+@pytest.mark.parametrize("qp", [22, 37])
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_matches_oracle(n, qp, frac):
+    w, h = 192, 128
+    ref = synth.smooth_field(w, h, seed=11)
+    cur = synth.multi_ref_current([ref], 8, 8, seed=12)
+    tus = tiles(w, h, n, np.random.default_rng(100 * n + qp), far=True)
+    got = gpu_recon(n, qp, cur, ref, tus, frac)
+    exp = O.inter_recon_batch(n, qp, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    assert got[2].any() and got[3].any()
+
+
+def test_arguments_and_empty_batch():
+    ref = X.DeviceRef(64, 64, frac=2)
+    rd = ref.desc()
+    for n, qp in ((12, 30), (8, -1), (8, 80)):
+        rc = X.lib().xme_inter_recon_batch(n, qp, None, 64, C.byref(rd), 64, 64, None, 1, None, None, None, None,
+                                           64, None, None, None)
+        assert rc == X.XME_ERR_ARG
+    assert X.lib().xme_inter_recon_batch(8, 30, None, 64, C.byref(rd), 64, 64, None, 0, None, None, None, None,
+                                         64, None, None, None) == X.XME_OK
+
+
+def test_1080p_8x8():
+    W, H = 1920, 1080
+    ref = synth.smooth_field(W, H, seed=21)
+    cur = synth.multi_ref_current([ref], 16, 16, seed=22)
+    tus = tiles(W, H, 8, np.random.default_rng(23), mv_span=120, far=True)
+    got = gpu_recon(8, 32, cur, ref, tus, 2)
+    exp = O.inter_recon_batch(8, 32, oracle_cur(cur), oracle_ref(ref, 2), W, H, tus)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name

@@ -1,0 +1,100 @@
+"""Pins the oracle's luma inter reconstruction (xo_inter_recon_batch: the TU
+path of encoder/rdo.c:895-945 with the prediction of rdo.c:1616-1633):
+
+* a known answer: the reference holds the picture moved by d and the MV is
+  4d, so the prediction is exact: zero levels, nnz 0, the reconstruction is
+  the picture and the SSD is 0;
+* an independent numpy prediction (separable FIR planes) composed with the
+  residual stages (pinned on their own in test_oracle_transform.py), numpy
+  add / clip and SSD, for every TU size and several QPs, including MVs that
+  get_mv_for_mc clamps with the PU's size;
+* the full-pel plane alone (block filters) gives the same results as the
+  interpolated planes.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import npref
+from tests.test_oracle_sym import cur_plane, planes_of, texture
+
+
+def tiles(w, h, n, rng, mv_span=60, far=False):
+    """Non-overlapping TUs: PUs of 2n x 2n (n x n for n = 32) split into n x n TUs, one MV per PU."""
+    pu = n if n == 32 else 2 * n
+    out = []
+    for py in range(0, h - pu + 1, pu):
+        for px in range(0, w - pu + 1, pu):
+            mv = (int(rng.integers(-mv_span, mv_span + 1)), int(rng.integers(-mv_span, mv_span + 1)))
+            if far and rng.random() < 0.3:
+                mv = (4 * int(rng.integers(-300, 300)) + int(rng.integers(0, 4)),
+                      4 * int(rng.integers(-300, 300)) + int(rng.integers(0, 4)))
+            for ty in range(py, py + pu, n):
+                for tx in range(px, px + pu, n):
+                    out.append((tx, ty, px, py, pu, pu, mv[0], mv[1]))
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_exact_prediction_known_answer(n):
+    w, h = 128, 96
+    cur = texture(w, h, 1)
+    d = (3, -2)
+    planes = planes_of(np.roll(cur, (d[1], d[0]), axis=(0, 1)))
+    tus = tiles(w, h, n, np.random.default_rng(0))
+    tus[:, 6], tus[:, 7] = 4 * d[0], 4 * d[1]
+    # keep TUs whose reference pixels do not wrap around the rolled picture
+    keep = (tus[:, 0] >= 4) & (tus[:, 1] >= 4) & (tus[:, 0] + n <= w - 4) & (tus[:, 1] + n <= h - 4)
+    tus = tus[keep]
+    level, rec, nnz, ssd = O.inter_recon_batch(n, 32, cur_plane(cur), planes, w, h, tus)
+    assert not level.any() and not nnz.any() and not ssd.any()
+    for t in tus:
+        assert np.array_equal(rec[t[1]:t[1] + n, t[0]:t[0] + n], cur[t[1]:t[1] + n, t[0]:t[0] + n])
+
+
+def numpy_recon(n, qp, cur, ref, tus):
+    h, w = cur.shape
+    pad = O.PAD
+    pl = [npref.interp_plane_padded(ref, k, pad) for k in range(16)]
+    res = np.zeros((len(tus), n, n), np.int16)
+    preds = []
+    for i, (tx, ty, px, py, pw, ph, mx, my) in enumerate(tus):
+        qx = O.lib().xo_get_mc_pos(w, int(pw), int(px), int(mx))
+        qy = O.lib().xo_get_mc_pos(h, int(ph), int(py), int(my))
+        x0, y0 = (qx >> 2) + (tx - px), (qy >> 2) + (ty - py)
+        p = pl[(qy & 3) * 4 + (qx & 3)][pad + y0:pad + y0 + n, pad + x0:pad + x0 + n].astype(np.int32)
+        preds.append(p)
+        res[i] = cur[ty:ty + n, tx:tx + n].astype(np.int32) - p
+    level, rr, nnz = O.residual_batch(n, qp, 0, res)
+    rec = np.zeros_like(cur)
+    ssd = np.zeros(len(tus), np.int32)
+    for i, (tx, ty, *_rest) in enumerate(tus):
+        v = np.clip(preds[i] + rr[i].astype(np.int32), 0, 255)
+        rec[ty:ty + n, tx:tx + n] = v
+        ssd[i] = int(((cur[ty:ty + n, tx:tx + n].astype(np.int64) - v) ** 2).sum())
+    return level, rec, nnz, ssd
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+@pytest.mark.parametrize("qp", [22, 37, 51])
+def test_matches_numpy_composition(n, qp):
+    w, h = 128, 96
+    cur, ref = texture(w, h, 2), texture(w, h, 3)
+    tus = tiles(w, h, n, np.random.default_rng(n + qp), far=True)
+    got = O.inter_recon_batch(n, qp, cur_plane(cur), planes_of(ref), w, h, tus)
+    exp = numpy_recon(n, qp, cur, ref, tus)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+    assert got[3].any() and got[0].any()
+
+
+def test_full_pel_plane_alone_gives_the_same_results():
+    w, h = 128, 96
+    cur, ref = texture(w, h, 4), texture(w, h, 5)
+    full = planes_of(ref)
+    only0 = [full[0]] + [None] * 15
+    tus = tiles(w, h, 8, np.random.default_rng(7), far=True)
+    a = O.inter_recon_batch(8, 27, cur_plane(cur), full, w, h, tus)
+    b = O.inter_recon_batch(8, 27, cur_plane(cur), only0, w, h, tus)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

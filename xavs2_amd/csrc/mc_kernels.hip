@@ -8,6 +8,7 @@
 // v_dot4_i32_i8 products of the packed taps with (pixel - 128) bytes plus
 // 128 * 64 (every AVS2 filter's taps sum to 64), so each sum is an exact 32-bit
 // integer; the compiler then has no 16-bit form to narrow the sums into.
+#include "xme_dpp.h"
 #include "xme_internal.h"
 
 namespace {
@@ -102,6 +103,53 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
     }
 }
 
+// A luma block's source for mc_luma at absolute qpel position (qx, qy):
+// the interpolated plane when it exists, else the full-pel plane and the
+// block filter's taps.
+struct LumaSrc {
+    const uint8_t *plane;
+    const uint8_t *p0;
+    int64_t i_ref;
+    int x0, y0, dx, dy, hlo, hhi, vlo, vhi;
+};
+
+__device__ __forceinline__ LumaSrc luma_src(const Planes16c &pl, int64_t i_ref, int qx, int qy)
+{
+    LumaSrc s;
+    s.x0 = qx >> 2;
+    s.y0 = qy >> 2;
+    s.dx = qx & 3;
+    s.dy = qy & 3;
+    s.i_ref = i_ref;
+    s.plane = pl.p[(s.dy << 2) + s.dx];
+    s.p0 = pl.p[0];
+    s.hlo = pack_taps(c_luma[s.dx]);
+    s.hhi = pack_taps(c_luma[s.dx] + 4);
+    s.vlo = pack_taps(c_luma[s.dy]);
+    s.vhi = pack_taps(c_luma[s.dy] + 4);
+    return s;
+}
+
+// Pixel (x, y) of the block.
+__device__ __forceinline__ int luma_px(const LumaSrc &b, int x, int y)
+{
+    if (b.plane != nullptr)
+        return b.plane[(int64_t)(b.y0 + y) * b.i_ref + b.x0 + x];
+    const uint8_t *s = b.p0 + (int64_t)(b.y0 + y) * b.i_ref + b.x0 + x;
+    if (b.dy == 0)
+        return clip_pel((fir8(s, 1, b.hlo, b.hhi) + 32) >> 6);
+    if (b.dx == 0)
+        return clip_pel((fir8(s, b.i_ref, b.vlo, b.vhi) + 32) >> 6);
+    // intpl_luma_block_ext_c (mc.c:362-398): the mct_t (int16) temporaries hold
+    // the horizontal sums unchanged, which lie in [-19 * 255, 88 * 255] for
+    // 8-bit pixels
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        a += (int)c_luma[b.dy][k] * fir8(s + (int64_t)(k - 3) * b.i_ref, 1, b.hlo, b.hhi);
+    return clip_pel((a + 2048) >> 12);
+}
+
 __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_ref, const int32_t *__restrict__ blocks,
                                                       int n, int fw, int fh, uint8_t *__restrict__ dst, int64_t i_dst,
                                                       const int64_t *__restrict__ dst_off)
@@ -111,35 +159,93 @@ __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_re
         return;
     const int32_t *bl = blocks + 6 * b;
     const int lx = bl[0], ly = bl[1], w = bl[2], h = bl[3];
-    const int qx = mc_pos(fw, w, lx, bl[4]), qy = mc_pos(fh, h, ly, bl[5]);
-    const int x0 = qx >> 2, y0 = qy >> 2, dx = qx & 3, dy = qy & 3;
-    const uint8_t *plane = pl.p[(dy << 2) + dx];
-    const int hlo = pack_taps(c_luma[dx]), hhi = pack_taps(c_luma[dx] + 4);
-    const int vlo = pack_taps(c_luma[dy]), vhi = pack_taps(c_luma[dy] + 4);
+    const LumaSrc src = luma_src(pl, i_ref, mc_pos(fw, w, lx, bl[4]), mc_pos(fh, h, ly, bl[5]));
     for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
         const int y = t / w, x = t % w;
-        int v;
-        if (plane != nullptr) {
-            v = plane[(int64_t)(y0 + y) * i_ref + x0 + x];
-        } else {
-            const uint8_t *s = pl.p[0] + (int64_t)(y0 + y) * i_ref + x0 + x;
-            if (dy == 0) {
-                v = clip_pel((fir8(s, 1, hlo, hhi) + 32) >> 6);
-            } else if (dx == 0) {
-                v = clip_pel((fir8(s, i_ref, vlo, vhi) + 32) >> 6);
-            } else {
-                // intpl_luma_block_ext_c (mc.c:362-398): the mct_t (int16)
-                // temporaries hold the horizontal sums unchanged, which lie in
-                // [-19 * 255, 88 * 255] for 8-bit pixels
-                int a = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    a += (int)c_luma[dy][k] * fir8(s + (int64_t)(k - 3) * i_ref, 1, hlo, hhi);
-                v = clip_pel((a + 2048) >> 12);
-            }
-        }
-        dst[dst_off[b] + (int64_t)y * i_dst + x] = (uint8_t)v;
+        dst[dst_off[b] + (int64_t)y * i_dst + x] = (uint8_t)luma_px(src, x, y);
     }
+}
+
+// ---- luma inter reconstruction of N x N transform blocks ------------------
+//
+// A record is {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}: the TU lies in
+// the PU, whose prediction is mc_luma at the PU's clamped position
+// (rdo_get_pred_inter, rdo.c:1616-1633, get_mv_for_mc with the PU size).  One
+// wave per TU; lanes over the TU's pixels.
+constexpr int RC_WAVES = 4;
+
+__device__ __forceinline__ LumaSrc tu_src(const Planes16c &pl, int64_t i_ref, const int32_t *r, int fw, int fh)
+{
+    const int qx = mc_pos(fw, r[4], r[2], r[6]) + 4 * (r[0] - r[2]);
+    const int qy = mc_pos(fh, r[5], r[3], r[7]) + 4 * (r[1] - r[3]);
+    return luma_src(pl, i_ref, qx, qy);
+}
+
+// The prediction error cur - pred (sub_ps, pixel.c:780-792) into N x N int16
+// blocks, contiguous per TU.
+template <int N>
+__global__ void __launch_bounds__(256) recon_sub_kernel(Planes16c pl, int64_t i_ref, const uint8_t *__restrict__ cur,
+                                                        int64_t i_cur, const int32_t *__restrict__ tus, int ntu,
+                                                        int fw, int fh, int16_t *__restrict__ resid)
+{
+    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6));
+    if (b >= ntu)
+        return;
+    const int32_t *r = tus + 8 * b;
+    const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
+    const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
+    for (int t = threadIdx.x & 63; t < N * N; t += 64) {
+        const int y = t / N, x = t % N;
+        resid[(int64_t)b * N * N + t] = (int16_t)((int)c[(int64_t)y * i_cur + x] - luma_px(src, x, y));
+    }
+}
+
+// The reconstruction clip(pred + residual) (add_ps, pixel.c:794-806) into the
+// picture, and its SSD against cur (pixf.ssd, rdo.c:945).
+template <int N>
+__global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, int64_t i_ref, const uint8_t *__restrict__ cur,
+                                                        int64_t i_cur, const int32_t *__restrict__ tus, int ntu,
+                                                        int fw, int fh, const int16_t *__restrict__ rres,
+                                                        uint8_t *__restrict__ rec, int64_t i_rec,
+                                                        int32_t *__restrict__ ssd)
+{
+    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6));
+    if (b >= ntu)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int32_t *r = tus + 8 * b;
+    const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
+    const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
+    uint8_t *d = rec + (int64_t)r[1] * i_rec + r[0];
+    int e = 0;
+    for (int t = lane; t < N * N; t += 64) {
+        const int y = t / N, x = t % N;
+        const int v = clip_pel(luma_px(src, x, y) + rres[(int64_t)b * N * N + t]);
+        d[(int64_t)y * i_rec + x] = (uint8_t)v;
+        const int diff = (int)c[(int64_t)y * i_cur + x] - v;
+        e += diff * diff;
+    }
+    e = wave_sum(e);
+    if (lane == 0)
+        ssd[b] = e;
+}
+
+template <int N>
+int launch_recon(int qp, const uint8_t *cur, int64_t i_cur, const Planes16c &pl, int64_t i_ref, int fw, int fh,
+                 const int32_t *tus, int ntu, int16_t *resid, int16_t *level, int16_t *rres, uint8_t *rec,
+                 int64_t i_rec, int32_t *nnz, int32_t *ssd, hipStream_t s)
+{
+    const int grid = (ntu + RC_WAVES - 1) / RC_WAVES;
+    hipLaunchKernelGGL(recon_sub_kernel<N>, dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, i_ref, cur, i_cur, tus, ntu, fw,
+                       fh, resid);
+    XME_HIP(hipGetLastError());
+    const int rc = xme_residual_batch(N, qp, 0, resid, level, rres, nnz, ntu, s);
+    if (rc != XME_OK)
+        return rc;
+    hipLaunchKernelGGL(recon_add_kernel<N>, dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, i_ref, cur, i_cur, tus, ntu, fw,
+                       fh, rres, rec, i_rec, ssd);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
 }
 
 } // namespace
@@ -176,4 +282,41 @@ extern "C" int xme_mc_luma_batch(const xme_ref_desc *ref, const int32_t *d_block
                        n, frame_w, frame_h, d_dst, (int64_t)i_dst, d_dst_off);
     XME_HIP(hipGetLastError());
     return XME_OK;
+}
+
+extern "C" int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
+                                     int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
+                                     coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,
+                                     int32_t *d_ssd, void *stream)
+{
+    if (!ref || !ref->planes[0] || ntu < 0 || (n != 4 && n != 8 && n != 16 && n != 32) || qp < 0 || qp > 79 ||
+        frame_w <= 0 || frame_h <= 0) {
+        xme_set_error("xme_inter_recon_batch: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (ntu == 0)
+        return XME_OK;
+    if (!d_cur || !d_tus || !d_resid || !d_level || !d_rres || !d_rec || !d_nnz || !d_ssd) {
+        xme_set_error("xme_inter_recon_batch: NULL buffer");
+        return XME_ERR_ARG;
+    }
+    Planes16c pl;
+    for (int k = 0; k < 16; k++)
+        pl.p[k] = ref->planes[k];
+    const hipStream_t s = (hipStream_t)stream;
+    const int64_t ir = (int64_t)ref->stride;
+    switch (n) {
+    case 4:
+        return launch_recon<4>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres, d_rec,
+                               i_rec, d_nnz, d_ssd, s);
+    case 8:
+        return launch_recon<8>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres, d_rec,
+                               i_rec, d_nnz, d_ssd, s);
+    case 16:
+        return launch_recon<16>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres,
+                                d_rec, i_rec, d_nnz, d_ssd, s);
+    default:
+        return launch_recon<32>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres,
+                                d_rec, i_rec, d_nnz, d_ssd, s);
+    }
 }

@@ -134,6 +134,7 @@ def lib():
         L.xme_quant_batch.argtypes = [I, P, I, I, I, P, P]
         L.xme_dequant_batch.argtypes = [I, P, I, I, P]
         L.xme_residual_batch.argtypes = [I, I, I, P, P, P, P, I, P]
+        L.xme_inter_recon_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), I, I, P, I, P, P, P, P, Z, P, P, P]
         L.xme_get_sad_table.argtypes = [P]
         L.xme_get_satd_table.argtypes = [P]
         L.xme_get_sad_x3_table.argtypes = [P]
