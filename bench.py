@@ -303,6 +303,93 @@ def bipred_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, re
     return out
 
 
+def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, qp=32):
+    """Luma inter reconstruction of the frame (xme_inter_recon_batch): every
+    PU split into the largest square TUs (4..32) that tile it, predicted from
+    reference 0 at 4 x its full-pel best, at QP 32; one call per TU size.
+    Algorithmic HBM bytes: 5 per pixel (current and prediction read, levels
+    (int16) and reconstruction written); the kernels as composed move about 15
+    (prediction read twice, the prediction error and the reconstructed
+    residual through HBM).  Parity with the oracle on the TUs of every 4th PU."""
+    import ctypes as C
+    from oracle import oracle as O
+    r0 = gpu_res[0::n_ref]
+    by_n = {4: [], 8: [], 16: [], 32: []}
+    sample = {4: [], 8: [], 16: [], 32: []}
+    for i, (x, y, pw, ph) in enumerate(pus):
+        n = 32
+        while pw % n or ph % n:
+            n >>= 1
+        mv = (4 * int(r0["bmv2_x"][i]), 4 * int(r0["bmv2_y"][i]))
+        for ty in range(y, y + ph, n):
+            for tx in range(x, x + pw, n):
+                by_n[n].append((tx, ty, x, y, pw, ph, mv[0], mv[1]))
+                sample[n].append(i % 4 == 0)
+    rec = X.DevicePlane(w, h, pad=0, stride=dcur.stride)
+    rd = d_refs[0].desc()
+    stream = X.Stream()
+    bufs = []
+    for n, rows in by_n.items():
+        if not rows:
+            continue
+        t = np.array(rows, np.int32)
+        ntu = len(t)
+        bufs.append((n, ntu, X.upload(t), X.DeviceBuffer(2 * ntu * n * n), X.DeviceBuffer(2 * ntu * n * n),
+                     X.DeviceBuffer(2 * ntu * n * n), X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)))
+
+    def frame():
+        for n, ntu, d_t, resid, level, rres, nnz, ssd in bufs:
+            X.check(X.lib().xme_inter_recon_batch(n, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(rd), w, h,
+                                                  d_t.at(), ntu, resid.at(), level.at(), rres.at(),
+                                                  C.c_void_p(rec.origin), rec.stride, nnz.at(), ssd.at(), stream.h),
+                    "xme_inter_recon_batch")
+
+    for _ in range(3):
+        frame()
+    stream.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        frame()
+    stream.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    pixels = sum(ntu * n * n for n, ntu, *_ in bufs)
+    leg = {
+        "workload": f"{w}x{h} luma, {sum(b[1] for b in bufs)} TUs ("
+                    + ", ".join(f"{b[1]} of {b[0]}x{b[0]}" for b in bufs) + f"), reference 0, QP {qp}",
+        "ms_per_frame": ms,
+        "pixels_per_s": pixels / (ms * 1e-3),
+        "algorithmic_gb_per_s": 5.0 * pixels / (ms * 1e-3) / 1e9,
+        "moved_gb_per_s": 15.0 * pixels / (ms * 1e-3) / 1e9,
+    }
+    if cpu:
+        p = O.Plane(w, h)
+        p.set_picture(refs[0])
+        p.expand_border()
+        planes = O.interp_planes(p, 2)
+        cp = O.Plane(w, h)
+        cp.set_picture(cur)
+        full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)[:h, :w]
+        ok, el, n_tu = True, 0.0, 0
+        for n, ntu, d_t, resid, level, rres, nnz, ssd in bufs:
+            sel = np.array(sample[n])
+            t = np.array(by_n[n], np.int32)[sel]
+            t0 = time.perf_counter()
+            el_level, el_rec, el_nnz, el_ssd = O.inter_recon_batch(n, qp, cp, planes, w, h, t)
+            el += time.perf_counter() - t0
+            n_tu += len(t)
+            g_level = level.download(ntu * n * n, np.int16).reshape(ntu, n, n)[sel]
+            ok = ok and np.array_equal(g_level, el_level) and np.array_equal(nnz.download(ntu, np.int32)[sel], el_nnz)
+            ok = ok and np.array_equal(ssd.download(ntu, np.int32)[sel], el_ssd)
+            for (tx, ty, *_r) in t:
+                ok = ok and np.array_equal(full[ty:ty + n, tx:tx + n], el_rec[ty:ty + n, tx:tx + n])
+        leg["parity_with_oracle_every_4th_pu"] = bool(ok)
+        leg["cpu_oracle"] = {"pixels_per_s": sum(len(np.array(by_n[n])[np.array(sample[n])]) * n * n
+                                                 for n in by_n if by_n[n]) / el,
+                             "threads": 1, "seconds": el, "sample": f"the {n_tu} TUs of every 4th PU"}
+    log(f"recon: {ms:.3f} ms per frame")
+    return leg
+
+
 def transform_leg(X, blocks_total, reps, seed=91):
     """DCT -> flat quant -> dequant -> IDCT over blocks_total int16 blocks, a
     quarter each of 4x4, 8x8, 16x16 and 32x32 (Laplacian b=6 residuals clipped
@@ -513,6 +600,7 @@ def main():
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
     ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX / UMH pattern-search leg")
     ap.add_argument("--no-bipred", action="store_true", help="skip the bi-prediction (sym / bid) leg")
+    ap.add_argument("--no-recon", action="store_true", help="skip the inter-reconstruction leg")
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
@@ -624,6 +712,11 @@ def main():
         log("bi-prediction leg (sym, bid)")
         bipred = bipred_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, max(20, args.steps),
                             not args.no_cpu)
+    recon = None
+    if not args.no_recon:
+        log("inter-reconstruction leg")
+        recon = recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, max(20, args.steps),
+                          not args.no_cpu)
     transform = None
     if not args.no_transform:
         log(f"residual transform leg: {args.transform_blocks} blocks")
@@ -677,6 +770,7 @@ def main():
         "cpu_baseline": cpu,
         "pattern_search": pattern,
         "bipred": bipred,
+        "inter_recon": recon,
         "transform": transform,
         "serving": serving,
     }
