@@ -358,6 +358,14 @@ int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, con
                           int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
                           coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,
                           int32_t *d_ssd, void *stream);
+/* The same for TUs of bi-predicted PUs: the prediction is the rounded
+ * average (pixf.avg) of mc_luma from ref1 at the record's MV and from ref2 at
+ * d_mv2[2 i], d_mv2[2 i + 1], each clamped with the PU's size
+ * (rdo_get_pred_inter, rdo.c:1616-1631).  ref1 and ref2 share one stride. */
+int xme_inter_recon_bi_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref1,
+                             const xme_ref_desc *ref2, int frame_w, int frame_h, const int32_t *d_tus,
+                             const int32_t *d_mv2, int ntu, coeff_t *d_resid, coeff_t *d_level, coeff_t *d_rres,
+                             pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd, void *stream);
 
 /* ---- 5. serving: continuous-batching ME for many concurrent encodes ------ *
  * The frame-level ME of many encodes (streams) share one GPU.  Each stream

@@ -119,6 +119,8 @@ def lib():
         L.xo_get_mc_pos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_inter_recon_batch.argtypes = [C.c_int, C.c_int, P, ip, P, ip, C.c_int, C.c_int, P, C.c_int, P, P, ip,
                                            P, P]
+        L.xo_inter_recon_bi_batch.argtypes = [C.c_int, C.c_int, P, ip, P, P, ip, C.c_int, C.c_int, P, P, C.c_int, P,
+                                              P, ip, P, P]
         L.xo_tmat.restype = C.POINTER(C.c_int16)
         L.xo_tmat.argtypes = [C.c_int]
         _lib = L
@@ -326,9 +328,10 @@ def residual_batch(n, qp, b_intra, blocks):
     return level, recon, nnz
 
 
-def inter_recon_batch(n, qp, cur_plane, planes, frame_w, frame_h, tus):
+def inter_recon_batch(n, qp, cur_plane, planes, frame_w, frame_h, tus, planes2=None, mv2=None):
     """Luma inter reconstruction of n x n TUs; tus: int32 (ntu, 8) records
-    {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}.  Returns (level (ntu, n, n),
+    {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}; with planes2 and mv2
+    (int32 (ntu, 2)) the TUs are bi-predicted.  Returns (level (ntu, n, n),
     rec (frame_h, frame_w) picture with the TUs written, nnz, ssd)."""
     tus = np.ascontiguousarray(tus, dtype=np.int32).reshape(-1, 8)
     ntu = len(tus)
@@ -336,9 +339,15 @@ def inter_recon_batch(n, qp, cur_plane, planes, frame_w, frame_h, tus):
     rec = np.zeros((frame_h, frame_w), np.uint8)
     nnz = np.zeros(ntu, np.int32)
     ssd = np.zeros(ntu, np.int32)
-    lib().xo_inter_recon_batch(n, qp, cur_plane.origin_ptr(), cur_plane.stride, plane_ptrs(planes),
-                               planes[0].stride, frame_w, frame_h, ptr(tus), ntu, ptr(level), ptr(rec), frame_w,
-                               ptr(nnz), ptr(ssd))
+    if planes2 is None:
+        lib().xo_inter_recon_batch(n, qp, cur_plane.origin_ptr(), cur_plane.stride, plane_ptrs(planes),
+                                   planes[0].stride, frame_w, frame_h, ptr(tus), ntu, ptr(level), ptr(rec), frame_w,
+                                   ptr(nnz), ptr(ssd))
+    else:
+        m2 = np.ascontiguousarray(mv2, dtype=np.int32).reshape(-1, 2)
+        lib().xo_inter_recon_bi_batch(n, qp, cur_plane.origin_ptr(), cur_plane.stride, plane_ptrs(planes),
+                                      plane_ptrs(planes2), planes[0].stride, frame_w, frame_h, ptr(tus), ptr(m2), ntu,
+                                      ptr(level), ptr(rec), frame_w, ptr(nnz), ptr(ssd))
     return level, rec, nnz, ssd
 
 

@@ -1196,17 +1196,27 @@ int xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv)
     return blk_pos * 4 + mv;
 }
 
-void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
-                          intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
-                          uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd)
+static void inter_recon(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
+                        const uint8_t *const planes2[16], intptr_t i_ref, int frame_w, int frame_h,
+                        const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
+                        intptr_t i_rec, int32_t *nnz, int32_t *ssd)
 {
     for (int i = 0; i < ntu; i++) {
         const int32_t *r = tus + 8 * i;
-        /* the PU's prediction (get_mv_for_mc with the PU size), then the TU's part of it */
+        /* the PU's prediction (get_mv_for_mc with the PU size), then the TU's
+         * part of it; bi-prediction averages the second reference's
+         * (rdo_get_pred_inter, rdo.c:1616-1631) */
         uint8_t pred[64 * 64];
         const int qx = (int16_t)xo_get_mc_pos(frame_w, r[4], r[2], r[6]);
         const int qy = (int16_t)xo_get_mc_pos(frame_h, r[5], r[3], r[7]);
         xo_mc_luma(pred, 64, qx, qy, r[4], r[5], planes, i_ref);
+        if (planes2) {
+            uint8_t tmp[64 * 64];
+            const int qx2 = (int16_t)xo_get_mc_pos(frame_w, r[4], r[2], mv2[2 * i]);
+            const int qy2 = (int16_t)xo_get_mc_pos(frame_h, r[5], r[3], mv2[2 * i + 1]);
+            xo_mc_luma(tmp, 64, qx2, qy2, r[4], r[5], planes2, i_ref);
+            xo_avg(r[4], r[5], pred, 64, pred, 64, tmp, 64);
+        }
         const uint8_t *p = pred + (r[1] - r[3]) * 64 + (r[0] - r[2]);
         const uint8_t *c = cur + (intptr_t)r[1] * i_cur + r[0];
         int16_t res[32 * 32], rr[32 * 32];
@@ -1224,6 +1234,23 @@ void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, con
             }
         ssd[i] = e;
     }
+}
+
+void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
+                          intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
+                          uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd)
+{
+    inter_recon(n, qp, cur, i_cur, planes, NULL, i_ref, frame_w, frame_h, tus, NULL, ntu, level, rec, i_rec, nnz,
+                ssd);
+}
+
+void xo_inter_recon_bi_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes1[16],
+                             const uint8_t *const planes2[16], intptr_t i_ref, int frame_w, int frame_h,
+                             const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
+                             intptr_t i_rec, int32_t *nnz, int32_t *ssd)
+{
+    inter_recon(n, qp, cur, i_cur, planes1, planes2, i_ref, frame_w, frame_h, tus, mv2, ntu, level, rec, i_rec, nnz,
+                ssd);
 }
 
 /* mc_chroma, mc.c:797-829: luma qpel MV = chroma 1/8-pel position. */

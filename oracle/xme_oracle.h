@@ -223,6 +223,12 @@ int  xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int
 void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
                           intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
                           uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd);
+/* The same for bi-predicted PUs: the average (xo_avg) of the predictions of
+ * planes1 at the record's MV and planes2 at mv2[2 i], mv2[2 i + 1]. */
+void xo_inter_recon_bi_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes1[16],
+                             const uint8_t *const planes2[16], intptr_t i_ref, int frame_w, int frame_h,
+                             const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
+                             intptr_t i_rec, int32_t *nnz, int32_t *ssd);
 
 /* cu_get_mc_pos (common/mc.h:52-66). */
 int  xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv);

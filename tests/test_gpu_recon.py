@@ -74,3 +74,51 @@ def test_1080p_8x8():
     exp = O.inter_recon_batch(8, 32, oracle_cur(cur), oracle_ref(ref, 2), W, H, tus)
     for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
         assert np.array_equal(g, e), name
+
+
+def gpu_recon_bi(n, qp, cur, r1_img, r2_img, tus, mv2, frac):
+    h, w = cur.shape
+    d1, d2 = X.DeviceRef(w, h, frac=frac), X.DeviceRef(w, h, frac=frac)
+    d1.load(r1_img)
+    d2.load(r2_img)
+    dcur = device_cur(cur)
+    ntu = len(tus)
+    d_tus, d_mv2 = X.upload(np.ascontiguousarray(tus, np.int32)), X.upload(np.ascontiguousarray(mv2, np.int32))
+    resid, level, rres = (X.DeviceBuffer(2 * ntu * n * n) for _ in range(3))
+    rec = X.DevicePlane(w, h, pad=0, stride=dcur.stride)
+    rec.buf.zero()
+    nnz, ssd = X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)
+    a, b = d1.desc(), d2.desc()
+    X.check(X.lib().xme_inter_recon_bi_batch(n, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(a), C.byref(b),
+                                             w, h, d_tus.at(), d_mv2.at(), ntu, resid.at(), level.at(), rres.at(),
+                                             C.c_void_p(rec.origin), rec.stride, nnz.at(), ssd.at(), None),
+            "xme_inter_recon_bi_batch")
+    X.check(X.lib().xme_stream_sync(None), "sync")
+    full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
+    return (level.download(ntu * n * n, np.int16).reshape(ntu, n, n), full[:h, :w],
+            nnz.download(ntu, np.int32), ssd.download(ntu, np.int32))
+
+
+@pytest.mark.parametrize("frac", [2, 0])
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_bi_prediction_matches_oracle(n, frac):
+    w, h = 192, 128
+    r1 = synth.smooth_field(w, h, seed=31)
+    r2 = synth.smooth_field(w, h, seed=32)
+    cur = synth.multi_ref_current([r1, r2], 8, 8, seed=33)
+    rng = np.random.default_rng(200 + n)
+    tus = tiles(w, h, n, rng, far=True)
+    # one second MV per PU, shared by its TUs; some far outside the picture
+    mv2 = np.zeros((len(tus), 2), np.int32)
+    per_pu = {}
+    for i, t in enumerate(tus):
+        key = (int(t[2]), int(t[3]))
+        if key not in per_pu:
+            per_pu[key] = ((int(rng.integers(-60, 61)), int(rng.integers(-60, 61))) if rng.random() < 0.8 else
+                           (4 * int(rng.integers(-300, 300)) + 1, 4 * int(rng.integers(-300, 300)) + 2))
+        mv2[i] = per_pu[key]
+    got = gpu_recon_bi(n, 32, cur, r1, r2, tus, mv2, frac)
+    exp = O.inter_recon_batch(n, 32, oracle_cur(cur), oracle_ref(r1, 2), w, h, tus, oracle_ref(r2, 2), mv2)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    assert got[3].any()

@@ -98,3 +98,55 @@ def test_full_pel_plane_alone_gives_the_same_results():
     b = O.inter_recon_batch(8, 27, cur_plane(cur), only0, w, h, tus)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_bi_prediction_known_answer_and_reductions():
+    """Two references moved by d and e with MVs 4d and 4e average to the
+    picture (zero residual); the same reference and MV twice reduce to the
+    uni-predicted result; and the average is numpy's (a + b + 1) >> 1."""
+    w, h = 128, 96
+    cur = texture(w, h, 6)
+    d, e = (2, -3), (-1, 2)
+    p1 = planes_of(np.roll(cur, (d[1], d[0]), axis=(0, 1)))
+    p2 = planes_of(np.roll(cur, (e[1], e[0]), axis=(0, 1)))
+    tus = tiles(w, h, 8, np.random.default_rng(1))
+    keep = (tus[:, 2] >= 8) & (tus[:, 3] >= 8) & (tus[:, 2] + tus[:, 4] <= w - 8) & (tus[:, 3] + tus[:, 5] <= h - 8)
+    tus = tus[keep]
+    tus[:, 6], tus[:, 7] = 4 * d[0], 4 * d[1]
+    mv2 = np.tile(np.array([4 * e[0], 4 * e[1]], np.int32), (len(tus), 1))
+    level, rec, nnz, ssd = O.inter_recon_batch(8, 27, cur_plane(cur), p1, w, h, tus, p2, mv2)
+    assert not level.any() and not nnz.any() and not ssd.any()
+    # the same reference and MV twice
+    r1, r2 = texture(w, h, 7), texture(w, h, 8)
+    q1, q2 = planes_of(r1), planes_of(r2)
+    tus = tiles(w, h, 16, np.random.default_rng(2), far=True)
+    same = O.inter_recon_batch(16, 30, cur_plane(cur), q1, w, h, tus, q1, tus[:, 6:8])
+    uni = O.inter_recon_batch(16, 30, cur_plane(cur), q1, w, h, tus)
+    for a, b in zip(same, uni):
+        assert np.array_equal(a, b)
+    # numpy: the average of the two numpy predictions, then the pinned residual stages
+    mv2 = np.random.default_rng(3).integers(-50, 51, (len(tus), 2)).astype(np.int32)
+    got = O.inter_recon_batch(16, 30, cur_plane(cur), q1, w, h, tus, q2, mv2)
+    pad = O.PAD
+    pl1 = [npref.interp_plane_padded(r1, k, pad) for k in range(16)]
+    pl2 = [npref.interp_plane_padded(r2, k, pad) for k in range(16)]
+
+    def pred(pl, t, mx, my):
+        tx, ty, px, py, pw, ph = (int(v) for v in t[:6])
+        qx = O.lib().xo_get_mc_pos(w, pw, px, int(mx))
+        qy = O.lib().xo_get_mc_pos(h, ph, py, int(my))
+        x0, y0 = (qx >> 2) + (tx - px), (qy >> 2) + (ty - py)
+        return pl[(qy & 3) * 4 + (qx & 3)][pad + y0:pad + y0 + 16, pad + x0:pad + x0 + 16].astype(np.int32)
+
+    res = np.zeros((len(tus), 16, 16), np.int16)
+    preds = []
+    for i, t in enumerate(tus):
+        p = (pred(pl1, t, t[6], t[7]) + pred(pl2, t, mv2[i, 0], mv2[i, 1]) + 1) >> 1
+        preds.append(p)
+        res[i] = cur[t[1]:t[1] + 16, t[0]:t[0] + 16].astype(np.int32) - p
+    lv, rr, nz = O.residual_batch(16, 30, 0, res)
+    assert np.array_equal(got[0], lv) and np.array_equal(got[2], nz)
+    for i, t in enumerate(tus):
+        v = np.clip(preds[i] + rr[i].astype(np.int32), 0, 255)
+        assert np.array_equal(got[1][t[1]:t[1] + 16, t[0]:t[0] + 16], v)
+        assert got[3][i] == int(((cur[t[1]:t[1] + 16, t[0]:t[0] + 16].astype(np.int64) - v) ** 2).sum())

@@ -181,33 +181,55 @@ __device__ __forceinline__ LumaSrc tu_src(const Planes16c &pl, int64_t i_ref, co
     return luma_src(pl, i_ref, qx, qy);
 }
 
+// The second prediction of a bi-predicted TU: the PU's second MV mv2[2 b],
+// mv2[2 b + 1] into the second reference.
+__device__ __forceinline__ LumaSrc tu_src2(const Planes16c &pl, int64_t i_ref, const int32_t *r, const int32_t *mv2,
+                                           int fw, int fh)
+{
+    const int qx = mc_pos(fw, r[4], r[2], mv2[0]) + 4 * (r[0] - r[2]);
+    const int qy = mc_pos(fh, r[5], r[3], mv2[1]) + 4 * (r[1] - r[3]);
+    return luma_src(pl, i_ref, qx, qy);
+}
+
+// The TU's prediction pixel: mc_luma, or for BI the rounded average of the
+// two (pixf.avg, rdo.c:1624-1631).
+template <bool BI>
+__device__ __forceinline__ int pred_px(const LumaSrc &a, const LumaSrc &b, int x, int y)
+{
+    return BI ? (luma_px(a, x, y) + luma_px(b, x, y) + 1) >> 1 : luma_px(a, x, y);
+}
+
 // The prediction error cur - pred (sub_ps, pixel.c:780-792) into N x N int16
 // blocks, contiguous per TU.
-template <int N>
-__global__ void __launch_bounds__(256) recon_sub_kernel(Planes16c pl, int64_t i_ref, const uint8_t *__restrict__ cur,
-                                                        int64_t i_cur, const int32_t *__restrict__ tus, int ntu,
-                                                        int fw, int fh, int16_t *__restrict__ resid)
+template <int N, bool BI>
+__global__ void __launch_bounds__(256) recon_sub_kernel(Planes16c pl, Planes16c pl2, int64_t i_ref,
+                                                        const uint8_t *__restrict__ cur, int64_t i_cur,
+                                                        const int32_t *__restrict__ tus,
+                                                        const int32_t *__restrict__ mv2, int ntu, int fw, int fh,
+                                                        int16_t *__restrict__ resid)
 {
     const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6));
     if (b >= ntu)
         return;
     const int32_t *r = tus + 8 * b;
     const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
+    const LumaSrc src2 = BI ? tu_src2(pl2, i_ref, r, mv2 + 2 * b, fw, fh) : src;
     const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
     for (int t = threadIdx.x & 63; t < N * N; t += 64) {
         const int y = t / N, x = t % N;
-        resid[(int64_t)b * N * N + t] = (int16_t)((int)c[(int64_t)y * i_cur + x] - luma_px(src, x, y));
+        resid[(int64_t)b * N * N + t] = (int16_t)((int)c[(int64_t)y * i_cur + x] - pred_px<BI>(src, src2, x, y));
     }
 }
 
 // The reconstruction clip(pred + residual) (add_ps, pixel.c:794-806) into the
 // picture, and its SSD against cur (pixf.ssd, rdo.c:945).
-template <int N>
-__global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, int64_t i_ref, const uint8_t *__restrict__ cur,
-                                                        int64_t i_cur, const int32_t *__restrict__ tus, int ntu,
-                                                        int fw, int fh, const int16_t *__restrict__ rres,
-                                                        uint8_t *__restrict__ rec, int64_t i_rec,
-                                                        int32_t *__restrict__ ssd)
+template <int N, bool BI>
+__global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, Planes16c pl2, int64_t i_ref,
+                                                        const uint8_t *__restrict__ cur, int64_t i_cur,
+                                                        const int32_t *__restrict__ tus,
+                                                        const int32_t *__restrict__ mv2, int ntu, int fw, int fh,
+                                                        const int16_t *__restrict__ rres, uint8_t *__restrict__ rec,
+                                                        int64_t i_rec, int32_t *__restrict__ ssd)
 {
     const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6));
     if (b >= ntu)
@@ -215,12 +237,13 @@ __global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, int64_t i_
     const int lane = threadIdx.x & 63;
     const int32_t *r = tus + 8 * b;
     const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
+    const LumaSrc src2 = BI ? tu_src2(pl2, i_ref, r, mv2 + 2 * b, fw, fh) : src;
     const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
     uint8_t *d = rec + (int64_t)r[1] * i_rec + r[0];
     int e = 0;
     for (int t = lane; t < N * N; t += 64) {
         const int y = t / N, x = t % N;
-        const int v = clip_pel(luma_px(src, x, y) + rres[(int64_t)b * N * N + t]);
+        const int v = clip_pel(pred_px<BI>(src, src2, x, y) + rres[(int64_t)b * N * N + t]);
         d[(int64_t)y * i_rec + x] = (uint8_t)v;
         const int diff = (int)c[(int64_t)y * i_cur + x] - v;
         e += diff * diff;
@@ -230,20 +253,20 @@ __global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, int64_t i_
         ssd[b] = e;
 }
 
-template <int N>
-int launch_recon(int qp, const uint8_t *cur, int64_t i_cur, const Planes16c &pl, int64_t i_ref, int fw, int fh,
-                 const int32_t *tus, int ntu, int16_t *resid, int16_t *level, int16_t *rres, uint8_t *rec,
-                 int64_t i_rec, int32_t *nnz, int32_t *ssd, hipStream_t s)
+template <int N, bool BI>
+int launch_recon(int qp, const uint8_t *cur, int64_t i_cur, const Planes16c &pl, const Planes16c &pl2, int64_t i_ref,
+                 int fw, int fh, const int32_t *tus, const int32_t *mv2, int ntu, int16_t *resid, int16_t *level,
+                 int16_t *rres, uint8_t *rec, int64_t i_rec, int32_t *nnz, int32_t *ssd, hipStream_t s)
 {
     const int grid = (ntu + RC_WAVES - 1) / RC_WAVES;
-    hipLaunchKernelGGL(recon_sub_kernel<N>, dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, i_ref, cur, i_cur, tus, ntu, fw,
-                       fh, resid);
+    hipLaunchKernelGGL((recon_sub_kernel<N, BI>), dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, pl2, i_ref, cur, i_cur,
+                       tus, mv2, ntu, fw, fh, resid);
     XME_HIP(hipGetLastError());
     const int rc = xme_residual_batch(N, qp, 0, resid, level, rres, nnz, ntu, s);
     if (rc != XME_OK)
         return rc;
-    hipLaunchKernelGGL(recon_add_kernel<N>, dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, i_ref, cur, i_cur, tus, ntu, fw,
-                       fh, rres, rec, i_rec, ssd);
+    hipLaunchKernelGGL((recon_add_kernel<N, BI>), dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, pl2, i_ref, cur, i_cur,
+                       tus, mv2, ntu, fw, fh, rres, rec, i_rec, ssd);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }
@@ -284,39 +307,64 @@ extern "C" int xme_mc_luma_batch(const xme_ref_desc *ref, const int32_t *d_block
     return XME_OK;
 }
 
+namespace {
+
+template <bool BI>
+int recon_entry(const char *name, int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
+                const xme_ref_desc *ref2, int frame_w, int frame_h, const int32_t *d_tus, const int32_t *d_mv2,
+                int ntu, coeff_t *d_resid, coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec,
+                int32_t *d_nnz, int32_t *d_ssd, hipStream_t s)
+{
+    if (!ref || !ref->planes[0] || (BI && (!ref2 || !ref2->planes[0] || ref2->stride != ref->stride)) || ntu < 0 ||
+        (n != 4 && n != 8 && n != 16 && n != 32) || qp < 0 || qp > 79 || frame_w <= 0 || frame_h <= 0) {
+        xme_set_error("%s: bad arguments", name);
+        return XME_ERR_ARG;
+    }
+    if (ntu == 0)
+        return XME_OK;
+    if (!d_cur || !d_tus || (BI && !d_mv2) || !d_resid || !d_level || !d_rres || !d_rec || !d_nnz || !d_ssd) {
+        xme_set_error("%s: NULL buffer", name);
+        return XME_ERR_ARG;
+    }
+    Planes16c pl, pl2;
+    for (int k = 0; k < 16; k++) {
+        pl.p[k] = ref->planes[k];
+        pl2.p[k] = BI ? ref2->planes[k] : nullptr;
+    }
+    const int64_t ir = (int64_t)ref->stride;
+    switch (n) {
+    case 4:
+        return launch_recon<4, BI>(qp, d_cur, i_cur, pl, pl2, ir, frame_w, frame_h, d_tus, d_mv2, ntu, d_resid,
+                                   d_level, d_rres, d_rec, i_rec, d_nnz, d_ssd, s);
+    case 8:
+        return launch_recon<8, BI>(qp, d_cur, i_cur, pl, pl2, ir, frame_w, frame_h, d_tus, d_mv2, ntu, d_resid,
+                                   d_level, d_rres, d_rec, i_rec, d_nnz, d_ssd, s);
+    case 16:
+        return launch_recon<16, BI>(qp, d_cur, i_cur, pl, pl2, ir, frame_w, frame_h, d_tus, d_mv2, ntu, d_resid,
+                                    d_level, d_rres, d_rec, i_rec, d_nnz, d_ssd, s);
+    default:
+        return launch_recon<32, BI>(qp, d_cur, i_cur, pl, pl2, ir, frame_w, frame_h, d_tus, d_mv2, ntu, d_resid,
+                                    d_level, d_rres, d_rec, i_rec, d_nnz, d_ssd, s);
+    }
+}
+
+} // namespace
+
 extern "C" int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
                                      int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
                                      coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,
                                      int32_t *d_ssd, void *stream)
 {
-    if (!ref || !ref->planes[0] || ntu < 0 || (n != 4 && n != 8 && n != 16 && n != 32) || qp < 0 || qp > 79 ||
-        frame_w <= 0 || frame_h <= 0) {
-        xme_set_error("xme_inter_recon_batch: bad arguments");
-        return XME_ERR_ARG;
-    }
-    if (ntu == 0)
-        return XME_OK;
-    if (!d_cur || !d_tus || !d_resid || !d_level || !d_rres || !d_rec || !d_nnz || !d_ssd) {
-        xme_set_error("xme_inter_recon_batch: NULL buffer");
-        return XME_ERR_ARG;
-    }
-    Planes16c pl;
-    for (int k = 0; k < 16; k++)
-        pl.p[k] = ref->planes[k];
-    const hipStream_t s = (hipStream_t)stream;
-    const int64_t ir = (int64_t)ref->stride;
-    switch (n) {
-    case 4:
-        return launch_recon<4>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres, d_rec,
-                               i_rec, d_nnz, d_ssd, s);
-    case 8:
-        return launch_recon<8>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres, d_rec,
-                               i_rec, d_nnz, d_ssd, s);
-    case 16:
-        return launch_recon<16>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres,
-                                d_rec, i_rec, d_nnz, d_ssd, s);
-    default:
-        return launch_recon<32>(qp, d_cur, i_cur, pl, ir, frame_w, frame_h, d_tus, ntu, d_resid, d_level, d_rres,
-                                d_rec, i_rec, d_nnz, d_ssd, s);
-    }
+    return recon_entry<false>("xme_inter_recon_batch", n, qp, d_cur, i_cur, ref, nullptr, frame_w, frame_h, d_tus,
+                              nullptr, ntu, d_resid, d_level, d_rres, d_rec, i_rec, d_nnz, d_ssd, (hipStream_t)stream);
+}
+
+extern "C" int xme_inter_recon_bi_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref1,
+                                        const xme_ref_desc *ref2, int frame_w, int frame_h, const int32_t *d_tus,
+                                        const int32_t *d_mv2, int ntu, coeff_t *d_resid, coeff_t *d_level,
+                                        coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,
+                                        int32_t *d_ssd, void *stream)
+{
+    return recon_entry<true>("xme_inter_recon_bi_batch", n, qp, d_cur, i_cur, ref1, ref2, frame_w, frame_h, d_tus,
+                             d_mv2, ntu, d_resid, d_level, d_rres, d_rec, i_rec, d_nnz, d_ssd, (hipStream_t)stream);
 }

@@ -135,6 +135,8 @@ def lib():
         L.xme_dequant_batch.argtypes = [I, P, I, I, P]
         L.xme_residual_batch.argtypes = [I, I, I, P, P, P, P, I, P]
         L.xme_inter_recon_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), I, I, P, I, P, P, P, P, Z, P, P, P]
+        L.xme_inter_recon_bi_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), C.POINTER(RefDesc), I, I, P, P, I,
+                                               P, P, P, P, Z, P, P, P]
         L.xme_get_sad_table.argtypes = [P]
         L.xme_get_satd_table.argtypes = [P]
         L.xme_get_sad_x3_table.argtypes = [P]
