@@ -353,7 +353,11 @@ int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t 
  * d_tus: ntu x {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y} (MV in qpel,
  * the TU inside the PU).  d_resid (prediction error) and d_rres
  * (reconstructed residual) are ntu * n * n coefficients each; d_level holds
- * the quantised levels, d_nnz / d_ssd one value per TU. */
+ * the quantised levels, d_nnz / d_ssd one value per TU.  A TU outside the
+ * picture, not inside its PU, at a column that is not a multiple of 4, or
+ * whose PU is not 4..64 on each side gets ssd -1, zero levels and no pixels.
+ * d_cur / d_rec: 4-byte aligned with strides % 4 == 0; d_resid / d_rres:
+ * 8-byte aligned. */
 int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
                           int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
                           coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,

@@ -122,3 +122,22 @@ def test_bi_prediction_matches_oracle(n, frac):
     for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
         assert np.array_equal(g, e), name
     assert got[3].any()
+
+
+def test_tus_outside_their_limits_are_skipped():
+    w, h = 64, 64
+    ref = synth.smooth_field(w, h, seed=41)
+    cur = synth.multi_ref_current([ref], 4, 4, seed=42)
+    good = (16, 16, 16, 16, 16, 16, 4, -4)
+    bad = [(18, 16, 16, 16, 16, 16, 0, 0),      # column not a multiple of 4
+           (56, 16, 16, 16, 8, 8, 0, 0),        # TU not inside its PU
+           (60, 56, 56, 56, 16, 16, 0, 0),      # beyond the picture's right edge
+           (0, 0, 0, 0, 128, 8, 0, 0)]          # PU wider than 64
+    tus = np.array([good] + bad, np.int32)
+    level, rec, nnz, ssd = gpu_recon(8, 30, cur, ref, tus, 2)
+    assert ssd[0] >= 0 and list(ssd[1:]) == [-1, -1, -1, -1]
+    assert not level[1:].any() and not nnz[1:].any()
+    exp = O.inter_recon_batch(8, 30, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus[:1])
+    assert np.array_equal(level[0], exp[0][0]) and ssd[0] == exp[3][0]
+    # only the good TU's pixels are written
+    assert np.array_equal(rec, exp[1])

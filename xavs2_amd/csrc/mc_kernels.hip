@@ -199,8 +199,52 @@ __device__ __forceinline__ int pred_px(const LumaSrc &a, const LumaSrc &b, int x
     return BI ? (luma_px(a, x, y) + luma_px(b, x, y) + 1) >> 1 : luma_px(a, x, y);
 }
 
+// Four prediction pixels (x .. x + 3, y) of the TU, x % 4 == 0, packed in a
+// dword: unaligned dword reads of the planes when they exist (the dword after
+// the aligned one stays inside the XME_PAD border), else the block filters.
+template <bool BI>
+__device__ __forceinline__ uint32_t pred4(const LumaSrc &a, const LumaSrc &b, int x, int y)
+{
+    auto plane4 = [](const LumaSrc &s, int x, int y) {
+        const uintptr_t p = (uintptr_t)(s.plane + (int64_t)(s.y0 + y) * s.i_ref + s.x0 + x);
+        const uint32_t *q = (const uint32_t *)(p & ~(uintptr_t)3);
+        return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(p & 3));
+    };
+    if (a.plane != nullptr && (!BI || b.plane != nullptr)) {
+        const uint32_t u = plane4(a, x, y);
+        if (!BI)
+            return u;
+        const uint32_t v = plane4(b, x, y);
+        return (u | v) - (((u ^ v) >> 1) & 0x7f7f7f7fu);   // (u + v + 1) >> 1 per byte
+    }
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        r |= (uint32_t)pred_px<BI>(a, b, x + k, y) << (8 * k);
+    return r;
+}
+
+// A TU the kernels reconstruct: inside the picture at a column that is a
+// multiple of 4, inside its PU, whose sides are 4 to 64 (the clamped
+// prediction then stays inside the XME_PAD border).  Others get SSD -1, zero
+// levels and no pixels.
+template <int N>
+__device__ __forceinline__ bool tu_ok(const int32_t *r, int fw, int fh)
+{
+    return !(r[0] & 3) && r[0] >= 0 && r[1] >= 0 && r[0] + N <= fw && r[1] + N <= fh && r[4] >= 4 && r[4] <= 64 &&
+           r[5] >= 4 && r[5] <= 64 && r[0] >= r[2] && r[1] >= r[3] && r[0] + N <= r[2] + r[4] &&
+           r[1] + N <= r[3] + r[5];
+}
+
+// Lanes per TU (P, a power of two: one per 4 pixels, at most 64) and TUs per
+// wave for N x N TUs.
+template <int N>
+struct RcShape {
+    static constexpr int Q = N * N / 4, P = Q < 64 ? Q : 64, TPW = 64 / P;
+};
+
 // The prediction error cur - pred (sub_ps, pixel.c:780-792) into N x N int16
-// blocks, contiguous per TU.
+// blocks, contiguous per TU.  A lane takes 4 pixels of a row at a time.
 template <int N, bool BI>
 __global__ void __launch_bounds__(256) recon_sub_kernel(Planes16c pl, Planes16c pl2, int64_t i_ref,
                                                         const uint8_t *__restrict__ cur, int64_t i_cur,
@@ -208,21 +252,39 @@ __global__ void __launch_bounds__(256) recon_sub_kernel(Planes16c pl, Planes16c 
                                                         const int32_t *__restrict__ mv2, int ntu, int fw, int fh,
                                                         int16_t *__restrict__ resid)
 {
-    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6));
+    using S = RcShape<N>;
+    const int lane = threadIdx.x & 63;
+    const int b = ((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6)) * S::TPW + lane / S::P;
     if (b >= ntu)
         return;
     const int32_t *r = tus + 8 * b;
+    uint32_t *out = reinterpret_cast<uint32_t *>(resid + (int64_t)b * N * N);
+    if (!tu_ok<N>(r, fw, fh)) {
+        for (int q = lane % S::P; q < S::Q; q += S::P) {
+            out[2 * q] = 0u;
+            out[2 * q + 1] = 0u;
+        }
+        return;
+    }
     const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
     const LumaSrc src2 = BI ? tu_src2(pl2, i_ref, r, mv2 + 2 * b, fw, fh) : src;
     const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
-    for (int t = threadIdx.x & 63; t < N * N; t += 64) {
-        const int y = t / N, x = t % N;
-        resid[(int64_t)b * N * N + t] = (int16_t)((int)c[(int64_t)y * i_cur + x] - pred_px<BI>(src, src2, x, y));
+    for (int q = lane % S::P; q < S::Q; q += S::P) {
+        const int y = q / (N / 4), x = (q % (N / 4)) * 4;
+        const uint32_t cv = *reinterpret_cast<const uint32_t *>(c + (int64_t)y * i_cur + x);
+        const uint32_t pv = pred4<BI>(src, src2, x, y);
+        int d[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            d[k] = (int)((cv >> (8 * k)) & 0xff) - (int)((pv >> (8 * k)) & 0xff);
+        out[(y * N + x) / 2] = ((uint32_t)d[0] & 0xffffu) | ((uint32_t)d[1] << 16);
+        out[(y * N + x) / 2 + 1] = ((uint32_t)d[2] & 0xffffu) | ((uint32_t)d[3] << 16);
     }
 }
 
 // The reconstruction clip(pred + residual) (add_ps, pixel.c:794-806) into the
-// picture, and its SSD against cur (pixf.ssd, rdo.c:945).
+// picture, and its SSD against cur (pixf.ssd, rdo.c:945), summed over the
+// TU's P lanes.
 template <int N, bool BI>
 __global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, Planes16c pl2, int64_t i_ref,
                                                         const uint8_t *__restrict__ cur, int64_t i_cur,
@@ -231,26 +293,39 @@ __global__ void __launch_bounds__(256) recon_add_kernel(Planes16c pl, Planes16c 
                                                         const int16_t *__restrict__ rres, uint8_t *__restrict__ rec,
                                                         int64_t i_rec, int32_t *__restrict__ ssd)
 {
-    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6));
-    if (b >= ntu)
-        return;
+    using S = RcShape<N>;
     const int lane = threadIdx.x & 63;
-    const int32_t *r = tus + 8 * b;
-    const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
-    const LumaSrc src2 = BI ? tu_src2(pl2, i_ref, r, mv2 + 2 * b, fw, fh) : src;
-    const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
-    uint8_t *d = rec + (int64_t)r[1] * i_rec + r[0];
+    const int b = ((int)blockIdx.x * RC_WAVES + (int)(threadIdx.x >> 6)) * S::TPW + lane / S::P;
     int e = 0;
-    for (int t = lane; t < N * N; t += 64) {
-        const int y = t / N, x = t % N;
-        const int v = clip_pel(pred_px<BI>(src, src2, x, y) + rres[(int64_t)b * N * N + t]);
-        d[(int64_t)y * i_rec + x] = (uint8_t)v;
-        const int diff = (int)c[(int64_t)y * i_cur + x] - v;
-        e += diff * diff;
+    const bool ok = b < ntu && tu_ok<N>(tus + 8 * b, fw, fh);
+    if (ok) {
+        const int32_t *r = tus + 8 * b;
+        const LumaSrc src = tu_src(pl, i_ref, r, fw, fh);
+        const LumaSrc src2 = BI ? tu_src2(pl2, i_ref, r, mv2 + 2 * b, fw, fh) : src;
+        const uint8_t *c = cur + (int64_t)r[1] * i_cur + r[0];
+        uint8_t *d = rec + (int64_t)r[1] * i_rec + r[0];
+        const uint32_t *rr = reinterpret_cast<const uint32_t *>(rres + (int64_t)b * N * N);
+        for (int q = lane % S::P; q < S::Q; q += S::P) {
+            const int y = q / (N / 4), x = (q % (N / 4)) * 4;
+            const uint32_t pv = pred4<BI>(src, src2, x, y);
+            const uint32_t cv = *reinterpret_cast<const uint32_t *>(c + (int64_t)y * i_cur + x);
+            const uint32_t r0 = rr[(y * N + x) / 2], r1 = rr[(y * N + x) / 2 + 1];
+            const int res[4] = { (int)(int16_t)(r0 & 0xffffu), (int)(int16_t)(r0 >> 16), (int)(int16_t)(r1 & 0xffffu),
+                                 (int)(int16_t)(r1 >> 16) };
+            uint32_t o = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int v = clip_pel((int)((pv >> (8 * k)) & 0xff) + res[k]);
+                o |= (uint32_t)v << (8 * k);
+                const int diff = (int)((cv >> (8 * k)) & 0xff) - v;
+                e += diff * diff;
+            }
+            *reinterpret_cast<uint32_t *>(d + (int64_t)y * i_rec + x) = o;
+        }
     }
-    e = wave_sum(e);
-    if (lane == 0)
-        ssd[b] = e;
+    e = group_sum(e, S::P);
+    if (b < ntu && lane % S::P == S::P - 1)
+        ssd[b] = ok ? e : -1;
 }
 
 template <int N, bool BI>
@@ -258,7 +333,9 @@ int launch_recon(int qp, const uint8_t *cur, int64_t i_cur, const Planes16c &pl,
                  int fw, int fh, const int32_t *tus, const int32_t *mv2, int ntu, int16_t *resid, int16_t *level,
                  int16_t *rres, uint8_t *rec, int64_t i_rec, int32_t *nnz, int32_t *ssd, hipStream_t s)
 {
-    const int grid = (ntu + RC_WAVES - 1) / RC_WAVES;
+    // one wave per group of TPW TUs
+    const int waves = (ntu + RcShape<N>::TPW - 1) / RcShape<N>::TPW;
+    const int grid = (waves + RC_WAVES - 1) / RC_WAVES;
     hipLaunchKernelGGL((recon_sub_kernel<N, BI>), dim3(grid), dim3(64 * RC_WAVES), 0, s, pl, pl2, i_ref, cur, i_cur,
                        tus, mv2, ntu, fw, fh, resid);
     XME_HIP(hipGetLastError());
@@ -324,6 +401,13 @@ int recon_entry(const char *name, int n, int qp, const pel_t *d_cur, intptr_t i_
         return XME_OK;
     if (!d_cur || !d_tus || (BI && !d_mv2) || !d_resid || !d_level || !d_rres || !d_rec || !d_nnz || !d_ssd) {
         xme_set_error("%s: NULL buffer", name);
+        return XME_ERR_ARG;
+    }
+    // four pixels per access: the current and reconstructed pictures are read
+    // and written as aligned dwords
+    if (((uintptr_t)d_cur & 3) || (i_cur & 3) || ((uintptr_t)d_rec & 3) || (i_rec & 3) || ((uintptr_t)d_resid & 7) ||
+        ((uintptr_t)d_rres & 7)) {
+        xme_set_error("%s: pictures must be 4-byte and coefficient buffers 8-byte aligned", name);
         return XME_ERR_ARG;
     }
     Planes16c pl, pl2;
