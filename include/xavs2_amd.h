@@ -311,6 +311,31 @@ int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_job *d_jobs,
                             intptr_t i_cur, const xme_ref_desc *ref1, const xme_ref_desc *ref2,
                             xme_sym_result *d_results, void *stream);
 
+/* MV predictors, get_mvp_default (source/encoder/md_inter.c:470-524), for a
+ * batch of PUs of one slice, frame sequences.  Per PU: the searched
+ * reference ref_idx and list bwd_2nd, the PU inside its CU (p_cb), and the
+ * left, top, top-left and top-right neighbours' ref_idx[bwd_2nd]
+ * (-1 = INVALID_REF) and mv[bwd_2nd]; the top-left stands in when the
+ * top-right is unavailable.  ref_dpoc / ref_dpoc_multi are
+ * h->fdec->ref_dpoc / ref_dpoc_multi (encoder.c:1795-1810).  Status 2: a
+ * reference index outside 0..XME_MAX_REFS-1; 3: a scaling product that
+ * leaves int (undefined in the reference; the clipped value is returned). */
+typedef struct {
+    int8_t ref_idx, bwd_2nd, ur_available, reserved;
+    int8_t cb_x, cb_y;               /* p_cb->x, y: the PU inside its CU */
+    uint8_t cb_w, cb_h;              /* p_cb->w, h */
+    int8_t ref[4];                   /* left, top, top-left, top-right */
+    int16_t mv[4][2];
+} xme_mvp_job;
+
+typedef struct {
+    int16_t mvp_x, mvp_y;
+    int32_t status;
+} xme_mvp_result;
+
+int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, const int32_t ref_dpoc[XME_MAX_REFS],
+                      const int32_t ref_dpoc_multi[XME_MAX_REFS], xme_mvp_result *d_out, void *stream);
+
 /* ---- 4. reference planes, MC and the residual path ---------------------- */
 /* plane_expand_border over a whole plane (source/common/frame.c:470-507). */
 int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, int pad, void *stream);

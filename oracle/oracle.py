@@ -49,6 +49,12 @@ class BidJob(C.Structure):
                                        "mvp1_x", "mvp1_y", "mvp2_x", "mvp2_y")]
 
 
+class MvpJob(C.Structure):
+    _fields_ = [("ref_idx", C.c_int), ("bwd_2nd", C.c_int), ("cb_x", C.c_int), ("cb_y", C.c_int),
+                ("cb_w", C.c_int), ("cb_h", C.c_int), ("ur_available", C.c_int), ("ref", C.c_int * 4),
+                ("mv", (C.c_int * 2) * 4)]
+
+
 class Pu(C.Structure):
     _fields_ = [("pix_x", C.c_int), ("pix_y", C.c_int), ("w", C.c_int), ("h", C.c_int),
                 ("mvp_x", C.c_int), ("mvp_y", C.c_int)]
@@ -117,6 +123,7 @@ def lib():
         L.xo_dequant_params.argtypes = [C.c_int, C.c_int, P, P]
         L.xo_residual_batch.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
         L.xo_get_mc_pos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_get_mvp_default.argtypes = [C.POINTER(MvpJob), C.c_int, P, P, P, P]
         L.xo_inter_recon_batch.argtypes = [C.c_int, C.c_int, P, ip, P, ip, C.c_int, C.c_int, P, C.c_int, P, P, ip,
                                            P, P]
         L.xo_inter_recon_bi_batch.argtypes = [C.c_int, C.c_int, P, ip, P, P, ip, C.c_int, C.c_int, P, P, C.c_int, P,
@@ -267,6 +274,24 @@ def me_search_bid(prm, job, cur_plane, planes1, planes2):
     lib().xo_me_search_bid(C.byref(prm), C.byref(j), cur_plane.origin_ptr(), cur_plane.stride,
                            plane_ptrs(planes1), plane_ptrs(planes2), planes1[0].stride, C.byref(r))
     return r.cost, r.mv_x, r.mv_y, r.mvcost
+
+
+def get_mvp_default(job, b_slice, ref_dpoc, ref_dpoc_multi):
+    """get_mvp_default for one PU: job has ref_idx, bwd_2nd, cb (x, y, w, h),
+    ur_available, ref (4 neighbour ref_idx: left, top, top-left, top-right)
+    and mv (4 neighbour MVs).  Returns (status, pmv_x, pmv_y)."""
+    j = MvpJob()
+    j.ref_idx, j.bwd_2nd = job["ref_idx"], job["bwd_2nd"]
+    j.cb_x, j.cb_y, j.cb_w, j.cb_h = job["cb"]
+    j.ur_available = job["ur_available"]
+    for k in range(4):
+        j.ref[k] = job["ref"][k]
+        j.mv[k][0], j.mv[k][1] = job["mv"][k]
+    d = np.ascontiguousarray(ref_dpoc, dtype=np.int32)
+    m = np.ascontiguousarray(ref_dpoc_multi, dtype=np.int32)
+    x, y = C.c_int(), C.c_int()
+    rc = lib().xo_get_mvp_default(C.byref(j), b_slice, ptr(d), ptr(m), C.byref(x), C.byref(y))
+    return rc, x.value, y.value
 
 
 def scale_mv_skip(mv, dist_dst, dist_src):

@@ -1253,6 +1253,91 @@ void xo_inter_recon_bi_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, 
                 ssd);
 }
 
+/* ------------------------------------------------------------------------ */
+/* MV predictor (get_mvp_default, md_inter.c:470-524).                       */
+
+/* get_mvp_type_default (md_inter.c:333-368): 0 median, 1 left, 2 top, 3 top-right. */
+static int mvp_type(int ref_frame, int rl, int ru, int rur, int cb_x, int cb_y, int cb_w, int cb_h)
+{
+    if (rl != -1 && ru == -1 && rur == -1)
+        return 1;
+    if (rl == -1 && ru != -1 && rur == -1)
+        return 2;
+    if (rl == -1 && ru == -1 && rur != -1)
+        return 3;
+    if (cb_w < cb_h)
+        return cb_x == 0 ? (rl == ref_frame ? 1 : 0) : (rur == ref_frame ? 3 : 0);
+    if (cb_w > cb_h)
+        return cb_y == 0 ? (ru == ref_frame ? 2 : 0) : (rl == ref_frame ? 1 : 0);
+    return 0;
+}
+
+/* scale_mv_default (md_inter.c:405-410); -1 in *bad where the product leaves int. */
+static int scale_default(int mv, int dist_dst, int dist_src_scale, int *bad)
+{
+    const int64_t prod = (int64_t)iabs(mv) * dist_dst * dist_src_scale;
+    if (prod + 8192 > INT32_MAX || prod < 0)
+        *bad = 1;
+    const int sign = mv > 0 ? 1 : (mv < 0 ? -1 : 0);
+    const int v = sign * (int)((prod + 8192) >> 14);
+    return v < -32768 ? -32768 : (v > 32767 ? 32767 : v);
+}
+
+/* derive_median_mv (md_inter.c:372-401): C division truncates toward zero. */
+static int median_mv(int a, int b, int c)
+{
+    if ((a < 0 && b > 0 && c > 0) || (a > 0 && b < 0 && c < 0))
+        return (int16_t)((b + c) / 2);
+    if ((b < 0 && a > 0 && c > 0) || (b > 0 && a < 0 && c < 0))
+        return (int16_t)((c + a) / 2);
+    if ((c < 0 && a > 0 && b > 0) || (c > 0 && a < 0 && b < 0))
+        return (int16_t)((a + b) / 2);
+    const int dab = iabs(a - b), dbc = iabs(b - c), dca = iabs(c - a);
+    const int m = dab < dbc ? (dab < dca ? dab : dca) : (dbc < dca ? dbc : dca);
+    if (m == dab)
+        return (int16_t)((a + b) / 2);
+    if (m == dbc)
+        return (int16_t)((b + c) / 2);
+    return (int16_t)((c + a) / 2);
+}
+
+int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[8], const int ref_dpoc_multi[8],
+                       int *pmv_x, int *pmv_y)
+{
+    /* neighbours: 0 left (a), 1 top (b), 3 top-right (c), or top-left (2) when
+     * the top-right is unavailable */
+    const int rl = j->ref[0], ru = j->ref[1], rur = j->ur_available ? j->ref[3] : j->ref[2];
+    int mv[3][2] = { { j->mv[0][0], j->mv[0][1] }, { j->mv[1][0], j->mv[1][1] },
+                     { j->ur_available ? j->mv[3][0] : j->mv[2][0], j->ur_available ? j->mv[3][1] : j->mv[2][1] } };
+    const int refs[3] = { rl, ru, rur };
+    const int type = mvp_type(j->ref_idx, rl, ru, rur, j->cb_x, j->cb_y, j->cb_w, j->cb_h);
+    int bad = 0;
+    for (int k = 0; k < 3; k++) {
+        if (refs[k] < 0) {
+            mv[k][0] = mv[k][1] = 0;
+            continue;
+        }
+        int dist, scale;
+        if (b_slice) {                    /* B_BWD = 0, B_FWD = 1 (avs2_defs.h:138-139) */
+            dist = ref_dpoc[j->bwd_2nd ? 0 : 1];
+            scale = ref_dpoc_multi[j->bwd_2nd ? 0 : 1];
+        } else {
+            dist = ref_dpoc[j->ref_idx];
+            scale = ref_dpoc_multi[refs[k]];
+        }
+        mv[k][1] = scale_default(mv[k][1], dist, scale, &bad);
+        mv[k][0] = scale_default(mv[k][0], dist, scale, &bad);
+    }
+    if (type == 0) {
+        *pmv_x = median_mv(mv[0][0], mv[1][0], mv[2][0]);
+        *pmv_y = median_mv(mv[0][1], mv[1][1], mv[2][1]);
+    } else {
+        *pmv_x = mv[type - 1][0];
+        *pmv_y = mv[type - 1][1];
+    }
+    return bad ? -1 : 0;
+}
+
 /* mc_chroma, mc.c:797-829: luma qpel MV = chroma 1/8-pel position. */
 void xo_mc_chroma(uint8_t *dst_u, uint8_t *dst_v, intptr_t i_dst, int qx, int qy, int w, int h,
                   const uint8_t *plane_u, const uint8_t *plane_v, intptr_t i_src)

@@ -230,6 +230,24 @@ void xo_inter_recon_bi_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, 
                              const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
                              intptr_t i_rec, int32_t *nnz, int32_t *ssd);
 
+/* MV predictor, get_mvp_default (encoder/md_inter.c:470-524), for frame
+ * sequences: the neighbours left, top, top-left and top-right (index 0..3)
+ * of list bwd_2nd, the MVP type rule of get_mvp_type_default (:333-368),
+ * the neighbour scaling scale_mv_default (:405-410; B slices use the list's
+ * distances, other slices the searched reference's and each neighbour's),
+ * then the median rule (derive_median_mv, :372-401) or one neighbour.
+ * Returns 0, or -1 where the scaling product leaves int (undefined in the
+ * reference). */
+typedef struct {
+    int ref_idx, bwd_2nd;
+    int cb_x, cb_y, cb_w, cb_h;       /* p_cb: the PU inside its CU */
+    int ur_available;
+    int ref[4];                       /* neighbours' ref_idx[bwd_2nd], -1 = INVALID_REF */
+    int mv[4][2];                     /* neighbours' mv[bwd_2nd] */
+} xo_mvp_job_t;
+int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[8], const int ref_dpoc_multi[8],
+                       int *pmv_x, int *pmv_y);
+
 /* cu_get_mc_pos (common/mc.h:52-66). */
 int  xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv);
 /* Block MC (mc.c:275-398, :559-589) and chroma MC (mc.c:797-829).

@@ -37,7 +37,7 @@ def test_header_declares_the_boundary():
                  "xme_mc_luma_batch", "xme_mc_chroma_batch", "xme_dct_batch", "xme_idct_batch", "xme_quant_batch",
                  "xme_dequant_batch", "xme_residual_batch", "xme_server_create", "xme_server_submit",
                  "xme_server_wait", "xme_me_search_sym_batch", "xme_me_search_bid_batch",
-                 "xme_inter_recon_batch", "xme_inter_recon_bi_batch"):
+                 "xme_inter_recon_batch", "xme_inter_recon_bi_batch", "xme_get_mvp_batch"):
         assert must in names
 
 
@@ -70,6 +70,7 @@ def test_no_cpu_fallback_in_the_product():
 LAYOUTS = {
     "xme_pu": "PU_DTYPE", "xme_result": "RESULT_DTYPE", "xme_umh_ctx": "UMH_CTX_DTYPE",
     "xme_sym_job": "SYM_JOB_DTYPE", "xme_sym_result": "SYM_RESULT_DTYPE", "xme_bid_job": "BID_JOB_DTYPE",
+    "xme_mvp_job": "MVP_JOB_DTYPE", "xme_mvp_result": "MVP_RESULT_DTYPE",
 }
 
 

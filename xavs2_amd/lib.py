@@ -57,6 +57,10 @@ SYM_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), 
 BID_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
                           ("fwd_x", "<i2"), ("fwd_y", "<i2"), ("bwd_x", "<i2"), ("bwd_y", "<i2"),
                           ("mvp1_x", "<i2"), ("mvp1_y", "<i2"), ("mvp2_x", "<i2"), ("mvp2_y", "<i2")])
+MVP_JOB_DTYPE = np.dtype([("ref_idx", "i1"), ("bwd_2nd", "i1"), ("ur_available", "i1"), ("reserved", "i1"),
+                          ("cb_x", "i1"), ("cb_y", "i1"), ("cb_w", "u1"), ("cb_h", "u1"), ("ref", "i1", (4,)),
+                          ("mv", "<i2", (4, 2))])
+MVP_RESULT_DTYPE = np.dtype([("mvp_x", "<i2"), ("mvp_y", "<i2"), ("status", "<i4")])
 SYM_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvcost", "<i4"),
                              ("status", "<i4")])
 
@@ -134,6 +138,7 @@ def lib():
         L.xme_quant_batch.argtypes = [I, P, I, I, I, P, P]
         L.xme_dequant_batch.argtypes = [I, P, I, I, P]
         L.xme_residual_batch.argtypes = [I, I, I, P, P, P, P, I, P]
+        L.xme_get_mvp_batch.argtypes = [P, I, I, P, P, P, P]
         L.xme_inter_recon_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), I, I, P, I, P, P, P, P, Z, P, P, P]
         L.xme_inter_recon_bi_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), C.POINTER(RefDesc), I, I, P, P, I,
                                                P, P, P, P, Z, P, P, P]
