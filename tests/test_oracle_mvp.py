@@ -67,9 +67,13 @@ def test_rectangular_pu_falls_back_to_the_median():
 
 
 @pytest.mark.parametrize("a,b,c,expected", [
-    (-4, 6, 10, 8),      # a alone negative: (b + c) / 2
-    (3, -5, 7, 5),       # b alone negative: (c + a) / 2
-    (3, 5, -9, 4),       # c alone negative: (a + b) / 2
+    # each sign rule against a closest pair that would give 0
+    (-1, 20, 2, 11),     # a alone negative: (b + c) / 2
+    (1, -20, -2, -11),   # a alone positive
+    (20, -1, 2, 11),     # b alone negative: (c + a) / 2
+    (-20, 1, -2, -11),   # b alone positive
+    (20, 2, -1, 11),     # c alone negative: (a + b) / 2
+    (-20, -2, 1, -11),   # c alone positive
     (1, 10, 4, 2),       # closest pair c, a: (4 + 1) / 2 truncates to 2
     (0, 2, 4, 1),        # tie |a-b| = |b-c|: a, b wins
     (-3, 0, -10, -1),    # (a + b) / 2 = -3 / 2 truncates toward zero
@@ -113,3 +117,23 @@ def test_scaling_overflow_is_flagged():
     rc, _, _ = O.get_mvp_default(job(ref_idx=0, refs=(1, -1, -1, -1), mvs=((30000, 0), U, UL, UR)), 0,
                                  [512] * 8, [16384] * 8)
     assert rc == -1
+
+
+@pytest.mark.parametrize("v,expected", [(5, 3), (-5, -3), (3, 2), (-3, -2), (1, 1), (-1, -1)])
+def test_scaling_rounds_exact_halves_away_from_zero(v, expected):
+    # target ref 0 at distance 1, left neighbour on ref 1 (multi 8192): the scaled
+    # value is v / 2, an exact half for odd v; sign * ((|v| * 8192 + 8192) >> 14)
+    # rounds it away from zero, where a floor of the signed product would not
+    dpoc = [1, 2, 3, 4, 5, 6, 7, 8]
+    multi = [16384, 8192, 5461, 4096, 3276, 2730, 2340, 2048]
+    got = mvp(job(ref_idx=0, refs=(1, -1, -1, -1), mvs=((v, -v), U, UL, UR)), 0, dpoc, multi)
+    assert got == (expected, -expected)
+
+
+def test_left_with_a_valid_top_right_is_a_median():
+    # left and top-right valid, top invalid: none of the single-neighbour rules
+    # applies, so a square PU takes the median of (L, (0, 0), UR)
+    got = mvp(job(refs=(0, -1, -1, 0), mvs=(L, U, UL, UR)))
+    # x: 4, 0, 7: |a-b| 4, |b-c| 7, |c-a| 3 -> (7 + 4) / 2 = 5
+    # y: -8, 0, -3: |a-b| 8, |b-c| 3, |c-a| 5 -> (0 - 3) / 2 = -1
+    assert got == (5, -1)
