@@ -31,6 +31,8 @@ SPIRAL = [(0, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), 
     (6, 1, 2, 3),        # (6*8192 + 8192) >> 14 = 57344 >> 14 = 3
     (1, 1, 2, 1),        # (8192 + 8192) >> 14 = 16384 >> 14 = 1
     (-1, 1, 2, 0),       # (-8192 + 8192) >> 14 = 0
+    (3, 1, 6, 0),        # 16384 / 6 truncates to 2730: (8190 + 8192) >> 14 = 0 (exact 1/2 would round to 1)
+    (2053, 1, 5, 410),   # 16384 / 5 = 3276: (6725628 + 8192) >> 14 = 410 (exact division: 411)
 ])
 def test_scale_mv_skip_hand_values(mv, dd, ds, expected):
     assert O.scale_mv_skip(mv, dd, ds) == expected
@@ -186,3 +188,18 @@ def test_matches_numpy_on_random_textures(had, seed):
                 dist=(int(rng.integers(1, 5)), int(rng.integers(1, 5))), b_slice=int(rng.integers(0, 2)))
         cost, mx, my, _ = O.me_search_sym(prm, j, cur_plane(cur), p1, p2)
         assert (cost, mx, my) == numpy_sym(dict(use_hadamard=had), j, cur, r1, r2), j
+
+
+def test_mirror_straddling_the_window():
+    """A PU at the left edge in a B slice with distances 1 and 8: the mirror of
+    a candidate m is -8m, and the window starts at -256 qpel, so the candidates
+    with m <= 32 keep their mirror inside and the others lose their cost."""
+    w, h = 80, 64
+    cur, r1, r2 = texture(w, h, 31), texture(w, h, 32), texture(w, h, 33)
+    prm = O.me_param(w, h, 16, LAM)
+    p1, p2 = planes_of(r1), planes_of(r2)
+    for mv in ((33, 0), (31, 2), (34, -1)):
+        j = job(0, 24, 16, 16, mv, dist=(1, 8), b_slice=1)
+        got = O.me_search_sym(prm, j, cur_plane(cur), p1, p2)
+        assert got[:3] == numpy_sym(dict(), j, cur, r1, r2), mv
+        assert got[1] <= 32
