@@ -141,3 +141,16 @@ def test_tus_outside_their_limits_are_skipped():
     assert np.array_equal(level[0], exp[0][0]) and ssd[0] == exp[3][0]
     # only the good TU's pixels are written
     assert np.array_equal(rec, exp[1])
+
+
+def test_reconstruction_clips_to_8_bits():
+    w, h = 64, 64
+    rng = np.random.default_rng(51)
+    cur = (rng.integers(0, 2, (h, w)) * 255).astype(np.uint8)
+    ref = np.full((h, w), 128, np.uint8)
+    tus = tiles(w, h, 8, np.random.default_rng(1), mv_span=8)
+    got = gpu_recon(8, 51, cur, ref, tus, 2)
+    exp = O.inter_recon_batch(8, 51, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    assert (got[1] == 0).any() and (got[1] == 255).any()

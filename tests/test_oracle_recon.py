@@ -150,3 +150,22 @@ def test_bi_prediction_known_answer_and_reductions():
         v = np.clip(preds[i] + rr[i].astype(np.int32), 0, 255)
         assert np.array_equal(got[1][t[1]:t[1] + 16, t[0]:t[0] + 16], v)
         assert got[3][i] == int(((cur[t[1]:t[1] + 16, t[0]:t[0] + 16].astype(np.int64) - v) ** 2).sum())
+
+
+@pytest.mark.parametrize("qp", [37, 51])
+def test_reconstruction_clips_to_8_bits(qp):
+    """A picture of 0 / 255 noise predicted from a flat 128 reference: the
+    quantised residual overshoots, and add_ps clips pred + residual to 0..255."""
+    w, h = 64, 64
+    rng = np.random.default_rng(qp)
+    cur = (rng.integers(0, 2, (h, w)) * 255).astype(np.uint8)
+    ref = np.full((h, w), 128, np.uint8)
+    tus = tiles(w, h, 8, np.random.default_rng(1), mv_span=8)
+    got = O.inter_recon_batch(8, qp, cur_plane(cur), planes_of(ref), w, h, tus)
+    exp = numpy_recon(8, qp, cur, ref, tus)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+    # the unclipped sums leave the 8-bit range somewhere
+    res = np.array([cur[t[1]:t[1] + 8, t[0]:t[0] + 8].astype(np.int16) - 128 for t in tus])
+    _, rr, _ = O.residual_batch(8, qp, 0, res)
+    assert (128 + rr.astype(np.int32) > 255).any() or (128 + rr.astype(np.int32) < 0).any()
