@@ -123,6 +123,7 @@ def lib():
         L.xo_dequant_params.argtypes = [C.c_int, C.c_int, P, P]
         L.xo_residual_batch.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
         L.xo_get_mc_pos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_pmvr_adapt.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_get_mvp_default.argtypes = [C.POINTER(MvpJob), C.c_int, P, P, P, P]
         L.xo_inter_recon_batch.argtypes = [C.c_int, C.c_int, P, ip, P, ip, C.c_int, C.c_int, P, C.c_int, P, P, ip,
                                            P, P]

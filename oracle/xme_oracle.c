@@ -246,6 +246,11 @@ static int pmvr_adapt(int *mx, int *my, int ctr_x, int ctr_y, int mv_x, int mv_y
     return iabs(*mx - ctr_x) > 2 || iabs(*my - ctr_y) > 2;
 }
 
+int xo_pmvr_adapt(int *mx, int *my, int ctr_x, int ctr_y, int mv_x, int mv_y, int sx, int sy)
+{
+    return pmvr_adapt(mx, my, ctr_x, ctr_y, mv_x, mv_y, sx, sy);
+}
+
 static inline int weighted_cost(int lambda, int bits) { return (lambda * bits) >> 16; }
 
 /* Radius-2 hexagon and radius-1 diamond with their wrap-around entries, and

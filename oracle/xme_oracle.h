@@ -58,6 +58,10 @@ int  xo_mvbits(int search_range, int v);
 int  xo_lambda_factor(double lambda_mode);
 /* CHECK_MV_RANGE (me.h:46-48) for the full-pel box [min_x, max_x] x [min_y, max_y]. */
 int  xo_check_mv_range(int mx, int my, int min_x, int min_y, int max_x, int max_y);
+/* pmvr_adapt_mv (encoder/me.c:412-424): the quarter-pel point for Spiral
+ * step (sx, sy) from the half-pel best (mv_x, mv_y) around the PMVR centre
+ * (ctr_x, ctr_y) = ((mvp >> 1) << 1); returns 1 when the point is skipped. */
+int  xo_pmvr_adapt(int *mx, int *my, int ctr_x, int ctr_y, int mv_x, int mv_y, int sx, int sy);
 
 /* ---- ME search (encoder/me.c, encoder/md_inter.c) ---------------------- */
 typedef struct {

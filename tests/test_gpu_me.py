@@ -145,6 +145,24 @@ def test_subpel_switches(kw):
     run_case(w, h, pus, 16, seed=11, prm_kw=kw)
 
 
+@pytest.mark.parametrize("mvp,bcost2,pmvr,exp", [((-237, -242), 64, 0, (-234, -235, 48)),
+                                                   ((-237, -242), 64, 1, (-236, -236, 40)),
+                                                   ((-237, -236), 56, 0, (-234, -235, 32)),
+                                                   ((-237, -236), 56, 1, (-234, -236, 24))])
+def test_pmvr_flat_frame_known_answers(mvp, bcost2, pmvr, exp):
+    """The hand-worked flat-frame cases of the oracle's PMVR test, on the GPU: every SAD is
+    0, so the costs are 4 * MV bits and the answers are fixed literals."""
+    w, h = 128, 96
+    flat = np.full((h, w), 77, np.uint8)
+    dref = X.DeviceRef(w, h, frac=2)
+    dref.load(flat)
+    plan = X.Plan(X.me_params(w, h, 8, LAM, use_hadamard=0, enable_pmvr=pmvr), X.make_pus([(0, 0, 16, 16) + mvp]), 1)
+    plan.run(device_cur(flat), [dref])
+    r = plan.fetch()[0]
+    assert (r["status"], r["bmv2_x"], r["bmv2_y"], r["bcost2"]) == (0, -58, -58, bcost2)
+    assert (r["bmv_x"], r["bmv_y"], r["cost"], r["mvcost"]) == exp + (exp[2],)
+
+
 def test_frame_edges_range64():
     w, h = 320, 192
     pus = [(0, 0, 16, 16, 0, 0), (304, 176, 16, 16, 0, 0), (0, 128, 64, 64, 0, 0), (256, 0, 64, 64, 0, 0),

@@ -120,6 +120,10 @@ def test_mc_luma_and_chroma_batches():
             y = int(rng.integers(0, h - bh + 1))
             blocks.append((x, y, bw, bh, int(rng.integers(-120, 121)), int(rng.integers(-120, 121))))
     blocks.append((0, 0, 16, 16, -4 * 200 + 3, 4 * 300 + 5))          # clamped far outside by get_mv_for_mc
+    # clamped with the phase bit of value 4 set on each side (cu_get_mc_pos keeps mv & 7)
+    blocks.append((0, 0, 16, 16, -795, -394))
+    blocks.append((w - 16, h - 16, 16, 16, 166, 4 * 40 + 6))
+    blocks.append((8, 4, 8, 8, -793, 4 * 50 + 7))
     n = len(blocks)
     barr = np.array(blocks, np.int32).ravel()
     off = np.arange(n, dtype=np.int64) * 64 * 64

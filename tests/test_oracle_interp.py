@@ -118,3 +118,39 @@ def test_get_mc_pos_clamps_far_mvs():
     assert L.xo_get_mc_pos(1920, 16, 100, 37) == 4 * 100 + 37
     assert L.xo_get_mc_pos(1920, 16, 100, -4 * 200 + 3) == 4 * (-16 - 8) + ((-800 + 3) & 7)
     assert L.xo_get_mc_pos(1920, 16, 1900, 4 * 40 + 1) == 4 * (1920 + 4) + 1
+
+
+def test_get_mc_pos_keeps_the_eighth_pel_phase_when_clamping():
+    """cu_get_mc_pos (mc.h:52-66) keeps mv & 7 when it clamps: the bit of value 4 survives
+    (chroma reads it as half a chroma pixel).  Every value worked by hand."""
+    L = O.lib()
+    # left: -795 >> 2 = -199, 100 - 199 < -16 - 8: (-24 << 2) + (-795 & 7 = 5)
+    assert L.xo_get_mc_pos(1920, 16, 100, -795) == -91
+    assert L.xo_get_mc_pos(1920, 16, 100, -793) == -89          # -793 & 7 = 7
+    # top: -396 >> 2 = -99 < -16: (-16 << 2) + (-396 & 7 = 4)
+    assert L.xo_get_mc_pos(1080, 8, 0, -396) == -60
+    # exactly at the left limit (-24 is not < -24): not clamped; one qpel further is
+    assert L.xo_get_mc_pos(1920, 16, 0, -96) == -96
+    assert L.xo_get_mc_pos(1920, 16, 0, -97) == -89             # -97 >> 2 = -25: -96 + (-97 & 7 = 7)
+    # right: 166 >> 2 = 41, 1941 > 1924: (1924 << 2) + (166 & 7 = 6)
+    assert L.xo_get_mc_pos(1920, 16, 1900, 166) == 7702
+    # bottom: 127 >> 2 = 31, 1101 > 1084: (1084 << 2) + 7
+    assert L.xo_get_mc_pos(1080, 32, 1070, 127) == 4343
+
+
+def test_chroma_mc_at_a_clamped_position():
+    """rdo_get_pred_inter (rdo.c:1616-1659) passes the clamped luma qpel position to
+    mc_chroma, which splits it into a chroma integer position (>> 3) and a 1/8-pel phase
+    (& 7).  Luma 112x96 (chroma 56x48, pad 40), 16x16 PU at (0, 0), MV (-795, -394):
+    positions (-91, -90) by hand, i.e. chroma integer (-12, -12), phases (5, 6)."""
+    L = O.lib()
+    qx = L.xo_get_mc_pos(112, 16, 0, -795)
+    qy = L.xo_get_mc_pos(96, 16, 0, -394)
+    assert (qx, qy) == (-91, -90)
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (48, 56), dtype=np.uint8)
+    pu = O.Plane(56, 48, pad=40); pu.set_picture(img); pu.expand_border()
+    pv = O.Plane(56, 48, pad=40); pv.set_picture(255 - img); pv.expand_border()
+    du, dv = O.mc_chroma(qx, qy, 8, 8, pu, pv)
+    assert np.array_equal(du, npref.block_filter_chroma(pu.view(), 40 - 12, 40 - 12, 8, 8, 5, 6))
+    assert np.array_equal(dv, npref.block_filter_chroma(pv.view(), 40 - 12, 40 - 12, 8, 8, 5, 6))

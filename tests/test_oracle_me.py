@@ -204,3 +204,73 @@ def test_packed_mv_range_check_equals_box_test():
             for my in ys:
                 box = x0 <= mx <= x1 and y0 <= my <= y1
                 assert bool(L.xo_check_mv_range(int(mx), int(my), x0, y0, x1, y1)) == box, (mx, my, x0, y0, x1, y1)
+
+
+# pmvr_adapt_mv (me.c:412-424, TH_PMVR = 2, defines.h:227), worked by hand from the text:
+# (centre, half-pel best, Spiral step) -> (quarter-pel point, skipped).
+PMVR_KNOWN = [
+    # near branch: |mv - ctr| <= 2 on both axes, one qpel step, skipped when it leaves the +-2 box
+    ((0, 0), (0, 0), (1, 0), (1, 0), 0),
+    ((0, 0), (2, 0), (1, 0), (3, 0), 1),          # |2| is not > 2: near branch; 3 > 2: skipped
+    ((0, 0), (2, 0), (-1, 0), (1, 0), 0),
+    ((0, 0), (1, 1), (1, 1), (2, 2), 0),          # lands on the box corner: kept
+    ((0, 0), (2, -2), (0, -1), (2, -3), 1),
+    ((0, 0), (-2, 2), (-1, 1), (-3, 3), 1),
+    ((4, -6), (6, -4), (1, 1), (7, -3), 1),
+    ((4, -6), (6, -4), (-1, -1), (5, -5), 0),
+    # far branch: |mv - ctr| > 2 on an axis, two qpel steps, skipped when it lands in the box
+    ((0, 0), (3, 0), (1, 0), (5, 0), 0),
+    ((0, 0), (3, 0), (-1, 0), (1, 0), 1),
+    ((0, 0), (0, 3), (0, -1), (0, 1), 1),
+    ((0, 0), (3, 0), (-1, 1), (1, 2), 1),         # both axes within 2: skipped
+    ((0, 0), (3, 1), (-1, 1), (1, 3), 0),         # y lands at 3: kept
+    ((0, 0), (4, 0), (-1, 0), (2, 0), 1),         # exactly 2 from the centre: inside
+    ((0, 0), (-3, -3), (1, 1), (-1, -1), 1),
+    ((-238, -242), (-234, -242), (-1, 0), (-236, -242), 1),
+    ((-238, -242), (-234, -244), (-1, -1), (-236, -246), 0),
+]
+
+
+@pytest.mark.parametrize("ctr,mv,step,point,skip", PMVR_KNOWN)
+def test_pmvr_adapt_known_answers(ctr, mv, step, point, skip):
+    import ctypes as C
+    mx, my = C.c_int(99999), C.c_int(99999)
+    s = O.lib().xo_pmvr_adapt(C.byref(mx), C.byref(my), ctr[0], ctr[1], mv[0], mv[1], step[0], step[1])
+    assert (s, mx.value, my.value) == (skip, point[0], point[1])
+
+
+# (MVP, integer cost, PMVR, expected (bmv_x, bmv_y, cost)); see the test's docstring.
+PMVR_FLAT_CASES = [
+    ((-237, -242), 64, 0, (-234, -235, 48)),
+    ((-237, -242), 64, 1, (-236, -236, 40)),
+    ((-237, -236), 56, 0, (-234, -235, 32)),
+    ((-237, -236), 56, 1, (-234, -236, 24)),
+]
+
+
+@pytest.mark.parametrize("mvp,bcost2,pmvr,exp", PMVR_FLAT_CASES)
+def test_pmvr_quarter_pel_stage_known_answer(mvp, bcost2, pmvr, exp):
+    """A flat frame (every SAD 0, interpolated planes flat too) and SAD costs: the cost is
+    4 * (mvbits(mx - pmx) + mvbits(my - pmy)), worked by hand (mvbits 1, 3, 5, 7, 9 for
+    |MVD| 0, 1, 2-3, 4-7, 8-15).  PU 16x16 at (0, 0) of a 128x96 frame: the window starts at
+    -58 (md_inter.c:913-926) and both MVPs lie outside it, so the integer best is (-58, -58).
+
+    MVP (-237, -242): integer MVD (5, 10), cost 4 * (7 + 9) = 64; half-pel best (-234, -234),
+    MVD (3, 8), cost 56.  The PMVR centre is ((-237 >> 1) << 1, (-242 >> 1) << 1) =
+    (-238, -242); the half-pel best is 4 qpel from it in x, so PMVR steps two qpel and
+    keeps (-236, -236) (MVD (1, 6), cost 40), which the plain +-1 pattern cannot reach
+    (its best is (-234, -235), MVD (3, 7), cost 48).
+
+    MVP (-237, -236): integer MVD (5, 4), cost 56; half-pel best (-234, -234), MVD (3, 2),
+    cost 40.  Centre (-238, -236) (the shift floors: -237 >> 1 = -119): x is 4 away, so two
+    qpel steps; (-236, -236) and (-236, -234) land inside the +-2 box and are skipped, and
+    (-234, -236) wins (MVD (3, 0), cost 24).  A centre truncated towards zero (-236, -236)
+    would take the near branch and end at (-234, -235), cost 32, the plain pattern's answer."""
+    w, h = 128, 96
+    flat = np.full((h, w), 77, np.uint8)
+    planes = planes_of(flat, 2)
+    prm = O.me_param(w, h, 8, 4 * 65536, use_hadamard=0, enable_pmvr=pmvr)
+    res, _ = O.me_search(prm, (0, 0, 16, 16) + mvp, cur_plane(flat), planes)
+    assert (res["status"], res["bmv2_x"], res["bmv2_y"], res["bcost2"]) == (0, -58, -58, bcost2)
+    assert (res["bmv_x"], res["bmv_y"], res["cost"]) == exp
+    assert res["mvcost"] == exp[2]
