@@ -330,16 +330,39 @@ static void ipel_point(pat_ctx_t *c, int mx, int my)
     }
 }
 
-/* DIA_ITER (me.c:328-334) through ME_COST_IPEL_X4 (me.c:241-267): with all
- * four points in the window they are taken in order without checks, else
- * each one is checked; either way it is the four range-checked points in
- * order. */
+/* One point of the sad_x4 branch of ME_COST_IPEL_X4 (me.c:244-258): SAD plus
+ * MV_COST_IPEL, no range check, COPY3_IF_LT (strict '<', no direction code). */
+static void free_point(pat_ctx_t *c, int mx, int my)
+{
+    int cost = xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
+               weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+    if (cost < c->bcost) {
+        c->bcost = cost;
+        c->bmx = mx;
+        c->bmy = my;
+    }
+}
+
+/* DIA_ITER (me.c:328-334): omx = mx, omy = my, then ME_COST_IPEL_X4 with the
+ * offsets (0,-1), (-1,0), (1,0), (0,1) (me.c:241-267).  Its CHECK_MV_RANGE_X4
+ * (me.h:51-56) tests those offsets themselves against the window, not the
+ * points: when the window holds all four, the points around (mx, my) are
+ * taken in order WITHOUT a range check (so a centre on the window edge
+ * evaluates a point one pixel outside it); otherwise the else branch's
+ * ME_COST_IPEL(m0x, m0y) ... take the offsets as absolute MVs (0,-1),
+ * (-1,0), (1,0), (0,1), each range-checked, whatever the centre. */
 static void dia_iter(pat_ctx_t *c, int mx, int my)
 {
-    ipel_point(c, mx, my - 1);
-    ipel_point(c, mx - 1, my);
-    ipel_point(c, mx + 1, my);
-    ipel_point(c, mx, my + 1);
+    static const int8_t off[4][2] = { { 0, -1 }, { -1, 0 }, { 1, 0 }, { 0, 1 } };
+    int inside = 1;
+    for (int k = 0; k < 4; k++)
+        inside &= check_mv_range(off[k][0], off[k][1], c->mv_min, c->mv_max);
+    for (int k = 0; k < 4; k++) {
+        if (inside)
+            free_point(c, mx + off[k][0], my + off[k][1]);
+        else
+            ipel_point(c, off[k][0], off[k][1]);
+    }
 }
 
 static inline int ipel(int v) { return (v + 2) >> 2; }       /* IPEL, me.h:60 */

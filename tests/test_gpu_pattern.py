@@ -266,3 +266,47 @@ def test_no_legal_candidate_status(method):
         run_umh_case(w, h, pus, 4, 1, 2, 2, kw, ctx)
     else:
         run_case(w, h, pus, 4, seed=2, prm_kw=dict(me_method=method, **kw))
+
+
+def umh_single(w, h, cur, ref, pu, R, **prm_kw):
+    """One PU's UMH search (no early termination, integer only) on the GPU and in the oracle."""
+    ctx = np.zeros(1, dtype=X.UMH_CTX_DTYPE)
+    ctx["beta2"] = -1.0
+    ctx["beta3"] = -1.0
+    dref = X.DeviceRef(w, h, frac=0)
+    dref.load(ref)
+    plan = X.Plan(X.me_params(w, h, R, LAM, me_method=X.ME_UMH, use_fractional_me=0, **prm_kw), X.make_pus([pu]), 1)
+    plan.set_umh_context(ctx)
+    plan.run(device_cur(cur), [dref])
+    got = plan.fetch()
+    p = O.Plane(w, h)
+    p.set_picture(ref)
+    p.expand_border()
+    cp = O.Plane(w, h)
+    cp.set_picture(cur)
+    exp = O.me_search_batch(O.me_param(w, h, R, LAM, me_method=O.ME_UMH, use_fractional_me=0, **prm_kw),
+                            np.array([pu], dtype=O.PU_DTYPE), cp, [p] + [None] * 15,
+                            ctxs=ctx.view(O.UMH_CTX_DTYPE))
+    compare(got, exp, [pu], 1)
+    return int(got["bmv2_x"][0]), int(got["bmv2_y"][0]), int(got["bcost2"][0])
+
+
+def test_umh_diamond_unchecked_at_the_window_edge():
+    """The oracle's hand-worked case: on a flat frame the first diamond around the clipped
+    MVP candidate (-58, 0) takes (-59, 0), outside the window (cost 16)."""
+    flat = np.full((96, 128), 60, np.uint8)
+    assert umh_single(128, 96, flat, flat, (0, 0, 16, 16, -237, 0), 8) == (-59, 0, 16)
+
+
+def test_umh_diamond_absolute_points_when_the_window_misses_an_offset():
+    """The oracle's hand-worked case: with the window ending at 0 on both axes the diamond
+    evaluates the absolute MV (-1, 0), where the current block matches exactly (cost 112)."""
+    w, h = 128, 96
+    rng = np.random.default_rng(3)
+    ref = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    blk = ref[32:48, 47:63].copy()
+    ref[12:28, 38:54] = np.clip(blk.astype(int) + rng.choice([-3, 3], blk.shape), 0, 255)
+    cur = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    cur[32:48, 48:64] = blk
+    assert umh_single(w, h, cur, ref, (48, 32, 16, 16, -40, -80), 8, mv_range_x=(-8192, 24),
+                      mv_range_y=(-2048, 24)) == (-1, 0, 112)

@@ -482,3 +482,55 @@ def test_umh_gpu_test_contexts_take_every_route():
     for where in (1, 2, 3, 4):
         assert any(k == where for k, _ in routes), (where, sorted(routes))
     assert {t for _, t in routes} == {2, 3}
+
+
+def test_umh_first_diamond_is_unchecked_at_the_window_edge():
+    """DIA_ITER (me.c:328-334) tests its relative offsets (0,-1), (-1,0), (1,0), (0,1)
+    against the window (CHECK_MV_RANGE_X4, me.h:51-56), so the points around a centre on
+    the window edge are evaluated unchecked.  Flat frame, costs 4 * MV bits (mvbits 1, 3, 5,
+    7, 9 for |MVD| 0, 1, 2-3, 4-7, 8-15); 16x16 PU at (0, 0) of 128x96: the window starts at
+    -58.  MVP (-237, 0) qpel: mvc[0] = clip(IPEL) = (-58, 0), MVD 5, cost 4 * (7 + 1) = 32.
+    The first diamond takes (-59, 0), one pixel outside the window (MVD 1: cost 16); the
+    second diamond around it finds nothing cheaper ((-60, 0): MVD -3, cost 24); cross and
+    grid points are checked (those cheaper than 16 are all outside); the hexagon around
+    (-59, 0) finds nothing below 16, and the diamond needs a legal centre.  Result (-59, 0),
+    16.  A diamond that checked its points would stay at (-58, 0) and the hexagon would
+    then take (-60, 0) (cost 24): the HEX method, which has no DIA_ITER, does exactly that."""
+    w, h = 128, 96
+    flat = np.full((h, w), 60, np.uint8)
+    pu, mvp = (0, 0, 16, 16), (-237, 0)
+    assert umh_search(flat, flat, pu, mvp, 4 * 65536, 8, umh_ctx()) == ((-59, 0), 16)
+    prm = O.me_param(w, h, 8, 4 * 65536, use_fractional_me=0, me_method=O.ME_HEX)
+    res = O.me_search_batch(prm, np.array([pu + mvp], dtype=O.PU_DTYPE), cur_plane(flat),
+                            [plane_of(flat)] + [None] * 15)
+    assert (int(res["bmv2_x"][0]), int(res["bmv2_y"][0]), int(res["bcost2"][0])) == (-60, 0, 24)
+
+
+def test_umh_diamond_takes_absolute_points_when_the_window_misses_an_offset():
+    """When the window does not hold all four unit offsets, ME_COST_IPEL_X4's else branch
+    evaluates ME_COST_IPEL(0,-1), (-1,0), (1,0), (0,1): absolute MVs, range-checked, whatever
+    the centre.  MV range x and y limited to 24 qpel: the window of the 16x16 PU at (48, 32)
+    ends at ((24 >> 2) - 6) = 0 on both axes, so (1, 0) and (0, 1) are outside.  The
+    current PU is the reference at (-1, 0) exactly; MVP (-40, -80): mvc[0] = (-10, -20),
+    where a noisy copy of the same block sits (SAD several hundred).  The first diamond,
+    centred at (-10, -20), evaluates the absolute (-1, 0): SAD 0, cost 4 * (mvbits(36) +
+    mvbits(80)) = 4 * (13 + 15) = 112, which no other point of a uniform-noise picture
+    beats.  A diamond around (-10, -20) never reaches (-1, 0) (nor do the cross and grid
+    around it)."""
+    w, h = 128, 96
+    rng = np.random.default_rng(3)
+    ref = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    blk = ref[32:48, 47:63].copy()
+    ref[12:28, 38:54] = np.clip(blk.astype(int) + rng.choice([-3, 3], blk.shape), 0, 255)
+    cur = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    cur[32:48, 48:64] = blk
+    prm = O.me_param(w, h, 8, 4 * 65536, use_fractional_me=0, me_method=O.ME_UMH,
+                     mv_range_x=(-8192, 24), mv_range_y=(-2048, 24))
+    res = O.me_search_batch(prm, np.array([(48, 32, 16, 16, -40, -80)], dtype=O.PU_DTYPE), cur_plane(cur),
+                            [plane_of(ref)] + [None] * 15, ctxs=umh_ctx())
+    assert (int(res["bmv2_x"][0]), int(res["bmv2_y"][0]), int(res["bcost2"][0])) == (-1, 0, 112)
+    # the same picture with the offsets inside the window: the diamond stays around (-10, -20)
+    prm = O.me_param(w, h, 8, 4 * 65536, use_fractional_me=0, me_method=O.ME_UMH)
+    res = O.me_search_batch(prm, np.array([(48, 32, 16, 16, -40, -80)], dtype=O.PU_DTYPE), cur_plane(cur),
+                            [plane_of(ref)] + [None] * 15, ctxs=umh_ctx())
+    assert (int(res["bmv2_x"][0]), int(res["bmv2_y"][0])) != (-1, 0)

@@ -141,21 +141,28 @@ struct UmhState {
     int pmx, pmy;               // pmv (me.c:910)
     int level;                  // big hexagon level j
     int p0x, p0y;               // mvc[0]
+    bool dfree;                 // the window holds the offsets (0,-1), (-1,0), (1,0), (0,1)
 };
 
-// Point t of the current UMH stage; false for a point the stage skips (a
-// missing predictor).  The range check is the caller's.
-__device__ __forceinline__ bool umh_point(const UmhState &u, const xme_umh_ctx &c, int R, int big, int t, int &mx,
-                                          int &my)
+// Point t of the current UMH stage: 0 for a point the stage skips (a missing
+// predictor), 1 for a point taken only inside the MV window (ME_COST_IPEL),
+// 2 for a point taken without a range check.  DIA_ITER (me.c:328-334) runs
+// ME_COST_IPEL_X4 on the offsets (0,-1), (-1,0), (1,0), (0,1), whose
+// CHECK_MV_RANGE_X4 (me.h:51-56) tests the offsets themselves: when the
+// window holds them (dfree) the four points around the centre are taken
+// unchecked (me.c:244-258), otherwise the offsets as absolute MVs, each
+// checked (me.c:259-264).
+__device__ __forceinline__ int umh_point(const UmhState &u, const xme_umh_ctx &c, int R, int big, int t, int &mx,
+                                         int &my)
 {
     const int ch = (R + 1) >> 1;           // odd i <= R
     switch (u.st) {
     case kUDia0:
     case kUDia1:
     case kUDia2:
-        mx = u.cx + tab(kDia4X, t, 2);
-        my = u.cy + tab(kDia4Y, t, 2);
-        return t < 4;
+        mx = (u.dfree ? u.cx : 0) + tab(kDia4X, t, 2);
+        my = (u.dfree ? u.cy : 0) + tab(kDia4Y, t, 2);
+        return t < 4 ? (u.dfree ? 2 : 1) : 0;
     case kUMvp1:
         mx = (c.mvp1_x + 2) >> 2;
         my = (c.mvp1_y + 2) >> 2;
@@ -335,9 +342,12 @@ __device__ __forceinline__ void umh_done_stage(UmhState &u, int bmx, int bmy)
 // of the group at once), so a round evaluates S points.  A round's points are
 // taken as the reference takes them one by one: the first strict minimum,
 // i.e. the smallest (cost, point) key, if its cost is below the running best
-// (COPY3_IF_LT in order, ME_COST_IPEL, me.c:187-193).  Every point is range
-// checked first, so the reads (straight from the plane: the points lie
-// anywhere in the window) stay inside the plane's padding.
+// (COPY3_IF_LT in order, ME_COST_IPEL, me.c:187-193).  Every point but a
+// diamond's is range checked first; a diamond's unchecked points lie one
+// pixel from its centre, which is a window point or the best of an earlier
+// diamond, so at most 3 pixels outside the window.  The window keeps 6 pixels
+// inside the 64-pixel search margin (md_inter.c:913), so the reads (straight
+// from the plane) stay inside its 80-pixel padding.
 //
 // LP > 1 (64-lane groups): LP lanes share a point, lane j of them summing
 // quads j, j + LP, ... of every row, so that a row of a point is one
@@ -361,6 +371,8 @@ __device__ __forceinline__ void umh_prefix_lanes(const DevParams &p, const DevPu
     u.level = 0;
     u.p0x = p0x;
     u.p0y = p0y;
+    u.dfree = xme_mv_legal(win, 0, -1) && xme_mv_legal(win, -1, 0) && xme_mv_legal(win, 1, 0) &&
+              xme_mv_legal(win, 0, 1);
     if (u.st != kUOff)
         phase = kDone;   // set again when the prefix ends
     for (;;) {
@@ -372,7 +384,8 @@ __device__ __forceinline__ void umh_prefix_lanes(const DevParams &p, const DevPu
         const int pi = l / LP, j = l % LP;     // point of the round, lane within it
         const int t = u.bi * PR + pi;
         int mx = 0, my = 0;
-        const bool ok = live && t < n_st && umh_point(u, c, p.R, p.big_hex, t, mx, my) && xme_mv_legal(win, mx, my);
+        const int kind = live && t < n_st ? umh_point(u, c, p.R, p.big_hex, t, mx, my) : 0;
+        const bool ok = kind == 2 || (kind == 1 && xme_mv_legal(win, mx, my));
         int key = 0x7fffffff;
         if constexpr (LP > 1) {
             uint32_t sad = 0;
@@ -535,7 +548,8 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
         if (__builtin_amdgcn_ballot_w64(phase != kDone) == 0)
             break;
         // every point of a step lies within 2 pixels of the current best,
-        // which is inside the MV window: the box (PU + 2 pixels a side, from
+        // which is inside the MV window or (after UMH's unchecked diamonds)
+        // at most 3 pixels outside it: the box (PU + 2 pixels a side, from
         // the dword below) stays inside the plane's padding
         const int x0 = pu.x + bmx - 2, ax0 = x0 & ~3, sh = x0 - ax0;
         {
