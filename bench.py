@@ -65,10 +65,43 @@ def legal_counts(w, h, x, y, pw, ph, R, mvr_x=(-8192, 8191), mvr_y=(-2048, 2047)
     return axis(x, pw, w, mvr_x), axis(y, ph, h, mvr_y)
 
 
-def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
+def host_cpus():
+    """The host's CPUs as this process sees them: nproc (os.cpu_count), the
+    CPUs it may run on (affinity), the cgroup CPU quota, the CPU model, and the
+    worker count used for CPU work: the CPUs this job is given, at most 16
+    (the host's share per GPU)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota_cpus": quota, "model": model,
+            "threads": max(1, min(16, avail))}
+
+
+def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads, host=None):
     """The oracle (CPU restatement) timed on a bounded sample of the same job:
     all PUs of the first CTU rows against reference 0, grown until the sample
-    takes about budget_s seconds.  Also checks the GPU results of the sample."""
+    takes about budget_s seconds, on `threads` workers; then the first CTU row
+    on one thread.  Also checks the GPU results of the sample."""
     from oracle import oracle as O
     w, h = refs[0].shape[1], refs[0].shape[0]
     p = O.Plane(w, h)
@@ -103,6 +136,21 @@ def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
         for f in ("cost", "bcost2", "bmv_x", "bmv_y", "bmv2_x", "bmv2_y", "mvcost", "status"):
             if int(g[f]) != int(res[k][f]):
                 ok = False
+    # one thread: the PUs of the first CTU row that fit about a quarter of the budget
+    row0 = [i for i in idx if pus[i][1] // 64 == 0]
+    n1 = len(row0)
+    while True:
+        sub = row0[:n1]
+        arr1 = np.array([(pus[i][0], pus[i][1], pus[i][2], pus[i][3], 0, 0) for i in sub], dtype=O.PU_DTYPE)
+        t0 = time.perf_counter()
+        O.me_search_batch(prm, arr1, cp, planes, nthreads=1)
+        t1 = time.perf_counter() - t0
+        if t1 < budget_s / 2 or n1 <= 8:
+            break
+        n1 = max(8, int(n1 * budget_s / 4 / t1))
+    c1 = sum(legal_counts(w, h, pus[i][0], pus[i][1], pus[i][2], pus[i][3], R)[0] *
+             legal_counts(w, h, pus[i][0], pus[i][1], pus[i][2], pus[i][3], R)[1] for i in sub)
+    host = host or host_cpus()
     return {
         "value": n_cand / elapsed,
         "unit": "cand-evals/s",
@@ -110,6 +158,9 @@ def cpu_baseline(prm_args, refs, cur, pus, gpu_res, n_ref, budget_s, threads):
         "kind": "port",
         "sample": f"{len(idx)} PUs ({sel_rows} CTU rows of 3840x2160) x 1 ref, full search [-64,64)^2 + "
                   f"half/quarter-pel SATD, {elapsed:.2f} s on {threads} threads",
+        "single_thread": {"value": c1 / t1, "unit": "cand-evals/s",
+                          "sample": f"{len(sub)} PUs of the first CTU row, {t1:.2f} s on 1 thread"},
+        "host": host,
         "parity_with_gpu_on_sample": ok,
     }
 
@@ -155,7 +206,7 @@ def pattern_leg(X, w, h, R, lam, pus, n_ref, dcur, d_refs, refs, cur, stream, st
         planes0 = O.interp_planes(p, 2)
         cp = O.Plane(w, h)
         cp.set_picture(cur)
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cpus()["threads"]
     out = {}
     ctx = umh_contexts(X, len(pus), n_ref, seed=2027)
     for name, method in (("dia", X.ME_DIA), ("hex", X.ME_HEX), ("umh", X.ME_UMH)):
@@ -723,9 +774,10 @@ def main():
         transform = transform_leg(X, args.transform_blocks, reps=5)
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"timing the CPU oracle on a sample ({threads} threads)")
-        cpu = cpu_baseline((R, lam, 1, 2, 0, 0), refs, cur, pus, gpu_res, n_ref, args.cpu_seconds, threads)
+        host = host_cpus()
+        threads = host["threads"]
+        log(f"timing the CPU oracle on a sample ({threads} threads; host {host})")
+        cpu = cpu_baseline((R, lam, 1, 2, 0, 0), refs, cur, pus, gpu_res, n_ref, args.cpu_seconds, threads, host)
 
     out = {
         "metric": METRIC,

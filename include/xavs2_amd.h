@@ -45,6 +45,7 @@ extern "C" {
 #define XME_FENC_STRIDE 64   /* FENC_STRIDE, source/common/common.h:1350 */
 #define XME_PAD         80   /* XAVS2_PAD, source/common/defines.h:298 */
 #define XME_MAX_REFS    8
+#define XME_MVP_REFS    4            /* MAX_REFS (source/common/defines.h:301, :364) */
 #define XME_MAX_DISTORTION (1 << 30)  /* source/common/defines.h:295 */
 
 typedef uint8_t pel_t;       /* source/common/basic_types.h:50 */
@@ -317,9 +318,12 @@ int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_job *d_jobs,
  * left, top, top-left and top-right neighbours' ref_idx[bwd_2nd]
  * (-1 = INVALID_REF) and mv[bwd_2nd]; the top-left stands in when the
  * top-right is unavailable.  ref_dpoc / ref_dpoc_multi are
- * h->fdec->ref_dpoc / ref_dpoc_multi (encoder.c:1795-1810).  Status 2: a
- * reference index outside 0..XME_MAX_REFS-1; 3: a scaling product that
- * leaves int (undefined in the reference; the clipped value is returned). */
+ * h->fdec->ref_dpoc / ref_dpoc_multi (int[MAX_REFS], common.h:1031-1032,
+ * set at encoder.c:1795-1810): XME_MVP_REFS entries each are read.  Status
+ * 2: a searched or neighbour reference index outside 0..XME_MVP_REFS-1 (the
+ * reference asserts blkref < MAX_REFS, predict.h:100); 3: a scaling product
+ * that leaves int (undefined in the reference; the clipped value is
+ * returned). */
 typedef struct {
     int8_t ref_idx, bwd_2nd, ur_available, reserved;
     int8_t cb_x, cb_y;               /* p_cb->x, y: the PU inside its CU */
@@ -333,8 +337,8 @@ typedef struct {
     int32_t status;
 } xme_mvp_result;
 
-int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, const int32_t ref_dpoc[XME_MAX_REFS],
-                      const int32_t ref_dpoc_multi[XME_MAX_REFS], xme_mvp_result *d_out, void *stream);
+int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, const int32_t ref_dpoc[XME_MVP_REFS],
+                      const int32_t ref_dpoc_multi[XME_MVP_REFS], xme_mvp_result *d_out, void *stream);
 
 /* ---- 4. reference planes, MC and the residual path ---------------------- */
 /* plane_expand_border over a whole plane (source/common/frame.c:470-507). */
@@ -373,8 +377,10 @@ int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t 
  * position (rdo_get_pred_inter, source/encoder/rdo.c:1616-1633), the
  * prediction error (sub_ps), DCT -> quant -> dequant -> IDCT
  * (xme_residual_batch, inter), the reconstruction clip(pred + residual)
- * (add_ps) written into d_rec and its SSD against d_cur, as the luma TU path
- * of rdo.c:895-945 without RDOQ or the secondary transform.
+ * (add_ps) written into d_rec and its SSD against d_cur: the prediction
+ * error of rdo.c:1751-1753, then tu_recon_inter_luma
+ * (source/encoder/rdo.c:1333-1366, inter quant with b_intra = 0) without
+ * RDOQ, and the SSD of cu_recon_inter_luma (rdo.c:1497-1499) per TU.
  * d_tus: ntu x {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y} (MV in qpel,
  * the TU inside the PU).  d_resid (prediction error) and d_rres
  * (reconstructed residual) are ntu * n * n coefficients each; d_level holds

@@ -290,6 +290,7 @@ def get_mvp_default(job, b_slice, ref_dpoc, ref_dpoc_multi):
         j.mv[k][0], j.mv[k][1] = job["mv"][k]
     d = np.ascontiguousarray(ref_dpoc, dtype=np.int32)
     m = np.ascontiguousarray(ref_dpoc_multi, dtype=np.int32)
+    assert d.size == 4 and m.size == 4, "ref_dpoc tables hold MAX_REFS = 4 entries"
     x, y = C.c_int(), C.c_int()
     rc = lib().xo_get_mvp_default(C.byref(j), b_slice, ptr(d), ptr(m), C.byref(x), C.byref(y))
     return rc, x.value, y.value

@@ -1329,9 +1329,17 @@ static int median_mv(int a, int b, int c)
     return (int16_t)((c + a) / 2);
 }
 
-int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[8], const int ref_dpoc_multi[8],
+int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[4], const int ref_dpoc_multi[4],
                        int *pmv_x, int *pmv_y)
 {
+    /* the reference lists hold MAX_REFS = 4 entries (defines.h:301, :364) and
+     * the reference asserts blkref < MAX_REFS (predict.h:100) */
+    const int rn = j->ur_available ? j->ref[3] : j->ref[2];
+    if (j->ref_idx < 0 || j->ref_idx >= 4 || j->ref[0] < -1 || j->ref[0] >= 4 || j->ref[1] < -1 || j->ref[1] >= 4 ||
+        rn < -1 || rn >= 4) {
+        *pmv_x = *pmv_y = 0;
+        return -2;
+    }
     /* neighbours: 0 left (a), 1 top (b), 3 top-right (c), or top-left (2) when
      * the top-right is unavailable */
     const int rl = j->ref[0], ru = j->ref[1], rur = j->ur_available ? j->ref[3] : j->ref[2];

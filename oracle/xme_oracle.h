@@ -218,8 +218,14 @@ void xo_expand_border(uint8_t *plane, intptr_t stride, int w, int h, int pad);
 int  xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int h,
                           int lcu_level, int use_fractional_me);
 
-/* Luma inter reconstruction of N x N TUs (rdo.c:895-945 without RDOQ or the
- * secondary transform; prediction as rdo_get_pred_inter, rdo.c:1616-1633):
+/* Luma inter reconstruction of N x N TUs: the prediction error (sub_ps,
+ * rdo.c:1751-1753), tu_recon_inter_luma (rdo.c:1333-1366: DCT, inter quant
+ * with b_intra = 0, dequant, IDCT, add_ps) without RDOQ, and the SSD of
+ * cu_recon_inter_luma (rdo.c:1497-1499); prediction as rdo_get_pred_inter,
+ * rdo.c:1616-1633.  The transform is the full DCT for every N, as
+ * tu_get_dct_coeff (rdo.c:764-771) uses it for TUs under 32x32 and in the
+ * second RD pass (the first pass of a 32x32 TU under OPT_BIT_EST_PSZT uses
+ * dct_half instead).
  * tus[8 * i] = {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}; the
  * prediction is mc_luma at the PU's clamped position, offset to the TU.
  * Writes level (n*n per TU), the reconstruction into rec (stride i_rec, at
@@ -240,8 +246,10 @@ void xo_inter_recon_bi_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, 
  * the neighbour scaling scale_mv_default (:405-410; B slices use the list's
  * distances, other slices the searched reference's and each neighbour's),
  * then the median rule (derive_median_mv, :372-401) or one neighbour.
- * Returns 0, or -1 where the scaling product leaves int (undefined in the
- * reference). */
+ * ref_dpoc / ref_dpoc_multi hold MAX_REFS = 4 entries (common.h:1031-1032).
+ * Returns 0, -1 where the scaling product leaves int (undefined in the
+ * reference), or -2 for a searched or used neighbour reference index outside
+ * 0..3 (the reference asserts blkref < MAX_REFS, predict.h:100). */
 typedef struct {
     int ref_idx, bwd_2nd;
     int cb_x, cb_y, cb_w, cb_h;       /* p_cb: the PU inside its CU */
@@ -249,7 +257,7 @@ typedef struct {
     int ref[4];                       /* neighbours' ref_idx[bwd_2nd], -1 = INVALID_REF */
     int mv[4][2];                     /* neighbours' mv[bwd_2nd] */
 } xo_mvp_job_t;
-int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[8], const int ref_dpoc_multi[8],
+int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[4], const int ref_dpoc_multi[4],
                        int *pmv_x, int *pmv_y);
 
 /* cu_get_mc_pos (common/mc.h:52-66). */

@@ -13,17 +13,17 @@ from xavs2_amd import lib as X
 
 pytestmark = pytest.mark.gpu
 
-P_DPOC = [2, 4, 6, 8, 10, 12, 14, 16]
+P_DPOC = [2, 4, 6, 8]
 P_MULTI = [16384 // d for d in P_DPOC]
-B_DPOC = [2, 6, 0, 0, 0, 0, 0, 0]
-B_MULTI = [8192, 2731, 0, 0, 0, 0, 0, 0]
+B_DPOC = [2, 6, 0, 0]
+B_MULTI = [8192, 2731, 0, 0]
 
 
 def run_gpu(jobs, b_slice, dpoc, multi):
     d_jobs = X.upload(jobs)
     d_out = X.DeviceBuffer(max(1, len(jobs)) * X.MVP_RESULT_DTYPE.itemsize)
-    dp = (C.c_int32 * 8)(*dpoc)
-    mu = (C.c_int32 * 8)(*multi)
+    dp = (C.c_int32 * X.XME_MVP_REFS)(*dpoc)
+    mu = (C.c_int32 * X.XME_MVP_REFS)(*multi)
     X.check(X.lib().xme_get_mvp_batch(d_jobs.at(), len(jobs), b_slice, dp, mu, d_out.at(), None), "xme_get_mvp_batch")
     X.check(X.lib().xme_stream_sync(None), "sync")
     return d_out.download(len(jobs), X.MVP_RESULT_DTYPE)
@@ -68,7 +68,7 @@ def test_random_neighbourhoods(b_slice):
 def test_overflow_and_bad_reference_indices():
     rng = np.random.default_rng(3)
     jobs = random_jobs(64, rng, 3, mv_span=30000)
-    dpoc, multi = [512] * 8, [16384] * 8
+    dpoc, multi = [512] * 4, [16384] * 4
     got = run_gpu(jobs, 0, dpoc, multi)
     flagged = 0
     for i, j in enumerate(jobs):
@@ -77,9 +77,16 @@ def test_overflow_and_bad_reference_indices():
         assert (got["mvp_x"][i], got["mvp_y"][i]) == (x, y), i
         flagged += rc == -1
     assert flagged > 0
-    bad = random_jobs(4, rng, 3)
+    bad = random_jobs(6, rng, 3)
     bad["ref_idx"][0] = 9
     bad["ref"][1][2] = 8
+    bad["ref_idx"][2] = 4                  # MAX_REFS = 4 entries in the lists
+    bad["ref"][3][0] = 4
+    bad["ref"][4][3] = 5
+    bad["ur_available"][4] = 1
     got = run_gpu(bad, 0, P_DPOC, P_MULTI)
-    assert got["status"][0] == 2
+    for i in range(6):
+        rc, _, _ = oracle_of(bad[i], 0, P_DPOC, P_MULTI)
+        assert (got["status"][i] == 2) == (rc == -2), i
+    assert list(got["status"][[0, 2, 3, 4]]) == [2, 2, 2, 2]
     assert got["status"][1] == (2 if bad["ur_available"][1] == 0 else got["status"][1])

@@ -1,6 +1,6 @@
 """GPU parity of the luma inter reconstruction (xme_inter_recon_batch: MC at
 the PU's clamped position -> sub_ps -> DCT / quant / dequant / IDCT ->
-add_ps -> SSD, encoder/rdo.c:895-945 and :1616-1633) against the oracle, for
+add_ps -> SSD, encoder/rdo.c:1333-1366, :1751-1753 and :1616-1633) against the oracle, for
 every TU size at several QPs, with MVs that get_mv_for_mc clamps, from the
 interpolated planes and from the full-pel plane alone (block filters), and a
 1080p frame of 8x8 TUs."""

@@ -10,7 +10,7 @@ import pytest
 from oracle import oracle as O
 
 # P-slice distances: ref 0 at 2, ref 1 at 4; ref_dpoc_multi = 16384 / dist
-DPOC = [2, 4, 6, 8, 10, 12, 14, 16]
+DPOC = [2, 4, 6, 8]
 MULTI = [16384 // d for d in DPOC]
 
 
@@ -93,15 +93,15 @@ def test_scaling_is_sign_symmetric(v, expected):
 
 
 def test_b_slice_uses_the_lists_distance():
-    dpoc = [2, 6, 0, 0, 0, 0, 0, 0]          # B_BWD = 0 at 2, B_FWD = 1 at 6
-    multi = [8192, 2731, 0, 0, 0, 0, 0, 0]
+    dpoc = [2, 6, 0, 0]                      # B_BWD = 0 at 2, B_FWD = 1 at 6
+    multi = [8192, 2731, 0, 0]
     j = job(refs=(0, -1, -1, -1), mvs=((10, -10), U, UL, UR))
     # forward list (bwd_2nd 0) uses index 1: (10 * 6 * 2731 + 8192) >> 14 = 10
     assert mvp(j, 1, dpoc, multi) == (10, -10)
     # backward list uses index 0: (10 * 2 * 8192 + 8192) >> 14 = 10
     assert mvp(dict(j, bwd_2nd=1), 1, dpoc, multi) == (10, -10)
     # a forward distance of 3 with the backward's multi halves it: (10 * 3 * 2731 + 8192) >> 14 = 5
-    assert mvp(j, 1, [2, 3, 0, 0, 0, 0, 0, 0], multi) == (5, -5)
+    assert mvp(j, 1, [2, 3, 0, 0], multi) == (5, -5)
 
 
 def test_invalid_neighbours_count_as_zero_in_the_median():
@@ -115,7 +115,7 @@ def test_invalid_neighbours_count_as_zero_in_the_median():
 
 def test_scaling_overflow_is_flagged():
     rc, _, _ = O.get_mvp_default(job(ref_idx=0, refs=(1, -1, -1, -1), mvs=((30000, 0), U, UL, UR)), 0,
-                                 [512] * 8, [16384] * 8)
+                                 [512] * 4, [16384] * 4)
     assert rc == -1
 
 
@@ -124,8 +124,8 @@ def test_scaling_rounds_exact_halves_away_from_zero(v, expected):
     # target ref 0 at distance 1, left neighbour on ref 1 (multi 8192): the scaled
     # value is v / 2, an exact half for odd v; sign * ((|v| * 8192 + 8192) >> 14)
     # rounds it away from zero, where a floor of the signed product would not
-    dpoc = [1, 2, 3, 4, 5, 6, 7, 8]
-    multi = [16384, 8192, 5461, 4096, 3276, 2730, 2340, 2048]
+    dpoc = [1, 2, 3, 4]
+    multi = [16384, 8192, 5461, 4096]
     got = mvp(job(ref_idx=0, refs=(1, -1, -1, -1), mvs=((v, -v), U, UL, UR)), 0, dpoc, multi)
     assert got == (expected, -expected)
 
@@ -137,3 +137,17 @@ def test_left_with_a_valid_top_right_is_a_median():
     # x: 4, 0, 7: |a-b| 4, |b-c| 7, |c-a| 3 -> (7 + 4) / 2 = 5
     # y: -8, 0, -3: |a-b| 8, |b-c| 3, |c-a| 5 -> (0 - 3) / 2 = -1
     assert got == (5, -1)
+
+
+@pytest.mark.parametrize("kw", [dict(ref_idx=4), dict(refs=(4, 0, 0, 0)), dict(refs=(0, -2, 0, 0)),
+                                dict(refs=(0, 0, 0, 4), ur=1), dict(refs=(0, 0, 4, 0), ur=0)])
+def test_reference_indices_outside_the_lists_are_refused(kw):
+    # the lists hold MAX_REFS = 4 entries (defines.h:364) and predict.h:100 asserts blkref < MAX_REFS
+    rc, _, _ = O.get_mvp_default(job(**kw), 0, DPOC, MULTI)
+    assert rc == -2
+
+
+def test_unused_neighbour_index_is_not_checked():
+    # with the top-right available the top-left is not read
+    rc, _, _ = O.get_mvp_default(job(refs=(0, 0, 7, 0), ur=1), 0, DPOC, MULTI)
+    assert rc == 0

@@ -1,5 +1,6 @@
 """Pins the oracle's luma inter reconstruction (xo_inter_recon_batch: the TU
-path of encoder/rdo.c:895-945 with the prediction of rdo.c:1616-1633):
+path of encoder/rdo.c:1333-1366, tu_recon_inter_luma, on the prediction error of
+rdo.c:1751-1753 with the prediction of rdo.c:1616-1633):
 
 * a known answer: the reference holds the picture moved by d and the MV is
   4d, so the prediction is exact: zero levels, nnz 0, the reconstruction is

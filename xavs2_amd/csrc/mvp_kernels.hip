@@ -7,7 +7,7 @@
 namespace {
 
 struct MvpTables {
-    int32_t dpoc[XME_MAX_REFS], multi[XME_MAX_REFS];
+    int32_t dpoc[XME_MVP_REFS], multi[XME_MVP_REFS];
 };
 
 __device__ __forceinline__ int iabs_(int v) { return v < 0 ? -v : v; }
@@ -68,10 +68,10 @@ __global__ void __launch_bounds__(256) mvp_kernel(const xme_mvp_job *__restrict_
     const int rl = j.ref[0], ru = j.ref[1], rur = j.ur_available ? j.ref[3] : j.ref[2];
     const int refs[3] = { rl, ru, rur };
     // a table index outside the reference lists cannot be derived
-    bool bad = j.ref_idx < 0 || j.ref_idx >= XME_MAX_REFS;
+    bool bad = j.ref_idx < 0 || j.ref_idx >= XME_MVP_REFS;
 #pragma unroll
     for (int k = 0; k < 3; k++)
-        bad |= refs[k] >= XME_MAX_REFS || refs[k] < -1;
+        bad |= refs[k] >= XME_MVP_REFS || refs[k] < -1;
     if (bad) {
         r.status = 2;
         out[i] = r;
@@ -106,8 +106,8 @@ __global__ void __launch_bounds__(256) mvp_kernel(const xme_mvp_job *__restrict_
 
 } // namespace
 
-extern "C" int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, const int32_t ref_dpoc[XME_MAX_REFS],
-                                 const int32_t ref_dpoc_multi[XME_MAX_REFS], xme_mvp_result *d_out, void *stream)
+extern "C" int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, const int32_t ref_dpoc[XME_MVP_REFS],
+                                 const int32_t ref_dpoc_multi[XME_MVP_REFS], xme_mvp_result *d_out, void *stream)
 {
     if (n < 0 || !ref_dpoc || !ref_dpoc_multi || (n > 0 && (!d_jobs || !d_out))) {
         xme_set_error("xme_get_mvp_batch: bad arguments");
@@ -116,7 +116,7 @@ extern "C" int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, 
     if (n == 0)
         return XME_OK;
     MvpTables t;
-    for (int k = 0; k < XME_MAX_REFS; k++) {
+    for (int k = 0; k < XME_MVP_REFS; k++) {
         t.dpoc[k] = ref_dpoc[k];
         t.multi[k] = ref_dpoc_multi[k];
     }

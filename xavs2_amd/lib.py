@@ -16,6 +16,7 @@ XME_OK = 0
 XME_ERR_ARG, XME_ERR_HIP, XME_ERR_NOMEM, XME_ERR_BUDGET, XME_ERR_STATE = -1, -2, -3, -4, -5
 XME_PAD = 80
 XME_MAX_REFS = 8
+XME_MVP_REFS = 4
 MAX_DISTORTION = 1 << 30
 PLAN_BATCHED, PLAN_PLAIN = 0, 1
 
