@@ -124,6 +124,17 @@ def lib():
         L.xo_residual_batch.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
         L.xo_get_mc_pos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_pmvr_adapt.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        for f in (L.xo_dct_part, L.xo_idct_part, L.xo_dct_half_part):
+            f.argtypes = [C.c_int, C.c_int, P, P, C.c_int]
+        L.xo_wavelet_fwd.argtypes = [C.c_int, C.c_int, P, P]
+        L.xo_wavelet_inv.argtypes = [C.c_int, C.c_int, P, P]
+        L.xo_intpl_luma_hor.argtypes = [P, ip, P, ip, P, ip, C.c_int, C.c_int, P]
+        L.xo_intpl_luma_ver.argtypes = [P, ip, P, ip, C.c_int, C.c_int, P]
+        L.xo_intpl_luma_ext.argtypes = [P, ip, P, ip, C.c_int, C.c_int, P]
+        L.xo_intpl_luma_block_ext.argtypes = [P, ip, P, ip, C.c_int, C.c_int, P, P]
+        L.xo_intpl_chroma_block_hor.argtypes = [P, ip, P, ip, C.c_int, C.c_int, P]
+        L.xo_intpl_chroma_block_ver.argtypes = [P, ip, P, ip, C.c_int, C.c_int, P]
+        L.xo_intpl_chroma_block_ext.argtypes = [P, ip, P, ip, C.c_int, C.c_int, P, P]
         L.xo_get_mvp_default.argtypes = [C.POINTER(MvpJob), C.c_int, P, P, P, P]
         L.xo_inter_recon_batch.argtypes = [C.c_int, C.c_int, P, ip, P, ip, C.c_int, C.c_int, P, C.c_int, P, P, ip,
                                            P, P]

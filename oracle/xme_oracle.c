@@ -1132,29 +1132,67 @@ int xo_interpolate_frame(uint8_t *const planes_in[16], intptr_t stride, int w, i
 }
 
 /* Block filters, mc.c:275-398 (8-bit: first-pass shift 0, second >> 12). */
-void xo_luma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx)
+/* The interpolation entries of g_funcs with their own taps (primitives.h:
+ * 157-171, the C versions at mc.c:275-547).  Sums are formed in int; the
+ * luma filters' intermediate tmp is mct_t (int16), the chroma one int32. */
+
+/* intpl_luma_hor_c (mc.c:402-418): tmp = int16(8-tap sum), dst = clip((sum + 32) >> 6). */
+void xo_intpl_luma_hor(uint8_t *dst, intptr_t i_dst, int16_t *tmp, intptr_t i_tmp, const uint8_t *src,
+                       intptr_t i_src, int w, int h, const int8_t *c)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const int v = fir8_u8(src + y * i_src + x, 1, c);
+            if (tmp)
+                tmp[y * i_tmp + x] = (int16_t)v;
+            dst[y * i_dst + x] = (uint8_t)clip_pel((v + 32) >> 6);
+        }
+}
+
+/* intpl_luma_ver_c (mc.c:422-435), also intpl_luma_block_ver (mc.c:358). */
+void xo_intpl_luma_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                       const int8_t *c)
 {
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++)
-            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_u8(src + y * i_src + x, 1, xo_luma_taps[dx]) + 32) >> 6);
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_u8(src + y * i_src + x, i_src, c) + 32) >> 6);
 }
 
-void xo_luma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy)
+/* intpl_luma_ext_c (mc.c:500-515): the 8-tap column sum of the int16 tmp
+ * rows, (sum + 2048) >> 12, clipped. */
+void xo_intpl_luma_ext(uint8_t *dst, intptr_t i_dst, const int16_t *tmp, intptr_t i_tmp, int w, int h,
+                       const int8_t *c)
 {
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++)
-            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_u8(src + y * i_src + x, i_src, xo_luma_taps[dy]) + 32) >> 6);
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_s16(tmp + y * i_tmp + x, i_tmp, c) + 2048) >> 12);
 }
 
-void xo_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy)
+/* intpl_luma_block_ext_c (mc.c:364-398): rows -3 .. h+3 filtered across into
+ * int16, then down. */
+void xo_intpl_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                             const int8_t *ch, const int8_t *cv)
 {
     int16_t tmp[(64 + 7) * 64];
     for (int y = -3; y < h + 4; y++)
         for (int x = 0; x < w; x++)
-            tmp[(y + 3) * 64 + x] = (int16_t)fir8_u8(src + y * i_src + x, 1, xo_luma_taps[dx]);
-    for (int y = 0; y < h; y++)
-        for (int x = 0; x < w; x++)
-            dst[y * i_dst + x] = (uint8_t)clip_pel((fir8_s16(tmp + (y + 3) * 64 + x, 64, xo_luma_taps[dy]) + 2048) >> 12);
+            tmp[(y + 3) * 64 + x] = (int16_t)fir8_u8(src + y * i_src + x, 1, ch);
+    xo_intpl_luma_ext(dst, i_dst, tmp + 3 * 64, 64, w, h, cv);
+}
+
+void xo_luma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx)
+{
+    xo_intpl_luma_hor(dst, i_dst, NULL, 0, src, i_src, w, h, xo_luma_taps[dx]);
+}
+
+void xo_luma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy)
+{
+    xo_intpl_luma_ver(dst, i_dst, src, i_src, w, h, xo_luma_taps[dy]);
+}
+
+void xo_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy)
+{
+    xo_intpl_luma_block_ext(dst, i_dst, src, i_src, w, h, xo_luma_taps[dx], xo_luma_taps[dy]);
 }
 
 static inline int fir4_u8(const uint8_t *s, intptr_t step, const int8_t *c)
@@ -1162,33 +1200,53 @@ static inline int fir4_u8(const uint8_t *s, intptr_t step, const int8_t *c)
     return c[0] * s[-step] + c[1] * s[0] + c[2] * s[step] + c[3] * s[2 * step];
 }
 
-void xo_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx)
+/* intpl_chroma_block_hor_c / _ver_c (mc.c:275-303): 4 taps at -1 .. 2. */
+void xo_intpl_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                               const int8_t *c)
 {
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++)
-            dst[y * i_dst + x] = (uint8_t)clip_pel((fir4_u8(src + y * i_src + x, 1, xo_chroma_taps[dx]) + 32) >> 6);
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir4_u8(src + y * i_src + x, 1, c) + 32) >> 6);
 }
 
-void xo_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy)
+void xo_intpl_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                               const int8_t *c)
 {
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++)
-            dst[y * i_dst + x] = (uint8_t)clip_pel((fir4_u8(src + y * i_src + x, i_src, xo_chroma_taps[dy]) + 32) >> 6);
+            dst[y * i_dst + x] = (uint8_t)clip_pel((fir4_u8(src + y * i_src + x, i_src, c) + 32) >> 6);
 }
 
-void xo_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy)
+/* intpl_chroma_block_ext_c (mc.c:307-337): rows -1 .. h+1 across into int32,
+ * then down with (sum + 2048) >> 12. */
+void xo_intpl_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                               const int8_t *ch, const int8_t *cv)
 {
     int32_t tmp[(32 + 3) * 32];
-    const int8_t *cv = xo_chroma_taps[dy];
     for (int y = -1; y < h + 2; y++)
         for (int x = 0; x < w; x++)
-            tmp[(y + 1) * 32 + x] = fir4_u8(src + y * i_src + x, 1, xo_chroma_taps[dx]);
+            tmp[(y + 1) * 32 + x] = fir4_u8(src + y * i_src + x, 1, ch);
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++) {
             const int32_t *t = tmp + (y + 1) * 32 + x;
             int v = cv[0] * t[-32] + cv[1] * t[0] + cv[2] * t[32] + cv[3] * t[64];
             dst[y * i_dst + x] = (uint8_t)clip_pel((v + 2048) >> 12);
         }
+}
+
+void xo_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx)
+{
+    xo_intpl_chroma_block_hor(dst, i_dst, src, i_src, w, h, xo_chroma_taps[dx]);
+}
+
+void xo_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy)
+{
+    xo_intpl_chroma_block_ver(dst, i_dst, src, i_src, w, h, xo_chroma_taps[dy]);
+}
+
+void xo_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy)
+{
+    xo_intpl_chroma_block_ext(dst, i_dst, src, i_src, w, h, xo_chroma_taps[dx], xo_chroma_taps[dy]);
 }
 
 /* mc_luma, mc.c:559-589: copy from the interpolated plane when it exists. */
@@ -1432,37 +1490,47 @@ static const int16_t *tmat(int n)
     }
 }
 
-/* One forward pass (partialButterflyN, transform.c:167-498): for every input
- * row j, out[k][j] = int16((sum_n T[k][n] * in[j][n] + rnd) >> shift).  The
- * butterflies compute exactly this sum because T[k][N-1-n] = (-1)^k T[k][n]
- * (checked by the tests), and every partial sum fits in int32. */
-static void fwd_pass(const int16_t *in, int16_t *out, int n, int shift)
+/* One forward pass (partialButterflyN(src, dst, shift, line), transform.c:
+ * 167-498): an n-point transform of each of `lines` input rows, written
+ * transposed: out[k][j] = int16((sum_m T[k][m] * in[j][m] + rnd) >> shift),
+ * out having `lines` columns.  The butterflies compute exactly this sum
+ * because T[k][N-1-m] = (-1)^k T[k][m] (checked by the tests), and every
+ * partial sum fits in int32. */
+static void fwd_pass_n(const int16_t *in, int16_t *out, int n, int lines, int shift)
 {
     const int16_t *T = tmat(n);
     const int add = (1 << shift) >> 1;
-    for (int j = 0; j < n; j++)
+    for (int j = 0; j < lines; j++)
         for (int k = 0; k < n; k++) {
             int s = 0;
             for (int m = 0; m < n; m++)
                 s += T[k * n + m] * in[j * n + m];
-            out[k * n + j] = (int16_t)((s + add) >> shift);
+            out[k * lines + j] = (int16_t)((s + add) >> shift);
         }
 }
 
-/* One inverse pass (partialButterflyInverseN): for every column j,
+static void fwd_pass(const int16_t *in, int16_t *out, int n, int shift) { fwd_pass_n(in, out, n, n, shift); }
+
+/* One inverse pass (partialButterflyInverseN(src, dst, shift, line, clip)):
+ * for each of the `lines` columns j of in (n rows),
  * out[j][m] = clip((sum_k T[k][m] * in[k][j] + rnd) >> shift). */
-static void inv_pass(const int16_t *in, int16_t *out, int n, int shift, int clip_depth)
+static void inv_pass_n(const int16_t *in, int16_t *out, int n, int lines, int shift, int clip_depth)
 {
     const int16_t *T = tmat(n);
     const int add = (1 << shift) >> 1;
     const int hi = ((1 << clip_depth) >> 1) - 1, lo = -hi - 1;
-    for (int j = 0; j < n; j++)
+    for (int j = 0; j < lines; j++)
         for (int m = 0; m < n; m++) {
             int s = 0;
             for (int k = 0; k < n; k++)
-                s += T[k * n + m] * in[k * n + j];
+                s += T[k * n + m] * in[k * lines + j];
             out[j * n + m] = (int16_t)clip3(lo, hi, (s + add) >> shift);
         }
+}
+
+static void inv_pass(const int16_t *in, int16_t *out, int n, int shift, int clip_depth)
+{
+    inv_pass_n(in, out, n, n, shift, clip_depth);
 }
 
 const int16_t *xo_tmat(int n) { return tmat(n); }
@@ -1501,6 +1569,185 @@ void xo_idct(int n, const int16_t *src, int16_t *dst, int i_dst)
     inv_pass(tmp, block, n, 12 - a_flag, 9 + a_flag);
     for (int y = 0; y < n; y++)
         memcpy(dst + y * i_dst, block + y * n, sizeof(int16_t) * (size_t)n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* The dct_t table: dctf.dct / idct / dct_half for every size the reference
+ * installs (transform.c:1599-1643).                                          */
+
+/* A W x H block transform (dct_WxH_c, transform.c:1123-1490): the W-point
+ * transform of each row with shift1 = log2(W) - 2, then the H-point
+ * transform of each column with shift2 = log2(H) + 5 (for 8-bit video:
+ * B*_IN_BIT + FACTO_BIT + bit depth + 1 - LIMIT_BIT and B*_IN_BIT +
+ * FACTO_BIT, with the "- 2" of the asymmetric shapes).  The 32x32, 32x8 and
+ * 8x32 transforms take a wavelet flag in the low bit of the stride and read
+ * the stride masked with 0xFE; the flag adds 1 to shift1 of 32x32 and 8x32
+ * and takes 1 from shift2 of 32x8. */
+static int takes_flag(int w, int h) { return (w == 32 && h == 32) || (w == 32 && h == 8) || (w == 8 && h == 32); }
+
+static void dct_rect(int w, int h, const int16_t *src, int16_t *dst, int i_src)
+{
+    int16_t block[32 * 32], tmp[32 * 32];
+    int s1 = ilog2(w) - 2, s2 = ilog2(h) + 5;
+    if (takes_flag(w, h)) {
+        const int f = i_src & 1;
+        if (w == 32 && h == 8)
+            s2 -= f;
+        else
+            s1 += f;
+        i_src &= 0xFE;
+    }
+    for (int y = 0; y < h; y++)
+        memcpy(block + y * w, src + (intptr_t)y * i_src, sizeof(int16_t) * (size_t)w);
+    fwd_pass_n(block, tmp, w, h, s1);
+    fwd_pass_n(tmp, dst, h, w, s2);
+}
+
+/* idct_WxH_c (transform.c:1145-1512): the H-point inverse of each column with
+ * shift 5 clipped to 16 bits, then the W-point inverse of each row with shift
+ * 12 clipped to 9 bits (the wavelet flag: shift 11, clipped to 10 bits). */
+static void idct_rect(int w, int h, const int16_t *src, int16_t *dst, int i_dst)
+{
+    int16_t block[32 * 32], tmp[32 * 32];
+    int f = 0;
+    if (takes_flag(w, h)) {
+        f = i_dst & 1;
+        i_dst &= 0xFE;
+    }
+    inv_pass_n(src, tmp, h, w, 5, 16);
+    inv_pass_n(tmp, block, w, h, 12 - f, 9 + f);
+    for (int y = 0; y < h; y++)
+        memcpy(dst + (intptr_t)y * i_dst, block + y * w, sizeof(int16_t) * (size_t)w);
+}
+
+/* One forward lifting pass of the 64-wide transforms (wavelet_64x64_c and
+ * kin, transform.c:586-785) over n samples x[0], x[step], ...: the signal is
+ * extended by mirroring two samples a side (x[-1] = x[1], x[-2] = x[2],
+ * x[n] = x[n-2], x[n+1] = x[n-3]), the odd samples -1, 1, ..., n-1 become
+ * x - ((left + right) >> 1), then the even samples become
+ * x + ((left + right + 2) >> 2) (rows) or 2x + ((left + right + 1) >> 1)
+ * (columns); the n/2 even samples are the low band.  Every store is to a
+ * coeff_t (int16). */
+static void lift_fwd(const int16_t *in, int step_in, int16_t *out, int step_out, int n, int columns)
+{
+    int16_t e[64 + 4] = { 0 };
+    int16_t *x = e + 2;
+    for (int i = 0; i < n; i++)
+        x[i] = in[i * step_in];
+    x[-1] = x[1];
+    x[-2] = x[2];
+    x[n] = x[n - 2];
+    x[n + 1] = x[n - 3];
+    for (int i = -1; i < n; i += 2)
+        x[i] = (int16_t)(x[i] - ((x[i - 1] + x[i + 1]) >> 1));
+    for (int i = 0; i < n; i += 2)
+        x[i] = columns ? (int16_t)((x[i] << 1) + ((x[i - 1] + x[i + 1] + 1) >> 1))
+                       : (int16_t)(x[i] + ((x[i - 1] + x[i + 1] + 2) >> 2));
+    for (int i = 0; i < n / 2; i++)
+        out[i * step_out] = x[2 * i];
+}
+
+/* The forward wavelet of a W x H block (W, H in {16, 64}, one of them 64):
+ * rows of the contiguous source (stride W), then columns of the half-width
+ * result; out is the (W/2) x (H/2) low band, contiguous. */
+static void wavelet_fwd(int w, int h, const int16_t *src, int16_t *out)
+{
+    int16_t mid[64 * 64];
+    for (int y = 0; y < h; y++)
+        lift_fwd(src + y * w, 1, mid + y * (w / 2), 1, w, 0);
+    for (int x = 0; x < w / 2; x++)
+        lift_fwd(mid + x, w / 2, out + x, w / 2, h, 1);
+}
+
+/* One inverse lifting pass (inv_wavelet_64x64_c and kin, transform.c:787-
+ * 956): the n/2 low-band samples go to the even places, x[n] = x[n-2]; for
+ * columns every even sample 0..n is halved first; each odd sample becomes
+ * (left + right) >> 1. */
+static void lift_inv(const int16_t *in, int step_in, int16_t *out, int step_out, int n, int columns)
+{
+    int16_t e[64 + 2] = { 0 };
+    for (int i = 0; i < n / 2; i++)
+        e[2 * i] = in[i * step_in];
+    e[n] = e[n - 2];
+    if (columns)
+        for (int i = 0; i <= n; i += 2)
+            e[i] = (int16_t)(e[i] >> 1);
+    for (int i = 1; i < n; i += 2)
+        e[i] = (int16_t)((e[i - 1] + e[i + 1]) >> 1);
+    for (int i = 0; i < n; i++)
+        out[i * step_out] = e[i];
+}
+
+/* The inverse wavelet: columns of the (W/2) x (H/2) low band (contiguous),
+ * then rows; out is W x H, contiguous. */
+static void wavelet_inv(int w, int h, const int16_t *low, int16_t *out)
+{
+    int16_t mid[64 * 64];
+    for (int x = 0; x < w / 2; x++)
+        lift_inv(low + x, w / 2, mid + x, w / 2, h, 1);
+    for (int y = 0; y < h; y++)
+        lift_inv(mid + y * (w / 2), 1, out + y * w, 1, w, 0);
+}
+
+void xo_wavelet_fwd(int w, int h, const int16_t *src, int16_t *low) { wavelet_fwd(w, h, src, low); }
+void xo_wavelet_inv(int w, int h, const int16_t *low, int16_t *out) { wavelet_inv(w, h, low, out); }
+
+static int dct_table_size(int w, int h)
+{
+    return (w == h && (w == 4 || w == 8 || w == 16 || w == 32 || w == 64)) || (w == 16 && h == 4) ||
+           (w == 4 && h == 16) || (w == 32 && h == 8) || (w == 8 && h == 32) || (w == 64 && h == 16) ||
+           (w == 16 && h == 64);
+}
+
+/* dctf.dct[PART_INDEX(w, h)]: the W x H transforms above; the 64-wide sizes
+ * (dct_64x64_c, dct_64x16_c, dct_16x64_c, transform.c:1514-1580) read a
+ * contiguous source whatever i_src, take the wavelet's low band and run the
+ * half-size transform on it with the wavelet flag, leaving (W/2) x (H/2)
+ * coefficients in dst.  Returns -1 for a size the table does not hold. */
+int xo_dct_part(int w, int h, const int16_t *src, int16_t *dst, int i_src)
+{
+    if (!dct_table_size(w, h))
+        return -1;
+    if (w == 64 || h == 64) {
+        int16_t low[32 * 32];
+        wavelet_fwd(w, h, src, low);
+        dct_rect(w / 2, h / 2, low, dst, (w / 2) | 1);
+        return 0;
+    }
+    dct_rect(w, h, src, dst, i_src);
+    return 0;
+}
+
+/* dctf.idct[PART_INDEX(w, h)]; the 64-wide sizes (idct_64x64_c and kin)
+ * invert the half-size transform with the wavelet flag and then the wavelet,
+ * writing W x H contiguous samples whatever i_dst. */
+int xo_idct_part(int w, int h, const int16_t *src, int16_t *dst, int i_dst)
+{
+    if (!dct_table_size(w, h))
+        return -1;
+    if (w == 64 || h == 64) {
+        int16_t low[32 * 32];
+        idct_rect(w / 2, h / 2, src, low, (w / 2) | 1);
+        wavelet_inv(w, h, low, dst);
+        return 0;
+    }
+    idct_rect(w, h, src, dst, i_dst);
+    return 0;
+}
+
+/* dctf.dct_half (dct_32x32_half_c, dct_64x64_half_c, transform.c:1276-1286,
+ * :1525-1530): the 32x32 transform (on the wavelet's low band for 64x64)
+ * with every coefficient outside the top-left 16x16 cleared. */
+int xo_dct_half_part(int w, int h, const int16_t *src, int16_t *dst, int i_src)
+{
+    if (!((w == 32 && h == 32) || (w == 64 && h == 64)))
+        return -1;
+    xo_dct_part(w, h, src, dst, i_src);
+    for (int y = 0; y < 32; y++)
+        for (int x = 0; x < 32; x++)
+            if (x >= 16 || y >= 16)
+                dst[y * 32 + x] = 0;
+    return 0;
 }
 
 /* Quantisation tables, common/quant.c:55-97. */

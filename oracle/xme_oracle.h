@@ -271,6 +271,23 @@ void xo_mc_chroma(uint8_t *dst_u, uint8_t *dst_v, intptr_t i_dst, int pix_quad_x
 void xo_luma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx);
 void xo_luma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy);
 void xo_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy);
+/* The interpolation members of g_funcs (primitives.h:157-171; C versions
+ * mc.c:275-547) with their own 8 (luma) or 4 (chroma) taps.  tmp of
+ * xo_intpl_luma_hor may be NULL (intpl_luma_block_hor). */
+void xo_intpl_luma_hor(uint8_t *dst, intptr_t i_dst, int16_t *tmp, intptr_t i_tmp, const uint8_t *src,
+                       intptr_t i_src, int w, int h, const int8_t *coeff);
+void xo_intpl_luma_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                       const int8_t *coeff);
+void xo_intpl_luma_ext(uint8_t *dst, intptr_t i_dst, const int16_t *tmp, intptr_t i_tmp, int w, int h,
+                       const int8_t *coeff);
+void xo_intpl_luma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                             const int8_t *coeff_h, const int8_t *coeff_v);
+void xo_intpl_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                               const int8_t *coeff);
+void xo_intpl_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                               const int8_t *coeff);
+void xo_intpl_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h,
+                               const int8_t *coeff_h, const int8_t *coeff_v);
 void xo_chroma_block_hor(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx);
 void xo_chroma_block_ver(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dy);
 void xo_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intptr_t i_src, int w, int h, int dx, int dy);
@@ -278,6 +295,20 @@ void xo_chroma_block_ext(uint8_t *dst, intptr_t i_dst, const uint8_t *src, intpt
 /* ---- transform and quantisation (common/transform.c, common/quant.c) --- */
 void xo_dct(int n, const int16_t *src, int16_t *dst, int i_src);   /* dct_NxN_c, dst contiguous */
 void xo_idct(int n, const int16_t *src, int16_t *dst, int i_dst);  /* idct_NxN_c, src contiguous */
+/* The dct_t table (transform.c:1599-1643): dctf.dct / idct / dct_half of a
+ * W x H block for the sizes the reference installs (4x4 .. 64x64, 16x4,
+ * 4x16, 32x8, 8x32, 64x16, 16x64; dct_half: 32x32 and 64x64), with the
+ * reference's stride conventions (the wavelet flag in the low bit of the
+ * 32x32 / 32x8 / 8x32 stride, contiguous blocks for the 64-wide sizes, whose
+ * coefficients are the (W/2) x (H/2) transform of the wavelet's low band).
+ * Coefficients are contiguous.  Return -1 for a size without an entry. */
+int  xo_dct_part(int w, int h, const int16_t *src, int16_t *dst, int i_src);
+/* The lifting wavelet of the 64-wide sizes alone (64x64, 64x16, 16x64):
+ * W x H contiguous samples <-> the (W/2) x (H/2) low band. */
+void xo_wavelet_fwd(int w, int h, const int16_t *src, int16_t *low);
+void xo_wavelet_inv(int w, int h, const int16_t *low, int16_t *out);
+int  xo_idct_part(int w, int h, const int16_t *src, int16_t *dst, int i_dst);
+int  xo_dct_half_part(int w, int h, const int16_t *src, int16_t *dst, int i_src);
 int  xo_quant(int16_t *coef, int n, int scale, int shift, int add);
 void xo_dequant(int16_t *coef, int n, int scale, int shift);
 /* Parameters of tu_quant_forward / tu_quant_inverse (encoder/rdo.c:658-718). */
