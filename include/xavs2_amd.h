@@ -50,6 +50,7 @@ extern "C" {
 
 typedef uint8_t pel_t;       /* source/common/basic_types.h:50 */
 typedef int16_t coeff_t;     /* source/common/basic_types.h:60 */
+typedef int16_t mct_t;       /* source/common/basic_types.h:59 (8-bit build) */
 
 const char *xme_last_error(void);
 /* Library version string. */
@@ -88,6 +89,62 @@ int xme_get_sad_table(xme_pixel_cmp_t table[XME_NUM_PU]);      /* replaces pixf.
 int xme_get_satd_table(xme_pixel_cmp_t table[XME_NUM_PU]);     /* replaces pixf.satd   (pixel.c:1144) */
 int xme_get_sad_x3_table(xme_pixel_cmp_x3_t table[XME_NUM_PU]);/* replaces pixf.sad_x3 (pixel.c:1142) */
 int xme_get_sad_x4_table(xme_pixel_cmp_x4_t table[XME_NUM_PU]);/* replaces pixf.sad_x4 (pixel.c:1143) */
+
+/* dct_t (source/common/primitives.h:92): dctf.dct / idct / dct_half[part].
+ * The getter fills the entries xavs2_dct_init installs (transform.c:1605-1643:
+ * 4x4, 8x8, 16x16, 32x32, 64x64, 16x4, 4x16, 32x8, 8x32, 64x16, 16x64;
+ * dct_half at 32x32 and 64x64) and NULL elsewhere, as the reference leaves
+ * them.  Conventions as the reference's: the low bit of the 32x32 / 32x8 /
+ * 8x32 stride is the wavelet flag (the stride read masked with 0xFE); the
+ * 64-wide sizes read and write contiguous blocks, their coefficients the
+ * (W/2) x (H/2) transform of the lifting wavelet's low band; src == dst is
+ * allowed. */
+typedef void (*xme_dct_t)(const coeff_t *src, coeff_t *dst, int i_src);
+int xme_get_dct_table(xme_dct_t dct[XME_NUM_PU], xme_dct_t idct[XME_NUM_PU], xme_dct_t dct_half[XME_NUM_PU]);
+/* dctf.quant / dctf.dequant (primitives.h:116-117; quant_c / dequant_c,
+ * source/common/quant.c:109-184): in place on i_coef coefficients; quant
+ * returns the nonzero count. */
+typedef int (*xme_quant_t)(coeff_t *coef, const int i_coef, const int scale, const int shift, const int add);
+typedef void (*xme_dequant_t)(coeff_t *coef, const int i_coef, const int scale, const int shift);
+int xme_get_quant_funcs(xme_quant_t *quant, xme_dequant_t *dequant);
+/* pixel_avg_pp_t (source/common/pixel.h:137): pixf.avg[part] (pixel.c:680-719),
+ * (src0 + src1 + 1) >> 1 over the part's W x H; weight is unused. */
+typedef void (*xme_pixel_avg_pp_t)(pel_t *dst, intptr_t dstride, const pel_t *src0, intptr_t sstride0,
+                                   const pel_t *src1, intptr_t sstride1, int weight);
+int xme_get_avg_table(xme_pixel_avg_pp_t table[XME_NUM_PU]);
+/* The interpolation members of g_funcs (source/common/primitives.h:157-171,
+ * types :62-71; C versions source/common/mc.c:275-547), any block size, with
+ * the caller's taps: luma 8 taps at -3 .. +4, chroma 4 at -1 .. +2; the
+ * intermediate of the luma filters is mct_t (int16), of the chroma one int32.
+ * intpl_luma_hor also writes the raw 8-tap sums to tmp; intpl_luma_ext filters
+ * such int16 rows down. */
+typedef struct {
+    void (*intpl_luma_hor)(pel_t *dst, int i_dst, mct_t *tmp, int i_tmp, pel_t *src, int i_src, int width, int height,
+                           const int8_t *coeff);                                          /* intpl_luma_hor_t */
+    void (*intpl_luma_ver)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                           int8_t const *coeff);                                          /* intpl_luma_ver_t */
+    void (*intpl_luma_ext)(pel_t *dst, int i_dst, mct_t *tmp, int i_tmp, int width, int height,
+                           const int8_t *coeff);                                          /* intpl_luma_ext_t */
+    void (*intpl_luma_ver_x3)(pel_t *const dst[3], int i_dst, pel_t *src, int i_src, int width, int height,
+                              int8_t const **coeff);                                      /* intpl_luma_ver_x3_t */
+    void (*intpl_luma_hor_x3)(pel_t *const dst[3], int i_dst, mct_t *const tmp[3], int i_tmp, pel_t *src,
+                              int i_src, int width, int height, const int8_t **coeff);    /* intpl_luma_hor_x3_t */
+    void (*intpl_luma_ext_x3)(pel_t *const dst[3], int i_dst, mct_t *tmp, int i_tmp, int width, int height,
+                              const int8_t **coeff);                                      /* intpl_luma_ext_x3_t */
+    void (*intpl_luma_block_hor)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                                 const int8_t *coeff);                                    /* intpl_t */
+    void (*intpl_luma_block_ver)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                                 const int8_t *coeff);                                    /* intpl_t */
+    void (*intpl_luma_block_ext)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                                 const int8_t *coeff_x, const int8_t *coeff_y);           /* intpl_ext_t */
+    void (*intpl_chroma_block_hor)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                                   const int8_t *coeff);                                  /* intpl_t */
+    void (*intpl_chroma_block_ver)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                                   const int8_t *coeff);                                  /* intpl_t */
+    void (*intpl_chroma_block_ext)(pel_t *dst, int i_dst, pel_t *src, int i_src, int width, int height,
+                                   const int8_t *coeff_x, const int8_t *coeff_y);         /* intpl_ext_t */
+} xme_intpl_funcs;
+int xme_get_intpl_funcs(xme_intpl_funcs *funcs);
 
 /* ---- 2. batched primitives on device buffers --------------------------- *
  * Call i of a batch evaluates the g_funcs member on (base + off[i]) pointers. */
@@ -382,13 +439,19 @@ int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t 
  * (source/encoder/rdo.c:1333-1366, inter quant with b_intra = 0) without
  * RDOQ, and the SSD of cu_recon_inter_luma (rdo.c:1497-1499) per TU.
  * d_tus: ntu x {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y} (MV in qpel,
- * the TU inside the PU).  d_resid (prediction error) and d_rres
- * (reconstructed residual) are ntu * n * n coefficients each; d_level holds
- * the quantised levels, d_nnz / d_ssd one value per TU.  A TU outside the
- * picture, not inside its PU, at a column that is not a multiple of 4, or
- * whose PU is not 4..64 on each side gets ssd -1, zero levels and no pixels.
- * d_cur / d_rec: 4-byte aligned with strides % 4 == 0; d_resid / d_rres:
- * 8-byte aligned. */
+ * the TU inside the PU).  d_level holds the quantised levels (ntu * n * n),
+ * d_nnz / d_ssd one value per TU.  d_resid (prediction error) and d_rres
+ * (reconstructed residual), ntu * n * n coefficients each, are written when
+ * given; with every interpolated plane of ref resident the path is one
+ * launch and they may be NULL, otherwise (block-filter prediction) they are
+ * required as the stages' intermediates.  A TU outside the picture, not
+ * inside its PU, at a column that is not a multiple of 4, or whose PU is not
+ * 4..64 on each side gets ssd -1, zero levels and no pixels.  d_cur / d_rec:
+ * 4-byte aligned with strides % 4 == 0; d_resid / d_rres: 8-byte aligned.
+ * n = 32 | XME_TU_DCT_HALF transforms with dct_half instead, as
+ * tu_get_dct_coeff (rdo.c:764-771) does for 32x32 TUs in the first RD pass
+ * under OPT_BIT_EST_PSZT (the default preset's, presets.c:277). */
+#define XME_TU_DCT_HALF 0x100
 int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
                           int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
                           coeff_t *d_level, coeff_t *d_rres, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz,

@@ -10,13 +10,20 @@
 // min(frame, n_ref) frames with the stream's plan, and then border-expanded
 // and interpolated in place so that it becomes the newest reference.
 //
-// Scheduling: frames are admitted in arrival order whenever one of the
-// max_inflight slots (one HIP stream each) is free and the frame's working
-// memory fits the budget beside everything resident.  Frames admitted into
-// different slots overlap on the device; a frame's search waits (by an event,
-// on the device) only for the previous frame of its own stream to have become
-// a reference.  Results are copied back into pinned host memory and a host
-// callback stamps the completion time, frees the slot and wakes the scheduler.
+// Scheduling: whenever one of the max_inflight slots (one HIP stream each) is
+// free, the scheduler admits, among the queued frames that are next in their
+// own stream, the one with the earliest deadline (submit time plus twice its
+// search work at a nominal device rate, so a small frame does not wait behind
+// a large one that arrived just before it), if its working memory fits the
+// budget beside everything resident.  Frames admitted into different slots
+// overlap on the device; a frame's search waits (by an event, on the device)
+// only for the previous frame of its own stream to have become a reference.
+// Results are copied back into pinned host memory and a host callback stamps
+// the completion time, frees the slot and wakes the scheduler; the frame's
+// reference build follows on the slot's stream, and a second callback marks
+// it done, which is what closing the stream waits for.  A failed reference
+// build marks the stream failed: its later frames and submissions return
+// that error.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -111,6 +118,7 @@ struct Job {
     xme_result *d_res = nullptr;
     size_t d_res_bytes = 0;
     Clock::time_point t_submit, t_done;
+    double deadline = 0;       // admission order: seconds since the server's start
     int slot = -1;
     int rc = XME_OK;
     bool done = false;
@@ -128,8 +136,13 @@ struct Stream {
     std::vector<xme_plan *> plans;   // plans[r - 1] searches r references
     hipEvent_t ref_ready[2] = { nullptr, nullptr };
     int64_t next_frame = 0;
+    int64_t next_admit = 0;      // the frame the scheduler may admit next
     int64_t resident = 0;
+    int64_t work = 0;            // byte absdiffs of a frame's search with n_ref references
     int pending = 0;             // submitted and not yet done
+    int building = 0;            // admitted frames whose reference build has not finished
+    int failed = XME_OK;         // a reference build failed: the stream cannot go on
+    xme_server *srv = nullptr;
     bool closing = false;
     std::mutex submit_mu;        // one submission of the stream at a time: frame order
 
@@ -164,8 +177,24 @@ struct xme_server {
     std::vector<double> lat_ms;
     bool have_first = false;
     Clock::time_point t_first, t_last;
+    Clock::time_point t_start = Clock::now();
 
-    bool can_admit_locked() const
+    // The queued frame to admit next: the earliest deadline among frames that
+    // are next in their stream (ties by arrival); null when none may go.
+    std::deque<Job *>::iterator pick_locked()
+    {
+        auto best = queue.end();
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            Job *j = *it;
+            if (j->frame != j->st->next_admit)
+                continue;
+            if (best == queue.end() || j->deadline < (*best)->deadline)
+                best = it;
+        }
+        return best;
+    }
+
+    bool can_admit_locked()
     {
         if (queue.empty())
             return false;
@@ -174,7 +203,8 @@ struct xme_server {
             slot = slot || !slot_busy[i];
         if (!slot)
             return false;
-        return resident + inflight + (int64_t)queue.front()->dev_bytes <= cfg.memory_budget;
+        auto it = pick_locked();
+        return it != queue.end() && resident + inflight + (int64_t)(*it)->dev_bytes <= cfg.memory_budget;
     }
 };
 
@@ -204,13 +234,24 @@ void finish_locked(xme_server *S, Job *j)
 }
 
 // Runs on a HIP runtime thread once the job's results are in host memory; it
-// only does bookkeeping (no HIP calls are allowed here).
+// only does bookkeeping (no HIP calls are allowed here).  After it the job may
+// be gone (a waiter deletes it).
 void on_results(void *arg)
 {
     Job *j = static_cast<Job *>(arg);
     xme_server *S = j->srv;
     std::lock_guard<std::mutex> lk(S->mu);
     finish_locked(S, j);
+}
+
+// Runs once a frame's reference build (and its event) has finished.
+void on_built(void *arg)
+{
+    Stream *st = static_cast<Stream *>(arg);
+    xme_server *S = st->srv;
+    std::lock_guard<std::mutex> lk(S->mu);
+    st->building--;
+    S->cv_done.notify_all();
 }
 
 // Device memory for a job: a cached block of the size, else a fresh one,
@@ -242,12 +283,20 @@ void *host_get(xme_server *S, size_t n)
     return hipHostMalloc(&p, n, hipHostMallocDefault) == hipSuccess ? p : nullptr;
 }
 
-int launch_job(xme_server *S, Job *j)
+// Queues a frame's work on its slot's stream.  *results_queued tells whether
+// the results callback was queued (after which the job must not be touched).
+int launch_job(xme_server *S, Job *j, bool *results_queued)
 {
+    *results_queued = false;
     Stream *st = j->st;
     const int n_slots = (int)st->ring.size();
-    const int cur_slot = (int)(j->frame % n_slots);
+    const int64_t frame = j->frame;      // j may be gone once on_results is queued
+    const int cur_slot = (int)(frame % n_slots);
     hipStream_t s = S->slots[j->slot];
+    if (st->failed != XME_OK) {
+        xme_set_error("xme_server: stream %d cannot go on (an earlier reference build failed)", st->id);
+        return st->failed;
+    }
     j->d_cur = static_cast<uint8_t *>(dev_get(S, j->d_cur_bytes));
     if (j->d_res_bytes)
         j->d_res = static_cast<xme_result *>(dev_get(S, j->d_res_bytes));
@@ -279,6 +328,7 @@ int launch_job(xme_server *S, Job *j)
         XME_HIP(hipMemcpyAsync(j->h_res, j->d_res, j->d_res_bytes, hipMemcpyDeviceToHost, s));
     }
     XME_HIP(hipLaunchHostFunc(s, on_results, j));
+    *results_queued = true;
     // the frame becomes its stream's newest reference
     int rc = xme_expand_border(cur, st->stride, st->w, st->h, XME_PAD, s);
     if (rc)
@@ -291,7 +341,8 @@ int launch_job(xme_server *S, Job *j)
         if (rc)
             return rc;
     }
-    XME_HIP(hipEventRecord(st->ref_ready[j->frame & 1], s));
+    XME_HIP(hipEventRecord(st->ref_ready[frame & 1], s));
+    XME_HIP(hipLaunchHostFunc(s, on_built, st));
     return XME_OK;
 }
 
@@ -303,8 +354,12 @@ void sched_loop(xme_server *S)
         S->cv_sched.wait(lk, [S] { return S->can_admit_locked() || (S->stop && S->queue.empty()); });
         if (!S->can_admit_locked())
             break;                       // stopping with an empty queue
-        Job *j = S->queue.front();
-        S->queue.pop_front();
+        auto it = S->pick_locked();
+        Job *j = *it;
+        S->queue.erase(it);
+        Stream *st = j->st;
+        st->next_admit++;
+        st->building++;
         int slot = 0;
         while (S->slot_busy[slot])
             slot++;
@@ -314,18 +369,23 @@ void sched_loop(xme_server *S)
         S->peak = std::max(S->peak, S->resident + S->inflight);
         S->running++;
         lk.unlock();
-        const int rc = launch_job(S, j);
+        bool results_queued = false;
+        const int rc = launch_job(S, j, &results_queued);
+        if (rc != XME_OK) {
+            // nothing after the failing call was queued: wait for what was
+            (void)hipStreamSynchronize(S->slots[slot]);
+        }
         lk.lock();
         if (rc != XME_OK) {
-            // nothing after the failing call was queued: wait for what was,
-            // then release the job with its error
-            lk.unlock();
-            (void)hipStreamSynchronize(S->slots[j->slot]);
-            lk.lock();
-            if (!j->done) {
+            if (!results_queued) {
                 j->rc = rc;
-                finish_locked(S, j);
+                finish_locked(S, j);     // the job's results never come
             }
+            // the frame's reference was not built: the stream cannot go on
+            if (st->failed == XME_OK)
+                st->failed = rc;
+            st->building--;
+            S->cv_done.notify_all();
         }
     }
 }
@@ -385,7 +445,14 @@ extern "C" int xme_server_destroy(xme_server *S)
     S->sched.join();
     {
         std::unique_lock<std::mutex> lk(S->mu);
-        S->cv_done.wait(lk, [S] { return S->running == 0; });
+        S->cv_done.wait(lk, [S] {
+            if (S->running != 0)
+                return false;
+            for (auto &kv : S->streams)
+                if (kv.second->building != 0)
+                    return false;
+            return true;
+        });
     }
     for (int i = 0; i < S->cfg.max_inflight; i++) {
         (void)hipStreamSynchronize(S->slots[i]);
@@ -410,6 +477,16 @@ extern "C" int xme_server_open_stream(xme_server *S, int w, int h, const xme_pu 
         return XME_ERR_ARG;
     }
     auto st = std::make_unique<Stream>();
+    st->srv = S;
+    // search work of a frame per reference, for the admission deadlines: PU
+    // area times the candidates (the full window, or about 64 pattern points)
+    {
+        int64_t area = 0;
+        for (int i = 0; i < n_pu; i++)
+            area += (int64_t)pus[i].w * pus[i].h;
+        const int64_t r2 = 2 * (int64_t)S->cfg.search_range;
+        st->work = area * (S->cfg.me_method == XME_ME_FS ? r2 * r2 : 64);
+    }
     st->w = w;
     st->h = h;
     st->n_pu = n_pu;
@@ -521,15 +598,13 @@ extern "C" int xme_server_close_stream(xme_server *S, int stream_id)
         }
         st = it->second;
         st->closing = true;
-        S->cv_done.wait(lk, [st] { return st->pending == 0; });
+        // every admitted frame's reference build has finished once building is 0
+        S->cv_done.wait(lk, [st] { return st->pending == 0 && st->building == 0; });
         for (auto &kv : S->jobs)
             if (kv.second->st == st)
                 kv.second->st = nullptr;
         S->streams.erase(it);
     }
-    // the last frame's reference build runs after its results: let it end
-    if (st->next_frame > 0)
-        (void)hipEventSynchronize(st->ref_ready[(st->next_frame - 1) & 1]);
     const int64_t r = st->resident;
     free_stream(st);
     std::lock_guard<std::mutex> lk(S->mu);
@@ -556,6 +631,10 @@ extern "C" int xme_server_submit(xme_server *S, int stream_id, const pel_t *luma
         if (stride < st->w) {
             xme_set_error("xme_server_submit: stride %lld < width %d", (long long)stride, st->w);
             return XME_ERR_ARG;
+        }
+        if (st->failed != XME_OK) {
+            xme_set_error("xme_server_submit: stream %d cannot go on (a reference build failed)", stream_id);
+            return st->failed;
         }
         st->pending++;           // holds the stream open until this submission ends
     }
@@ -606,6 +685,9 @@ extern "C" int xme_server_submit(xme_server *S, int stream_id, const pel_t *luma
     Job *jp = j.release();
     st->next_frame++;            // frame order of the stream = submission order
     jp->t_submit = Clock::now();
+    // twice the frame's search work at a nominal 100 T absdiff/s
+    jp->deadline = std::chrono::duration<double>(jp->t_submit - S->t_start).count() +
+                   2.0 * (double)st->work * jp->n_refs / 1e14;
     jp->id = S->next_job++;
     if (!S->have_first) {
         S->have_first = true;

@@ -17,11 +17,29 @@ XME_ERR_ARG, XME_ERR_HIP, XME_ERR_NOMEM, XME_ERR_BUDGET, XME_ERR_STATE = -1, -2,
 XME_PAD = 80
 XME_MAX_REFS = 8
 XME_MVP_REFS = 4
+XME_TU_DCT_HALF = 0x100
 MAX_DISTORTION = 1 << 30
 PLAN_BATCHED, PLAN_PLAIN = 0, 1
 
+
+class IntplFuncs(C.Structure):
+    """xme_intpl_funcs: the interpolation members of g_funcs, in order."""
+    _fields_ = [(n, C.c_void_p) for n in (
+        "intpl_luma_hor", "intpl_luma_ver", "intpl_luma_ext", "intpl_luma_ver_x3", "intpl_luma_hor_x3",
+        "intpl_luma_ext_x3", "intpl_luma_block_hor", "intpl_luma_block_ver", "intpl_luma_block_ext",
+        "intpl_chroma_block_hor", "intpl_chroma_block_ver", "intpl_chroma_block_ext")]
+
+
 PU_W = [4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 12, 16, 4, 32, 24, 32, 8, 64, 48, 64, 16]
 PU_H = [4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 12, 16, 4, 16, 24, 32, 8, 32, 48, 64, 16, 64]
+
+
+def part_index(w, h):
+    """enum LumaPU index of a w x h block (PART_INDEX, pixel.h:178)."""
+    for i in range(25):
+        if PU_W[i] == w and PU_H[i] == h:
+            return i
+    raise ValueError(f"{w}x{h} is not a LumaPU size")
 
 
 class XmeError(RuntimeError):
@@ -144,6 +162,10 @@ def lib():
         L.xme_inter_recon_bi_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), C.POINTER(RefDesc), I, I, P, P, I,
                                                P, P, P, P, Z, P, P, P]
         L.xme_get_sad_table.argtypes = [P]
+        L.xme_get_dct_table.argtypes = [P, P, P]
+        L.xme_get_quant_funcs.argtypes = [P, P]
+        L.xme_get_avg_table.argtypes = [P]
+        L.xme_get_intpl_funcs.argtypes = [P]
         L.xme_get_satd_table.argtypes = [P]
         L.xme_get_sad_x3_table.argtypes = [P]
         L.xme_get_sad_x4_table.argtypes = [P]
