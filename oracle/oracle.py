@@ -55,6 +55,23 @@ class MvpJob(C.Structure):
                 ("mv", (C.c_int * 2) * 4)]
 
 
+class UmhCtx(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("mvp1_x", "mvp1_y", "mvp2_x", "mvp2_y", "mvp3_x", "mvp3_y", "ref_idx",
+                                       "pred_sad", "pred_sad_uplayer", "pad")] + \
+               [("beta2", C.c_double), ("beta3", C.c_double)]
+
+
+class UmhJob(C.Structure):
+    _fields_ = [("mvp", MvpJob), ("nb_sad", C.c_int * 3), ("mode", C.c_int), ("mincost_up", C.c_int * 3),
+                ("best_up", (C.c_int * 2) * 3), ("mincost_prev", C.c_int), ("best_prev", C.c_int * 2),
+                ("col_valid", C.c_int), ("col_ref", C.c_int), ("col_mv", C.c_int * 2), ("col_dpoc", C.c_int)]
+
+
+class UmhOut(C.Structure):
+    _fields_ = [("pmv_x", C.c_int), ("pmv_y", C.c_int), ("pred_sad_space", C.c_int), ("status", C.c_int),
+                ("ctx", UmhCtx)]
+
+
 class Pu(C.Structure):
     _fields_ = [("pix_x", C.c_int), ("pix_y", C.c_int), ("w", C.c_int), ("h", C.c_int),
                 ("mvp_x", C.c_int), ("mvp_y", C.c_int)]
@@ -124,6 +141,8 @@ def lib():
         L.xo_residual_batch.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
         L.xo_get_mc_pos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_pmvr_adapt.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_umh_prepare.argtypes = [C.POINTER(UmhJob), C.c_int, P, P, P, C.POINTER(UmhOut)]
+        L.xo_umh_bsize.argtypes = [C.c_int, C.c_int, P]
         for f in (L.xo_dct_part, L.xo_idct_part, L.xo_dct_half_part):
             f.argtypes = [C.c_int, C.c_int, P, P, C.c_int]
         L.xo_wavelet_fwd.argtypes = [C.c_int, C.c_int, P, P]
@@ -307,6 +326,44 @@ def get_mvp_default(job, b_slice, ref_dpoc, ref_dpoc_multi):
     return rc, x.value, y.value
 
 
+def umh_bsize(i_qp, slice_type):
+    out = np.zeros(8, np.float64)
+    lib().xo_umh_bsize(i_qp, slice_type, ptr(out))
+    return out
+
+
+def umh_job(mvp_job, mode, nb_sad=(0, 0, 0), mincost_up=(0, 0, 0), best_up=((0, 0),) * 3, mincost_prev=0,
+            best_prev=(0, 0), col=None):
+    """A UmhJob record; mvp_job as for get_mvp_default; col = (col_ref, (mv_x, mv_y), col_dpoc) or None."""
+    j = UmhJob()
+    m = j.mvp
+    m.ref_idx, m.bwd_2nd = mvp_job["ref_idx"], mvp_job["bwd_2nd"]
+    m.cb_x, m.cb_y, m.cb_w, m.cb_h = mvp_job["cb"]
+    m.ur_available = mvp_job["ur_available"]
+    for k in range(4):
+        m.ref[k] = mvp_job["ref"][k]
+        m.mv[k][0], m.mv[k][1] = mvp_job["mv"][k]
+    for k in range(3):
+        j.nb_sad[k] = nb_sad[k]
+        j.mincost_up[k] = mincost_up[k]
+        j.best_up[k][0], j.best_up[k][1] = best_up[k]
+    j.mode, j.mincost_prev = mode, mincost_prev
+    j.best_prev[0], j.best_prev[1] = best_prev
+    if col is not None:
+        j.col_valid = 1
+        j.col_ref, (j.col_mv[0], j.col_mv[1]), j.col_dpoc = col
+    return j
+
+
+def umh_prepare(job, slice_type, ref_dpoc, ref_dpoc_multi, bsize):
+    out = UmhOut()
+    d = np.ascontiguousarray(ref_dpoc, dtype=np.int32)
+    m = np.ascontiguousarray(ref_dpoc_multi, dtype=np.int32)
+    b = np.ascontiguousarray(bsize, dtype=np.float64)
+    lib().xo_umh_prepare(C.byref(job), slice_type, ptr(d), ptr(m), ptr(b), C.byref(out))
+    return out
+
+
 def scale_mv_skip(mv, dist_dst, dist_src):
     return lib().xo_scale_mv_skip(mv, dist_dst, dist_src)
 
@@ -366,14 +423,19 @@ def residual_batch(n, qp, b_intra, blocks):
     return level, recon, nnz
 
 
+TU_DCT_HALF = 0x100
+
+
 def inter_recon_batch(n, qp, cur_plane, planes, frame_w, frame_h, tus, planes2=None, mv2=None):
     """Luma inter reconstruction of n x n TUs; tus: int32 (ntu, 8) records
     {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}; with planes2 and mv2
-    (int32 (ntu, 2)) the TUs are bi-predicted.  Returns (level (ntu, n, n),
-    rec (frame_h, frame_w) picture with the TUs written, nnz, ssd)."""
+    (int32 (ntu, 2)) the TUs are bi-predicted; n = 32 | TU_DCT_HALF uses
+    dct_half.  Returns (level (ntu, n, n), rec (frame_h, frame_w) picture with
+    the TUs written, nnz, ssd)."""
     tus = np.ascontiguousarray(tus, dtype=np.int32).reshape(-1, 8)
     ntu = len(tus)
-    level = np.zeros((ntu, n, n), np.int16)
+    ns = n & ~TU_DCT_HALF
+    level = np.zeros((ntu, ns, ns), np.int16)
     rec = np.zeros((frame_h, frame_w), np.uint8)
     nnz = np.zeros(ntu, np.int32)
     ssd = np.zeros(ntu, np.int32)

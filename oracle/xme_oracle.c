@@ -1282,11 +1282,13 @@ int xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv)
     return blk_pos * 4 + mv;
 }
 
-static void inter_recon(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
+static void inter_recon(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
                         const uint8_t *const planes2[16], intptr_t i_ref, int frame_w, int frame_h,
                         const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
                         intptr_t i_rec, int32_t *nnz, int32_t *ssd)
 {
+    /* tu_get_dct_coeff (rdo.c:764-771): dct_half for a 32x32 TU when asked */
+    const int n = n_arg & ~XO_TU_DCT_HALF, half = (n_arg & XO_TU_DCT_HALF) && n == 32;
     for (int i = 0; i < ntu; i++) {
         const int32_t *r = tus + 8 * i;
         /* the PU's prediction (get_mv_for_mc with the PU size), then the TU's
@@ -1309,7 +1311,19 @@ static void inter_recon(int n, int qp, const uint8_t *cur, intptr_t i_cur, const
         for (int y = 0; y < n; y++)                      /* sub_ps */
             for (int x = 0; x < n; x++)
                 res[y * n + x] = (int16_t)(c[y * i_cur + x] - p[y * 64 + x]);
-        xo_residual_batch(n, qp, 0, res, level + (size_t)i * n * n, rr, nnz + i, 1);
+        if (half) {
+            int qs, qsh, qa, ds, dsh;
+            int16_t co[32 * 32];
+            xo_quant_params(5, qp, 0, &qs, &qsh, &qa);
+            xo_dequant_params(5, qp, &ds, &dsh);
+            xo_dct_half_part(32, 32, res, co, 32);
+            nnz[i] = xo_quant(co, 1024, qs, qsh, qa);
+            memcpy(level + (size_t)i * 1024, co, sizeof(co));
+            xo_dequant(co, 1024, ds, dsh);
+            xo_idct(32, co, rr, 32);
+        } else {
+            xo_residual_batch(n, qp, 0, res, level + (size_t)i * n * n, rr, nnz + i, 1);
+        }
         uint8_t *d = rec + (intptr_t)r[1] * i_rec + r[0];
         int e = 0;
         for (int y = 0; y < n; y++)                      /* add_ps, then pixf.ssd */
@@ -1430,6 +1444,138 @@ int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[4]
         *pmv_y = mv[type - 1][1];
     }
     return bad ? -1 : 0;
+}
+
+/* derive_median_mv (md_inter.c:370-402) returning which neighbour's pair it
+ * took: 0 (a, b), 1 (b, c), 2 (c, a), as the reference's mvp_type. */
+static int median_src(int a, int b, int c)
+{
+    if ((a < 0 && b > 0 && c > 0) || (a > 0 && b < 0 && c < 0))
+        return 1;
+    if ((b < 0 && a > 0 && c > 0) || (b > 0 && a < 0 && c < 0))
+        return 2;
+    if ((c < 0 && a > 0 && b > 0) || (c > 0 && a < 0 && b < 0))
+        return 0;
+    const int dab = iabs(a - b), dbc = iabs(b - c), dca = iabs(c - a);
+    const int m = dab < dbc ? (dab < dca ? dab : dca) : (dbc < dca ? dbc : dca);
+    return m == dab ? 0 : (m == dbc ? 1 : 2);
+}
+
+/* tab_umh_alpha_2nd / _3rd (md_inter.c:49-54): float literals held in doubles. */
+static const float UMH_ALPHA2[8] = { 0.0f, 0.01f, 0.01f, 0.01f, 0.02f, 0.03f, 0.03f, 0.04f };
+static const float UMH_ALPHA3[8] = { 0.0f, 0.06f, 0.07f, 0.07f, 0.08f, 0.12f, 0.11f, 0.15f };
+
+/* (int16_t)(mv * dpoc_curr / (float)dpoc_src): the int product converted to
+ * float, divided in float, truncated (md_inter.c:673-674, 690-691). */
+static int scale_float(int mv, int dpoc_curr, int dpoc_src)
+{
+    const float f = (float)(mv * dpoc_curr) / (float)dpoc_src;
+    return (int16_t)(int)f;
+}
+
+void xo_umh_bsize(int i_qp, int slice_type, double bsize[8])
+{
+    static const int quant_coef[6] = { 13107, 11916, 10082, 9362, 8192, 7282 };
+    const int per = i_qp / 6, rem = i_qp % 6, q_bits = 15 + per;
+    const int qp_const = slice_type == 0 ? (1 << q_bits) / 3 : (1 << q_bits) / 6;
+    const int threshold_4x4 = ((1 << q_bits) - qp_const) / quant_coef[rem];
+    const double quantize_step = (float)threshold_4x4 / (4 * 5.61f);
+    for (int m = 0; m < 8; m++)
+        bsize[m] = 0;
+    bsize[7] = (16 * 16) * quantize_step;        /* PRED_nRx2N */
+    bsize[6] = 4 * bsize[7];                     /* PRED_nLx2N */
+    bsize[5] = 4 * bsize[7];                     /* PRED_2NxnD */
+    bsize[4] = 4 * bsize[5];                     /* PRED_2NxnU */
+    bsize[3] = 4 * bsize[4];                     /* PRED_Nx2N */
+    bsize[2] = 4 * bsize[4];                     /* PRED_2NxN */
+    bsize[1] = 4 * bsize[2];                     /* PRED_2Nx2N */
+}
+
+int xo_umh_prepare(const xo_umh_job_t *j, int slice_type, const int ref_dpoc[4], const int ref_dpoc_multi[4],
+                   const double umh_bsize[8], xo_umh_out_t *o)
+{
+    memset(o, 0, sizeof(*o));
+    const int ref_idx = j->mvp.ref_idx, mode = j->mode, b_slice = slice_type == 2;
+    /* get_mvp_default_sad: the MVP as get_mvp_default, and the SAD of the
+     * neighbour it came from (the median's y component picks it) */
+    int rc = xo_get_mvp_default(&j->mvp, b_slice, ref_dpoc, ref_dpoc_multi, &o->pmv_x, &o->pmv_y);
+    if (rc == -2 || mode < 1 || mode > 7) {
+        o->status = -2;
+        return o->status;
+    }
+    o->status = rc;
+    {
+        const int rl = j->mvp.ref[0], ru = j->mvp.ref[1], rur = j->mvp.ur_available ? j->mvp.ref[3] : j->mvp.ref[2];
+        const int type = mvp_type(ref_idx, rl, ru, rur, j->mvp.cb_x, j->mvp.cb_y, j->mvp.cb_w, j->mvp.cb_h);
+        if (type == 0) {
+            /* the y components after scaling, as derive_median_mv sees them */
+            int my[3];
+            const int refs[3] = { rl, ru, rur };
+            for (int k = 0; k < 3; k++) {
+                const int src = k < 2 ? k : (j->mvp.ur_available ? 3 : 2);
+                int v = j->mvp.mv[src][1], bad = 0;
+                if (refs[k] < 0)
+                    v = 0;
+                else if (b_slice)
+                    v = scale_default(v, ref_dpoc[j->mvp.bwd_2nd ? 0 : 1], ref_dpoc_multi[j->mvp.bwd_2nd ? 0 : 1], &bad);
+                else
+                    v = scale_default(v, ref_dpoc[ref_idx], ref_dpoc_multi[refs[k]], &bad);
+                my[k] = v;
+            }
+            o->pred_sad_space = j->nb_sad[median_src(my[0], my[1], my[2])];
+        } else {
+            o->pred_sad_space = j->nb_sad[type - 1];
+        }
+    }
+    /* fast_me_prepare_info: the upper layer's MV and half its cost */
+    xo_umh_ctx_t *c = &o->ctx;
+    int sad_up = 0;
+    if (mode == 4 || mode == 5) {                /* PRED_2NxnU, PRED_2NxnD: from PRED_2NxN */
+        c->mvp1_x = j->best_up[0][0]; c->mvp1_y = j->best_up[0][1];
+        sad_up = j->mincost_up[0] / 2;
+    } else if (mode == 6 || mode == 7) {         /* PRED_nLx2N, PRED_nRx2N: from PRED_Nx2N */
+        c->mvp1_x = j->best_up[1][0]; c->mvp1_y = j->best_up[1][1];
+        sad_up = j->mincost_up[1] / 2;
+    } else if (mode > 1) {                       /* PRED_2NxN, PRED_Nx2N: from PRED_2Nx2N */
+        c->mvp1_x = j->best_up[2][0]; c->mvp1_y = j->best_up[2][1];
+        sad_up = j->mincost_up[2] / 2;
+    }
+    c->pred_sad_uplayer = sad_up;
+    /* the nearest reference's MV and cost (mvp2 is unused for ref_idx 0) */
+    int sad_ref = 0;
+    if (b_slice && ref_idx == 1) {               /* B_FWD: the negated B_BWD MV */
+        c->mvp2_x = (int16_t)(-j->best_prev[0]);
+        c->mvp2_y = (int16_t)(-j->best_prev[1]);
+        sad_ref = j->mincost_prev;
+    } else if (ref_idx > 0) {
+        if (ref_dpoc[ref_idx - 1] == 0) {
+            o->status = -3;
+            return o->status;
+        }
+        c->mvp2_x = scale_float(j->best_prev[0], ref_dpoc[ref_idx], ref_dpoc[ref_idx - 1]);
+        c->mvp2_y = scale_float(j->best_prev[1], ref_dpoc[ref_idx], ref_dpoc[ref_idx - 1]);
+        sad_ref = j->mincost_prev;
+    }
+    /* the co-located MV, scaled */
+    if (j->col_valid && j->col_ref >= 0) {
+        if (j->col_dpoc == 0) {
+            o->status = -3;
+            return o->status;
+        }
+        c->mvp3_x = scale_float(j->col_mv[0], ref_dpoc[ref_idx], j->col_dpoc);
+        c->mvp3_y = scale_float(j->col_mv[1], ref_dpoc[ref_idx], j->col_dpoc);
+    }
+    c->ref_idx = ref_idx;
+    const int pred_sad = (!b_slice && ref_idx > 0) ? sad_ref : (mode == 1 ? o->pred_sad_space : sad_up);
+    c->pred_sad = pred_sad;
+    if (pred_sad != 0) {
+        /* dist_t (int32) product: the reference squares in int, wrapping */
+        const int32_t sq = (int32_t)((uint32_t)pred_sad * (uint32_t)pred_sad);
+        const double threshold = umh_bsize[mode] / sq;
+        c->beta2 = threshold - (double)UMH_ALPHA2[mode];
+        c->beta3 = threshold - (double)UMH_ALPHA3[mode];
+    }
+    return o->status;
 }
 
 /* mc_chroma, mc.c:797-829: luma qpel MV = chroma 1/8-pel position. */

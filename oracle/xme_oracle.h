@@ -230,6 +230,7 @@ int  xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int
  * prediction is mc_luma at the PU's clamped position, offset to the TU.
  * Writes level (n*n per TU), the reconstruction into rec (stride i_rec, at
  * the TU's position), nnz and the SSD against cur per TU. */
+#define XO_TU_DCT_HALF 0x100     /* n = 32 | XO_TU_DCT_HALF: dct_half (rdo.c:764-771) */
 void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
                           intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
                           uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd);
@@ -259,6 +260,42 @@ typedef struct {
 } xo_mvp_job_t;
 int xo_get_mvp_default(const xo_mvp_job_t *j, int b_slice, const int ref_dpoc[4], const int ref_dpoc_multi[4],
                        int *pmv_x, int *pmv_y);
+
+/* The UMH search's encoder state of one (PU, reference): get_mvp_default_sad
+ * (md_inter.c:527-608: get_mvp_default's MVP plus pred_sad_space, the
+ * neighbour min cost of the MVP's source) and fast_me_prepare_info
+ * (md_inter.c:640-718: mvp1 from the upper layer, mvp2 from the nearest
+ * reference, mvp3 co-located, pred_sad and the UMH betas).  The record holds
+ * what those functions read from the encoder state. */
+typedef struct {
+    xo_mvp_job_t mvp;                 /* neighbours, PU in its CU, ref_idx, bwd_2nd */
+    int nb_sad[3];                    /* all_mincost[...][mode][ref_idx] at the left, top, top-right 4x4 blocks
+                                       * (0 where the reference reads none: left/top edge, top-right unavailable) */
+    int mode;                         /* PRED_2Nx2N (1) .. PRED_nRx2N (7), common.h:198-207 */
+    int mincost_up[3];                /* mincosts[PRED_2NxN / PRED_Nx2N / PRED_2Nx2N][ref_idx] of the CU */
+    int best_up[3][2];                /* all_best_mv[PRED_2NxN / Nx2N / 2Nx2N][pu_idx][ref_idx] */
+    int mincost_prev;                 /* mincosts[mode][ref_idx - 1] */
+    int best_prev[2];                 /* all_best_mv[mode][pu_idx][ref_idx - 1] */
+    int col_valid;                    /* fref[0] exists and is not an I frame */
+    int col_ref;                      /* fref[0]->pu_ref at the PU's 16x16 block (-1: none) */
+    int col_mv[2];                    /* fref[0]->pu_mv there */
+    int col_dpoc;                     /* fref[0]->ref_dpoc[col_ref] */
+} xo_umh_job_t;
+
+typedef struct {
+    int pmv_x, pmv_y;                 /* p_me->mvp */
+    int pred_sad_space;
+    int status;                       /* 0; -1 a scaling product leaves int; -2 a reference index outside 0..3;
+                                       * -3 a distance of 0 divides */
+    xo_umh_ctx_t ctx;
+} xo_umh_out_t;
+
+/* slice_type: SLICE_TYPE_I 0, P 1, B 2, F 3 (common.h:158-161); umh_bsize =
+ * h->umh_bsize (xo_umh_bsize). */
+int xo_umh_prepare(const xo_umh_job_t *j, int slice_type, const int ref_dpoc[4], const int ref_dpoc_multi[4],
+                   const double umh_bsize[8], xo_umh_out_t *out);
+/* xavs2_me_init_umh_threshold (me.c:647-665). */
+void xo_umh_bsize(int i_qp, int slice_type, double bsize[8]);
 
 /* cu_get_mc_pos (common/mc.h:52-66). */
 int  xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv);

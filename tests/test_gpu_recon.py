@@ -18,7 +18,10 @@ from xavs2_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-def gpu_recon(n, qp, cur, ref_img, tus, frac):
+def gpu_recon(n_arg, qp, cur, ref_img, tus, frac, intermediates=True, want_intermediates=False):
+    """xme_inter_recon_batch; n_arg may carry XME_TU_DCT_HALF.  With frac 2 every plane is
+    resident (the one-launch path), with frac 0 the block filters predict (staged path)."""
+    n = n_arg & 0xff
     h, w = cur.shape
     dref = X.DeviceRef(w, h, frac=frac)
     dref.load(ref_img)
@@ -30,13 +33,17 @@ def gpu_recon(n, qp, cur, ref_img, tus, frac):
     rec.buf.zero()
     nnz, ssd = X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)
     rd = dref.desc()
-    X.check(X.lib().xme_inter_recon_batch(n, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(rd), w, h,
-                                          d_tus.at(), ntu, resid.at(), level.at(), rres.at(), C.c_void_p(rec.origin),
+    X.check(X.lib().xme_inter_recon_batch(n_arg, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(rd), w, h,
+                                          d_tus.at(), ntu, resid.at() if intermediates else None, level.at(),
+                                          rres.at() if intermediates else None, C.c_void_p(rec.origin),
                                           rec.stride, nnz.at(), ssd.at(), None), "xme_inter_recon_batch")
     X.check(X.lib().xme_stream_sync(None), "sync")
     full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
-    return (level.download(ntu * n * n, np.int16).reshape(ntu, n, n), full[:h, :w],
-            nnz.download(ntu, np.int32), ssd.download(ntu, np.int32))
+    out = (level.download(ntu * n * n, np.int16).reshape(ntu, n, n), full[:h, :w],
+           nnz.download(ntu, np.int32), ssd.download(ntu, np.int32))
+    if want_intermediates:
+        out += (resid.download(ntu * n * n, np.int16), rres.download(ntu * n * n, np.int16))
+    return out
 
 
 @pytest.mark.parametrize("frac", [2, 0])
@@ -76,7 +83,8 @@ def test_1080p_8x8():
         assert np.array_equal(g, e), name
 
 
-def gpu_recon_bi(n, qp, cur, r1_img, r2_img, tus, mv2, frac):
+def gpu_recon_bi(n_arg, qp, cur, r1_img, r2_img, tus, mv2, frac):
+    n = n_arg & 0xff
     h, w = cur.shape
     d1, d2 = X.DeviceRef(w, h, frac=frac), X.DeviceRef(w, h, frac=frac)
     d1.load(r1_img)
@@ -89,7 +97,7 @@ def gpu_recon_bi(n, qp, cur, r1_img, r2_img, tus, mv2, frac):
     rec.buf.zero()
     nnz, ssd = X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)
     a, b = d1.desc(), d2.desc()
-    X.check(X.lib().xme_inter_recon_bi_batch(n, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(a), C.byref(b),
+    X.check(X.lib().xme_inter_recon_bi_batch(n_arg, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(a), C.byref(b),
                                              w, h, d_tus.at(), d_mv2.at(), ntu, resid.at(), level.at(), rres.at(),
                                              C.c_void_p(rec.origin), rec.stride, nnz.at(), ssd.at(), None),
             "xme_inter_recon_bi_batch")
@@ -154,3 +162,54 @@ def test_reconstruction_clips_to_8_bits():
     for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
         assert np.array_equal(g, e), name
     assert (got[1] == 0).any() and (got[1] == 255).any()
+
+
+@pytest.mark.parametrize("frac", [2, 0])
+def test_dct_half_matches_oracle(frac):
+    """32x32 TUs with XME_TU_DCT_HALF (dct_half, rdo.c:764-771), uni- and bi-predicted."""
+    w, h = 192, 128
+    ref = synth.smooth_field(w, h, seed=61)
+    r2 = synth.smooth_field(w, h, seed=62)
+    cur = synth.multi_ref_current([ref, r2], 8, 8, seed=63)
+    tus = tiles(w, h, 32, np.random.default_rng(64), far=True)
+    n_arg = 32 | X.XME_TU_DCT_HALF
+    got = gpu_recon(n_arg, 27, cur, ref, tus, frac)
+    exp = O.inter_recon_batch(32 | O.TU_DCT_HALF, 27, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    assert not got[0][:, 16:, :].any() and not got[0][:, :, 16:].any()
+    full = gpu_recon(32, 27, cur, ref, tus, frac)
+    assert not np.array_equal(full[0], got[0])
+    mv2 = np.array([(int(t[6]) // 2 + 3, -int(t[7]) // 2 + 1) for t in tus], np.int32)
+    got = gpu_recon_bi(n_arg, 27, cur, ref, r2, tus, mv2, frac)
+    exp = O.inter_recon_batch(32 | O.TU_DCT_HALF, 27, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus,
+                              oracle_ref(r2, 2), mv2)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_one_launch_path_without_intermediates(n):
+    """With every plane resident the path needs no prediction-error or residual buffers;
+    when given, they hold what the staged path (block filters) writes there."""
+    w, h = 192, 128
+    ref = synth.smooth_field(w, h, seed=71)
+    cur = synth.multi_ref_current([ref], 8, 8, seed=72)
+    tus = tiles(w, h, n, np.random.default_rng(73 + n), far=True)
+    exp = O.inter_recon_batch(n, 32, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus)
+    got = gpu_recon(n, 32, cur, ref, tus, 2, intermediates=False)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    fused = gpu_recon(n, 32, cur, ref, tus, 2, want_intermediates=True)
+    staged = gpu_recon(n, 32, cur, ref, tus, 0, want_intermediates=True)
+    assert np.array_equal(fused[4], staged[4]) and np.array_equal(fused[5], staged[5])
+    # the staged path needs them
+    dref = X.DeviceRef(w, h, frac=0)
+    rd = dref.desc()
+    dcur = device_cur(cur)
+    d_tus = X.upload(np.ascontiguousarray(tus, np.int32))
+    bufs = [X.DeviceBuffer(2 * len(tus) * n * n), X.DeviceBuffer(4 * len(tus)), X.DeviceBuffer(4 * len(tus))]
+    rc = X.lib().xme_inter_recon_batch(n, 32, C.c_void_p(dcur.origin), dcur.stride, C.byref(rd), w, h, d_tus.at(),
+                                       len(tus), None, bufs[0].at(), None, C.c_void_p(dcur.origin), dcur.stride,
+                                       bufs[1].at(), bufs[2].at(), None)
+    assert rc == X.XME_ERR_ARG

@@ -170,3 +170,40 @@ def test_reconstruction_clips_to_8_bits(qp):
     res = np.array([cur[t[1]:t[1] + 8, t[0]:t[0] + 8].astype(np.int16) - 128 for t in tus])
     _, rr, _ = O.residual_batch(8, qp, 0, res)
     assert (128 + rr.astype(np.int32) > 255).any() or (128 + rr.astype(np.int32) < 0).any()
+
+
+@pytest.mark.parametrize("d", [3, 20])
+def test_dct_half_drops_a_checkerboard_residual(d):
+    """dct_half (rdo.c:764-771, transform.c:1276-1286) keeps only the low 16x16 of a 32x32
+    TU's coefficients.  A residual that is a +-d checkerboard has all its energy at the
+    highest frequency, so with dct_half every level is 0, nnz is 0, the reconstruction is
+    the prediction and the SSD is 1024 d^2; the full DCT keeps it (nonzero levels)."""
+    w, h = 96, 64
+    ref = texture(w, h, 5)
+    cur = ref.copy().astype(np.int32)
+    chk = np.where((np.add.outer(np.arange(32), np.arange(32)) & 1) == 0, d, -d)
+    cur[16:48, 32:64] = np.clip(cur[16:48, 32:64] + chk, 0, 255)
+    cur = cur.astype(np.uint8)
+    planes = planes_of(ref)
+    tus = np.array([(32, 16, 32, 16, 32, 32, 0, 0)], np.int32)
+    level, rec, nnz, ssd = O.inter_recon_batch(32 | O.TU_DCT_HALF, 22, cur_plane(cur), planes, w, h, tus)
+    assert not level.any() and nnz[0] == 0
+    assert np.array_equal(rec[16:48, 32:64], ref[16:48, 32:64])
+    exp = int(((cur[16:48, 32:64].astype(np.int64) - ref[16:48, 32:64]) ** 2).sum())
+    assert ssd[0] == exp and (exp == 1024 * d * d or d == 20)      # +-20 may clip at 0 / 255
+    full = O.inter_recon_batch(32, 22, cur_plane(cur), planes, w, h, tus)
+    assert full[2][0] > 0
+
+
+def test_dct_half_keeps_a_smooth_residual_like_the_full_transform():
+    """A residual whose coefficients all lie in the low 16x16 (a constant offset: DC only)
+    gives the same levels, reconstruction and SSD with dct_half as with the full DCT."""
+    w, h = 96, 64
+    ref = texture(w, h, 6)
+    cur = np.clip(ref.astype(np.int32) + 9, 0, 255).astype(np.uint8)
+    planes = planes_of(ref)
+    tus = np.array([(0, 0, 0, 0, 32, 32, 0, 0), (32, 32, 32, 32, 32, 32, 0, 0)], np.int32)
+    a = O.inter_recon_batch(32 | O.TU_DCT_HALF, 27, cur_plane(cur), planes, w, h, tus)
+    b = O.inter_recon_batch(32, 27, cur_plane(cur), planes, w, h, tus)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

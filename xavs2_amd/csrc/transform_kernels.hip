@@ -30,7 +30,7 @@
 #include <cstring>
 #include <mutex>
 
-#include "xme_internal.h"
+#include "xme_mc.h"
 
 namespace {
 
@@ -43,7 +43,27 @@ struct QParams {
     int dscale, dshift;
 };
 
-enum { DO_FWD = 1, DO_QUANT = 2, DO_DEQUANT = 4, DO_INV = 8 };
+enum { DO_FWD = 1, DO_QUANT = 2, DO_DEQUANT = 4, DO_INV = 8, DO_HALF = 16 };
+
+// The fused luma inter reconstruction (residual_kernel with FU = 1 or 2):
+// each tile's blocks are TUs of the record list {tu_x, tu_y, pu_x, pu_y, pu_w,
+// pu_h, mv_x, mv_y}; the prediction comes from the interpolated planes (and
+// for FU = 2 the rounded average with planes2 at mv2), the input is cur - pred,
+// and the reconstruction clip(pred + residual) goes into rec with its SSD.
+struct FusedArgs {
+    const uint8_t *planes[16];
+    const uint8_t *planes2[16];
+    int64_t i_ref;
+    const uint8_t *cur;
+    int64_t i_cur;
+    uint8_t *rec;
+    int64_t i_rec;
+    const int32_t *tus;
+    const int32_t *mv2;
+    int fw, fh;
+    int16_t *resid;        // the prediction error, when not null
+    int32_t *ssd;
+};
 
 // Constant MFMA operands and accumulator starts of one block size, per lane.
 struct TileConsts {
@@ -141,12 +161,35 @@ __host__ __device__ constexpr int rel_x(int x)
     return (x / N) * (N * N) + x % N;
 }
 
-template <int N>
+// Four prediction bytes at p (any alignment): the aligned dword pair around
+// them, realigned.  The dword after stays inside the plane's XME_PAD border.
+__device__ __forceinline__ uint32_t load4u(const uint8_t *p)
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
+}
+
+__device__ __forceinline__ uint64_t bperm64(int src_lane, uint64_t v)
+{
+    const int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(uint32_t)v);
+    const int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(uint32_t)(v >> 32));
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// FU = 0: the residual path on int16 blocks from src.  FU = 1 / 2: the fused
+// inter reconstruction (FusedArgs), uni- or bi-predicted; src is unused and
+// recon (the reconstructed residual) may be null.
+template <int N, int FU>
 __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict__ src, int16_t *__restrict__ level,
                                                        int16_t *__restrict__ recon, int32_t *__restrict__ nnz,
-                                                       int nblk, QParams q, int mode)
+                                                       int nblk, QParams q, int mode, FusedArgs fa)
 {
     constexpr int NB = 32 / N, BPT = NB * NB, L2 = log2n_of(N);
+    constexpr int W = N < 16 ? N : 16, NP = 16 / W;    // a lane's 16 columns: NP pieces of W
+    __shared__ int16_t s_res[4][FU ? 32 * 32 : 1];
+    __shared__ int s_ssd[4][FU ? BPT : 1];
+    const int wv = threadIdx.x >> 6;
     const TileConsts &tc = g_tc[L2 - 2];
     const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
     const int ntiles = (nblk + BPT - 1) / BPT;
@@ -165,20 +208,95 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
         // in block b_lane + (rho(0, i) / N) * NB
         const int64_t e_lane = t * 1024 + rel_x<N>(col) + half * rel_y<N>(4);
         const int64_t b_lane = t * BPT + col / N + (N == 4 ? half * NB : 0);
+        // fused: this lane's 16 current and prediction bytes (tile row `col`,
+        // columns 16 half .. 16 half + 15), kept for the reconstruction
+        uint32_t c4[4] = { 0u, 0u, 0u, 0u }, p4[4] = { 0u, 0u, 0u, 0u };
+        bool tu_good = false;            // lane < BPT: its TU of the tile is reconstructed
+        uint64_t my_rec = 0;             // lane < BPT: the TU's first reconstructed pixel
+        if constexpr (FU != 0) {
+            // lane b < BPT takes TU t * BPT + b: validity and base pointers
+            uint64_t my_pred = 0, my_pred2 = 0, my_cur = 0;
+            const int64_t bt = t * BPT + lane;
+            if (lane < BPT && bt < nblk) {
+                int r[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    r[k] = fa.tus[8 * bt + k];
+                tu_good = tu_ok<N>(r, fa.fw, fa.fh);
+                if (tu_good) {
+                    const int qx = mc_pos(fa.fw, r[4], r[2], r[6]) + 4 * (r[0] - r[2]);
+                    const int qy = mc_pos(fa.fh, r[5], r[3], r[7]) + 4 * (r[1] - r[3]);
+                    my_pred = (uint64_t)(uintptr_t)(fa.planes[((qy & 3) << 2) + (qx & 3)] + (int64_t)(qy >> 2) * fa.i_ref +
+                                                    (qx >> 2));
+                    if constexpr (FU == 2) {
+                        const int qx2 = mc_pos(fa.fw, r[4], r[2], fa.mv2[2 * bt]) + 4 * (r[0] - r[2]);
+                        const int qy2 = mc_pos(fa.fh, r[5], r[3], fa.mv2[2 * bt + 1]) + 4 * (r[1] - r[3]);
+                        my_pred2 = (uint64_t)(uintptr_t)(fa.planes2[((qy2 & 3) << 2) + (qx2 & 3)] +
+                                                         (int64_t)(qy2 >> 2) * fa.i_ref + (qx2 >> 2));
+                    }
+                    my_cur = (uint64_t)(uintptr_t)(fa.cur + (int64_t)r[1] * fa.i_cur + r[0]);
+                    my_rec = (uint64_t)(uintptr_t)(fa.rec + (int64_t)r[1] * fa.i_rec + r[0]);
+                }
+            }
+            if (lane < BPT)
+                s_ssd[wv][lane] = 0;
+            const int r_in = col % N, c0 = (16 * half) % N;
+#pragma unroll
+            for (int piece = 0; piece < NP; piece++) {
+                const int bl = (col / N) * NB + (16 * half) / N + piece;
+                const uint64_t pc = bperm64(bl, my_cur), pp = bperm64(bl, my_pred);
+                const uint64_t pp2 = FU == 2 ? bperm64(bl, my_pred2) : 0;
+                if (pc != 0) {
+                    const uint8_t *cp = (const uint8_t *)(uintptr_t)pc + (int64_t)r_in * fa.i_cur + c0;
+                    const uint8_t *pr = (const uint8_t *)(uintptr_t)pp + (int64_t)r_in * fa.i_ref + c0;
+                    const uint8_t *pr2 = (const uint8_t *)(uintptr_t)pp2 + (int64_t)r_in * fa.i_ref + c0;
+#pragma unroll
+                    for (int k = 0; k < W / 4; k++) {
+                        c4[piece * (W / 4) + k] = reinterpret_cast<const uint32_t *>(cp)[k];
+                        uint32_t u = load4u(pr + 4 * k);
+                        if constexpr (FU == 2) {
+                            const uint32_t v2 = load4u(pr2 + 4 * k);
+                            u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);   // (u + v + 1) >> 1 per byte
+                        }
+                        p4[piece * (W / 4) + k] = u;
+                    }
+                }
+            }
+        }
         if (mode & DO_FWD) {
             // F1 operand: tile row `col`, columns 16 half .. 16 half + 15
             uint32_t d[8];
-            // pieces of W contiguous coefficients: row `col`, columns
-            // 16 half + piece W, i.e. blocks (col / N, 16 half / N + piece)
-            const int16_t *p0 = src + t * 1024 + rel_y<N>(col) + rel_x<N>(16 * half);
-            const int64_t blk0 = t * BPT + (col / N) * NB + (16 * half) / N;
+            if constexpr (FU != 0) {
+                // the prediction error cur - pred (sub_ps, pixel.c:780-792)
 #pragma unroll
-            for (int piece = 0; piece < 16 / (N < 16 ? N : 16); piece++) {
-                constexpr int W = N < 16 ? N : 16;
-                const uint32_t *p = reinterpret_cast<const uint32_t *>(p0 + piece * (N * N));
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t cw = c4[k / 2] >> (16 * (k & 1)), pw = p4[k / 2] >> (16 * (k & 1));
+                    const int e0 = (int)(cw & 0xff) - (int)(pw & 0xff);
+                    const int e1 = (int)((cw >> 8) & 0xff) - (int)((pw >> 8) & 0xff);
+                    d[k] = ((uint32_t)e0 & 0xffffu) | ((uint32_t)e1 << 16);
+                }
+                if (fa.resid) {
+                    const int64_t blk0 = t * BPT + (col / N) * NB + (16 * half) / N;
+                    int16_t *p0 = fa.resid + t * 1024 + rel_y<N>(col) + rel_x<N>(16 * half);
 #pragma unroll
-                for (int k = 0; k < W / 2; k++)
-                    d[piece * (W / 2) + k] = blk0 + piece < nblk ? p[k] : 0u;
+                    for (int piece = 0; piece < NP; piece++)
+#pragma unroll
+                        for (int k = 0; k < W / 2; k++)
+                            if (blk0 + piece < nblk)
+                                reinterpret_cast<uint32_t *>(p0 + piece * (N * N))[k] = d[piece * (W / 2) + k];
+                }
+            } else {
+                // pieces of W contiguous coefficients: row `col`, columns
+                // 16 half + piece W, i.e. blocks (col / N, 16 half / N + piece)
+                const int16_t *p0 = src + t * 1024 + rel_y<N>(col) + rel_x<N>(16 * half);
+                const int64_t blk0 = t * BPT + (col / N) * NB + (16 * half) / N;
+#pragma unroll
+                for (int piece = 0; piece < NP; piece++) {
+                    const uint32_t *p = reinterpret_cast<const uint32_t *>(p0 + piece * (N * N));
+#pragma unroll
+                    for (int k = 0; k < W / 2; k++)
+                        d[piece * (W / 2) + k] = blk0 + piece < nblk ? p[k] : 0u;
+                }
             }
             split_pairs(d, hi, lo);
             v16i D = mm_data_a(hi, lo, f1b, splat16(acc_f1));
@@ -200,6 +318,9 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 c[i] = (int)(int16_t)(D[i] >> s2);
+                // dct_32x32_half_c (transform.c:1276-1286): the low 16x16 only
+                if (N == 32 && (mode & DO_HALF) && (rho(half, i) >= 16 || col >= 16))
+                    c[i] = 0;
                 if (mode & DO_QUANT) {
                     const int a = c[i] < 0 ? -c[i] : c[i];
                     const int m = (int)(((uint32_t)a * (uint32_t)q.qscale + (uint32_t)q.qadd) >> q.qshift);
@@ -245,9 +366,53 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             split16(v, hi, lo);
             D = mm_data_a(hi, lo, ib, splat16(acc_i2));
 #pragma unroll
-            for (int i = 0; i < 16; i++)
-                if (b_lane + (rho(0, i) / N) * NB < nblk)
-                    recon[e_lane + rel_y<N>(rho(0, i))] = (int16_t)xme_clip3(-256, 255, D[i] >> 12);
+            for (int i = 0; i < 16; i++) {
+                const int rv = xme_clip3(-256, 255, D[i] >> 12);
+                if (recon && b_lane + (rho(0, i) / N) * NB < nblk)
+                    recon[e_lane + rel_y<N>(rho(0, i))] = (int16_t)rv;
+                if constexpr (FU != 0)
+                    s_res[wv][rho(half, i) * 32 + col] = (int16_t)rv;
+            }
+        }
+        if constexpr (FU != 0) {
+            // the residual, now by rows: this lane's 16 columns of row `col`;
+            // reconstruction clip(pred + residual) (add_ps, pixel.c:794-806)
+            // and its SSD (pixf.ssd, rdo.c:1497-1499) per TU
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int r_in = col % N, c0 = (16 * half) % N;
+#pragma unroll
+            for (int piece = 0; piece < NP; piece++) {
+                const int bl = (col / N) * NB + (16 * half) / N + piece;
+                const uint64_t prc = bperm64(bl, my_rec);
+                if (prc == 0)
+                    continue;
+                uint8_t *rp = (uint8_t *)(uintptr_t)prc + (int64_t)r_in * fa.i_rec + c0;
+                int e = 0;
+#pragma unroll
+                for (int k = 0; k < W / 4; k++) {
+                    const int kk = piece * (W / 4) + k;
+                    uint32_t o = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int x = 16 * half + 4 * kk + j;
+                        const int pv = (int)((p4[kk] >> (8 * j)) & 0xff), cv = (int)((c4[kk] >> (8 * j)) & 0xff);
+                        const int v = clip_pel(pv + s_res[wv][col * 32 + x]);
+                        o |= (uint32_t)v << (8 * j);
+                        e += (cv - v) * (cv - v);
+                    }
+                    reinterpret_cast<uint32_t *>(rp)[k] = o;
+                }
+                atomicAdd(&s_ssd[wv][bl], e);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < BPT && t * BPT + lane < nblk)
+                fa.ssd[t * BPT + lane] = tu_good ? s_ssd[wv][lane] : -1;
+            // the next tile rewrites s_res and s_ssd only after these reads
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -332,8 +497,9 @@ QParams qparams(int n, int qp, int b_intra)
     return q;
 }
 
-int launch(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t *nnz, int nblk, QParams q, int mode,
-           hipStream_t s)
+template <int FU>
+int launch_fu(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t *nnz, int nblk, QParams q, int mode,
+              const FusedArgs &fa, hipStream_t s)
 {
     std::call_once(g_tab_once, upload_tables);
     if (g_tab_err != hipSuccess)
@@ -344,20 +510,30 @@ int launch(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t *n
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 256 * 16));
     switch (n) {
     case 4:
-        hipLaunchKernelGGL(residual_kernel<4>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
+        hipLaunchKernelGGL((residual_kernel<4, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode,
+                           fa);
         break;
     case 8:
-        hipLaunchKernelGGL(residual_kernel<8>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
+        hipLaunchKernelGGL((residual_kernel<8, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode,
+                           fa);
         break;
     case 16:
-        hipLaunchKernelGGL(residual_kernel<16>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
+        hipLaunchKernelGGL((residual_kernel<16, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q,
+                           mode, fa);
         break;
     default:
-        hipLaunchKernelGGL(residual_kernel<32>, dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode);
+        hipLaunchKernelGGL((residual_kernel<32, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q,
+                           mode, fa);
         break;
     }
     XME_HIP(hipGetLastError());
     return XME_OK;
+}
+
+int launch(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t *nnz, int nblk, QParams q, int mode,
+           hipStream_t s)
+{
+    return launch_fu<0>(n, src, level, recon, nnz, nblk, q, mode, FusedArgs{}, s);
 }
 
 int check(int n, int nblk, const void *a, const void *b, int qp, const char *what)
@@ -371,6 +547,40 @@ int check(int n, int nblk, const void *a, const void *b, int qp, const char *wha
 }
 
 } // namespace
+
+int xme_residual_run(int n, int qp, int b_intra, int half, const int16_t *in, int16_t *level, int16_t *recon,
+                     int32_t *nnz, int nblk, hipStream_t s)
+{
+    return launch(n, in, level, recon, nnz, nblk, qparams(n, qp, b_intra ? 1 : 0),
+                  DO_FWD | DO_QUANT | DO_DEQUANT | DO_INV | (half ? DO_HALF : 0), s);
+}
+
+int xme_recon_fused(int n, int qp, int half, int bi, const uint8_t *const planes[16], const uint8_t *const planes2[16],
+                    int64_t i_ref, const uint8_t *cur, int64_t i_cur, uint8_t *rec, int64_t i_rec, const int32_t *tus,
+                    const int32_t *mv2, int ntu, int fw, int fh, int16_t *resid, int16_t *level, int16_t *rres,
+                    int32_t *nnz, int32_t *ssd, hipStream_t s)
+{
+    FusedArgs fa{};
+    for (int k = 0; k < 16; k++) {
+        fa.planes[k] = planes[k];
+        fa.planes2[k] = bi ? planes2[k] : nullptr;
+    }
+    fa.i_ref = i_ref;
+    fa.cur = cur;
+    fa.i_cur = i_cur;
+    fa.rec = rec;
+    fa.i_rec = i_rec;
+    fa.tus = tus;
+    fa.mv2 = mv2;
+    fa.fw = fw;
+    fa.fh = fh;
+    fa.resid = resid;
+    fa.ssd = ssd;
+    const int mode = DO_FWD | DO_QUANT | DO_DEQUANT | DO_INV | (half ? DO_HALF : 0);
+    const QParams q = qparams(n, qp, 0);
+    return bi ? launch_fu<2>(n, nullptr, level, rres, nnz, ntu, q, mode, fa, s)
+              : launch_fu<1>(n, nullptr, level, rres, nnz, ntu, q, mode, fa, s);
+}
 
 extern "C" int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *in, coeff_t *level, coeff_t *recon,
                                   int32_t *nnz, int nblk, void *stream)

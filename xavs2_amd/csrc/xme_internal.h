@@ -152,6 +152,19 @@ __host__ __device__ inline bool xme_mv_legal(const MvWindow &m, int mx, int my)
 }
 
 // ---------------------------------------------------------------------------
+// The residual path of transform_kernels.hip for other kernels' hosts: the
+// fused DCT -> quant -> dequant -> IDCT on int16 blocks (half: dct_half for
+// N = 32), and the fused luma inter reconstruction (one launch per TU size;
+// resid / rres may be null; every plane of `planes` (and `planes2` when bi)
+// must exist).
+int xme_residual_run(int n, int qp, int b_intra, int half, const int16_t *in, int16_t *level, int16_t *recon,
+                     int32_t *nnz, int nblk, hipStream_t s);
+int xme_recon_fused(int n, int qp, int half, int bi, const uint8_t *const planes[16], const uint8_t *const planes2[16],
+                    int64_t i_ref, const uint8_t *cur, int64_t i_cur, uint8_t *rec, int64_t i_rec, const int32_t *tus,
+                    const int32_t *mv2, int ntu, int fw, int fh, int16_t *resid, int16_t *level, int16_t *rres,
+                    int32_t *nnz, int32_t *ssd, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // Host-side helpers (defined in runtime.cpp).
 void xme_set_error(const char *fmt, ...);
 int xme_hip_fail(hipError_t e, const char *what);
