@@ -272,6 +272,22 @@ int xme_plan_set_launch_split(xme_plan *plan, int64_t max_absdiff_per_launch);
  * rejects.  */
 int xme_me_run(xme_plan *plan, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
                xme_result *d_results, int32_t *d_cost_map, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture elsewhere in the process, tight
+ * serving loops, harnesses that forbid hipMalloc): the frame-level ME job of
+ * xme_plan_create (batched) + xme_me_run in one call, with the plan's device
+ * buffers (work lists, per-instance state and keys, queue counters, frame
+ * descriptor) in a static device scratch of XME_STATIC_SCRATCH_BYTES per GPU,
+ * held from the library's load until the process ends.  The call makes no
+ * device allocation; it returns once the results are in d_results (it waits
+ * for `stream`).  Full search, DIA and HEX (UMH needs its context: XME_ERR_ARG).
+ * XME_ERR_BUDGET when the job's plan does not fit the scratch (about 100
+ * bytes per PU and reference: a 1080p frame of 8x8 PUs with 2 references
+ * fits).  Calls of one process are serialised; calls from several processes
+ * on one GPU each have their own scratch. */
+#define XME_STATIC_SCRATCH_BYTES (64u << 20)
+int xme_me_search_static(const xme_me_params *prm, const xme_pu *pus, int n_pu, int n_ref, const pel_t *d_cur,
+                         intptr_t i_cur, const xme_ref_desc *refs, xme_result *d_results, void *stream);
 /* Capture the kernels of xme_me_run into a HIP graph on the first call and
  * replay it afterwards (steady-state frames).  Pointers may change between
  * calls: they travel through a device-side frame descriptor. */

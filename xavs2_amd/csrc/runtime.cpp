@@ -213,6 +213,12 @@ struct xme_plan {
     xme_umh_ctx *d_umh = nullptr;         // UMH plans: per-instance encoder state
     bool umh_set = false;
     std::vector<int32_t> inst_area;        // PU area per instance
+    // device buffers carved from caller-given memory instead of hipMalloc
+    // (xme_me_search_static): nothing of it is freed with the plan
+    uint8_t *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    hipStream_t arena_stream = nullptr;
+    FrameDesc arena_desc;                  // host copy of the frame descriptor (pageable)
 };
 
 namespace {
@@ -422,6 +428,10 @@ static size_t make_task(const DevParams &p, const std::vector<DevPu> &pus, const
 
 static void plan_free(xme_plan *pl)
 {
+    if (pl->arena) {
+        delete pl;
+        return;
+    }
     hipFree(pl->d_pus);
     hipFree(pl->d_inst);
     hipFree(pl->d_best);
@@ -454,8 +464,51 @@ extern "C" int xme_plan_destroy(xme_plan *pl)
     return XME_OK;
 }
 
+// Device memory of a plan: hipMalloc, or (arena plans) the next 256-byte
+// aligned block of the arena.
+static hipError_t plan_dev_alloc(xme_plan *pl, void **d, size_t n)
+{
+    n = n ? n : 16;
+    if (!pl->arena) {
+        pl->dev_bytes += n;
+        return hipMalloc(d, n);
+    }
+    const size_t at = (pl->arena_used + 255) & ~(size_t)255;
+    if (at + n > pl->arena_cap) {
+        *d = nullptr;
+        pl->arena_used = pl->arena_cap + 1;    // marks the overflow for the caller
+        return hipErrorOutOfMemory;
+    }
+    *d = pl->arena + at;
+    pl->arena_used = at + n;
+    pl->dev_bytes = pl->arena_used;
+    return hipSuccess;
+}
+
+static hipError_t plan_upload(xme_plan *pl, void *d, const void *h, size_t n)
+{
+    if (!n)
+        return hipSuccess;
+    return pl->arena ? hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, pl->arena_stream)
+                     : hipMemcpy(d, h, n, hipMemcpyHostToDevice);
+}
+
+static hipError_t plan_zero(xme_plan *pl, void *d, size_t n)
+{
+    return pl->arena ? hipMemsetAsync(d, 0, n, pl->arena_stream) : hipMemset(d, 0, n);
+}
+
+static int plan_create_impl(const xme_me_params *prm, const xme_pu *pus_in, int n_pu, int n_ref, int mode,
+                            uint8_t *arena, size_t arena_cap, hipStream_t arena_stream, xme_plan **out);
+
 extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, int n_pu, int n_ref, int mode,
                                xme_plan **out)
+{
+    return plan_create_impl(prm, pus_in, n_pu, n_ref, mode, nullptr, 0, nullptr, out);
+}
+
+static int plan_create_impl(const xme_me_params *prm, const xme_pu *pus_in, int n_pu, int n_ref, int mode,
+                            uint8_t *arena, size_t arena_cap, hipStream_t arena_stream, xme_plan **out)
 {
     if (!prm || !pus_in || !out || n_pu <= 0 || n_ref <= 0 || n_ref > XME_MAX_REFS ||
         (mode != XME_PLAN_BATCHED && mode != XME_PLAN_PLAIN)) {
@@ -533,8 +586,22 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         }
     }
 
+    if (arena) {
+        // what every plan holds at least: the PU records, and per instance its
+        // state, best key and place in an item list
+        const size_t floor_bytes = (size_t)n_pu * sizeof(DevPu) +
+                                   (size_t)n_pu * n_ref * (sizeof(DevInst) + sizeof(uint64_t) + sizeof(int32_t));
+        if (floor_bytes > arena_cap) {
+            xme_set_error("xme_me_search_static: the job's plan needs at least %zu bytes, more than the %zu bytes of "
+                          "static scratch (fewer PUs or references per call)", floor_bytes, arena_cap);
+            return XME_ERR_BUDGET;
+        }
+    }
     xme_plan *pl = new xme_plan();
     hipGetDevice(&pl->device);
+    pl->arena = arena;
+    pl->arena_cap = arena_cap;
+    pl->arena_stream = arena_stream;
     pl->mode = mode;
     pl->kmax = xme_fs_kmax();
     pl->p = p;
@@ -740,10 +807,9 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     auto up = [&](void **d, const void *h, size_t n) {
         if (e != hipSuccess)
             return;
-        e = hipMalloc(d, n ? n : 16);
-        pl->dev_bytes += n ? n : 16;
-        if (e == hipSuccess && n)
-            e = hipMemcpy(*d, h, n, hipMemcpyHostToDevice);
+        e = plan_dev_alloc(pl, d, n);
+        if (e == hipSuccess)
+            e = plan_upload(pl, *d, h, n);
     };
     up((void **)&pl->d_pus, pus.data(), pus.size() * sizeof(DevPu));
     up((void **)&pl->d_tasks, tasks.data(), tasks.size() * sizeof(FsTask));
@@ -751,20 +817,17 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
     up((void **)&pl->d_item_insts, item_insts.data(), item_insts.size() * sizeof(int32_t));
     up((void **)&pl->d_pat_list, pat_list.data(), pat_list.size() * sizeof(int32_t));
     if (e == hipSuccess && p.method != XME_ME_FS) {
-        e = hipMalloc((void **)&pl->d_pat_points, sizeof(uint32_t) * pl->n_inst);
-        pl->dev_bytes += sizeof(uint32_t) * pl->n_inst;
+        e = plan_dev_alloc(pl, (void **)&pl->d_pat_points, sizeof(uint32_t) * pl->n_inst);
         if (e == hipSuccess)
-            e = hipMemset(pl->d_pat_points, 0, sizeof(uint32_t) * pl->n_inst);
+            e = plan_zero(pl, pl->d_pat_points, sizeof(uint32_t) * pl->n_inst);
         pl->inst_area.resize(pl->n_inst);
         for (int i = 0; i < pl->n_inst; i++)
             pl->inst_area[i] = pus[i / n_ref].w * pus[i / n_ref].h;
     }
-    if (e == hipSuccess && p.method == XME_ME_UMH) {
-        e = hipMalloc((void **)&pl->d_umh, sizeof(xme_umh_ctx) * pl->n_inst);
-        pl->dev_bytes += sizeof(xme_umh_ctx) * pl->n_inst;
-    }
+    if (e == hipSuccess && p.method == XME_ME_UMH)
+        e = plan_dev_alloc(pl, (void **)&pl->d_umh, sizeof(xme_umh_ctx) * pl->n_inst);
+    std::vector<int32_t> sp_list;      // lives until the arena uploads have been consumed
     {
-        std::vector<int32_t> sp_list;
         sp_list.reserve(pus.size());
         for (int large = 0; large < 2; large++)
             for (size_t i = 0; i < pus.size(); i++)
@@ -777,27 +840,43 @@ extern "C" int xme_plan_create(const xme_me_params *prm, const xme_pu *pus_in, i
         up((void **)&pl->d_sp_list, sp_list.data(), sp_list.size() * sizeof(int32_t));
     }
     if (e == hipSuccess)
-        e = hipMalloc((void **)&pl->d_inst, sizeof(DevInst) * pl->n_inst);
+        e = plan_dev_alloc(pl, (void **)&pl->d_inst, sizeof(DevInst) * pl->n_inst);
     if (e == hipSuccess)
-        e = hipMalloc((void **)&pl->d_best, sizeof(uint64_t) * pl->n_inst);
+        e = plan_dev_alloc(pl, (void **)&pl->d_best, sizeof(uint64_t) * pl->n_inst);
+    // the search launches' queue counters start at zero (the last workgroup
+    // of a launch resets them)
     if (e == hipSuccess)
-        e = hipMalloc((void **)&pl->d_counters, 64);
+        e = plan_dev_alloc(pl, (void **)&pl->d_counters, 64);
     if (e == hipSuccess)
-        e = hipMemset(pl->d_counters, 0, 64);
+        e = plan_zero(pl, pl->d_counters, 64);
     if (e == hipSuccess)
-        e = hipMalloc((void **)&pl->d_desc, sizeof(FrameDesc));
-    pl->dev_bytes += (sizeof(DevInst) + sizeof(uint64_t)) * pl->n_inst + 64 + sizeof(FrameDesc);
+        e = plan_dev_alloc(pl, (void **)&pl->d_desc, sizeof(FrameDesc));
     for (auto &s : pl->slots) {
         s.host = nullptr;
         s.ev = nullptr;
         s.used = false;
+        if (pl->arena)
+            continue;              // the descriptor goes up from arena_desc
         if (e == hipSuccess)
             e = hipHostMalloc((void **)&s.host, sizeof(FrameDesc), hipHostMallocDefault);
         if (e == hipSuccess)
             e = hipEventCreateWithFlags(&s.ev, hipEventDisableTiming);
     }
+    // arena uploads are asynchronous copies from the host lists above, which
+    // end with this call
+    if (e == hipSuccess && pl->arena)
+        e = hipStreamSynchronize(arena_stream);
     if (e != hipSuccess) {
-        int rc = xme_hip_fail(e, "xme_plan_create");
+        int rc;
+        if (pl->arena)
+            (void)hipStreamSynchronize(arena_stream);   // copies already queued read the host lists
+        if (pl->arena && pl->arena_used > pl->arena_cap) {
+            xme_set_error("xme_me_search_static: the job's plan needs more than the %zu bytes of static scratch "
+                          "(fewer PUs or references per call)", pl->arena_cap);
+            rc = XME_ERR_BUDGET;
+        } else {
+            rc = xme_hip_fail(e, "xme_plan_create");
+        }
         plan_free(pl);
         return rc;
     }
@@ -956,10 +1035,12 @@ static int fill_desc(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_r
         }
     }
     DescSlot &slot = pl->slots[pl->slot_next];
-    pl->slot_next = (pl->slot_next + 1) % 8;
-    if (slot.used)
-        XME_HIP(hipEventSynchronize(slot.ev));
-    FrameDesc *d = slot.host;
+    if (!pl->arena) {
+        pl->slot_next = (pl->slot_next + 1) % 8;
+        if (slot.used)
+            XME_HIP(hipEventSynchronize(slot.ev));
+    }
+    FrameDesc *d = pl->arena ? &pl->arena_desc : slot.host;
     memset(d, 0, sizeof(*d));
     d->cur = cur;
     d->i_cur = i_cur;
@@ -971,8 +1052,10 @@ static int fill_desc(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const xme_r
     d->results = res;
     d->cost_map = cost_map;
     XME_HIP(hipMemcpyAsync(pl->d_desc, d, sizeof(FrameDesc), hipMemcpyHostToDevice, s));
-    XME_HIP(hipEventRecord(slot.ev, s));
-    slot.used = true;
+    if (!pl->arena) {
+        XME_HIP(hipEventRecord(slot.ev, s));
+        slot.used = true;
+    }
     return XME_OK;
 }
 
@@ -1128,4 +1211,62 @@ extern "C" int xme_me_time(xme_plan *pl, const pel_t *cur, intptr_t i_cur, const
     if (job_ms_mean)
         *job_ms_mean = sum_j / reps;
     return XME_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Frame-level ME without device allocation.  The plan's device buffers (work
+// lists, per-instance state and best keys, queue counters, frame descriptor)
+// are carved from this file-scope array, one per GPU for the life of the
+// process.  Host code never passes the array's name to HIP: its device
+// address comes from hipGetSymbolAddress, once per device.
+
+static __device__ __align__(256) uint8_t xme_scratch[XME_STATIC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_static_mu;                 // one static search per process at a time
+uint8_t *g_scratch_ptr[64] = {};        // device address of xme_scratch, per device
+}
+
+extern "C" int xme_me_search_static(const xme_me_params *prm, const xme_pu *pus, int n_pu, int n_ref,
+                                    const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
+                                    xme_result *d_results, void *stream)
+{
+    if (!prm || !pus || !refs || !d_results || n_pu <= 0 || n_ref <= 0) {
+        xme_set_error("xme_me_search_static: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (prm->me_method == XME_ME_UMH) {
+        xme_set_error("xme_me_search_static: the UMH search needs its per-PU context (xme_plan_set_umh_context)");
+        return XME_ERR_ARG;
+    }
+    int dev = 0;
+    XME_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        xme_set_error("xme_me_search_static: device %d out of range", dev);
+        return XME_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(xme_scratch));
+        if (e != hipSuccess || !ptr)
+            return xme_hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(xme_scratch)");
+        g_scratch_ptr[dev] = static_cast<uint8_t *>(ptr);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    xme_plan *pl = nullptr;
+    int rc = plan_create_impl(prm, pus, n_pu, n_ref, XME_PLAN_BATCHED, g_scratch_ptr[dev], XME_STATIC_SCRATCH_BYTES,
+                              s, &pl);
+    if (rc)
+        return rc;
+    rc = fill_desc(pl, d_cur, i_cur, refs, d_results, nullptr, s);
+    if (rc == XME_OK)
+        rc = launch_all(pl, s);
+    // the scratch is free for the next call, and the plan's host copy of the
+    // frame descriptor may go, once the kernels have finished
+    hipError_t e = hipStreamSynchronize(s);
+    plan_free(pl);
+    if (rc == XME_OK && e != hipSuccess)
+        rc = xme_hip_fail(e, "xme_me_search_static");
+    return rc;
 }

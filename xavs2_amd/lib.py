@@ -135,6 +135,7 @@ def lib():
         L.xme_plan_set_umh_context.argtypes = [P, P, P]
         L.xme_me_run.argtypes = [P, P, Z, P, P, P, P]
         L.xme_me_run_graph.argtypes = [P, P, Z, P, P, P]
+        L.xme_me_search_static.argtypes = [C.POINTER(MeParams), P, I, I, P, Z, P, P, P]
         L.xme_me_time.argtypes = [P, P, Z, P, P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), P]
         L.xme_copy_kernel.argtypes = [P, P, C.c_size_t, P]
         L.xme_tune.argtypes = [C.POINTER(MeParams), P, I, P]
