@@ -357,11 +357,11 @@ def bipred_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, re
 def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, qp=32):
     """Luma inter reconstruction of the frame (xme_inter_recon_batch): every
     PU split into the largest square TUs (4..32) that tile it, predicted from
-    reference 0 at 4 x its full-pel best, at QP 32; one call per TU size.
-    Algorithmic HBM bytes: 5 per pixel (current and prediction read, levels
-    (int16) and reconstruction written); the kernels as composed move about 15
-    (prediction read twice, the prediction error and the reconstructed
-    residual through HBM).  Parity with the oracle on the TUs of every 4th PU."""
+    reference 0 at 4 x its full-pel best, at QP 32; one call per TU size,
+    each one launch (the planes are resident; no prediction-error or residual
+    buffers).  Algorithmic HBM bytes: 5 per pixel (current and prediction
+    read, levels (int16) and reconstruction written).  Parity with the oracle
+    on the TUs of every 4th PU."""
     import ctypes as C
     from oracle import oracle as O
     r0 = gpu_res[0::n_ref]
@@ -385,13 +385,13 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
             continue
         t = np.array(rows, np.int32)
         ntu = len(t)
-        bufs.append((n, ntu, X.upload(t), X.DeviceBuffer(2 * ntu * n * n), X.DeviceBuffer(2 * ntu * n * n),
-                     X.DeviceBuffer(2 * ntu * n * n), X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)))
+        bufs.append((n, ntu, X.upload(t), None, X.DeviceBuffer(2 * ntu * n * n), None, X.DeviceBuffer(4 * ntu),
+                     X.DeviceBuffer(4 * ntu)))
 
     def frame():
-        for n, ntu, d_t, resid, level, rres, nnz, ssd in bufs:
+        for n, ntu, d_t, _resid, level, _rres, nnz, ssd in bufs:
             X.check(X.lib().xme_inter_recon_batch(n, qp, C.c_void_p(dcur.origin), dcur.stride, C.byref(rd), w, h,
-                                                  d_t.at(), ntu, resid.at(), level.at(), rres.at(),
+                                                  d_t.at(), ntu, None, level.at(), None,
                                                   C.c_void_p(rec.origin), rec.stride, nnz.at(), ssd.at(), stream.h),
                     "xme_inter_recon_batch")
 
@@ -410,7 +410,7 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
         "ms_per_frame": ms,
         "pixels_per_s": pixels / (ms * 1e-3),
         "algorithmic_gb_per_s": 5.0 * pixels / (ms * 1e-3) / 1e9,
-        "moved_gb_per_s": 15.0 * pixels / (ms * 1e-3) / 1e9,
+        "hbm_frac": 5.0 * pixels / (ms * 1e-3) / 8e12,
     }
     if cpu:
         p = O.Plane(w, h)

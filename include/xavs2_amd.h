@@ -171,7 +171,7 @@ typedef struct {
     int32_t use_fractional_me;       /* 0 integer only, 1 + half-pel, 2 + quarter-pel */
     int32_t use_fast_sub_me;         /* Spiral2 with step 2 (me.c:501-502) */
     int32_t enable_pmvr;             /* PMVR quarter-pel adaptation (me.c:549-556) */
-    int32_t me_method;               /* h->param->me_method: XME_ME_FS, _DIA, _HEX or _UMH */
+    int32_t me_method;               /* h->param->me_method: XME_ME_FS, _DIA, _HEX, _UMH or _TZ */
     int32_t umh_big_hex_level;       /* h->UMH_big_hex_level (presets.c:341): 0, 1 or 2 */
 } xme_me_params;
 
@@ -179,12 +179,21 @@ typedef struct {
  * XME_ME_FS is the full search over [-R, R)^2 (me.c:1156-1164); XME_ME_DIA
  * the small-diamond descent (me.c:1133-1154); XME_ME_HEX the hexagon descent
  * followed by the diamond (me.c:1107-1131); XME_ME_UMH the UMH steps
- * (me.c:1030-1099) followed by the hexagon or the diamond.  TZ (4) is not
- * provided. */
+ * (me.c:1030-1099) followed by the hexagon or the diamond; XME_ME_TZ the TZ
+ * steps (me.c:926-1028, tz_pattern_search me.c:669-874) followed by the
+ * hexagon and the diamond, or stopping after its two-point search.  TZ's
+ * raster step evaluates its points offset once more by the diamond's centre
+ * (pix_base of ME_COST_IPEL_X4, me.c:245); where such a block leaves the
+ * padded reference (XME_PAD pixels; within 4 pixels of its right edge counts
+ * as leaving) or its MV difference leaves the mvbits table, the reference
+ * reads outside its arrays: the point is skipped and the result's status is
+ * 2.  So is a search whose best after such a point is too close to the
+ * padding's edge for the hexagon's reads: it stops there. */
 #define XME_ME_FS  0
 #define XME_ME_DIA 1
 #define XME_ME_HEX 2
 #define XME_ME_UMH 3
+#define XME_ME_TZ  4
 
 /* Per (PU, reference) encoder state that the UMH search reads from
  * xavs2_me_t (set by pred_inter_search_single's helpers,
@@ -214,7 +223,8 @@ typedef struct {
     int16_t bmv_x, bmv_y;            /* p_me->bmv (qpel) */
     int16_t bmv2_x, bmv2_y;          /* p_me->bmv2 (full-pel) */
     int32_t mvcost;                  /* p_me->mvcost[PDIR_FWD] */
-    int32_t status;                  /* 0 ok, 1 no legal candidate (returns XME_MAX_DISTORTION) */
+    int32_t status;                  /* 0 ok, 1 no legal candidate (returns XME_MAX_DISTORTION), 2 TZ read
+                                      * outside the reference's arrays (undefined there; see XME_ME_TZ) */
 } xme_result;
 
 /* One reference picture resident in device memory: planes[0] is the full-pel
@@ -413,6 +423,56 @@ typedef struct {
 int xme_get_mvp_batch(const xme_mvp_job *d_jobs, int n, int b_slice, const int32_t ref_dpoc[XME_MVP_REFS],
                       const int32_t ref_dpoc_multi[XME_MVP_REFS], xme_mvp_result *d_out, void *stream);
 
+/* The UMH search's encoder state of a batch of (PU, reference) jobs, replacing
+ * get_mvp_default_sad (source/encoder/md_inter.c:527-608: the MVP of
+ * get_mvp_default and pred_sad_space, the neighbour cost of the MVP's source)
+ * and fast_me_prepare_info (md_inter.c:640-718: mvp1 from the upper
+ * partition layer, mvp2 from the nearest reference, mvp3 from the co-located
+ * block, pred_sad and the betas), which pred_inter_search_single calls per
+ * reference before xavs2_me_search (md_inter.c:779-790).  A job holds what
+ * those functions read from the encoder state. */
+typedef struct {
+    xme_mvp_job mvp;                 /* neighbours, PU in its CU, ref_idx, bwd_2nd */
+    int32_t nb_sad[3];               /* all_mincost[..][mode][ref_idx] at the left, top and top-right 4x4 blocks
+                                      * (0 where the reference reads none) */
+    int32_t mode;                    /* PRED_2Nx2N (1) .. PRED_nRx2N (7), common.h:198-207 */
+    int32_t mincost_up[3];           /* mincosts[PRED_2NxN / PRED_Nx2N / PRED_2Nx2N][ref_idx] of the CU */
+    int16_t best_up[3][2];           /* all_best_mv[PRED_2NxN / Nx2N / 2Nx2N][pu_idx][ref_idx] */
+    int32_t mincost_prev;            /* mincosts[mode][ref_idx - 1] */
+    int16_t best_prev[2];            /* all_best_mv[mode][pu_idx][ref_idx - 1] */
+    int16_t col_mv[2];               /* fref[0]->pu_mv at the PU's 16x16 block */
+    int8_t col_valid;                /* fref[0] exists and is not an I picture */
+    int8_t col_ref;                  /* fref[0]->pu_ref there (-1: none) */
+    int16_t reserved;
+    int32_t col_dpoc;                /* fref[0]->ref_dpoc[col_ref] */
+} xme_umh_job;
+
+/* Status: 0; 2 an index outside 0..XME_MVP_REFS-1 or a mode outside 1..7;
+ * 3 a scaling product that leaves int (as xme_mvp_result); 4 a division by
+ * a zero POC distance (undefined in the reference: the fields after it stay
+ * 0).  ctx is what xme_plan_set_umh_context takes. */
+typedef struct {
+    int16_t mvp_x, mvp_y;            /* p_me->mvp */
+    int32_t pred_sad_space;          /* get_mvp_default_sad's return */
+    int32_t status;
+    int32_t reserved;
+    xme_umh_ctx ctx;
+} xme_umh_state;
+
+/* slice_type: SLICE_TYPE_I 0, P 1, B 2, F 3 (common.h:158-161); ref_dpoc and
+ * ref_dpoc_multi as for xme_get_mvp_batch; umh_bsize = h->umh_bsize. */
+int xme_umh_prepare_batch(const xme_umh_job *d_jobs, int n, int slice_type, const int32_t ref_dpoc[XME_MVP_REFS],
+                          const int32_t ref_dpoc_multi[XME_MVP_REFS], const double umh_bsize[8],
+                          xme_umh_state *d_out, void *stream);
+/* h->umh_bsize for a picture's QP and slice type, xavs2_me_init_umh_threshold
+ * (source/encoder/me.c:647-665). */
+int xme_umh_bsize(int i_qp, int slice_type, double umh_bsize[8]);
+/* A UMH plan's context from device-resident states, as
+ * xme_plan_set_umh_context: the ctx of d_states[pu * n_ref + r] as
+ * xme_umh_prepare_batch wrote them (a copy on the device, on `stream`).  No
+ * reference counterpart, as xme_plan_set_umh_context. */
+int xme_plan_set_umh_states(xme_plan *plan, const xme_umh_state *d_states, void *stream);
+
 /* ---- 4. reference planes, MC and the residual path ---------------------- */
 /* plane_expand_border over a whole plane (source/common/frame.c:470-507). */
 int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, int pad, void *stream);
@@ -504,7 +564,7 @@ typedef struct {
     int32_t enable_pmvr;
     int32_t workgroup_share_pct;     /* % of a full-device search launch per frame (1..100; 0 = 100) */
     int64_t launch_split_absdiff;    /* search work per launch (0: 8e9; < 0: no split) */
-    int32_t me_method;               /* XME_ME_FS, XME_ME_DIA or XME_ME_HEX (UMH needs per-job state) */
+    int32_t me_method;               /* XME_ME_FS, _DIA, _HEX or _TZ (UMH needs per-job state) */
     int32_t reserved;
 } xme_server_config;
 typedef struct {

@@ -27,7 +27,7 @@ class MeParam(C.Structure):
 
 
 # h->param->me_method (common/common.h:145-151)
-ME_FS, ME_DIA, ME_HEX, ME_UMH = 0, 1, 2, 3
+ME_FS, ME_DIA, ME_HEX, ME_UMH, ME_TZ = 0, 1, 2, 3, 4
 
 # xo_umh_ctx_t: per-(PU, reference) state read by the UMH search
 UMH_CTX_DTYPE = np.dtype([("mvp1_x", "<i4"), ("mvp1_y", "<i4"), ("mvp2_x", "<i4"), ("mvp2_y", "<i4"),
@@ -116,6 +116,10 @@ def lib():
         L.xo_me_search_batch.argtypes = [C.POINTER(MeParam), P, C.c_int, P, ip, P, ip, P, C.c_int]
         L.xo_me_pattern_path.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, C.c_int, C.c_int,
                                          C.c_int, C.c_int, C.c_int, P, C.c_int, P]
+        L.xo_pattern_landscape.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P]
+        L.xo_me_pattern_trace.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P]
         L.xo_scale_mv_skip.argtypes = [C.c_int, C.c_int, C.c_int]
         L.xo_me_search_sym.restype = C.c_int32
         L.xo_me_search_sym.argtypes = [C.POINTER(MeParam), C.POINTER(SymJob), P, ip, P, P, ip, C.POINTER(SymResult)]
@@ -285,6 +289,43 @@ def me_pattern_path(prm, pu_tuple, cur_plane, ref_plane, start, start_cost, max_
     assert n <= max_pts
     pts = [(int(path[2 * k]), int(path[2 * k + 1])) for k in range(n)]
     return (pts, route.value) if with_route else pts
+
+
+def pattern_landscape(prm, pu_tuple, costs, start, start_cost=None, p0=None, ctx=None, max_trace=4096):
+    """The integer stage of prm.me_method over a cost table instead of
+    pictures: costs is a (2 span + 1)^2 int array indexed [y + span, x + span].
+    Returns (trace of evaluated points, (bx, by), bcost, route)."""
+    costs = np.ascontiguousarray(costs, dtype=np.int32)
+    span = (costs.shape[0] - 1) // 2
+    assert costs.shape == (2 * span + 1, 2 * span + 1)
+    if start_cost is None:
+        start_cost = int(costs[start[1] + span, start[0] + span])
+    p0 = start if p0 is None else p0
+    trace = np.zeros(2 * max_trace, dtype=np.int32)
+    bx, by, bc, route = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    c = np.ascontiguousarray(ctx, dtype=UMH_CTX_DTYPE) if ctx is not None else None
+    n = lib().xo_pattern_landscape(C.byref(prm), C.byref(Pu(*pu_tuple)), ptr(c) if c is not None else None,
+                                   ptr(costs), span, start[0], start[1], start_cost, p0[0], p0[1], ptr(trace),
+                                   max_trace, C.byref(bx), C.byref(by), C.byref(bc), C.byref(route))
+    assert n <= max_trace
+    pts = [(int(trace[2 * k]), int(trace[2 * k + 1])) for k in range(n)]
+    return pts, (bx.value, by.value), bc.value, route.value
+
+
+def pattern_trace(prm, pu_tuple, cur_plane, ref_plane, start, start_cost, p0=None, ctx=None, max_trace=8192):
+    """The integer stage of prm.me_method on pictures: (evaluated points in
+    order, (bx, by), bcost, route)."""
+    p0 = start if p0 is None else p0
+    trace = np.zeros(2 * max_trace, dtype=np.int32)
+    bx, by, bc, route = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    c = np.ascontiguousarray(ctx, dtype=UMH_CTX_DTYPE) if ctx is not None else None
+    n = lib().xo_me_pattern_trace(C.byref(prm), C.byref(Pu(*pu_tuple)), ptr(c) if c is not None else None,
+                                  cur_plane.origin_ptr(), cur_plane.stride, ref_plane.origin_ptr(), ref_plane.stride,
+                                  start[0], start[1], start_cost, p0[0], p0[1], ptr(trace), max_trace, C.byref(bx),
+                                  C.byref(by), C.byref(bc), C.byref(route))
+    assert n <= max_trace
+    pts = [(int(trace[2 * k]), int(trace[2 * k + 1])) for k in range(n)]
+    return pts, (bx.value, by.value), bc.value, route.value
 
 
 def me_search_sym(prm, job, cur_plane, planes1, planes2):

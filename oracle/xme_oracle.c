@@ -289,15 +289,43 @@ typedef struct {
     uint32_t mv_min, mv_max;
     int bcost, bmx, bmy, dir;
     int *path, max_pts, n_pts;
+    /* TZ: the MV window's full-pel limits, the PU and the picture (the raster
+     * step's reads are checked against the padded plane) */
+    int xmin, ymin, xmax, ymax;
+    int pix_x, pix_y, fw, fh;
+    int undefined;
+    /* test hook: costs from a table over [-span, span]^2 instead of the
+     * pictures, and the evaluated points in order */
+    const int32_t *land;
+    int span;
+    int *trace, max_trace, n_trace;
 } pat_ctx_t;
+
+/* The cost of one integer point: SAD + MV_COST_IPEL (me.c:181-185). */
+static int point_cost(pat_ctx_t *c, int mx, int my)
+{
+    if (c->trace) {
+        if (c->n_trace < c->max_trace) {
+            c->trace[2 * c->n_trace] = mx;
+            c->trace[2 * c->n_trace + 1] = my;
+        }
+        c->n_trace++;
+    }
+    if (c->land) {
+        if (mx < -c->span || mx > c->span || my < -c->span || my > c->span)
+            return 1 << 28;
+        return c->land[(my + c->span) * (2 * c->span + 1) + (mx + c->span)];
+    }
+    return xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
+           weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+}
 
 /* One point of ME_COST_IPEL_X3_DIR / ME_COST_IPEL_X4_DIR (me.c:223-287): the
  * SAD plus MV_COST_IPEL, evaluated without a range check, taken on a strict
  * '<' with its direction code. */
 static void pat_point(pat_ctx_t *c, int mx, int my, int d)
 {
-    int cost = xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
-               weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+    int cost = point_cost(c, mx, my);
     if (cost < c->bcost) {
         c->bcost = cost;
         c->bmx = mx;
@@ -321,8 +349,7 @@ static void ipel_point(pat_ctx_t *c, int mx, int my)
 {
     if (!check_mv_range(mx, my, c->mv_min, c->mv_max))
         return;
-    int cost = xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
-               weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+    int cost = point_cost(c, mx, my);
     if (cost < c->bcost) {
         c->bcost = cost;
         c->bmx = mx;
@@ -334,8 +361,7 @@ static void ipel_point(pat_ctx_t *c, int mx, int my)
  * MV_COST_IPEL, no range check, COPY3_IF_LT (strict '<', no direction code). */
 static void free_point(pat_ctx_t *c, int mx, int my)
 {
-    int cost = xo_sad(c->W, c->H, c->org, c->i_cur, c->fref + (intptr_t)my * c->i_ref + mx, c->i_ref) +
-               weighted_cost(c->lambda, xo_mvbits(c->R, 4 * mx - c->pmx) + xo_mvbits(c->R, 4 * my - c->pmy));
+    int cost = point_cost(c, mx, my);
     if (cost < c->bcost) {
         c->bcost = cost;
         c->bmx = mx;
@@ -462,14 +488,312 @@ static int umh_prefix(pat_ctx_t *c, const xo_umh_ctx_t *u, int p0x, int p0y, int
     return 2;
 }
 
+/* TZ: mv_info (me.c:51-57), the running best of tz_pattern_search. */
+typedef struct {
+    int bmx, bmy, bdir, bcost, bdist;
+} tz_info_t;
+
+/* ME_COST_IPEL_DIR_DIST (me.c:289-294), and one point of
+ * ME_COST_IPEL_X4_DIR_DIST (me.c:298-322), which computes the four SADs but
+ * takes each point only inside the window: COPY5_IF_LT into mv_info. */
+static void tz_point(pat_ctx_t *c, tz_info_t *m, int mx, int my, int dir, int dist)
+{
+    if (!check_mv_range(mx, my, c->mv_min, c->mv_max))
+        return;
+    const int cost = point_cost(c, mx, my);
+    if (cost < m->bcost) {
+        m->bcost = cost;
+        m->bmx = mx;
+        m->bmy = my;
+        m->bdir = dir;
+        m->bdist = dist;
+    }
+}
+
+/* A round's exit test (me.c:721-725 and the same after every round):
+ * returns 1 when the pattern search stops. */
+static int tz_round_end(const tz_info_t *m, int bcost, int *rounds, int early_exit)
+{
+    if (m->bcost < bcost)
+        *rounds = 0;
+    else if (++*rounds >= early_exit)
+        return 1;
+    return 0;
+}
+
+/* tz_pattern_search (me.c:669-874): a cross at distance 1, then diamonds of
+ * 8 points at 2, 4, 8 (the four at dist and the four diagonal ones at
+ * dist / 2), then 16-point rings at 16, 32, .. merange, each round centred on
+ * the best so far, stopping after early_exit rounds without a gain.  The
+ * reference sets omy = mv->bmx in every round (me.c:690, :733, :796), so the
+ * rounds are centred on (bmx, bmx); restated as written. */
+static void tz_pattern(pat_ctx_t *c, tz_info_t *m, int early_exit, int merange)
+{
+    int rounds = 0, dist = 1;
+    int omx = m->bmx, omy = m->bmx;
+    int bcost = m->bcost;
+    int top = omy - dist, bottom = omy + dist, left = omx - dist, right = omx + dist;
+    if (top >= c->ymin && left >= c->xmin && right <= c->xmax && bottom <= c->ymax) {
+        tz_point(c, m, omx, top, 2, dist);
+        tz_point(c, m, left, omy, 4, dist);
+        tz_point(c, m, right, omy, 5, dist);
+        tz_point(c, m, omx, bottom, 7, dist);
+    } else {
+        if (top >= c->ymin)
+            tz_point(c, m, omx, top, 2, dist);
+        if (left >= c->xmin)
+            tz_point(c, m, left, omy, 4, dist);
+        if (right <= c->xmax)
+            tz_point(c, m, right, omy, 5, dist);
+        if (bottom <= c->ymax)
+            tz_point(c, m, omx, bottom, 7, dist);
+    }
+    if (tz_round_end(m, bcost, &rounds, early_exit))
+        return;
+    for (dist = 2; dist <= 8; dist <<= 1) {
+        omx = m->bmx;
+        omy = m->bmx;
+        bcost = m->bcost;
+        top = omy - dist;
+        bottom = omy + dist;
+        left = omx - dist;
+        right = omx + dist;
+        const int top2 = omy - (dist >> 1), bottom2 = omy + (dist >> 1);
+        const int left2 = omx - (dist >> 1), right2 = omx + (dist >> 1);
+        if (top >= c->ymin && left >= c->xmin && right <= c->xmax && bottom <= c->ymax) {
+            tz_point(c, m, omx, top, 2, dist);
+            tz_point(c, m, left2, top2, 1, dist >> 1);
+            tz_point(c, m, right2, top2, 3, dist >> 1);
+            tz_point(c, m, left, omy, 4, dist);
+            tz_point(c, m, right, omy, 5, dist);
+            tz_point(c, m, left2, bottom2, 6, dist >> 1);
+            tz_point(c, m, right2, bottom2, 8, dist >> 1);
+            tz_point(c, m, omx, bottom, 7, dist);
+        } else {
+            if (top >= c->ymin)
+                tz_point(c, m, omx, top, 2, dist);
+            if (top2 >= c->ymin) {
+                if (left2 >= c->xmin)
+                    tz_point(c, m, left2, top2, 1, dist >> 1);
+                if (right2 <= c->xmax)
+                    tz_point(c, m, right2, top2, 3, dist >> 1);
+            }
+            if (left >= c->xmin)
+                tz_point(c, m, left, omy, 4, dist);
+            if (right <= c->xmax)
+                tz_point(c, m, right, omy, 5, dist);
+            if (bottom2 <= c->ymax) {
+                if (left2 >= c->xmin)
+                    tz_point(c, m, left2, bottom2, 6, dist >> 1);
+                if (right2 <= c->xmax)
+                    tz_point(c, m, right2, bottom2, 8, dist >> 1);
+            }
+            if (bottom <= c->ymax)
+                tz_point(c, m, omx, bottom, 7, dist);
+        }
+        if (tz_round_end(m, bcost, &rounds, early_exit))
+            return;
+    }
+    for (dist = 16; dist <= merange; dist <<= 1) {
+        omx = m->bmx;
+        omy = m->bmx;
+        bcost = m->bcost;
+        top = omy - dist;
+        bottom = omy + dist;
+        left = omx - dist;
+        right = omx + dist;
+        const int inside = top >= c->ymin && left >= c->xmin && right <= c->xmax && bottom <= c->ymax;
+        /* the four at dist, then per idx the four at (dist / 4) * idx in from
+         * the corners of the ring, in the same order in both branches */
+        if (inside || top >= c->ymin)
+            tz_point(c, m, omx, top, 0, dist);
+        if (inside || left >= c->xmin)
+            tz_point(c, m, left, omy, 0, dist);
+        if (inside || right <= c->xmax)
+            tz_point(c, m, right, omy, 0, dist);
+        if (inside || bottom <= c->ymax)
+            tz_point(c, m, omx, bottom, 0, dist);
+        for (int idx = 1; idx < 4; idx++) {
+            const int yt = top + (dist >> 2) * idx, yb = bottom - (dist >> 2) * idx;
+            const int xl = omx - (dist >> 2) * idx, xr = omx + (dist >> 2) * idx;
+            if (inside || yt >= c->ymin) {
+                if (inside || xl >= c->xmin)
+                    tz_point(c, m, xl, yt, 0, dist);
+                if (inside || xr <= c->xmax)
+                    tz_point(c, m, xr, yt, 0, dist);
+            }
+            if (inside || yb <= c->ymax) {
+                if (inside || xl >= c->xmin)
+                    tz_point(c, m, xl, yb, 0, dist);
+                if (inside || xr <= c->xmax)
+                    tz_point(c, m, xr, yb, 0, dist);
+            }
+        }
+        if (tz_round_end(m, bcost, &rounds, early_exit))
+            return;
+    }
+}
+
+/* Two Point Search offsets (me.c:144-153): for direction d (1..8), the two
+ * outer points next to it. */
+static const int8_t TZ_OFFSETS[16][2] = {
+    { -1, 0 }, { 0, -1 }, { -1, -1 }, { 1, -1 }, { -1, 0 }, { 1, 0 }, { -1, 1 }, { -1, -1 },
+    { 1, -1 }, { 1, 1 }, { -1, 0 }, { 0, 1 }, { -1, 1 }, { 1, 1 }, { 1, 0 }, { 0, 1 }
+};
+
+static void tz_two_points(pat_ctx_t *c, int dir)
+{
+    const int bx = c->bmx, by = c->bmy;
+    ipel_point(c, bx + TZ_OFFSETS[(dir - 1) * 2][0], by + TZ_OFFSETS[(dir - 1) * 2][1]);
+    ipel_point(c, bx + TZ_OFFSETS[(dir - 1) * 2 + 1][0], by + TZ_OFFSETS[(dir - 1) * 2 + 1][1]);
+}
+
+/* One point of the sad_x4 branch of ME_COST_IPEL_X4 as the raster step calls
+ * it (me.c:990-991): the macro offsets the absolute MV (mx, my) by (omx, omy)
+ * once more (pix_base, me.c:245), so the SAD, the MV cost and the stored MV
+ * are those of (omx + mx, omy + my), unchecked.  Where that block leaves the
+ * padded reference (XO_PAD pixels around the picture) or its MV difference
+ * leaves the mvbits table, the reference reads outside its arrays
+ * (undefined): the point is not taken and the search is marked undefined.
+ * A block within 4 pixels of the padded reference's right edge counts as
+ * leaving it (the dword reads of a row reach 4 bytes past the block). */
+static void tz_raster_point(pat_ctx_t *c, int omx, int omy, int mx, int my)
+{
+    const int x = omx + mx, y = omy + my, lim = xo_mvbits_max_mvd(c->R);
+    if (c->pix_x + x < -XO_PAD || c->pix_x + x + c->W + 4 > c->fw + XO_PAD ||
+        c->pix_y + y < -XO_PAD || c->pix_y + y + c->H > c->fh + XO_PAD ||
+        iabs(4 * x - c->pmx) > lim || iabs(4 * y - c->pmy) > lim) {
+        c->undefined = 1;
+        return;
+    }
+    free_point(c, x, y);
+}
+
+/* The hexagon after TZ starts unchecked at +-2 pixels around the best, which
+ * an offset raster point may have put anywhere in the padded reference: it
+ * runs only if the block grown by 2 pixels, widened to whole dwords
+ * ([x - 2 rounded down to 4, x + w + 8)), lies in the padded reference;
+ * otherwise the reference may read outside it and the search stops
+ * undefined. */
+static int tz_hexagon_box_ok(const pat_ctx_t *c)
+{
+    const int x0 = (c->pix_x + c->bmx - 2) & ~3;
+    return x0 >= -XO_PAD && x0 + 4 * ((c->W + 10) >> 2) <= c->fw + XO_PAD &&
+           c->pix_y + c->bmy - 2 >= -XO_PAD && c->pix_y + c->bmy + c->H + 2 <= c->fh + XO_PAD;
+}
+
+/* The TZ branch of xavs2_me_search (me.c:926-1028) from the best MV candidate:
+ * six points around it (ME_COST_IPEL_X3, unchecked), DIA_ITER, the pattern
+ * search with EarlyExitIters = 3, the two-point search after a best at
+ * distance 1 (which may stop the search), the raster refinement after a best
+ * beyond RasterDistance = 16, pattern searches with MaxIters = 32 while the
+ * best moves by 2 or more, then umh_step_2 (the hexagon, then the diamond).
+ * Returns 1 to go on with the hexagon, 0 when the search stops here; *route
+ * gets bit 0 for a stop, bit 1 when the raster ran, bit 2 when one of its
+ * cells took the offset sad_x4 branch with (omx, omy) != (0, 0), and the
+ * number of pattern searches in bits 8..15. */
+static int tz_prefix(pat_ctx_t *c, int *route)
+{
+    const int R = c->R;
+    int omx = c->bmx, omy = c->bmy, lx = c->bmx, ly = c->bmy;
+    int calls = 0;
+    *route = 0;
+    free_point(c, omx - 2, omy);
+    free_point(c, omx - 1, omy + 2);
+    free_point(c, omx + 1, omy + 2);
+    free_point(c, omx + 2, omy);
+    free_point(c, omx + 1, omy - 2);
+    free_point(c, omx - 1, omy - 2);
+    if (check_mv_range(c->bmx, c->bmy, c->mv_min, c->mv_max)) {
+        omx = c->bmx;                         /* DIA_ITER sets omx, omy (me.c:330-331) */
+        omy = c->bmy;
+        dia_iter(c, c->bmx, c->bmy);
+    }
+    accept_if_moved(c, &lx, &ly);
+    tz_info_t m = { c->bmx, c->bmy, 0, c->bcost, 0 };
+    tz_pattern(c, &m, 3, R);
+    calls++;
+    c->bcost = m.bcost;
+    c->bmx = m.bmx;
+    c->bmy = m.bmy;
+    int bdist = m.bdist, dir = m.bdir;
+    accept_if_moved(c, &lx, &ly);
+    if (bdist == 1) {
+        if (!dir) {
+            *route |= 1 | (calls << 8);
+            return 0;
+        }
+        tz_two_points(c, dir);
+        accept_if_moved(c, &lx, &ly);
+        if (c->bcost == m.bcost) {
+            *route |= 1 | (calls << 8);
+            return 0;
+        }
+    }
+    if (bdist > 16) {
+        const int rd = 16 >> 1, rd2 = 16 >> 2;
+        const int rymin = c->ymin > c->bmy - 16 + 2 ? c->ymin : c->bmy - 16 + 2;
+        const int rymax = c->ymax < c->bmy + 16 - 2 ? c->ymax : c->bmy + 16 - 2;
+        const int rxmin = c->xmin > c->bmx - 16 + 2 ? c->xmin : c->bmx - 16 + 2;
+        const int rxmax = c->xmax < c->bmx + 16 - 2 ? c->xmax : c->bmx + 16 - 2;
+        *route |= 2;
+        for (int j = rymin; j < rymax; j += rd) {
+            for (int i = rxmin; i < rxmax; i += rd) {
+                const int px[4] = { i, i, i + rd2, i + rd2 }, py[4] = { j, j + rd2, j, j + rd2 };
+                int all = 1;
+                for (int k = 0; k < 4; k++)
+                    all &= check_mv_range(px[k], py[k], c->mv_min, c->mv_max);
+                for (int k = 0; k < 4; k++) {
+                    if (all)
+                        tz_raster_point(c, omx, omy, px[k], py[k]);
+                    else
+                        ipel_point(c, px[k], py[k]);
+                }
+                if (all && (omx || omy))
+                    *route |= 4;
+            }
+        }
+        accept_if_moved(c, &lx, &ly);
+    }
+    while (bdist > 0) {
+        tz_info_t n = { c->bmx, c->bmy, 0, c->bcost, 0 };
+        tz_pattern(c, &n, 32, R);
+        calls++;
+        c->bcost = n.bcost;
+        c->bmx = n.bmx;
+        c->bmy = n.bmy;
+        bdist = n.bdist;
+        dir = n.bdir;
+        accept_if_moved(c, &lx, &ly);
+        if (bdist == 1) {
+            if (dir)
+                tz_two_points(c, dir);
+            accept_if_moved(c, &lx, &ly);
+            break;
+        }
+    }
+    *route |= calls << 8;
+    return 1;
+}
+
 /* The integer stage of XAVS2_ME_HEX (me.c:1107-1131, falling through to the
- * diamond), XAVS2_ME_DIA (me.c:1133-1154) and XAVS2_ME_UMH (me.c:1030-1099,
- * then the hexagon or the diamond), from the best MV candidate. */
+ * diamond), XAVS2_ME_DIA (me.c:1133-1154), XAVS2_ME_UMH (me.c:1030-1099,
+ * then the hexagon or the diamond) and XAVS2_ME_TZ (me.c:926-1028, then the
+ * hexagon, or nothing more), from the best MV candidate. */
 static int pattern_search(pat_ctx_t *c, int method, const xo_umh_ctx_t *u, int p0x, int p0y, int big_hex_level)
 {
     int omx, omy, route = 0;
     pat_accept(c);
     int hexagon = method == XO_ME_HEX;
+    if (method == XO_ME_TZ) {
+        if (!tz_prefix(c, &route))
+            return route;
+        if (!tz_hexagon_box_ok(c)) {
+            c->undefined = 1;
+            return route;
+        }
+        hexagon = 1;
+    }
     if (method == XO_ME_UMH) {
         static const xo_umh_ctx_t zero;
         int where = 0;
@@ -549,6 +873,14 @@ static void pat_init(pat_ctx_t *c, const xo_me_param_t *prm, const xo_pu_t *pu, 
     c->i_ref = i_ref;
     c->mv_min = win->mv_min;
     c->mv_max = win->mv_max;
+    c->xmin = win->mv_min_fpel[0];
+    c->ymin = win->mv_min_fpel[1];
+    c->xmax = win->mv_max_fpel[0];
+    c->ymax = win->mv_max_fpel[1];
+    c->pix_x = pu->pix_x;
+    c->pix_y = pu->pix_y;
+    c->fw = prm->frame_w;
+    c->fh = prm->frame_h;
 }
 
 int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
@@ -567,8 +899,51 @@ int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh
     c.max_pts = max_pts;
     const int r = pattern_search(&c, prm->me_method, ctx, p0x, p0y, prm->umh_big_hex_level);
     if (route)
-        *route = r;
+        *route = r | (c.undefined ? 8 : 0);
     return c.n_pts;
+}
+
+static int pattern_trace(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                         const uint8_t *cur, intptr_t i_cur, const uint8_t *plane0, intptr_t i_ref,
+                         const int32_t *costs, int span, int sx, int sy, int scost, int p0x, int p0y,
+                         int *trace, int max_trace, int *bx, int *by, int *bcost, int *route)
+{
+    mv_window_t win;
+    pat_ctx_t c;
+    pu_mv_window(prm, pu, &win);
+    pat_init(&c, prm, pu, cur, i_cur, plane0, i_ref, &win);
+    c.land = costs;
+    c.span = span;
+    c.trace = trace;
+    c.max_trace = max_trace;
+    c.bcost = scost;
+    c.bmx = sx;
+    c.bmy = sy;
+    const int r = pattern_search(&c, prm->me_method, ctx, p0x, p0y, prm->umh_big_hex_level);
+    *bx = c.bmx;
+    *by = c.bmy;
+    *bcost = c.bcost;
+    if (route)
+        *route = r | (c.undefined ? 8 : 0);
+    return c.n_trace;
+}
+
+int xo_pattern_landscape(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                         const int32_t *costs, int span, int sx, int sy, int scost, int p0x, int p0y,
+                         int *trace, int max_trace, int *bx, int *by, int *bcost, int *route)
+{
+    static const uint8_t none[1] = { 0 };
+    return pattern_trace(prm, pu, ctx, none, 0, none, 0, costs, span, sx, sy, scost, p0x, p0y, trace, max_trace,
+                         bx, by, bcost, route);
+}
+
+int xo_me_pattern_trace(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                        const uint8_t *cur, intptr_t i_cur, const uint8_t *plane0, intptr_t i_ref,
+                        int sx, int sy, int scost, int p0x, int p0y,
+                        int *trace, int max_trace, int *bx, int *by, int *bcost, int *route)
+{
+    return pattern_trace(prm, pu, ctx, cur, i_cur, plane0, i_ref, NULL, 0, sx, sy, scost, p0x, p0y, trace,
+                         max_trace, bx, by, bcost, route);
 }
 
 static int32_t me_search_impl(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
@@ -628,7 +1003,9 @@ static int32_t me_search_impl(const xo_me_param_t *prm, const xo_pu_t *pu, const
         return XO_MAX_DISTORTION;
     }
 
-    if (prm->me_method == XO_ME_DIA || prm->me_method == XO_ME_HEX || prm->me_method == XO_ME_UMH) {
+    int undefined = 0;
+    if (prm->me_method == XO_ME_DIA || prm->me_method == XO_ME_HEX || prm->me_method == XO_ME_UMH ||
+        prm->me_method == XO_ME_TZ) {
         pat_ctx_t c;
         pat_init(&c, prm, pu, cur, i_cur, planes[0], i_ref, &win);
         c.bcost = bcost;
@@ -638,6 +1015,7 @@ static int32_t me_search_impl(const xo_me_param_t *prm, const xo_pu_t *pu, const
         bcost = c.bcost;
         bmx = c.bmx;
         bmy = c.bmy;
+        undefined = c.undefined;
         goto store;
     }
 
@@ -665,7 +1043,7 @@ store:
     res->bmv2_y = (int16_t)bmy;
     res->bcost2 = bcost;
     res->mvcost = weighted_cost(lambda, xo_mvbits(R, 4 * bmx - pmx) + xo_mvbits(R, 4 * bmy - pmy));
-    res->status = 0;
+    res->status = undefined ? 2 : 0;
     res->cost   = bcost;
 
     if (!prm->use_fractional_me)

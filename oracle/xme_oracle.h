@@ -74,15 +74,16 @@ typedef struct {
     int use_fractional_me;            /* 0 none, 1 half-pel, 2 half + quarter */
     int use_fast_sub_me;              /* Spiral2 with step 2 */
     int enable_pmvr;                  /* h->param->enable_pmvr */
-    int me_method;                    /* h->param->me_method: XO_ME_FS, _DIA, _HEX or _UMH */
+    int me_method;                    /* h->param->me_method: XO_ME_FS, _DIA, _HEX, _UMH or _TZ */
     int umh_big_hex_level;            /* h->UMH_big_hex_level (presets.c:341): 0, 1 or 2 */
 } xo_me_param_t;
 
-/* enum me_mothod_e, common/common.h:145-151 (TZ is not restated). */
+/* enum me_mothod_e, common/common.h:145-151. */
 #define XO_ME_FS  0
 #define XO_ME_DIA 1
 #define XO_ME_HEX 2
 #define XO_ME_UMH 3
+#define XO_ME_TZ  4
 
 /* The per-(PU, reference) encoder state the UMH search reads from xavs2_me_t
  * (set by md_inter.c:640-718): predictor MVs in qpel, the reference index,
@@ -111,7 +112,9 @@ typedef struct {
     int16_t bmv_x, bmv_y;             /* p_me->bmv, qpel */
     int16_t bmv2_x, bmv2_y;           /* p_me->bmv2, full-pel */
     int32_t mvcost;                   /* p_me->mvcost[PDIR_FWD] */
-    int32_t status;                   /* 0 ok; 1 no legal MV candidate (_me_error) */
+    int32_t status;                   /* 0 ok; 1 no legal MV candidate (_me_error); 2 (TZ) a raster read
+                                       * outside the padded reference, undefined in the reference: the
+                                       * point is skipped and the result is not the reference's */
 } xo_me_result_t;
 
 /* Integer search (full search, or the DIA / HEX / UMH pattern search by
@@ -135,8 +138,8 @@ int32_t xo_me_search_ctx(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_u
                          const uint8_t *const planes[16], intptr_t i_ref,
                          xo_me_result_t *res);
 
-/* The integer stage of a pattern search (prm->me_method XO_ME_DIA, XO_ME_HEX
- * or XO_ME_UMH) from the full-pel start point (sx, sy) with cost scost, which
+/* The integer stage of a pattern search (prm->me_method XO_ME_DIA, XO_ME_HEX,
+ * XO_ME_UMH or XO_ME_TZ) from the full-pel start point (sx, sy) with cost scost, which
  * xo_me_search takes from the MV candidate list; (p0x, p0y) is the first
  * candidate (the clipped MVP), where UMH starts its first diamond.  Writes
  * the best point after each step in order to path[0..2*n) as (x, y) pairs
@@ -145,11 +148,31 @@ int32_t xo_me_search_ctx(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_u
  * stage the prefix ended in: 0 the big hexagon ran out, 1..4 an early
  * termination after the predictor points, the first cross centre, the cross,
  * the grid, or 5 one after a big-hexagon level; and 2 or 3 in bits 4..7 for a
- * jump to the hexagon or the diamond. */
+ * jump to the hexagon or the diamond.  For TZ: bit 0 when the search stopped
+ * before the hexagon, bit 1 when the raster step ran, bit 2 when a raster
+ * cell read at the offset position (omx, omy) != (0, 0), bit 3 when such a
+ * read left the padded plane (undefined), and the number of pattern searches
+ * in bits 8..15. */
 int xo_me_pattern_path(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
                        const uint8_t *cur, intptr_t i_cur,
                        const uint8_t *plane0, intptr_t i_ref,
                        int sx, int sy, int scost, int p0x, int p0y, int *path, int max_pts, int *route);
+
+/* Test hook for the pattern searches' control flow: the integer stage of
+ * prm->me_method from (sx, sy) with cost scost over a table of costs instead
+ * of pictures, costs[(y + span) * (2 span + 1) + (x + span)] for |x|, |y| <=
+ * span (points beyond cost 2^28).  The PU (position, size) and the picture
+ * size set the MV window.  Writes the evaluated points in order to trace (at
+ * most max_trace (x, y) pairs) and returns their count; *bx, *by and *bcost
+ * get the integer best, *route as for xo_me_pattern_path. */
+int xo_pattern_landscape(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                         const int32_t *costs, int span, int sx, int sy, int scost, int p0x, int p0y,
+                         int *trace, int max_trace, int *bx, int *by, int *bcost, int *route);
+/* The same on pictures (as xo_me_pattern_path): the evaluated points in order. */
+int xo_me_pattern_trace(const xo_me_param_t *prm, const xo_pu_t *pu, const xo_umh_ctx_t *ctx,
+                        const uint8_t *cur, intptr_t i_cur, const uint8_t *plane0, intptr_t i_ref,
+                        int sx, int sy, int scost, int p0x, int p0y,
+                        int *trace, int max_trace, int *bx, int *by, int *bcost, int *route);
 
 /* Symmetric bi-prediction refinement, xavs2_me_search_sym (me.c:1190-1262),
  * for frame (not field) sequences.  The second MV mirrors the first,

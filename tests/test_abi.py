@@ -71,6 +71,7 @@ LAYOUTS = {
     "xme_pu": "PU_DTYPE", "xme_result": "RESULT_DTYPE", "xme_umh_ctx": "UMH_CTX_DTYPE",
     "xme_sym_job": "SYM_JOB_DTYPE", "xme_sym_result": "SYM_RESULT_DTYPE", "xme_bid_job": "BID_JOB_DTYPE",
     "xme_mvp_job": "MVP_JOB_DTYPE", "xme_mvp_result": "MVP_RESULT_DTYPE",
+    "xme_umh_job": "UMH_JOB_DTYPE", "xme_umh_state": "UMH_STATE_DTYPE",
 }
 
 

@@ -1,7 +1,10 @@
-"""GPU parity of the DIA, HEX and UMH integer searches (followed by the sub-pel
-refinement) against the oracle: every PU shape, MV predictors, several
+"""GPU parity of the DIA, HEX, UMH and TZ integer searches (followed by the
+sub-pel refinement) against the oracle: every PU shape, MV predictors, several
 references, MV windows narrowed so that walks reach their edges, graph
-replays, and a 1080p frame of mixed shapes with a random MVP per PU."""
+replays, a 1080p frame of mixed shapes with a random MVP per PU, and for TZ
+motion up to 48 pixels (rings to 64, the raster step offset by the diamond's
+centre, the while loop) and picture-edge PUs whose raster reads leave the
+padded reference (status 2)."""
 import numpy as np
 import pytest
 
@@ -12,7 +15,7 @@ from xavs2_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-METHODS = [pytest.param(X.ME_DIA, id="dia"), pytest.param(X.ME_HEX, id="hex")]
+METHODS = [pytest.param(X.ME_DIA, id="dia"), pytest.param(X.ME_HEX, id="hex"), pytest.param(X.ME_TZ, id="tz")]
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -80,7 +83,7 @@ def test_cost_map_and_unknown_method_are_rejected():
     with pytest.raises(RuntimeError):
         plan.run(cur, [ref], cost_map=cm)
     with pytest.raises(RuntimeError):
-        X.Plan(X.me_params(64, 64, 8, LAM, me_method=4), X.make_pus([(16, 16, 16, 16, 0, 0)]), 1)
+        X.Plan(X.me_params(64, 64, 8, LAM, me_method=5), X.make_pus([(16, 16, 16, 16, 0, 0)]), 1)
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -129,7 +132,9 @@ def random_umh_ctx(n_pu, n_ref, seed, zero_frac=0.3):
     return c
 
 
-def run_umh_case(w, h, pus, R, n_ref, seed, level, prm_kw=None, ctx=None):
+def run_umh_case(w, h, pus, R, n_ref, seed, level, prm_kw=None, ctx=None, d_states=None):
+    """The UMH plan against the oracle, the plan's context set from ctx (host)
+    or, when given, from device-resident xme_umh_state records."""
     prm_kw = dict(prm_kw or {})
     refs = [synth.smooth_field(w, h, seed=seed * 10 + r) for r in range(n_ref)]
     cur = synth.multi_ref_current(refs, 16, min(R, 24), seed=seed + 99)
@@ -142,7 +147,10 @@ def run_umh_case(w, h, pus, R, n_ref, seed, level, prm_kw=None, ctx=None):
     ctx = random_umh_ctx(len(pus), n_ref, seed + 5) if ctx is None else ctx
     plan = X.Plan(X.me_params(w, h, R, LAM, me_method=X.ME_UMH, umh_big_hex_level=level, **prm_kw),
                   X.make_pus(pus), n_ref)
-    plan.set_umh_context(ctx)
+    if d_states is None:
+        plan.set_umh_context(ctx)
+    else:
+        plan.set_umh_states(d_states)
     plan.run(dcur, d_refs)
     got = plan.fetch()
     prm_o = O.me_param(w, h, R, LAM, me_method=O.ME_UMH, umh_big_hex_level=level, **prm_kw)
@@ -310,3 +318,83 @@ def test_umh_diamond_absolute_points_when_the_window_misses_an_offset():
     cur[32:48, 48:64] = blk
     assert umh_single(w, h, cur, ref, (48, 32, 16, 16, -40, -80), 8, mv_range_x=(-8192, 24),
                       mv_range_y=(-2048, 24)) == (-1, 0, 112)
+
+
+# ---------------------------------------------------------------------------
+# TZ: long walks, the offset raster step and reads outside the reference
+
+def tz_case(seed, pus, W=256, H=192, R=64, mv=48, n_ref=1):
+    """The TZ plan against the oracle on a smooth picture whose 16x16 tiles move by
+    up to `mv` pixels; returns the oracle's results."""
+    refs = [synth.smooth_field(W, H, seed=seed + 7 * r) for r in range(n_ref)]
+    cur, _ = synth.displaced(refs[0], 16, mv, seed=seed + 1)
+    d_refs = []
+    for r in refs:
+        d = X.DeviceRef(W, H, frac=2)
+        d.load(r)
+        d_refs.append(d)
+    dcur = device_cur(cur)
+    plan = X.Plan(X.me_params(W, H, R, LAM, me_method=X.ME_TZ), X.make_pus(pus), n_ref)
+    plan.run(dcur, d_refs)
+    got = plan.fetch()
+    planes = []
+    for r in refs:
+        p = O.Plane(W, H)
+        p.set_picture(r)
+        p.expand_border()
+        planes.append(O.interp_planes(p, 2))
+    exp = oracle_results(O.me_param(W, H, R, LAM, me_method=O.ME_TZ), pus, cur, planes, n_ref)
+    compare(got, exp, pus, n_ref)
+    return exp
+
+
+@pytest.mark.parametrize("seed", [0, 4])
+def test_tz_long_walks_raster_and_while_loop(seed):
+    """Motion up to 48 pixels over a mixed tiling, two references: about a quarter
+    of the PUs take the raster step (offset by the diamond's centre) and half run
+    the while loop's pattern searches (counted on the oracle's routes)."""
+    W, H = 256, 192
+    pus = [(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in synth.mixed_pus(W, H, seed)]
+    tz_case(seed, pus, n_ref=2)
+    # route coverage of the same inputs, reference 0
+    ref = synth.smooth_field(W, H, seed=seed)
+    cur, _ = synth.displaced(ref, 16, 48, seed=seed + 1)
+    rp = O.Plane(W, H)
+    rp.set_picture(ref)
+    rp.expand_border()
+    cp = O.Plane(W, H)
+    cp.set_picture(cur)
+    prm = O.me_param(W, H, 64, LAM, me_method=O.ME_TZ, use_fractional_me=0)
+    res = O.me_search_batch(prm, np.array([p for p in pus], dtype=O.PU_DTYPE), cp, [rp] + [None] * 15)
+    raster = loops = 0
+    pad = rp.view()
+    for i, pu in enumerate(pus):
+        x, y, pw, ph = pu[:4]
+        # the start: the only MV candidate (0,0), SAD + MV cost (mvbits(0) = 1 per component)
+        blk = pad[O.PAD + y:O.PAD + y + ph, O.PAD + x:O.PAD + x + pw].astype(int)
+        c0 = int(np.abs(blk - cur[y:y + ph, x:x + pw].astype(int)).sum()) + ((LAM * 2) >> 16)
+        _, _, _, route = O.pattern_trace(prm, pu, cp, rp, (0, 0), c0)
+        raster += (route & 6) == 6
+        loops += (route >> 8) >= 2
+    assert raster >= len(pus) // 8 and loops >= len(pus) // 3, (raster, loops, len(pus))
+    assert (res["status"] == 0).all()
+
+
+def test_tz_reads_outside_the_reference_are_status_2():
+    """PUs along the picture's edges with MVPs of 30-57 pixels pointing out of it:
+    their diamond centres lie far out, and some offset raster reads leave the
+    padded reference; GPU and oracle skip the same points and both give status 2
+    there, equal results elsewhere."""
+    W, H = 256, 192
+    n2 = 0
+    for seed in (1, 4):
+        rng = np.random.default_rng(seed)
+        pus = []
+        for y in range(0, H - 16, 16):
+            pus.append((0, y, 16, 16, int(rng.integers(-230, -120)), int(rng.integers(-60, 60))))
+            pus.append((W - 16, y, 16, 16, int(rng.integers(120, 230)), int(rng.integers(-60, 60))))
+        for x in range(0, W - 16, 16):
+            pus.append((x, 0, 16, 16, int(rng.integers(-60, 60)), int(rng.integers(-230, -120))))
+        exp = tz_case(seed, pus)
+        n2 += int((exp["status"] == 2).sum())
+    assert n2 >= 2

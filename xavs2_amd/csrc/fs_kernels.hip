@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) mvc_kernel(DevParams p, const FrameDesc *
     }
     d.n_mvc = (int8_t)n;
     d.status = key == XME_KEY_NONE ? 1 : 0;
-    d.pad = 0;
+    d.flags = 0;
     inst[id] = d;
     best[id] = key;
 }
@@ -695,9 +695,12 @@ __device__ __forceinline__ void fs_item_narrow(const DevParams &p, const DevPu &
 }
 
 // The other PUs of at most 256 pixels (4, 12, 16 or 32 wide; 8 x 32) use the
-// same 16 candidate rows per lane.  Their SAD alone may fill the u16 lanes
-// (256 * 255 < 2^16), so the MV costs (packed u16, cached in the wave's LDS
-// term area like the narrow path's) are added in 32 bits in the epilogue.
+// same 16 candidate rows per lane, with the packed u16 MV costs cached in the
+// wave's LDS term area like the narrow path's.  PUs of at most 128 pixels
+// (16 x 8, 16 x 4, 4 x 16, 4 x 8, 4 x 4) start their qsad accumulators at the
+// MV cost, as the narrow path does (128 * 255 + MV cost < 2^16); the SAD of
+// the larger ones alone may fill the u16 lanes (256 * 255 < 2^16), so their
+// MV costs are added in 32 bits in the epilogue.
 template <int H, int NQ>
 __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu, const DevInst &in,
                                                const FsItem &it, int inst_id, const uint8_t *__restrict__ win,
@@ -707,6 +710,8 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
                                                TermState &ts)
 {
     constexpr int K = XME_NARROW_K;
+    constexpr bool PRE = H * 4 * NQ <= 128;    // the MV cost rides in the u16 accumulators
+    constexpr int KS = PRE ? 16 : 6;           // the lane key's index bits
     const int lane = threadIdx.x & 63;
     const int LC = it.LC;
     const int lc = lane % LC, rg = lane / LC;
@@ -724,8 +729,9 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     uint64_t a16[K], mvc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        a16[k] = 0;
-        mvc[k] = init[k * 64 + lane];   // read now: the loads land during the strip
+        // read now: the loads land during the strip
+        a16[k] = PRE ? init[k * 64 + lane] : 0;
+        mvc[k] = PRE ? 0 : init[k * 64 + lane];
     }
     const uint8_t *cblk = cur + (pu.y - cy0) * cpitch + (pu.x - cx0);
     if ((lane_base & 3) == 0)      // uniform: col and the pitch are multiples of 4
@@ -739,9 +745,22 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     const bool full = it.ncols == 4 * LC && it.nrows == (64 / LC) * K;
     const bool fast = full && bmx0 >= mw.min_fpel[0] && bmx0 + it.ncols - 1 <= mw.max_fpel[0] &&
                       bmy0 >= mw.min_fpel[1] && bmy0 + it.nrows - 1 <= mw.max_fpel[1];
-    // cost = SAD + MV cost < 2^17: the lane key (cost << 6 | k * 4 + d) fits
+    // cost = SAD + MV cost < 2^17: the lane key (cost << 6 | k * 4 + d) fits;
+    // with the cost in the u16 lanes, (cost << 16 | k * 4 + d) is one shift-or
+    // or and-or per candidate, as in the narrow path
     uint32_t m = 0xFFFFFFFFu;
-    if (fast) {
+    if (fast && PRE) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t lo = (uint32_t)a16[k], hi = (uint32_t)(a16[k] >> 32);
+            const uint32_t k0 = (lo << 16) | (uint32_t)(k * 4 + 0);
+            const uint32_t k1 = (lo & 0xFFFF0000u) | (uint32_t)(k * 4 + 1);
+            const uint32_t k2 = (hi << 16) | (uint32_t)(k * 4 + 2);
+            const uint32_t k3 = (hi & 0xFFFF0000u) | (uint32_t)(k * 4 + 3);
+            m = min(m, min(k0, k1));
+            m = min(m, min(k2, k3));
+        }
+    } else if (fast) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint64_t a = a16[k], mv = mvc[k];
@@ -766,7 +785,7 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
                 const bool legal = i < col_end && j < row_end && xme_mv_legal(mw, in.omx + i - R, my);
                 const uint32_t v = ((uint32_t)(a >> (16 * d)) & 0xFFFFu) + ((uint32_t)(mv >> (16 * d)) & 0xFFFFu);
                 if (legal)
-                    m = min(m, (v << 6) | (uint32_t)(k * 4 + d));
+                    m = min(m, (v << KS) | (uint32_t)(k * 4 + d));
             }
         }
     }
@@ -774,7 +793,7 @@ __device__ __forceinline__ void fs_item_wide(const DevParams &p, const DevPu &pu
     if (m != 0xFFFFFFFFu) {
         const int idx = (int)(m & 63u);
         const int lrow = row - it.row0 + (idx >> 2), lcol = col - it.col0 + (idx & 3);
-        lkey = ((m >> 6) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
+        lkey = ((m >> KS) << 12) | (uint32_t)(lrow * 4 * LC + lcol);
     }
     lkey = wave_min_u32(lkey);
     if (lkey != 0xFFFFFFFFu && lane == 0) {

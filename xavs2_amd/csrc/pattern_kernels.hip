@@ -335,57 +335,342 @@ __device__ __forceinline__ void umh_done_stage(UmhState &u, int bmx, int bmy)
     u.bi = 0;
 }
 
-// The UMH steps of every live group of the wave, up to the phase each one
-// goes on with (kHexFirst or kDiaFirst).  Each lane of a group takes one point
-// of the stage per round and sums the whole PU there, with the current PU
-// staged in the group's LDS share (`cur`, gw dwords a row, read by every lane
-// of the group at once), so a round evaluates S points.  A round's points are
-// taken as the reference takes them one by one: the first strict minimum,
-// i.e. the smallest (cost, point) key, if its cost is below the running best
-// (COPY3_IF_LT in order, ME_COST_IPEL, me.c:187-193).  Every point but a
-// diamond's is range checked first; a diamond's unchecked points lie one
-// pixel from its centre, which is a window point or the best of an earlier
-// diamond, so at most 3 pixels outside the window.  The window keeps 6 pixels
-// inside the 64-pixel search margin (md_inter.c:913), so the reads (straight
-// from the plane) stay inside its 80-pixel padding.
+// TZ (me.c:926-1028): the stages before umh_step_2.  Evaluating stages hold
+// their points; the decisions between them are tz_next's.
+enum {
+    kT6 = 0,         // ME_COST_IPEL_X3 twice around the best candidate (unchecked)
+    kTDia,           // DIA_ITER around the best, if it lies in the window
+    kTRound,         // one round of tz_pattern_search
+    kTTwo,           // the two points next to a best at distance 1
+    kTRaster,        // the raster refinement
+    kTOff            // left for the hexagon, or done
+};
+
+struct TzState {
+    int st, bi;                 // stage, batch of points within it
+    int cx, cy;                 // centre of the stage's points
+    int ox, oy;                 // omx, omy of xavs2_me_search: the raster's offset
+    int dist, rounds, early;    // the round's distance, rounds without a gain, EarlyExitIters or MaxIters
+    int r0;                     // the best cost when the round started
+    int bdist, bdir;            // mv_info's bdist and bdir
+    int call_cost;              // mvinfo.bcost after the first pattern search
+    int rx, ry, rnx, rny;       // raster: first cell, cells a row and rows
+    bool first;                 // the two points follow the first pattern search
+    bool dfree;                 // the window holds the offsets (0,-1), (-1,0), (1,0), (0,1)
+    bool undef;                 // a raster read left the reference's arrays
+};
+
+// The two ME_COST_IPEL_X3 of the TZ start (me.c:937-938) in order, and the
+// Two Point Search offsets (me.c:144-153), 16 entries.
+constexpr uint32_t kTz6X = pack8(-2, -1, 1, 2, 1, -1, 0, 0, 2);
+constexpr uint32_t kTz6Y = pack8(0, 2, 2, 0, -2, -2, 0, 0, 2);
+constexpr int kTzOffXV[16] = { -1, 0, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0, -1, 1, 1, 0 };
+constexpr int kTzOffYV[16] = { 0, -1, -1, -1, 0, 0, 1, -1, -1, 1, 0, 1, 1, 1, 0, 1 };
+constexpr uint64_t kTzOffX = pack16(kTzOffXV), kTzOffY = pack16(kTzOffYV);
+// The rounds at distance d <= 8 (me.c:700-719, :745-782), in evaluation
+// order: (0,-d) (-h,-h) (h,-h) (-d,0) (d,0) (-h,h) (h,h) (0,d) with h = d/2,
+// as multiples of h; the round at distance 1 takes entries 0, 3, 4 and 7.
+// Their direction codes, and whether an entry lies at distance h.
+constexpr uint32_t kTz8X = pack8(0, -1, 1, -2, 2, -1, 1, 0, 2);
+constexpr uint32_t kTz8Y = pack8(-2, -1, -1, 0, 0, 1, 1, 2, 2);
+constexpr uint32_t kTz8D = pack8(2, 1, 3, 4, 5, 6, 8, 7, 0);
+
+// Point t of the current TZ stage: 0 none, 1 taken only inside the MV window
+// (ME_COST_IPEL, ME_COST_IPEL_DIR_DIST), 2 taken without a range check, 3 a
+// raster point at (omx + i, omy + j) (the sad_x4 branch of ME_COST_IPEL_X4,
+// which offsets the absolute MV (i, j) by pix_base, me.c:245), taken
+// unchecked if the reference's arrays hold it.
+__device__ __forceinline__ int tz_stage_points(const TzState &z)
+{
+    return z.st == kT6 ? 6 : z.st == kTDia ? 4 : z.st == kTTwo ? 2 : z.st == kTRaster ? 4 * z.rnx * z.rny
+         : z.dist == 1 ? 4 : z.dist <= 8 ? 8 : 16;
+}
+
+__device__ __forceinline__ int tz_point(const TzState &z, const MvWindow &win, int t, int &mx, int &my)
+{
+    mx = my = 0;
+    if (t >= tz_stage_points(z))
+        return 0;
+    switch (z.st) {
+    case kT6:
+        mx = z.cx + tab(kTz6X, t, 2);
+        my = z.cy + tab(kTz6Y, t, 2);
+        return 2;
+    case kTDia:
+        mx = (z.dfree ? z.cx : 0) + tab(kDia4X, t, 2);
+        my = (z.dfree ? z.cy : 0) + tab(kDia4Y, t, 2);
+        return z.dfree ? 2 : 1;
+    case kTRound: {
+        const int d = z.dist;
+        if (d <= 8) {
+            // at distance 1 the entries 0, 3, 4, 7 are (0, -+2 h) and (-+2 h, 0) with h = 1/2
+            const int e = d == 1 ? (t == 0 ? 0 : t == 1 ? 3 : t == 2 ? 4 : 7) : t;
+            mx = z.cx + (d == 1 ? tab(kTz8X, e, 2) / 2 : tab(kTz8X, e, 2) * (d >> 1));
+            my = z.cy + (d == 1 ? tab(kTz8Y, e, 2) / 2 : tab(kTz8Y, e, 2) * (d >> 1));
+            return 1;
+        }
+        if (t < 4) {
+            mx = z.cx + (t == 1 ? -d : t == 2 ? d : 0);
+            my = z.cy + (t == 0 ? -d : t == 3 ? d : 0);
+        } else {
+            const int idx = ((t - 4) >> 2) + 1, k = (t - 4) & 3, q = (d >> 2) * idx;
+            mx = z.cx + ((k & 1) ? q : -q);
+            my = z.cy + ((k & 2) ? d - q : q - d);
+        }
+        return 1;
+    }
+    case kTTwo:
+        mx = z.cx + tab4(kTzOffX, (z.bdir - 1) * 2 + t);
+        my = z.cy + tab4(kTzOffY, (z.bdir - 1) * 2 + t);
+        return 1;
+    default: {   // kTRaster: cells of 4 points (i, j) (i, j + 4) (i + 4, j) (i + 4, j + 4)
+        const int c = t >> 2, k = t & 3, cy = c / z.rnx;
+        const int i = z.rx + 8 * (c - cy * z.rnx), j = z.ry + 8 * cy;
+        const bool all = xme_mv_legal(win, i, j) && xme_mv_legal(win, i, j + 4) && xme_mv_legal(win, i + 4, j) &&
+                         xme_mv_legal(win, i + 4, j + 4);
+        mx = i + ((k & 2) ? 4 : 0);
+        my = j + ((k & 1) ? 4 : 0);
+        if (!all)
+            return 1;
+        mx += z.ox;
+        my += z.oy;
+        return 3;
+    }
+    }
+}
+
+// A pattern search call (me.c:946-948, :1001): mv_info from the best so far.
+__device__ __forceinline__ void tz_call(TzState &z, int bcost, int bmx, int early)
+{
+    z.st = kTRound;
+    z.dist = 1;
+    z.rounds = 0;
+    z.early = early;
+    z.r0 = bcost;
+    z.bdist = z.bdir = 0;
+    z.cx = bmx;     // tz_pattern_search's omy is mv->bmx (me.c:689-690)
+    z.cy = bmx;
+}
+
+// After the first pattern search and its two-point search: the raster step
+// for a best beyond RasterDistance (me.c:980-995), then the while loop's
+// first test (me.c:997).
+__device__ __forceinline__ void tz_after_first(TzState &z, const MvWindow &win, int bcost, int bmx, int bmy,
+                                               int &phase)
+{
+    if (z.bdist > 16) {
+        const int y0 = max(win.min_fpel[1], bmy - 14), y1 = min(win.max_fpel[1], bmy + 14);
+        const int x0 = max(win.min_fpel[0], bmx - 14), x1 = min(win.max_fpel[0], bmx + 14);
+        z.rny = y1 > y0 ? (y1 - y0 + 7) >> 3 : 0;
+        z.rnx = x1 > x0 ? (x1 - x0 + 7) >> 3 : 0;
+        if (z.rnx > 0 && z.rny > 0) {
+            z.st = kTRaster;
+            z.rx = x0;
+            z.ry = y0;
+            return;
+        }
+    }
+    if (z.bdist > 0) {
+        tz_call(z, bcost, bmx, 32);
+        return;
+    }
+    z.st = kTOff;
+    phase = kHexFirst;
+}
+
+// The decision after the last batch of a TZ stage.
+__device__ __forceinline__ void tz_next(TzState &z, const DevParams &p, const MvWindow &win, int bcost, int bmx,
+                                        int bmy, int &phase)
+{
+    z.bi = 0;
+    switch (z.st) {
+    case kT6:
+        if (xme_mv_legal(win, bmx, bmy)) {
+            z.st = kTDia;     // DIA_ITER sets omx, omy (me.c:330-331)
+            z.cx = z.ox = bmx;
+            z.cy = z.oy = bmy;
+            return;
+        }
+        tz_call(z, bcost, bmx, 3);
+        return;
+    case kTDia:
+        tz_call(z, bcost, bmx, 3);
+        return;
+    case kTRound: {
+        bool end = false;
+        if (bcost < z.r0)
+            z.rounds = 0;
+        else if (++z.rounds >= z.early)
+            end = true;
+        const int nd = z.dist == 1 ? 2 : 2 * z.dist;   // 1, 2, 4, 8, then 16 .. merange
+        if (!end && (nd <= 8 || nd <= p.R)) {
+            z.dist = nd;
+            z.r0 = bcost;
+            z.cx = bmx;
+            z.cy = bmx;
+            return;
+        }
+        if (z.early == 3) {      // the first pattern search (me.c:949-977)
+            if (z.bdist == 1) {
+                if (!z.bdir) {
+                    z.st = kTOff;
+                    phase = kDone;
+                    return;
+                }
+                z.first = true;
+                z.call_cost = bcost;
+                z.st = kTTwo;
+                z.cx = bmx;
+                z.cy = bmy;
+                return;
+            }
+            tz_after_first(z, win, bcost, bmx, bmy, phase);
+            return;
+        }
+        // a pattern search of the while loop (me.c:1002-1024)
+        if (z.bdist == 1 && z.bdir) {
+            z.first = false;
+            z.st = kTTwo;
+            z.cx = bmx;
+            z.cy = bmy;
+            return;
+        }
+        if (z.bdist > 1) {
+            tz_call(z, bcost, bmx, 32);
+            return;
+        }
+        z.st = kTOff;
+        phase = kHexFirst;
+        return;
+    }
+    case kTTwo:
+        if (z.first) {
+            if (bcost == z.call_cost) {   // no new point: the search stops (me.c:974-976)
+                z.st = kTOff;
+                phase = kDone;
+                return;
+            }
+            tz_after_first(z, win, bcost, bmx, bmy, phase);
+            return;
+        }
+        z.st = kTOff;
+        phase = kHexFirst;
+        return;
+    default:   // kTRaster: the while loop runs (bdist > 16)
+        tz_call(z, bcost, bmx, 32);
+        return;
+    }
+}
+
+// The prefix stages as the lane-parallel loop below sees them: UMH's
+// (umh_point, umh_done_stage, umh_next) and TZ's.
+struct UmhMachine {
+    UmhState u;
+    xme_umh_ctx c;
+    __device__ bool live() const { return u.st != kUOff; }
+    __device__ int batch() const { return u.bi; }
+    __device__ int points(const DevParams &p) const { return umh_stage_points(u.st, p.R, p.big_hex); }
+    __device__ int point(const DevParams &p, const MvWindow &, int t, int &mx, int &my) const
+    {
+        return umh_point(u, c, p.R, p.big_hex, t, mx, my);
+    }
+    __device__ void won(int) {}
+    __device__ void undefined() {}
+    __device__ void next_batch(const DevParams &p, const MvWindow &, int S, int bcost, int bmx, int bmy,
+                               int &phase)
+    {
+        u.bi++;
+        if (u.bi * S >= points(p)) {
+            umh_done_stage(u, bmx, bmy);
+            umh_next(u, c, p.R, p.big_hex, bcost, bmx, bmy, phase);
+        }
+    }
+};
+
+struct TzMachine {
+    TzState z;
+    __device__ bool live() const { return z.st != kTOff; }
+    __device__ int batch() const { return z.bi; }
+    __device__ int points(const DevParams &) const { return tz_stage_points(z); }
+    __device__ int point(const DevParams &, const MvWindow &win, int t, int &mx, int &my) const
+    {
+        return tz_point(z, win, t, mx, my);
+    }
+    // mv_info's bdir / bdist follow a round's winning point (COPY5_IF_LT)
+    __device__ void won(int t)
+    {
+        if (z.st != kTRound)
+            return;
+        const int d = z.dist;
+        if (d <= 8) {
+            const int e = d == 1 ? (t == 0 ? 0 : t == 1 ? 3 : t == 2 ? 4 : 7) : t;
+            z.bdir = tab(kTz8D, e, 0);
+            z.bdist = (e == 1 || e == 2 || e == 5 || e == 6) ? d >> 1 : d;
+        } else {
+            z.bdir = 0;
+            z.bdist = d;
+        }
+    }
+    __device__ void undefined() { z.undef = true; }
+    __device__ void next_batch(const DevParams &p, const MvWindow &win, int S, int bcost, int bmx, int bmy,
+                               int &phase)
+    {
+        z.bi++;
+        if (z.bi * S >= points(p))
+            tz_next(z, p, win, bcost, bmx, bmy, phase);
+    }
+};
+
+// The prefix steps (UMH's or TZ's) of every live group of the wave, up to the
+// phase each one goes on with (kHexFirst, kDiaFirst, or kDone).  Each lane of
+// a group takes one point of the stage per round and sums the whole PU there,
+// with the current PU staged in the group's LDS share (`cur`, gw dwords a
+// row, read by every lane of the group at once), so a round evaluates S
+// points.  A round's points are taken as the reference takes them one by one:
+// the first strict minimum, i.e. the smallest (cost, point) key, if its cost
+// is below the running best (COPY3_IF_LT in order, ME_COST_IPEL, me.c:187-193).
+// A point is range checked first unless it is one of the unchecked ones:
+// UMH's and TZ's diamond points lie one pixel from their centre, a window
+// point or the best of an earlier diamond, and TZ's six start points two
+// pixels from a window point, so at most 3 pixels outside the window.  The
+// window keeps 6 pixels inside the 64-pixel search margin (md_inter.c:913),
+// so those reads (straight from the plane) stay inside its 80-pixel padding.
+// TZ's offset raster points are tested against the padding and the mvbits
+// table and skipped outside them (the reference reads outside its arrays).
 //
 // LP > 1 (64-lane groups): LP lanes share a point, lane j of them summing
 // quads j, j + LP, ... of every row, so that a row of a point is one
 // coalesced read; a round then evaluates S / LP points.
-template <int S, int GWMAX, int LP>
-__device__ __forceinline__ void umh_prefix_lanes(const DevParams &p, const DevPu &pu, const xme_umh_ctx &c,
-                                                 const MvWindow &win, gbytes ref, int64_t i_ref,
-                                                 const uint32_t *cur, int p0x, int p0y, int &phase, int &bcost,
-                                                 int &bmx, int &bmy, uint32_t &evals)
+template <int S, int GWMAX, int LP, class M>
+__device__ __forceinline__ void prefix_lanes(const DevParams &p, const DevPu &pu, M &m, const MvWindow &win,
+                                             gbytes ref, int64_t i_ref, const uint32_t *cur, int &phase, int &bcost,
+                                             int &bmx, int &bmy, uint32_t &evals)
 {
     const int lane = threadIdx.x & 63;
     const int l = lane & (S - 1);
     const int gw = pu.w >> 2;
-    UmhState u;
-    u.st = phase == kDone ? kUOff : kUDia0;
-    u.bi = 0;
-    u.cx = p0x;
-    u.cy = p0y;
-    u.pmx = p0x;
-    u.pmy = p0y;
-    u.level = 0;
-    u.p0x = p0x;
-    u.p0y = p0y;
-    u.dfree = xme_mv_legal(win, 0, -1) && xme_mv_legal(win, -1, 0) && xme_mv_legal(win, 1, 0) &&
-              xme_mv_legal(win, 0, 1);
-    if (u.st != kUOff)
-        phase = kDone;   // set again when the prefix ends
+    const uint64_t gmask = (S == 64 ? ~0ull : ((1ull << S) - 1)) << (lane & ~(S - 1));
     for (;;) {
-        if (__builtin_amdgcn_ballot_w64(u.st != kUOff) == 0)
+        if (__builtin_amdgcn_ballot_w64(m.live()) == 0)
             break;
-        const bool live = u.st != kUOff;
-        const int n_st = live ? umh_stage_points(u.st, p.R, p.big_hex) : 0;
+        const bool live = m.live();
+        const int n_st = live ? m.points(p) : 0;
         constexpr int PR = S / LP;             // points per round
         const int pi = l / LP, j = l % LP;     // point of the round, lane within it
-        const int t = u.bi * PR + pi;
+        const int t = m.batch() * PR + pi;
         int mx = 0, my = 0;
-        const int kind = live && t < n_st ? umh_point(u, c, p.R, p.big_hex, t, mx, my) : 0;
-        const bool ok = kind == 2 || (kind == 1 && xme_mv_legal(win, mx, my));
+        const int kind = live && t < n_st ? m.point(p, win, t, mx, my) : 0;
+        bool inside = true;
+        if (kind == 3) {
+            const int ax = pu.x + mx, ay = pu.y + my;
+            // a row's dword reads reach 4 bytes past the block
+            inside = ax >= -XME_PAD && ax + pu.w + 4 <= p.frame_w + XME_PAD && ay >= -XME_PAD &&
+                     ay + pu.h <= p.frame_h + XME_PAD && abs(4 * mx - pu.mvp_x) <= p.max_mvd &&
+                     abs(4 * my - pu.mvp_y) <= p.max_mvd;
+        }
+        if (__builtin_amdgcn_ballot_w64(kind == 3 && !inside) & gmask)
+            m.undefined();
+        const bool ok = kind == 2 || (kind == 3 && inside) || (kind == 1 && xme_mv_legal(win, mx, my));
         int key = 0x7fffffff;
         if constexpr (LP > 1) {
             uint32_t sad = 0;
@@ -439,22 +724,21 @@ __device__ __forceinline__ void umh_prefix_lanes(const DevParams &p, const DevPu
         evals += (uint32_t)__builtin_amdgcn_ds_bpermute((lane | (S - 1)) << 2, nok);
         if (live && kmin != 0x7fffffff && (kmin >> 6) < bcost) {
             bcost = kmin >> 6;
+            const int tw = m.batch() * PR + (kmin & 63);
             int wx = 0, wy = 0;
-            umh_point(u, c, p.R, p.big_hex, u.bi * PR + (kmin & 63), wx, wy);
+            m.point(p, win, tw, wx, wy);
             bmx = wx;
             bmy = wy;
+            m.won(tw);
         }
-        if (live) {
-            u.bi++;
-            if (u.bi * PR >= n_st) {
-                umh_done_stage(u, bmx, bmy);
-                umh_next(u, c, p.R, p.big_hex, bcost, bmx, bmy, phase);
-            }
-        }
+        if (live)
+            m.next_batch(p, win, PR, bcost, bmx, bmy, phase);
     }
 }
 
-template <int S, int QMAX, int NB, bool SEQ, bool UMH>
+// MODE: 0 DIA / HEX, 1 UMH (its prefix, then the hexagon or the diamond),
+// 2 TZ (its prefix, then the hexagon and the diamond, or done).
+template <int S, int QMAX, int NB, bool SEQ, int MODE>
 __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc *__restrict__ desc, const DevPu *__restrict__ pus,
                              DevInst *__restrict__ inst, uint64_t *__restrict__ best, int id,
                              uint32_t *__restrict__ points, uint32_t *__restrict__ box,
@@ -517,10 +801,8 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
 
     int idx = 0, hex_i = 0, dia_i = 0;
     uint32_t evals = 0;
-    if constexpr (UMH) {
-        xme_umh_ctx c = {};
-        if (phase != kDone)
-            c = umh[id];
+    bool undef = false;
+    if constexpr (MODE != 0) {
         // the group's current quads, row-major, in its LDS share (free until
         // the first step box; at most 1024 dwords for 64x64 in 1224)
 #pragma unroll
@@ -531,9 +813,48 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
         // 64-lane groups (PUs of 192 quads and more): 16 lanes a point.  For
         // the smaller PUs a lane a point measured faster: sharing a point
         // cuts the points of a round, and rounds carry the stage logic
-        umh_prefix_lanes<S, (S <= 8 ? 4 : S <= 32 ? 8 : 16), (S == 64 ? 16 : 1)>(
-            p, pu, c, win, plane + (int64_t)pu.y * i_ref + pu.x, i_ref, gbox, in.mvc[0][0], in.mvc[0][1], phase,
-            bcost, bmx, bmy, evals);
+        constexpr int GW = S <= 8 ? 4 : S <= 32 ? 8 : 16, LP = S == 64 ? 16 : 1;
+        const gbytes ref0 = plane + (int64_t)pu.y * i_ref + pu.x;
+        const bool dfree = xme_mv_legal(win, 0, -1) && xme_mv_legal(win, -1, 0) && xme_mv_legal(win, 1, 0) &&
+                           xme_mv_legal(win, 0, 1);
+        if constexpr (MODE == 1) {
+            UmhMachine m;
+            m.c = {};
+            if (phase != kDone)
+                m.c = umh[id];
+            UmhState &u = m.u;
+            u.st = phase == kDone ? kUOff : kUDia0;
+            u.bi = 0;
+            u.cx = u.pmx = u.p0x = in.mvc[0][0];
+            u.cy = u.pmy = u.p0y = in.mvc[0][1];
+            u.level = 0;
+            u.dfree = dfree;
+            if (u.st != kUOff)
+                phase = kDone;   // set again when the prefix ends
+            prefix_lanes<S, GW, LP>(p, pu, m, win, ref0, i_ref, gbox, phase, bcost, bmx, bmy, evals);
+        } else {
+            TzMachine m;
+            TzState &z = m.z;
+            z = {};
+            z.st = phase == kDone ? kTOff : kT6;
+            z.cx = z.ox = bmx;   // omx, omy = the best candidate (me.c:935-936)
+            z.cy = z.oy = bmy;
+            z.dfree = dfree;
+            if (z.st != kTOff)
+                phase = kDone;   // set again when the prefix ends
+            prefix_lanes<S, GW, LP>(p, pu, m, win, ref0, i_ref, gbox, phase, bcost, bmx, bmy, evals);
+            undef = z.undef;
+            // an offset raster point may leave the best anywhere in the padded
+            // plane: the hexagon's box (the block grown by 2 pixels, whole
+            // dwords) must lie in it, or the reference reads outside it
+            const int x0 = (pu.x + bmx - 2) & ~3;
+            if (phase == kHexFirst &&
+                !(x0 >= -XME_PAD && x0 + 4 * DW <= p.frame_w + XME_PAD && pu.y + bmy - 2 >= -XME_PAD &&
+                  pu.y + bmy + pu.h + 2 <= p.frame_h + XME_PAD)) {
+                undef = true;
+                phase = kDone;
+            }
+        }
         __builtin_amdgcn_wave_barrier();
     }
     for (;;) {
@@ -674,6 +995,8 @@ __device__ __forceinline__ void pattern_wave(const DevParams &p, const FrameDesc
             // candidate (0, 0): order XME_ORDER_RASTER + R * 2R + R
             inst[id].omx = (int16_t)bmx;
             inst[id].omy = (int16_t)bmy;
+            if (undef)
+                inst[id].flags = 1;
             best[id] = xme_key((uint32_t)bcost, (uint32_t)(XME_ORDER_RASTER + p.R * 2 * p.R + p.R));
         }
         points[id] = evals;
@@ -696,7 +1019,7 @@ struct PatLaunch {
 //  2: <= 64, S 16, 9 (8x32: 144);  1: <= 32, S 8, 10 (8x16: 80);
 //  0: <= 16, S 4, 15 (4x16: 60).  Every lane holds at most 4 quads of the
 // PU below class 5.
-template <bool UMH>
+template <int MODE>
 __global__ void __launch_bounds__(256) pattern_kernel(DevParams p, const FrameDesc *__restrict__ desc,
                                                       const DevPu *__restrict__ pus, DevInst *__restrict__ inst,
                                                       uint64_t *__restrict__ best, const int32_t *__restrict__ list,
@@ -722,20 +1045,20 @@ __global__ void __launch_bounds__(256) pattern_kernel(DevParams p, const FrameDe
     const int lane = threadIdx.x & 63;
     uint32_t *box = boxes[threadIdx.x >> 6];
     if (w < L.end[0]) {
-        pattern_wave<64, 16, 20, true, UMH>(p, desc, pus, inst, best, list[L.first[0] + w], points, box, umh);
+        pattern_wave<64, 16, 20, true, MODE>(p, desc, pus, inst, best, list[L.first[0] + w], points, box, umh);
     } else if (w < L.end[1]) {
-        pattern_wave<64, 4, 7, false, UMH>(p, desc, pus, inst, best, list[L.first[1] + (w - L.end[0])], points, box, umh);
+        pattern_wave<64, 4, 7, false, MODE>(p, desc, pus, inst, best, list[L.first[1] + (w - L.end[0])], points, box, umh);
     } else if (w < L.end[2]) {
-        pattern_wave<32, 4, 7, false, UMH>(p, desc, pus, inst, best, list[L.first[2] + (w - L.end[1]) * 2 + lane / 32],
+        pattern_wave<32, 4, 7, false, MODE>(p, desc, pus, inst, best, list[L.first[2] + (w - L.end[1]) * 2 + lane / 32],
                                       points, box, umh);
     } else if (w < L.end[3]) {
-        pattern_wave<16, 4, 9, false, UMH>(p, desc, pus, inst, best, list[L.first[3] + (w - L.end[2]) * 4 + lane / 16],
+        pattern_wave<16, 4, 9, false, MODE>(p, desc, pus, inst, best, list[L.first[3] + (w - L.end[2]) * 4 + lane / 16],
                                       points, box, umh);
     } else if (w < L.end[4]) {
-        pattern_wave<8, 4, 10, false, UMH>(p, desc, pus, inst, best, list[L.first[4] + (w - L.end[3]) * 8 + lane / 8],
+        pattern_wave<8, 4, 10, false, MODE>(p, desc, pus, inst, best, list[L.first[4] + (w - L.end[3]) * 8 + lane / 8],
                                       points, box, umh);
     } else if (w < L.end[5]) {
-        pattern_wave<4, 4, 15, false, UMH>(p, desc, pus, inst, best, list[L.first[5] + (w - L.end[4]) * 16 + lane / 4],
+        pattern_wave<4, 4, 15, false, MODE>(p, desc, pus, inst, best, list[L.first[5] + (w - L.end[4]) * 16 + lane / 4],
                                       points, box, umh);
     }
 }
@@ -766,10 +1089,13 @@ int xme_launch_pattern(const DevParams &p, const FrameDesc *d_desc, const DevPu 
         prev = wave_end[k];
     }
     if (p.method == XME_ME_UMH)
-        hipLaunchKernelGGL(pattern_kernel<true>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+        hipLaunchKernelGGL(pattern_kernel<1>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+                           d_best, d_list, L, d_points, d_umh);
+    else if (p.method == XME_ME_TZ)
+        hipLaunchKernelGGL(pattern_kernel<2>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
                            d_best, d_list, L, d_points, d_umh);
     else
-        hipLaunchKernelGGL(pattern_kernel<false>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
+        hipLaunchKernelGGL(pattern_kernel<0>, dim3((waves + 3) / 4), dim3(256), 0, s, p, d_desc, d_pus, d_inst,
                            d_best, d_list, L, d_points, d_umh);
     XME_HIP(hipGetLastError());
     return XME_OK;

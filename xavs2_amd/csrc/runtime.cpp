@@ -524,10 +524,11 @@ static int plan_create_impl(const xme_me_params *prm, const xme_pu *pus_in, int 
         xme_set_error("xme_plan_create: search range must be 1..128 and the frame 8..8192 pixels");
         return XME_ERR_ARG;
     }
-    if (prm->me_method < XME_ME_FS || prm->me_method > XME_ME_UMH || prm->umh_big_hex_level < 0 ||
+    if (prm->me_method < XME_ME_FS || prm->me_method > XME_ME_TZ || prm->umh_big_hex_level < 0 ||
         prm->umh_big_hex_level > 2) {
-        xme_set_error("xme_plan_create: me_method %d / umh_big_hex_level %d: full search (0), DIA (1), HEX (2) "
-                      "and UMH (3, big-hexagon level 0..2) are provided", prm->me_method, prm->umh_big_hex_level);
+        xme_set_error("xme_plan_create: me_method %d / umh_big_hex_level %d: full search (0), DIA (1), HEX (2), "
+                      "UMH (3, big-hexagon level 0..2) and TZ (4) are provided", prm->me_method,
+                      prm->umh_big_hex_level);
         return XME_ERR_ARG;
     }
     DevParams p;
@@ -549,6 +550,7 @@ static int plan_create_impl(const xme_me_params *prm, const xme_pu *pus_in, int 
     p.n_ref = n_ref;
 
     const int max_mvd = xme_mvbits_max_mvd(p.R);
+    p.max_mvd = max_mvd;
     std::vector<DevPu> pus(n_pu);
     std::vector<int> parts(n_pu);
     for (int i = 0; i < n_pu; i++) {
@@ -926,6 +928,18 @@ extern "C" int xme_plan_set_umh_context(xme_plan *pl, const xme_umh_ctx *ctx, vo
     }
     XME_HIP(hipMemcpyAsync(pl->d_umh, ctx, sizeof(xme_umh_ctx) * pl->n_inst, hipMemcpyHostToDevice,
                            (hipStream_t)stream));
+    pl->umh_set = true;
+    return XME_OK;
+}
+
+extern "C" int xme_plan_set_umh_states(xme_plan *pl, const xme_umh_state *d_states, void *stream)
+{
+    if (!pl || !d_states || pl->p.method != XME_ME_UMH) {
+        xme_set_error("xme_plan_set_umh_states: needs a UMH plan and a state array");
+        return XME_ERR_ARG;
+    }
+    XME_HIP(hipMemcpy2DAsync(pl->d_umh, sizeof(xme_umh_ctx), &d_states->ctx, sizeof(xme_umh_state),
+                             sizeof(xme_umh_ctx), (size_t)pl->n_inst, hipMemcpyDeviceToDevice, (hipStream_t)stream));
     pl->umh_set = true;
     return XME_OK;
 }

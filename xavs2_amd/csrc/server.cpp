@@ -411,7 +411,8 @@ extern "C" int xme_server_create(const xme_server_config *cfg, xme_server **out)
         cfg->search_range < 1 || cfg->search_range > 128 || cfg->n_ref < 1 || cfg->n_ref > XME_MAX_REFS ||
         cfg->lambda_factor < 0 || cfg->use_fractional_me < 0 || cfg->use_fractional_me > 2 ||
         cfg->workgroup_share_pct < 0 || cfg->workgroup_share_pct > 100 ||
-        (cfg->me_method != XME_ME_FS && cfg->me_method != XME_ME_DIA && cfg->me_method != XME_ME_HEX)) {
+        (cfg->me_method != XME_ME_FS && cfg->me_method != XME_ME_DIA && cfg->me_method != XME_ME_HEX &&
+         cfg->me_method != XME_ME_TZ)) {
         xme_set_error("xme_server_create: bad configuration");
         return XME_ERR_ARG;
     }

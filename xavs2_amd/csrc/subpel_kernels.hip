@@ -264,7 +264,7 @@ __device__ void refine(const DevParams &p, const FrameDesc *__restrict__ desc, c
     res.bcost2 = bcost_int;
     res.bmv2_x = (int16_t)bmx;
     res.bmv2_y = (int16_t)bmy;
-    res.status = 0;
+    res.status = (in.flags & 1) ? 2 : 0;
     int bx = 4 * bmx, by = 4 * bmy, bcost = bcost_int;
     if (p.frac) {
         RefineCtx c;

@@ -26,8 +26,9 @@ struct DevParams {
     int32_t min_mv[2], max_mv[2];
     int32_t hadamard, frac, fast, pmvr;
     int32_t n_pu, n_ref;
-    int32_t method;            // XME_ME_FS, _DIA, _HEX or _UMH
+    int32_t method;            // XME_ME_FS, _DIA, _HEX, _UMH or _TZ
     int32_t big_hex;           // UMH big-hexagon level (0, 1, 2)
+    int32_t max_mvd;           // largest |MV difference| of the reference's mvbits table
 };
 
 // Per-frame pointers, read by the kernels from device memory so that a captured
@@ -46,7 +47,7 @@ struct DevInst {
     int16_t omx, omy;          // full-search centre (best MV candidate)
     int16_t mvc[2][2];         // candidate list (md_inter.c:957-959)
     int8_t n_mvc, status;      // status 1: no legal candidate (me.c:919-921)
-    int16_t pad;
+    int16_t flags;             // bit 0: a TZ raster read left the padded plane (undefined in the reference)
 };
 
 static_assert(sizeof(DevPu) == 16 && sizeof(DevInst) == 16, "the search kernel moves these as uint4");

@@ -54,7 +54,7 @@ class MeParams(C.Structure):
 
 
 # integer search methods (XME_ME_*, h->param->me_method)
-ME_FS, ME_DIA, ME_HEX, ME_UMH = 0, 1, 2, 3
+ME_FS, ME_DIA, ME_HEX, ME_UMH, ME_TZ = 0, 1, 2, 3, 4
 
 # xme_umh_ctx: per-(PU, reference) encoder state of the UMH search
 UMH_CTX_DTYPE = np.dtype([("mvp1_x", "<i4"), ("mvp1_y", "<i4"), ("mvp2_x", "<i4"), ("mvp2_y", "<i4"),
@@ -80,6 +80,13 @@ MVP_JOB_DTYPE = np.dtype([("ref_idx", "i1"), ("bwd_2nd", "i1"), ("ur_available",
                           ("cb_x", "i1"), ("cb_y", "i1"), ("cb_w", "u1"), ("cb_h", "u1"), ("ref", "i1", (4,)),
                           ("mv", "<i2", (4, 2))])
 MVP_RESULT_DTYPE = np.dtype([("mvp_x", "<i2"), ("mvp_y", "<i2"), ("status", "<i4")])
+# xme_umh_job / xme_umh_state: the UMH search's encoder-state batch
+UMH_JOB_DTYPE = np.dtype([("mvp", MVP_JOB_DTYPE), ("nb_sad", "<i4", (3,)), ("mode", "<i4"),
+                          ("mincost_up", "<i4", (3,)), ("best_up", "<i2", (3, 2)), ("mincost_prev", "<i4"),
+                          ("best_prev", "<i2", (2,)), ("col_mv", "<i2", (2,)), ("col_valid", "i1"),
+                          ("col_ref", "i1"), ("reserved", "<i2"), ("col_dpoc", "<i4")])
+UMH_STATE_DTYPE = np.dtype([("mvp_x", "<i2"), ("mvp_y", "<i2"), ("pred_sad_space", "<i4"), ("status", "<i4"),
+                            ("reserved", "<i4"), ("ctx", UMH_CTX_DTYPE)])
 SYM_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvcost", "<i4"),
                              ("status", "<i4")])
 
@@ -159,6 +166,9 @@ def lib():
         L.xme_dequant_batch.argtypes = [I, P, I, I, P]
         L.xme_residual_batch.argtypes = [I, I, I, P, P, P, P, I, P]
         L.xme_get_mvp_batch.argtypes = [P, I, I, P, P, P, P]
+        L.xme_umh_prepare_batch.argtypes = [P, I, I, P, P, P, P, P]
+        L.xme_umh_bsize.argtypes = [I, I, P]
+        L.xme_plan_set_umh_states.argtypes = [P, P, P]
         L.xme_inter_recon_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), I, I, P, I, P, P, P, P, Z, P, P, P]
         L.xme_inter_recon_bi_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), C.POINTER(RefDesc), I, I, P, P, I,
                                                P, P, P, P, Z, P, P, P]
@@ -379,6 +389,11 @@ class Plan:
         assert len(self._umh) == self.n
         check(lib().xme_plan_set_umh_context(self.h, ptr(self._umh), stream), "xme_plan_set_umh_context")
         check(lib().xme_stream_sync(stream), "sync")
+
+    def set_umh_states(self, d_states, stream=None):
+        """d_states: device address (int or c_void_p) of n_pu * n_ref xme_umh_state records."""
+        d = d_states if isinstance(d_states, C.c_void_p) else C.c_void_p(d_states)
+        check(lib().xme_plan_set_umh_states(self.h, d, stream), "xme_plan_set_umh_states")
 
     def _refs(self, refs):
         arr = (RefDesc * len(refs))()
