@@ -604,7 +604,8 @@ def serving_leg(X, seconds, loads, slots, n_cycle, share=0, split=0, seed=77):
     return {
         "workload": "16 concurrent encodes (6x1280x720, 6x1920x1080, 4x3840x2160), Poisson arrivals at 30-60 fps "
                     "per stream x load, 2-ref ME over mixed PUs, R=64, 64 GB budget, "
-                    f"{slots} device slots, search launches capped at {share or 100} % of the device and "
+                    f"{slots} device slots on {os.environ.get('GPU_MAX_HW_QUEUES', '4')} hardware queues, "
+                    f"search launches capped at {share or 100} % of the device and "
                     f"split at {'8e9' if split == 0 else ('none' if split < 0 else f'{split:.0e}')} absdiffs, "
                     f"{seconds:.0f} s per load",
         "stream_fps": [round(float(f), 2) for f in fps],
@@ -641,11 +642,14 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-seconds", type=float, default=2.5)
     ap.add_argument("--serve-loads", default="0.1,0.2,0.4,0.7,1.0")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's): the serving leg's "
+                         "slots are HIP streams, and streams beyond the hardware queues share them")
     ap.add_argument("--serve-frames", type=int, default=4, help="distinct frames per stream, cycled")
     ap.add_argument("--serve-split", type=float, default=0,
                     help="search work per launch in the serving leg, absdiffs (0: server default, <0: no split)")
-    ap.add_argument("--serve-share", type=int, default=75,
+    ap.add_argument("--serve-share", type=int, default=50,
                     help="%% of a full-device search launch per frame in the serving leg (0 = 100); a cap "
                          "leaves CUs for frames that arrive while a large one searches")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
@@ -659,12 +663,18 @@ def main():
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--refs", type=int, default=4)
     ap.add_argument("--range", type=int, default=64)
+    ap.add_argument("--pu-shape", default="", metavar="WxH",
+                    help="tile the picture with PUs of this one shape instead of the mixed tiling (profiling; "
+                         "the metric is quoted on the mixed tiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the ME path runs on a single MI355X; --gpus {args.gpus} is not "
               "supported (multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    # read by the HIP runtime when it starts (the first HIP call below)
+    if args.serve_hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.serve_hw_queues)
     from xavs2_amd import lib as X
 
     if args.copy_probe:
@@ -672,7 +682,7 @@ def main():
         return
 
     # the serving leg runs first: its device slots are the process's first HIP
-    # streams, so each gets a hardware queue of its own (GPU_MAX_HW_QUEUES = 4)
+    # streams, so each gets a hardware queue of its own (up to GPU_MAX_HW_QUEUES)
     serving = None
     if not args.no_serve:
         if not args.no_tune:
@@ -686,6 +696,10 @@ def main():
     lam = 4 * 65536
     log(f"generating {w}x{h} inputs, {n_ref} refs")
     refs, cur, pus = build_inputs(w, h, n_ref, seed=2026)
+    if args.pu_shape:
+        pw, ph = (int(v) for v in args.pu_shape.lower().split("x"))
+        pus = [(x, y, pw, ph) for y in range(0, h - ph + 1, ph) for x in range(0, w - pw + 1, pw)]
+        log(f"one-shape tiling: {len(pus)} PUs of {pw}x{ph}")
     stream = X.Stream()
     d_refs = []
     for r in refs:
@@ -793,9 +807,11 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {
-            "workload": f"batched multi-ref ME {w}x{h} 8-bit, {n_ref} refs, {len(pus)} mixed PUs "
-                        f"(8x8..64x64 incl. 64x16/16x64/32x8/8x32), full search [-{R},{R})^2 + half/quarter-pel "
-                        "SATD refine, one persistent launch replayed from a HIP graph",
+            "workload": f"batched multi-ref ME {w}x{h} 8-bit, {n_ref} refs, {len(pus)} "
+                        + (f"PUs of {args.pu_shape}" if args.pu_shape else
+                           "mixed PUs (8x8..64x64 incl. 64x16/16x64/32x8/8x32)")
+                        + f", full search [-{R},{R})^2 + half/quarter-pel SATD refine, one persistent launch "
+                        "replayed from a HIP graph",
             "pu_refs": int(n_inst),
             "integer_candidates_per_step": int(n_cand),
             "absdiffs_per_step": int(n_absdiff),
@@ -809,7 +825,8 @@ def main():
             "peak": VALU_PEAK_TABSDIFF,
             "unit": "Tabsdiff/s",
             "frac": achieved / VALU_PEAK_TABSDIFF,
-            "traffic": MEASURED_TRAFFIC.get((w, h, n_ref, R)) if not (args.no_tune or args.retune) else None,
+            "traffic": MEASURED_TRAFFIC.get((w, h, n_ref, R)) if not (args.no_tune or args.retune or args.pu_shape)
+            else None,
             "traffic_unit": "HBM bytes per search step (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "kernel": "integer full search: fs_kernel<true,8> (generic PUs) + fs_kernel<true,16> (PUs of <= 256 px), both persistent",
             "kernel_ms": search_ms,

@@ -26,11 +26,16 @@ typedef const __attribute__((address_space(1))) uint32_t *gdwords;
 //  M1MOD6 / M1MOD4 (me.c:96, :112),
 //  the first hexagon of umh_step_2 in evaluation order with its direction
 //  codes (me.c:1111-1112), the first diamond of umh_step_3 (me.c:1137).
+// An entry that does not fit its 3 bits sets bit 31, which the
+// static_asserts below the tables reject.
+constexpr bool fits3(int v) { return v >= 0 && v < 8; }
 constexpr uint32_t pack8(int a0, int a1, int a2, int a3, int a4, int a5, int a6, int a7, int bias)
 {
+    const bool ok = fits3(a0 + bias) && fits3(a1 + bias) && fits3(a2 + bias) && fits3(a3 + bias) &&
+                    fits3(a4 + bias) && fits3(a5 + bias) && fits3(a6 + bias) && fits3(a7 + bias);
     return (uint32_t)(a0 + bias) | (uint32_t)(a1 + bias) << 3 | (uint32_t)(a2 + bias) << 6 |
            (uint32_t)(a3 + bias) << 9 | (uint32_t)(a4 + bias) << 12 | (uint32_t)(a5 + bias) << 15 |
-           (uint32_t)(a6 + bias) << 18 | (uint32_t)(a7 + bias) << 21;
+           (uint32_t)(a6 + bias) << 18 | (uint32_t)(a7 + bias) << 21 | (ok ? 0u : 1u << 31);
 }
 constexpr uint32_t kHex2X = pack8(-1, -2, -1, 1, 2, 1, -1, -2, 2);
 constexpr uint32_t kHex2Y = pack8(-2, 0, 2, 2, 0, -2, -2, 0, 2);
@@ -44,6 +49,8 @@ constexpr uint32_t kHex6D = pack8(6, 5, 1, 4, 2, 3, 0, 0, 0);
 constexpr uint32_t kDia4X = pack8(0, -1, 1, 0, 0, 0, 0, 0, 2);
 constexpr uint32_t kDia4Y = pack8(-1, 0, 0, 1, 0, 0, 0, 0, 2);
 constexpr uint32_t kDia4D = pack8(4, 1, 3, 2, 0, 0, 0, 0, 0);
+static_assert(!((kHex2X | kHex2Y | kDia1X | kDia1Y | kM1Mod6 | kM1Mod4 | kHex6X | kHex6Y | kHex6D | kDia4X |
+                 kDia4Y | kDia4D) >> 31), "a 3-bit table entry out of range");
 
 __device__ __forceinline__ int tab(uint32_t t, int i, int bias) { return (int)((t >> (3 * i)) & 7u) - bias; }
 
@@ -57,6 +64,13 @@ constexpr uint64_t pack16(const int (&v)[16])
     for (int i = 0; i < 16; i++)
         r |= (uint64_t)(v[i] + 4) << (4 * i);
     return r;
+}
+constexpr bool fits16(const int (&v)[16])
+{
+    for (int i = 0; i < 16; i++)
+        if (v[i] + 4 < 0 || v[i] + 4 > 15)
+            return false;
+    return true;
 }
 constexpr int kHex4XV[16] = { 0, 0, -2, 2, -4, 4, -4, 4, -4, 4, -4, 4, -4, 4, -2, 2 };
 constexpr int kHex4YV[16] = { -4, 4, -3, -3, -2, -2, -1, -1, 0, 0, 1, 1, 2, 2, 3, 3 };
@@ -73,6 +87,8 @@ constexpr uint64_t kFHex4X = pack16(kFHex4XV), kFHex4Y = pack16(kFHex4YV);
 constexpr uint64_t kGridXa = pack16(kGridXA), kGridXb = pack16(kGridXB);
 constexpr uint64_t kGridYa = pack16(kGridYA), kGridYb = pack16(kGridYB);
 static_assert(kGridXV[10] == 0 && kGridYV[10] == 2 && kGridXV[21] == 0 && kGridYV[21] == 2, "grid entries 10, 21");
+static_assert(fits16(kHex4XV) && fits16(kHex4YV) && fits16(kFHex4XV) && fits16(kFHex4YV) && fits16(kGridXA) &&
+              fits16(kGridXB) && fits16(kGridYA) && fits16(kGridYB), "a 4-bit table entry out of range");
 
 __device__ __forceinline__ int tab4(uint64_t t, int i) { return (int)((t >> (4 * i)) & 15u) - 4; }
 
@@ -367,13 +383,16 @@ constexpr uint32_t kTz6Y = pack8(0, 2, 2, 0, -2, -2, 0, 0, 2);
 constexpr int kTzOffXV[16] = { -1, 0, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0, -1, 1, 1, 0 };
 constexpr int kTzOffYV[16] = { 0, -1, -1, -1, 0, 0, 1, -1, -1, 1, 0, 1, 1, 1, 0, 1 };
 constexpr uint64_t kTzOffX = pack16(kTzOffXV), kTzOffY = pack16(kTzOffYV);
+static_assert(!((kTz6X | kTz6Y) >> 31) && fits16(kTzOffXV) && fits16(kTzOffYV), "TZ table entry out of range");
 // The rounds at distance d <= 8 (me.c:700-719, :745-782), in evaluation
 // order: (0,-d) (-h,-h) (h,-h) (-d,0) (d,0) (-h,h) (h,h) (0,d) with h = d/2,
 // as multiples of h; the round at distance 1 takes entries 0, 3, 4 and 7.
-// Their direction codes, and whether an entry lies at distance h.
+// Their direction codes (1..8, 4-bit entries).
 constexpr uint32_t kTz8X = pack8(0, -1, 1, -2, 2, -1, 1, 0, 2);
 constexpr uint32_t kTz8Y = pack8(-2, -1, -1, 0, 0, 1, 1, 2, 2);
-constexpr uint32_t kTz8D = pack8(2, 1, 3, 4, 5, 6, 8, 7, 0);
+constexpr int kTz8DV[16] = { 2, 1, 3, 4, 5, 6, 8, 7, 0, 0, 0, 0, 0, 0, 0, 0 };
+constexpr uint64_t kTz8D = pack16(kTz8DV);
+static_assert(!((kTz8X | kTz8Y) >> 31) && fits16(kTz8DV), "TZ table entry out of range");
 
 // Point t of the current TZ stage: 0 none, 1 taken only inside the MV window
 // (ME_COST_IPEL, ME_COST_IPEL_DIR_DIST), 2 taken without a range check, 3 a
@@ -604,7 +623,7 @@ struct TzMachine {
         const int d = z.dist;
         if (d <= 8) {
             const int e = d == 1 ? (t == 0 ? 0 : t == 1 ? 3 : t == 2 ? 4 : 7) : t;
-            z.bdir = tab(kTz8D, e, 0);
+            z.bdir = tab4(kTz8D, e);
             z.bdist = (e == 1 || e == 2 || e == 5 || e == 6) ? d >> 1 : d;
         } else {
             z.bdir = 0;

@@ -202,7 +202,11 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
     for (int i = 0; i < 16; i++)
         acc_f2[i] = tc.rowsum_rho[lane][i] + (1 << (s2 - 1));
 
-    for (int64_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
+    // workgroup b runs on XCD b % 8: each XCD takes a contiguous run of the
+    // tiles (the grid is a multiple of 8), so that neighbouring TUs, which
+    // share current and reference lines, meet in one L2
+    const int lb = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    for (int64_t t = lb * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
         v4i hi, lo;
         // this lane's elements (rho(half, i), col): e_lane + rel_y(rho(0, i)),
         // in block b_lane + (rho(0, i) / N) * NB
@@ -507,7 +511,8 @@ int launch_fu(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t
     // one wave per 32x32 tile of (32/n)^2 blocks; waves loop over tiles
     const int bpt = (32 / n) * (32 / n);
     const int64_t tiles = ((int64_t)nblk + bpt - 1) / bpt;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 256 * 16));
+    // a multiple of 8 workgroups, one range of tiles per XCD
+    const int grid = (int)((std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 256 * 16)) + 7) / 8 * 8);
     switch (n) {
     case 4:
         hipLaunchKernelGGL((residual_kernel<4, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q, mode,
