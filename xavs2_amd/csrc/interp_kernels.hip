@@ -26,6 +26,8 @@ struct Planes16 {
     uint8_t *p[16];
 };
 
+typedef short short2v __attribute__((ext_vector_type(2)));
+
 constexpr int TX = 64;   // output columns per tile
 constexpr int TY = 16;   // output rows per tile
 constexpr int SPW = (TX + 8) / 4;   // staged dwords per row
@@ -61,17 +63,19 @@ __device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2
     col[3] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// Interpolated planes on the core area x in [-4, W+4), y in [-4, H+4).  The
+// Interpolated planes on the core area x in [-4, W+4), y in [-4, H+4), in
+// tiles whose columns start at xs <= -4 (a 64-byte boundary of the output
+// rows); columns left of -4 are written too and left to the border pass.  The
 // full-pel tile (with its 3 + 4 pixel filter margins) is staged in LDS as
 // signed bytes (pixel - 128); every 8-tap sum is two v_dot4_i32_i8 with an
 // exact 32-bit result, and the horizontal sums feeding the diagonal planes
 // are kept in LDS as int32.
-__global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t stride, int W, int H, int frac)
+__global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t stride, int W, int H, int frac,
+                                                          int xs)
 {
     __shared__ uint32_t sP[TY + 7][SPW];
     __shared__ int32_t sH[3][TY + 7][TX];
-    __shared__ __align__(4) uint8_t sO[9][TY][TX];
-    const int x0 = -4 + (int)blockIdx.x * TX, y0 = -4 + (int)blockIdx.y * TY;
+    const int x0 = xs + (int)blockIdx.x * TX, y0 = -4 + (int)blockIdx.y * TY;
     const uint8_t *P = pl.p[0];
     const int tid = threadIdx.x;
     // stage: column x0 - 4 + 4q .. +3 of rows y0 - 3 .. y0 + TY + 3 (dword aligned
@@ -148,9 +152,16 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
         }
     }
     // diagonal e..r from the horizontal sums (MC_SHIFT = 12): a thread takes one
-    // column and four rows so that every LDS access is a single dword
+    // column c (its lane) and four rows; the vertical 8-tap sums are four
+    // v_dot2_i32_i16 on int16 pairs of the horizontal sums, which lie in
+    // [-4845, 22440].  The lanes of a quad hold 4 adjacent columns: they
+    // transpose their 4 x 4 bytes with DPP broadcasts, and lane i of the quad
+    // stores row i of the 4 columns as one dword.
     {
-        const int c = tid & 63, g = tid >> 6;
+        const int c = tid & 63, g = tid >> 6, i4 = c & 3;
+        const int yy = y0 + 4 * g + i4, xx = x0 + (c & ~3);
+        const bool st = yy < H + 4 && xx < W + 4;
+        const uint32_t sel = (uint32_t)i4 | (uint32_t)(4 + i4) << 8 | 0x0C0C0000u;   // byte i4 of two dwords
         for (int dx = 1; dx <= 3; dx++) {
             if (frac < 2 && dx != 2)
                 continue;
@@ -158,62 +169,120 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
 #pragma unroll
             for (int r = 0; r < 11; r++)
                 hv[r] = sH[dx - 1][4 * g + r][c];
+            // E[m] = (hv[2m], hv[2m + 1]), O[m] = (hv[2m + 1], hv[2m + 2])
+            short2v E[5], O[5];
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                E[m] = short2v{ (short)hv[2 * m], (short)hv[2 * m + 1] };
+                O[m] = short2v{ (short)hv[2 * m + 1], (short)hv[2 * m + 2] };
+            }
             for (int dy = 1; dy <= 3; dy++) {
                 if (frac < 2 && dy != 2)
                     continue;
+                short2v T[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    T[m] = short2v{ (short)c_taps[dy][2 * m], (short)c_taps[dy][2 * m + 1] };
+                uint32_t px[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    int s = 0;
+                    // rows j .. j + 7: pairs from E for even j, from O for odd j
+                    int s = 2048;
 #pragma unroll
-                    for (int k = 0; k < 8; k++)
-                        s += (int)c_taps[dy][k] * hv[j + k];
-                    sO[(dy - 1) * 3 + (dx - 1)][4 * g + j][c] = (uint8_t)clip_pel((s + 2048) >> 12);
+                    for (int m = 0; m < 4; m++)
+                        s = __builtin_amdgcn_sdot2((j & 1) ? O[(j >> 1) + m] : E[(j >> 1) + m], T[m], s, false);
+                    px[j] = clip_pel(s >> 12);
                 }
+                // bytes packed with v_perm_b32: written as shifts and ors, the
+                // clips and packing become v_ashr_pk_u8_i32, which gave wrong
+                // pixels here
+                const uint32_t col = __builtin_amdgcn_perm(__builtin_amdgcn_perm(px[3], px[2], 0x0C0C0400u),
+                                                           __builtin_amdgcn_perm(px[1], px[0], 0x0C0C0400u),
+                                                           0x05040100u);
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0x00, 0xF, 0xF, false);
+                const uint32_t d1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0x55, 0xF, 0xF, false);
+                const uint32_t d2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0xAA, 0xF, 0xF, false);
+                const uint32_t d3 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0xFF, 0xF, 0xF, false);
+                const uint32_t lo = __builtin_amdgcn_perm(d1, d0, sel), hi = __builtin_amdgcn_perm(d3, d2, sel);
+                if (st)
+                    *reinterpret_cast<uint32_t *>(pl.p[(dy << 2) + dx] + (int64_t)yy * stride + xx) =
+                        __builtin_amdgcn_perm(hi, lo, 0x05040100u);
             }
         }
     }
-    __syncthreads();
-    for (int e = tid; e < 9 * TY * (TX / 4); e += 256) {
-        const int pk = e / (TY * (TX / 4)), rem = e - pk * (TY * (TX / 4));
-        const int r = rem / (TX / 4), q = rem - r * (TX / 4);
-        const int dy = pk / 3 + 1, dx = pk % 3 + 1;
-        if (frac < 2 && (dx != 2 || dy != 2))
-            continue;
-        const int yy = y0 + r, xx = x0 + 4 * q;
-        if (yy >= H + 4 || xx >= W + 4)
-            continue;
-        const uint32_t v = *reinterpret_cast<const uint32_t *>(&sO[pk][r][4 * q]);
-        *reinterpret_cast<uint32_t *>(pl.p[(dy << 2) + dx] + (int64_t)yy * stride + xx) = v;
-    }
 }
 
-// Fill [-pad, W+pad) x [-pad, H+pad) outside the core rectangle with the
-// nearest core value.  One thread per dword; core edges are dword aligned.
-__global__ void __launch_bounds__(256) border_kernel(Planes16 pl, int first, int64_t stride, int W, int H, int pad,
-                                                     int cx0, int cy0, int cx1, int cy1)
+// Fill [-pad, W+pad) x [-pad, H+pad) outside the core rectangle [cx0, cx1) x
+// [cy0, cy1) with the nearest core value.  The grid covers only the border's
+// dwords (core edges and pad are dword aligned), in four regions laid end to
+// end: the bands above and below the core (full width), then the strips left
+// and right of it (core rows).  blockIdx.y selects the plane.
+struct BorderGeom {
+    int64_t stride;
+    int pad, W, H, cx0, cy0, cx1, cy1;
+    int wdw;                   // dwords of a full row
+    int ldw, rdw;              // dwords of the left and right strips
+    int64_t n_top, n_bot, n_side;
+};
+
+__global__ void __launch_bounds__(256) border_kernel(Planes16 pl, int first, BorderGeom g)
 {
-    uint8_t *base = pl.p[first + blockIdx.z];
+    uint8_t *base = pl.p[first + blockIdx.y];
     if (base == nullptr)
         return;
-    const int wdw = (W + 2 * pad) >> 2;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = -pad + (int)blockIdx.y;
-    if (q >= wdw)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int x, y;
+    if (t < g.n_top) {
+        y = -g.pad + (int)(t / g.wdw);
+        x = -g.pad + 4 * (int)(t % g.wdw);
+    } else if ((t -= g.n_top) < g.n_bot) {
+        y = g.cy1 + (int)(t / g.wdw);
+        x = -g.pad + 4 * (int)(t % g.wdw);
+    } else if ((t -= g.n_bot) < g.n_side) {
+        const int sdw = g.ldw + g.rdw, r = (int)(t / sdw), q = (int)(t % sdw);
+        y = g.cy0 + r;
+        x = q < g.ldw ? -g.pad + 4 * q : g.cx1 + 4 * (q - g.ldw);
+    } else {
         return;
-    const int x = -pad + 4 * q;
-    const bool row_in = y >= cy0 && y < cy1;
-    if (row_in && x >= cx0 && x < cx1)
-        return;
-    const int sy = xme_clip3(cy0, cy1 - 1, y);
-    const uint8_t *src = base + (int64_t)sy * stride;
+    }
+    const int sy = xme_clip3(g.cy0, g.cy1 - 1, y);
+    const uint8_t *src = base + (int64_t)sy * g.stride;
     uint32_t v;
-    if (x >= cx0 && x < cx1) {
+    if (x >= g.cx0 && x < g.cx1) {
         v = *reinterpret_cast<const uint32_t *>(src + x);
     } else {
-        const int sx = x < cx0 ? cx0 : cx1 - 1;
+        const int sx = x < g.cx0 ? g.cx0 : g.cx1 - 1;
         v = (uint32_t)src[sx] * 0x01010101u;
     }
-    *reinterpret_cast<uint32_t *>(base + (int64_t)y * stride + x) = v;
+    *reinterpret_cast<uint32_t *>(base + (int64_t)y * g.stride + x) = v;
+}
+
+// The border of `planes` planes from pl.p[first]; every edge and the pad are
+// multiples of 4.
+int launch_border(const Planes16 &pl, int first, int planes, int64_t stride, int W, int H, int pad, int cx0,
+                  int cy0, int cx1, int cy1, hipStream_t s)
+{
+    BorderGeom g;
+    g.stride = stride;
+    g.pad = pad;
+    g.W = W;
+    g.H = H;
+    g.cx0 = cx0;
+    g.cy0 = cy0;
+    g.cx1 = cx1;
+    g.cy1 = cy1;
+    g.wdw = (W + 2 * pad) >> 2;
+    g.ldw = (cx0 + pad) >> 2;
+    g.rdw = (W + pad - cx1) >> 2;
+    g.n_top = (int64_t)(cy0 + pad) * g.wdw;
+    g.n_bot = (int64_t)(H + pad - cy1) * g.wdw;
+    g.n_side = (int64_t)(cy1 - cy0) * (g.ldw + g.rdw);
+    const int64_t n = g.n_top + g.n_bot + g.n_side;
+    if (n <= 0)
+        return XME_OK;
+    hipLaunchKernelGGL(border_kernel, dim3((unsigned)((n + 255) / 256), planes), dim3(256), 0, s, pl, first, g);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
 }
 
 } // namespace
@@ -229,11 +298,7 @@ extern "C" int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, 
         return XME_OK;
     Planes16 pl = {};
     pl.p[0] = d_plane;
-    dim3 grid((((w + 2 * pad) >> 2) + 255) / 256, h + 2 * pad, 1);
-    hipLaunchKernelGGL(border_kernel, grid, dim3(256), 0, (hipStream_t)stream, pl, 0, (int64_t)stride, w, h, pad, 0,
-                       0, w, h);
-    XME_HIP(hipGetLastError());
-    return XME_OK;
+    return launch_border(pl, 0, 1, (int64_t)stride, w, h, pad, 0, 0, w, h, (hipStream_t)stream);
 }
 
 extern "C" int xme_interpolate_frame(pel_t *const d_planes[16], intptr_t stride, int w, int h, int frac, void *stream)
@@ -256,12 +321,13 @@ extern "C" int xme_interpolate_frame(pel_t *const d_planes[16], intptr_t stride,
         }
     }
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid((w + 8 + TX - 1) / TX, (h + 8 + TY - 1) / TY);
-    hipLaunchKernelGGL(interp_core_kernel, grid, dim3(256), 0, s, pl, (int64_t)stride, w, h, frac);
+    // tile columns start where the output planes' rows reach a 64-byte
+    // boundary, at or up to 60 columns left of the core's first column -4, so
+    // that every 64-column row segment is one whole 64-byte line; the columns
+    // left of -4 are overwritten by the border pass
+    const int xs = -4 - (int)((reinterpret_cast<uintptr_t>(pl.p[2]) - 4) & 63);
+    dim3 grid((w + 4 - xs + TX - 1) / TX, (h + 8 + TY - 1) / TY);
+    hipLaunchKernelGGL(interp_core_kernel, grid, dim3(256), 0, s, pl, (int64_t)stride, w, h, frac, xs);
     XME_HIP(hipGetLastError());
-    dim3 bgrid((((w + 2 * XME_PAD) >> 2) + 255) / 256, h + 2 * XME_PAD, 15);
-    hipLaunchKernelGGL(border_kernel, bgrid, dim3(256), 0, s, pl, 1, (int64_t)stride, w, h, XME_PAD, -4, -4, w + 4,
-                       h + 4);
-    XME_HIP(hipGetLastError());
-    return XME_OK;
+    return launch_border(pl, 1, 15, (int64_t)stride, w, h, XME_PAD, -4, -4, w + 4, h + 4, s);
 }
