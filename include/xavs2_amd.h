@@ -395,6 +395,83 @@ int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_job *d_jobs,
                             intptr_t i_cur, const xme_ref_desc *ref1, const xme_ref_desc *ref2,
                             xme_sym_result *d_results, void *stream);
 
+/* The callers of the two refinements for a batch of PUs of one frame, frame
+ * sequences, every reference row reconstructed (so check_mv_range's
+ * num_lcu_coded_in_row test holds, source/common/predict.h:86-95), one wave
+ * per PU: pred_inter_search_bi (source/encoder/md_inter.c:1036-1128, B
+ * slices) and pred_inter_search_dual (md_inter.c:1133-1219, F slices).  Each
+ * PU goes through the reference's gates: check_mvd (md_inter.c:68-76: +-4096,
+ * vertically +-(4 << lcu_level) when frame_threads > 1), check_mv_range's box
+ * (predict.h:68-78: the PU may reach 1 << lcu_level pixels past the picture,
+ * inside min/max_mv_range) and check_mv_range_sym (md_inter.c:83-110: the MV
+ * and its mirror scaled to the second reference, negated in B slices).  The
+ * searches they let through are those of xme_me_search_sym_batch and
+ * xme_me_search_bid_batch; then the stored MVs pass the reference's own
+ * checks (md_inter.c:1107-1122, :1206-1217).  A gate that fails leaves its
+ * search out: XME_MAX_DISTORTION and the MV as it came in.  lcu_level is
+ * h->i_lcu_level (3..6); frame_threads is h->param->i_frame_threads.
+ * Distances are calculate_distance's (nonzero, within +-255); strides of the
+ * references must match; d_cur is 4-byte aligned with i_cur % 4 == 0.
+ *
+ * Status 2 marks a PU that is not evaluated: a PU shape the kernels do not
+ * take (xme_sym_job's), a refinement that reported status 2, or a gate whose
+ * scale_mv_skip product leaves int (undefined in the reference). */
+typedef struct {
+    int16_t x, y;                    /* PU position (luma pixels, x % 4 == 0) */
+    uint8_t w, h;                    /* PU size, multiples of 4 up to 64 */
+    uint8_t reserved[2];
+    int16_t fwd_imv_x, fwd_imv_y;    /* p_me->all_best_imv[B_FWD] (full-pel; x 4 in int16 as mv_t) */
+    int16_t bwd_x, bwd_y;            /* all_single_mv[B_BWD] (qpel) */
+    int16_t mvp_x, mvp_y;            /* p_me->mvp on entry: the symmetric search's MV rate */
+    int16_t mvp1_x, mvp1_y;          /* all_mvp[B_FWD] */
+    int16_t mvp2_x, mvp2_y;          /* all_mvp[B_BWD] */
+} xme_bi_job;
+
+typedef struct {
+    int32_t sym_cost;                /* *sym_mcost */
+    int16_t sym_mv_x, sym_mv_y;      /* all_sym_mv[0] */
+    int32_t sym_mvcost;              /* p_me->bmvcost[PDIR_SYM]; -1 when the search did not run or set it */
+    int32_t bid_cost;                /* *bid_mcost */
+    int16_t fwd_x, fwd_y;            /* all_dual_mv_1st[0] */
+    int16_t bwd_x, bwd_y;            /* all_dual_mv_2nd[0] */
+    int32_t bid_mvcost;              /* p_me->bmvcost[PDIR_BID]; -1 likewise */
+    int32_t status;                  /* 0 ok, 2 not evaluated */
+} xme_bi_result;
+
+int xme_pred_inter_search_bi_batch(const xme_me_params *prm, int lcu_level, int frame_threads,
+                                   const xme_bi_job *d_jobs, int n, int dist_fwd, int dist_bwd,
+                                   const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref_fwd,
+                                   const xme_ref_desc *ref_bwd, xme_bi_result *d_results, void *stream);
+
+/* pred_inter_search_dual over references 0 .. n_ref - 1 (h->i_ref, 2..4):
+ * reference r is searched with the second reference !r, distances dist[r]
+ * and dist[!r] (h->fdec->ref_dpoc), and costs REF_COST(r) = (lambda *
+ * REF_BITS[r + 1]) >> 16 on top (predict.h:47-52); the first strict minimum
+ * wins.  refs holds n_ref descriptors. */
+typedef struct {
+    int16_t x, y;                    /* PU position (luma pixels, x % 4 == 0) */
+    uint8_t w, h;                    /* PU size, multiples of 4 up to 64 */
+    uint8_t reserved[2];
+    int16_t mvp_x, mvp_y;            /* p_me->mvp on entry */
+    int16_t mvp1[4][2];              /* all_mvp[ref_idx] */
+    int16_t imv[4][2];               /* p_me->all_best_imv[ref_idx] (full-pel) */
+} xme_dual_job;
+
+typedef struct {
+    int32_t cost;                    /* *dual_mcost (XME_MAX_DISTORTION: none passed) */
+    int8_t fst_ref, snd_ref;         /* *dual_best_fst_ref / _snd_ref (-1: none) */
+    int16_t reserved;
+    int32_t mvcost;                  /* p_me->bmvcost[PDIR_DUAL] (-1: not set) */
+    int16_t fst[4][2];               /* all_dual_mv_1st[ref_idx] */
+    int16_t snd[4][2];               /* all_dual_mv_2nd[ref_idx] */
+    int32_t status;                  /* 0 ok, 2 not evaluated */
+} xme_dual_result;
+
+int xme_pred_inter_search_dual_batch(const xme_me_params *prm, int lcu_level, int frame_threads,
+                                     const xme_dual_job *d_jobs, int n, int n_ref, const int dist[4],
+                                     const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
+                                     xme_dual_result *d_results, void *stream);
+
 /* MV predictors, get_mvp_default (source/encoder/md_inter.c:470-524), for a
  * batch of PUs of one slice, frame sequences.  Per PU: the searched
  * reference ref_idx and list bwd_2nd, the PU inside its CU (p_cb), and the

@@ -49,6 +49,30 @@ class BidJob(C.Structure):
                                        "mvp1_x", "mvp1_y", "mvp2_x", "mvp2_y")]
 
 
+class GateParam(C.Structure):
+    _fields_ = [("lcu_level", C.c_int), ("frame_threads", C.c_int)]
+
+
+class BiJob(C.Structure):
+    _fields_ = [(f, C.c_int) for f in ("pix_x", "pix_y", "w", "h", "fwd_imv_x", "fwd_imv_y", "bwd_x", "bwd_y",
+                                       "mvp_x", "mvp_y", "mvp1_x", "mvp1_y", "mvp2_x", "mvp2_y")]
+
+
+class BiResult(C.Structure):
+    _fields_ = [(f, C.c_int32) for f in ("sym_cost", "sym_mv_x", "sym_mv_y", "sym_mvcost", "bid_cost", "fwd_x",
+                                         "fwd_y", "bwd_x", "bwd_y", "bid_mvcost")]
+
+
+class DualJob(C.Structure):
+    _fields_ = [("pix_x", C.c_int), ("pix_y", C.c_int), ("w", C.c_int), ("h", C.c_int), ("mvp_x", C.c_int),
+                ("mvp_y", C.c_int), ("mvp1", (C.c_int * 2) * 4), ("imv", (C.c_int * 2) * 4)]
+
+
+class DualResult(C.Structure):
+    _fields_ = [("cost", C.c_int32), ("fst_ref", C.c_int32), ("snd_ref", C.c_int32), ("mvcost", C.c_int32),
+                ("fst", (C.c_int32 * 2) * 4), ("snd", (C.c_int32 * 2) * 4)]
+
+
 class MvpJob(C.Structure):
     _fields_ = [("ref_idx", C.c_int), ("bwd_2nd", C.c_int), ("cb_x", C.c_int), ("cb_y", C.c_int),
                 ("cb_w", C.c_int), ("cb_h", C.c_int), ("ur_available", C.c_int), ("ref", C.c_int * 4),
@@ -121,6 +145,10 @@ def lib():
         L.xo_me_pattern_trace.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, C.c_int, C.c_int,
                                           C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P]
         L.xo_scale_mv_skip.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.xo_pred_inter_search_bi.argtypes = [C.POINTER(MeParam), C.POINTER(GateParam), C.POINTER(BiJob), C.c_int,
+                                              C.c_int, P, ip, P, P, ip, C.POINTER(BiResult)]
+        L.xo_pred_inter_search_dual.argtypes = [C.POINTER(MeParam), C.POINTER(GateParam), C.POINTER(DualJob),
+                                                C.c_int, P, P, ip, P, ip, C.POINTER(DualResult)]
         L.xo_me_search_sym.restype = C.c_int32
         L.xo_me_search_sym.argtypes = [C.POINTER(MeParam), C.POINTER(SymJob), P, ip, P, P, ip, C.POINTER(SymResult)]
         L.xo_me_search_bid.restype = C.c_int32
@@ -346,6 +374,41 @@ def me_search_bid(prm, job, cur_plane, planes1, planes2):
     lib().xo_me_search_bid(C.byref(prm), C.byref(j), cur_plane.origin_ptr(), cur_plane.stride,
                            plane_ptrs(planes1), plane_ptrs(planes2), planes1[0].stride, C.byref(r))
     return r.cost, r.mv_x, r.mv_y, r.mvcost
+
+
+def pred_inter_search_bi(prm, job, dist_fwd, dist_bwd, cur_plane, planes_fwd, planes_bwd, lcu_level=6,
+                         frame_threads=1):
+    """pred_inter_search_bi for one PU: job is a dict of BiJob fields.
+    Returns a dict of the BiResult fields."""
+    g = GateParam(lcu_level, frame_threads)
+    r = BiResult()
+    lib().xo_pred_inter_search_bi(C.byref(prm), C.byref(g), C.byref(BiJob(**job)), dist_fwd, dist_bwd,
+                                  cur_plane.origin_ptr(), cur_plane.stride, plane_ptrs(planes_fwd),
+                                  plane_ptrs(planes_bwd), planes_fwd[0].stride, C.byref(r))
+    return {f: getattr(r, f) for f, _ in BiResult._fields_}
+
+
+def pred_inter_search_dual(prm, job, dist, cur_plane, planes_by_ref, lcu_level=6, frame_threads=1):
+    """pred_inter_search_dual for one PU: job = dict(pix_x, pix_y, w, h, mvp_x,
+    mvp_y, mvp1=[(x, y)] * n_ref, imv=[(x, y)] * n_ref); planes_by_ref: one
+    16-plane list per reference.  Returns (cost, fst_ref, snd_ref, mvcost, fst,
+    snd) with fst / snd lists of n_ref (x, y)."""
+    n_ref = len(planes_by_ref)
+    j = DualJob()
+    for f in ("pix_x", "pix_y", "w", "h", "mvp_x", "mvp_y"):
+        setattr(j, f, job[f])
+    for r in range(n_ref):
+        j.mvp1[r][0], j.mvp1[r][1] = job["mvp1"][r]
+        j.imv[r][0], j.imv[r][1] = job["imv"][r]
+    d = (C.c_int * 4)(*(list(dist) + [0] * (4 - len(dist))))
+    tabs = [plane_ptrs(p) for p in planes_by_ref]
+    arr = (C.c_void_p * 4)(*[C.cast(t, C.c_void_p) for t in tabs] + [None] * (4 - n_ref))
+    res = DualResult()
+    lib().xo_pred_inter_search_dual(C.byref(prm), C.byref(GateParam(lcu_level, frame_threads)), C.byref(j),
+                                    n_ref, d, cur_plane.origin_ptr(), cur_plane.stride, arr,
+                                    planes_by_ref[0][0].stride, C.byref(res))
+    return (res.cost, res.fst_ref, res.snd_ref, res.mvcost, [tuple(res.fst[r]) for r in range(n_ref)],
+            [tuple(res.snd[r]) for r in range(n_ref)])
 
 
 def get_mvp_default(job, b_slice, ref_dpoc, ref_dpoc_multi):

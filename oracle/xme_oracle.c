@@ -1205,6 +1205,167 @@ int32_t xo_me_search_sym(const xo_me_param_t *prm, const xo_sym_job_t *job,
     return bcost;
 }
 
+/* check_mvd (md_inter.c:68-76). */
+static int gate_mvd(const xo_gate_param_t *g, int mvd_x, int mvd_y)
+{
+    if (g->frame_threads > 1)
+        return mvd_x < 4096 && mvd_x >= -4096 && mvd_y < ((1 << g->lcu_level) << 2) &&
+               mvd_y >= -((1 << g->lcu_level) << 2);
+    return mvd_x < 4096 && mvd_x >= -4096 && mvd_y < 4096 && mvd_y >= -4096;
+}
+
+/* The qpel box of check_mv_range and check_mv_range_sym (predict.h:68-78,
+ * md_inter.c:83-94): the PU may reach 1 << lcu_level pixels past the
+ * picture, inside h->min/max_mv_range. */
+static void gate_box(const xo_me_param_t *prm, const xo_gate_param_t *g, int pix_x, int pix_y, int w, int h,
+                     int *min_x, int *min_y, int *max_x, int *max_y)
+{
+    const int bsize = 1 << g->lcu_level;
+    *min_x = -((pix_x + bsize) << 2);
+    *min_y = -((pix_y + bsize) << 2);
+    *max_x = ((prm->frame_w - (pix_x + w)) + bsize) << 2;
+    *max_y = ((prm->frame_h - (pix_y + h)) + bsize) << 2;
+    if (*min_x < prm->min_mv_range[0]) *min_x = prm->min_mv_range[0];
+    if (*min_y < prm->min_mv_range[1]) *min_y = prm->min_mv_range[1];
+    if (*max_x > prm->max_mv_range[0]) *max_x = prm->max_mv_range[0];
+    if (*max_y > prm->max_mv_range[1]) *max_y = prm->max_mv_range[1];
+}
+
+/* check_mv_range (predict.h:68-96) with every reference row reconstructed. */
+static int gate_range(const xo_me_param_t *prm, const xo_gate_param_t *g, int pix_x, int pix_y, int w, int h,
+                      int mx, int my)
+{
+    int min_x, min_y, max_x, max_y;
+    gate_box(prm, g, pix_x, pix_y, w, h, &min_x, &min_y, &max_x, &max_y);
+    return mx <= max_x && mx >= min_x && my <= max_y && my >= min_y;
+}
+
+/* check_mv_range_sym (md_inter.c:83-110), frame sequences: the MV and its
+ * mirror scaled to the second reference (negated in B slices). */
+static int gate_range_sym(const xo_me_param_t *prm, const xo_gate_param_t *g, int pix_x, int pix_y, int w, int h,
+                          int mx, int my, int dist_fwd, int dist_bwd, int b_slice)
+{
+    int min_x, min_y, max_x, max_y;
+    gate_box(prm, g, pix_x, pix_y, w, h, &min_x, &min_y, &max_x, &max_y);
+    int bx = xo_scale_mv_skip(mx, dist_bwd, dist_fwd), by = xo_scale_mv_skip(my, dist_bwd, dist_fwd);
+    if (b_slice) {
+        bx = -bx;
+        by = -by;
+    }
+    if (mx > max_x || mx < min_x || my > max_y || my < min_y)
+        return 0;
+    if (bx > max_x || bx < min_x || by > max_y || by < min_y)
+        return 0;
+    return 1;
+}
+
+void xo_pred_inter_search_bi(const xo_me_param_t *prm, const xo_gate_param_t *g, const xo_bi_job_t *job,
+                             int dist_fwd, int dist_bwd, const uint8_t *cur, intptr_t i_cur,
+                             const uint8_t *const planes_fwd[16], const uint8_t *const planes_bwd[16],
+                             intptr_t i_ref, xo_bi_result_t *res)
+{
+    const int px = job->pix_x, py = job->pix_y, w = job->w, h = job->h;
+    /* mvp = p_me->mvp1 = all_mvp[B_FWD]; mv = fwd_mv = all_best_imv[B_FWD] << 2,
+     * shifted in the int16 components of mv_t (md_inter.c:1068-1069) */
+    int mv_x = (int16_t)(job->fwd_imv_x * 4), mv_y = (int16_t)(job->fwd_imv_y * 4);
+    int fwd_x = mv_x, fwd_y = mv_y;
+    const int bwd_x = job->bwd_x, bwd_y = job->bwd_y;
+    int cost = XO_MAX_DISTORTION, cost_bid = XO_MAX_DISTORTION;
+    res->sym_mvcost = -1;
+    res->bid_mvcost = -1;
+    int ok = gate_range_sym(prm, g, px, py, w, h, job->mvp1_x, job->mvp1_y, dist_fwd, dist_bwd, 1);
+    ok &= gate_range_sym(prm, g, px, py, w, h, mv_x, mv_y, dist_fwd, dist_bwd, 1);
+    ok &= gate_mvd(g, job->mvp1_x, job->mvp1_y);
+    if (ok) {
+        xo_sym_job_t sj = { px, py, w, h, mv_x, mv_y, job->mvp_x, job->mvp_y, job->mvp1_x, job->mvp1_y,
+                            dist_fwd, dist_bwd, 1 };
+        xo_sym_result_t sr;
+        cost = xo_me_search_sym(prm, &sj, cur, i_cur, planes_fwd, planes_bwd, i_ref, &sr);
+        mv_x = sr.mv_x;
+        mv_y = sr.mv_y;
+        res->sym_mvcost = sr.mvcost;
+    }
+    ok = gate_range(prm, g, px, py, w, h, fwd_x, fwd_y) && gate_range(prm, g, px, py, w, h, bwd_x, bwd_y) &&
+         gate_mvd(g, job->mvp1_x, job->mvp1_y) && gate_mvd(g, job->mvp2_x, job->mvp2_y);
+    if (ok) {
+        xo_bid_job_t bj = { px, py, w, h, fwd_x, fwd_y, bwd_x, bwd_y, job->mvp1_x, job->mvp1_y, job->mvp2_x,
+                            job->mvp2_y };
+        xo_sym_result_t br;
+        cost_bid = xo_me_search_bid(prm, &bj, cur, i_cur, planes_fwd, planes_bwd, i_ref, &br);
+        fwd_x = br.mv_x;
+        fwd_y = br.mv_y;
+        res->bid_mvcost = br.mvcost;
+    }
+    /* the stored MVs' own checks (md_inter.c:1107-1122) */
+    if (!(gate_range(prm, g, px, py, w, h, fwd_x, fwd_y) &&
+          gate_mvd(g, fwd_x - job->mvp1_x, fwd_y - job->mvp1_y)))
+        cost_bid = XO_MAX_DISTORTION;
+    if (!(gate_range(prm, g, px, py, w, h, bwd_x, bwd_y) &&
+          gate_mvd(g, bwd_x - job->mvp2_x, bwd_y - job->mvp2_y)))
+        cost_bid = XO_MAX_DISTORTION;
+    if (!(gate_range_sym(prm, g, px, py, w, h, mv_x, mv_y, dist_fwd, dist_bwd, 1) &&
+          gate_mvd(g, mv_x - job->mvp1_x, mv_y - job->mvp1_y)))
+        cost = XO_MAX_DISTORTION;
+    res->sym_cost = cost;
+    res->sym_mv_x = mv_x;
+    res->sym_mv_y = mv_y;
+    res->bid_cost = cost_bid;
+    res->fwd_x = fwd_x;
+    res->fwd_y = fwd_y;
+    res->bwd_x = bwd_x;
+    res->bwd_y = bwd_y;
+}
+
+/* REF_BITS (predict.h:47-52) for REF_COST(ref_idx) = WEIGHTED_COST(lambda,
+ * REF_BITS[ref_idx + 1]). */
+static const int REF_BITS[8] = { 1, 1, 3, 3, 5, 5, 5, 5 };
+
+void xo_pred_inter_search_dual(const xo_me_param_t *prm, const xo_gate_param_t *g, const xo_dual_job_t *job,
+                               int n_ref, const int dist[4], const uint8_t *cur, intptr_t i_cur,
+                               const uint8_t *const *const planes[4], intptr_t i_ref, xo_dual_result_t *res)
+{
+    const int px = job->pix_x, py = job->pix_y, w = job->w, h = job->h;
+    res->cost = XO_MAX_DISTORTION;
+    res->fst_ref = res->snd_ref = -1;
+    res->mvcost = -1;
+    memset(res->fst, 0, sizeof(res->fst));
+    memset(res->snd, 0, sizeof(res->snd));
+    for (int r = 0; r < n_ref; r++) {
+        const int snd = !r;
+        const int d_fwd = dist[r], d_bwd = dist[snd];
+        const int m1x = job->mvp1[r][0], m1y = job->mvp1[r][1];
+        int fx = (int16_t)(job->imv[r][0] * 4), fy = (int16_t)(job->imv[r][1] * 4);   /* mv_t (:1177-1178) */
+        int cost = XO_MAX_DISTORTION, mvcost = -1;
+        int ok = gate_range_sym(prm, g, px, py, w, h, fx, fy, d_fwd, d_bwd, 0);
+        ok &= gate_mvd(g, fx - m1x, fy - m1y);
+        ok &= gate_mvd(g, m1x, m1y);
+        ok &= gate_mvd(g, job->mvp_x, job->mvp_y);
+        if (ok) {
+            xo_sym_job_t sj = { px, py, w, h, fx, fy, job->mvp_x, job->mvp_y, m1x, m1y, d_fwd, d_bwd, 0 };
+            xo_sym_result_t sr;
+            cost = xo_me_search_sym(prm, &sj, cur, i_cur, planes[r], planes[snd], i_ref, &sr);
+            fx = sr.mv_x;
+            fy = sr.mv_y;
+            mvcost = sr.mvcost;
+        }
+        res->fst[r][0] = fx;
+        res->fst[r][1] = fy;
+        res->snd[r][0] = (int16_t)xo_scale_mv_skip(fx, d_bwd, d_fwd);
+        res->snd[r][1] = (int16_t)xo_scale_mv_skip(fy, d_bwd, d_fwd);
+        ok &= gate_range_sym(prm, g, px, py, w, h, fx, fy, d_fwd, d_bwd, 0);
+        ok &= gate_mvd(g, fx - m1x, fy - m1y);
+        if (ok) {
+            cost += weighted_cost(prm->lambda_factor, REF_BITS[r + 1]);
+            if (cost < res->cost) {
+                res->cost = cost;
+                res->fst_ref = r;
+                res->snd_ref = snd;
+                res->mvcost = mvcost;
+            }
+        }
+    }
+}
+
 /* The qpel window of md_inter.c:913-924 (p_me->mv_min / mv_max), packed as
  * the refinement functions pack it (me.c:1207-1208, :1300-1301). */
 static mv_window_t qpel_window(const xo_me_param_t *prm, int pix_x, int pix_y, int w, int h)

@@ -221,6 +221,60 @@ int32_t xo_me_search_bid(const xo_me_param_t *prm, const xo_bid_job_t *job,
                          const uint8_t *const planes1[16], const uint8_t *const planes2[16], intptr_t i_ref,
                          xo_sym_result_t *res);
 
+/* The callers of the two refinements, frame sequences, every reference row
+ * reconstructed (so check_mv_range's num_lcu_coded_in_row test holds,
+ * predict.h:86-95): pred_inter_search_bi (md_inter.c:1036-1128, B slices)
+ * and pred_inter_search_dual (md_inter.c:1133-1219, F slices).  lcu_level is
+ * h->i_lcu_level (the valid padding is 1 << lcu_level); frame_threads > 1
+ * narrows check_mvd's vertical range (md_inter.c:68-76). */
+typedef struct {
+    int lcu_level;
+    int frame_threads;
+} xo_gate_param_t;
+
+typedef struct {
+    int pix_x, pix_y, w, h;           /* the PU */
+    int fwd_imv_x, fwd_imv_y;         /* p_me->all_best_imv[B_FWD] (full-pel) */
+    int bwd_x, bwd_y;                 /* all_single_mv[B_BWD] (qpel) */
+    int mvp_x, mvp_y;                 /* p_me->mvp on entry: the symmetric search's MV rate */
+    int mvp1_x, mvp1_y;               /* all_mvp[B_FWD] */
+    int mvp2_x, mvp2_y;               /* all_mvp[B_BWD] */
+} xo_bi_job_t;
+
+typedef struct {
+    int32_t sym_cost;                 /* *sym_mcost */
+    int32_t sym_mv_x, sym_mv_y;       /* all_sym_mv[0] */
+    int32_t sym_mvcost;               /* p_me->bmvcost[PDIR_SYM] (-1: the search did not run) */
+    int32_t bid_cost;                 /* *bid_mcost */
+    int32_t fwd_x, fwd_y, bwd_x, bwd_y;   /* all_dual_mv_1st / _2nd[0] */
+    int32_t bid_mvcost;               /* p_me->bmvcost[PDIR_BID] (-1: the search did not run) */
+} xo_bi_result_t;
+
+void xo_pred_inter_search_bi(const xo_me_param_t *prm, const xo_gate_param_t *g, const xo_bi_job_t *job,
+                             int dist_fwd, int dist_bwd, const uint8_t *cur, intptr_t i_cur,
+                             const uint8_t *const planes_fwd[16], const uint8_t *const planes_bwd[16],
+                             intptr_t i_ref, xo_bi_result_t *res);
+
+typedef struct {
+    int pix_x, pix_y, w, h;           /* the PU */
+    int mvp_x, mvp_y;                 /* p_me->mvp on entry */
+    int mvp1[4][2];                   /* all_mvp[ref_idx] */
+    int imv[4][2];                    /* p_me->all_best_imv[ref_idx] (full-pel) */
+} xo_dual_job_t;
+
+typedef struct {
+    int32_t cost;                     /* *dual_mcost (XO_MAX_DISTORTION: none) */
+    int32_t fst_ref, snd_ref;         /* *dual_best_fst_ref / _snd_ref (-1: none) */
+    int32_t mvcost;                   /* p_me->bmvcost[PDIR_DUAL] (-1: not set) */
+    int32_t fst[4][2], snd[4][2];     /* all_dual_mv_1st / _2nd[ref_idx] */
+} xo_dual_result_t;
+
+/* planes[r] = the 16 planes of reference r (r < n_ref, n_ref 2..4); dist[r] =
+ * h->fdec->ref_dpoc[r] (calculate_distance). */
+void xo_pred_inter_search_dual(const xo_me_param_t *prm, const xo_gate_param_t *g, const xo_dual_job_t *job,
+                               int n_ref, const int dist[4], const uint8_t *cur, intptr_t i_cur,
+                               const uint8_t *const *const planes[4], intptr_t i_ref, xo_dual_result_t *res);
+
 /* The same over n PUs on nthreads POSIX threads (CPU baseline); ctxs holds
  * one UMH context per PU or is NULL. */
 void xo_me_search_batch(const xo_me_param_t *prm, const xo_pu_t *pus, int n,

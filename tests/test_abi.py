@@ -72,6 +72,8 @@ LAYOUTS = {
     "xme_sym_job": "SYM_JOB_DTYPE", "xme_sym_result": "SYM_RESULT_DTYPE", "xme_bid_job": "BID_JOB_DTYPE",
     "xme_mvp_job": "MVP_JOB_DTYPE", "xme_mvp_result": "MVP_RESULT_DTYPE",
     "xme_umh_job": "UMH_JOB_DTYPE", "xme_umh_state": "UMH_STATE_DTYPE",
+    "xme_bi_job": "BI_JOB_DTYPE", "xme_bi_result": "BI_RESULT_DTYPE",
+    "xme_dual_job": "DUAL_JOB_DTYPE", "xme_dual_result": "DUAL_RESULT_DTYPE",
 }
 
 

@@ -250,27 +250,17 @@ __device__ __forceinline__ void bi_stage(const Eval &ev, int stage, int pos0, in
     }
 }
 
-__global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int64_t i_ref, const uint8_t *cur,
-                                                  int64_t i_cur, const xme_sym_job *__restrict__ jobs, int n,
-                                                  xme_sym_result *__restrict__ out)
+// A refinement's outcome, the same in every lane of the wave.
+struct SearchOut {
+    int cost, mv_x, mv_y, mvcost, status;
+};
+
+// xavs2_me_search_sym for job j, by the whole wave.
+__device__ SearchOut sym_search(const SymParams &p, const SymPlanes &pl, int64_t i_ref, const uint8_t *cur,
+                                int64_t i_cur, const xme_sym_job &j)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
-    if (ji >= n)
-        return;
-    const int lane = threadIdx.x & 63;
-    const xme_sym_job j = jobs[ji];
-    xme_sym_result res;
-    if (!sym_job_ok(p, j)) {
-        if (lane == 0) {
-            res.cost = XME_MAX_DISTORTION;
-            res.mv_x = j.mv_x;
-            res.mv_y = j.mv_y;
-            res.mvcost = -1;
-            res.status = 2;
-            out[ji] = res;
-        }
-        return;
-    }
+    if (!sym_job_ok(p, j))
+        return SearchOut{ XME_MAX_DISTORTION, j.mv_x, j.mv_y, -1, 2 };
     SymJob c;
     c.cur = (gbytes)cur + (int64_t)j.y * i_cur + j.x;
     c.i_cur = i_cur;
@@ -310,15 +300,7 @@ __global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int
         // the start point alone; its cost is returned even when there is none
         // (me.c:1219-1226)
         bi_stage(SymEval{ c, pl }, 0, 0, 1, j.mv_x, j.mv_y, false, 0, 0, P, bcost, bx, by);
-        if (lane == 0) {
-            res.cost = bcost;
-            res.mv_x = j.mv_x;
-            res.mv_y = j.mv_y;
-            res.mvcost = -1;
-            res.status = 0;
-            out[ji] = res;
-        }
-        return;
+        return SearchOut{ bcost, j.mv_x, j.mv_y, -1, 0 };
     }
     // half-pel: Spiral[0..4] with SATD, Spiral[1..4] with SAD (me.c:1210-1211)
     const int pos0 = c.hadamard ? 0 : 1;
@@ -328,12 +310,25 @@ __global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int
         const int ctr_x = (j.mvp1_x >> 1) << 1, ctr_y = (j.mvp1_y >> 1) << 1;
         bi_stage(ev, 1, 1, 4, bx, by, p.pmvr != 0, ctr_x, ctr_y, P, bcost, bx, by);
     }
-    if (lane == 0) {
-        res.cost = bcost;
-        res.mv_x = (int16_t)bx;
-        res.mv_y = (int16_t)by;
-        res.mvcost = xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp_x) + xme_mvbits(by - j.mvp_y));
-        res.status = 0;
+    return SearchOut{ bcost, (int16_t)bx, (int16_t)by,
+                      xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp_x) + xme_mvbits(by - j.mvp_y)), 0 };
+}
+
+__global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int64_t i_ref, const uint8_t *cur,
+                                                  int64_t i_cur, const xme_sym_job *__restrict__ jobs, int n,
+                                                  xme_sym_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const SearchOut o = sym_search(p, pl, i_ref, cur, i_cur, jobs[ji]);
+    if ((threadIdx.x & 63) == 0) {
+        xme_sym_result res;
+        res.cost = o.cost;
+        res.mv_x = (int16_t)o.mv_x;
+        res.mv_y = (int16_t)o.mv_y;
+        res.mvcost = o.mvcost;
+        res.status = o.status;
         out[ji] = res;
     }
 }
@@ -419,27 +414,13 @@ struct BidEval {
     }
 };
 
-__global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int64_t i_ref, const uint8_t *cur,
-                                                  int64_t i_cur, const xme_bid_job *__restrict__ jobs, int n,
-                                                  xme_sym_result *__restrict__ out)
+// xavs2_me_search_bid for job j, by the whole wave.
+__device__ SearchOut bid_search(const SymParams &p, const SymPlanes &pl, int64_t i_ref, const uint8_t *cur,
+                                int64_t i_cur, const xme_bid_job &j)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
-    if (ji >= n)
-        return;
     const int lane = threadIdx.x & 63;
-    const xme_bid_job j = jobs[ji];
-    xme_sym_result res;
-    if (!bid_job_ok(p, j)) {
-        if (lane == 0) {
-            res.cost = XME_MAX_DISTORTION;
-            res.mv_x = j.fwd_x;
-            res.mv_y = j.fwd_y;
-            res.mvcost = -1;
-            res.status = 2;
-            out[ji] = res;
-        }
-        return;
-    }
+    if (!bid_job_ok(p, j))
+        return SearchOut{ XME_MAX_DISTORTION, j.fwd_x, j.fwd_y, -1, 2 };
     SymJob c;
     c.cur = (gbytes)cur + (int64_t)j.y * i_cur + j.x;
     c.i_cur = i_cur;
@@ -478,15 +459,7 @@ __global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int
     const uint8_t *p2 = pick_plane(pl.p2, ((j.bwd_y & 3) << 2) + (j.bwd_x & 3));
     if (bwd_legal && p2 == nullptr) {
         // the reference predicts from mc_luma here (me.c:1329-1335): not provided
-        if (lane == 0) {
-            res.cost = XME_MAX_DISTORTION;
-            res.mv_x = j.fwd_x;
-            res.mv_y = j.fwd_y;
-            res.mvcost = -1;
-            res.status = 2;
-            out[ji] = res;
-        }
-        return;
+        return SearchOut{ XME_MAX_DISTORTION, j.fwd_x, j.fwd_y, -1, 2 };
     }
     // the target rows of the lane's tiles: clip(2 * cur - pred_bwd)
     // (me.c:1322-1337)
@@ -511,15 +484,7 @@ __global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int
     int bcost = XME_MAX_DISTORTION, bx = j.fwd_x, by = j.fwd_y;
     if (p.frac == 0) {
         bi_stage(ev, 0, 0, 1, j.fwd_x, j.fwd_y, false, 0, 0, P, bcost, bx, by);
-        if (lane == 0) {
-            res.cost = bcost;
-            res.mv_x = j.fwd_x;
-            res.mv_y = j.fwd_y;
-            res.mvcost = -1;
-            res.status = 0;
-            out[ji] = res;
-        }
-        return;
+        return SearchOut{ bcost, j.fwd_x, j.fwd_y, -1, 0 };
     }
     // half-pel: Spiral[0..8] with SATD, Spiral[1..8] with SAD (me.c:1340-1349)
     const int pos0 = c.hadamard ? 0 : 1;
@@ -528,13 +493,272 @@ __global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int
         const int ctr_x = (j.mvp1_x >> 1) << 1, ctr_y = (j.mvp1_y >> 1) << 1;
         bi_stage(ev, 1, 1, 8, bx, by, p.pmvr != 0, ctr_x, ctr_y, P, bcost, bx, by);
     }
-    if (lane == 0) {
-        res.cost = bcost;
-        res.mv_x = (int16_t)bx;
-        res.mv_y = (int16_t)by;
-        res.mvcost = xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp1_x) + xme_mvbits(by - j.mvp1_y)) + bid_bits;
-        res.status = 0;
+    return SearchOut{ bcost, (int16_t)bx, (int16_t)by,
+                      xme_weighted_cost(p.lambda, xme_mvbits(bx - j.mvp1_x) + xme_mvbits(by - j.mvp1_y)) + bid_bits,
+                      0 };
+}
+
+__global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int64_t i_ref, const uint8_t *cur,
+                                                  int64_t i_cur, const xme_bid_job *__restrict__ jobs, int n,
+                                                  xme_sym_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const SearchOut o = bid_search(p, pl, i_ref, cur, i_cur, jobs[ji]);
+    if ((threadIdx.x & 63) == 0) {
+        xme_sym_result res;
+        res.cost = o.cost;
+        res.mv_x = (int16_t)o.mv_x;
+        res.mv_y = (int16_t)o.mv_y;
+        res.mvcost = o.mvcost;
+        res.status = o.status;
         out[ji] = res;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The callers, pred_inter_search_bi (md_inter.c:1036-1128) and
+// pred_inter_search_dual (md_inter.c:1133-1219): one wave per PU runs the
+// gates on uniform registers and the refinements they let through.
+
+struct GateParams {
+    int lcu_level, frame_threads;
+};
+
+// check_mvd (md_inter.c:68-76).
+__device__ __forceinline__ bool gate_mvd(const GateParams &g, int dx, int dy)
+{
+    const int vy = g.frame_threads > 1 ? ((1 << g.lcu_level) << 2) : 4096;
+    return dx < 4096 && dx >= -4096 && dy < vy && dy >= -vy;
+}
+
+// The qpel box of check_mv_range and check_mv_range_sym (predict.h:70-79,
+// md_inter.c:85-95): the PU may reach 1 << lcu_level pixels past the
+// picture, inside min/max_mv_range.
+struct GateBox {
+    int min_x, min_y, max_x, max_y;
+    __device__ bool holds(int x, int y) const { return x <= max_x && x >= min_x && y <= max_y && y >= min_y; }
+};
+
+__device__ __forceinline__ GateBox gate_box(const SymParams &p, const GateParams &g, int px, int py, int w, int h)
+{
+    const int bsize = 1 << g.lcu_level;
+    GateBox b;
+    b.min_x = max(-((px + bsize) << 2), p.min_mv[0]);
+    b.min_y = max(-((py + bsize) << 2), p.min_mv[1]);
+    b.max_x = min(((p.frame_w - (px + w)) + bsize) << 2, p.max_mv[0]);
+    b.max_y = min(((p.frame_h - (py + h)) + bsize) << 2, p.max_mv[1]);
+    return b;
+}
+
+// scale_mv_skip (predict.h:148-151); *undef is set when the reference's int
+// product overflows (undefined there).
+__device__ __forceinline__ int gate_scale(int mv, int dist_dst, int dist_src, bool &undef)
+{
+    const int64_t s = (int64_t)mv * dist_dst * (16384 / dist_src) + 8192;
+    if (s > 0x7fffffffLL || s < -0x80000000LL)
+        undef = true;
+    return (int16_t)((int)s >> 14);
+}
+
+// check_mv_range_sym (md_inter.c:83-110), frame sequences.
+__device__ __forceinline__ bool gate_range_sym(const GateBox &b, int mx, int my, int dist_fwd, int dist_bwd, bool b_slice,
+                                               bool &undef)
+{
+    int bx = gate_scale(mx, dist_bwd, dist_fwd, undef), by = gate_scale(my, dist_bwd, dist_fwd, undef);
+    if (b_slice) {
+        bx = -bx;
+        by = -by;
+    }
+    return b.holds(mx, my) && b.holds(bx, by);
+}
+
+// A PU the refinements take (their job checks repeat this).
+__device__ __forceinline__ bool caller_pu_ok(const SymParams &p, int x, int y, int w, int h)
+{
+    return !((w & 3) || (h & 3) || w < 4 || h < 4 || w > 64 || h > 64 || (x & 3) || x < 0 || y < 0 ||
+             x + w > p.frame_w || y + h > p.frame_h);
+}
+
+__global__ void __launch_bounds__(256) bi_kernel(SymParams p, GateParams g, SymPlanes pl, int64_t i_ref,
+                                                 const uint8_t *cur, int64_t i_cur, const xme_bi_job *__restrict__ jobs,
+                                                 int n, int dist_fwd, int dist_bwd, xme_bi_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const xme_bi_job j = jobs[ji];
+    bool undef = !caller_pu_ok(p, j.x, j.y, j.w, j.h);
+    const GateBox box = gate_box(p, g, j.x, j.y, j.w, j.h);
+    // fwd_mv.x <<= 2 on the int16 components of mv_t (md_inter.c:1068-1069)
+    int mv_x = (int16_t)(j.fwd_imv_x * 4), mv_y = (int16_t)(j.fwd_imv_y * 4);
+    int fwd_x = mv_x, fwd_y = mv_y;
+    int cost = XME_MAX_DISTORTION, cost_bid = XME_MAX_DISTORTION, sym_mvcost = -1, bid_mvcost = -1;
+    bool ok = gate_range_sym(box, j.mvp1_x, j.mvp1_y, dist_fwd, dist_bwd, true, undef);
+    ok &= gate_range_sym(box, mv_x, mv_y, dist_fwd, dist_bwd, true, undef);
+    ok &= gate_mvd(g, j.mvp1_x, j.mvp1_y);
+    if (ok && !undef) {
+        xme_sym_job sj;
+        sj.x = j.x;
+        sj.y = j.y;
+        sj.w = j.w;
+        sj.h = j.h;
+        sj.mv_x = (int16_t)mv_x;
+        sj.mv_y = (int16_t)mv_y;
+        sj.mvp_x = j.mvp_x;
+        sj.mvp_y = j.mvp_y;
+        sj.mvp1_x = j.mvp1_x;
+        sj.mvp1_y = j.mvp1_y;
+        sj.dist_fwd = (int16_t)dist_fwd;
+        sj.dist_bwd = (int16_t)dist_bwd;
+        sj.b_slice = 1;
+        const SearchOut o = sym_search(p, pl, i_ref, cur, i_cur, sj);
+        undef |= o.status != 0;
+        cost = o.cost;
+        mv_x = o.mv_x;
+        mv_y = o.mv_y;
+        sym_mvcost = o.mvcost;
+    }
+    ok = box.holds(fwd_x, fwd_y) && box.holds(j.bwd_x, j.bwd_y) && gate_mvd(g, j.mvp1_x, j.mvp1_y) &&
+         gate_mvd(g, j.mvp2_x, j.mvp2_y);
+    if (ok && !undef) {
+        xme_bid_job bj;
+        bj.x = j.x;
+        bj.y = j.y;
+        bj.w = j.w;
+        bj.h = j.h;
+        bj.fwd_x = (int16_t)fwd_x;
+        bj.fwd_y = (int16_t)fwd_y;
+        bj.bwd_x = j.bwd_x;
+        bj.bwd_y = j.bwd_y;
+        bj.mvp1_x = j.mvp1_x;
+        bj.mvp1_y = j.mvp1_y;
+        bj.mvp2_x = j.mvp2_x;
+        bj.mvp2_y = j.mvp2_y;
+        const SearchOut o = bid_search(p, pl, i_ref, cur, i_cur, bj);
+        undef |= o.status != 0;
+        cost_bid = o.cost;
+        fwd_x = o.mv_x;
+        fwd_y = o.mv_y;
+        bid_mvcost = o.mvcost;
+    }
+    // the stored MVs' own checks (md_inter.c:1107-1122)
+    if (!(box.holds(fwd_x, fwd_y) && gate_mvd(g, fwd_x - j.mvp1_x, fwd_y - j.mvp1_y)))
+        cost_bid = XME_MAX_DISTORTION;
+    if (!(box.holds(j.bwd_x, j.bwd_y) && gate_mvd(g, j.bwd_x - j.mvp2_x, j.bwd_y - j.mvp2_y)))
+        cost_bid = XME_MAX_DISTORTION;
+    if (!(gate_range_sym(box, mv_x, mv_y, dist_fwd, dist_bwd, true, undef) &&
+          gate_mvd(g, mv_x - j.mvp1_x, mv_y - j.mvp1_y)))
+        cost = XME_MAX_DISTORTION;
+    if ((threadIdx.x & 63) == 0) {
+        xme_bi_result r;
+        r.sym_cost = cost;
+        r.sym_mv_x = (int16_t)mv_x;
+        r.sym_mv_y = (int16_t)mv_y;
+        r.sym_mvcost = sym_mvcost;
+        r.bid_cost = cost_bid;
+        r.fwd_x = (int16_t)fwd_x;
+        r.fwd_y = (int16_t)fwd_y;
+        r.bwd_x = j.bwd_x;
+        r.bwd_y = j.bwd_y;
+        r.bid_mvcost = bid_mvcost;
+        r.status = undef ? 2 : 0;
+        out[ji] = r;
+    }
+}
+
+// One reference of pred_inter_search_dual's loop (md_inter.c:1160-1218) for
+// every PU: launch r searches reference r with the second reference !r and
+// updates the running minimum that launch r - 1 left in out[] (launch 0
+// starts it), so the launches on one stream keep the reference's order.
+__global__ void __launch_bounds__(256) dual_kernel(SymParams p, GateParams g, SymPlanes pl, int64_t i_ref,
+                                                   const uint8_t *cur, int64_t i_cur,
+                                                   const xme_dual_job *__restrict__ jobs, int n, int r, int d_fwd,
+                                                   int d_bwd, xme_dual_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const xme_dual_job &jr = jobs[ji];
+    const int x = jr.x, y = jr.y, w = jr.w, h = jr.h;
+    int best = XME_MAX_DISTORTION, best_fst = -1, best_snd = -1, best_mvcost = -1;
+    bool undef;
+    if (r == 0) {
+        undef = !caller_pu_ok(p, x, y, w, h);
+    } else {
+        best = out[ji].cost;
+        best_fst = out[ji].fst_ref;
+        best_snd = out[ji].snd_ref;
+        best_mvcost = out[ji].mvcost;
+        undef = out[ji].status != 0;
+    }
+    const GateBox box = gate_box(p, g, x, y, w, h);
+    const int s = r == 0 ? 1 : 0;
+    const int m1x = jr.mvp1[r][0], m1y = jr.mvp1[r][1], mvp_x = jr.mvp_x, mvp_y = jr.mvp_y;
+    // fst_dual.x <<= 2 on the int16 components of mv_t (md_inter.c:1177-1178)
+    int fx = (int16_t)(jr.imv[r][0] * 4), fy = (int16_t)(jr.imv[r][1] * 4);
+    int cost = XME_MAX_DISTORTION, mvcost = -1;
+    bool ok = gate_range_sym(box, fx, fy, d_fwd, d_bwd, false, undef);
+    ok &= gate_mvd(g, fx - m1x, fy - m1y);
+    ok &= gate_mvd(g, m1x, m1y);
+    ok &= gate_mvd(g, mvp_x, mvp_y);
+    if (ok && !undef) {
+        xme_sym_job sj;
+        sj.x = (int16_t)x;
+        sj.y = (int16_t)y;
+        sj.w = (uint8_t)w;
+        sj.h = (uint8_t)h;
+        sj.mv_x = (int16_t)fx;
+        sj.mv_y = (int16_t)fy;
+        sj.mvp_x = (int16_t)mvp_x;
+        sj.mvp_y = (int16_t)mvp_y;
+        sj.mvp1_x = (int16_t)m1x;
+        sj.mvp1_y = (int16_t)m1y;
+        sj.dist_fwd = (int16_t)d_fwd;
+        sj.dist_bwd = (int16_t)d_bwd;
+        sj.b_slice = 0;
+        const SearchOut o = sym_search(p, pl, i_ref, cur, i_cur, sj);
+        undef |= o.status != 0;
+        fx = o.mv_x;
+        fy = o.mv_y;
+        cost = o.cost;
+        mvcost = o.mvcost;
+    }
+    // snd_dual (md_inter.c:1192-1193)
+    const int sx = gate_scale(fx, d_bwd, d_fwd, undef), sy = gate_scale(fy, d_bwd, d_fwd, undef);
+    ok &= gate_range_sym(box, fx, fy, d_fwd, d_bwd, false, undef);
+    ok &= gate_mvd(g, fx - m1x, fy - m1y);
+    if (ok) {
+        // REF_COST(ref_idx) = WEIGHTED_COST(lambda, REF_BITS[ref_idx + 1]),
+        // REF_BITS {1, 1, 3, 3, 5, ...} (predict.h:47-52): 1, 3, 3, 5 bits
+        // for references 0..3
+        cost += xme_weighted_cost(p.lambda, r == 0 ? 1 : (r == 3 ? 5 : 3));
+        if (cost < best) {
+            best = cost;
+            best_fst = r;
+            best_snd = s;
+            best_mvcost = mvcost;
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        xme_dual_result &o = out[ji];
+        if (r == 0) {
+            for (int q = 1; q < 4; q++) {
+                o.fst[q][0] = o.fst[q][1] = 0;
+                o.snd[q][0] = o.snd[q][1] = 0;
+            }
+            o.reserved = 0;
+        }
+        o.cost = best;
+        o.fst_ref = (int8_t)best_fst;
+        o.snd_ref = (int8_t)best_snd;
+        o.mvcost = best_mvcost;
+        o.fst[r][0] = (int16_t)fx;
+        o.fst[r][1] = (int16_t)fy;
+        o.snd[r][0] = (int16_t)sx;
+        o.snd[r][1] = (int16_t)sy;
+        o.status = undef ? 2 : 0;
     }
 }
 
@@ -626,5 +850,132 @@ extern "C" int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_j
     hipLaunchKernelGGL(bid_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, pl,
                        (int64_t)ref1->stride, d_cur, (int64_t)i_cur, d_jobs, n, d_results);
     XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+namespace {
+
+// SymParams from the ME parameters; false (with the error set) when they are
+// out of range.
+bool bi_params(const xme_me_params *prm, const char *who, SymParams &p)
+{
+    if (!prm || prm->use_fractional_me < 0 || prm->use_fractional_me > 2 || prm->frame_w <= 0 ||
+        prm->frame_h <= 0 || prm->search_range <= 0) {
+        xme_set_error("%s: bad arguments", who);
+        return false;
+    }
+    p.frame_w = prm->frame_w;
+    p.frame_h = prm->frame_h;
+    p.min_mv[0] = prm->min_mv_range[0];
+    p.min_mv[1] = prm->min_mv_range[1];
+    p.max_mv[0] = prm->max_mv_range[0];
+    p.max_mv[1] = prm->max_mv_range[1];
+    p.lambda = prm->lambda_factor;
+    p.hadamard = prm->use_hadamard ? 1 : 0;
+    p.frac = prm->use_fractional_me;
+    p.pmvr = prm->enable_pmvr ? 1 : 0;
+    p.max_mvd = xme_mvbits_max_mvd(prm->search_range);
+    return true;
+}
+
+// The gate parameters and a distance pair of calculate_distance.
+bool gate_params(int lcu_level, int frame_threads, const char *who, GateParams &g)
+{
+    if (lcu_level < 3 || lcu_level > 6 || frame_threads < 1) {
+        xme_set_error("%s: lcu_level must be 3..6 and frame_threads >= 1", who);
+        return false;
+    }
+    g.lcu_level = lcu_level;
+    g.frame_threads = frame_threads;
+    return true;
+}
+
+bool dist_ok(int d) { return d != 0 && d <= 255 && d >= -255; }
+
+} // namespace
+
+extern "C" int xme_pred_inter_search_bi_batch(const xme_me_params *prm, int lcu_level, int frame_threads,
+                                              const xme_bi_job *d_jobs, int n, int dist_fwd, int dist_bwd,
+                                              const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref_fwd,
+                                              const xme_ref_desc *ref_bwd, xme_bi_result *d_results, void *stream)
+{
+    static const char *who = "xme_pred_inter_search_bi_batch";
+    SymParams p;
+    GateParams g;
+    if (!bi_params(prm, who, p) || !gate_params(lcu_level, frame_threads, who, g))
+        return XME_ERR_ARG;
+    if (n < 0 || !ref_fwd || !ref_bwd || !dist_ok(dist_fwd) || !dist_ok(dist_bwd)) {
+        xme_set_error("%s: bad arguments (n >= 0, both references, distances nonzero within +-255)", who);
+        return XME_ERR_ARG;
+    }
+    if (n == 0)
+        return XME_OK;
+    if (!d_jobs || !d_cur || !d_results || !ref_fwd->planes[0] || !ref_bwd->planes[0]) {
+        xme_set_error("%s: NULL buffer", who);
+        return XME_ERR_ARG;
+    }
+    if (ref_fwd->stride != ref_bwd->stride || ((uintptr_t)d_cur & 3) || (i_cur & 3)) {
+        xme_set_error("%s: reference strides differ or the current plane is not 4-byte aligned", who);
+        return XME_ERR_ARG;
+    }
+    SymPlanes pl;
+    for (int k = 0; k < 16; k++) {
+        pl.p1[k] = ref_fwd->planes[k];
+        pl.p2[k] = ref_bwd->planes[k];
+    }
+    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    hipLaunchKernelGGL(bi_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, g, pl,
+                       (int64_t)ref_fwd->stride, d_cur, (int64_t)i_cur, d_jobs, n, dist_fwd, dist_bwd, d_results);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+extern "C" int xme_pred_inter_search_dual_batch(const xme_me_params *prm, int lcu_level, int frame_threads,
+                                                const xme_dual_job *d_jobs, int n, int n_ref, const int dist[4],
+                                                const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *refs,
+                                                xme_dual_result *d_results, void *stream)
+{
+    static const char *who = "xme_pred_inter_search_dual_batch";
+    SymParams p;
+    GateParams g;
+    if (!bi_params(prm, who, p) || !gate_params(lcu_level, frame_threads, who, g))
+        return XME_ERR_ARG;
+    if (n < 0 || n_ref < 2 || n_ref > 4 || !refs || !dist) {
+        xme_set_error("%s: bad arguments (n >= 0, 2..4 references and their distances)", who);
+        return XME_ERR_ARG;
+    }
+    for (int r = 0; r < n_ref; r++) {
+        if (!dist_ok(dist[r])) {
+            xme_set_error("%s: distance %d of reference %d is 0 or beyond +-255", who, dist[r], r);
+            return XME_ERR_ARG;
+        }
+    }
+    if (n == 0)
+        return XME_OK;
+    if (!d_jobs || !d_cur || !d_results || ((uintptr_t)d_cur & 3) || (i_cur & 3)) {
+        xme_set_error("%s: NULL buffer or the current plane is not 4-byte aligned", who);
+        return XME_ERR_ARG;
+    }
+    for (int r = 0; r < n_ref; r++) {
+        if (!refs[r].planes[0] || refs[r].stride != refs[0].stride) {
+            xme_set_error("%s: reference %d has no full-pel plane or another stride", who, r);
+            return XME_ERR_ARG;
+        }
+    }
+    // one launch per reference, in the reference's order (each updates the
+    // minimum the previous one stored)
+    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    for (int r = 0; r < n_ref; r++) {
+        const int s = r == 0 ? 1 : 0;
+        SymPlanes pl;
+        for (int k = 0; k < 16; k++) {
+            pl.p1[k] = refs[r].planes[k];
+            pl.p2[k] = refs[s].planes[k];
+        }
+        hipLaunchKernelGGL(dual_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, g, pl,
+                           (int64_t)refs[0].stride, d_cur, (int64_t)i_cur, d_jobs, n, r, dist[r], dist[s],
+                           d_results);
+        XME_HIP(hipGetLastError());
+    }
     return XME_OK;
 }

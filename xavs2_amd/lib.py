@@ -89,6 +89,19 @@ UMH_STATE_DTYPE = np.dtype([("mvp_x", "<i2"), ("mvp_y", "<i2"), ("pred_sad_space
                             ("reserved", "<i4"), ("ctx", UMH_CTX_DTYPE)])
 SYM_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("mv_x", "<i2"), ("mv_y", "<i2"), ("mvcost", "<i4"),
                              ("status", "<i4")])
+# xme_bi_job / xme_bi_result and xme_dual_job / xme_dual_result: the callers
+# pred_inter_search_bi and pred_inter_search_dual
+BI_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
+                         ("fwd_imv_x", "<i2"), ("fwd_imv_y", "<i2"), ("bwd_x", "<i2"), ("bwd_y", "<i2"),
+                         ("mvp_x", "<i2"), ("mvp_y", "<i2"), ("mvp1_x", "<i2"), ("mvp1_y", "<i2"),
+                         ("mvp2_x", "<i2"), ("mvp2_y", "<i2")])
+BI_RESULT_DTYPE = np.dtype([("sym_cost", "<i4"), ("sym_mv_x", "<i2"), ("sym_mv_y", "<i2"), ("sym_mvcost", "<i4"),
+                            ("bid_cost", "<i4"), ("fwd_x", "<i2"), ("fwd_y", "<i2"), ("bwd_x", "<i2"),
+                            ("bwd_y", "<i2"), ("bid_mvcost", "<i4"), ("status", "<i4")])
+DUAL_JOB_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("r0", "u1"), ("r1", "u1"),
+                           ("mvp_x", "<i2"), ("mvp_y", "<i2"), ("mvp1", "<i2", (4, 2)), ("imv", "<i2", (4, 2))])
+DUAL_RESULT_DTYPE = np.dtype([("cost", "<i4"), ("fst_ref", "i1"), ("snd_ref", "i1"), ("reserved", "<i2"),
+                              ("mvcost", "<i4"), ("fst", "<i2", (4, 2)), ("snd", "<i2", (4, 2)), ("status", "<i4")])
 
 
 class RefDesc(C.Structure):
@@ -160,6 +173,10 @@ def lib():
                                               C.POINTER(RefDesc), P, P]
         L.xme_me_search_bid_batch.argtypes = [C.POINTER(MeParams), P, I, P, Z, C.POINTER(RefDesc),
                                               C.POINTER(RefDesc), P, P]
+        L.xme_pred_inter_search_bi_batch.argtypes = [C.POINTER(MeParams), I, I, P, I, I, I, P, Z,
+                                                     C.POINTER(RefDesc), C.POINTER(RefDesc), P, P]
+        L.xme_pred_inter_search_dual_batch.argtypes = [C.POINTER(MeParams), I, I, P, I, I, C.POINTER(C.c_int),
+                                                       P, Z, C.POINTER(RefDesc), P, P]
         L.xme_dct_batch.argtypes = [I, P, P, I, P]
         L.xme_idct_batch.argtypes = [I, P, P, I, P]
         L.xme_quant_batch.argtypes = [I, P, I, I, I, P, P]
@@ -333,6 +350,42 @@ def me_search_bid_batch(params, jobs, cur, ref1, ref2, stream=None):
     """xme_me_search_bid_batch on host job records (BID_JOB_DTYPE), as
     me_search_sym_batch."""
     return _bipred_batch("xme_me_search_bid_batch", BID_JOB_DTYPE, params, jobs, cur, ref1, ref2, stream)
+
+
+def pred_inter_search_bi_batch(params, jobs, dist_fwd, dist_bwd, cur, ref_fwd, ref_bwd, lcu_level=6,
+                               frame_threads=1, stream=None):
+    """xme_pred_inter_search_bi_batch on host job records (BI_JOB_DTYPE):
+    `cur` a DevicePlane, ref_fwd / ref_bwd DeviceRefs of one stride.  Returns
+    the BI_RESULT_DTYPE records (blocking)."""
+    name = "xme_pred_inter_search_bi_batch"
+    jobs = np.ascontiguousarray(jobs, dtype=BI_JOB_DTYPE)
+    n = len(jobs)
+    d_jobs = upload(jobs) if n else DeviceBuffer(1)
+    d_res = DeviceBuffer(max(n, 1) * BI_RESULT_DTYPE.itemsize)
+    r1, r2 = ref_fwd.desc(), ref_bwd.desc()
+    check(lib().xme_pred_inter_search_bi_batch(C.byref(params), lcu_level, frame_threads, C.c_void_p(d_jobs.ptr),
+                                               n, dist_fwd, dist_bwd, C.c_void_p(cur.origin), cur.stride,
+                                               C.byref(r1), C.byref(r2), C.c_void_p(d_res.ptr), stream), name)
+    check(lib().xme_stream_sync(stream), "xme_stream_sync")
+    return d_res.download(n, BI_RESULT_DTYPE)
+
+
+def pred_inter_search_dual_batch(params, jobs, dist, cur, refs, lcu_level=6, frame_threads=1, stream=None):
+    """xme_pred_inter_search_dual_batch on host job records (DUAL_JOB_DTYPE):
+    refs a list of 2..4 DeviceRefs of one stride, dist their distances.
+    Returns the DUAL_RESULT_DTYPE records (blocking)."""
+    name = "xme_pred_inter_search_dual_batch"
+    jobs = np.ascontiguousarray(jobs, dtype=DUAL_JOB_DTYPE)
+    n = len(jobs)
+    d_jobs = upload(jobs) if n else DeviceBuffer(1)
+    d_res = DeviceBuffer(max(n, 1) * DUAL_RESULT_DTYPE.itemsize)
+    descs = (RefDesc * 4)(*[r.desc() for r in refs])
+    dists = (C.c_int * 4)(*(list(dist) + [0] * (4 - len(dist))))
+    check(lib().xme_pred_inter_search_dual_batch(C.byref(params), lcu_level, frame_threads, C.c_void_p(d_jobs.ptr),
+                                                 n, len(refs), dists, C.c_void_p(cur.origin), cur.stride, descs,
+                                                 C.c_void_p(d_res.ptr), stream), name)
+    check(lib().xme_stream_sync(stream), "xme_stream_sync")
+    return d_res.download(n, DUAL_RESULT_DTYPE)
 
 
 def _bipred_batch(name, dtype, params, jobs, cur, ref1, ref2, stream):
