@@ -351,6 +351,115 @@ def bipred_leg(X, w, h, R, lam, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, re
     out["workload"] = (f"{n} PUs of the frame, references 0 and 1; sym: start = 4 x full-pel best of reference 0, "
                        "distances 2 / 4 (F frame); bid: forward / backward = 4 x full-pel bests of references "
                        "0 / 1; MVP (0,0), SATD, quarter-pel")
+    out["callers"] = callers_leg(X, w, h, R, lam, arr, gpu_res, n_ref, dcur, d_refs, refs, cp, planes, reps, cpu,
+                                 stream)
+    return out
+
+
+def callers_leg(X, w, h, R, lam, arr, gpu_res, n_ref, dcur, d_refs, refs, cp, planes, reps, cpu, stream):
+    """The callers of the refinements over the frame's PUs, LCU level 6, one
+    frame thread: pred_inter_search_bi (xme_pred_inter_search_bi_batch, a B
+    frame between references 0 and 1 at distances 2 / 2: the gates, the
+    symmetric search from 4 x the full-pel best of reference 0 and the
+    bi-directional one with the backward MV 4 x the full-pel best of
+    reference 1) and pred_inter_search_dual (xme_pred_inter_search_dual_batch,
+    an F frame over every reference at distances 1, 2, ...: per reference the
+    gates and the symmetric search from its full-pel best, plus REF_COST).
+    MVPs (0,0).  Parity with the oracle and the CPU time on every 8th PU."""
+    import ctypes as C
+    from oracle import oracle as O
+    n = len(arr)
+    bi = np.zeros(n, dtype=X.BI_JOB_DTYPE)
+    dual = np.zeros(n, dtype=X.DUAL_JOB_DTYPE)
+    for jobs in (bi, dual):
+        jobs["x"], jobs["y"], jobs["w"], jobs["h"] = arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3]
+    r0, r1 = gpu_res[0::n_ref], gpu_res[1::n_ref]
+    bi["fwd_imv_x"], bi["fwd_imv_y"] = r0["bmv2_x"], r0["bmv2_y"]
+    bi["bwd_x"], bi["bwd_y"] = 4 * r1["bmv2_x"], 4 * r1["bmv2_y"]
+    nd = min(n_ref, 4)
+    dist = list(range(1, nd + 1))
+    for r in range(nd):
+        rr = gpu_res[r::n_ref]
+        dual["imv"][:, r, 0], dual["imv"][:, r, 1] = rr["bmv2_x"], rr["bmv2_y"]
+    prm = X.me_params(w, h, R, lam)
+    L = X.lib()
+    d_bi, d_dual = X.upload(bi), X.upload(dual)
+    d_bres = X.DeviceBuffer(n * X.BI_RESULT_DTYPE.itemsize)
+    d_dres = X.DeviceBuffer(n * X.DUAL_RESULT_DTYPE.itemsize)
+    r01 = (d_refs[0].desc(), d_refs[1].desc())
+    descs = (X.RefDesc * 4)(*[r.desc() for r in d_refs[:nd]])
+    dists = (C.c_int * 4)(*(dist + [0] * (4 - nd)))
+
+    def run_bi():
+        X.check(L.xme_pred_inter_search_bi_batch(C.byref(prm), 6, 1, C.c_void_p(d_bi.ptr), n, 2, 2,
+                                                 C.c_void_p(dcur.origin), dcur.stride, C.byref(r01[0]),
+                                                 C.byref(r01[1]), C.c_void_p(d_bres.ptr), stream.h), "bi caller")
+
+    def run_dual():
+        X.check(L.xme_pred_inter_search_dual_batch(C.byref(prm), 6, 1, C.c_void_p(d_dual.ptr), n, nd, dists,
+                                                   C.c_void_p(dcur.origin), dcur.stride, descs,
+                                                   C.c_void_p(d_dres.ptr), stream.h), "dual caller")
+
+    out = {}
+    for name, run, d_res, dtype, points in (("bi", run_bi, d_bres, X.BI_RESULT_DTYPE, 26),
+                                            ("dual", run_dual, d_dres, X.DUAL_RESULT_DTYPE, 9 * nd)):
+        for _ in range(3):
+            run()
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        stream.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        res = d_res.download(n, dtype)
+        leg = {"ms_per_batch": ms, "pus_per_s": n / (ms * 1e-3), "searched_points_per_pu_at_most": points,
+               "status_ok": bool(np.all(res["status"] == 0))}
+        if name == "bi":
+            leg["sym_ran"] = int(np.sum(res["sym_mvcost"] >= 0))
+            leg["bid_ran"] = int(np.sum(res["bid_mvcost"] >= 0))
+        else:
+            leg["references"] = nd
+            leg["winners"] = {str(r): int(np.sum(res["fst_ref"] == r)) for r in range(-1, nd)}
+        if cpu:
+            idx = range(0, n, 8)
+            prm_o = O.me_param(w, h, R, lam)
+            t0 = time.perf_counter()
+            if name == "bi":
+                exp = [O.pred_inter_search_bi(prm_o, {"pix_x": int(bi["x"][i]), "pix_y": int(bi["y"][i]),
+                                                      "w": int(bi["w"][i]), "h": int(bi["h"][i]),
+                                                      "fwd_imv_x": int(bi["fwd_imv_x"][i]),
+                                                      "fwd_imv_y": int(bi["fwd_imv_y"][i]),
+                                                      "bwd_x": int(bi["bwd_x"][i]), "bwd_y": int(bi["bwd_y"][i]),
+                                                      "mvp_x": 0, "mvp_y": 0, "mvp1_x": 0, "mvp1_y": 0,
+                                                      "mvp2_x": 0, "mvp2_y": 0}, 2, 2, cp, planes[0], planes[1])
+                       for i in idx]
+                got = [tuple(int(res[f][i]) for f in ("sym_cost", "sym_mv_x", "sym_mv_y", "sym_mvcost",
+                                                      "bid_cost", "fwd_x", "fwd_y", "bwd_x", "bwd_y",
+                                                      "bid_mvcost")) for i in idx]
+                exp = [tuple(e[f] for f in ("sym_cost", "sym_mv_x", "sym_mv_y", "sym_mvcost", "bid_cost", "fwd_x",
+                                            "fwd_y", "bwd_x", "bwd_y", "bid_mvcost")) for e in exp]
+            else:
+                for r in refs[len(planes):nd]:
+                    p = O.Plane(w, h)
+                    p.set_picture(r)
+                    p.expand_border()
+                    planes.append(O.interp_planes(p, 2))
+                exp = [O.pred_inter_search_dual(prm_o, {"pix_x": int(dual["x"][i]), "pix_y": int(dual["y"][i]),
+                                                        "w": int(dual["w"][i]), "h": int(dual["h"][i]),
+                                                        "mvp_x": 0, "mvp_y": 0, "mvp1": [(0, 0)] * nd,
+                                                        "imv": [tuple(int(v) for v in dual["imv"][i, r])
+                                                                for r in range(nd)]},
+                                                dist, cp, planes[:nd]) for i in idx]
+                got = [(int(res["cost"][i]), int(res["fst_ref"][i]), int(res["snd_ref"][i]), int(res["mvcost"][i]),
+                        [tuple(int(v) for v in res["fst"][i, r]) for r in range(nd)],
+                        [tuple(int(v) for v in res["snd"][i, r]) for r in range(nd)]) for i in idx]
+                exp = [tuple(e) for e in exp]
+            el = time.perf_counter() - t0
+            leg["parity_with_oracle_every_8th"] = got == exp
+            leg["cpu_oracle"] = {"pus_per_s": len(exp) / el, "threads": 1, "seconds": el,
+                                 "sample": f"every 8th PU ({len(exp)}), one thread, Python loop over the C oracle"}
+        log(f"{name} caller: {ms:.3f} ms per batch of {n} PUs")
+        out[name] = leg
     return out
 
 

@@ -34,6 +34,17 @@ constexpr int SPW = (TX + 8) / 4;   // staged dwords per row
 
 __device__ __forceinline__ uint32_t clip_pel(int v) { return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v)); }
 
+// Four clipped pixels (each 0..255 in an int) packed into one dword, byte i
+// from p[i], with v_perm_b32.  Written as shifts and ors, the clips and the
+// packing compile to v_ashr_pk_u8_i32; a build of the diagonal planes in that
+// form gave wrong pixels (the cause was not isolated), so every plane packs
+// this way.
+__device__ __forceinline__ uint32_t pack_pels(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3)
+{
+    return __builtin_amdgcn_perm(__builtin_amdgcn_perm(p3, p2, 0x0C0C0400u), __builtin_amdgcn_perm(p1, p0, 0x0C0C0400u),
+                                 0x05040100u);
+}
+
 // Taps of a phase packed as two dwords of signed bytes (taps 0-3, 4-7).
 __device__ __forceinline__ void packed_taps(int d, int &lo, int &hi)
 {
@@ -127,11 +138,11 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
     for (int dx = 1; dx <= 3; dx++) {
         if (!active || (frac < 2 && dx != 2))
             continue;
-        uint32_t v = 0;
+        uint32_t px[4];
 #pragma unroll
         for (int i = 0; i < 4; i++)
-            v |= clip_pel((sH[dx - 1][ty + 3][c0 + i] + 32) >> 6) << (8 * i);
-        *reinterpret_cast<uint32_t *>(pl.p[dx] + off) = v;
+            px[i] = clip_pel((sH[dx - 1][ty + 3][c0 + i] + 32) >> 6);
+        *reinterpret_cast<uint32_t *>(pl.p[dx] + off) = pack_pels(px[0], px[1], px[2], px[3]);
     }
     // vertical d, h, n: pixels of rows y - 3 .. y + 4 at columns x .. x + 3,
     // i.e. staged dword (c0 + 4) / 4 of rows ty .. ty + 7, transposed to columns
@@ -144,11 +155,11 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
         for (int dy = 1; dy <= 3; dy++) {
             if (frac < 2 && dy != 2)
                 continue;
-            uint32_t v = 0;
+            uint32_t px[4];
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                v |= clip_pel((fir8(tl[dy], th[dy], colA[i], colB[i]) + 32) >> 6) << (8 * i);
-            *reinterpret_cast<uint32_t *>(pl.p[dy << 2] + off) = v;
+                px[i] = clip_pel((fir8(tl[dy], th[dy], colA[i], colB[i]) + 32) >> 6);
+            *reinterpret_cast<uint32_t *>(pl.p[dy << 2] + off) = pack_pels(px[0], px[1], px[2], px[3]);
         }
     }
     // diagonal e..r from the horizontal sums (MC_SHIFT = 12): a thread takes one
@@ -193,12 +204,7 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
                         s = __builtin_amdgcn_sdot2((j & 1) ? O[(j >> 1) + m] : E[(j >> 1) + m], T[m], s, false);
                     px[j] = clip_pel(s >> 12);
                 }
-                // bytes packed with v_perm_b32: written as shifts and ors, the
-                // clips and packing become v_ashr_pk_u8_i32, which gave wrong
-                // pixels here
-                const uint32_t col = __builtin_amdgcn_perm(__builtin_amdgcn_perm(px[3], px[2], 0x0C0C0400u),
-                                                           __builtin_amdgcn_perm(px[1], px[0], 0x0C0C0400u),
-                                                           0x05040100u);
+                const uint32_t col = pack_pels(px[0], px[1], px[2], px[3]);
                 const uint32_t d0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0x00, 0xF, 0xF, false);
                 const uint32_t d1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0x55, 0xF, 0xF, false);
                 const uint32_t d2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)col, 0xAA, 0xF, 0xF, false);

@@ -13,6 +13,8 @@
 
 namespace {
 
+typedef short short2v __attribute__((ext_vector_type(2)));
+
 // AVS2 DCT basis T_32 (transform.c:59-135, derived from its cosine closed
 // form); T_N[k][n] = T_32[k * 32 / N][n].
 __constant__ int8_t g_pc_t32[32 * 32];
@@ -37,11 +39,14 @@ __device__ __forceinline__ int clip3i(int lo, int hi, int v) { return v < lo ? l
 // writing the n / 2 low-band samples to out (stride ostep): mirror two
 // samples a side, odd samples minus the floor mean of their neighbours, then
 // the even ones plus (l + r + 2) >> 2 (rows) or doubled plus (l + r + 1) >> 1
-// (columns); int16 stores.
+// (columns); every stored sample is truncated to int16 as the reference's
+// coeff_t array keeps it.  The work array holds int: with int16 storage the
+// compiler turns the averages into packed 16-bit arithmetic, which this
+// library keeps out of its kernels.
 __device__ void lift_fwd(const int16_t *x, int step, int16_t *out, int ostep, int n, bool columns)
 {
-    int16_t e[68];
-    int16_t *v = e + 2;
+    int e[68];
+    int *v = e + 2;
     for (int i = 0; i < n; i++)
         v[i] = x[i * step];
     v[-1] = v[1];
@@ -54,7 +59,7 @@ __device__ void lift_fwd(const int16_t *x, int step, int16_t *out, int ostep, in
         v[i] = columns ? (int16_t)((v[i] << 1) + ((v[i - 1] + v[i + 1] + 1) >> 1))
                        : (int16_t)(v[i] + ((v[i - 1] + v[i + 1] + 2) >> 2));
     for (int i = 0; i < n / 2; i++)
-        out[i * ostep] = v[2 * i];
+        out[i * ostep] = (int16_t)v[2 * i];
 }
 
 // One inverse lifting pass: n / 2 low samples to the even places, the last
@@ -62,7 +67,7 @@ __device__ void lift_fwd(const int16_t *x, int step, int16_t *out, int ostep, in
 // samples are the floor mean of their neighbours.
 __device__ void lift_inv(const int16_t *x, int step, int16_t *out, int ostep, int n, bool columns)
 {
-    int16_t e[66];
+    int e[66];      // int16 values held in int (as in lift_fwd)
     for (int i = 0; i < n / 2; i++)
         e[2 * i] = x[i * step];
     e[n] = e[n - 2];
@@ -72,7 +77,7 @@ __device__ void lift_inv(const int16_t *x, int step, int16_t *out, int ostep, in
     for (int i = 1; i < n; i += 2)
         e[i] = (int16_t)((e[i - 1] + e[i + 1]) >> 1);
     for (int i = 0; i < n; i++)
-        out[i * ostep] = e[i];
+        out[i * ostep] = (int16_t)e[i];
 }
 
 struct PcXform {
@@ -230,15 +235,23 @@ __global__ void __launch_bounds__(256) pc_intpl_kernel(PcIntpl p, const uint8_t 
         v = (v + 2048) >> 12;
         break;
     }
-    case kPcLumaBlockExt:
+    case kPcLumaBlockExt: {
+        // the row sums as v_dot2_i32_i16 on (pixel, pixel) and (tap, tap)
+        // pairs: an int result the compiler cannot narrow to packed 16-bit
+        // arithmetic when it is truncated to the mct_t intermediate
+        short2v t[4];
+        for (int k = 0; k < 4; k++)
+            t[k] = short2v{ (short)p.ch[2 * k], (short)p.ch[2 * k + 1] };
         for (int r = 0; r < 8; r++) {
+            const uint8_t *row = s + (r - 3) * p.i_src - 3;
             int u = 0;
-            for (int k = 0; k < 8; k++)
-                u += p.ch[k] * s[(r - 3) * p.i_src + k - 3];
-            v += p.cv[r] * (int16_t)u;          // the intermediate is mct_t
+            for (int k = 0; k < 4; k++)
+                u = __builtin_amdgcn_sdot2(short2v{ (short)row[2 * k], (short)row[2 * k + 1] }, t[k], u, false);
+            v += p.cv[r] * (int)(int16_t)u;     // the intermediate is mct_t
         }
         v = (v + 2048) >> 12;
         break;
+    }
     case kPcChromaHor:
         for (int k = 0; k < 4; k++)
             v += p.ch[k] * s[k - 1];
