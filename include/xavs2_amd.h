@@ -558,6 +558,24 @@ int xme_expand_border(pel_t *d_plane, intptr_t stride, int w, int h, int pad, vo
  * XME_PAD border as d_planes[0], which must already be border-expanded). */
 int xme_interpolate_frame(pel_t *const d_planes[16], intptr_t stride, int w, int h,
                           int use_fractional_me, void *stream);
+/* The same, one LCU row at a time, for an encoder that keeps its reference
+ * planes resident in device memory and finishes a picture row by row
+ * (source/encoder/slice.c:535-543): xavs2_frame_expand_border_lcurow for one
+ * plane (source/common/frame.c:537-575: rows [i << row_log2, (i + 1) <<
+ * row_log2), 4 rows up except at the top and bottom of the picture, with the
+ * left / right pads, and the band above the first row / below the last;
+ * row_log2 = lcu_level for luma, lcu_level - 1 for 4:2:0 chroma), and
+ * interpolate_lcu_row (source/common/mc.c:748-785: planes 1..15 on rows
+ * [(i << lcu_level) - 8, ((i + 1) << lcu_level) - 8), the first row from -4,
+ * the last to h + 4, with their pads).  Called in row order after row i of
+ * the picture is final, each leaves every plane equal to xme_expand_border /
+ * xme_interpolate_frame on all rows above its last one; interpolate_lcu_row
+ * reads the full-pel plane only up to 4 rows below its last one, which the
+ * expansion of the same LCU row has completed.  One slice (slice_num 1). */
+int xme_expand_border_lcu_row(pel_t *d_plane, intptr_t stride, int w, int h, int pad, int row_log2, int lcu_y,
+                              void *stream);
+int xme_interpolate_lcu_row(pel_t *const d_planes[16], intptr_t stride, int w, int h, int use_fractional_me,
+                            int lcu_level, int lcu_y, void *stream);
 /* mc_chroma (source/common/mc.c:797-829) for a batch of blocks: block i has
  * chroma size w[i] x h[i] at absolute position (qx[i], qy[i]) = 4 * luma
  * position + MV (qpel), clamped by get_mv_for_mc (mc.h:69-75) when

@@ -157,6 +157,10 @@ def lib():
         L.xo_me_search_ctx.argtypes = [C.POINTER(MeParam), C.POINTER(Pu), P, P, ip, P, ip, P]
         L.xo_me_search_batch_ctx.argtypes = [C.POINTER(MeParam), P, P, C.c_int, P, ip, P, ip, P, C.c_int]
         L.xo_expand_border.argtypes = [P, ip, C.c_int, C.c_int, C.c_int]
+        L.xo_expand_border_lcu_row.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_interpolate_lcu_row.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_lcu_row_window.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.xo_border_row_window.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.xo_interpolate_frame.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_mc_luma.argtypes = [P, ip, C.c_int, C.c_int, C.c_int, C.c_int, P, ip]
         L.xo_mc_chroma.argtypes = [P, P, ip, C.c_int, C.c_int, C.c_int, C.c_int, P, P, ip]
@@ -281,6 +285,31 @@ def interp_planes(ref_plane, use_fractional_me=2, lcu_level=6):
                                     use_fractional_me)
     assert rc == 0
     return planes
+
+
+def lcu_row_window(h, lcu_level, lcu_y):
+    """Rows [y0, y1) interpolate_lcu_row produces for LCU row lcu_y (one slice)."""
+    y0, y1 = C.c_int(), C.c_int()
+    lib().xo_lcu_row_window(h, lcu_level, lcu_y, C.byref(y0), C.byref(y1))
+    return y0.value, y1.value
+
+
+def border_row_window(h, row_log2, lcu_y):
+    """Rows [y0, y1) xavs2_frame_expand_border_lcurow expands for LCU row lcu_y."""
+    y0, y1 = C.c_int(), C.c_int()
+    lib().xo_border_row_window(h, row_log2, lcu_y, C.byref(y0), C.byref(y1))
+    return y0.value, y1.value
+
+
+def expand_border_lcu_row(plane, row_log2, lcu_y):
+    lib().xo_expand_border_lcu_row(plane.origin_ptr(), plane.stride, plane.w, plane.h, plane.pad, row_log2, lcu_y)
+
+
+def interpolate_lcu_row(planes, lcu_level, lcu_y, use_fractional_me=2):
+    """interpolate_lcu_row on a list of 16 Planes (index 0 the full-pel one)."""
+    p0 = planes[0]
+    arr = (C.c_void_p * 16)(*[(p.origin_ptr().value if p is not None else None) for p in planes])
+    assert lib().xo_interpolate_lcu_row(arr, p0.stride, p0.w, p0.h, lcu_level, lcu_y, use_fractional_me) == 0
 
 
 def plane_ptrs(planes):

@@ -1554,6 +1554,29 @@ void xo_expand_border(uint8_t *plane, intptr_t stride, int w, int h, int pad)
     expand_border(plane, stride, w, h, pad, pad, 1, 1);
 }
 
+/* The rows xavs2_frame_expand_border_lcurow (frame.c:537-575) expands for
+ * LCU row lcu_y of a plane h rows high, one slice: [lcu_y << row_log2,
+ * (lcu_y + 1) << row_log2), each end UP_SHIFT = 4 rows up except at the
+ * picture's first / last LCU row, the end clipped to h. */
+void xo_border_row_window(int h, int row_log2, int lcu_y, int *y0, int *y1)
+{
+    const int rows = (h + (1 << row_log2) - 1) >> row_log2;
+    const int first = lcu_y == 0, last = lcu_y == rows - 1;
+    *y0 = (lcu_y << row_log2) - (first ? 0 : 4);
+    *y1 = ((lcu_y + 1) << row_log2) - (last ? 0 : 4);
+    if (*y1 > h)
+        *y1 = h;
+}
+
+void xo_expand_border_lcu_row(uint8_t *plane, intptr_t stride, int w, int h, int pad, int row_log2, int lcu_y)
+{
+    const int rows = (h + (1 << row_log2) - 1) >> row_log2;
+    int y0, y1;
+    xo_border_row_window(h, row_log2, lcu_y, &y0, &y1);
+    if (y1 > y0)
+        expand_border(plane + y0 * stride, stride, w, y1 - y0, pad, pad, lcu_y == 0, lcu_y == rows - 1);
+}
+
 /* Luma 8-tap and chroma 4-tap phases, common/mc.c:73-91. */
 const int8_t xo_luma_taps[4][8] = {
     { 0, 0, 0, 64, 0, 0, 0, 0 },
@@ -1635,8 +1658,20 @@ static void interp_rows(uint8_t *const planes[16], intptr_t stride, int w,
     }
 }
 
-int xo_interpolate_frame(uint8_t *const planes_in[16], intptr_t stride, int w, int h,
-                         int lcu_level, int use_fractional_me)
+/* The rows interpolate_lcu_row (mc.c:748-785) produces for LCU row lcu_y,
+ * one slice: [lcu_y << lcu_level, (lcu_y + 1) << lcu_level) moved up by
+ * MC_OFFSET = 8 (mc.c:95), the first row from -PAD_OFFSET = -4 and the last
+ * to h + MC_OFFSET - PAD_OFFSET = h + 4. */
+void xo_lcu_row_window(int h, int lcu_level, int lcu_y, int *y0, int *y1)
+{
+    const int lcu = 1 << lcu_level, rows = (h + lcu - 1) / lcu;
+    const int b_start = lcu_y == 0, b_end = lcu_y == rows - 1;
+    *y0 = lcu_y * lcu - (b_start ? 4 : 8);
+    *y1 = b_end ? h + 8 - 4 : (lcu_y + 1) * lcu - 8;
+}
+
+static int interpolate_lcu_rows(uint8_t *const planes_in[16], intptr_t stride, int w, int h, int lcu_level,
+                                int row_first, int row_end, int use_fractional_me)
 {
     if (!use_fractional_me)
         return 0;
@@ -1656,18 +1691,28 @@ int xo_interpolate_frame(uint8_t *const planes_in[16], intptr_t stride, int w, i
     for (int k = 0; k < 3; k++)
         tmp_org[k] = buf + k * tmp_sz + XO_PAD * i_tmp + XO_PAD;
 
-    /* interpolate_lcu_row, mc.c:748-785 (one slice): MC_OFFSET = 8. */
+    /* interpolate_lcu_row, mc.c:748-785 (one slice) */
     const int lcu = 1 << lcu_level;
     const int rows = (h + lcu - 1) / lcu;
-    for (int r = 0; r < rows; r++) {
-        int b_start = r == 0, b_end = r == rows - 1;
-        int y0 = r * lcu, y1 = (r + 1) * lcu;
-        y0 -= b_start ? 4 : 8;
-        y1 = b_end ? h + 8 - 4 : y1 - 8;
-        interp_rows(planes, stride, w, tmp_org, i_tmp, y0, y1 - y0, b_start, b_end, use_fractional_me);
+    for (int r = row_first; r < row_end && r < rows; r++) {
+        int y0, y1;
+        xo_lcu_row_window(h, lcu_level, r, &y0, &y1);
+        interp_rows(planes, stride, w, tmp_org, i_tmp, y0, y1 - y0, r == 0, r == rows - 1, use_fractional_me);
     }
     free(buf);
     return 0;
+}
+
+int xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int h, int lcu_level,
+                         int use_fractional_me)
+{
+    return interpolate_lcu_rows(planes, stride, w, h, lcu_level, 0, 1 << 30, use_fractional_me);
+}
+
+int xo_interpolate_lcu_row(uint8_t *const planes[16], intptr_t stride, int w, int h, int lcu_level, int lcu_y,
+                           int use_fractional_me)
+{
+    return interpolate_lcu_rows(planes, stride, w, h, lcu_level, lcu_y, lcu_y + 1, use_fractional_me);
 }
 
 /* Block filters, mc.c:275-398 (8-bit: first-pass shift 0, second >> 12). */

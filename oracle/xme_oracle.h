@@ -294,6 +294,16 @@ void xo_expand_border(uint8_t *plane, intptr_t stride, int w, int h, int pad);
  * same stride and padding. */
 int  xo_interpolate_frame(uint8_t *const planes[16], intptr_t stride, int w, int h,
                           int lcu_level, int use_fractional_me);
+/* One LCU row of it (interpolate_lcu_row, mc.c:748-785, one slice), and the
+ * rows [*y0, *y1) that row produces; the horizontal planes a, b, c are also
+ * written 4 rows above and below, as the reference's horizontal pass does. */
+int  xo_interpolate_lcu_row(uint8_t *const planes[16], intptr_t stride, int w, int h, int lcu_level, int lcu_y,
+                            int use_fractional_me);
+void xo_lcu_row_window(int h, int lcu_level, int lcu_y, int *y0, int *y1);
+/* xavs2_frame_expand_border_lcurow for one plane (frame.c:537-575, one
+ * slice; row_log2 = lcu_level, less 1 for 4:2:0 chroma), and its rows. */
+void xo_expand_border_lcu_row(uint8_t *plane, intptr_t stride, int w, int h, int pad, int row_log2, int lcu_y);
+void xo_border_row_window(int h, int row_log2, int lcu_y, int *y0, int *y1);
 
 /* Luma inter reconstruction of N x N TUs: the prediction error (sub_ps,
  * rdo.c:1751-1753), tu_recon_inter_luma (rdo.c:1333-1366: DCT, inter quant

@@ -12,6 +12,8 @@
 // then fills the 76-pixel frame around it.  The horizontal sums of 8-bit
 // pictures lie in [-4845, 22440], so the reference's int16 temporaries
 // (mct_t, mc.c:466-496) hold them without wrapping and int32 is equivalent.
+#include <algorithm>
+
 #include "xme_internal.h"
 
 namespace {
@@ -74,19 +76,22 @@ __device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2
     col[3] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// Interpolated planes on the core area x in [-4, W+4), y in [-4, H+4), in
-// tiles whose columns start at xs <= -4 (a 64-byte boundary of the output
-// rows); columns left of -4 are written too and left to the border pass.  The
+// Interpolated planes on the core columns x in [-4, W+4) of rows [ys, ye)
+// (the whole core area is ys = -4, ye = H + 4), in tiles whose columns start
+// at xs <= -4 (a 64-byte boundary of the output rows); columns left of -4 are
+// written too and left to the border pass.  The staged rows below ye + 4 may
+// not be final yet (an LCU row still being reconstructed): they only feed
+// outputs at or below ye, which are not stored.  The
 // full-pel tile (with its 3 + 4 pixel filter margins) is staged in LDS as
 // signed bytes (pixel - 128); every 8-tap sum is two v_dot4_i32_i8 with an
 // exact 32-bit result, and the horizontal sums feeding the diagonal planes
 // are kept in LDS as int32.
 __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t stride, int W, int H, int frac,
-                                                          int xs)
+                                                          int xs, int ys, int ye)
 {
     __shared__ uint32_t sP[TY + 7][SPW];
     __shared__ int32_t sH[3][TY + 7][TX];
-    const int x0 = xs + (int)blockIdx.x * TX, y0 = -4 + (int)blockIdx.y * TY;
+    const int x0 = xs + (int)blockIdx.x * TX, y0 = ys + (int)blockIdx.y * TY;
     const uint8_t *P = pl.p[0];
     const int tid = threadIdx.x;
     // stage: column x0 - 4 + 4q .. +3 of rows y0 - 3 .. y0 + TY + 3 (dword aligned
@@ -131,7 +136,7 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
     __syncthreads();
     const int ty = tid >> 4, c0 = (tid & 15) * 4;
     const int y = y0 + ty, x = x0 + c0;
-    const bool active = y < H + 4 && x < W + 4;
+    const bool active = y < ye && x < W + 4;
     const int64_t off = (int64_t)y * stride + x;
     // horizontal a, b, c
 #pragma unroll
@@ -171,7 +176,7 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
     {
         const int c = tid & 63, g = tid >> 6, i4 = c & 3;
         const int yy = y0 + 4 * g + i4, xx = x0 + (c & ~3);
-        const bool st = yy < H + 4 && xx < W + 4;
+        const bool st = yy < ye && xx < W + 4;
         const uint32_t sel = (uint32_t)i4 | (uint32_t)(4 + i4) << 8 | 0x0C0C0000u;   // byte i4 of two dwords
         for (int dx = 1; dx <= 3; dx++) {
             if (frac < 2 && dx != 2)
@@ -221,8 +226,9 @@ __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t s
 // Fill [-pad, W+pad) x [-pad, H+pad) outside the core rectangle [cx0, cx1) x
 // [cy0, cy1) with the nearest core value.  The grid covers only the border's
 // dwords (core edges and pad are dword aligned), in four regions laid end to
-// end: the bands above and below the core (full width), then the strips left
-// and right of it (core rows).  blockIdx.y selects the plane.
+// end: the bands above and below the core (full width; either may be left
+// out, as for an LCU row inside the picture), then the strips left and right
+// of the core rows.  blockIdx.y selects the plane.
 struct BorderGeom {
     int64_t stride;
     int pad, W, H, cx0, cy0, cx1, cy1;
@@ -264,9 +270,9 @@ __global__ void __launch_bounds__(256) border_kernel(Planes16 pl, int first, Bor
 }
 
 // The border of `planes` planes from pl.p[first]; every edge and the pad are
-// multiples of 4.
+// multiples of 4.  top / bottom: fill the band above cy0 / below cy1.
 int launch_border(const Planes16 &pl, int first, int planes, int64_t stride, int W, int H, int pad, int cx0,
-                  int cy0, int cx1, int cy1, hipStream_t s)
+                  int cy0, int cx1, int cy1, hipStream_t s, bool top = true, bool bottom = true)
 {
     BorderGeom g;
     g.stride = stride;
@@ -280,8 +286,8 @@ int launch_border(const Planes16 &pl, int first, int planes, int64_t stride, int
     g.wdw = (W + 2 * pad) >> 2;
     g.ldw = (cx0 + pad) >> 2;
     g.rdw = (W + pad - cx1) >> 2;
-    g.n_top = (int64_t)(cy0 + pad) * g.wdw;
-    g.n_bot = (int64_t)(H + pad - cy1) * g.wdw;
+    g.n_top = top ? (int64_t)(cy0 + pad) * g.wdw : 0;
+    g.n_bot = bottom ? (int64_t)(H + pad - cy1) * g.wdw : 0;
     g.n_side = (int64_t)(cy1 - cy0) * (g.ldw + g.rdw);
     const int64_t n = g.n_top + g.n_bot + g.n_side;
     if (n <= 0)
@@ -289,6 +295,41 @@ int launch_border(const Planes16 &pl, int first, int planes, int64_t stride, int
     hipLaunchKernelGGL(border_kernel, dim3((unsigned)((n + 255) / 256), planes), dim3(256), 0, s, pl, first, g);
     XME_HIP(hipGetLastError());
     return XME_OK;
+}
+
+// Planes 1..15 on the core columns of rows [ys, ye), then their border:
+// the side strips of those rows, and the bands above / below when top /
+// bottom.
+int interp_rows(const Planes16 &pl, int64_t stride, int w, int h, int frac, int ys, int ye, bool top, bool bottom,
+                hipStream_t s)
+{
+    // tile columns start where the output planes' rows reach a 64-byte
+    // boundary, at or up to 60 columns left of the core's first column -4, so
+    // that every 64-column row segment is one whole 64-byte line; the columns
+    // left of -4 are overwritten by the border pass
+    const int xs = -4 - (int)((reinterpret_cast<uintptr_t>(pl.p[2]) - 4) & 63);
+    if (ye > ys) {
+        dim3 grid((w + 4 - xs + TX - 1) / TX, (ye - ys + TY - 1) / TY);
+        hipLaunchKernelGGL(interp_core_kernel, grid, dim3(256), 0, s, pl, stride, w, h, frac, xs, ys, ye);
+        XME_HIP(hipGetLastError());
+    }
+    return launch_border(pl, 1, 15, stride, w, h, XME_PAD, -4, ys, w + 4, ye, s, top, bottom);
+}
+
+// The planes a caller passes: 16 device origins, of which frac 1 needs 0, 2,
+// 8 and 10 (frame.c:343-350); false (with the error set) when one is missing
+// or unaligned.
+bool pick_planes(pel_t *const d_planes[16], int frac, const char *who, Planes16 &pl)
+{
+    for (int k = 0; k < 16; k++) {
+        const bool need = k == 0 || frac == 2 || k == 2 || k == 8 || k == 10;
+        pl.p[k] = need ? d_planes[k] : nullptr;
+        if (need && (!pl.p[k] || (reinterpret_cast<uintptr_t>(pl.p[k]) & 3))) {
+            xme_set_error("%s: plane %d missing or not 4-byte aligned", who, k);
+            return false;
+        }
+    }
+    return true;
 }
 
 } // namespace
@@ -318,22 +359,59 @@ extern "C" int xme_interpolate_frame(pel_t *const d_planes[16], intptr_t stride,
     if (frac == 0)
         return XME_OK;
     Planes16 pl;
-    for (int k = 0; k < 16; k++) {
-        const bool need = k == 0 || frac == 2 || k == 2 || k == 8 || k == 10;
-        pl.p[k] = need ? d_planes[k] : nullptr;
-        if (need && (!pl.p[k] || (reinterpret_cast<uintptr_t>(pl.p[k]) & 3))) {
-            xme_set_error("xme_interpolate_frame: plane %d missing or not 4-byte aligned", k);
-            return XME_ERR_ARG;
-        }
+    if (!pick_planes(d_planes, frac, "xme_interpolate_frame", pl))
+        return XME_ERR_ARG;
+    return interp_rows(pl, (int64_t)stride, w, h, frac, -4, h + 4, true, true, (hipStream_t)stream);
+}
+
+extern "C" int xme_expand_border_lcu_row(pel_t *d_plane, intptr_t stride, int w, int h, int pad, int row_log2,
+                                         int lcu_y, void *stream)
+{
+    if (!d_plane || w < 1 || h < 1 || (w & 3) || (pad & 3) || pad < 0 || stride < w + 2 * pad ||
+        (reinterpret_cast<uintptr_t>(d_plane) & 3) || (stride & 3) || row_log2 < 2 || row_log2 > 7) {
+        xme_set_error("xme_expand_border_lcu_row: need 4-aligned plane, stride, width and pad, row_log2 2..7");
+        return XME_ERR_ARG;
     }
-    hipStream_t s = (hipStream_t)stream;
-    // tile columns start where the output planes' rows reach a 64-byte
-    // boundary, at or up to 60 columns left of the core's first column -4, so
-    // that every 64-column row segment is one whole 64-byte line; the columns
-    // left of -4 are overwritten by the border pass
-    const int xs = -4 - (int)((reinterpret_cast<uintptr_t>(pl.p[2]) - 4) & 63);
-    dim3 grid((w + 4 - xs + TX - 1) / TX, (h + 8 + TY - 1) / TY);
-    hipLaunchKernelGGL(interp_core_kernel, grid, dim3(256), 0, s, pl, (int64_t)stride, w, h, frac, xs);
-    XME_HIP(hipGetLastError());
-    return launch_border(pl, 1, 15, (int64_t)stride, w, h, XME_PAD, -4, -4, w + 4, h + 4, s);
+    const int n_rows = (h + (1 << row_log2) - 1) >> row_log2;
+    if (lcu_y < 0 || lcu_y >= n_rows) {
+        xme_set_error("xme_expand_border_lcu_row: LCU row %d outside 0..%d", lcu_y, n_rows - 1);
+        return XME_ERR_ARG;
+    }
+    // xavs2_frame_expand_border_lcurow (frame.c:537-575), one slice: rows
+    // [y0, y1), UP_SHIFT 4 rows up except at the picture's top and bottom
+    const bool first = lcu_y == 0, last = lcu_y == n_rows - 1;
+    const int y0 = (lcu_y << row_log2) - (first ? 0 : 4);
+    const int y1 = std::min(h, ((lcu_y + 1) << row_log2) - (last ? 0 : 4));
+    if (pad == 0 || y1 <= y0)
+        return XME_OK;
+    Planes16 pl = {};
+    pl.p[0] = d_plane;
+    return launch_border(pl, 0, 1, (int64_t)stride, w, h, pad, 0, y0, w, y1, (hipStream_t)stream, first, last);
+}
+
+extern "C" int xme_interpolate_lcu_row(pel_t *const d_planes[16], intptr_t stride, int w, int h, int frac,
+                                       int lcu_level, int lcu_y, void *stream)
+{
+    if (!d_planes || !d_planes[0] || w < 8 || h < 8 || (w & 7) || frac < 0 || frac > 2 || (stride & 3) ||
+        stride < w + 2 * XME_PAD || lcu_level < 3 || lcu_level > 6) {
+        xme_set_error("xme_interpolate_lcu_row: need width multiple of 8, stride >= width + %d, multiple of 4, "
+                      "lcu_level 3..6", 2 * XME_PAD);
+        return XME_ERR_ARG;
+    }
+    const int n_rows = (h + (1 << lcu_level) - 1) >> lcu_level;
+    if (lcu_y < 0 || lcu_y >= n_rows) {
+        xme_set_error("xme_interpolate_lcu_row: LCU row %d outside 0..%d", lcu_y, n_rows - 1);
+        return XME_ERR_ARG;
+    }
+    if (frac == 0)
+        return XME_OK;
+    Planes16 pl;
+    if (!pick_planes(d_planes, frac, "xme_interpolate_lcu_row", pl))
+        return XME_ERR_ARG;
+    // interpolate_lcu_row (mc.c:748-785), one slice: rows [y0, y1) with
+    // PAD_OFFSET 4 and MC_OFFSET 8 (mc.c:95-96)
+    const bool first = lcu_y == 0, last = lcu_y == n_rows - 1;
+    const int y0 = (lcu_y << lcu_level) - (first ? 4 : 8);
+    const int y1 = last ? h + 4 : ((lcu_y + 1) << lcu_level) - 8;
+    return interp_rows(pl, (int64_t)stride, w, h, frac, y0, y1, first, last, (hipStream_t)stream);
 }

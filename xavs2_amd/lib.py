@@ -163,6 +163,8 @@ def lib():
         L.xme_tune_set.argtypes = [I, I, I, I]
         L.xme_expand_border.argtypes = [P, Z, I, I, I, P]
         L.xme_interpolate_frame.argtypes = [P, Z, I, I, I, P]
+        L.xme_expand_border_lcu_row.argtypes = [P, Z, I, I, I, I, I, P]
+        L.xme_interpolate_lcu_row.argtypes = [P, Z, I, I, I, I, I, P]
         L.xme_sad_batch.argtypes = [I, P, Z, P, Z, P, P, I, P, P]
         L.xme_satd_batch.argtypes = [I, P, Z, P, Z, P, P, I, P, P]
         L.xme_sad_x3_batch.argtypes = [I, P, P, Z, P, P, I, P, P]
@@ -292,6 +294,10 @@ class DevicePlane:
         check(lib().xme_expand_border(C.c_void_p(self.origin), self.stride, self.w, self.h, self.pad, stream),
               "xme_expand_border")
 
+    def expand_border_lcu_row(self, row_log2, lcu_y, stream=None):
+        check(lib().xme_expand_border_lcu_row(C.c_void_p(self.origin), self.stride, self.w, self.h, self.pad,
+                                              row_log2, lcu_y, stream), "xme_expand_border_lcu_row")
+
 
 class DeviceRef:
     """A reference picture: full-pel plane plus its 15 interpolated planes."""
@@ -316,6 +322,13 @@ class DeviceRef:
             arr = (C.c_void_p * 16)(*[(p.origin if p is not None else None) for p in self.planes])
             check(lib().xme_interpolate_frame(arr, self.stride, self.w, self.h, self.frac, stream),
                   "xme_interpolate_frame")
+
+    def interpolate_lcu_row(self, lcu_level, lcu_y, stream=None):
+        """interpolate_lcu_row for one LCU row of the planes (the full-pel
+        plane's rows must be final and border-expanded down to 4 rows below)."""
+        arr = (C.c_void_p * 16)(*[(p.origin if p is not None else None) for p in self.planes])
+        check(lib().xme_interpolate_lcu_row(arr, self.stride, self.w, self.h, self.frac, lcu_level, lcu_y, stream),
+              "xme_interpolate_lcu_row")
 
     def desc(self):
         d = RefDesc()
