@@ -87,11 +87,19 @@ __device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2
 // exact 32-bit result, and the horizontal sums feeding the diagonal planes
 // are kept in LDS as int32.
 __global__ void __launch_bounds__(256) interp_core_kernel(Planes16 pl, int64_t stride, int W, int H, int frac,
-                                                          int xs, int ys, int ye)
+                                                          int xs, int ys, int ye, int gx, int n_tiles)
 {
+    // workgroup b runs on XCD b % 8: each XCD takes a contiguous raster run of
+    // tiles, so that the 128-byte lines shared by side-by-side tiles (their
+    // staged source rows and the halves of their output rows) meet in one L2
+    const int per_xcd = (int)gridDim.x >> 3;
+    const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= n_tiles)
+        return;
+    const int tx_i = tile % gx, ty_i = tile / gx;
     __shared__ uint32_t sP[TY + 7][SPW];
     __shared__ int32_t sH[3][TY + 7][TX];
-    const int x0 = xs + (int)blockIdx.x * TX, y0 = ys + (int)blockIdx.y * TY;
+    const int x0 = xs + tx_i * TX, y0 = ys + ty_i * TY;
     const uint8_t *P = pl.p[0];
     const int tid = threadIdx.x;
     // stage: column x0 - 4 + 4q .. +3 of rows y0 - 3 .. y0 + TY + 3 (dword aligned
@@ -309,8 +317,11 @@ int interp_rows(const Planes16 &pl, int64_t stride, int w, int h, int frac, int 
     // left of -4 are overwritten by the border pass
     const int xs = -4 - (int)((reinterpret_cast<uintptr_t>(pl.p[2]) - 4) & 63);
     if (ye > ys) {
-        dim3 grid((w + 4 - xs + TX - 1) / TX, (ye - ys + TY - 1) / TY);
-        hipLaunchKernelGGL(interp_core_kernel, grid, dim3(256), 0, s, pl, stride, w, h, frac, xs, ys, ye);
+        const int gx = (w + 4 - xs + TX - 1) / TX, gy = (ye - ys + TY - 1) / TY;
+        const int n_tiles = gx * gy;
+        const int grid = (n_tiles + 7) & ~7;     // a multiple of 8: the same tile count per XCD
+        hipLaunchKernelGGL(interp_core_kernel, dim3(grid), dim3(256), 0, s, pl, stride, w, h, frac, xs, ys, ye, gx,
+                           n_tiles);
         XME_HIP(hipGetLastError());
     }
     return launch_border(pl, 1, 15, stride, w, h, XME_PAD, -4, ys, w + 4, ye, s, top, bottom);
