@@ -374,13 +374,17 @@ __device__ __forceinline__ uint32_t target4(uint32_t o, uint32_t p)
     return r;
 }
 
-// ME_COST_QPEL_BID (me.c:388-398) over the lane's tiles (at most 4: a PU
-// has at most 256 tiles and P = 64 beyond 64), against the target rows
-// `tgt` built once per job.
+// ME_COST_QPEL_BID (me.c:388-398) over the lane's tiles t0, t0 + P, ...
+// (at most 4: a PU has at most 256 tiles and P = 64 there) against the
+// target clip(2 * cur - pred_bwd).  The target rows of the lane's first tile
+// are built once per job (`tgt`); a PU of more than 64 tiles builds those of
+// its other tiles per candidate, so that one small register file serves
+// every PU size.
 struct BidEval {
     const SymJob &c;             // cur, strides, offset, tiles, window, lambda, switches
     const uint8_t *const (&p1)[16];
-    const uint32_t (&tgt)[4][4];
+    const uint32_t (&tgt)[4];
+    gbytes gb;                   // the backward prediction at the PU origin
     bool bwd_legal;
     int mvp1_x, mvp1_y, bid_bits;
     __device__ __forceinline__ int tiles(int mx, int my, int t0, int P) const
@@ -392,17 +396,24 @@ struct BidEval {
             return -1;
         const gbytes ga = (gbytes)a + c.off + (int64_t)(my >> 2) * c.i_ref + (mx >> 2);
         int s = 0;
+        if (t0 < c.ntiles) {
+            const int ty = (t0 / c.tw) * 4, tx = (t0 % c.tw) * 4;
+            uint32_t rr[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int t = t0 + i * P;
-            if (t < c.ntiles) {
-                const int ty = (t / c.tw) * 4, tx = (t % c.tw) * 4;
-                uint32_t rr[4];
+            for (int y = 0; y < 4; y++)
+                rr[y] = sym_load_dw(ga + (int64_t)(ty + y) * c.i_ref + tx);
+            s = xme_tile_cost_rows(tgt, rr, c.hadamard);
+        }
+        for (int t = t0 + P; t < c.ntiles; t += P) {
+            const int ty = (t / c.tw) * 4, tx = (t % c.tw) * 4;
+            uint32_t rr[4], tg[4];
 #pragma unroll
-                for (int y = 0; y < 4; y++)
-                    rr[y] = sym_load_dw(ga + (int64_t)(ty + y) * c.i_ref + tx);
-                s += xme_tile_cost_rows(tgt[i], rr, c.hadamard);
+            for (int y = 0; y < 4; y++) {
+                rr[y] = sym_load_dw(ga + (int64_t)(ty + y) * c.i_ref + tx);
+                tg[y] = target4(*(gdwords)(c.cur + (ty + y) * c.i_cur + tx),
+                                sym_load_dw(gb + (int64_t)(ty + y) * c.i_ref + tx));
             }
+            s += xme_tile_cost_rows(tg, rr, c.hadamard);
         }
         return s;
     }
@@ -461,26 +472,18 @@ __device__ SearchOut bid_search(const SymParams &p, const SymPlanes &pl, int64_t
         // the reference predicts from mc_luma here (me.c:1329-1335): not provided
         return SearchOut{ XME_MAX_DISTORTION, j.fwd_x, j.fwd_y, -1, 2 };
     }
-    // the target rows of the lane's tiles: clip(2 * cur - pred_bwd)
+    // the target rows of the lane's first tile: clip(2 * cur - pred_bwd)
     // (me.c:1322-1337)
-    uint32_t tgt[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int t = t0 + i * P;
+    const gbytes gb = bwd_legal ? (gbytes)p2 + c.off + (int64_t)(j.bwd_y >> 2) * i_ref + (j.bwd_x >> 2) : nullptr;
+    uint32_t tgt[4] = { 0, 0, 0, 0 };
+    if (bwd_legal && t0 < c.ntiles) {
+        const int ty = (t0 / c.tw) * 4, tx = (t0 % c.tw) * 4;
 #pragma unroll
         for (int y = 0; y < 4; y++)
-            tgt[i][y] = 0;
-        if (bwd_legal && t < c.ntiles) {
-            const int ty = (t / c.tw) * 4, tx = (t % c.tw) * 4;
-            const gbytes gb = (gbytes)p2 + c.off + (int64_t)(j.bwd_y >> 2) * i_ref + (j.bwd_x >> 2);
-#pragma unroll
-            for (int y = 0; y < 4; y++)
-                tgt[i][y] = target4(*(gdwords)(c.cur + (ty + y) * i_cur + tx),
-                                    sym_load_dw(gb + (int64_t)(ty + y) * i_ref + tx));
-        }
+            tgt[y] = target4(*(gdwords)(c.cur + (ty + y) * i_cur + tx), sym_load_dw(gb + (int64_t)(ty + y) * i_ref + tx));
     }
     const int bid_bits = xme_weighted_cost(p.lambda, xme_mvbits(j.bwd_x - j.mvp2_x) + xme_mvbits(j.bwd_y - j.mvp2_y));
-    const BidEval ev{ c, pl.p1, tgt, bwd_legal, j.mvp1_x, j.mvp1_y, bid_bits };
+    const BidEval ev{ c, pl.p1, tgt, gb, bwd_legal, j.mvp1_x, j.mvp1_y, bid_bits };
     int bcost = XME_MAX_DISTORTION, bx = j.fwd_x, by = j.fwd_y;
     if (p.frac == 0) {
         bi_stage(ev, 0, 0, 1, j.fwd_x, j.fwd_y, false, 0, 0, P, bcost, bx, by);
@@ -581,9 +584,15 @@ __device__ __forceinline__ bool caller_pu_ok(const SymParams &p, int x, int y, i
              x + w > p.frame_w || y + h > p.frame_h);
 }
 
-__global__ void __launch_bounds__(256) bi_kernel(SymParams p, GateParams g, SymPlanes pl, int64_t i_ref,
-                                                 const uint8_t *cur, int64_t i_cur, const xme_bi_job *__restrict__ jobs,
-                                                 int n, int dist_fwd, int dist_bwd, xme_bi_result *__restrict__ out)
+// pred_inter_search_bi in two launches, so that each has the register file
+// of its own search.  The first (md_inter.c:1083-1090) runs the symmetric
+// search's gates and search and stores the PU's result as if the
+// bi-directional search had not run; the second runs that search's gates
+// and search and the stored MVs' checks, and completes the result.
+__global__ void __launch_bounds__(256) bi_sym_kernel(SymParams p, GateParams g, SymPlanes pl, int64_t i_ref,
+                                                     const uint8_t *cur, int64_t i_cur,
+                                                     const xme_bi_job *__restrict__ jobs, int n, int dist_fwd,
+                                                     int dist_bwd, xme_bi_result *__restrict__ out)
 {
     const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
     if (ji >= n)
@@ -593,8 +602,8 @@ __global__ void __launch_bounds__(256) bi_kernel(SymParams p, GateParams g, SymP
     const GateBox box = gate_box(p, g, j.x, j.y, j.w, j.h);
     // fwd_mv.x <<= 2 on the int16 components of mv_t (md_inter.c:1068-1069)
     int mv_x = (int16_t)(j.fwd_imv_x * 4), mv_y = (int16_t)(j.fwd_imv_y * 4);
-    int fwd_x = mv_x, fwd_y = mv_y;
-    int cost = XME_MAX_DISTORTION, cost_bid = XME_MAX_DISTORTION, sym_mvcost = -1, bid_mvcost = -1;
+    const int fwd_x = mv_x, fwd_y = mv_y;
+    int cost = XME_MAX_DISTORTION, sym_mvcost = -1;
     bool ok = gate_range_sym(box, j.mvp1_x, j.mvp1_y, dist_fwd, dist_bwd, true, undef);
     ok &= gate_range_sym(box, mv_x, mv_y, dist_fwd, dist_bwd, true, undef);
     ok &= gate_mvd(g, j.mvp1_x, j.mvp1_y);
@@ -620,8 +629,42 @@ __global__ void __launch_bounds__(256) bi_kernel(SymParams p, GateParams g, SymP
         mv_y = o.mv_y;
         sym_mvcost = o.mvcost;
     }
-    ok = box.holds(fwd_x, fwd_y) && box.holds(j.bwd_x, j.bwd_y) && gate_mvd(g, j.mvp1_x, j.mvp1_y) &&
-         gate_mvd(g, j.mvp2_x, j.mvp2_y);
+    if ((threadIdx.x & 63) == 0) {
+        xme_bi_result r;
+        r.sym_cost = cost;
+        r.sym_mv_x = (int16_t)mv_x;
+        r.sym_mv_y = (int16_t)mv_y;
+        r.sym_mvcost = sym_mvcost;
+        r.bid_cost = XME_MAX_DISTORTION;
+        r.fwd_x = (int16_t)fwd_x;
+        r.fwd_y = (int16_t)fwd_y;
+        r.bwd_x = j.bwd_x;
+        r.bwd_y = j.bwd_y;
+        r.bid_mvcost = -1;
+        r.status = undef ? 2 : 0;
+        out[ji] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) bi_bid_kernel(SymParams p, GateParams g, SymPlanes pl, int64_t i_ref,
+                                                     const uint8_t *cur, int64_t i_cur,
+                                                     const xme_bi_job *__restrict__ jobs, int n, int dist_fwd,
+                                                     int dist_bwd, xme_bi_result *__restrict__ out)
+{
+    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    if (ji >= n)
+        return;
+    const xme_bi_job j = jobs[ji];
+    const xme_bi_result first = out[ji];
+    bool undef = first.status != 0;
+    const GateBox box = gate_box(p, g, j.x, j.y, j.w, j.h);
+    const int mv_x = first.sym_mv_x, mv_y = first.sym_mv_y;
+    int cost = first.sym_cost;
+    int fwd_x = first.fwd_x, fwd_y = first.fwd_y;
+    int cost_bid = XME_MAX_DISTORTION, bid_mvcost = -1;
+    // md_inter.c:1092-1102
+    const bool ok = box.holds(fwd_x, fwd_y) && box.holds(j.bwd_x, j.bwd_y) && gate_mvd(g, j.mvp1_x, j.mvp1_y) &&
+                    gate_mvd(g, j.mvp2_x, j.mvp2_y);
     if (ok && !undef) {
         xme_bid_job bj;
         bj.x = j.x;
@@ -652,16 +695,11 @@ __global__ void __launch_bounds__(256) bi_kernel(SymParams p, GateParams g, SymP
           gate_mvd(g, mv_x - j.mvp1_x, mv_y - j.mvp1_y)))
         cost = XME_MAX_DISTORTION;
     if ((threadIdx.x & 63) == 0) {
-        xme_bi_result r;
+        xme_bi_result r = first;
         r.sym_cost = cost;
-        r.sym_mv_x = (int16_t)mv_x;
-        r.sym_mv_y = (int16_t)mv_y;
-        r.sym_mvcost = sym_mvcost;
         r.bid_cost = cost_bid;
         r.fwd_x = (int16_t)fwd_x;
         r.fwd_y = (int16_t)fwd_y;
-        r.bwd_x = j.bwd_x;
-        r.bwd_y = j.bwd_y;
         r.bid_mvcost = bid_mvcost;
         r.status = undef ? 2 : 0;
         out[ji] = r;
@@ -924,8 +962,13 @@ extern "C" int xme_pred_inter_search_bi_batch(const xme_me_params *prm, int lcu_
         pl.p2[k] = ref_bwd->planes[k];
     }
     const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
-    hipLaunchKernelGGL(bi_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, g, pl,
-                       (int64_t)ref_fwd->stride, d_cur, (int64_t)i_cur, d_jobs, n, dist_fwd, dist_bwd, d_results);
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t i_ref = (int64_t)ref_fwd->stride;
+    hipLaunchKernelGGL(bi_sym_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, s, p, g, pl, i_ref, d_cur,
+                       (int64_t)i_cur, d_jobs, n, dist_fwd, dist_bwd, d_results);
+    XME_HIP(hipGetLastError());
+    hipLaunchKernelGGL(bi_bid_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, s, p, g, pl, i_ref, d_cur,
+                       (int64_t)i_cur, d_jobs, n, dist_fwd, dist_bwd, d_results);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }
