@@ -38,6 +38,19 @@ struct SymParams {
 };
 
 constexpr int SYM_WAVES = 4;
+
+// One wave per job, SYM_WAVES jobs per workgroup; the grid is a multiple of
+// 8 workgroups.  Workgroup b runs on XCD b % 8, and each XCD takes one
+// contiguous range of the jobs, so the plane lines that neighbouring PUs
+// share meet in one L2.
+__host__ __device__ inline int job_grid(int n) { return ((n + SYM_WAVES - 1) / SYM_WAVES + 7) & ~7; }
+
+__device__ __forceinline__ int job_index()
+{
+    const int per_xcd = (int)gridDim.x >> 3;
+    const int lb = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    return __builtin_amdgcn_readfirstlane(lb * SYM_WAVES + (int)(threadIdx.x >> 6));
+}
 constexpr int SYM_MAX_MV = 8191;     // |MV| bound of a valid job (qpel), see sym_job_ok
 
 // Four bytes at any address; the dword after the aligned one is read, which
@@ -318,7 +331,7 @@ __global__ void __launch_bounds__(256) sym_kernel(SymParams p, SymPlanes pl, int
                                                   int64_t i_cur, const xme_sym_job *__restrict__ jobs, int n,
                                                   xme_sym_result *__restrict__ out)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    const int ji = job_index();
     if (ji >= n)
         return;
     const SearchOut o = sym_search(p, pl, i_ref, cur, i_cur, jobs[ji]);
@@ -505,7 +518,7 @@ __global__ void __launch_bounds__(256) bid_kernel(SymParams p, SymPlanes pl, int
                                                   int64_t i_cur, const xme_bid_job *__restrict__ jobs, int n,
                                                   xme_sym_result *__restrict__ out)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    const int ji = job_index();
     if (ji >= n)
         return;
     const SearchOut o = bid_search(p, pl, i_ref, cur, i_cur, jobs[ji]);
@@ -594,7 +607,7 @@ __global__ void __launch_bounds__(256) bi_sym_kernel(SymParams p, GateParams g, 
                                                      const xme_bi_job *__restrict__ jobs, int n, int dist_fwd,
                                                      int dist_bwd, xme_bi_result *__restrict__ out)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    const int ji = job_index();
     if (ji >= n)
         return;
     const xme_bi_job j = jobs[ji];
@@ -651,7 +664,7 @@ __global__ void __launch_bounds__(256) bi_bid_kernel(SymParams p, GateParams g, 
                                                      const xme_bi_job *__restrict__ jobs, int n, int dist_fwd,
                                                      int dist_bwd, xme_bi_result *__restrict__ out)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    const int ji = job_index();
     if (ji >= n)
         return;
     const xme_bi_job j = jobs[ji];
@@ -715,7 +728,7 @@ __global__ void __launch_bounds__(256) dual_kernel(SymParams p, GateParams g, Sy
                                                    const xme_dual_job *__restrict__ jobs, int n, int r, int d_fwd,
                                                    int d_bwd, xme_dual_result *__restrict__ out)
 {
-    const int ji = __builtin_amdgcn_readfirstlane((int)blockIdx.x * SYM_WAVES + (int)(threadIdx.x >> 6));
+    const int ji = job_index();
     if (ji >= n)
         return;
     const xme_dual_job &jr = jobs[ji];
@@ -840,7 +853,7 @@ extern "C" int xme_me_search_sym_batch(const xme_me_params *prm, const xme_sym_j
         pl.p1[k] = ref1->planes[k];
         pl.p2[k] = ref2->planes[k];
     }
-    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    const int grid = job_grid(n);
     hipLaunchKernelGGL(sym_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, pl,
                        (int64_t)ref1->stride, d_cur, (int64_t)i_cur, d_jobs, n, d_results);
     XME_HIP(hipGetLastError());
@@ -884,7 +897,7 @@ extern "C" int xme_me_search_bid_batch(const xme_me_params *prm, const xme_bid_j
         pl.p1[k] = ref1->planes[k];
         pl.p2[k] = ref2->planes[k];
     }
-    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    const int grid = job_grid(n);
     hipLaunchKernelGGL(bid_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, (hipStream_t)stream, p, pl,
                        (int64_t)ref1->stride, d_cur, (int64_t)i_cur, d_jobs, n, d_results);
     XME_HIP(hipGetLastError());
@@ -961,7 +974,7 @@ extern "C" int xme_pred_inter_search_bi_batch(const xme_me_params *prm, int lcu_
         pl.p1[k] = ref_fwd->planes[k];
         pl.p2[k] = ref_bwd->planes[k];
     }
-    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    const int grid = job_grid(n);
     hipStream_t s = (hipStream_t)stream;
     const int64_t i_ref = (int64_t)ref_fwd->stride;
     hipLaunchKernelGGL(bi_sym_kernel, dim3(grid), dim3(64 * SYM_WAVES), 0, s, p, g, pl, i_ref, d_cur,
@@ -1007,7 +1020,7 @@ extern "C" int xme_pred_inter_search_dual_batch(const xme_me_params *prm, int lc
     }
     // one launch per reference, in the reference's order (each updates the
     // minimum the previous one stored)
-    const int grid = (n + SYM_WAVES - 1) / SYM_WAVES;
+    const int grid = job_grid(n);
     for (int r = 0; r < n_ref; r++) {
         const int s = r == 0 ? 1 : 0;
         SymPlanes pl;
