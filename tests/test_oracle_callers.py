@@ -196,3 +196,68 @@ def test_mvp_prices_the_symmetric_mv_and_mvp1_checks_it(avg_case):
     rate = (LAM * (O.mvbits(32, 12 - 9000) + O.mvbits(32, -8))) >> 16
     assert (r["sym_mv_x"], r["sym_mv_y"], r["sym_mvcost"]) == (12, -8, rate)
     assert r["sym_cost"] == rate
+
+
+def test_backward_mv_gate_and_its_stored_check(avg_case):
+    """The backward MV is gated by check_mv_range before the bid search
+    (md_inter.c:1092-1096) and checked again, against mvp2, after it
+    (:1113-1116).  PU (48, 32) 16x16: the box's right edge is x = 512 qpel.
+    bwd_x 516 leaves the search out (no MV cost, fwd as it came).  With
+    mvp2_x = -4000 and bwd_x = 200 the gate (check_mvd(mvp2)) passes and the
+    search runs, but 200 - (-4000) = 4200 fails check_mvd afterwards: the
+    search's MV cost is kept, its cost is voided."""
+    r = run_bi(avg_case, bi_job(bwd_x=516))
+    assert r["bid_mvcost"] == -1 and r["bid_cost"] == MAX and (r["fwd_x"], r["fwd_y"]) == (12, -8)
+    r = run_bi(avg_case, bi_job(bwd_x=512))
+    assert r["bid_mvcost"] != -1
+    r = run_bi(avg_case, bi_job(bwd_x=200, mvp2_x=-4000))
+    assert r["bid_mvcost"] != -1 and r["bid_cost"] == MAX
+    r = run_bi(avg_case, bi_job(bwd_x=200, mvp2_x=-3800))
+    assert r["bid_cost"] != MAX
+
+
+def test_x4_of_the_full_pel_best_wraps_in_int16(avg_case):
+    """fwd_mv.x <<= 2 shifts an int16 component of mv_t (md_inter.c:1068):
+    4 * 16387 = 65548 wraps to 12, the same start as a full-pel best of 3."""
+    assert run_bi(avg_case, bi_job(fwd_imv_x=16387)) == run_bi(avg_case, bi_job(fwd_imv_x=3))
+
+
+def test_dual_mirror_keeps_its_sign_in_f_slices():
+    """F slices mirror without the B-slice negation (md_inter.c:97-101): with
+    reference 0 the picture moved by d at distance 1 and reference 1 moved by
+    2d at distance 2, the symmetric MV 4d averages two exact predictions
+    (its mirror is +8d), so reference 0 costs its MV rate plus REF_COST(0)."""
+    d = (3, -2)
+    img = texture(W, H, 31)
+    cp = cur_plane(img)
+    r0 = planes_of(np.roll(img, (d[1], d[0]), axis=(0, 1)))
+    r1 = planes_of(np.roll(img, (2 * d[1], 2 * d[0]), axis=(0, 1)))
+    prm = O.me_param(W, H, 32, LAM)
+    job = dict(pix_x=48, pix_y=32, w=16, h=16, mvp_x=0, mvp_y=0, mvp1=[(0, 0), (0, 0)], imv=[d, (0, 0)])
+    cost, fr, _, mvc, fst, snd = O.pred_inter_search_dual(prm, job, [1, 2], cp, [r0, r1])
+    rate = (LAM * (O.mvbits(32, 4 * d[0]) + O.mvbits(32, 4 * d[1]))) >> 16
+    assert fst[0] == (4 * d[0], 4 * d[1]) and snd[0] == (8 * d[0], 8 * d[1])
+    assert (cost, fr, mvc) == (rate + 4, 0, rate)
+    # PU at x = 0: the box starts at -(0 + 64) * 4 = -256 qpel.  A start of
+    # 4 * 40 = 160 mirrors to +320 (inside); negated, -320 would gate
+    # reference 0 (reference 1 is voided here by its mvp1)
+    job = dict(pix_x=0, pix_y=32, w=16, h=16, mvp_x=0, mvp_y=0, mvp1=[(160, 0), (5000, 0)], imv=[(40, 0), (0, 0)])
+    cost, fr, _, _, _, _ = O.pred_inter_search_dual(prm, job, [1, 2], cp, [r0, r1])
+    assert fr == 0 and cost < MAX
+
+
+def test_dual_first_minimum_among_equal_totals():
+    """References 1 and 2 identical, at one distance, with the same inputs,
+    and reference 0 voided by an mvp1 beyond check_mvd: the totals of 1 and 2
+    are equal (REF_BITS[2] = REF_BITS[3] = 3), so the first strict minimum
+    keeps reference 1."""
+    img = texture(W, H, 41)
+    cp = cur_plane(img)
+    a = planes_of(np.roll(img, (1, 2), axis=(0, 1)))
+    b = planes_of(np.roll(img, (-2, 1), axis=(0, 1)))
+    prm = O.me_param(W, H, 32, LAM)
+    for x, y in ((16, 16), (48, 32), (96, 64)):
+        job = dict(pix_x=x, pix_y=y, w=16, h=16, mvp_x=0, mvp_y=0, mvp1=[(5000, 0), (0, 0), (0, 0)],
+                   imv=[(0, 0), (-2, 1), (-2, 1)])
+        cost, fr, sr, _, _, _ = O.pred_inter_search_dual(prm, job, [2, 3, 3], cp, [a, b, b])
+        assert (fr, sr) == (1, 0) and cost < MAX
