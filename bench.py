@@ -31,14 +31,9 @@ sys.path.insert(0, ROOT)
 METRIC = "ME cand-evals/s + roofline frac, batched vs plain; p50/p99 frame-ME latency"
 VALU_PEAK_TABSDIFF = 256 * 4 * 16 * 2.4e9 * 4 / 1e12     # v_sad_u8 / v_qsad: 157.3 T absdiff/s
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per search step (the two persistent search dispatches) on the
-# default workload (3840x2160, 4 refs, R=64, shipped tile table), measured with
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
-# reading for a 256 MiB copy kernel (FETCH x2.000, WRITE x1.000):
-# (104,490.1 + 182,787.8) KiB fetched -> 588.3 MB, (1,803.9 + 26,294.3) KiB
-# written -> 28.8 MB.  The counters cannot be read from inside this process, so
-# the figure is quoted for the workload it was measured on and null for any other.
-MEASURED_TRAFFIC = {(3840, 2160, 4, 64): (104490.1 + 182787.8) * 1024 * 2.0 + (1803.9 + 26294.3) * 1024 * 1.0}
+# The copy of known size that scales the HBM traffic counters, run beside the
+# search step in each counter pass.
+TRAFFIC_COPY_MIB = 256
 
 
 def log(msg):
@@ -738,6 +733,119 @@ def copy_probe(X, mib):
     print(json.dumps({"copy_probe_bytes": n, "dispatches": 20}))
 
 
+def traffic_probe(X, args):
+    """Child of measure_traffic, run under rocprofv3 --pmc: the copy of
+    TRAFFIC_COPY_MIB MiB 3 times, then the bench's search step (the plan's
+    graph) 3 times after one warm-up."""
+    n = TRAFFIC_COPY_MIB << 20
+    src, dst = X.DeviceBuffer(n), X.DeviceBuffer(n)
+    src.zero()
+    for _ in range(3):
+        X.check(X.lib().xme_copy_kernel(dst.at(), src.at(), n, None), "xme_copy_kernel")
+    X.lib().xme_device_sync()
+    del src, dst
+    s = setup_search(X, args)
+    for _ in range(4):
+        s["plan"].run_graph(s["dcur"], s["d_refs"], s["stream"].h)
+    s["stream"].sync()
+    print(json.dumps({"traffic_probe": "done"}), flush=True)
+
+
+def measure_traffic(args, timeout_s=240):
+    """roofline.traffic: HBM bytes per search step (its two persistent
+    dispatches), from two rocprofv3 --pmc children of this script in
+    --traffic-probe mode, FETCH_SIZE then WRITE_SIZE (KiB per dispatch), each
+    scaled by what the same counter reads for the copy of TRAFFIC_COPY_MIB MiB
+    in the same run.  Returns (bytes or None, details)."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, {"error": "rocprofv3 not found"}
+    workload = ["--width", str(args.width), "--height", str(args.height), "--refs", str(args.refs), "--range",
+                str(args.range)] + (["--no-tune"] if args.no_tune else []) + (["--retune"] if args.retune else []) + \
+               (["--pu-shape", args.pu_shape] if args.pu_shape else [])
+    env = dict(os.environ, TMPDIR="/tmp")
+    details, total = {}, 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="xme_traffic_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--", sys.executable,
+               os.path.abspath(__file__), "--serve-hw-queues", "0", "--traffic-probe"] + workload
+        log(f"traffic: {counter} pass")
+        proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                                start_new_session=True)
+        try:
+            out, _ = proc.communicate(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.communicate()
+            shutil.rmtree(d, ignore_errors=True)
+            return None, {"error": f"{counter} pass timed out"}
+        path = os.path.join(d, "t_counter_collection.csv")
+        # the profiler may end with an error of its own after the program has
+        # finished; its results are whole once the probe printed its last line
+        if "traffic_probe" not in out or not os.path.exists(path):
+            shutil.rmtree(d, ignore_errors=True)
+            return None, {"error": f"{counter} pass failed (exit {proc.returncode})"}
+        copy_kib, search_kib = [], {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                if "copy_kernel" in name:
+                    copy_kib.append(v)
+                elif "fs_kernel<true" in name:
+                    key = name[name.index("fs_kernel<"):]
+                    search_kib.setdefault(key[:key.index(">") + 1], []).append(v)
+        shutil.rmtree(d, ignore_errors=True)
+        if not copy_kib or not search_kib:
+            return None, {"error": f"{counter} pass: no copy or search dispatch in the counters"}
+        scale = (TRAFFIC_COPY_MIB << 20) / (float(np.median(copy_kib)) * 1024.0)
+        per_step = sum(float(np.median(v)) for v in search_kib.values()) * 1024.0 * scale
+        details[counter] = {"kib_per_dispatch": {k: float(np.median(v)) for k, v in search_kib.items()},
+                            "copy_kib": float(np.median(copy_kib)), "scale": scale, "bytes_per_step": per_step}
+        total += per_step
+    return total, details
+
+
+def setup_search(X, args):
+    """The bench's search job: inputs, resident reference planes, tile
+    shapes and the batched plan."""
+    w, h, n_ref, R = args.width, args.height, args.refs, args.range
+    lam = 4 * 65536
+    log(f"generating {w}x{h} inputs, {n_ref} refs")
+    refs, cur, pus = build_inputs(w, h, n_ref, seed=2026)
+    if args.pu_shape:
+        pw, ph = (int(v) for v in args.pu_shape.lower().split("x"))
+        pus = [(x, y, pw, ph) for y in range(0, h - ph + 1, ph) for x in range(0, w - pw + 1, pw)]
+        log(f"one-shape tiling: {len(pus)} PUs of {pw}x{ph}")
+    stream = X.Stream()
+    d_refs = []
+    for r in refs:
+        d = X.DeviceRef(w, h, frac=2)
+        d.load(r, stream.h)
+        d_refs.append(d)
+    dcur = X.DevicePlane(w, h, pad=0, stride=((w + 255) // 256) * 256)
+    dcur.upload_picture(cur)
+    stream.sync()
+    prm = X.me_params(w, h, R, lam, use_hadamard=1, use_fractional_me=2, use_fast_sub_me=0, enable_pmvr=0)
+    part_of = {(X.PU_W[i], X.PU_H[i]): i for i in range(25)}
+    parts = sorted({part_of[(pw, ph)] for (_, _, pw, ph) in pus})
+    tiles, tuned_now = {}, []
+    if not args.no_tune:
+        from xavs2_amd import tiles as T
+        tiles, tuned_now = T.install(prm, parts, stream.h, retune=args.retune)
+        log(f"tile shapes for {len(parts)} PU shapes at R={R}: {len(parts) - len(tuned_now)} from the shipped "
+            f"table, {len(tuned_now)} tuned now")
+    pu_arr = X.make_pus([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus])
+    log(f"planning {len(pus)} PUs x {n_ref} refs")
+    plan = X.Plan(prm, pu_arr, n_ref, X.PLAN_BATCHED)
+    return dict(w=w, h=h, n_ref=n_ref, R=R, lam=lam, refs=refs, cur=cur, pus=pus, stream=stream, d_refs=d_refs,
+                dcur=dcur, prm=prm, tiles=tiles, tuned_now=tuned_now, pu_arr=pu_arr, plan=plan)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -768,6 +876,11 @@ def main():
     ap.add_argument("--transform-blocks", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the 16-byte copy kernel over MIB MiB 20 times (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure roofline.traffic")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="child mode of the traffic measurement: a copy and the search step, nothing printed "
+                         "but a marker line")
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--refs", type=int, default=4)
@@ -789,6 +902,9 @@ def main():
     if args.copy_probe:
         copy_probe(X, args.copy_probe)
         return
+    if args.traffic_probe:
+        traffic_probe(X, args)
+        return
 
     # the serving leg runs first: its device slots are the process's first HIP
     # streams, so each gets a hardware queue of its own (up to GPU_MAX_HW_QUEUES)
@@ -801,36 +917,10 @@ def main():
         serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share,
                               int(args.serve_split))
 
-    w, h, n_ref, R = args.width, args.height, args.refs, args.range
-    lam = 4 * 65536
-    log(f"generating {w}x{h} inputs, {n_ref} refs")
-    refs, cur, pus = build_inputs(w, h, n_ref, seed=2026)
-    if args.pu_shape:
-        pw, ph = (int(v) for v in args.pu_shape.lower().split("x"))
-        pus = [(x, y, pw, ph) for y in range(0, h - ph + 1, ph) for x in range(0, w - pw + 1, pw)]
-        log(f"one-shape tiling: {len(pus)} PUs of {pw}x{ph}")
-    stream = X.Stream()
-    d_refs = []
-    for r in refs:
-        d = X.DeviceRef(w, h, frac=2)
-        d.load(r, stream.h)
-        d_refs.append(d)
-    dcur = X.DevicePlane(w, h, pad=0, stride=((w + 255) // 256) * 256)
-    dcur.upload_picture(cur)
-    stream.sync()
-    prm = X.me_params(w, h, R, lam, use_hadamard=1, use_fractional_me=2, use_fast_sub_me=0, enable_pmvr=0)
-
-    part_of = {(X.PU_W[i], X.PU_H[i]): i for i in range(25)}
-    parts = sorted({part_of[(pw, ph)] for (_, _, pw, ph) in pus})
-    tiles, tuned_now = {}, []
-    if not args.no_tune:
-        from xavs2_amd import tiles as T
-        tiles, tuned_now = T.install(prm, parts, stream.h, retune=args.retune)
-        log(f"tile shapes for {len(parts)} PU shapes at R={R}: {len(parts) - len(tuned_now)} from the shipped "
-            f"table, {len(tuned_now)} tuned now")
-    pu_arr = X.make_pus([(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in pus])
-    log(f"planning {len(pus)} PUs x {n_ref} refs")
-    plan = X.Plan(prm, pu_arr, n_ref, X.PLAN_BATCHED)
+    su = setup_search(X, args)
+    w, h, n_ref, R, lam = su["w"], su["h"], su["n_ref"], su["R"], su["lam"]
+    refs, cur, pus, stream, d_refs, dcur = su["refs"], su["cur"], su["pus"], su["stream"], su["d_refs"], su["dcur"]
+    prm, tiles, tuned_now, pu_arr, plan = su["prm"], su["tiles"], su["tuned_now"], su["pu_arr"], su["plan"]
     n_inst, n_cand, n_absdiff = plan.info()
 
     # warmup (also captures the graph)
@@ -901,6 +991,10 @@ def main():
         threads = host["threads"]
         log(f"timing the CPU oracle on a sample ({threads} threads; host {host})")
         cpu = cpu_baseline((R, lam, 1, 2, 0, 0), refs, cur, pus, gpu_res, n_ref, args.cpu_seconds, threads, host)
+    traffic, traffic_detail = None, {"skipped": "--no-traffic"}
+    if not args.no_traffic:
+        traffic, traffic_detail = measure_traffic(args)
+        log(f"traffic: {traffic} B per search step")
 
     out = {
         "metric": METRIC,
@@ -934,9 +1028,11 @@ def main():
             "peak": VALU_PEAK_TABSDIFF,
             "unit": "Tabsdiff/s",
             "frac": achieved / VALU_PEAK_TABSDIFF,
-            "traffic": MEASURED_TRAFFIC.get((w, h, n_ref, R)) if not (args.no_tune or args.retune or args.pu_shape)
-            else None,
-            "traffic_unit": "HBM bytes per search step (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "traffic": traffic,
+            "traffic_unit": "HBM bytes per search step, measured in this run: rocprofv3 --pmc FETCH_SIZE and "
+                            "WRITE_SIZE on a child running the same step, each scaled by its reading for a "
+                            f"{TRAFFIC_COPY_MIB} MiB copy in the same pass",
+            "traffic_detail": traffic_detail,
             "kernel": "integer full search: fs_kernel<true,8> (generic PUs) + fs_kernel<true,16> (PUs of <= 256 px), both persistent",
             "kernel_ms": search_ms,
         },
