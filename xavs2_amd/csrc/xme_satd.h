@@ -34,12 +34,15 @@ __device__ __forceinline__ int xme_tile_cost_rows(const uint32_t (&cr)[4], const
         d[y][2] = a - c;
         d[y][3] = b - e;
     }
+    // the last butterfly level pairs (a + c, a - c) and (b + e, b - e), and
+    // |p + q| + |p - q| = 2 max(|p|, |q|) for integers: the pairs' sums are
+    // never formed
     int s = 0;
 #pragma unroll
     for (int x = 0; x < 4; x++) {
         const int a = d[0][x] + d[1][x], b = d[0][x] - d[1][x];
         const int c = d[2][x] + d[3][x], e = d[2][x] - d[3][x];
-        s += abs(a + c) + abs(b + e) + abs(a - c) + abs(b - e);
+        s += max(abs(a), abs(c)) + max(abs(b), abs(e));
     }
-    return s;
+    return 2 * s;
 }
