@@ -609,7 +609,7 @@ def transform_leg(X, blocks_total, reps, seed=91):
 SERVE_STREAMS = [(1280, 720)] * 6 + [(1920, 1080)] * 6 + [(3840, 2160)] * 4
 
 
-def serving_leg(X, seconds, loads, slots, n_cycle, share=0, split=0, seed=77):
+def serving_leg(X, seconds, loads, slots, n_cycle, share=0, split=0, seed=77, small_work=0):
     """16 concurrent encodes (6 x 720p, 6 x 1080p, 4 x 2160p), each a moving
     smooth+noise sequence (pan <= 2 px/frame, local block MVs in [-32, 32]^2),
     frames arriving as Poisson processes at 30-60 fps per stream (scaled by the
@@ -624,7 +624,7 @@ def serving_leg(X, seconds, loads, slots, n_cycle, share=0, split=0, seed=77):
     log(f"serving: generating {len(SERVE_STREAMS)} sequences of {n_cycle} frames")
     seqs = [synth.moving_sequence(w, h, n_cycle, seed=seed + 10 * i) for i, (w, h) in enumerate(SERVE_STREAMS)]
     srv = X.Server(64 << 30, max_inflight=slots, search_range=64, n_ref=2, workgroup_share_pct=share,
-                   launch_split_absdiff=split)
+                   launch_split_absdiff=split, full_share_below_absdiff=small_work)
     sids = []
     for i, (w, h) in enumerate(SERVE_STREAMS):
         pus = [(x, y, pw, ph, 0, 0) for (x, y, pw, ph) in synth.mixed_pus(w, h, seed + 10 * i + 1)]
@@ -709,7 +709,9 @@ def serving_leg(X, seconds, loads, slots, n_cycle, share=0, split=0, seed=77):
         "workload": "16 concurrent encodes (6x1280x720, 6x1920x1080, 4x3840x2160), Poisson arrivals at 30-60 fps "
                     "per stream x load, 2-ref ME over mixed PUs, R=64, 64 GB budget, "
                     f"{slots} device slots on {os.environ.get('GPU_MAX_HW_QUEUES', '4')} hardware queues, "
-                    f"search launches capped at {share or 100} % of the device and "
+                    f"search launches capped at {share or 100} % of the device"
+                    + (f" for streams of at least {small_work:.0e} absdiffs per reference (smaller ones uncapped)"
+                       if small_work > 0 else "") + " and "
                     f"split at {'8e9' if split == 0 else ('none' if split < 0 else f'{split:.0e}')} absdiffs, "
                     f"{seconds:.0f} s per load",
         "stream_fps": [round(float(f), 2) for f in fps],
@@ -869,6 +871,9 @@ def main():
     ap.add_argument("--serve-share", type=int, default=50,
                     help="%% of a full-device search launch per frame in the serving leg (0 = 100); a cap "
                          "leaves CUs for frames that arrive while a large one searches")
+    ap.add_argument("--serve-small-work", type=float, default=5e10,
+                    help="streams whose search work per reference is below this many absdiffs search on the "
+                         "whole device (the 720p and 1080p ones at R = 64); --serve-share caps the others")
     ap.add_argument("--no-transform", action="store_true", help="skip the residual-transform leg")
     ap.add_argument("--no-pattern", action="store_true", help="skip the DIA / HEX / UMH pattern-search leg")
     ap.add_argument("--no-bipred", action="store_true", help="skip the bi-prediction (sym / bid) leg")
@@ -915,7 +920,7 @@ def main():
             T.install(X.me_params(1920, 1080, 64, 4 * 65536), list(range(25)), retune=args.retune)
         loads = [float(v) for v in args.serve_loads.split(",") if v]
         serving = serving_leg(X, args.serve_seconds, loads, args.serve_slots, args.serve_frames, args.serve_share,
-                              int(args.serve_split))
+                              int(args.serve_split), small_work=int(args.serve_small_work))
 
     su = setup_search(X, args)
     w, h, n_ref, R, lam = su["w"], su["h"], su["n_ref"], su["R"], su["lam"]

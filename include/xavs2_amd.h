@@ -661,6 +661,9 @@ typedef struct {
     int64_t launch_split_absdiff;    /* search work per launch (0: 8e9; < 0: no split) */
     int32_t me_method;               /* XME_ME_FS, _DIA, _HEX or _TZ (UMH needs per-job state) */
     int32_t reserved;
+    int64_t full_share_below_absdiff; /* streams whose search work per reference (absdiffs) is below
+                                         this run on the whole device; workgroup_share_pct caps the
+                                         others (0: it caps every stream) */
 } xme_server_config;
 typedef struct {
     int64_t submitted, completed, rejected;

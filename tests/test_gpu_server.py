@@ -84,9 +84,9 @@ def test_pattern_search_server_matches_oracle(method):
         X.Server(GIB, max_inflight=2, search_range=16, n_ref=2, me_method=X.ME_UMH)
 
 
-def run_streams(specs, n_frames, interleave, max_inflight, budget=8 * GIB, R=32):
+def run_streams(specs, n_frames, interleave, max_inflight, budget=8 * GIB, R=32, **server_kw):
     """specs: list of (w, h, seed).  Returns {stream index: [results per frame]}."""
-    srv = X.Server(budget, max_inflight=max_inflight, search_range=R, n_ref=2)
+    srv = X.Server(budget, max_inflight=max_inflight, search_range=R, n_ref=2, **server_kw)
     seqs = [synth.moving_sequence(w, h, n_frames, seed=s) for (w, h, s) in specs]
     sids = [srv.open_stream(w, h, X.make_pus(pu_tuples(w, h, s))) for (w, h, s) in specs]
     out = {i: [None] * n_frames for i in range(len(specs))}
@@ -108,6 +108,23 @@ def test_same_results_alone_and_in_the_mix():
     mixed = run_streams([a] + others, 4, interleave=True, max_inflight=4)[0]
     for f in range(4):
         assert np.array_equal(alone[f], mixed[f]), f"frame {f} differs between alone and mix"
+
+
+def test_capped_and_uncapped_streams_keep_results():
+    """A 50 % workgroup cap on the streams of large frames (search work per
+    reference of at least 3e8 absdiffs at R = 32: 640x352 has 640 * 352 * 64^2
+    = 9.2e8, 320x192 has 2.5e8), split launches, and the small streams on the
+    whole device: every stream's results equal its results alone."""
+    a = (320, 192, 21)
+    others = [(256, 128, 22), (640, 360 - 360 % 8, 23), (128, 96, 24)]
+    alone = run_streams([a, others[1]], 3, interleave=False, max_inflight=1)
+    mixed = run_streams([a] + others, 3, interleave=True, max_inflight=4, workgroup_share_pct=50,
+                        launch_split_absdiff=int(2e8), full_share_below_absdiff=int(3e8))
+    for f in range(3):
+        assert np.array_equal(alone[0][f], mixed[0][f]), f"small stream, frame {f}"
+        assert np.array_equal(alone[1][f], mixed[2][f]), f"capped stream, frame {f}"
+    with pytest.raises(X.XmeError):
+        X.Server(GIB, max_inflight=2, search_range=16, n_ref=2, full_share_below_absdiff=-1)
 
 
 def test_concurrent_submitters_and_waiters():

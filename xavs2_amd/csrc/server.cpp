@@ -410,7 +410,7 @@ extern "C" int xme_server_create(const xme_server_config *cfg, xme_server **out)
     if (!cfg || !out || cfg->memory_budget <= 0 || cfg->max_inflight < 1 || cfg->max_inflight > kMaxSlots ||
         cfg->search_range < 1 || cfg->search_range > 128 || cfg->n_ref < 1 || cfg->n_ref > XME_MAX_REFS ||
         cfg->lambda_factor < 0 || cfg->use_fractional_me < 0 || cfg->use_fractional_me > 2 ||
-        cfg->workgroup_share_pct < 0 || cfg->workgroup_share_pct > 100 ||
+        cfg->workgroup_share_pct < 0 || cfg->workgroup_share_pct > 100 || cfg->full_share_below_absdiff < 0 ||
         (cfg->me_method != XME_ME_FS && cfg->me_method != XME_ME_DIA && cfg->me_method != XME_ME_HEX &&
          cfg->me_method != XME_ME_TZ)) {
         xme_set_error("xme_server_create: bad configuration");
@@ -527,7 +527,11 @@ extern "C" int xme_server_open_stream(xme_server *S, int w, int h, const xme_pu 
         st->resident += (int64_t)xme_plan_device_bytes(p);
         const int64_t split = S->cfg.launch_split_absdiff;
         xme_plan_set_launch_split(p, split == 0 ? (int64_t)8000000000LL : split < 0 ? 0 : split);
-        const int share = S->cfg.workgroup_share_pct;
+        // the share caps the streams of large frames; a small frame's search
+        // takes every workgroup that frees, so it finishes before the large
+        // frames around it instead of sharing the device with them
+        const bool small = S->cfg.full_share_below_absdiff > 0 && st->work < S->cfg.full_share_below_absdiff;
+        const int share = small ? 100 : S->cfg.workgroup_share_pct;
         if (share > 0 && share < 100) {
             int g = 0, full = 0;
             xme_plan_workgroups(p, &g, &full);

@@ -113,7 +113,7 @@ class ServerConfig(C.Structure):
                 ("lambda_factor", C.c_int32), ("n_ref", C.c_int32), ("use_hadamard", C.c_int32),
                 ("use_fractional_me", C.c_int32), ("use_fast_sub_me", C.c_int32), ("enable_pmvr", C.c_int32),
                 ("workgroup_share_pct", C.c_int32), ("launch_split_absdiff", C.c_int64),
-                ("me_method", C.c_int32), ("reserved", C.c_int32)]
+                ("me_method", C.c_int32), ("reserved", C.c_int32), ("full_share_below_absdiff", C.c_int64)]
 
 
 class ServerStats(C.Structure):
@@ -535,9 +535,10 @@ class Server:
 
     def __init__(self, memory_budget, max_inflight=4, search_range=64, lambda_factor=4 * 65536, n_ref=2,
                  use_hadamard=1, use_fractional_me=2, use_fast_sub_me=0, enable_pmvr=0, workgroup_share_pct=0,
-                 launch_split_absdiff=0, me_method=0):
+                 launch_split_absdiff=0, me_method=0, full_share_below_absdiff=0):
         cfg = ServerConfig()
         cfg.workgroup_share_pct = workgroup_share_pct
+        cfg.full_share_below_absdiff = int(full_share_below_absdiff)
         cfg.launch_split_absdiff = launch_split_absdiff
         cfg.me_method = me_method
         cfg.memory_budget, cfg.max_inflight, cfg.search_range = int(memory_budget), max_inflight, search_range
