@@ -545,7 +545,154 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
     return leg
 
 
-def transform_leg(X, blocks_total, reps, seed=91):
+def chroma_cus(pus):
+    """The CU of each PU of the mixed tiling: a square PU is its own CU; the
+    four 64x16 / 16x64 PUs of a 64x64 CU and the four 32x8 / 8x32 of a 32x32
+    share theirs.  Returns {(x, y, size)} in first-seen order."""
+    seen = {}
+    for (x, y, pw, ph) in pus:
+        s = max(pw, ph)
+        cu = (x & ~(s - 1), y & ~(s - 1), s)
+        seen.setdefault(cu, None)
+    return list(seen)
+
+
+def chroma_recon_leg(X, w, h, pus, gpu_res, n_ref, refs, cur, reps, cpu, luma_qp=32):
+    """Chroma reconstruction of the frame's inter CUs (cu_recon_chroma): each
+    PU's U and V prediction by mc_chroma (xme_mc_chroma_batch) into two
+    prediction pictures, at reference 0 and 4 x the PU's full-pel best, then
+    each CU's (size / 2)^2 block of both planes through xme_recon_chroma_batch
+    at the chroma QP of luma QP 32; one MC launch and one launch per block
+    size for both planes.  4:2:0 planes: U = the 2x2 mean of the luma pictures,
+    V = 255 - U.  Algorithmic HBM bytes of the reconstruction: 5 per chroma
+    pixel (current and prediction read, levels (int16) and reconstruction
+    written).  Parity with the oracle on every 4th CU."""
+    import ctypes as C
+    from oracle import oracle as O
+    cw, ch = w // 2, h // 2
+
+    def planes420(img):
+        u = img[: 2 * ch, : 2 * cw].reshape(ch, 2, cw, 2).astype(np.int32).sum(axis=(1, 3))
+        u = ((u + 2) >> 2).astype(np.uint8)
+        return u, (255 - u).astype(np.uint8)
+
+    ref_uv, cur_uv = planes420(refs[0]), planes420(cur)
+    d_ref = []
+    for img in ref_uv:
+        d = X.DevicePlane(cw, ch)
+        d.upload_picture(img)
+        d.expand_border()
+        d_ref.append(d)
+    d_cur = []
+    for img in cur_uv:
+        d = X.DevicePlane(cw, ch, pad=0, stride=((cw + 63) // 64) * 64)
+        d.upload_picture(img)
+        d_cur.append(d)
+    pred = [X.DevicePlane(cw, ch, pad=0, stride=d_cur[0].stride) for _ in range(2)]
+    rec = [X.DevicePlane(cw, ch, pad=0, stride=d_cur[0].stride) for _ in range(2)]
+    r0 = gpu_res[0::n_ref]
+    mvs = np.stack([4 * r0["bmv2_x"].astype(np.int32), 4 * r0["bmv2_y"].astype(np.int32)], axis=1)
+    blocks = np.array([(x, y, pw, ph, int(mvs[i, 0]), int(mvs[i, 1])) for i, (x, y, pw, ph) in enumerate(pus)],
+                      np.int32)
+    off = np.array([(int(b[1]) // 2) * pred[0].stride + int(b[0]) // 2 for b in blocks], np.int64)
+    d_pu, d_off = X.upload(blocks.ravel()), X.upload(off)
+    cus = chroma_cus(pus)
+    qp_c = X.lib().xme_chroma_qp(luma_qp)
+    by_n = {}
+    for k, (x, y, s) in enumerate(cus):
+        by_n.setdefault(s // 2, []).append((x // 2, y // 2, k % 4 == 0))
+    stream = X.Stream()
+    bufs = []
+    for n in sorted(by_n):
+        b = np.array([(x, y) for x, y, _ in by_n[n]], np.int32)
+        nb = len(b)
+        bufs.append((n, nb, b, X.upload(b.ravel()),
+                     (X.DeviceBuffer(2 * 2 * nb * n * n), X.DeviceBuffer(2 * 4 * nb), X.DeviceBuffer(2 * 4 * nb))))
+
+    def mc():
+        X.check(X.lib().xme_mc_chroma_batch(C.c_void_p(d_ref[0].origin), C.c_void_p(d_ref[1].origin),
+                                            d_ref[0].stride, d_pu.at(), len(blocks), w, h,
+                                            C.c_void_p(pred[0].origin), C.c_void_p(pred[1].origin), pred[0].stride,
+                                            d_off.at(), stream.h), "xme_mc_chroma_batch")
+
+    def recon():
+        for n, nb, _b, d_b, (level, nnz, ssd) in bufs:
+            X.check(X.lib().xme_recon_chroma_batch(n, qp_c, C.c_void_p(d_cur[0].origin), C.c_void_p(d_cur[1].origin),
+                                                   d_cur[0].stride, C.c_void_p(pred[0].origin),
+                                                   C.c_void_p(pred[1].origin), pred[0].stride, cw, ch, d_b.at(), nb,
+                                                   level.at(), C.c_void_p(rec[0].origin), C.c_void_p(rec[1].origin),
+                                                   rec[0].stride, nnz.at(), ssd.at(), stream.h),
+                    "xme_recon_chroma_batch")
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        stream.sync()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    mc_ms = timed(mc)
+    recon_ms = timed(recon)
+    pixels = 2 * sum(nb * n * n for n, nb, *_ in bufs)
+    leg = {
+        "workload": f"{cw}x{ch} U and V of the {w}x{h} frame's {len(cus)} CUs ("
+                    + ", ".join(f"{nb} of {n}x{n}" for n, nb, *_ in bufs)
+                    + f" per plane), {len(blocks)} PUs predicted from reference 0, chroma QP {qp_c} (luma {luma_qp})",
+        "mc_ms_per_frame": mc_ms,
+        "recon_ms_per_frame": recon_ms,
+        "recon_pixels_per_s": pixels / (recon_ms * 1e-3),
+        "recon_algorithmic_gb_per_s": 5.0 * pixels / (recon_ms * 1e-3) / 1e9,
+        "recon_hbm_frac": 5.0 * pixels / (recon_ms * 1e-3) / 8e12,
+    }
+    if cpu:
+        ok, el, n_blk = True, 0.0, 0
+        o_ref = []
+        for img in ref_uv:
+            op = O.Plane(cw, ch, pad=40)
+            op.set_picture(img)
+            op.expand_border()
+            o_ref.append(op)
+        g_pred = [pr.buf.download(pr.rows * pr.stride, np.uint8).reshape(pr.rows, pr.stride)[:ch, :cw]
+                  for pr in pred]
+        g_rec = [rc.buf.download(rc.rows * rc.stride, np.uint8).reshape(rc.rows, rc.stride)[:ch, :cw]
+                 for rc in rec]
+        e_pred = [np.zeros((ch, cw), np.uint8) for _ in range(2)]
+        sampled = {(x, y, s) for k, (x, y, s) in enumerate(cus) if k % 4 == 0}
+        for (x, y, pw, ph, mx, my) in blocks:
+            s = max(int(pw), int(ph))
+            if (int(x) & ~(s - 1), int(y) & ~(s - 1), s) not in sampled:
+                continue
+            qx = O.lib().xo_get_mc_pos(w, int(pw), int(x), int(mx))
+            qy = O.lib().xo_get_mc_pos(h, int(ph), int(y), int(my))
+            bu, bv = O.mc_chroma(qx, qy, int(pw) // 2, int(ph) // 2, o_ref[0], o_ref[1])
+            e_pred[0][y // 2:y // 2 + ph // 2, x // 2:x // 2 + pw // 2] = bu
+            e_pred[1][y // 2:y // 2 + ph // 2, x // 2:x // 2 + pw // 2] = bv
+        for n, nb, b, _d_b, (level, nnz, ssd) in bufs:
+            sel = np.array([t[2] for t in by_n[n]])
+            bs = b[sel]
+            g_level = level.download(2 * nb * n * n, np.int16).reshape(2, nb, n, n)
+            g_nnz, g_ssd = nnz.download(2 * nb, np.int32).reshape(2, nb), ssd.download(2 * nb, np.int32).reshape(2, nb)
+            for p in range(2):
+                t0 = time.perf_counter()
+                el_level, el_rec, el_nnz, el_ssd = O.recon_pred_batch(n, qp_c, cur_uv[p], e_pred[p], bs)
+                el += time.perf_counter() - t0
+                n_blk += len(bs)
+                ok = ok and np.array_equal(g_level[p][sel], el_level)
+                ok = ok and np.array_equal(g_nnz[p][sel], el_nnz) and np.array_equal(g_ssd[p][sel], el_ssd)
+                for (x, y) in bs:
+                    ok = ok and np.array_equal(g_pred[p][y:y + n, x:x + n], e_pred[p][y:y + n, x:x + n])
+                    ok = ok and np.array_equal(g_rec[p][y:y + n, x:x + n], el_rec[y:y + n, x:x + n])
+        leg["parity_with_oracle_every_4th_cu"] = bool(ok)
+        leg["cpu_oracle"] = {"recon_pixels_per_s": 2 * sum(int(np.sum([t[2] for t in by_n[n]])) * n * n
+                                                           for n in by_n) / el,
+                             "threads": 1, "seconds": el,
+                             "sample": f"the U and V blocks of every 4th CU ({n_blk} blocks)"}
+    log(f"chroma: MC {mc_ms:.3f} ms, recon {recon_ms:.3f} ms per frame")
+    return leg
+
     """DCT -> flat quant -> dequant -> IDCT over blocks_total int16 blocks, a
     quarter each of 4x4, 8x8, 16x16 and 32x32 (Laplacian b=6 residuals clipped
     to +-255, 30 % all-zero blocks), at QP 22, 27, 32, 37, 42.  Algorithmic HBM
@@ -986,6 +1133,9 @@ def main():
         log("inter-reconstruction leg")
         recon = recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, max(20, args.steps),
                           not args.no_cpu)
+        log("chroma reconstruction leg")
+        recon["chroma"] = chroma_recon_leg(X, w, h, pus, gpu_res, n_ref, refs, cur, max(20, args.steps),
+                                           not args.no_cpu)
     transform = None
     if not args.no_transform:
         log(f"residual transform leg: {args.transform_blocks} blocks")

@@ -635,6 +635,36 @@ int xme_inter_recon_bi_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, 
                              const xme_ref_desc *ref2, int frame_w, int frame_h, const int32_t *d_tus,
                              const int32_t *d_mv2, int ntu, coeff_t *d_resid, coeff_t *d_level, coeff_t *d_rres,
                              pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd, void *stream);
+/* Reconstruction of nblk N x N blocks (n in 4, 8, 16, 32, or
+ * 32 | XME_TU_DCT_HALF) whose prediction is a picture d_pred (stride i_pred)
+ * at the block's own position: the chroma of an inter CU, one call per
+ * plane (cu_recon_chroma, source/encoder/rdo.c:785-846: sub_ps, the
+ * transform of tu_get_dct_coeff, inter quant, dequant, IDCT, add_ps, and
+ * pixf.ssd against d_cur), with the prediction of xme_mc_chroma_batch
+ * written into d_pred.  qp is the chroma QP (xme_chroma_qp).  d_blocks:
+ * nblk x {x, y} in the plane_w x plane_h plane.  d_level (nblk * n * n,
+ * 16-byte aligned), d_nnz and d_ssd per block; the reconstruction goes into
+ * d_rec at the block.  A block outside the plane or at a column that is not a
+ * multiple of 4 gets ssd -1, zero levels and no pixels.  d_cur / d_rec:
+ * 4-byte aligned with strides % 4 == 0. */
+int xme_recon_pred_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const pel_t *d_pred, intptr_t i_pred,
+                         int plane_w, int plane_h, const int32_t *d_blocks, int nblk, coeff_t *d_level, pel_t *d_rec,
+                         intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd, void *stream);
+/* cu_recon_chroma (source/encoder/rdo.c:785-846) for nblk CUs at once: both
+ * planes' blocks in one launch, as xme_recon_pred_batch on U (d_cur_u,
+ * d_pred_u, d_rec_u) and then on V with the same blocks, QP and strides.
+ * d_level holds 2 * nblk blocks (U's, then V's), d_nnz and d_ssd 2 * nblk
+ * values.  One chroma QP serves both planes: the reference adds a per-plane
+ * delta only under ENABLE_WQUANT (rdo.c:818-820), which its build disables
+ * (common/defines.h:134). */
+int xme_recon_chroma_batch(int n, int qp, const pel_t *d_cur_u, const pel_t *d_cur_v, intptr_t i_cur,
+                           const pel_t *d_pred_u, const pel_t *d_pred_v, intptr_t i_pred, int plane_w, int plane_h,
+                           const int32_t *d_blocks, int nblk, coeff_t *d_level, pel_t *d_rec_u, pel_t *d_rec_v,
+                           intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd, void *stream);
+/* The chroma QP of a luma QP: tab_qp_scale_chroma at clip(0, 63, luma_qp),
+ * cu_get_chroma_qp (source/common/cudata.h:118-125, table cudata.c:98-106).
+ * Host only. */
+int xme_chroma_qp(int luma_qp);
 
 /* ---- 5. serving: continuous-batching ME for many concurrent encodes ------ *
  * The frame-level ME of many encodes (streams) share one GPU.  Each stream

@@ -195,6 +195,8 @@ def lib():
                                            P, P]
         L.xo_inter_recon_bi_batch.argtypes = [C.c_int, C.c_int, P, ip, P, P, ip, C.c_int, C.c_int, P, P, C.c_int, P,
                                               P, ip, P, P]
+        L.xo_recon_pred_batch.argtypes = [C.c_int, C.c_int, P, ip, P, ip, P, C.c_int, P, P, ip, P, P]
+        L.xo_chroma_qp.argtypes = [C.c_int]
         L.xo_tmat.restype = C.POINTER(C.c_int16)
         L.xo_tmat.argtypes = [C.c_int]
         _lib = L
@@ -582,6 +584,32 @@ def inter_recon_batch(n, qp, cur_plane, planes, frame_w, frame_h, tus, planes2=N
                                       plane_ptrs(planes2), planes[0].stride, frame_w, frame_h, ptr(tus), ptr(m2), ntu,
                                       ptr(level), ptr(rec), frame_w, ptr(nnz), ptr(ssd))
     return level, rec, nnz, ssd
+
+
+def recon_pred_batch(n, qp, cur, pred, blocks):
+    """Reconstruction of n x n blocks whose prediction is the picture pred
+    at the block's position (chroma of inter CUs); cur, pred: uint8 (h, w)
+    planes; blocks: int32 (nblk, 2) {x, y}.  Returns (level (nblk, n, n),
+    rec (h, w) with the blocks written, nnz, ssd)."""
+    cur = np.ascontiguousarray(cur, dtype=np.uint8)
+    pred = np.ascontiguousarray(pred, dtype=np.uint8)
+    assert cur.shape == pred.shape
+    blocks = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 2)
+    nblk = len(blocks)
+    ns = n & ~TU_DCT_HALF
+    h, w = cur.shape
+    assert nblk == 0 or (blocks.min() >= 0 and (blocks[:, 0] + ns).max() <= w and (blocks[:, 1] + ns).max() <= h)
+    level = np.zeros((nblk, ns, ns), np.int16)
+    rec = np.zeros((h, w), np.uint8)
+    nnz = np.zeros(nblk, np.int32)
+    ssd = np.zeros(nblk, np.int32)
+    lib().xo_recon_pred_batch(n, qp, ptr(cur), w, ptr(pred), w, ptr(blocks), nblk, ptr(level), ptr(rec), w,
+                              ptr(nnz), ptr(ssd))
+    return level, rec, nnz, ssd
+
+
+def chroma_qp(luma_qp):
+    return lib().xo_chroma_qp(int(luma_qp))
 
 
 def mc_chroma(qx, qy, w, h, pu, pv):

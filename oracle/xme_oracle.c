@@ -1866,6 +1866,40 @@ int xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv)
     return blk_pos * 4 + mv;
 }
 
+/* One n x n block of an inter CU, prediction p given: sub_ps, the residual
+ * path (dct or dct_half, quant, dequant, idct), add_ps with the clip, and the
+ * SSD of the reconstruction (cu_recon_chroma, rdo.c:807-838; the luma TUs of
+ * cu_recon_luma the same way). */
+static void recon_block(int n, int half, int qp, const uint8_t *c, intptr_t i_cur, const uint8_t *p, intptr_t i_p,
+                        int16_t *level, uint8_t *d, intptr_t i_rec, int32_t *nnz, int32_t *ssd)
+{
+    int16_t res[32 * 32], rr[32 * 32];
+    for (int y = 0; y < n; y++)                          /* sub_ps */
+        for (int x = 0; x < n; x++)
+            res[y * n + x] = (int16_t)(c[y * i_cur + x] - p[y * i_p + x]);
+    if (half) {
+        int qs, qsh, qa, ds, dsh;
+        int16_t co[32 * 32];
+        xo_quant_params(5, qp, 0, &qs, &qsh, &qa);
+        xo_dequant_params(5, qp, &ds, &dsh);
+        xo_dct_half_part(32, 32, res, co, 32);
+        *nnz = xo_quant(co, 1024, qs, qsh, qa);
+        memcpy(level, co, sizeof(co));
+        xo_dequant(co, 1024, ds, dsh);
+        xo_idct(32, co, rr, 32);
+    } else {
+        xo_residual_batch(n, qp, 0, res, level, rr, nnz, 1);
+    }
+    int e = 0;
+    for (int y = 0; y < n; y++)                          /* add_ps, then pixf.ssd */
+        for (int x = 0; x < n; x++) {
+            const int v = clip_pel(p[y * i_p + x] + rr[y * n + x]);
+            d[y * i_rec + x] = (uint8_t)v;
+            e += (c[y * i_cur + x] - v) * (c[y * i_cur + x] - v);
+        }
+    *ssd = e;
+}
+
 static void inter_recon(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
                         const uint8_t *const planes2[16], intptr_t i_ref, int frame_w, int frame_h,
                         const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
@@ -1890,34 +1924,37 @@ static void inter_recon(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, c
             xo_avg(r[4], r[5], pred, 64, pred, 64, tmp, 64);
         }
         const uint8_t *p = pred + (r[1] - r[3]) * 64 + (r[0] - r[2]);
-        const uint8_t *c = cur + (intptr_t)r[1] * i_cur + r[0];
-        int16_t res[32 * 32], rr[32 * 32];
-        for (int y = 0; y < n; y++)                      /* sub_ps */
-            for (int x = 0; x < n; x++)
-                res[y * n + x] = (int16_t)(c[y * i_cur + x] - p[y * 64 + x]);
-        if (half) {
-            int qs, qsh, qa, ds, dsh;
-            int16_t co[32 * 32];
-            xo_quant_params(5, qp, 0, &qs, &qsh, &qa);
-            xo_dequant_params(5, qp, &ds, &dsh);
-            xo_dct_half_part(32, 32, res, co, 32);
-            nnz[i] = xo_quant(co, 1024, qs, qsh, qa);
-            memcpy(level + (size_t)i * 1024, co, sizeof(co));
-            xo_dequant(co, 1024, ds, dsh);
-            xo_idct(32, co, rr, 32);
-        } else {
-            xo_residual_batch(n, qp, 0, res, level + (size_t)i * n * n, rr, nnz + i, 1);
-        }
-        uint8_t *d = rec + (intptr_t)r[1] * i_rec + r[0];
-        int e = 0;
-        for (int y = 0; y < n; y++)                      /* add_ps, then pixf.ssd */
-            for (int x = 0; x < n; x++) {
-                const int v = clip_pel(p[y * 64 + x] + rr[y * n + x]);
-                d[y * i_rec + x] = (uint8_t)v;
-                e += (c[y * i_cur + x] - v) * (c[y * i_cur + x] - v);
-            }
-        ssd[i] = e;
+        recon_block(n, half, qp, cur + (intptr_t)r[1] * i_cur + r[0], i_cur, p, 64, level + (size_t)i * n * n,
+                    rec + (intptr_t)r[1] * i_rec + r[0], i_rec, nnz + i, ssd + i);
     }
+}
+
+void xo_recon_pred_batch(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *pred, intptr_t i_pred,
+                         const int32_t *blocks, int nblk, int16_t *level, uint8_t *rec, intptr_t i_rec, int32_t *nnz,
+                         int32_t *ssd)
+{
+    const int n = n_arg & ~XO_TU_DCT_HALF, half = (n_arg & XO_TU_DCT_HALF) && n == 32;
+    for (int i = 0; i < nblk; i++) {
+        const int x = blocks[2 * i], y = blocks[2 * i + 1];
+        recon_block(n, half, qp, cur + (intptr_t)y * i_cur + x, i_cur, pred + (intptr_t)y * i_pred + x, i_pred,
+                    level + (size_t)i * n * n, rec + (intptr_t)y * i_rec + x, i_rec, nnz + i, ssd + i);
+    }
+}
+
+/* cu_get_chroma_qp (cudata.h:118-125): the luma QP clipped to 0..63, then
+ * tab_qp_scale_chroma (cudata.c:98-106), written here as its runs: the
+ * identity to 41, then each value twice from 42 to 47, 48..50 three times
+ * each, and 51. */
+int xo_chroma_qp(int luma_qp)
+{
+    const int q = luma_qp < 0 ? 0 : (luma_qp > 63 ? 63 : luma_qp);
+    if (q < 42)
+        return q;
+    if (q < 54)
+        return 42 + (q - 42) / 2;
+    if (q < 63)
+        return 48 + (q - 54) / 3;
+    return 51;
 }
 
 void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],

@@ -327,6 +327,15 @@ void xo_inter_recon_bi_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, 
                              const uint8_t *const planes2[16], intptr_t i_ref, int frame_w, int frame_h,
                              const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
                              intptr_t i_rec, int32_t *nnz, int32_t *ssd);
+/* The reconstruction of n x n blocks whose prediction is a picture (pred,
+ * stride i_pred) at the block's own position: the chroma blocks of inter CUs
+ * (cu_recon_chroma, rdo.c:785-846), one call per plane.  blocks[2 i] =
+ * {x, y}; level, nnz, ssd per block as above. */
+void xo_recon_pred_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *pred, intptr_t i_pred,
+                         const int32_t *blocks, int nblk, int16_t *level, uint8_t *rec, intptr_t i_rec, int32_t *nnz,
+                         int32_t *ssd);
+/* The chroma QP of a luma QP, cu_get_chroma_qp (cudata.h:118-125). */
+int xo_chroma_qp(int luma_qp);
 
 /* MV predictor, get_mvp_default (encoder/md_inter.c:470-524), for frame
  * sequences: the neighbours left, top, top-left and top-right (index 0..3)

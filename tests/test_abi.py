@@ -111,3 +111,12 @@ def test_struct_layouts_match_the_header(tmp_path):
     for cname, sname in (("xme_me_params", "MeParams"), ("xme_server_config", "ServerConfig"),
                          ("xme_ref_desc", "RefDesc")):
         assert int(got[cname]) == ctypes.sizeof(getattr(X, sname)), cname
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_chroma_qp_is_host_only_and_matches_the_oracle():
+    """xme_chroma_qp is a table lookup on the host: it runs without a GPU."""
+    from oracle import oracle as O
+    L = C.CDLL(LIB)
+    for q in range(-10, 81):
+        assert L.xme_chroma_qp(q) == O.chroma_qp(q), q

@@ -213,3 +213,202 @@ def test_one_launch_path_without_intermediates(n):
                                        len(tus), None, bufs[0].at(), None, C.c_void_p(dcur.origin), dcur.stride,
                                        bufs[1].at(), bufs[2].at(), None)
     assert rc == X.XME_ERR_ARG
+
+
+def gpu_recon_pred(n_arg, qp, cur, pred, blocks, pred_stride=None, d_pred=None):
+    """xme_recon_pred_batch on one plane; pred is uploaded at pred_stride (any), unless a
+    device plane d_pred already holds it."""
+    n = n_arg & 0xff
+    h, w = cur.shape
+    dcur = device_cur(cur)
+    if d_pred is None:
+        d_pred = X.DevicePlane(w, h, pad=0, stride=pred_stride or w)
+        d_pred.upload_picture(pred)
+    nblk = len(blocks)
+    d_blocks = X.upload(np.ascontiguousarray(blocks, np.int32).reshape(-1, 2))
+    level = X.DeviceBuffer(2 * max(nblk, 1) * n * n)
+    rec = X.DevicePlane(w, h, pad=0, stride=dcur.stride)
+    rec.buf.zero()
+    nnz, ssd = X.DeviceBuffer(4 * max(nblk, 1)), X.DeviceBuffer(4 * max(nblk, 1))
+    X.check(X.lib().xme_recon_pred_batch(n_arg, qp, C.c_void_p(dcur.origin), dcur.stride, C.c_void_p(d_pred.origin),
+                                         d_pred.stride, w, h, d_blocks.at(), nblk, level.at(), C.c_void_p(rec.origin),
+                                         rec.stride, nnz.at(), ssd.at(), None), "xme_recon_pred_batch")
+    X.check(X.lib().xme_stream_sync(None), "sync")
+    full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
+    return (level.download(nblk * n * n, np.int16).reshape(nblk, n, n), full[:h, :w],
+            nnz.download(nblk, np.int32), ssd.download(nblk, np.int32))
+
+
+def gpu_recon_chroma(n_arg, qp, curs, d_preds, blocks):
+    """xme_recon_chroma_batch: U and V of the same blocks in one launch; returns the
+    (level, rec, nnz, ssd) of each plane."""
+    n = n_arg & 0xff
+    h, w = curs[0].shape
+    dcur = [device_cur(c) for c in curs]
+    nblk = len(blocks)
+    d_blocks = X.upload(np.ascontiguousarray(blocks, np.int32).reshape(-1, 2))
+    level = X.DeviceBuffer(2 * 2 * max(nblk, 1) * n * n)
+    rec = [X.DevicePlane(w, h, pad=0, stride=dcur[0].stride) for _ in range(2)]
+    for r in rec:
+        r.buf.zero()
+    nnz, ssd = X.DeviceBuffer(8 * max(nblk, 1)), X.DeviceBuffer(8 * max(nblk, 1))
+    X.check(X.lib().xme_recon_chroma_batch(n_arg, qp, C.c_void_p(dcur[0].origin), C.c_void_p(dcur[1].origin),
+                                           dcur[0].stride, C.c_void_p(d_preds[0].origin),
+                                           C.c_void_p(d_preds[1].origin), d_preds[0].stride, w, h, d_blocks.at(),
+                                           nblk, level.at(), C.c_void_p(rec[0].origin), C.c_void_p(rec[1].origin),
+                                           rec[0].stride, nnz.at(), ssd.at(), None), "xme_recon_chroma_batch")
+    X.check(X.lib().xme_stream_sync(None), "sync")
+    lv = level.download(2 * nblk * n * n, np.int16).reshape(2, nblk, n, n)
+    nz, sd = nnz.download(2 * nblk, np.int32).reshape(2, nblk), ssd.download(2 * nblk, np.int32).reshape(2, nblk)
+    out = []
+    for p in range(2):
+        full = rec[p].buf.download(rec[p].rows * rec[p].stride, np.uint8).reshape(rec[p].rows, rec[p].stride)
+        out.append((lv[p], full[:h, :w], nz[p], sd[p]))
+    return out
+
+
+def pred_grid(w, h, n, rng, keep=0.8):
+    out = [(x, y) for y in range(0, h - n + 1, n) for x in range(0, w - n + 1, n) if rng.random() < keep]
+    rng.shuffle(out)
+    return np.array(out, np.int32).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("qp", [22, 43])
+@pytest.mark.parametrize("n_arg", [4, 8, 16, 32, 32 | X.XME_TU_DCT_HALF])
+def test_recon_pred_matches_oracle(n_arg, qp):
+    """xme_recon_pred_batch (cu_recon_chroma, rdo.c:807-838) on a prediction picture whose
+    rows start at odd addresses, blocks in random order."""
+    n = n_arg & 0xff
+    w, h = 160, 96
+    pred = synth.smooth_field(w, h, seed=81)
+    cur = synth.multi_ref_current([pred], 8, 8, seed=82)
+    blocks = pred_grid(w, h, n, np.random.default_rng(83 + n_arg))
+    got = gpu_recon_pred(n_arg, qp, cur, pred, blocks, pred_stride=w + 13)
+    exp = O.recon_pred_batch(n_arg, qp, cur, pred, blocks)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    assert got[2].any()
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_chroma_of_inter_cus(n):
+    """The chroma of 2n x 2n inter CUs end to end: xme_mc_chroma_batch writes each CU's
+    n x n U and V predictions into prediction pictures at the CU's chroma position, then
+    xme_recon_pred_batch reconstructs each plane at the chroma QP of luma QP 45
+    (xme_chroma_qp), 32x32 blocks with dct_half as a 64x64 CU's first RD pass does."""
+    W, H = 256, 128
+    cw, ch = W // 2, H // 2
+    rng = np.random.default_rng(90 + n)
+    src_u, src_v = synth.smooth_field(cw, ch, seed=91), synth.smooth_field(cw, ch, seed=92)
+    pu_o, pv_o = O.Plane(cw, ch, pad=40), O.Plane(cw, ch, pad=40)
+    for p, img in ((pu_o, src_u), (pv_o, src_v)):
+        p.set_picture(img)
+        p.expand_border()
+    du, dv = X.DevicePlane(cw, ch, pad=40), X.DevicePlane(cw, ch, pad=40)
+    du.upload_padded(pu_o.view()[: ch + 80, : cw + 80])
+    dv.upload_padded(pv_o.view()[: ch + 80, : cw + 80])
+    cus = []
+    for y in range(0, H, 2 * n):
+        for x in range(0, W, 2 * n):
+            mv = (int(rng.integers(-40, 41)), int(rng.integers(-40, 41)))
+            if rng.random() < 0.2:
+                mv = (4 * int(rng.integers(-200, 200)) + 3, 4 * int(rng.integers(-200, 200)) + 1)
+            cus.append((x, y, 2 * n, 2 * n) + mv)
+    cus = np.array(cus, np.int32)
+    ncu = len(cus)
+    pred_u, pred_v = X.DevicePlane(cw, ch, pad=0, stride=cw + 8), X.DevicePlane(cw, ch, pad=0, stride=cw + 8)
+    off = np.array([(int(c[1]) // 2) * pred_u.stride + int(c[0]) // 2 for c in cus], np.int64)
+    d_cus, d_off = X.upload(cus.ravel()), X.upload(off)
+    X.check(X.lib().xme_mc_chroma_batch(C.c_void_p(du.origin), C.c_void_p(dv.origin), du.stride, d_cus.at(), ncu, W,
+                                        H, C.c_void_p(pred_u.origin), C.c_void_p(pred_v.origin), pred_u.stride,
+                                        d_off.at(), None), "xme_mc_chroma_batch")
+    # the oracle's prediction pictures
+    eu, ev = np.zeros((ch, cw), np.uint8), np.zeros((ch, cw), np.uint8)
+    for x, y, w, h, mx, my in cus:
+        qx = O.lib().xo_get_mc_pos(W, int(w), int(x), int(mx))
+        qy = O.lib().xo_get_mc_pos(H, int(h), int(y), int(my))
+        bu, bv = O.mc_chroma(qx, qy, int(w) // 2, int(h) // 2, pu_o, pv_o)
+        eu[y // 2:y // 2 + n, x // 2:x // 2 + n], ev[y // 2:y // 2 + n, x // 2:x // 2 + n] = bu, bv
+    qp_c = X.lib().xme_chroma_qp(45)
+    assert qp_c == O.chroma_qp(45) == 43
+    n_arg = 32 | X.XME_TU_DCT_HALF if n == 32 else n
+    blocks = cus[:, :2] // 2
+    curs, exps = [], []
+    for seed, exp_pred, d_pred in ((93, eu, pred_u), (94, ev, pred_v)):
+        cur = synth.multi_ref_current([exp_pred], 4, 4, seed=seed)
+        got = gpu_recon_pred(n_arg, qp_c, cur, None, blocks, d_pred=d_pred)
+        exp = O.recon_pred_batch(n_arg, qp_c, cur, exp_pred, blocks)
+        for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+            assert np.array_equal(g, e), name
+        curs.append(cur)
+        exps.append(exp)
+    # both planes in one launch (xme_recon_chroma_batch)
+    both = gpu_recon_chroma(n_arg, qp_c, curs, (pred_u, pred_v), blocks)
+    for p in range(2):
+        for name, g, e in zip(("level", "rec", "nnz", "ssd"), both[p], exps[p]):
+            assert np.array_equal(g, e), (p, name)
+
+
+def test_recon_pred_skips_blocks_outside_the_plane():
+    w, h = 64, 48
+    pred = synth.smooth_field(w, h, seed=95)
+    cur = synth.multi_ref_current([pred], 4, 4, seed=96)
+    blocks = np.array([(16, 8), (18, 8), (60, 0), (0, 44), (-8, 0), (0, -8)], np.int32)
+    level, rec, nnz, ssd = gpu_recon_pred(8, 30, cur, pred, blocks)
+    assert ssd[0] >= 0 and list(ssd[1:]) == [-1] * 5
+    assert not level[1:].any() and not nnz[1:].any()
+    exp = O.recon_pred_batch(8, 30, cur, pred, blocks[:1])
+    assert np.array_equal(rec, exp[1]) and np.array_equal(level[0], exp[0][0]) and ssd[0] == exp[3][0]
+
+
+def test_recon_pred_arguments_and_empty_batch():
+    buf = X.DeviceBuffer(4096)
+    p = buf.at()
+    for n, qp in ((12, 30), (8, -1), (8, 80), (16 | X.XME_TU_DCT_HALF, 30)):
+        rc = X.lib().xme_recon_pred_batch(n, qp, p, 64, p, 64, 64, 16, p, 1, p, p, 64, p, p, None)
+        assert rc == X.XME_ERR_ARG
+    # a NULL buffer, a misaligned current picture or stride
+    assert X.lib().xme_recon_pred_batch(8, 30, None, 64, p, 64, 64, 16, p, 1, p, p, 64, p, p, None) == X.XME_ERR_ARG
+    assert X.lib().xme_recon_pred_batch(8, 30, buf.at(2), 64, p, 64, 64, 16, p, 1, p, p, 64, p, p,
+                                        None) == X.XME_ERR_ARG
+    assert X.lib().xme_recon_pred_batch(8, 30, p, 66, p, 64, 64, 16, p, 1, p, p, 64, p, p, None) == X.XME_ERR_ARG
+    assert X.lib().xme_recon_pred_batch(8, 30, None, 64, None, 64, 64, 16, None, 0, None, None, 64, None, None,
+                                        None) == X.XME_OK
+    # the two-plane call: a missing V plane, a misaligned V reconstruction, bad sizes
+    L = X.lib()
+    assert L.xme_recon_chroma_batch(8, 30, p, None, 64, p, p, 64, 64, 16, p, 1, p, p, p, 64, p, p,
+                                    None) == X.XME_ERR_ARG
+    assert L.xme_recon_chroma_batch(8, 30, p, p, 64, p, None, 64, 64, 16, p, 1, p, p, p, 64, p, p,
+                                    None) == X.XME_ERR_ARG
+    assert L.xme_recon_chroma_batch(8, 30, p, p, 64, p, p, 64, 64, 16, p, 1, p, p, buf.at(1), 64, p, p,
+                                    None) == X.XME_ERR_ARG
+    assert L.xme_recon_chroma_batch(12, 30, p, p, 64, p, p, 64, 64, 16, p, 1, p, p, p, 64, p, p,
+                                    None) == X.XME_ERR_ARG
+    assert L.xme_recon_chroma_batch(8, 30, None, None, 64, None, None, 64, 64, 16, None, 0, None, None, None, 64,
+                                    None, None, None) == X.XME_OK
+
+
+@pytest.mark.parametrize("n_arg", [4, 16, 32 | X.XME_TU_DCT_HALF])
+def test_recon_chroma_matches_two_single_plane_calls(n_arg):
+    """U and V in one launch equal two xme_recon_pred_batch calls, with a ragged block
+    count (not a multiple of a tile's blocks) so that the planes meet inside a tile."""
+    n = n_arg & 0xff
+    w, h = 160, 96
+    rng = np.random.default_rng(97 + n_arg)
+    blocks = pred_grid(w, h, n, rng, keep=0.9)
+    per_tile = (32 // n) ** 2
+    if per_tile > 1 and len(blocks) % per_tile == 0:
+        blocks = blocks[:-1]
+    curs, preds, d_preds, exps = [], [], [], []
+    for seed in (98, 99):
+        pred = synth.smooth_field(w, h, seed=seed)
+        cur = synth.multi_ref_current([pred], 8, 8, seed=seed + 10)
+        d = X.DevicePlane(w, h, pad=0, stride=w + 5)
+        d.upload_picture(pred)
+        curs.append(cur)
+        d_preds.append(d)
+        exps.append(gpu_recon_pred(n_arg, 37, cur, None, blocks, d_pred=d))
+    both = gpu_recon_chroma(n_arg, 37, curs, d_preds, blocks)
+    for p in range(2):
+        for name, g, e in zip(("level", "rec", "nnz", "ssd"), both[p], exps[p]):
+            assert np.array_equal(g, e), (p, name)

@@ -207,3 +207,69 @@ def test_dct_half_keeps_a_smooth_residual_like_the_full_transform():
     b = O.inter_recon_batch(32, 27, cur_plane(cur), planes, w, h, tus)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def pred_blocks(w, h, n, rng, keep=0.7):
+    """Non-overlapping n x n blocks of a w x h plane, a random subset, in random order."""
+    out = [(x, y) for y in range(0, h - n + 1, n) for x in range(0, w - n + 1, n) if rng.random() < keep]
+    rng.shuffle(out)
+    return np.array(out, np.int32).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_recon_pred_known_answer(n):
+    """A prediction equal to the picture (cu_recon_chroma, rdo.c:807-838, on an exact
+    prediction): zero levels, nnz 0, the reconstruction is the picture, SSD 0."""
+    w, h = 96, 64
+    cur = texture(w, h, 11)
+    blocks = pred_blocks(w, h, n, np.random.default_rng(n))
+    level, rec, nnz, ssd = O.recon_pred_batch(n, 30, cur, cur, blocks)
+    assert not level.any() and not nnz.any() and not ssd.any()
+    for x, y in blocks:
+        assert np.array_equal(rec[y:y + n, x:x + n], cur[y:y + n, x:x + n])
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+@pytest.mark.parametrize("qp", [22, 39, 51])
+def test_recon_pred_matches_numpy_composition(n, qp):
+    """cur - pred, the residual stages (pinned on their own), numpy add / clip and SSD."""
+    w, h = 96, 64
+    cur, pred = texture(w, h, 12), texture(w, h, 13)
+    blocks = pred_blocks(w, h, n, np.random.default_rng(qp + n))
+    level, rec, nnz, ssd = O.recon_pred_batch(n, qp, cur, pred, blocks)
+    res = np.array([cur[y:y + n, x:x + n].astype(np.int16) - pred[y:y + n, x:x + n] for x, y in blocks])
+    lv, rr, nz = O.residual_batch(n, qp, 0, res)
+    assert np.array_equal(level, lv) and np.array_equal(nnz, nz)
+    exp = np.zeros_like(cur)
+    for i, (x, y) in enumerate(blocks):
+        v = np.clip(pred[y:y + n, x:x + n].astype(np.int32) + rr[i], 0, 255)
+        exp[y:y + n, x:x + n] = v
+        assert ssd[i] == int(((cur[y:y + n, x:x + n].astype(np.int64) - v) ** 2).sum())
+    assert np.array_equal(rec, exp) and nnz.any()
+
+
+@pytest.mark.parametrize("n_arg", [8, 32, 32 | O.TU_DCT_HALF])
+def test_recon_pred_equals_the_luma_path_at_mv_zero(n_arg):
+    """At MV (0, 0) the luma TU's prediction is the reference picture at the TU, so the
+    luma path (pinned above, dct_half included) and the prediction-picture path agree."""
+    n = n_arg & 0xff
+    w, h = 128, 96
+    cur, ref = texture(w, h, 14), texture(w, h, 15)
+    blocks = pred_blocks(w, h, n, np.random.default_rng(n_arg), keep=1.0)
+    tus = np.array([(x, y, x, y, n, n, 0, 0) for x, y in blocks], np.int32)
+    a = O.inter_recon_batch(n_arg, 33, cur_plane(cur), planes_of(ref), w, h, tus)
+    b = O.recon_pred_batch(n_arg, 33, cur, ref, blocks)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_chroma_qp():
+    """cu_get_chroma_qp (cudata.h:118-125): the identity to 41, then the runs of
+    tab_qp_scale_chroma (cudata.c:98-106), read off the table at these points; the luma QP
+    is clipped to 0..63 first."""
+    hand = {0: 0, 17: 17, 41: 41, 42: 42, 43: 42, 44: 43, 47: 44, 48: 45, 51: 46, 53: 47, 54: 48, 56: 48,
+            57: 49, 59: 49, 60: 50, 62: 50, 63: 51, -3: 0, 64: 51, 79: 51}
+    for q, c in hand.items():
+        assert O.chroma_qp(q) == c, q
+    seq = [O.chroma_qp(q) for q in range(64)]
+    assert all(0 <= b - a <= 1 for a, b in zip(seq, seq[1:]))

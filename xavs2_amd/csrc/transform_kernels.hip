@@ -50,6 +50,11 @@ enum { DO_FWD = 1, DO_QUANT = 2, DO_DEQUANT = 4, DO_INV = 8, DO_HALF = 16 };
 // pu_h, mv_x, mv_y}; the prediction comes from the interpolated planes (and
 // for FU = 2 the rounded average with planes2 at mv2), the input is cur - pred,
 // and the reconstruction clip(pred + residual) goes into rec with its SSD.
+// FU = 3 takes the prediction from a picture (planes[0], stride i_ref) at
+// the block's own position, blocks listed as {x, y} in `tus`, the plane
+// fw x fh: the chroma reconstruction of inter CUs.  Blocks nb1 .. 2 nb1 - 1
+// repeat the list on a second plane (planes[1], cur2, rec2; same strides):
+// U and V in one launch.
 struct FusedArgs {
     const uint8_t *planes[16];
     const uint8_t *planes2[16];
@@ -63,6 +68,9 @@ struct FusedArgs {
     int fw, fh;
     int16_t *resid;        // the prediction error, when not null
     int32_t *ssd;
+    const uint8_t *cur2;   // FU = 3, second plane
+    uint8_t *rec2;
+    int64_t nb1;           // FU = 3: blocks per plane
 };
 
 // Constant MFMA operands and accumulator starts of one block size, per lane.
@@ -221,7 +229,21 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             // lane b < BPT takes TU t * BPT + b: validity and base pointers
             uint64_t my_pred = 0, my_pred2 = 0, my_cur = 0;
             const int64_t bt = t * BPT + lane;
-            if (lane < BPT && bt < nblk) {
+            if constexpr (FU == 3) {
+                // the prediction picture: block (x, y) of every plane
+                if (lane < BPT && bt < nblk) {
+                    const bool second = bt >= fa.nb1;
+                    const int64_t bi = second ? bt - fa.nb1 : bt;
+                    const int bx = fa.tus[2 * bi], by = fa.tus[2 * bi + 1];
+                    tu_good = (bx & 3) == 0 && bx >= 0 && by >= 0 && bx <= fa.fw - N && by <= fa.fh - N;
+                    if (tu_good) {
+                        my_pred = (uint64_t)(uintptr_t)((second ? fa.planes[1] : fa.planes[0]) +
+                                                        (int64_t)by * fa.i_ref + bx);
+                        my_cur = (uint64_t)(uintptr_t)((second ? fa.cur2 : fa.cur) + (int64_t)by * fa.i_cur + bx);
+                        my_rec = (uint64_t)(uintptr_t)((second ? fa.rec2 : fa.rec) + (int64_t)by * fa.i_rec + bx);
+                    }
+                }
+            } else if (lane < BPT && bt < nblk) {
                 int r[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++)
@@ -535,6 +557,12 @@ int launch_fu(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t
     return XME_OK;
 }
 
+// tab_qp_scale_chroma, source/common/cudata.c:98-106.
+const uint8_t kQpScaleChroma[64] = { 0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13, 14, 15,
+                                     16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31,
+                                     32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 42, 43, 43, 44, 44,
+                                     45, 45, 46, 46, 47, 47, 48, 48, 48, 49, 49, 49, 50, 50, 50, 51 };
+
 int launch(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t *nnz, int nblk, QParams q, int mode,
            hipStream_t s)
 {
@@ -585,6 +613,84 @@ int xme_recon_fused(int n, int qp, int half, int bi, const uint8_t *const planes
     const QParams q = qparams(n, qp, 0);
     return bi ? launch_fu<2>(n, nullptr, level, rres, nnz, ntu, q, mode, fa, s)
               : launch_fu<1>(n, nullptr, level, rres, nnz, ntu, q, mode, fa, s);
+}
+
+extern "C" int xme_chroma_qp(int luma_qp) { return kQpScaleChroma[luma_qp < 0 ? 0 : (luma_qp > 63 ? 63 : luma_qp)]; }
+
+namespace {
+
+// U and V (or one plane when cur_v is null) of nblk blocks in one launch.
+int recon_pred(const char *who, int n_arg, int qp, const pel_t *cur_u, const pel_t *cur_v, intptr_t i_cur,
+               const pel_t *pred_u, const pel_t *pred_v, intptr_t i_pred, int plane_w, int plane_h,
+               const int32_t *blocks, int nblk, coeff_t *level, pel_t *rec_u, pel_t *rec_v, intptr_t i_rec,
+               int32_t *nnz, int32_t *ssd, hipStream_t s)
+{
+    const int n = n_arg & ~XME_TU_DCT_HALF, half = (n_arg & XME_TU_DCT_HALF) ? 1 : 0;
+    if ((n != 4 && n != 8 && n != 16 && n != 32) || (half && n != 32) || qp < 0 || qp > 79 || nblk < 0 ||
+        plane_w <= 0 || plane_h <= 0) {
+        xme_set_error("%s: bad arguments (n 4, 8, 16 or 32; XME_TU_DCT_HALF with 32 only; qp 0..79)", who);
+        return XME_ERR_ARG;
+    }
+    if (nblk == 0)
+        return XME_OK;
+    const bool two = cur_v != nullptr;
+    if (!cur_u || !pred_u || !blocks || !level || !rec_u || !nnz || !ssd || (two && (!pred_v || !rec_v))) {
+        xme_set_error("%s: NULL buffer", who);
+        return XME_ERR_ARG;
+    }
+    // the current and reconstructed pictures are read and written as aligned
+    // dwords; the prediction is read at any alignment
+    const uintptr_t al = (uintptr_t)cur_u | (uintptr_t)rec_u | (two ? ((uintptr_t)cur_v | (uintptr_t)rec_v) : 0);
+    if ((al & 3) || (i_cur & 3) || (i_rec & 3) || ((uintptr_t)level & 15)) {
+        xme_set_error("%s: pictures must be 4-byte and the levels 16-byte aligned", who);
+        return XME_ERR_ARG;
+    }
+    FusedArgs fa{};
+    fa.planes[0] = pred_u;
+    fa.planes[1] = two ? pred_v : pred_u;
+    fa.i_ref = i_pred;
+    fa.cur = cur_u;
+    fa.cur2 = two ? cur_v : cur_u;
+    fa.i_cur = i_cur;
+    fa.rec = rec_u;
+    fa.rec2 = two ? rec_v : rec_u;
+    fa.i_rec = i_rec;
+    fa.tus = blocks;
+    fa.nb1 = nblk;
+    fa.fw = plane_w;
+    fa.fh = plane_h;
+    fa.ssd = ssd;
+    const int mode = DO_FWD | DO_QUANT | DO_DEQUANT | DO_INV | (half ? DO_HALF : 0);
+    return launch_fu<3>(n, nullptr, level, nullptr, nnz, two ? 2 * nblk : nblk, qparams(n, qp, 0), mode, fa, s);
+}
+
+} // namespace
+
+extern "C" int xme_recon_pred_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const pel_t *d_pred,
+                                    intptr_t i_pred, int plane_w, int plane_h, const int32_t *d_blocks, int nblk,
+                                    coeff_t *d_level, pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd,
+                                    void *stream)
+{
+    return recon_pred("xme_recon_pred_batch", n, qp, d_cur, nullptr, i_cur, d_pred, nullptr, i_pred, plane_w, plane_h,
+                      d_blocks, nblk, d_level, d_rec, nullptr, i_rec, d_nnz, d_ssd, (hipStream_t)stream);
+}
+
+extern "C" int xme_recon_chroma_batch(int n, int qp, const pel_t *d_cur_u, const pel_t *d_cur_v, intptr_t i_cur,
+                                      const pel_t *d_pred_u, const pel_t *d_pred_v, intptr_t i_pred, int plane_w,
+                                      int plane_h, const int32_t *d_blocks, int nblk, coeff_t *d_level,
+                                      pel_t *d_rec_u, pel_t *d_rec_v, intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd,
+                                      void *stream)
+{
+    if (nblk > 0 && !d_cur_v) {
+        xme_set_error("xme_recon_chroma_batch: NULL buffer");
+        return XME_ERR_ARG;
+    }
+    if ((int64_t)nblk * 2 > INT32_MAX) {
+        xme_set_error("xme_recon_chroma_batch: too many blocks");
+        return XME_ERR_ARG;
+    }
+    return recon_pred("xme_recon_chroma_batch", n, qp, d_cur_u, d_cur_v, i_cur, d_pred_u, d_pred_v, i_pred, plane_w,
+                      plane_h, d_blocks, nblk, d_level, d_rec_u, d_rec_v, i_rec, d_nnz, d_ssd, (hipStream_t)stream);
 }
 
 extern "C" int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *in, coeff_t *level, coeff_t *recon,

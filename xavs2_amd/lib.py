@@ -191,6 +191,9 @@ def lib():
         L.xme_inter_recon_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), I, I, P, I, P, P, P, P, Z, P, P, P]
         L.xme_inter_recon_bi_batch.argtypes = [I, I, P, Z, C.POINTER(RefDesc), C.POINTER(RefDesc), I, I, P, P, I,
                                                P, P, P, P, Z, P, P, P]
+        L.xme_recon_pred_batch.argtypes = [I, I, P, Z, P, Z, I, I, P, I, P, P, Z, P, P, P]
+        L.xme_chroma_qp.argtypes = [I]
+        L.xme_recon_chroma_batch.argtypes = [I, I, P, P, Z, P, P, Z, I, I, P, I, P, P, P, Z, P, P, P]
         L.xme_get_sad_table.argtypes = [P]
         L.xme_get_dct_table.argtypes = [P, P, P]
         L.xme_get_quant_funcs.argtypes = [P, P]
