@@ -460,19 +460,20 @@ def callers_leg(X, w, h, R, lam, arr, gpu_res, n_ref, dcur, d_refs, refs, cp, pl
 
 def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, qp=32):
     """Luma inter reconstruction of the frame (xme_inter_recon_batch): every
-    PU split into the largest square TUs (4..32) that tile it, predicted from
-    reference 0 at 4 x its full-pel best, at QP 32; one call per TU size,
-    each one launch (the planes are resident; no prediction-error or residual
-    buffers).  Algorithmic HBM bytes: 5 per pixel (current and prediction
-    read, levels (int16) and reconstruction written).  Parity with the oracle
-    on the TUs of every 4th PU."""
+    PU split into the largest square TUs (4..64) that tile it, a 64x64 PU
+    one 64x64 TU (wavelet), predicted from reference 0 at 4 x its full-pel
+    best, at QP 32; one call per TU size, each one launch (the planes are
+    resident; no prediction-error or residual buffers).  Algorithmic HBM
+    bytes: 3 per pixel (current and prediction read, reconstruction written)
+    and 2 per level (n x n levels, 32 x 32 for a 64x64 TU).  Parity with the
+    oracle on the TUs of every 4th PU."""
     import ctypes as C
     from oracle import oracle as O
     r0 = gpu_res[0::n_ref]
-    by_n = {4: [], 8: [], 16: [], 32: []}
-    sample = {4: [], 8: [], 16: [], 32: []}
+    by_n = {4: [], 8: [], 16: [], 32: [], 64: []}
+    sample = {4: [], 8: [], 16: [], 32: [], 64: []}
     for i, (x, y, pw, ph) in enumerate(pus):
-        n = 32
+        n = 64
         while pw % n or ph % n:
             n >>= 1
         mv = (4 * int(r0["bmv2_x"][i]), 4 * int(r0["bmv2_y"][i]))
@@ -489,7 +490,8 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
             continue
         t = np.array(rows, np.int32)
         ntu = len(t)
-        bufs.append((n, ntu, X.upload(t), None, X.DeviceBuffer(2 * ntu * n * n), None, X.DeviceBuffer(4 * ntu),
+        nl = min(n, 32)
+        bufs.append((n, ntu, X.upload(t), None, X.DeviceBuffer(2 * ntu * nl * nl), None, X.DeviceBuffer(4 * ntu),
                      X.DeviceBuffer(4 * ntu)))
 
     def frame():
@@ -508,13 +510,15 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
     stream.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
     pixels = sum(ntu * n * n for n, ntu, *_ in bufs)
+    alg_bytes = sum(ntu * (3 * n * n + 2 * min(n, 32) ** 2) for n, ntu, *_ in bufs)
     leg = {
         "workload": f"{w}x{h} luma, {sum(b[1] for b in bufs)} TUs ("
                     + ", ".join(f"{b[1]} of {b[0]}x{b[0]}" for b in bufs) + f"), reference 0, QP {qp}",
         "ms_per_frame": ms,
         "pixels_per_s": pixels / (ms * 1e-3),
-        "algorithmic_gb_per_s": 5.0 * pixels / (ms * 1e-3) / 1e9,
-        "hbm_frac": 5.0 * pixels / (ms * 1e-3) / 8e12,
+        "algorithmic_bytes_per_frame": alg_bytes,
+        "algorithmic_gb_per_s": alg_bytes / (ms * 1e-3) / 1e9,
+        "hbm_frac": alg_bytes / (ms * 1e-3) / 8e12,
     }
     if cpu:
         p = O.Plane(w, h)
@@ -532,7 +536,8 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
             el_level, el_rec, el_nnz, el_ssd = O.inter_recon_batch(n, qp, cp, planes, w, h, t)
             el += time.perf_counter() - t0
             n_tu += len(t)
-            g_level = level.download(ntu * n * n, np.int16).reshape(ntu, n, n)[sel]
+            nl = min(n, 32)
+            g_level = level.download(ntu * nl * nl, np.int16).reshape(ntu, nl, nl)[sel]
             ok = ok and np.array_equal(g_level, el_level) and np.array_equal(nnz.download(ntu, np.int32)[sel], el_nnz)
             ok = ok and np.array_equal(ssd.download(ntu, np.int32)[sel], el_ssd)
             for (tx, ty, *_r) in t:

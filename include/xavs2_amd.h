@@ -619,9 +619,15 @@ int xme_residual_batch(int n, int qp, int b_intra, const coeff_t *d_in, coeff_t 
  * inside its PU, at a column that is not a multiple of 4, or whose PU is not
  * 4..64 on each side gets ssd -1, zero levels and no pixels.  d_cur / d_rec:
  * 4-byte aligned with strides % 4 == 0; d_resid / d_rres: 8-byte aligned.
- * n = 32 | XME_TU_DCT_HALF transforms with dct_half instead, as
- * tu_get_dct_coeff (rdo.c:764-771) does for 32x32 TUs in the first RD pass
- * under OPT_BIT_EST_PSZT (the default preset's, presets.c:277). */
+ * n = 64: the TU of an unsplit 64x64 CU (its PU 64x64), transformed as
+ * dct_64x64 / idct_64x64 (source/common/transform.c:1514-1541: the lifting
+ * wavelet's 32x32 low band through the 32x32 transform with the wavelet
+ * flag) and quantised at level 6 (tu_recon_inter_luma's i_level); d_level
+ * then holds 32 * 32 levels per TU, and d_resid / d_rres are not written.
+ * n = 64 needs every interpolated plane of the reference(s).
+ * n = 32 or 64 | XME_TU_DCT_HALF transforms with dct_half instead, as
+ * tu_get_dct_coeff (rdo.c:764-771) does for TUs of 32 and 64 in the first RD
+ * pass under OPT_BIT_EST_PSZT (the default preset's, presets.c:277). */
 #define XME_TU_DCT_HALF 0x100
 int xme_inter_recon_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const xme_ref_desc *ref,
                           int frame_w, int frame_h, const int32_t *d_tus, int ntu, coeff_t *d_resid,
@@ -635,18 +641,22 @@ int xme_inter_recon_bi_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, 
                              const xme_ref_desc *ref2, int frame_w, int frame_h, const int32_t *d_tus,
                              const int32_t *d_mv2, int ntu, coeff_t *d_resid, coeff_t *d_level, coeff_t *d_rres,
                              pel_t *d_rec, intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd, void *stream);
-/* Reconstruction of nblk N x N blocks (n in 4, 8, 16, 32, or
- * 32 | XME_TU_DCT_HALF) whose prediction is a picture d_pred (stride i_pred)
+/* Reconstruction of nblk N x N blocks (n in 4, 8, 16, 32, 64, or 32 / 64 |
+ * XME_TU_DCT_HALF) whose prediction is a picture d_pred (stride i_pred)
  * at the block's own position: the chroma of an inter CU, one call per
  * plane (cu_recon_chroma, source/encoder/rdo.c:785-846: sub_ps, the
  * transform of tu_get_dct_coeff, inter quant, dequant, IDCT, add_ps, and
  * pixf.ssd against d_cur), with the prediction of xme_mc_chroma_batch
- * written into d_pred.  qp is the chroma QP (xme_chroma_qp).  d_blocks:
- * nblk x {x, y} in the plane_w x plane_h plane.  d_level (nblk * n * n,
- * 16-byte aligned), d_nnz and d_ssd per block; the reconstruction goes into
- * d_rec at the block.  A block outside the plane or at a column that is not a
- * multiple of 4 gets ssd -1, zero levels and no pixels.  d_cur / d_rec:
- * 4-byte aligned with strides % 4 == 0. */
+ * written into d_pred; and the luma TUs of a CU whose prediction
+ * (buf_pred_inter of cu_recon_inter_luma, rdo.c:1378-1493) is written by
+ * xme_mc_luma_batch, such as an unsplit TU over two PUs, with n = 64 as in
+ * xme_inter_recon_batch.  qp is the chroma QP (xme_chroma_qp) or the luma
+ * one.  d_blocks: nblk x {x, y} in the plane_w x plane_h plane.  d_level
+ * (nblk * n * n, 32 * 32 for n = 64; 16-byte aligned), d_nnz and d_ssd per
+ * block; the reconstruction goes into d_rec at the block.  A block outside
+ * the plane or at a column that is not a multiple of 4 gets ssd -1, zero
+ * levels and no pixels.  d_cur / d_pred / d_rec: 4-byte aligned with strides
+ * % 4 == 0. */
 int xme_recon_pred_batch(int n, int qp, const pel_t *d_cur, intptr_t i_cur, const pel_t *d_pred, intptr_t i_pred,
                          int plane_w, int plane_h, const int32_t *d_blocks, int nblk, coeff_t *d_level, pel_t *d_rec,
                          intptr_t i_rec, int32_t *d_nnz, int32_t *d_ssd, void *stream);

@@ -565,12 +565,14 @@ def inter_recon_batch(n, qp, cur_plane, planes, frame_w, frame_h, tus, planes2=N
     """Luma inter reconstruction of n x n TUs; tus: int32 (ntu, 8) records
     {tu_x, tu_y, pu_x, pu_y, pu_w, pu_h, mv_x, mv_y}; with planes2 and mv2
     (int32 (ntu, 2)) the TUs are bi-predicted; n = 32 | TU_DCT_HALF uses
-    dct_half.  Returns (level (ntu, n, n), rec (frame_h, frame_w) picture with
-    the TUs written, nnz, ssd)."""
+    dct_half; n = 64 transforms the wavelet's low band and keeps 32x32 levels.
+    Returns (level (ntu, n, n), rec (frame_h, frame_w) picture with the TUs
+    written, nnz, ssd)."""
     tus = np.ascontiguousarray(tus, dtype=np.int32).reshape(-1, 8)
     ntu = len(tus)
     ns = n & ~TU_DCT_HALF
-    level = np.zeros((ntu, ns, ns), np.int16)
+    nl = min(ns, 32)                 # a 64x64 TU keeps the 32x32 levels of its wavelet low band
+    level = np.zeros((ntu, nl, nl), np.int16)
     rec = np.zeros((frame_h, frame_w), np.uint8)
     nnz = np.zeros(ntu, np.int32)
     ssd = np.zeros(ntu, np.int32)
@@ -597,9 +599,10 @@ def recon_pred_batch(n, qp, cur, pred, blocks):
     blocks = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 2)
     nblk = len(blocks)
     ns = n & ~TU_DCT_HALF
+    nl = min(ns, 32)
     h, w = cur.shape
     assert nblk == 0 or (blocks.min() >= 0 and (blocks[:, 0] + ns).max() <= w and (blocks[:, 1] + ns).max() <= h)
-    level = np.zeros((nblk, ns, ns), np.int16)
+    level = np.zeros((nblk, nl, nl), np.int16)
     rec = np.zeros((h, w), np.uint8)
     nnz = np.zeros(nblk, np.int32)
     ssd = np.zeros(nblk, np.int32)

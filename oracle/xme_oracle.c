@@ -1873,20 +1873,28 @@ int xo_get_mc_pos(int img_size, int blk_size, int blk_pos, int mv)
 static void recon_block(int n, int half, int qp, const uint8_t *c, intptr_t i_cur, const uint8_t *p, intptr_t i_p,
                         int16_t *level, uint8_t *d, intptr_t i_rec, int32_t *nnz, int32_t *ssd)
 {
-    int16_t res[32 * 32], rr[32 * 32];
+    int16_t res[64 * 64], rr[64 * 64];
     for (int y = 0; y < n; y++)                          /* sub_ps */
         for (int x = 0; x < n; x++)
             res[y * n + x] = (int16_t)(c[y * i_cur + x] - p[y * i_p + x]);
-    if (half) {
+    if (half || n == 64) {
+        /* dct_half (32x32), or a 64x64 TU: dct / dct_half of LUMA_64x64 (the
+         * wavelet's 32x32 low band transformed with the wavelet flag,
+         * transform.c:1514-1541), quantised at level 6 (i_level of
+         * tu_recon_inter_luma, rdo.c:1351-1356), 32x32 levels */
+        const int l = n == 64 ? 6 : 5;
         int qs, qsh, qa, ds, dsh;
         int16_t co[32 * 32];
-        xo_quant_params(5, qp, 0, &qs, &qsh, &qa);
-        xo_dequant_params(5, qp, &ds, &dsh);
-        xo_dct_half_part(32, 32, res, co, 32);
+        xo_quant_params(l, qp, 0, &qs, &qsh, &qa);
+        xo_dequant_params(l, qp, &ds, &dsh);
+        if (half)
+            xo_dct_half_part(n, n, res, co, n);
+        else
+            xo_dct_part(n, n, res, co, n);
         *nnz = xo_quant(co, 1024, qs, qsh, qa);
         memcpy(level, co, sizeof(co));
         xo_dequant(co, 1024, ds, dsh);
-        xo_idct(32, co, rr, 32);
+        xo_idct_part(n, n, co, rr, n);
     } else {
         xo_residual_batch(n, qp, 0, res, level, rr, nnz, 1);
     }
@@ -1905,8 +1913,10 @@ static void inter_recon(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, c
                         const int32_t *tus, const int32_t *mv2, int ntu, int16_t *level, uint8_t *rec,
                         intptr_t i_rec, int32_t *nnz, int32_t *ssd)
 {
-    /* tu_get_dct_coeff (rdo.c:764-771): dct_half for a 32x32 TU when asked */
-    const int n = n_arg & ~XO_TU_DCT_HALF, half = (n_arg & XO_TU_DCT_HALF) && n == 32;
+    /* tu_get_dct_coeff (rdo.c:764-771): dct_half for a 32x32 or 64x64 TU when
+     * asked; a 64x64 TU keeps 32x32 levels */
+    const int n = n_arg & ~XO_TU_DCT_HALF, half = (n_arg & XO_TU_DCT_HALF) && n >= 32;
+    const size_t ncoef = n == 64 ? 1024 : (size_t)n * n;
     for (int i = 0; i < ntu; i++) {
         const int32_t *r = tus + 8 * i;
         /* the PU's prediction (get_mv_for_mc with the PU size), then the TU's
@@ -1924,7 +1934,7 @@ static void inter_recon(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, c
             xo_avg(r[4], r[5], pred, 64, pred, 64, tmp, 64);
         }
         const uint8_t *p = pred + (r[1] - r[3]) * 64 + (r[0] - r[2]);
-        recon_block(n, half, qp, cur + (intptr_t)r[1] * i_cur + r[0], i_cur, p, 64, level + (size_t)i * n * n,
+        recon_block(n, half, qp, cur + (intptr_t)r[1] * i_cur + r[0], i_cur, p, 64, level + (size_t)i * ncoef,
                     rec + (intptr_t)r[1] * i_rec + r[0], i_rec, nnz + i, ssd + i);
     }
 }
@@ -1933,11 +1943,12 @@ void xo_recon_pred_batch(int n_arg, int qp, const uint8_t *cur, intptr_t i_cur, 
                          const int32_t *blocks, int nblk, int16_t *level, uint8_t *rec, intptr_t i_rec, int32_t *nnz,
                          int32_t *ssd)
 {
-    const int n = n_arg & ~XO_TU_DCT_HALF, half = (n_arg & XO_TU_DCT_HALF) && n == 32;
+    const int n = n_arg & ~XO_TU_DCT_HALF, half = (n_arg & XO_TU_DCT_HALF) && n >= 32;
+    const size_t ncoef = n == 64 ? 1024 : (size_t)n * n;
     for (int i = 0; i < nblk; i++) {
         const int x = blocks[2 * i], y = blocks[2 * i + 1];
         recon_block(n, half, qp, cur + (intptr_t)y * i_cur + x, i_cur, pred + (intptr_t)y * i_pred + x, i_pred,
-                    level + (size_t)i * n * n, rec + (intptr_t)y * i_rec + x, i_rec, nnz + i, ssd + i);
+                    level + i * ncoef, rec + (intptr_t)y * i_rec + x, i_rec, nnz + i, ssd + i);
     }
 }
 

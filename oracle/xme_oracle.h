@@ -317,7 +317,9 @@ void xo_border_row_window(int h, int row_log2, int lcu_y, int *y0, int *y1);
  * prediction is mc_luma at the PU's clamped position, offset to the TU.
  * Writes level (n*n per TU), the reconstruction into rec (stride i_rec, at
  * the TU's position), nnz and the SSD against cur per TU. */
-#define XO_TU_DCT_HALF 0x100     /* n = 32 | XO_TU_DCT_HALF: dct_half (rdo.c:764-771) */
+/* n = 64: the TU's transform is dct_64x64 (the wavelet's 32x32 low band,
+ * transform.c:1514-1541) at quant level 6, 1024 levels per TU. */
+#define XO_TU_DCT_HALF 0x100     /* n = 32 or 64 | XO_TU_DCT_HALF: dct_half (rdo.c:764-771) */
 void xo_inter_recon_batch(int n, int qp, const uint8_t *cur, intptr_t i_cur, const uint8_t *const planes[16],
                           intptr_t i_ref, int frame_w, int frame_h, const int32_t *tus, int ntu, int16_t *level,
                           uint8_t *rec, intptr_t i_rec, int32_t *nnz, int32_t *ssd);

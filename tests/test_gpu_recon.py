@@ -22,6 +22,7 @@ def gpu_recon(n_arg, qp, cur, ref_img, tus, frac, intermediates=True, want_inter
     """xme_inter_recon_batch; n_arg may carry XME_TU_DCT_HALF.  With frac 2 every plane is
     resident (the one-launch path), with frac 0 the block filters predict (staged path)."""
     n = n_arg & 0xff
+    nl = min(n, 32)
     h, w = cur.shape
     dref = X.DeviceRef(w, h, frac=frac)
     dref.load(ref_img)
@@ -39,7 +40,7 @@ def gpu_recon(n_arg, qp, cur, ref_img, tus, frac, intermediates=True, want_inter
                                           rec.stride, nnz.at(), ssd.at(), None), "xme_inter_recon_batch")
     X.check(X.lib().xme_stream_sync(None), "sync")
     full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
-    out = (level.download(ntu * n * n, np.int16).reshape(ntu, n, n), full[:h, :w],
+    out = (level.download(ntu * nl * nl, np.int16).reshape(ntu, nl, nl), full[:h, :w],
            nnz.download(ntu, np.int32), ssd.download(ntu, np.int32))
     if want_intermediates:
         out += (resid.download(ntu * n * n, np.int16), rres.download(ntu * n * n, np.int16))
@@ -85,6 +86,7 @@ def test_1080p_8x8():
 
 def gpu_recon_bi(n_arg, qp, cur, r1_img, r2_img, tus, mv2, frac):
     n = n_arg & 0xff
+    nl = min(n, 32)
     h, w = cur.shape
     d1, d2 = X.DeviceRef(w, h, frac=frac), X.DeviceRef(w, h, frac=frac)
     d1.load(r1_img)
@@ -103,7 +105,7 @@ def gpu_recon_bi(n_arg, qp, cur, r1_img, r2_img, tus, mv2, frac):
             "xme_inter_recon_bi_batch")
     X.check(X.lib().xme_stream_sync(None), "sync")
     full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
-    return (level.download(ntu * n * n, np.int16).reshape(ntu, n, n), full[:h, :w],
+    return (level.download(ntu * nl * nl, np.int16).reshape(ntu, nl, nl), full[:h, :w],
             nnz.download(ntu, np.int32), ssd.download(ntu, np.int32))
 
 
@@ -219,6 +221,7 @@ def gpu_recon_pred(n_arg, qp, cur, pred, blocks, pred_stride=None, d_pred=None):
     """xme_recon_pred_batch on one plane; pred is uploaded at pred_stride (any), unless a
     device plane d_pred already holds it."""
     n = n_arg & 0xff
+    nl = min(n, 32)
     h, w = cur.shape
     dcur = device_cur(cur)
     if d_pred is None:
@@ -235,7 +238,7 @@ def gpu_recon_pred(n_arg, qp, cur, pred, blocks, pred_stride=None, d_pred=None):
                                          rec.stride, nnz.at(), ssd.at(), None), "xme_recon_pred_batch")
     X.check(X.lib().xme_stream_sync(None), "sync")
     full = rec.buf.download(rec.rows * rec.stride, np.uint8).reshape(rec.rows, rec.stride)
-    return (level.download(nblk * n * n, np.int16).reshape(nblk, n, n), full[:h, :w],
+    return (level.download(nblk * nl * nl, np.int16).reshape(nblk, nl, nl), full[:h, :w],
             nnz.download(nblk, np.int32), ssd.download(nblk, np.int32))
 
 
@@ -274,16 +277,17 @@ def pred_grid(w, h, n, rng, keep=0.8):
 
 
 @pytest.mark.parametrize("qp", [22, 43])
-@pytest.mark.parametrize("n_arg", [4, 8, 16, 32, 32 | X.XME_TU_DCT_HALF])
+@pytest.mark.parametrize("n_arg", [4, 8, 16, 32, 32 | X.XME_TU_DCT_HALF, 64, 64 | X.XME_TU_DCT_HALF])
 def test_recon_pred_matches_oracle(n_arg, qp):
-    """xme_recon_pred_batch (cu_recon_chroma, rdo.c:807-838) on a prediction picture whose
-    rows start at odd addresses, blocks in random order."""
+    """xme_recon_pred_batch (cu_recon_chroma, rdo.c:807-838; 64x64 luma TUs through the
+    wavelet, transform.c:1514-1541) on a prediction picture with its own stride, blocks in
+    random order."""
     n = n_arg & 0xff
-    w, h = 160, 96
+    w, h = 192, 128
     pred = synth.smooth_field(w, h, seed=81)
     cur = synth.multi_ref_current([pred], 8, 8, seed=82)
-    blocks = pred_grid(w, h, n, np.random.default_rng(83 + n_arg))
-    got = gpu_recon_pred(n_arg, qp, cur, pred, blocks, pred_stride=w + 13)
+    blocks = pred_grid(w, h, n, np.random.default_rng(83 + n_arg), keep=0.8 if n < 64 else 1.0)
+    got = gpu_recon_pred(n_arg, qp, cur, pred, blocks, pred_stride=w + 12)
     exp = O.recon_pred_batch(n_arg, qp, cur, pred, blocks)
     for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
         assert np.array_equal(g, e), name
@@ -372,6 +376,10 @@ def test_recon_pred_arguments_and_empty_batch():
     assert X.lib().xme_recon_pred_batch(8, 30, buf.at(2), 64, p, 64, 64, 16, p, 1, p, p, 64, p, p,
                                         None) == X.XME_ERR_ARG
     assert X.lib().xme_recon_pred_batch(8, 30, p, 66, p, 64, 64, 16, p, 1, p, p, 64, p, p, None) == X.XME_ERR_ARG
+    # the prediction picture is read as aligned dwords too
+    assert X.lib().xme_recon_pred_batch(8, 30, p, 64, buf.at(1), 64, 64, 16, p, 1, p, p, 64, p, p,
+                                        None) == X.XME_ERR_ARG
+    assert X.lib().xme_recon_pred_batch(8, 30, p, 64, p, 65, 64, 16, p, 1, p, p, 64, p, p, None) == X.XME_ERR_ARG
     assert X.lib().xme_recon_pred_batch(8, 30, None, 64, None, 64, 64, 16, None, 0, None, None, 64, None, None,
                                         None) == X.XME_OK
     # the two-plane call: a missing V plane, a misaligned V reconstruction, bad sizes
@@ -403,7 +411,7 @@ def test_recon_chroma_matches_two_single_plane_calls(n_arg):
     for seed in (98, 99):
         pred = synth.smooth_field(w, h, seed=seed)
         cur = synth.multi_ref_current([pred], 8, 8, seed=seed + 10)
-        d = X.DevicePlane(w, h, pad=0, stride=w + 5)
+        d = X.DevicePlane(w, h, pad=0, stride=w + 4)
         d.upload_picture(pred)
         curs.append(cur)
         d_preds.append(d)
@@ -412,3 +420,64 @@ def test_recon_chroma_matches_two_single_plane_calls(n_arg):
     for p in range(2):
         for name, g, e in zip(("level", "rec", "nnz", "ssd"), both[p], exps[p]):
             assert np.array_equal(g, e), (p, name)
+
+
+def tus64(w, h, rng, far=True):
+    """64x64 PUs tiling the picture, each one TU (TU_SPLIT_NON of a 64x64 CU), one MV per PU,
+    some far outside the picture."""
+    out = []
+    for y in range(0, h - 63, 64):
+        for x in range(0, w - 63, 64):
+            mv = (int(rng.integers(-60, 61)), int(rng.integers(-60, 61)))
+            if far and rng.random() < 0.3:
+                mv = (4 * int(rng.integers(-300, 300)) + int(rng.integers(0, 4)),
+                      4 * int(rng.integers(-300, 300)) + int(rng.integers(0, 4)))
+            out.append((x, y, x, y, 64, 64) + mv)
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("qp", [22, 37])
+@pytest.mark.parametrize("half", [0, 1])
+def test_64x64_tus_uni_and_bi(half, qp):
+    """64x64 TUs of 64x64 PUs through xme_inter_recon_batch and _bi_batch (one launch with
+    the planes resident; dct_64x64 and its dct_half at quant level 6)."""
+    w, h = 256, 192
+    r1 = synth.smooth_field(w, h, seed=111)
+    r2 = synth.smooth_field(w, h, seed=112)
+    cur = synth.multi_ref_current([r1, r2], 8, 8, seed=113)
+    rng = np.random.default_rng(114 + qp + half)
+    tus = tus64(w, h, rng)
+    n_arg = 64 | (X.XME_TU_DCT_HALF if half else 0)
+    got = gpu_recon(n_arg, qp, cur, r1, tus, 2, intermediates=False)
+    exp = O.inter_recon_batch(n_arg, qp, oracle_cur(cur), oracle_ref(r1, 2), w, h, tus)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), name
+    assert got[2].any() and (got[3] > 0).all()
+    mv2 = np.array([(int(rng.integers(-40, 41)), int(rng.integers(-40, 41))) for _ in tus], np.int32)
+    got = gpu_recon_bi(n_arg, qp, cur, r1, r2, tus, mv2, 2)
+    exp = O.inter_recon_batch(n_arg, qp, oracle_cur(cur), oracle_ref(r1, 2), w, h, tus, oracle_ref(r2, 2), mv2)
+    for name, g, e in zip(("level", "rec", "nnz", "ssd"), got, exp):
+        assert np.array_equal(g, e), ("bi", name)
+
+
+def test_64x64_tus_limits():
+    """A 64x64 TU needs a 64x64 PU inside the picture and the interpolated planes."""
+    w, h = 192, 128
+    ref = synth.smooth_field(w, h, seed=121)
+    cur = synth.multi_ref_current([ref], 8, 8, seed=122)
+    tus = np.array([(64, 64, 64, 64, 64, 64, 5, -7),      # good
+                    (64, 0, 64, 0, 64, 32, 0, 0),         # PU 64x32: the TU is not inside it
+                    (160, 0, 160, 0, 64, 64, 0, 0),       # past the right edge
+                    (66, 0, 64, 0, 64, 64, 0, 0)],        # column not a multiple of 4 / not inside
+                   np.int32)
+    level, rec, nnz, ssd = gpu_recon(64, 30, cur, ref, tus, 2, intermediates=False)
+    assert ssd[0] > 0 and list(ssd[1:]) == [-1, -1, -1]
+    assert not level[1:].any() and not nnz[1:].any()
+    exp = O.inter_recon_batch(64, 30, oracle_cur(cur), oracle_ref(ref, 2), w, h, tus[:1])
+    assert np.array_equal(rec, exp[1]) and np.array_equal(level[0], exp[0][0])
+    rd_ref = X.DeviceRef(w, h, frac=0)
+    rd = rd_ref.desc()
+    buf = X.DeviceBuffer(1 << 16)
+    p = buf.at()
+    assert X.lib().xme_inter_recon_batch(64, 30, p, 192, C.byref(rd), w, h, p, 1, p, p, p, p, 192, p, p,
+                                         None) == X.XME_ERR_ARG

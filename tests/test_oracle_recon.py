@@ -273,3 +273,63 @@ def test_chroma_qp():
         assert O.chroma_qp(q) == c, q
     seq = [O.chroma_qp(q) for q in range(64)]
     assert all(0 <= b - a <= 1 for a, b in zip(seq, seq[1:]))
+
+
+def np_recon64(qp, half, cur_blk, pred_blk):
+    """A 64x64 TU from pinned pieces: the lifting wavelet (xo_wavelet_fwd / _inv, pinned on
+    hand-worked ramps), numpy's 32x32 transforms with the wavelet flag, and numpy quant /
+    dequant at level 6, whose shifts are level 5's (pinned) moved by one, as
+    tu_quant_forward / tu_quant_inverse (rdo.c:658-718) write them."""
+    from tests.test_oracle_percall import np_dct, np_idct
+    res = cur_blk.astype(np.int16) - pred_blk.astype(np.int16)
+    low = np.zeros(32 * 32, np.int16)
+    O.lib().xo_wavelet_fwd(64, 64, O.ptr(np.ascontiguousarray(res)), O.ptr(low))
+    coef = np_dct(32, 32, low.reshape(32, 32).astype(np.int64), flag=1)
+    if half:
+        coef[16:, :] = 0
+        coef[:, 16:] = 0
+    qs, qsh5, _ = O.quant_params(5, qp, 0)
+    ds, dsh5 = O.dequant_params(5, qp)
+    qsh = qsh5 - 1
+    lv = npref.quant(coef, qs, qsh, ((1 << qsh) * 5) // 31)
+    dq = npref.dequant(lv, ds, dsh5 + 1)
+    rr_low = np_idct(32, 32, dq.astype(np.int64), flag=1).astype(np.int16)
+    rr = np.zeros(64 * 64, np.int16)
+    O.lib().xo_wavelet_inv(64, 64, O.ptr(np.ascontiguousarray(rr_low.ravel())), O.ptr(rr))
+    v = np.clip(pred_blk.astype(np.int32) + rr.reshape(64, 64), 0, 255).astype(np.uint8)
+    return lv, v, int(np.count_nonzero(lv)), int(((cur_blk.astype(np.int64) - v) ** 2).sum())
+
+
+@pytest.mark.parametrize("half", [0, 1])
+@pytest.mark.parametrize("qp", [22, 37, 51])
+def test_recon_64_matches_the_pinned_pieces(qp, half):
+    """64x64 TUs (dct_64x64 / dct_64x64_half and idct_64x64, transform.c:1514-1541, at quant
+    level 6): 32x32 levels, the reconstruction and the SSD."""
+    w, h = 192, 128
+    cur, pred = texture(w, h, 21), texture(w, h, 22)
+    blocks = np.array([(0, 0), (128, 64), (64, 0), (0, 64)], np.int32)
+    n_arg = 64 | (O.TU_DCT_HALF if half else 0)
+    level, rec, nnz, ssd = O.recon_pred_batch(n_arg, qp, cur, pred, blocks)
+    assert level.shape == (4, 32, 32)
+    for i, (x, y) in enumerate(blocks):
+        lv, v, nz, e = np_recon64(qp, half, cur[y:y + 64, x:x + 64], pred[y:y + 64, x:x + 64])
+        assert np.array_equal(level[i], lv) and nnz[i] == nz and ssd[i] == e
+        assert np.array_equal(rec[y:y + 64, x:x + 64], v)
+    assert nnz.any()
+
+
+def test_recon_64_known_answer_and_the_luma_path():
+    """An exact prediction gives zero levels and the picture back; at MV (0, 0) a 64x64 PU's
+    TU through the luma path equals the prediction-picture path (dct_half too)."""
+    w, h = 128, 128
+    cur, ref = texture(w, h, 23), texture(w, h, 24)
+    blocks = np.array([(0, 0), (64, 64)], np.int32)
+    level, rec, nnz, ssd = O.recon_pred_batch(64, 30, cur, cur, blocks)
+    assert not level.any() and not nnz.any() and not ssd.any()
+    assert np.array_equal(rec[:64, :64], cur[:64, :64]) and np.array_equal(rec[64:, 64:], cur[64:, 64:])
+    tus = np.array([(x, y, x, y, 64, 64, 0, 0) for x, y in blocks], np.int32)
+    for n_arg in (64, 64 | O.TU_DCT_HALF):
+        a = O.inter_recon_batch(n_arg, 33, cur_plane(cur), planes_of(ref), w, h, tus)
+        b = O.recon_pred_batch(n_arg, 33, cur, ref, blocks)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)

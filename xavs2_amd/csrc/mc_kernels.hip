@@ -231,9 +231,9 @@ int recon_entry(const char *name, int n_arg, int qp, const pel_t *d_cur, intptr_
 {
     const int n = n_arg & ~XME_TU_DCT_HALF, half = (n_arg & XME_TU_DCT_HALF) ? 1 : 0;
     if (!ref || !ref->planes[0] || (BI && (!ref2 || !ref2->planes[0] || ref2->stride != ref->stride)) || ntu < 0 ||
-        (n != 4 && n != 8 && n != 16 && n != 32) || (half && n != 32) || qp < 0 || qp > 79 || frame_w <= 0 ||
-        frame_h <= 0) {
-        xme_set_error("%s: bad arguments (n 4, 8, 16 or 32; XME_TU_DCT_HALF with 32 only)", name);
+        (n != 4 && n != 8 && n != 16 && n != 32 && n != 64) || (half && n < 32) || qp < 0 || qp > 79 ||
+        frame_w <= 0 || frame_h <= 0) {
+        xme_set_error("%s: bad arguments (n 4, 8, 16, 32 or 64; XME_TU_DCT_HALF with 32 or 64 only)", name);
         return XME_ERR_ARG;
     }
     if (ntu == 0)
@@ -247,6 +247,10 @@ int recon_entry(const char *name, int n_arg, int qp, const pel_t *d_cur, intptr_
     if (!d_cur || !d_tus || (BI && !d_mv2) || !d_level || !d_rec || !d_nnz || !d_ssd ||
         (!planes && (!d_resid || !d_rres))) {
         xme_set_error("%s: NULL buffer", name);
+        return XME_ERR_ARG;
+    }
+    if (n == 64 && !planes) {
+        xme_set_error("%s: 64x64 TUs need every interpolated plane of the reference", name);
         return XME_ERR_ARG;
     }
     // four pixels per access: the current and reconstructed pictures are read

@@ -178,6 +178,19 @@ __device__ __forceinline__ uint32_t load4u(const uint8_t *p)
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
+// Four prediction bytes: from the interpolated planes at any alignment
+// (load4u, inside their border), or from a prediction picture (FU = 3) at a
+// column that is a multiple of 4 of a 4-byte aligned picture, so that no
+// read passes the picture's last byte.
+template <int FU>
+__device__ __forceinline__ uint32_t load_pred(const uint8_t *p)
+{
+    if constexpr (FU == 3)
+        return *reinterpret_cast<const uint32_t *>(p);
+    else
+        return load4u(p);
+}
+
 __device__ __forceinline__ uint64_t bperm64(int src_lane, uint64_t v)
 {
     const int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(uint32_t)v);
@@ -185,26 +198,75 @@ __device__ __forceinline__ uint64_t bperm64(int src_lane, uint64_t v)
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
+// The 5/3 lifting of a 64-sample line held in registers, v[2 + i] = sample
+// i (wavelet_64x64_c / inv_wavelet_64x64_c, transform.c:586-842).  Forward:
+// mirror two samples a side, the odd samples minus the floor mean of their
+// neighbours, then the even ones plus (l + r + 2) >> 2 (rows) or doubled
+// plus (l + r + 1) >> 1 (columns); the low band is v[2 + 2 i].  Every stored
+// sample is truncated to int16, as the reference's coeff_t line keeps it.
+template <bool COLS>
+__device__ __forceinline__ void lift64_fwd(int (&v)[68])
+{
+    v[1] = v[3];
+    v[0] = v[4];
+    v[66] = v[64];
+    v[67] = v[63];
+#pragma unroll
+    for (int i = -1; i < 64; i += 2)
+        v[2 + i] = (int16_t)(v[2 + i] - ((v[1 + i] + v[3 + i]) >> 1));
+#pragma unroll
+    for (int i = 0; i < 64; i += 2)
+        v[2 + i] = COLS ? (int16_t)((v[2 + i] << 1) + ((v[1 + i] + v[3 + i] + 1) >> 1))
+                        : (int16_t)(v[2 + i] + ((v[1 + i] + v[3 + i] + 2) >> 2));
+}
+
+// Inverse: the 32 low samples at the even places e[2 i], the last mirrored
+// (e[64] = e[62]); columns halve the even samples 0..64; the odd samples are
+// the floor mean of their neighbours.
+template <bool COLS>
+__device__ __forceinline__ void lift64_inv(int (&e)[65])
+{
+    e[64] = e[62];
+    if constexpr (COLS) {
+#pragma unroll
+        for (int i = 0; i <= 64; i += 2)
+            e[i] = (int16_t)(e[i] >> 1);
+    }
+#pragma unroll
+    for (int i = 1; i < 64; i += 2)
+        e[i] = (int16_t)((e[i - 1] + e[i + 1]) >> 1);
+}
+
 // FU = 0: the residual path on int16 blocks from src.  FU = 1 / 2: the fused
 // inter reconstruction (FusedArgs), uni- or bi-predicted; src is unused and
-// recon (the reconstructed residual) may be null.
-template <int N, int FU>
+// recon (the reconstructed residual) may be null.  WL (N = 32, FU != 0):
+// 64x64 blocks, one per tile (dct_64x64 / idct_64x64, transform.c:1514-1541):
+// the prediction error's wavelet low band takes the 32x32 transform with the
+// wavelet flag (first forward shift + 1, last inverse shift - 1 and a 10-bit
+// clip), quantised at level 6, and the inverse wavelet gives the 64x64
+// residual; the prediction error and the reconstructed residual are not
+// written (resid, recon unused).
+template <int N, int FU, bool WL = false>
 __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict__ src, int16_t *__restrict__ level,
                                                        int16_t *__restrict__ recon, int32_t *__restrict__ nnz,
                                                        int nblk, QParams q, int mode, FusedArgs fa)
 {
     constexpr int NB = 32 / N, BPT = NB * NB, L2 = log2n_of(N);
     constexpr int W = N < 16 ? N : 16, NP = 16 / W;    // a lane's 16 columns: NP pieces of W
+    static_assert(!WL || (N == 32 && FU != 0), "the 64x64 path is a fused 32x32 tile");
+    constexpr int NS = WL ? 64 : N;    // the block's side in the picture
     __shared__ int16_t s_res[4][FU ? 32 * 32 : 1];
     __shared__ int s_ssd[4][FU ? BPT : 1];
+    __shared__ int16_t s_lw[4][WL ? 64 * 32 : 1];   // WL: the half-transformed 64 x 32 block
     const int wv = threadIdx.x >> 6;
     const TileConsts &tc = g_tc[L2 - 2];
     const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
     const int ntiles = (nblk + BPT - 1) / BPT;
     const v4i f1b = load4(tc.f1b[lane]), f2a = load4(tc.f2a[lane]), ib = load4(tc.ib[lane]);
-    const int s1 = L2 - 2, s2 = L2 + 5;
+    const int s1 = L2 - 2 + (WL ? 1 : 0), s2 = L2 + 5;
     const int acc_f1 = tc.rowsum[lane] + ((1 << s1) >> 1);
-    const int acc_i1 = tc.colsum[lane] + 16, acc_i2 = tc.colsum[lane] + 2048;
+    constexpr int SI2 = WL ? 11 : 12, RLIM = WL ? 512 : 256;    // last inverse shift, clip of the residual
+    const int acc_i1 = tc.colsum[lane] + 16, acc_i2 = tc.colsum[lane] + (1 << (SI2 - 1));
     v16i acc_f2;
 #pragma unroll
     for (int i = 0; i < 16; i++)
@@ -225,6 +287,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
         uint32_t c4[4] = { 0u, 0u, 0u, 0u }, p4[4] = { 0u, 0u, 0u, 0u };
         bool tu_good = false;            // lane < BPT: its TU of the tile is reconstructed
         uint64_t my_rec = 0;             // lane < BPT: the TU's first reconstructed pixel
+        uint64_t wl_cur = 0, wl_pred = 0, wl_pred2 = 0, wl_rec = 0;   // WL: the block's, in every lane
         if constexpr (FU != 0) {
             // lane b < BPT takes TU t * BPT + b: validity and base pointers
             uint64_t my_pred = 0, my_pred2 = 0, my_cur = 0;
@@ -235,7 +298,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                     const bool second = bt >= fa.nb1;
                     const int64_t bi = second ? bt - fa.nb1 : bt;
                     const int bx = fa.tus[2 * bi], by = fa.tus[2 * bi + 1];
-                    tu_good = (bx & 3) == 0 && bx >= 0 && by >= 0 && bx <= fa.fw - N && by <= fa.fh - N;
+                    tu_good = (bx & 3) == 0 && bx >= 0 && by >= 0 && bx <= fa.fw - NS && by <= fa.fh - NS;
                     if (tu_good) {
                         my_pred = (uint64_t)(uintptr_t)((second ? fa.planes[1] : fa.planes[0]) +
                                                         (int64_t)by * fa.i_ref + bx);
@@ -248,7 +311,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
 #pragma unroll
                 for (int k = 0; k < 8; k++)
                     r[k] = fa.tus[8 * bt + k];
-                tu_good = tu_ok<N>(r, fa.fw, fa.fh);
+                tu_good = tu_ok<NS>(r, fa.fw, fa.fh);
                 if (tu_good) {
                     const int qx = mc_pos(fa.fw, r[4], r[2], r[6]) + 4 * (r[0] - r[2]);
                     const int qy = mc_pos(fa.fh, r[5], r[3], r[7]) + 4 * (r[1] - r[3]);
@@ -266,9 +329,15 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             }
             if (lane < BPT)
                 s_ssd[wv][lane] = 0;
+            if constexpr (WL) {
+                wl_cur = bperm64(0, my_cur);
+                wl_pred = bperm64(0, my_pred);
+                wl_pred2 = FU == 2 ? bperm64(0, my_pred2) : 0;
+                wl_rec = bperm64(0, my_rec);
+            }
             const int r_in = col % N, c0 = (16 * half) % N;
 #pragma unroll
-            for (int piece = 0; piece < NP; piece++) {
+            for (int piece = 0; piece < (WL ? 0 : NP); piece++) {
                 const int bl = (col / N) * NB + (16 * half) / N + piece;
                 const uint64_t pc = bperm64(bl, my_cur), pp = bperm64(bl, my_pred);
                 const uint64_t pp2 = FU == 2 ? bperm64(bl, my_pred2) : 0;
@@ -279,7 +348,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
 #pragma unroll
                     for (int k = 0; k < W / 4; k++) {
                         c4[piece * (W / 4) + k] = reinterpret_cast<const uint32_t *>(cp)[k];
-                        uint32_t u = load4u(pr + 4 * k);
+                        uint32_t u = load_pred<FU>(pr + 4 * k);
                         if constexpr (FU == 2) {
                             const uint32_t v2 = load4u(pr2 + 4 * k);
                             u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);   // (u + v + 1) >> 1 per byte
@@ -292,7 +361,57 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
         if (mode & DO_FWD) {
             // F1 operand: tile row `col`, columns 16 half .. 16 half + 15
             uint32_t d[8];
-            if constexpr (FU != 0) {
+            if constexpr (WL) {
+                // lane r: row r of the 64x64 prediction error, its row
+                // lifting into s_lw (64 rows of 32), then lanes 0..31 lift
+                // the columns in place: rows 0..31 hold the 32x32 low band
+                int16_t *lw = s_lw[wv];
+                {
+                    int v[68];
+                    const uint8_t *cp = (const uint8_t *)(uintptr_t)wl_cur + (int64_t)lane * fa.i_cur;
+                    const uint8_t *pr = (const uint8_t *)(uintptr_t)wl_pred + (int64_t)lane * fa.i_ref;
+                    const uint8_t *pr2 = (const uint8_t *)(uintptr_t)wl_pred2 + (int64_t)lane * fa.i_ref;
+#pragma unroll
+                    for (int k = 0; k < 16; k++) {
+                        uint32_t cw = 0, u = 0;
+                        if (wl_cur != 0) {
+                            cw = reinterpret_cast<const uint32_t *>(cp)[k];
+                            u = load_pred<FU>(pr + 4 * k);
+                            if constexpr (FU == 2) {
+                                const uint32_t v2 = load4u(pr2 + 4 * k);
+                                u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
+                            }
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            v[2 + 4 * k + j] = (int)((cw >> (8 * j)) & 0xff) - (int)((u >> (8 * j)) & 0xff);
+                    }
+                    lift64_fwd<false>(v);
+#pragma unroll
+                    for (int i = 0; i < 32; i++)
+                        lw[lane * 32 + i] = (int16_t)v[2 + 2 * i];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane < 32) {
+                    int v[68];
+#pragma unroll
+                    for (int i = 0; i < 64; i++)
+                        v[2 + i] = lw[i * 32 + lane];
+                    lift64_fwd<true>(v);
+#pragma unroll
+                    for (int i = 0; i < 32; i++)
+                        lw[i * 32 + lane] = (int16_t)v[2 + 2 * i];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t *row = reinterpret_cast<const uint32_t *>(lw + col * 32 + 16 * half);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    d[k] = row[k];
+            } else if constexpr (FU != 0) {
                 // the prediction error cur - pred (sub_ps, pixel.c:780-792)
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
@@ -393,14 +512,72 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             D = mm_data_a(hi, lo, ib, splat16(acc_i2));
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const int rv = xme_clip3(-256, 255, D[i] >> 12);
-                if (recon && b_lane + (rho(0, i) / N) * NB < nblk)
+                const int rv = xme_clip3(-RLIM, RLIM - 1, D[i] >> SI2);
+                if (!WL && recon && b_lane + (rho(0, i) / N) * NB < nblk)
                     recon[e_lane + rel_y<N>(rho(0, i))] = (int16_t)rv;
                 if constexpr (FU != 0)
                     s_res[wv][rho(half, i) * 32 + col] = (int16_t)rv;
             }
         }
-        if constexpr (FU != 0) {
+        if constexpr (WL) {
+            // the inverse wavelet of the 32x32 residual in s_res: lanes 0..31
+            // a column each into s_lw (64 rows of 32), then lane r row r:
+            // clip(pred + residual) into the picture and the block's SSD
+            int16_t *lw = s_lw[wv];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < 32) {
+                int e[65];
+#pragma unroll
+                for (int i = 0; i < 32; i++)
+                    e[2 * i] = s_res[wv][i * 32 + lane];
+                lift64_inv<true>(e);
+#pragma unroll
+                for (int i = 0; i < 64; i++)
+                    lw[i * 32 + lane] = (int16_t)e[i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            int e2 = 0;
+            if (wl_rec != 0) {
+                int e[65];
+#pragma unroll
+                for (int i = 0; i < 32; i++)
+                    e[2 * i] = lw[lane * 32 + i];
+                lift64_inv<false>(e);
+                const uint8_t *cp = (const uint8_t *)(uintptr_t)wl_cur + (int64_t)lane * fa.i_cur;
+                const uint8_t *pr = (const uint8_t *)(uintptr_t)wl_pred + (int64_t)lane * fa.i_ref;
+                const uint8_t *pr2 = (const uint8_t *)(uintptr_t)wl_pred2 + (int64_t)lane * fa.i_ref;
+                uint8_t *rp = (uint8_t *)(uintptr_t)wl_rec + (int64_t)lane * fa.i_rec;
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    const uint32_t cw = reinterpret_cast<const uint32_t *>(cp)[k];
+                    uint32_t u = load_pred<FU>(pr + 4 * k);
+                    if constexpr (FU == 2) {
+                        const uint32_t v2 = load4u(pr2 + 4 * k);
+                        u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
+                    }
+                    uint32_t o = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int v = clip_pel((int)((u >> (8 * j)) & 0xff) + e[4 * k + j]);
+                        const int dc = (int)((cw >> (8 * j)) & 0xff) - v;
+                        o |= (uint32_t)v << (8 * j);
+                        e2 += dc * dc;
+                    }
+                    reinterpret_cast<uint32_t *>(rp)[k] = o;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                e2 += __shfl_xor(e2, off, 64);
+            if (lane == 0 && t < nblk)
+                fa.ssd[t] = tu_good ? e2 : -1;
+            // the next tile rewrites s_lw and s_res only after these reads
+            __builtin_amdgcn_wave_barrier();
+        } else if constexpr (FU != 0) {
             // the residual, now by rows: this lane's 16 columns of row `col`;
             // reconstruction clip(pred + residual) (add_ps, pixel.c:794-806)
             // and its SSD (pixf.ssd, rdo.c:1497-1499) per TU
@@ -508,7 +685,7 @@ const uint8_t kIQS[80] = { 15, 15, 15, 15, 15, 15, 15, 15, 14, 14, 14, 14, 14, 1
                            11, 10, 10, 10, 10, 10, 10, 10, 10, 9, 9, 9, 9, 9, 9, 9, 8, 8, 8, 8,
                            8, 8, 8, 8, 7, 7, 7, 7, 7, 7, 7, 7, 6, 6, 6, 6, 6, 6, 6, 6 };
 
-int log2n(int n) { return n == 4 ? 2 : n == 8 ? 3 : n == 16 ? 4 : 5; }
+int log2n(int n) { return n == 4 ? 2 : n == 8 ? 3 : n == 16 ? 4 : n == 32 ? 5 : 6; }
 
 // tu_quant_forward / tu_quant_inverse (rdo.c:658-718) for 8-bit video.
 QParams qparams(int n, int qp, int b_intra)
@@ -530,8 +707,9 @@ int launch_fu(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t
     std::call_once(g_tab_once, upload_tables);
     if (g_tab_err != hipSuccess)
         return xme_hip_fail(g_tab_err, "upload transform tables");
-    // one wave per 32x32 tile of (32/n)^2 blocks; waves loop over tiles
-    const int bpt = (32 / n) * (32 / n);
+    // one wave per 32x32 tile of (32/n)^2 blocks (a 64x64 block's tile is its
+    // wavelet low band); waves loop over tiles
+    const int bpt = n >= 32 ? 1 : (32 / n) * (32 / n);
     const int64_t tiles = ((int64_t)nblk + bpt - 1) / bpt;
     // a multiple of 8 workgroups, one range of tiles per XCD
     const int grid = (int)((std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 256 * 16)) + 7) / 8 * 8);
@@ -548,9 +726,14 @@ int launch_fu(int n, const int16_t *src, int16_t *level, int16_t *recon, int32_t
         hipLaunchKernelGGL((residual_kernel<16, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q,
                            mode, fa);
         break;
-    default:
+    case 32:
         hipLaunchKernelGGL((residual_kernel<32, FU>), dim3(grid), dim3(256), 0, s, src, level, recon, nnz, nblk, q,
                            mode, fa);
+        break;
+    default:
+        if constexpr (FU != 0)
+            hipLaunchKernelGGL((residual_kernel<32, FU, true>), dim3(grid), dim3(256), 0, s, src, level, nullptr, nnz,
+                               nblk, q, mode, fa);
         break;
     }
     XME_HIP(hipGetLastError());
@@ -626,9 +809,9 @@ int recon_pred(const char *who, int n_arg, int qp, const pel_t *cur_u, const pel
                int32_t *nnz, int32_t *ssd, hipStream_t s)
 {
     const int n = n_arg & ~XME_TU_DCT_HALF, half = (n_arg & XME_TU_DCT_HALF) ? 1 : 0;
-    if ((n != 4 && n != 8 && n != 16 && n != 32) || (half && n != 32) || qp < 0 || qp > 79 || nblk < 0 ||
+    if ((n != 4 && n != 8 && n != 16 && n != 32 && n != 64) || (half && n < 32) || qp < 0 || qp > 79 || nblk < 0 ||
         plane_w <= 0 || plane_h <= 0) {
-        xme_set_error("%s: bad arguments (n 4, 8, 16 or 32; XME_TU_DCT_HALF with 32 only; qp 0..79)", who);
+        xme_set_error("%s: bad arguments (n 4, 8, 16, 32 or 64; XME_TU_DCT_HALF with 32 or 64 only; qp 0..79)", who);
         return XME_ERR_ARG;
     }
     if (nblk == 0)
@@ -638,10 +821,10 @@ int recon_pred(const char *who, int n_arg, int qp, const pel_t *cur_u, const pel
         xme_set_error("%s: NULL buffer", who);
         return XME_ERR_ARG;
     }
-    // the current and reconstructed pictures are read and written as aligned
-    // dwords; the prediction is read at any alignment
-    const uintptr_t al = (uintptr_t)cur_u | (uintptr_t)rec_u | (two ? ((uintptr_t)cur_v | (uintptr_t)rec_v) : 0);
-    if ((al & 3) || (i_cur & 3) || (i_rec & 3) || ((uintptr_t)level & 15)) {
+    // every picture is read or written as aligned dwords
+    const uintptr_t al = (uintptr_t)cur_u | (uintptr_t)rec_u | (uintptr_t)pred_u |
+                         (two ? ((uintptr_t)cur_v | (uintptr_t)rec_v | (uintptr_t)pred_v) : 0);
+    if ((al & 3) || (i_cur & 3) || (i_rec & 3) || (i_pred & 3) || ((uintptr_t)level & 15)) {
         xme_set_error("%s: pictures must be 4-byte and the levels 16-byte aligned", who);
         return XME_ERR_ARG;
     }
