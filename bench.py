@@ -698,6 +698,8 @@ def chroma_recon_leg(X, w, h, pus, gpu_res, n_ref, refs, cur, reps, cpu, luma_qp
     log(f"chroma: MC {mc_ms:.3f} ms, recon {recon_ms:.3f} ms per frame")
     return leg
 
+
+def transform_leg(X, blocks_total, reps, seed=91):
     """DCT -> flat quant -> dequant -> IDCT over blocks_total int16 blocks, a
     quarter each of 4x4, 8x8, 16x16 and 32x32 (Laplacian b=6 residuals clipped
     to +-255, 30 % all-zero blocks), at QP 22, 27, 32, 37, 42.  Algorithmic HBM
