@@ -4,19 +4,122 @@
 // 4-tap intpl_chroma_block_* filters (mc.c:275-337).  Block positions follow
 // get_mv_for_mc / cu_get_mc_pos (source/common/mc.h:52-75).
 //
-// One workgroup per block; a thread computes one output pixel (the FIR
-// helpers are in xme_mc.h).
+// mc_luma: one workgroup per block, a thread per output pixel; mc_chroma: a
+// wave per block (the FIR helpers are in xme_mc.h).
 #include "xme_dpp.h"
 #include "xme_mc.h"
 
 namespace {
 
+// One chroma pixel of mc_chroma at s: the 4-tap filters of the block's phase
+// (intpl_chroma_block_hor / _ver / _ext, mc.c:275-337; the 8-bit first pass
+// of _ext unshifted).
+__device__ __forceinline__ int chroma_px(const uint8_t *s, int64_t i_src, int px, int py, int th, int tv)
+{
+    if (px == 0 && py == 0)
+        return s[0];
+    if (py == 0)
+        return clip_pel((fir4(s, 1, th) + 32) >> 6);
+    if (px == 0)
+        return clip_pel((fir4(s, i_src, tv) + 32) >> 6);
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        acc += (int)c_chroma[py][k] * fir4(s + (int64_t)(k - 1) * i_src, 1, th);
+    return clip_pel((acc + 2048) >> 12);
+}
+
+// Bytes p[0..3] and p[4..7] at any alignment, from the three aligned dwords
+// around them (inside the plane's border).
+__device__ __forceinline__ void load8u(const uint8_t *p, uint32_t &lo, uint32_t &hi)
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
+    const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], sh = (uint32_t)(a & 3);
+    lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+    hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+}
+
+// The horizontal 4-tap sums (taps th, unshifted) of pixels s[0..3] of a row:
+// output j takes s[j - 1 .. j + 2], bytes j..j + 3 of s[-1 .. 6].
+__device__ __forceinline__ void hsum4(const uint8_t *s, int th, int (&o)[4])
+{
+    uint32_t lo, hi;
+    load8u(s - 1, lo, hi);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        o[j] = __builtin_amdgcn_sdot4(th, (int)(__builtin_amdgcn_alignbyte(hi, lo, (uint32_t)j) ^ 0x80808080u), 8192,
+                                      false);
+}
+
+// Four pixels (each 0..255) as the bytes of a dword, by v_perm: left to
+// shifts and ORs, the compiler packs pairs with v_ashr_pk_u8_i32, whose
+// upper half is not zero, and the OR of the other pair then corrupts it.
+__device__ __forceinline__ uint32_t pack4_pels(const int (&v)[4])
+{
+    return __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)v[3], (uint32_t)v[2], 0x0C0C0400u),
+                                 __builtin_amdgcn_perm((uint32_t)v[1], (uint32_t)v[0], 0x0C0C0400u), 0x05040100u);
+}
+
+// Pixels s[0..3] of mc_chroma packed in a dword (chroma_px of each).
+__device__ __forceinline__ uint32_t chroma_px4(const uint8_t *s, int64_t i_src, int px, int py, int th, int tv)
+{
+    int o[4];
+    if (px == 0 && py == 0) {
+        uint32_t lo, hi;
+        load8u(s, lo, hi);
+        return lo;
+    }
+    if (py == 0) {
+        int h[4];
+        hsum4(s, th, h);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            o[j] = clip_pel((h[j] + 32) >> 6);
+        return pack4_pels(o);
+    }
+    if (px == 0) {
+        uint32_t r[4], hi;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            load8u(s + (int64_t)(k - 1) * i_src, r[k], hi);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t col = ((r[0] >> (8 * j)) & 0xffu) | (((r[1] >> (8 * j)) & 0xffu) << 8) |
+                                 (((r[2] >> (8 * j)) & 0xffu) << 16) | (((r[3] >> (8 * j)) & 0xffu) << 24);
+            const int v = __builtin_amdgcn_sdot4(tv, (int)(col ^ 0x80808080u), 8192, false);
+            o[j] = clip_pel((v + 32) >> 6);
+        }
+        return pack4_pels(o);
+    }
+    int acc[4] = { 0, 0, 0, 0 };
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int h[4];
+        hsum4(s + (int64_t)(k - 1) * i_src, th, h);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[j] += (int)c_chroma[py][k] * h[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        o[j] = clip_pel((acc[j] + 2048) >> 12);
+    return pack4_pels(o);
+}
+
+// One wave per block, four blocks per workgroup.  Workgroup b runs on XCD
+// b % 8 and takes blocks of one contiguous range per XCD (the grid is a
+// multiple of 8), so that the rows of neighbouring blocks, which share
+// destination lines, are written through one L2.  A lane writes four pixels
+// of a row as one dword where the block's width and destination allow it.
 __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restrict__ pu_, const uint8_t *__restrict__ pv_,
                                                         int64_t i_src, const int32_t *__restrict__ blocks, int n,
                                                         int fw, int fh, uint8_t *__restrict__ du, uint8_t *__restrict__ dv,
                                                         int64_t i_dst, const int64_t *__restrict__ dst_off)
 {
-    const int b = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int lb = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    const int b = lb * 4 + (int)(threadIdx.x >> 6);
     if (b >= n)
         return;
     const int32_t *bl = blocks + 6 * b;
@@ -25,27 +128,24 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
     const int qx = mc_pos(fw, lw, lx, bl[4]), qy = mc_pos(fh, lh, ly, bl[5]);
     const int w = lw >> 1, h = lh >> 1;
     const int px = qx & 7, py = qy & 7;
-    const int th = pack_taps(c_chroma[px]);
+    const int th = pack_taps(c_chroma[px]), tv = pack_taps(c_chroma[py]);
     const int64_t off = (int64_t)(qy >> 3) * i_src + (qx >> 3);
-    for (int t = threadIdx.x; t < 2 * w * h; t += blockDim.x) {
-        const int c = t / (w * h), e = t % (w * h), y = e / w, x = e % w;
-        const uint8_t *s = (c ? pv_ : pu_) + off + (int64_t)y * i_src + x;
-        int v;
-        if (px == 0 && py == 0) {
-            v = s[0];
-        } else if (py == 0) {
-            v = clip_pel((fir4(s, 1, th) + 32) >> 6);
-        } else if (px == 0) {
-            v = clip_pel((fir4(s, i_src, pack_taps(c_chroma[py])) + 32) >> 6);
-        } else {
-            // intpl_chroma_block_ext_c (mc.c:313-337): 8-bit first pass unshifted
-            int acc = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                acc += (int)c_chroma[py][k] * fir4(s + (int64_t)(k - 1) * i_src, 1, th);
-            v = clip_pel((acc + 2048) >> 12);
+    uint8_t *ou = du + dst_off[b], *ov = dv + dst_off[b];
+    const bool wide = (w & 3) == 0 && ((((uintptr_t)ou | (uintptr_t)ov) & 3) | (uintptr_t)(i_dst & 3)) == 0;
+    if (wide) {
+        const int q = w >> 2, per = h * q;
+        for (int t = lane; t < 2 * per; t += 64) {
+            const int c = t >= per, e = c ? t - per : t, y = e / q, x = 4 * (e - y * q);
+            const uint8_t *s = (c ? pv_ : pu_) + off + (int64_t)y * i_src + x;
+            *reinterpret_cast<uint32_t *>((c ? ov : ou) + (int64_t)y * i_dst + x) =
+                chroma_px4(s, i_src, px, py, th, tv);
         }
-        (c ? dv : du)[dst_off[b] + (int64_t)y * i_dst + x] = (uint8_t)v;
+    } else {
+        for (int t = lane; t < 2 * w * h; t += 64) {
+            const int c = t / (w * h), e = t % (w * h), y = e / w, x = e % w;
+            const uint8_t *s = (c ? pv_ : pu_) + off + (int64_t)y * i_src + x;
+            (c ? ov : ou)[(int64_t)y * i_dst + x] = (uint8_t)chroma_px(s, i_src, px, py, th, tv);
+        }
     }
 }
 
@@ -197,7 +297,8 @@ extern "C" int xme_mc_chroma_batch(const pel_t *d_u, const pel_t *d_v, intptr_t 
     }
     if (n == 0)
         return XME_OK;
-    hipLaunchKernelGGL(mc_chroma_kernel, dim3(n), dim3(128), 0, (hipStream_t)stream, d_u, d_v, (int64_t)i_src,
+    const int grid = ((n + 3) / 4 + 7) / 8 * 8;
+    hipLaunchKernelGGL(mc_chroma_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, d_u, d_v, (int64_t)i_src,
                        d_blocks, n, frame_w_luma, frame_h_luma, d_dst_u, d_dst_v, (int64_t)i_dst, d_dst_off);
     XME_HIP(hipGetLastError());
     return XME_OK;

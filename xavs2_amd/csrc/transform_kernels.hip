@@ -257,7 +257,10 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
     constexpr int NS = WL ? 64 : N;    // the block's side in the picture
     __shared__ int16_t s_res[4][FU ? 32 * 32 : 1];
     __shared__ int s_ssd[4][FU ? BPT : 1];
-    __shared__ int16_t s_lw[4][WL ? 64 * 32 : 1];   // WL: the half-transformed 64 x 32 block
+    // WL: the 64x64 block in LDS, rows of LWS int16 (a dword-aligned stride
+    // of 33 dwords, so that a lane per row meets no bank twice)
+    constexpr int LWS = 66;
+    __shared__ int16_t s_lw[4][WL ? 64 * LWS : 1];
     const int wv = threadIdx.x >> 6;
     const TileConsts &tc = g_tc[L2 - 2];
     const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
@@ -362,34 +365,50 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             // F1 operand: tile row `col`, columns 16 half .. 16 half + 15
             uint32_t d[8];
             if constexpr (WL) {
-                // lane r: row r of the 64x64 prediction error, its row
-                // lifting into s_lw (64 rows of 32), then lanes 0..31 lift
-                // the columns in place: rows 0..31 hold the 32x32 low band
+                // the 64x64 prediction error into s_lw, 16 lanes a row (4
+                // rows per access); then lane r lifts row r into its first
+                // 32 places, and lanes 0..31 lift the columns in place: rows
+                // 0..31 then hold the 32x32 low band
                 int16_t *lw = s_lw[wv];
+                const int qr = lane >> 4, qc = 4 * (lane & 15);
+#pragma unroll 4
+                for (int g = 0; g < 64; g += 4) {
+                    uint32_t cw = 0, u = 0;
+                    if (wl_cur != 0) {
+                        cw = *reinterpret_cast<const uint32_t *>((const uint8_t *)(uintptr_t)wl_cur +
+                                                                 (int64_t)(g + qr) * fa.i_cur + qc);
+                        u = load_pred<FU>((const uint8_t *)(uintptr_t)wl_pred + (int64_t)(g + qr) * fa.i_ref + qc);
+                        if constexpr (FU == 2) {
+                            const uint32_t v2 = load_pred<FU>((const uint8_t *)(uintptr_t)wl_pred2 +
+                                                              (int64_t)(g + qr) * fa.i_ref + qc);
+                            u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
+                        }
+                    }
+                    int e4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        e4[j] = (int)((cw >> (8 * j)) & 0xff) - (int)((u >> (8 * j)) & 0xff);
+                    uint32_t *dst = reinterpret_cast<uint32_t *>(lw + (g + qr) * LWS + qc);
+                    dst[0] = ((uint32_t)e4[0] & 0xffffu) | ((uint32_t)e4[1] << 16);
+                    dst[1] = ((uint32_t)e4[2] & 0xffffu) | ((uint32_t)e4[3] << 16);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 {
                     int v[68];
-                    const uint8_t *cp = (const uint8_t *)(uintptr_t)wl_cur + (int64_t)lane * fa.i_cur;
-                    const uint8_t *pr = (const uint8_t *)(uintptr_t)wl_pred + (int64_t)lane * fa.i_ref;
-                    const uint8_t *pr2 = (const uint8_t *)(uintptr_t)wl_pred2 + (int64_t)lane * fa.i_ref;
+                    const uint32_t *src = reinterpret_cast<const uint32_t *>(lw + lane * LWS);
 #pragma unroll
-                    for (int k = 0; k < 16; k++) {
-                        uint32_t cw = 0, u = 0;
-                        if (wl_cur != 0) {
-                            cw = reinterpret_cast<const uint32_t *>(cp)[k];
-                            u = load_pred<FU>(pr + 4 * k);
-                            if constexpr (FU == 2) {
-                                const uint32_t v2 = load4u(pr2 + 4 * k);
-                                u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
-                            }
-                        }
-#pragma unroll
-                        for (int j = 0; j < 4; j++)
-                            v[2 + 4 * k + j] = (int)((cw >> (8 * j)) & 0xff) - (int)((u >> (8 * j)) & 0xff);
+                    for (int k = 0; k < 32; k++) {
+                        const uint32_t w2 = src[k];
+                        v[2 + 2 * k] = (int)(int16_t)(w2 & 0xffffu);
+                        v[3 + 2 * k] = (int)(int16_t)(w2 >> 16);
                     }
                     lift64_fwd<false>(v);
+                    uint32_t *dst = reinterpret_cast<uint32_t *>(lw + lane * LWS);
 #pragma unroll
-                    for (int i = 0; i < 32; i++)
-                        lw[lane * 32 + i] = (int16_t)v[2 + 2 * i];
+                    for (int k = 0; k < 16; k++)
+                        dst[k] = ((uint32_t)v[2 + 4 * k] & 0xffffu) | ((uint32_t)v[4 + 4 * k] << 16);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -398,16 +417,16 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                     int v[68];
 #pragma unroll
                     for (int i = 0; i < 64; i++)
-                        v[2 + i] = lw[i * 32 + lane];
+                        v[2 + i] = lw[i * LWS + lane];
                     lift64_fwd<true>(v);
 #pragma unroll
                     for (int i = 0; i < 32; i++)
-                        lw[i * 32 + lane] = (int16_t)v[2 + 2 * i];
+                        lw[i * LWS + lane] = (int16_t)v[2 + 2 * i];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const uint32_t *row = reinterpret_cast<const uint32_t *>(lw + col * 32 + 16 * half);
+                const uint32_t *row = reinterpret_cast<const uint32_t *>(lw + col * LWS + 16 * half);
 #pragma unroll
                 for (int k = 0; k < 8; k++)
                     d[k] = row[k];
@@ -521,8 +540,9 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
         }
         if constexpr (WL) {
             // the inverse wavelet of the 32x32 residual in s_res: lanes 0..31
-            // a column each into s_lw (64 rows of 32), then lane r row r:
-            // clip(pred + residual) into the picture and the block's SSD
+            // a column each into s_lw (64 rows of 32), lane r the row r (64
+            // values back into its row); then 16 lanes a row add the
+            // prediction, clip, store and sum the block's SSD
             int16_t *lw = s_lw[wv];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -535,39 +555,56 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                 lift64_inv<true>(e);
 #pragma unroll
                 for (int i = 0; i < 64; i++)
-                    lw[i * 32 + lane] = (int16_t)e[i];
+                    lw[i * LWS + lane] = (int16_t)e[i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                int e[65];
+                const uint32_t *src = reinterpret_cast<const uint32_t *>(lw + lane * LWS);
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    const uint32_t w2 = src[k];
+                    e[4 * k] = (int)(int16_t)(w2 & 0xffffu);
+                    e[4 * k + 2] = (int)(int16_t)(w2 >> 16);
+                }
+                lift64_inv<false>(e);
+                uint32_t *dst = reinterpret_cast<uint32_t *>(lw + lane * LWS);
+#pragma unroll
+                for (int k = 0; k < 32; k++)
+                    dst[k] = ((uint32_t)e[2 * k] & 0xffffu) | ((uint32_t)e[2 * k + 1] << 16);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             int e2 = 0;
             if (wl_rec != 0) {
-                int e[65];
-#pragma unroll
-                for (int i = 0; i < 32; i++)
-                    e[2 * i] = lw[lane * 32 + i];
-                lift64_inv<false>(e);
-                const uint8_t *cp = (const uint8_t *)(uintptr_t)wl_cur + (int64_t)lane * fa.i_cur;
-                const uint8_t *pr = (const uint8_t *)(uintptr_t)wl_pred + (int64_t)lane * fa.i_ref;
-                const uint8_t *pr2 = (const uint8_t *)(uintptr_t)wl_pred2 + (int64_t)lane * fa.i_ref;
-                uint8_t *rp = (uint8_t *)(uintptr_t)wl_rec + (int64_t)lane * fa.i_rec;
-#pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    const uint32_t cw = reinterpret_cast<const uint32_t *>(cp)[k];
-                    uint32_t u = load_pred<FU>(pr + 4 * k);
+                const int qr = lane >> 4, qc = 4 * (lane & 15);
+#pragma unroll 4
+                for (int g = 0; g < 64; g += 4) {
+                    const int r = g + qr;
+                    const uint32_t cw = *reinterpret_cast<const uint32_t *>((const uint8_t *)(uintptr_t)wl_cur +
+                                                                            (int64_t)r * fa.i_cur + qc);
+                    uint32_t u = load_pred<FU>((const uint8_t *)(uintptr_t)wl_pred + (int64_t)r * fa.i_ref + qc);
                     if constexpr (FU == 2) {
-                        const uint32_t v2 = load4u(pr2 + 4 * k);
+                        const uint32_t v2 = load_pred<FU>((const uint8_t *)(uintptr_t)wl_pred2 + (int64_t)r * fa.i_ref +
+                                                          qc);
                         u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
                     }
+                    const uint32_t *res = reinterpret_cast<const uint32_t *>(lw + r * LWS + qc);
+                    const uint32_t r01 = res[0], r23 = res[1];
+                    const int rv[4] = { (int)(int16_t)(r01 & 0xffffu), (int)(int16_t)(r01 >> 16),
+                                        (int)(int16_t)(r23 & 0xffffu), (int)(int16_t)(r23 >> 16) };
                     uint32_t o = 0;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        const int v = clip_pel((int)((u >> (8 * j)) & 0xff) + e[4 * k + j]);
+                        const int v = clip_pel((int)((u >> (8 * j)) & 0xff) + rv[j]);
                         const int dc = (int)((cw >> (8 * j)) & 0xff) - v;
                         o |= (uint32_t)v << (8 * j);
                         e2 += dc * dc;
                     }
-                    reinterpret_cast<uint32_t *>(rp)[k] = o;
+                    *reinterpret_cast<uint32_t *>((uint8_t *)(uintptr_t)wl_rec + (int64_t)r * fa.i_rec + qc) = o;
                 }
             }
 #pragma unroll
