@@ -546,7 +546,38 @@ def recon_leg(X, w, h, pus, gpu_res, n_ref, dcur, d_refs, refs, cur, reps, cpu, 
         leg["cpu_oracle"] = {"pixels_per_s": sum(len(np.array(by_n[n])[np.array(sample[n])]) * n * n
                                                  for n in by_n if by_n[n]) / el,
                              "threads": 1, "seconds": el, "sample": f"the {n_tu} TUs of every 4th PU"}
-    log(f"recon: {ms:.3f} ms per frame")
+    # the luma MC of every PU into a prediction picture (xme_mc_luma_batch),
+    # the input of xme_recon_pred_batch for CUs of several PUs
+    mvs = [(4 * int(r0["bmv2_x"][i]), 4 * int(r0["bmv2_y"][i])) for i in range(len(pus))]
+    pblocks = np.array([(x, y, pw, ph) + mvs[i] for i, (x, y, pw, ph) in enumerate(pus)], np.int32)
+    pred = X.DevicePlane(w, h, pad=0, stride=dcur.stride)
+    poff = np.array([int(b[1]) * pred.stride + int(b[0]) for b in pblocks], np.int64)
+    d_pb, d_poff = X.upload(pblocks.ravel()), X.upload(poff)
+
+    def mc():
+        X.check(X.lib().xme_mc_luma_batch(C.byref(rd), d_pb.at(), len(pblocks), w, h, C.c_void_p(pred.origin),
+                                          pred.stride, d_poff.at(), stream.h), "xme_mc_luma_batch")
+
+    for _ in range(3):
+        mc()
+    stream.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mc()
+    stream.sync()
+    leg["mc_luma_ms_per_frame"] = (time.perf_counter() - t0) * 1e3 / reps
+    if cpu:
+        gp = pred.buf.download(pred.rows * pred.stride, np.uint8).reshape(pred.rows, pred.stride)
+        ok = True
+        for i in range(0, len(pblocks), 8):
+            x, y, pw, ph, mx, my = (int(v) for v in pblocks[i])
+            qx = O.lib().xo_get_mc_pos(w, pw, x, mx)
+            qy = O.lib().xo_get_mc_pos(h, ph, y, my)
+            e = np.zeros((ph, pw), np.uint8)
+            O.lib().xo_mc_luma(O.ptr(e), pw, qx, qy, pw, ph, O.plane_ptrs(planes), planes[0].stride)
+            ok = ok and np.array_equal(gp[y:y + ph, x:x + pw], e)
+        leg["mc_luma_parity_with_oracle_every_8th_pu"] = bool(ok)
+    log(f"recon: {ms:.3f} ms per frame; luma MC of every PU {leg['mc_luma_ms_per_frame']:.3f} ms")
     return leg
 
 

@@ -4,8 +4,7 @@
 // 4-tap intpl_chroma_block_* filters (mc.c:275-337).  Block positions follow
 // get_mv_for_mc / cu_get_mc_pos (source/common/mc.h:52-75).
 //
-// mc_luma: one workgroup per block, a thread per output pixel; mc_chroma: a
-// wave per block (the FIR helpers are in xme_mc.h).
+// A wave per block, four pixels per lane (the FIR helpers are in xme_mc.h).
 #include "xme_dpp.h"
 #include "xme_mc.h"
 
@@ -149,19 +148,42 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
     }
 }
 
+// mc_luma: a wave per block, blocks in one contiguous range per XCD as in
+// mc_chroma_kernel; a lane writes four pixels of a row as one dword (an
+// unaligned dword of the interpolated plane, or four block-filter pixels)
+// where the block's width and destination allow it.
 __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_ref, const int32_t *__restrict__ blocks,
                                                       int n, int fw, int fh, uint8_t *__restrict__ dst, int64_t i_dst,
                                                       const int64_t *__restrict__ dst_off)
 {
-    const int b = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int lb = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    const int b = lb * 4 + (int)(threadIdx.x >> 6);
     if (b >= n)
         return;
     const int32_t *bl = blocks + 6 * b;
     const int lx = bl[0], ly = bl[1], w = bl[2], h = bl[3];
     const LumaSrc src = luma_src(pl, i_ref, mc_pos(fw, w, lx, bl[4]), mc_pos(fh, h, ly, bl[5]));
-    for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
-        const int y = t / w, x = t % w;
-        dst[dst_off[b] + (int64_t)y * i_dst + x] = (uint8_t)luma_px(src, x, y);
+    uint8_t *o = dst + dst_off[b];
+    if ((w & 3) == 0 && (((uintptr_t)o | (uintptr_t)i_dst) & 3) == 0) {
+        const int q = w >> 2;
+        for (int t = lane; t < h * q; t += 64) {
+            const int y = t / q, x = 4 * (t - y * q);
+            uint32_t v;
+            if (src.plane != nullptr) {
+                v = pred4<false>(src, src, x, y);
+            } else {
+                const int px[4] = { luma_px(src, x, y), luma_px(src, x + 1, y), luma_px(src, x + 2, y),
+                                    luma_px(src, x + 3, y) };
+                v = pack4_pels(px);
+            }
+            *reinterpret_cast<uint32_t *>(o + (int64_t)y * i_dst + x) = v;
+        }
+    } else {
+        for (int t = lane; t < w * h; t += 64) {
+            const int y = t / w, x = t % w;
+            o[(int64_t)y * i_dst + x] = (uint8_t)luma_px(src, x, y);
+        }
     }
 }
 
@@ -316,8 +338,9 @@ extern "C" int xme_mc_luma_batch(const xme_ref_desc *ref, const int32_t *d_block
     Planes16c pl;
     for (int k = 0; k < 16; k++)
         pl.p[k] = ref->planes[k];
-    hipLaunchKernelGGL(mc_luma_kernel, dim3(n), dim3(256), 0, (hipStream_t)stream, pl, (int64_t)ref->stride, d_blocks,
-                       n, frame_w, frame_h, d_dst, (int64_t)i_dst, d_dst_off);
+    const int grid = ((n + 3) / 4 + 7) / 8 * 8;
+    hipLaunchKernelGGL(mc_luma_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, pl, (int64_t)ref->stride,
+                       d_blocks, n, frame_w, frame_h, d_dst, (int64_t)i_dst, d_dst_off);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }
