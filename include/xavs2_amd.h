@@ -576,10 +576,17 @@ int xme_expand_border_lcu_row(pel_t *d_plane, intptr_t stride, int w, int h, int
                               void *stream);
 int xme_interpolate_lcu_row(pel_t *const d_planes[16], intptr_t stride, int w, int h, int use_fractional_me,
                             int lcu_level, int lcu_y, void *stream);
-/* mc_chroma (source/common/mc.c:797-829) for a batch of blocks: block i has
- * chroma size w[i] x h[i] at absolute position (qx[i], qy[i]) = 4 * luma
- * position + MV (qpel), clamped by get_mv_for_mc (mc.h:69-75) when
- * frame_w_luma > 0.  Outputs are written at d_dst_u/v + dst_off[i]. */
+/* mc_chroma (source/common/mc.c:797-829) for a batch of blocks given in luma
+ * units: block i (luma_w x luma_h at luma_x, luma_y with its qpel MV) has the
+ * chroma size luma_w / 2 x luma_h / 2 at the qpel position 4 * luma position
+ * + MV, clamped by get_mv_for_mc (mc.h:69-75) when frame_w_luma > 0, read as
+ * a chroma integer position (>> 3) and eighth-pel phase (& 7) as
+ * rdo_get_pred_inter does (rdo.c:1641-1655).  Outputs are written at
+ * d_dst_u/v + dst_off[i] (a dword at a time where the chroma width is a
+ * multiple of 4 and the destinations and i_dst are 4-byte aligned).  Source
+ * reads stay inside the planes' 40-pixel chroma border, except that a row's
+ * last reads may pass its padded end by one byte, into the next row's
+ * border. */
 int xme_mc_chroma_batch(const pel_t *d_u, const pel_t *d_v, intptr_t i_src,
                         const int32_t *d_blocks /* n x {luma_x, luma_y, luma_w, luma_h, mv_x, mv_y} */,
                         int n, int frame_w_luma, int frame_h_luma,
