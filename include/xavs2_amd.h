@@ -594,6 +594,19 @@ int xme_mc_chroma_batch(const pel_t *d_u, const pel_t *d_v, intptr_t i_src,
 /* mc_luma (source/common/mc.c:559-589) from the interpolated planes, same block format. */
 int xme_mc_luma_batch(const xme_ref_desc *ref, const int32_t *d_blocks, int n, int frame_w, int frame_h,
                       pel_t *d_dst, intptr_t i_dst, const int64_t *d_dst_off, void *stream);
+/* The bi-predicted forms (rdo_get_pred_inter, source/encoder/rdo.c:1616-1666):
+ * the rounded average (a + b + 1) >> 1 of the predictions from the first
+ * reference at the first MV and the second at the second (pixf.avg and
+ * pixf.average, source/common/pixel.c:844-856), blocks as
+ * n x {x, y, w, h, mv1_x, mv1_y, mv2_x, mv2_y} in luma units, for the
+ * prediction pictures of bi-predicted PUs (xme_recon_pred_batch,
+ * xme_recon_chroma_batch).  Both references share one stride. */
+int xme_mc_luma_bi_batch(const xme_ref_desc *ref1, const xme_ref_desc *ref2, const int32_t *d_blocks, int n,
+                         int frame_w, int frame_h, pel_t *d_dst, intptr_t i_dst, const int64_t *d_dst_off,
+                         void *stream);
+int xme_mc_chroma_bi_batch(const pel_t *d_u1, const pel_t *d_v1, const pel_t *d_u2, const pel_t *d_v2,
+                           intptr_t i_src, const int32_t *d_blocks, int n, int frame_w_luma, int frame_h_luma,
+                           pel_t *d_dst_u, pel_t *d_dst_v, intptr_t i_dst, const int64_t *d_dst_off, void *stream);
 
 /* dct_NxN / idct_NxN (source/common/transform.c:1123-1312) on nblk contiguous
  * n x n blocks (n in 4, 8, 16, 32). */

@@ -158,3 +158,86 @@ def test_mc_luma_and_chroma_batches():
                              pu.stride)
         assert np.array_equal(gu[i, :bh // 2, :bw // 2], eu), i
         assert np.array_equal(gv[i, :bh // 2, :bw // 2], ev), i
+
+
+def test_mc_bi_batches():
+    """xme_mc_luma_bi_batch / xme_mc_chroma_bi_batch: the rounded average (a + b + 1) >> 1
+    (pixel.c:844-856) of the oracle's mc_luma / mc_chroma from two references at two MVs
+    (rdo.c:1616-1666), luma from the planes and from the full-pel planes alone, chroma with
+    widths of 2 and 6 (the reference's pixf.average branch) beside the others."""
+    w, h = 192, 128
+    imgs = [synth.smooth_field(w, h, seed=41), synth.smooth_field(w, h, seed=42)]
+    drefs, oplanes = [], []
+    for img in imgs:
+        d = X.DeviceRef(w, h, frac=2)
+        d.load(img)
+        drefs.append(d)
+        rp = O.Plane(w, h)
+        rp.set_picture(img)
+        rp.expand_border()
+        oplanes.append((rp, O.interp_planes(rp, 2)))
+    cps, dps = [], []
+    for seed in (43, 44, 45, 46):
+        c = synth.smooth_field(w // 2, h // 2, seed=seed)
+        op = O.Plane(w // 2, h // 2, pad=40)
+        op.set_picture(c)
+        op.expand_border()
+        dp = X.DevicePlane(w // 2, h // 2, pad=40)
+        dp.upload_padded(op.view()[: h // 2 + 80, : w // 2 + 80])
+        cps.append(op)
+        dps.append(dp)
+    rng = np.random.default_rng(47)
+    blocks = []
+    for bw, bh in ((8, 8), (16, 16), (32, 32), (64, 64), (32, 8), (8, 32), (4, 8), (12, 16), (16, 12), (64, 16)):
+        for _ in range(5):
+            x = int(rng.integers(0, (w - bw) // 4 + 1)) * 4
+            y = int(rng.integers(0, h - bh + 1))
+            mv = [int(rng.integers(-120, 121)) for _ in range(4)]
+            if rng.random() < 0.2:
+                mv[2], mv[3] = 4 * int(rng.integers(-300, 300)) + 5, 4 * int(rng.integers(-300, 300)) + 3
+            blocks.append((x, y, bw, bh, *mv))
+    n = len(blocks)
+    barr = np.array(blocks, np.int32).ravel()
+    off = np.arange(n, dtype=np.int64) * 64 * 64
+    d_blocks, d_off = X.upload(barr), X.upload(off)
+
+    def mc_pos(size, blk, pos, mv):
+        return O.lib().xo_get_mc_pos(size, blk, pos, mv)
+
+    out = X.DeviceBuffer(n * 64 * 64)
+    for use_planes in (True, False):
+        rd = [d.desc() for d in drefs]
+        if not use_planes:
+            for r in rd:
+                for k in range(1, 16):
+                    r.planes[k] = None
+        X.check(X.lib().xme_mc_luma_bi_batch(C.byref(rd[0]), C.byref(rd[1]), d_blocks.at(), n, w, h, out.at(), 64,
+                                             d_off.at(), None), "luma bi")
+        got = out.download(n * 64 * 64, np.uint8).reshape(n, 64, 64)
+        for i, (x, y, bw, bh, m1x, m1y, m2x, m2y) in enumerate(blocks):
+            preds = []
+            for (rp, planes), (mx, my) in zip(oplanes, ((m1x, m1y), (m2x, m2y))):
+                e = np.zeros((bh, bw), np.uint8)
+                O.lib().xo_mc_luma(O.ptr(e), bw, mc_pos(w, bw, x, mx), mc_pos(h, bh, y, my), bw, bh,
+                                   O.plane_ptrs(planes), rp.stride)
+                preds.append(e.astype(np.int32))
+            exp = ((preds[0] + preds[1] + 1) >> 1).astype(np.uint8)
+            assert np.array_equal(got[i, :bh, :bw], exp), (i, use_planes)
+    ou, ov = X.DeviceBuffer(n * 64 * 64), X.DeviceBuffer(n * 64 * 64)
+    X.check(X.lib().xme_mc_chroma_bi_batch(C.c_void_p(dps[0].origin), C.c_void_p(dps[1].origin),
+                                           C.c_void_p(dps[2].origin), C.c_void_p(dps[3].origin), dps[0].stride,
+                                           d_blocks.at(), n, w, h, ou.at(), ov.at(), 64, d_off.at(), None), "chroma bi")
+    gu = ou.download(n * 64 * 64, np.uint8).reshape(n, 64, 64)
+    gv = ov.download(n * 64 * 64, np.uint8).reshape(n, 64, 64)
+    for i, (x, y, bw, bh, m1x, m1y, m2x, m2y) in enumerate(blocks):
+        u1, v1 = O.mc_chroma(mc_pos(w, bw, x, m1x), mc_pos(h, bh, y, m1y), bw // 2, bh // 2, cps[0], cps[1])
+        u2, v2 = O.mc_chroma(mc_pos(w, bw, x, m2x), mc_pos(h, bh, y, m2y), bw // 2, bh // 2, cps[2], cps[3])
+        eu = ((u1.astype(np.int32) + u2 + 1) >> 1).astype(np.uint8)
+        ev = ((v1.astype(np.int32) + v2 + 1) >> 1).astype(np.uint8)
+        assert np.array_equal(gu[i, :bh // 2, :bw // 2], eu), i
+        assert np.array_equal(gv[i, :bh // 2, :bw // 2], ev), i
+    # argument checks
+    assert X.lib().xme_mc_chroma_bi_batch(None, None, None, None, 64, None, 1, w, h, None, None, 64, None,
+                                          None) == X.XME_ERR_ARG
+    assert X.lib().xme_mc_chroma_bi_batch(None, None, None, None, 64, None, 0, w, h, None, None, 64, None,
+                                          None) == X.XME_OK

@@ -111,7 +111,34 @@ __device__ __forceinline__ uint32_t chroma_px4(const uint8_t *s, int64_t i_src, 
 // multiple of 8), so that the rows of neighbouring blocks, which share
 // destination lines, are written through one L2.  A lane writes four pixels
 // of a row as one dword where the block's width and destination allow it.
+// BI: the rounded average (a + b + 1) >> 1 of the predictions of the two
+// references (rdo_get_pred_inter, rdo.c:1656-1666: pixf.avg or
+// pixf.average, pixel.c:844-856), block records {x, y, w, h, mv1, mv2}.
+struct ChromaSrc {
+    const uint8_t *u, *v;
+    int64_t off;
+    int px, py, th, tv;
+};
+
+__device__ __forceinline__ ChromaSrc chroma_src(const uint8_t *pu_, const uint8_t *pv_, int64_t i_src, int fw, int fh,
+                                                int lx, int ly, int lw, int lh, int mvx, int mvy)
+{
+    // get_mv_for_mc on the luma block, then chroma 1/8-pel position (rdo.c:1641-1655)
+    const int qx = mc_pos(fw, lw, lx, mvx), qy = mc_pos(fh, lh, ly, mvy);
+    ChromaSrc c;
+    c.u = pu_;
+    c.v = pv_;
+    c.px = qx & 7;
+    c.py = qy & 7;
+    c.th = pack_taps(c_chroma[c.px]);
+    c.tv = pack_taps(c_chroma[c.py]);
+    c.off = (int64_t)(qy >> 3) * i_src + (qx >> 3);
+    return c;
+}
+
+template <bool BI>
 __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restrict__ pu_, const uint8_t *__restrict__ pv_,
+                                                        const uint8_t *__restrict__ pu2, const uint8_t *__restrict__ pv2,
                                                         int64_t i_src, const int32_t *__restrict__ blocks, int n,
                                                         int fw, int fh, uint8_t *__restrict__ du, uint8_t *__restrict__ dv,
                                                         int64_t i_dst, const int64_t *__restrict__ dst_off)
@@ -121,29 +148,33 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
     const int b = lb * 4 + (int)(threadIdx.x >> 6);
     if (b >= n)
         return;
-    const int32_t *bl = blocks + 6 * b;
+    const int32_t *bl = blocks + (BI ? 8 : 6) * b;
     const int lx = bl[0], ly = bl[1], lw = bl[2], lh = bl[3];
-    // get_mv_for_mc on the luma block, then chroma 1/8-pel position (rdo.c:1618-1658)
-    const int qx = mc_pos(fw, lw, lx, bl[4]), qy = mc_pos(fh, lh, ly, bl[5]);
+    const ChromaSrc c1 = chroma_src(pu_, pv_, i_src, fw, fh, lx, ly, lw, lh, bl[4], bl[5]);
+    const ChromaSrc c2 = BI ? chroma_src(pu2, pv2, i_src, fw, fh, lx, ly, lw, lh, bl[6], bl[7]) : c1;
     const int w = lw >> 1, h = lh >> 1;
-    const int px = qx & 7, py = qy & 7;
-    const int th = pack_taps(c_chroma[px]), tv = pack_taps(c_chroma[py]);
-    const int64_t off = (int64_t)(qy >> 3) * i_src + (qx >> 3);
     uint8_t *ou = du + dst_off[b], *ov = dv + dst_off[b];
     const bool wide = (w & 3) == 0 && ((((uintptr_t)ou | (uintptr_t)ov) & 3) | (uintptr_t)(i_dst & 3)) == 0;
     if (wide) {
         const int q = w >> 2, per = h * q;
         for (int t = lane; t < 2 * per; t += 64) {
             const int c = t >= per, e = c ? t - per : t, y = e / q, x = 4 * (e - y * q);
-            const uint8_t *s = (c ? pv_ : pu_) + off + (int64_t)y * i_src + x;
-            *reinterpret_cast<uint32_t *>((c ? ov : ou) + (int64_t)y * i_dst + x) =
-                chroma_px4(s, i_src, px, py, th, tv);
+            const int64_t so = (int64_t)y * i_src + x;
+            uint32_t u = chroma_px4((c ? c1.v : c1.u) + c1.off + so, i_src, c1.px, c1.py, c1.th, c1.tv);
+            if constexpr (BI) {
+                const uint32_t v2 = chroma_px4((c ? c2.v : c2.u) + c2.off + so, i_src, c2.px, c2.py, c2.th, c2.tv);
+                u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);   // (u + v + 1) >> 1 per byte
+            }
+            *reinterpret_cast<uint32_t *>((c ? ov : ou) + (int64_t)y * i_dst + x) = u;
         }
     } else {
         for (int t = lane; t < 2 * w * h; t += 64) {
             const int c = t / (w * h), e = t % (w * h), y = e / w, x = e % w;
-            const uint8_t *s = (c ? pv_ : pu_) + off + (int64_t)y * i_src + x;
-            (c ? ov : ou)[(int64_t)y * i_dst + x] = (uint8_t)chroma_px(s, i_src, px, py, th, tv);
+            const int64_t so = (int64_t)y * i_src + x;
+            int v = chroma_px((c ? c1.v : c1.u) + c1.off + so, i_src, c1.px, c1.py, c1.th, c1.tv);
+            if constexpr (BI)
+                v = (v + chroma_px((c ? c2.v : c2.u) + c2.off + so, i_src, c2.px, c2.py, c2.th, c2.tv) + 1) >> 1;
+            (c ? ov : ou)[(int64_t)y * i_dst + x] = (uint8_t)v;
         }
     }
 }
@@ -152,8 +183,10 @@ __global__ void __launch_bounds__(256) mc_chroma_kernel(const uint8_t *__restric
 // mc_chroma_kernel; a lane writes four pixels of a row as one dword (an
 // unaligned dword of the interpolated plane, or four block-filter pixels)
 // where the block's width and destination allow it.
-__global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_ref, const int32_t *__restrict__ blocks,
-                                                      int n, int fw, int fh, uint8_t *__restrict__ dst, int64_t i_dst,
+template <bool BI>
+__global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, Planes16c pl2, int64_t i_ref,
+                                                      const int32_t *__restrict__ blocks, int n, int fw, int fh,
+                                                      uint8_t *__restrict__ dst, int64_t i_dst,
                                                       const int64_t *__restrict__ dst_off)
 {
     const int lane = threadIdx.x & 63;
@@ -161,20 +194,21 @@ __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_re
     const int b = lb * 4 + (int)(threadIdx.x >> 6);
     if (b >= n)
         return;
-    const int32_t *bl = blocks + 6 * b;
+    const int32_t *bl = blocks + (BI ? 8 : 6) * b;
     const int lx = bl[0], ly = bl[1], w = bl[2], h = bl[3];
     const LumaSrc src = luma_src(pl, i_ref, mc_pos(fw, w, lx, bl[4]), mc_pos(fh, h, ly, bl[5]));
+    const LumaSrc src2 = BI ? luma_src(pl2, i_ref, mc_pos(fw, w, lx, bl[6]), mc_pos(fh, h, ly, bl[7])) : src;
     uint8_t *o = dst + dst_off[b];
     if ((w & 3) == 0 && (((uintptr_t)o | (uintptr_t)i_dst) & 3) == 0) {
         const int q = w >> 2;
         for (int t = lane; t < h * q; t += 64) {
             const int y = t / q, x = 4 * (t - y * q);
             uint32_t v;
-            if (src.plane != nullptr) {
-                v = pred4<false>(src, src, x, y);
+            if (src.plane != nullptr && (!BI || src2.plane != nullptr)) {
+                v = pred4<BI>(src, src2, x, y);
             } else {
-                const int px[4] = { luma_px(src, x, y), luma_px(src, x + 1, y), luma_px(src, x + 2, y),
-                                    luma_px(src, x + 3, y) };
+                const int px[4] = { pred_px<BI>(src, src2, x, y), pred_px<BI>(src, src2, x + 1, y),
+                                    pred_px<BI>(src, src2, x + 2, y), pred_px<BI>(src, src2, x + 3, y) };
                 v = pack4_pels(px);
             }
             *reinterpret_cast<uint32_t *>(o + (int64_t)y * i_dst + x) = v;
@@ -182,7 +216,7 @@ __global__ void __launch_bounds__(256) mc_luma_kernel(Planes16c pl, int64_t i_re
     } else {
         for (int t = lane; t < w * h; t += 64) {
             const int y = t / w, x = t % w;
-            o[(int64_t)y * i_dst + x] = (uint8_t)luma_px(src, x, y);
+            o[(int64_t)y * i_dst + x] = (uint8_t)pred_px<BI>(src, src2, x, y);
         }
     }
 }
@@ -320,8 +354,9 @@ extern "C" int xme_mc_chroma_batch(const pel_t *d_u, const pel_t *d_v, intptr_t 
     if (n == 0)
         return XME_OK;
     const int grid = ((n + 3) / 4 + 7) / 8 * 8;
-    hipLaunchKernelGGL(mc_chroma_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, d_u, d_v, (int64_t)i_src,
-                       d_blocks, n, frame_w_luma, frame_h_luma, d_dst_u, d_dst_v, (int64_t)i_dst, d_dst_off);
+    hipLaunchKernelGGL(mc_chroma_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream, d_u, d_v, d_u, d_v,
+                       (int64_t)i_src, d_blocks, n, frame_w_luma, frame_h_luma, d_dst_u, d_dst_v, (int64_t)i_dst,
+                       d_dst_off);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }
@@ -339,8 +374,51 @@ extern "C" int xme_mc_luma_batch(const xme_ref_desc *ref, const int32_t *d_block
     for (int k = 0; k < 16; k++)
         pl.p[k] = ref->planes[k];
     const int grid = ((n + 3) / 4 + 7) / 8 * 8;
-    hipLaunchKernelGGL(mc_luma_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, pl, (int64_t)ref->stride,
-                       d_blocks, n, frame_w, frame_h, d_dst, (int64_t)i_dst, d_dst_off);
+    hipLaunchKernelGGL(mc_luma_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream, pl, pl,
+                       (int64_t)ref->stride, d_blocks, n, frame_w, frame_h, d_dst, (int64_t)i_dst, d_dst_off);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+extern "C" int xme_mc_luma_bi_batch(const xme_ref_desc *ref1, const xme_ref_desc *ref2, const int32_t *d_blocks, int n,
+                                    int frame_w, int frame_h, pel_t *d_dst, intptr_t i_dst, const int64_t *d_dst_off,
+                                    void *stream)
+{
+    if (!ref1 || !ref1->planes[0] || !ref2 || !ref2->planes[0] || ref1->stride != ref2->stride || n < 0 ||
+        (n > 0 && (!d_blocks || !d_dst || !d_dst_off))) {
+        xme_set_error("xme_mc_luma_bi_batch: bad arguments (two references of one stride)");
+        return XME_ERR_ARG;
+    }
+    if (n == 0)
+        return XME_OK;
+    Planes16c pl, pl2;
+    for (int k = 0; k < 16; k++) {
+        pl.p[k] = ref1->planes[k];
+        pl2.p[k] = ref2->planes[k];
+    }
+    const int grid = ((n + 3) / 4 + 7) / 8 * 8;
+    hipLaunchKernelGGL(mc_luma_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, pl, pl2,
+                       (int64_t)ref1->stride, d_blocks, n, frame_w, frame_h, d_dst, (int64_t)i_dst, d_dst_off);
+    XME_HIP(hipGetLastError());
+    return XME_OK;
+}
+
+extern "C" int xme_mc_chroma_bi_batch(const pel_t *d_u1, const pel_t *d_v1, const pel_t *d_u2, const pel_t *d_v2,
+                                      intptr_t i_src, const int32_t *d_blocks, int n, int frame_w_luma,
+                                      int frame_h_luma, pel_t *d_dst_u, pel_t *d_dst_v, intptr_t i_dst,
+                                      const int64_t *d_dst_off, void *stream)
+{
+    if (n < 0 ||
+        (n > 0 && (!d_u1 || !d_v1 || !d_u2 || !d_v2 || !d_blocks || !d_dst_u || !d_dst_v || !d_dst_off))) {
+        xme_set_error("xme_mc_chroma_bi_batch: bad arguments");
+        return XME_ERR_ARG;
+    }
+    if (n == 0)
+        return XME_OK;
+    const int grid = ((n + 3) / 4 + 7) / 8 * 8;
+    hipLaunchKernelGGL(mc_chroma_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, d_u1, d_v1, d_u2, d_v2,
+                       (int64_t)i_src, d_blocks, n, frame_w_luma, frame_h_luma, d_dst_u, d_dst_v, (int64_t)i_dst,
+                       d_dst_off);
     XME_HIP(hipGetLastError());
     return XME_OK;
 }

@@ -170,7 +170,9 @@ def lib():
         L.xme_sad_x3_batch.argtypes = [I, P, P, Z, P, P, I, P, P]
         L.xme_sad_x4_batch.argtypes = [I, P, P, Z, P, P, I, P, P]
         L.xme_mc_chroma_batch.argtypes = [P, P, Z, P, I, I, I, P, P, Z, P, P]
+        L.xme_mc_chroma_bi_batch.argtypes = [P, P, P, P, Z, P, I, I, I, P, P, Z, P, P]
         L.xme_mc_luma_batch.argtypes = [P, P, I, I, I, P, Z, P, P]
+        L.xme_mc_luma_bi_batch.argtypes = [C.POINTER(RefDesc), C.POINTER(RefDesc), P, I, I, I, P, Z, P, P]
         L.xme_me_search_sym_batch.argtypes = [C.POINTER(MeParams), P, I, P, Z, C.POINTER(RefDesc),
                                               C.POINTER(RefDesc), P, P]
         L.xme_me_search_bid_batch.argtypes = [C.POINTER(MeParams), P, I, P, Z, C.POINTER(RefDesc),
