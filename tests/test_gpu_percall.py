@@ -142,9 +142,13 @@ def test_luma_interpolation_members_match_oracle(taps):
     sets = [np.ascontiguousarray(npref.LUMA_TAPS[k], np.int8) for k in (1, 2, 3)] if taps == "reference" else \
         [rng.integers(-128, 128, 8).astype(np.int8) for _ in range(3)]
     W, H, S = 300, 40, 320
-    src = rng.integers(0, 256, (H + 16, S), dtype=np.uint8)
+    # rows 5 - 3 .. 5 + h + 4 and columns 6 - 3 .. 6 + w + 4 of every case inside src: a
+    # read past it would take whatever memory follows, which may change between the GPU
+    # call and the oracle's
+    src = rng.integers(0, 256, (H + 32, S), dtype=np.uint8)
     s0 = at(src, 5 * S + 6)                   # the block's first sample, margins inside src
     for w, h in ((300, 40), (64, 64 - 16), (8, 4), (17, 3)):
+        assert 5 + h + 4 <= src.shape[0] and 6 + w + 4 <= src.shape[1], (w, h)
         c = sets[0]
         gd, ed = np.zeros((h, w + 9), np.uint8), np.zeros((h, w + 9), np.uint8)
         gt, et = np.zeros((h, w + 3), np.int16), np.zeros((h, w + 3), np.int16)
