@@ -481,3 +481,82 @@ def test_64x64_tus_limits():
     p = buf.at()
     assert X.lib().xme_inter_recon_batch(64, 30, p, 192, C.byref(rd), w, h, p, 1, p, p, p, p, 192, p, p,
                                          None) == X.XME_ERR_ARG
+
+
+@pytest.mark.parametrize("bi", [False, True])
+def test_2160p_records_and_prediction_picture_agree(bi):
+    """At 3840x2160 with the mixed PU tiling, every PU split into its largest square TUs
+    (64x64 PUs one wavelet TU), two independent paths give the same levels, nnz, SSD and
+    reconstruction: the record form (xme_inter_recon_batch / _bi_batch, the prediction
+    formed inside the fused kernel) and the prediction picture (xme_mc_luma_batch /
+    _bi_batch over every PU, then xme_recon_pred_batch)."""
+    W, H = 3840, 2160
+    rng = np.random.default_rng(131 + bi)
+    r1 = synth.smooth_field(W, H, seed=132)
+    r2 = synth.smooth_field(W, H, seed=133)
+    cur = synth.multi_ref_current([r1, r2], 32, 16, seed=134)
+    pus = synth.mixed_pus(W, H, seed=135)
+    mv = rng.integers(-64, 65, (len(pus), 4)).astype(np.int32)
+    far = rng.random(len(pus)) < 0.05
+    mv[far, 0] = 4 * rng.integers(-3000, 3000, far.sum()) + 3
+    d1, d2 = X.DeviceRef(W, H, frac=2), X.DeviceRef(W, H, frac=2)
+    d1.load(r1)
+    d2.load(r2)
+    dcur = device_cur(cur)
+    a, b = d1.desc(), d2.desc()
+    pred = X.DevicePlane(W, H, pad=0, stride=dcur.stride)
+    blocks = np.array([(x, y, pw, ph) + tuple(int(v) for v in mv[i]) for i, (x, y, pw, ph) in enumerate(pus)],
+                      np.int32)
+    off = np.array([int(bb[1]) * pred.stride + int(bb[0]) for bb in blocks], np.int64)
+    d_off = X.upload(off)
+    if bi:
+        d_pb = X.upload(blocks.ravel())
+        X.check(X.lib().xme_mc_luma_bi_batch(C.byref(a), C.byref(b), d_pb.at(), len(blocks), W, H,
+                                             C.c_void_p(pred.origin), pred.stride, d_off.at(), None), "mc bi")
+    else:
+        d_pb = X.upload(np.ascontiguousarray(blocks[:, :6]).ravel())
+        X.check(X.lib().xme_mc_luma_batch(C.byref(a), d_pb.at(), len(blocks), W, H, C.c_void_p(pred.origin),
+                                          pred.stride, d_off.at(), None), "mc")
+    by_n = {}
+    for i, (x, y, pw, ph) in enumerate(pus):
+        n = 64
+        while pw % n or ph % n:
+            n >>= 1
+        for ty in range(y, y + ph, n):
+            for tx in range(x, x + pw, n):
+                by_n.setdefault(n, []).append((tx, ty, x, y, pw, ph, int(mv[i, 0]), int(mv[i, 1]), int(mv[i, 2]),
+                                               int(mv[i, 3])))
+    rec_a = X.DevicePlane(W, H, pad=0, stride=dcur.stride)
+    rec_b = X.DevicePlane(W, H, pad=0, stride=dcur.stride)
+    rec_a.buf.zero()
+    rec_b.buf.zero()
+    for n, rows in sorted(by_n.items()):
+        t = np.array(rows, np.int32)
+        ntu, nl = len(t), min(n, 32)
+        d_t = X.upload(np.ascontiguousarray(t[:, :8]))
+        d_blk = X.upload(np.ascontiguousarray(t[:, :2]))
+        outs = []
+        for _ in range(2):
+            outs.append((X.DeviceBuffer(2 * ntu * nl * nl), X.DeviceBuffer(4 * ntu), X.DeviceBuffer(4 * ntu)))
+        (la, na, sa), (lb, nb, sb) = outs
+        if bi:
+            d_mv2 = X.upload(np.ascontiguousarray(t[:, 8:10]))
+            X.check(X.lib().xme_inter_recon_bi_batch(n, 32, C.c_void_p(dcur.origin), dcur.stride, C.byref(a),
+                                                     C.byref(b), W, H, d_t.at(), d_mv2.at(), ntu, None, la.at(), None,
+                                                     C.c_void_p(rec_a.origin), rec_a.stride, na.at(), sa.at(), None),
+                    "records bi")
+        else:
+            X.check(X.lib().xme_inter_recon_batch(n, 32, C.c_void_p(dcur.origin), dcur.stride, C.byref(a), W, H,
+                                                  d_t.at(), ntu, None, la.at(), None, C.c_void_p(rec_a.origin),
+                                                  rec_a.stride, na.at(), sa.at(), None), "records")
+        X.check(X.lib().xme_recon_pred_batch(n, 32, C.c_void_p(dcur.origin), dcur.stride, C.c_void_p(pred.origin),
+                                             pred.stride, W, H, d_blk.at(), ntu, lb.at(), C.c_void_p(rec_b.origin),
+                                             rec_b.stride, nb.at(), sb.at(), None), "prediction picture")
+        X.check(X.lib().xme_stream_sync(None), "sync")
+        for name, x, y, cnt, dt in (("level", la, lb, ntu * nl * nl, np.int16), ("nnz", na, nb, ntu, np.int32),
+                                    ("ssd", sa, sb, ntu, np.int32)):
+            assert np.array_equal(x.download(cnt, dt), y.download(cnt, dt)), (n, name)
+        assert (sa.download(ntu, np.int32) >= 0).all(), n
+    ga = rec_a.buf.download(rec_a.rows * rec_a.stride, np.uint8)
+    gb = rec_b.buf.download(rec_b.rows * rec_b.stride, np.uint8)
+    assert np.array_equal(ga, gb)
