@@ -291,6 +291,9 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
         bool tu_good = false;            // lane < BPT: its TU of the tile is reconstructed
         uint64_t my_rec = 0;             // lane < BPT: the TU's first reconstructed pixel
         uint64_t wl_cur = 0, wl_pred = 0, wl_pred2 = 0, wl_rec = 0;   // WL: the block's, in every lane
+        // WL: this lane's current and prediction dwords (rows 4 i + lane / 16,
+        // columns 4 (lane % 16) ..), loaded once for both ends of the path
+        uint32_t wl_c[WL ? 16 : 1], wl_p[WL ? 16 : 1];
         if constexpr (FU != 0) {
             // lane b < BPT takes TU t * BPT + b: validity and base pointers
             uint64_t my_pred = 0, my_pred2 = 0, my_cur = 0;
@@ -371,8 +374,9 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                 // 0..31 then hold the 32x32 low band
                 int16_t *lw = s_lw[wv];
                 const int qr = lane >> 4, qc = 4 * (lane & 15);
-#pragma unroll 4
-                for (int g = 0; g < 64; g += 4) {
+#pragma unroll
+                for (int gi = 0; gi < 16; gi++) {
+                    const int g = 4 * gi;
                     uint32_t cw = 0, u = 0;
                     if (wl_cur != 0) {
                         cw = *reinterpret_cast<const uint32_t *>((const uint8_t *)(uintptr_t)wl_cur +
@@ -384,6 +388,8 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                             u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
                         }
                     }
+                    wl_c[gi] = cw;
+                    wl_p[gi] = u;
                     int e4[4];
 #pragma unroll
                     for (int j = 0; j < 4; j++)
@@ -581,17 +587,10 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             int e2 = 0;
             if (wl_rec != 0) {
                 const int qr = lane >> 4, qc = 4 * (lane & 15);
-#pragma unroll 4
-                for (int g = 0; g < 64; g += 4) {
-                    const int r = g + qr;
-                    const uint32_t cw = *reinterpret_cast<const uint32_t *>((const uint8_t *)(uintptr_t)wl_cur +
-                                                                            (int64_t)r * fa.i_cur + qc);
-                    uint32_t u = load_pred<FU>((const uint8_t *)(uintptr_t)wl_pred + (int64_t)r * fa.i_ref + qc);
-                    if constexpr (FU == 2) {
-                        const uint32_t v2 = load_pred<FU>((const uint8_t *)(uintptr_t)wl_pred2 + (int64_t)r * fa.i_ref +
-                                                          qc);
-                        u = (u | v2) - (((u ^ v2) >> 1) & 0x7f7f7f7fu);
-                    }
+#pragma unroll
+                for (int gi = 0; gi < 16; gi++) {
+                    const int r = 4 * gi + qr;
+                    const uint32_t cw = wl_c[gi], u = wl_p[gi];
                     const uint32_t *res = reinterpret_cast<const uint32_t *>(lw + r * LWS + qc);
                     const uint32_t r01 = res[0], r23 = res[1];
                     const int rv[4] = { (int)(int16_t)(r01 & 0xffffu), (int)(int16_t)(r01 >> 16),
