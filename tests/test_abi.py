@@ -120,3 +120,37 @@ def test_chroma_qp_is_host_only_and_matches_the_oracle():
     L = C.CDLL(LIB)
     for q in range(-10, 81):
         assert L.xme_chroma_qp(q) == O.chroma_qp(q), q
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_reconstruction_and_mc_arguments_are_checked_on_the_host():
+    """The checks of the prediction-picture, chroma and bi-predicted MC entry points refuse
+    bad sizes, QPs and missing buffers, and accept empty batches, before any device call."""
+    from xavs2_amd import lib as X
+    L = X.lib()
+    p = C.c_void_p(4096)          # never dereferenced: every call below fails or returns early
+    for n, qp in ((12, 30), (8, -1), (8, 80), (16 | X.XME_TU_DCT_HALF, 30), (128, 30)):
+        assert L.xme_recon_pred_batch(n, qp, p, 64, p, 64, 64, 64, p, 1, p, p, 64, p, p, None) == X.XME_ERR_ARG
+        assert L.xme_recon_chroma_batch(n, qp, p, p, 64, p, p, 64, 64, 64, p, 1, p, p, p, 64, p, p,
+                                        None) == X.XME_ERR_ARG
+    assert L.xme_recon_pred_batch(64 | X.XME_TU_DCT_HALF, 30, None, 64, None, 64, 64, 64, None, 0, None, None, 64,
+                                  None, None, None) == X.XME_OK
+    assert L.xme_recon_pred_batch(8, 30, None, 64, p, 64, 64, 64, p, 1, p, p, 64, p, p, None) == X.XME_ERR_ARG
+    assert L.xme_recon_chroma_batch(8, 30, p, None, 64, p, p, 64, 64, 64, p, 1, p, p, p, 64, p, p,
+                                    None) == X.XME_ERR_ARG
+    # misaligned pictures, strides or levels
+    odd = C.c_void_p(4097)
+    for args in ((odd, 64, p, 64, p), (p, 66, p, 64, p), (p, 64, odd, 64, p), (p, 64, p, 65, p),
+                 (p, 64, p, 64, C.c_void_p(4104))):
+        cur, ic, pred, ip, lev = args
+        assert L.xme_recon_pred_batch(8, 30, cur, ic, pred, ip, 64, 64, p, 1, lev, p, 64, p, p,
+                                      None) == X.XME_ERR_ARG
+    assert L.xme_mc_chroma_bi_batch(p, p, p, p, 64, None, 1, 64, 64, p, p, 64, p, None) == X.XME_ERR_ARG
+    assert L.xme_mc_chroma_bi_batch(p, p, p, p, 64, p, -1, 64, 64, p, p, 64, p, None) == X.XME_ERR_ARG
+    assert L.xme_mc_chroma_bi_batch(None, None, None, None, 64, None, 0, 64, 64, None, None, 64, None,
+                                    None) == X.XME_OK
+    r1, r2 = X.RefDesc(), X.RefDesc()
+    r1.planes[0], r2.planes[0], r1.stride, r2.stride = 4096, 4096, 256, 320
+    assert L.xme_mc_luma_bi_batch(C.byref(r1), C.byref(r2), p, 1, 64, 64, p, 64, p, None) == X.XME_ERR_ARG
+    r2.stride = 256
+    assert L.xme_mc_luma_bi_batch(C.byref(r1), C.byref(r2), None, 0, 64, 64, None, 64, None, None) == X.XME_OK
