@@ -1081,7 +1081,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the ME path runs on a single MI355X; --gpus {args.gpus} is not "
-              "supported (multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
+              "supported (several GPUs would each run whole streams with their resident references).",
+              file=sys.stderr)
         sys.exit(2)
 
     # read by the HIP runtime when it starts (the first HIP call below)
