@@ -419,15 +419,32 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (lane < 32) {
-                    int v[68];
+                {
+                    // column lane % 32, low-band half lane / 32: rows -2 .. 33
+                    // or 30 .. 65 (mirrored at the ends as the lifting
+                    // mirrors them) give low samples 0..15 or 16..31; every
+                    // lane reads before any writes, so the halves may share
+                    // rows 16..33
+                    const int cc = lane & 31, b0 = (lane >> 5) ? 30 : -2;
+                    int v[36];
 #pragma unroll
-                    for (int i = 0; i < 64; i++)
-                        v[2 + i] = lw[i * LWS + lane];
-                    lift64_fwd<true>(v);
+                    for (int j = 0; j < 36; j++) {
+                        int r = b0 + j;
+                        r = r < 0 ? -r : (r > 63 ? 126 - r : r);
+                        v[j] = lw[r * LWS + cc];
+                    }
 #pragma unroll
-                    for (int i = 0; i < 32; i++)
-                        lw[i * LWS + lane] = (int16_t)v[2 + 2 * i];
+                    for (int j = 1; j < 35; j += 2)
+                        v[j] = (int16_t)(v[j] - ((v[j - 1] + v[j + 1]) >> 1));
+#pragma unroll
+                    for (int j = 2; j <= 32; j += 2)
+                        v[j] = (int16_t)((v[j] << 1) + ((v[j - 1] + v[j + 1] + 1) >> 1));
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                    for (int m = 0; m < 16; m++)
+                        lw[((b0 + 2) / 2 + m) * LWS + cc] = (int16_t)v[2 + 2 * m];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -553,15 +570,22 @@ __global__ void __launch_bounds__(256) residual_kernel(const int16_t *__restrict
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane < 32) {
-                int e[65];
+            {
+                // column lane % 32, half lane / 32: low samples 0..16 give rows
+                // 0..31, low samples 16..31 (the last mirrored) rows 32..63;
+                // the even samples halved, the odd ones the floor mean
+                const int cc = lane & 31, k0 = (lane >> 5) ? 16 : 0;
+                int ev[17];
 #pragma unroll
-                for (int i = 0; i < 32; i++)
-                    e[2 * i] = s_res[wv][i * 32 + lane];
-                lift64_inv<true>(e);
+                for (int m = 0; m < 17; m++) {
+                    const int k = k0 + m > 31 ? 31 : k0 + m;
+                    ev[m] = (int16_t)((int)s_res[wv][k * 32 + cc] >> 1);
+                }
 #pragma unroll
-                for (int i = 0; i < 64; i++)
-                    lw[i * LWS + lane] = (int16_t)e[i];
+                for (int m = 0; m < 16; m++) {
+                    lw[(2 * (k0 + m)) * LWS + cc] = (int16_t)ev[m];
+                    lw[(2 * (k0 + m) + 1) * LWS + cc] = (int16_t)((ev[m] + ev[m + 1]) >> 1);
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
